@@ -1,0 +1,234 @@
+/*
+ * mslr.h - C ABI of the MI355X (gfx950) MSLR reconstruction library.
+ *
+ * The library accelerates one POGM iteration of multi-scale low-rank (MSLR)
+ * 3D+time MRI reconstruction: SENSE forward/adjoint, per-scale patch singular
+ * value soft-thresholding (SVST) and the POGM momentum/restart update.
+ *
+ * Conventions (all entry points):
+ *   - Arrays are column-major (Julia order, x fastest).  Complex values are
+ *     interleaved (re, im) pairs: ComplexF32 = 2 x float, ComplexF64 = 2 x double.
+ *   - Pointers named *_dev are device pointers obtained from mslr_malloc (or any
+ *     hipMalloc'd memory on the context's device); *_host are host pointers.
+ *   - Scalars are double.  Sizes are int64_t.
+ *   - Every call returns an mslr_status_t; on failure mslr_last_error() returns
+ *     a message for the calling thread.  Status codes map to the exception kinds
+ *     the reference throws: MSLR_ERR_ARG ~ ArgumentError, MSLR_ERR_SHAPE ~
+ *     AssertionError/DimensionMismatch, MSLR_ERR_DEVICE ~ error("use_gpu=true
+ *     requires ...").
+ *   - A context owns one HIP stream.  All work of a context is enqueued on that
+ *     stream in call order; calls on one context must be serialised by the
+ *     caller (like the reference GPU path, which is single-stream).  Entry
+ *     points that return host-visible results synchronise the stream.
+ *   - The library never frees memory it does not own.
+ *
+ * Reference interfaces replaced (rextlfung/mslr-recon, file:line):
+ *   Asense_gpu / MIRT.Asense          src/sense_gpu.jl:56-108, scripts/reconstruct.jl:135-142
+ *   block_diag over frames + flatten   scripts/reconstruct.jl:142,145-147
+ *   img2patches / patches2img          src/recon.jl:53-78, 96-126
+ *   SVST / patchSVST / patchSVST_gpu   src/recon.jl:183-269, src/sense_gpu.jl:122-160
+ *   patch_nucnorm                      src/recon.jl:138-146
+ *   pogm_mod (+ _gr_restart)           src/mirt_mod.jl:26-29, 74-188
+ *   poweriter_mod                      src/mirt_mod.jl:215-238
+ *   dc_cost_grad / g_prox / lambdas    scripts/reconstruct.jl:159-202
+ */
+#ifndef MSLR_H
+#define MSLR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MSLR_OK = 0,
+    MSLR_ERR_ARG = 1,         /* invalid argument value (ArgumentError) */
+    MSLR_ERR_SHAPE = 2,       /* shape / mask / sample-count mismatch (AssertionError) */
+    MSLR_ERR_DEVICE = 3,      /* HIP runtime error or no usable GPU */
+    MSLR_ERR_NOMEM = 4,       /* device allocation failed */
+    MSLR_ERR_UNSUPPORTED = 5, /* size outside what the kernels support (e.g. FFT length with a prime factor > 5) */
+    MSLR_ERR_NUMERIC = 6      /* non-finite values or eigensolver failure */
+} mslr_status_t;
+
+typedef struct mslr_ctx_s *mslr_ctx;
+typedef struct mslr_sense_s *mslr_sense;
+typedef struct mslr_pogm_s *mslr_pogm;
+
+/* ---------------------------------------------------------------- library */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mslr_last_error(void);
+/* ABI version (major*10000 + minor*100 + patch). */
+int mslr_version(void);
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the reference's CUDA.functional() probe and device selection
+ * (scripts/reconstruct.jl:45-55, 73-81).  device = HIP device ordinal. */
+mslr_status_t mslr_ctx_create(int device, mslr_ctx *out);
+mslr_status_t mslr_ctx_destroy(mslr_ctx ctx);
+mslr_status_t mslr_ctx_sync(mslr_ctx ctx);
+/* Device name and total / free memory in bytes (println at scripts/reconstruct.jl:78-80). */
+mslr_status_t mslr_ctx_info(mslr_ctx ctx, char *name, size_t name_len,
+                            size_t *total_bytes, size_t *free_bytes);
+
+/* ------------------------------------------------------- device buffers */
+/* Replaces cu(...) / Array(...) transfers (scripts/reconstruct.jl:122-130, 233-238). */
+mslr_status_t mslr_malloc(mslr_ctx ctx, size_t bytes, void **dev_ptr);
+mslr_status_t mslr_free(mslr_ctx ctx, void *dev_ptr);
+mslr_status_t mslr_upload(mslr_ctx ctx, void *dst_dev, const void *src_host, size_t bytes);
+mslr_status_t mslr_download(mslr_ctx ctx, void *dst_host, const void *src_dev, size_t bytes);
+mslr_status_t mslr_copy_d2d(mslr_ctx ctx, void *dst_dev, const void *src_dev, size_t bytes);
+mslr_status_t mslr_zero(mslr_ctx ctx, void *dev_ptr, size_t bytes);
+
+/* ------------------------------------------------------ SENSE operator */
+/*
+ * All-frames SENSE plan: the block-diagonal stack of per-frame Asense operators
+ *   A = block_diag([Asense(mask[:,:,:,t], smaps) for t in 1:Nt]...)
+ * (scripts/reconstruct.jl:135-142; per-frame operator src/sense_gpu.jl:56-108).
+ *
+ *   forward : y[:, c, t] = scale * (fftshift o FFT3 o ifftshift)(smaps[...,c] .* x[...,t])[mask_t]
+ *   adjoint : the exact conjugate transpose of forward (unnormalised inverse DFT
+ *             times the same scale).  src/sense_gpu.jl:93 applies a normalised
+ *             ifft instead, which is 1/prod(N) times this adjoint.
+ *   scale   = 1/sqrt(Nx*Ny*Nz) if unitary else 1.
+ *   fft_forward = 0 uses the positive-exponent DFT normalised by 1/prod(N) in the
+ *             forward (fftshift o ifft o ifftshift, src/sense_gpu.jl:77); the adjoint
+ *             stays the exact conjugate transpose.
+ *
+ * smaps_host : ComplexF32 (Nx,Ny,Nz,Nc), used as given (normalise before, as
+ *              scripts/reconstruct.jl:89 does).
+ * mask_host  : uint8 (Nx,Ny,Nz,Nt), nonzero = sampled.  Every frame must have
+ *              the same sample count K (scripts/reconstruct.jl:116-118), else
+ *              MSLR_ERR_SHAPE.
+ * k-space layout (K, Nc, Nt): sample k of frame t is the k-th sampled voxel of
+ * mask_t in column-major order (findall(vec(samp)), src/sense_gpu.jl:68).
+ * Each of Nx, Ny, Nz must factor into 2, 3 and 5 and Ny*Nz <= 8192.
+ */
+mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, int64_t Nt,
+                                const float *smaps_host, const uint8_t *mask_host,
+                                int fft_forward, int unitary, mslr_sense *out);
+mslr_status_t mslr_sense_destroy(mslr_sense plan);
+/* Samples per frame per coil (K = sum(samp), src/sense_gpu.jl:63). */
+int64_t mslr_sense_nsamples(mslr_sense plan);
+
+/* x_dev: ComplexF32 (Nx,Ny,Nz,nframes) holding frames t0..t0+nframes-1;
+ * y_dev: ComplexF32 (K,Nc,nframes).  Frames are 0-based. */
+mslr_status_t mslr_sense_forward(mslr_sense plan, const void *x_dev, void *y_dev,
+                                 int64_t t0, int64_t nframes);
+mslr_status_t mslr_sense_adjoint(mslr_sense plan, const void *y_dev, void *x_dev,
+                                 int64_t t0, int64_t nframes);
+/* The same in ComplexF64 (used by the FP64 power iteration). */
+mslr_status_t mslr_sense_forward_f64(mslr_sense plan, const void *x_dev, void *y_dev,
+                                     int64_t t0, int64_t nframes);
+mslr_status_t mslr_sense_adjoint_f64(mslr_sense plan, const void *y_dev, void *x_dev,
+                                     int64_t t0, int64_t nframes);
+/*
+ * Gradient of the data-consistency term on the scale sum
+ * (dc_cost_grad, scripts/reconstruct.jl:174,176-183):
+ *   s = sum_k X[:,:,:,:,k];  r = A s - y;  g = A' r;  dc = 0.5*||r||^2
+ * X_dev: ComplexF32 (Nx,Ny,Nz,Nt,nscales); y_dev: (K,Nc,Nt); g_dev: (Nx,Ny,Nz,Nt).
+ * dc_cost (host, nullable) receives 0.5*||r||^2 accumulated in FP64.
+ */
+mslr_status_t mslr_sense_grad(mslr_sense plan, const void *X_dev, int64_t nscales,
+                              const void *y_dev, void *g_dev, double *dc_cost);
+/* sigma_1(A) by power iteration in FP64 (poweriter_mod, src/mirt_mod.jl:215-238):
+ * x0 = ones, at most niter steps, stop when |ratio-ratio_old|/ratio < tol.
+ * Returns sigma1 = ||A x||/||x|| and the number of steps taken. */
+mslr_status_t mslr_sense_poweriter(mslr_sense plan, int niter, double tol,
+                                   double *sigma1, int *iters);
+
+/* ------------------------------------------------------- patch operators */
+/*
+ * Patch geometry (src/recon.jl:53-78): patch size clamped to the image, stride s,
+ * Nsteps = cld(N - ps, s), start_i = min(i*s, N - ps) (0-based), patches ordered
+ * x fastest, voxels within a patch column-major.  Np = prod(Nsteps+1).
+ */
+mslr_status_t mslr_patch_geometry(const int64_t N[3], const int64_t ps[3], const int64_t ss[3],
+                                  int64_t ps_clamped[3], int64_t nsteps[3], int64_t *Np);
+/* img_dev: ComplexF32 (Nx,Ny,Nz,Nt) -> P_dev: ComplexF32 (p,Nt,Np). */
+mslr_status_t mslr_img2patches(mslr_ctx ctx, const void *img_dev, const int64_t N[3], int64_t Nt,
+                               const int64_t ps[3], const int64_t ss[3], void *P_dev);
+/* P_dev (p,Nt,Np) -> img_dev (Nx,Ny,Nz,Nt): patch sum / max(coverage,1), summed in
+ * patch order (src/recon.jl:96-126). */
+mslr_status_t mslr_patches2img(mslr_ctx ctx, const void *P_dev, const int64_t N[3], int64_t Nt,
+                               const int64_t ps[3], const int64_t ss[3], void *img_dev);
+
+/* ------------------------------------------------------------------ SVST */
+/*
+ * Batched SVST (src/recon.jl:183-198, 217-241): for each of the Np matrices
+ * X_i (p x Nt, ComplexF32, stored consecutively) in place
+ *   X_i <- U diag(max(s - beta_i, 0)) V^H,     X_i == 0 -> 0.
+ * beta_host: one value (per_patch = 0) or Np values (per_patch = 1).  Each
+ * beta is rounded to Float32 first (src/recon.jl:195).  beta < 0 -> MSLR_ERR_ARG.
+ * Method: Gram matrix (FP32 products, FP64 sum), FP64 Hermitian eigensolve,
+ * reconstruction X_i V_r diag(f) V_r^H.
+ */
+mslr_status_t mslr_svst_batched(mslr_ctx ctx, void *P_dev, int64_t p, int64_t Nt, int64_t Np,
+                                const double *beta_host, int per_patch);
+/* Sum over patches of the nuclear norms (patch_nucnorm, src/recon.jl:138-146). */
+mslr_status_t mslr_patch_nucnorm(mslr_ctx ctx, const void *P_dev, int64_t p, int64_t Nt,
+                                 int64_t Np, double *out);
+/* patchSVST (src/recon.jl:251-256): img2patches -> SVST -> patches2img. */
+mslr_status_t mslr_patch_svst(mslr_ctx ctx, const void *img_dev, void *out_dev,
+                              const int64_t N[3], int64_t Nt, const int64_t ps[3],
+                              const int64_t ss[3], double beta);
+
+/* ------------------------------------------------------------------ POGM */
+/* Forms of the POGM iteration. */
+#define MSLR_FORM_PLAIN   0  /* per-scale launches, FP32 Gram products */
+#define MSLR_FORM_BATCHED 1  /* all scales per launch + hipGraph-captured iteration */
+#define MSLR_FORM_MIXED   2  /* BATCHED with split-BF16 Gram products (FP32 accumulate) */
+
+/* Momentum / restart options of pogm_mod (src/mirt_mod.jl:74-93). */
+#define MSLR_MOM_POGM 0
+#define MSLR_MOM_FPGM 1
+#define MSLR_MOM_PGM  2
+#define MSLR_RESTART_GR   0
+#define MSLR_RESTART_NONE 1
+
+typedef struct {
+    int64_t N[3];
+    int64_t Nt;
+    int64_t nscales;
+    const int64_t *patch_sizes; /* nscales x 3 */
+    const int64_t *strides;     /* nscales x 3 */
+    const double *lambdas;      /* nscales: regularisation weights lambda_k */
+    double L;                   /* Lipschitz constant f_L (nscales * sigma1^2) */
+    int niter;                  /* total iterations of the run (the last one uses the 8*t^2 rule) */
+    int mom;                    /* MSLR_MOM_* (only POGM is implemented on the device) */
+    int restart;                /* MSLR_RESTART_* */
+    double restart_cutoff;      /* must satisfy |restart_cutoff| < 1 */
+    double bsig;                /* gamma decrease factor, >= 0 */
+    int form;                   /* MSLR_FORM_* */
+} mslr_pogm_desc;
+
+/*
+ * POGM state on the device for the aliased in-place prox of the reference
+ * (scripts/reconstruct.jl:196-202 with src/mirt_mod.jl:131-184).  Holds
+ * X (Nx,Ny,Nz,Nt,nscales) and U_old, the per-iteration scalars (t, sigma,
+ * zeta) and a per-iteration restart log.
+ */
+mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, mslr_pogm *out);
+mslr_status_t mslr_pogm_destroy(mslr_pogm st);
+/* Set y (ComplexF32 (K,Nc,Nt), reference sample order) and X0.  X0_dev == NULL
+ * means X0[...,1] = A' y and zeros elsewhere (scripts/reconstruct.jl:209-214). */
+mslr_status_t mslr_pogm_init(mslr_pogm st, const void *y_dev, const void *X0_dev);
+/* Run n further iterations (iter counts continue across calls). */
+mslr_status_t mslr_pogm_run(mslr_pogm st, int n);
+/* Device pointer to the current X (ComplexF32 (Nx,Ny,Nz,Nt,nscales)); valid
+ * until mslr_pogm_destroy.  After the last iteration it is the returned xnew. */
+void *mslr_pogm_x(mslr_pogm st);
+/* Iterations done so far and the restart flags (one per iteration, host array
+ * of length >= iterations done). */
+mslr_status_t mslr_pogm_log(mslr_pogm st, int *iters_done, int *restarts);
+/* Per-scale SVST thresholds zeta*lambda_k used in the last iteration (host, nscales). */
+mslr_status_t mslr_pogm_thresholds(mslr_pogm st, double *beta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSLR_H */

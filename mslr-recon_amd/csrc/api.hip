@@ -1,0 +1,173 @@
+// Context, device buffers and error reporting of the C ABI.
+#include <mutex>
+
+#include "common.hpp"
+
+namespace mslr {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+void clear_last_error() { g_last_error.clear(); }
+
+}  // namespace mslr
+
+using namespace mslr;
+
+void mslr_ctx_s::activate() const {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess)
+        throw Error(MSLR_ERR_DEVICE, std::string("hipSetDevice failed: ") + hipGetErrorString(e));
+}
+
+void *mslr_ctx_s::scratch_get(size_t slot, size_t bytes) {
+    if (slot >= scratch.size()) scratch.resize(slot + 1);
+    Scratch &s = scratch[slot];
+    if (bytes == 0) bytes = 256;
+    if (s.bytes >= bytes) return s.ptr;
+    // Growing: work queued on the old buffer must finish before it is freed.
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (s.ptr) HIP_CHECK(hipFree(s.ptr));
+    s.ptr = nullptr;
+    s.bytes = 0;
+    const size_t want = (bytes + 4095) & ~(size_t)4095;
+    hipError_t e = hipMalloc(&s.ptr, want);
+    if (e != hipSuccess) {
+        s.ptr = nullptr;
+        throw Error(MSLR_ERR_NOMEM, "scratch allocation of " + std::to_string(want) +
+                                        " bytes failed: " + hipGetErrorString(e));
+    }
+    s.bytes = want;
+    return s.ptr;
+}
+
+extern "C" {
+
+const char *mslr_last_error(void) { return g_last_error.c_str(); }
+
+int mslr_version(void) { return 100; }
+
+mslr_status_t mslr_ctx_create(int device, mslr_ctx *out) {
+    return guarded([&] {
+        MSLR_REQUIRE(out != nullptr, MSLR_ERR_ARG, "out is null");
+        *out = nullptr;
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n == 0)
+            throw Error(MSLR_ERR_DEVICE, "no HIP device is available");
+        MSLR_REQUIRE(device >= 0 && device < n, MSLR_ERR_ARG, "device ordinal out of range");
+        HIP_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        auto *c = new mslr_ctx_s();
+        c->device = device;
+        c->num_cus = prop.multiProcessorCount;
+        hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (se != hipSuccess) {
+            delete c;
+            throw Error(MSLR_ERR_DEVICE, std::string("hipStreamCreate failed: ") +
+                                             hipGetErrorString(se));
+        }
+        c->scratch.resize(SLOT_COUNT);
+        *out = c;
+    });
+}
+
+mslr_status_t mslr_ctx_destroy(mslr_ctx ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        ctx->activate();
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (auto &s : ctx->scratch)
+            if (s.ptr) hipFree(s.ptr);
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+mslr_status_t mslr_ctx_sync(mslr_ctx ctx) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+mslr_status_t mslr_ctx_info(mslr_ctx ctx, char *name, size_t name_len, size_t *total_bytes,
+                            size_t *free_bytes) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, c->device));
+        if (name && name_len > 0) {
+            std::snprintf(name, name_len, "%s", prop.name);
+        }
+        size_t fr = 0, tot = 0;
+        HIP_CHECK(hipMemGetInfo(&fr, &tot));
+        if (total_bytes) *total_bytes = tot;
+        if (free_bytes) *free_bytes = fr;
+    });
+}
+
+mslr_status_t mslr_malloc(mslr_ctx ctx, size_t bytes, void **dev_ptr) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        (void)c;
+        MSLR_REQUIRE(dev_ptr != nullptr, MSLR_ERR_ARG, "dev_ptr is null");
+        *dev_ptr = nullptr;
+        hipError_t e = hipMalloc(dev_ptr, bytes == 0 ? 256 : bytes);
+        if (e != hipSuccess) {
+            *dev_ptr = nullptr;
+            throw Error(MSLR_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) +
+                                            " bytes failed: " + hipGetErrorString(e));
+        }
+    });
+}
+
+mslr_status_t mslr_free(mslr_ctx ctx, void *dev_ptr) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        if (!dev_ptr) return;
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        HIP_CHECK(hipFree(dev_ptr));
+    });
+}
+
+mslr_status_t mslr_upload(mslr_ctx ctx, void *dst_dev, const void *src_host, size_t bytes) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        if (bytes == 0) return;
+        MSLR_REQUIRE(dst_dev && src_host, MSLR_ERR_ARG, "null pointer");
+        HIP_CHECK(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+mslr_status_t mslr_download(mslr_ctx ctx, void *dst_host, const void *src_dev, size_t bytes) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        if (bytes == 0) return;
+        MSLR_REQUIRE(dst_host && src_dev, MSLR_ERR_ARG, "null pointer");
+        HIP_CHECK(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+mslr_status_t mslr_copy_d2d(mslr_ctx ctx, void *dst_dev, const void *src_dev, size_t bytes) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        if (bytes == 0) return;
+        MSLR_REQUIRE(dst_dev && src_dev, MSLR_ERR_ARG, "null pointer");
+        HIP_CHECK(hipMemcpyAsync(dst_dev, src_dev, bytes, hipMemcpyDeviceToDevice, c->stream));
+    });
+}
+
+mslr_status_t mslr_zero(mslr_ctx ctx, void *dev_ptr, size_t bytes) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        if (bytes == 0) return;
+        MSLR_REQUIRE(dev_ptr, MSLR_ERR_ARG, "null pointer");
+        HIP_CHECK(hipMemsetAsync(dev_ptr, 0, bytes, c->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// Shared definitions of the MSLR HIP library: complex arithmetic, error
+// reporting, the context (device + stream + workspace) and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mslr.h"
+
+namespace mslr {
+
+// ------------------------------------------------------------------ complex
+template <class T>
+struct C2 {
+    T re, im;
+};
+
+template <class T>
+__host__ __device__ __forceinline__ C2<T> mk(T r, T i) { return C2<T>{r, i}; }
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator+(C2<T> a, C2<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator-(C2<T> a, C2<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator*(C2<T> a, C2<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator*(T s, C2<T> a) { return {s * a.re, s * a.im}; }
+template <class T>
+__host__ __device__ __forceinline__ C2<T> conj(C2<T> a) { return {a.re, -a.im}; }
+// conj(a) * b
+template <class T>
+__host__ __device__ __forceinline__ C2<T> cmulc(C2<T> a, C2<T> b) {
+    return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+template <class T>
+__host__ __device__ __forceinline__ T abs2(C2<T> a) { return a.re * a.re + a.im * a.im; }
+
+using c64 = C2<float>;
+using c128 = C2<double>;
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+    mslr_status_t code;
+    Error(mslr_status_t c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+void clear_last_error();
+
+#define MSLR_FAIL(code, msg)                                               \
+    do {                                                                   \
+        throw ::mslr::Error((code), std::string(__func__) + ": " + (msg)); \
+    } while (0)
+
+#define MSLR_REQUIRE(cond, code, msg) \
+    do {                              \
+        if (!(cond)) MSLR_FAIL(code, msg); \
+    } while (0)
+
+#define HIP_CHECK(expr)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            throw ::mslr::Error(MSLR_ERR_DEVICE, std::string(#expr) + " failed: " +  \
+                                                     hipGetErrorString(e_));          \
+    } while (0)
+
+#define HIP_LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+// Wrap an entry point body: converts exceptions into status codes.
+template <class F>
+mslr_status_t guarded(F &&f) {
+    try {
+        clear_last_error();
+        f();
+        return MSLR_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return MSLR_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return MSLR_ERR_DEVICE;
+    }
+}
+
+// ------------------------------------------------------------------ context
+// Grow-only device scratch arena, reused across calls on one context.
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace mslr
+
+struct mslr_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    std::vector<mslr::Scratch> scratch;  // indexed slots, see ctx_scratch()
+
+    void *scratch_get(size_t slot, size_t bytes);
+    void activate() const;
+};
+
+namespace mslr {
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Lookup of the context behind a handle with validation.
+inline mslr_ctx_s *check_ctx(mslr_ctx c) {
+    if (!c) throw Error(MSLR_ERR_ARG, "null context");
+    c->activate();
+    return c;
+}
+
+// Scratch slots used by the library (one arena per slot per context).
+enum ScratchSlot : size_t {
+    SLOT_FFT = 0,       // SENSE intermediate (coil k-space planes)
+    SLOT_RESID = 1,     // SENSE residual / staging
+    SLOT_REDUCE = 2,    // FP64 partial sums
+    SLOT_GRAM_PART = 3, // split-K partial Gram tiles (FP32)
+    SLOT_GRAM = 4,      // FP64 Gram matrices / eigen workspace
+    SLOT_EIG = 5,       // eigenvectors + filter factors
+    SLOT_APPLY = 6,     // apply intermediate X V_r
+    SLOT_PATCH = 7,     // materialised patches
+    SLOT_PATCH_OUT = 8, // image staging for patch operators
+    SLOT_PWR = 9,       // power-iteration vectors
+    SLOT_COUNT = 10
+};
+
+}  // namespace mslr

@@ -1,0 +1,37 @@
+// Patch geometry (src/recon.jl:53-78) and the patch gather / scatter kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace mslr {
+
+struct PatchGeom {
+    int N[3];     // image size
+    int ps[3];    // patch size, clamped to the image
+    int ss[3];    // stride
+    int ns[3];    // patches per axis (Nsteps + 1)
+    int64_t Np;   // number of patches
+    int64_t p;    // voxels per patch
+    int Nt;
+
+    // 0-based start of patch index i along axis a: min(i*s, N - ps).
+    __host__ __device__ __forceinline__ int start(int a, int i) const {
+        const int s = i * ss[a];
+        const int lim = N[a] - ps[a];
+        return s < lim ? s : lim;
+    }
+    // true when no voxel is covered by more than one patch
+    bool disjoint() const;
+    // true when every voxel is covered exactly once (non-overlapping tiling)
+    bool exact_tiling() const;
+};
+
+PatchGeom make_geom(const int64_t N[3], int64_t Nt, const int64_t ps[3], const int64_t ss[3]);
+
+// P (p, Nt, Np) <- img (Nx,Ny,Nz,Nt), patches [ip0, ip0+np) only (P holds those np patches).
+void launch_img2patches(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *P,
+                        int64_t ip0, int64_t np);
+// img <- sum of covering patches / max(count, 1), summed in patch order.
+void launch_patches2img(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *P, c64 *img);
+
+}  // namespace mslr

@@ -1,0 +1,399 @@
+// POGM iteration on the device (src/mirt_mod.jl:111-185 with the driver
+// closures of scripts/reconstruct.jl:174-202).
+//
+// With the reference's in-place prox (g_prox overwrites and returns its
+// argument), xnew and znew are the same array, so per iteration
+//   g   = A'(A sum_s X_s - y)                      (same g for every scale)
+//   u   = x - alpha g
+//   z   = u + beta (u - u_old) + gamma (u - x)     (the zeta_old term is 0)
+//   x   = patchSVST_s(z_s, zeta lambda_s)  for each scale s
+//   restart iff sum Re(-g .* (u - u_old)) <= cutoff ||g|| ||u - u_old||
+// The scalar recurrence (t, beta, gamma, zeta, sigma) lives in device memory
+// and is advanced by single-workgroup kernels, so an iteration has no host
+// synchronisation and can be captured in a hipGraph.
+#include <algorithm>
+#include <cmath>
+
+#include "patches.hpp"
+#include "reduce.hpp"
+#include "sense.hpp"
+#include "svst.hpp"
+
+namespace mslr {
+
+struct PogmDev {
+    double alpha;
+    double told, sig, tnew;
+    double beta, gamma, zeta;
+    double restart_cutoff;
+    int iter;    // iterations completed
+    int niter;   // iterations of the run
+    int restart; // MSLR_RESTART_*
+    int err;     // set when a Gram matrix held non-finite values
+};
+
+constexpr int UPD_BLOCKS = 2048;
+
+__global__ void k_pogm_pre(PogmDev *s, const double *__restrict__ lambdas, int nscales,
+                           double *__restrict__ thresholds) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int k = s->iter + 1;
+    const double told = s->told;
+    const double tnew = (k == s->niter) ? 0.5 * (1.0 + sqrt(1.0 + 8.0 * told * told))
+                                        : 0.5 * (1.0 + sqrt(1.0 + 4.0 * told * told));
+    const double beta = (told - 1.0) / tnew;
+    const double gamma = s->sig * told / tnew;
+    const double zeta = s->alpha * (1.0 + beta + gamma);
+    s->tnew = tnew;
+    s->beta = beta;
+    s->gamma = gamma;
+    s->zeta = zeta;
+    for (int i = 0; i < nscales; ++i) thresholds[i] = zeta * lambdas[i];
+}
+
+// X <- z, U <- u; block partials of (restart dot, ||g||^2, ||u - u_old||^2).
+__global__ __launch_bounds__(256) void k_pogm_update(c64 *__restrict__ X, c64 *__restrict__ U,
+                                                     const c64 *__restrict__ g, int64_t n,
+                                                     int nscales, int64_t sstride,
+                                                     const PogmDev *__restrict__ s,
+                                                     double *__restrict__ partials) {
+    __shared__ double red[16];
+    const double alpha = s->alpha, beta = s->beta, gamma = s->gamma;
+    double dot = 0.0, gg = 0.0, dd = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const c64 gv = g[e];
+        const double gr = gv.re, gi = gv.im;
+        for (int k = 0; k < nscales; ++k) {
+            const int64_t i = (int64_t)k * sstride + e;
+            const c64 xv = X[i], uo = U[i];
+            const double ur = xv.re - alpha * gr, ui = xv.im - alpha * gi;
+            const double dr = ur - uo.re, di = ui - uo.im;
+            const double zr = ur + beta * dr + gamma * (ur - xv.re);
+            const double zi = ui + beta * di + gamma * (ui - xv.im);
+            X[i] = c64{(float)zr, (float)zi};
+            U[i] = c64{(float)ur, (float)ui};
+            dot -= gr * dr - gi * di;  // Re(-g .* d), no conjugate
+            gg += gr * gr + gi * gi;
+            dd += dr * dr + di * di;
+        }
+    }
+    const double a = block_sum(dot, red);
+    const double b = block_sum(gg, red);
+    const double c = block_sum(dd, red);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = a;
+        partials[3 * blockIdx.x + 1] = b;
+        partials[3 * blockIdx.x + 2] = c;
+    }
+}
+
+__global__ void k_pogm_post(PogmDev *s, const double *__restrict__ partials, int nblk,
+                            int *__restrict__ restart_log) {
+    __shared__ double red[16];
+    double a = 0.0, b = 0.0, c = 0.0;
+    const int per = (nblk + blockDim.x - 1) / blockDim.x;
+    const int i0 = threadIdx.x * per;
+    const int i1 = min(nblk, i0 + per);
+    for (int i = i0; i < i1; ++i) {
+        a += partials[3 * i];
+        b += partials[3 * i + 1];
+        c += partials[3 * i + 2];
+    }
+    a = block_sum(a, red);
+    b = block_sum(b, red);
+    c = block_sum(c, red);
+    if (threadIdx.x == 0) {
+        const int k = s->iter + 1;
+        int rs = 0;
+        if (s->restart == MSLR_RESTART_GR) rs = a <= s->restart_cutoff * sqrt(b) * sqrt(c);
+        if (rs) {
+            s->tnew = 1.0;
+            s->sig = 1.0;
+        }
+        restart_log[k - 1] = rs;
+        s->told = s->tnew;
+        s->iter = k;
+    }
+}
+
+__global__ void k_fill_zero(c64 *__restrict__ x, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = c64{0.f, 0.f};
+}
+
+static void fill_zero(mslr_ctx_s *ctx, c64 *x, int64_t n) {
+    if (n <= 0) return;
+    const unsigned nb = (unsigned)std::min<int64_t>(cdiv(n, 256), 1 << 16);
+    k_fill_zero<<<nb, 256, 0, ctx->stream>>>(x, n);
+    HIP_LAUNCH_CHECK();
+}
+
+// patchSVST for one scale: out <- patches2img(SVST(img2patches(img))).
+static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *out,
+                       const double *beta_dev, int *err) {
+    c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, (size_t)std::max<int64_t>(1, g.p * g.Nt * g.Np) *
+                                                     sizeof(c64));
+    launch_img2patches(ctx, g, img, P, 0, g.Np);
+    svst_batched(ctx, P, g.p, g.Nt, g.Np, beta_dev, 0, err);
+    launch_patches2img(ctx, g, P, out);
+}
+
+}  // namespace mslr
+
+using namespace mslr;
+
+struct mslr_pogm_s {
+    mslr_sense_s *sense = nullptr;
+    mslr_ctx_s *ctx = nullptr;
+    int64_t V = 0, Nt = 0, S = 0, K = 0, Nc = 0;
+    int niter = 0, form = 0;
+    std::vector<PatchGeom> geoms;
+    std::vector<double> lambdas;
+    double L = 0;
+    // device
+    c64 *X = nullptr, *U = nullptr, *g = nullptr, *y = nullptr;
+    PogmDev *dev = nullptr;
+    double *lam_dev = nullptr, *thr_dev = nullptr, *partials = nullptr;
+    int *log_dev = nullptr;
+    int iters_done = 0;
+    bool initialised = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> allocs;
+
+    void *alloc(size_t b) {
+        void *p = nullptr;
+        HIP_CHECK(hipMalloc(&p, std::max<size_t>(b, 256)));
+        allocs.push_back(p);
+        return p;
+    }
+};
+
+static void pogm_free(mslr_pogm_s *st) {
+    if (!st) return;
+    if (st->exec) hipGraphExecDestroy(st->exec);
+    if (st->graph) hipGraphDestroy(st->graph);
+    for (void *p : st->allocs) hipFree(p);
+    delete st;
+}
+
+// Enqueue one iteration on the context stream.
+static void pogm_iteration(mslr_pogm_s *st) {
+    mslr_ctx_s *ctx = st->ctx;
+    const int64_t n = st->V * st->Nt;
+    sense_grad_internal(st->sense, st->X, (int)st->S, n, st->y, st->g, nullptr);
+    k_pogm_pre<<<1, 64, 0, ctx->stream>>>(st->dev, st->lam_dev, (int)st->S, st->thr_dev);
+    HIP_LAUNCH_CHECK();
+    k_pogm_update<<<UPD_BLOCKS, 256, 0, ctx->stream>>>(st->X, st->U, st->g, n, (int)st->S, n,
+                                                       st->dev, st->partials);
+    HIP_LAUNCH_CHECK();
+    k_pogm_post<<<1, 256, 0, ctx->stream>>>(st->dev, st->partials, UPD_BLOCKS, st->log_dev);
+    HIP_LAUNCH_CHECK();
+    int *err = (int *)((char *)st->dev + offsetof(PogmDev, err));
+    for (int64_t k = 0; k < st->S; ++k) {
+        c64 *Xk = st->X + k * n;
+        prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err);
+    }
+}
+
+extern "C" {
+
+mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, mslr_pogm *out) {
+    return guarded([&] {
+        MSLR_REQUIRE(out && sense && desc, MSLR_ERR_ARG, "null argument");
+        *out = nullptr;
+        sense->ctx->activate();
+        const mslr_pogm_desc &d = *desc;
+        MSLR_REQUIRE(d.mom == MSLR_MOM_POGM || d.mom == MSLR_MOM_FPGM || d.mom == MSLR_MOM_PGM,
+                     MSLR_ERR_ARG, "mom must be pogm, fpgm or pgm");
+        MSLR_REQUIRE(d.restart == MSLR_RESTART_GR || d.restart == MSLR_RESTART_NONE,
+                     MSLR_ERR_ARG, "restart must be gr or none");
+        MSLR_REQUIRE(d.L >= 0, MSLR_ERR_ARG, "f_L < 0");
+        MSLR_REQUIRE(d.bsig >= 0, MSLR_ERR_ARG, "bsig < 0");
+        MSLR_REQUIRE(std::fabs(d.restart_cutoff) < 1, MSLR_ERR_ARG, "restart_cutoff");
+        MSLR_REQUIRE(d.mom == MSLR_MOM_POGM, MSLR_ERR_UNSUPPORTED,
+                     "the device loop implements mom = pogm");
+        MSLR_REQUIRE(d.bsig == 1.0, MSLR_ERR_UNSUPPORTED, "the device loop implements bsig = 1");
+        MSLR_REQUIRE(d.L > 0, MSLR_ERR_ARG, "f_L must be > 0");
+        MSLR_REQUIRE(d.niter >= 0, MSLR_ERR_ARG, "niter < 0");
+        MSLR_REQUIRE(d.nscales >= 1 && d.patch_sizes && d.strides && d.lambdas, MSLR_ERR_ARG,
+                     "scales");
+        MSLR_REQUIRE(d.N[0] == sense->N[0] && d.N[1] == sense->N[1] && d.N[2] == sense->N[2] &&
+                         d.Nt == sense->Nt,
+                     MSLR_ERR_SHAPE, "image size differs from the SENSE plan");
+        MSLR_REQUIRE(d.form == MSLR_FORM_PLAIN || d.form == MSLR_FORM_BATCHED ||
+                         d.form == MSLR_FORM_MIXED,
+                     MSLR_ERR_ARG, "form");
+        auto *st = new mslr_pogm_s();
+        try {
+            st->sense = sense;
+            st->ctx = sense->ctx;
+            st->V = sense->V;
+            st->Nt = sense->Nt;
+            st->S = d.nscales;
+            st->K = sense->K;
+            st->Nc = sense->Nc;
+            st->niter = d.niter;
+            st->form = d.form;
+            st->L = d.L;
+            for (int64_t k = 0; k < d.nscales; ++k) {
+                st->geoms.push_back(make_geom(d.N, d.Nt, d.patch_sizes + 3 * k, d.strides + 3 * k));
+                st->lambdas.push_back(d.lambdas[k]);
+            }
+            const int64_t n = st->V * st->Nt;
+            st->X = (c64 *)st->alloc((size_t)(n * st->S) * sizeof(c64));
+            st->U = (c64 *)st->alloc((size_t)(n * st->S) * sizeof(c64));
+            st->g = (c64 *)st->alloc((size_t)n * sizeof(c64));
+            st->y = (c64 *)st->alloc((size_t)std::max<int64_t>(1, st->K * st->Nc * st->Nt) *
+                                     sizeof(c64));
+            st->dev = (PogmDev *)st->alloc(sizeof(PogmDev));
+            st->lam_dev = (double *)st->alloc(sizeof(double) * st->S);
+            st->thr_dev = (double *)st->alloc(sizeof(double) * st->S);
+            st->partials = (double *)st->alloc(sizeof(double) * 3 * UPD_BLOCKS);
+            st->log_dev = (int *)st->alloc(sizeof(int) * std::max(1, d.niter));
+            HIP_CHECK(hipMemcpy(st->lam_dev, st->lambdas.data(), sizeof(double) * st->S,
+                                hipMemcpyHostToDevice));
+            PogmDev h{};
+            h.alpha = 1.0 / d.L;
+            h.told = 1.0;
+            h.sig = 1.0;
+            h.tnew = 1.0;
+            h.restart_cutoff = d.restart_cutoff;
+            h.iter = 0;
+            h.niter = d.niter;
+            h.restart = d.restart;
+            HIP_CHECK(hipMemcpy(st->dev, &h, sizeof(h), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemset(st->log_dev, 0, sizeof(int) * std::max(1, d.niter)));
+            *out = st;
+        } catch (...) {
+            pogm_free(st);
+            throw;
+        }
+    });
+}
+
+mslr_status_t mslr_pogm_destroy(mslr_pogm st) {
+    return guarded([&] {
+        if (!st) return;
+        st->ctx->activate();
+        HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+        pogm_free(st);
+    });
+}
+
+mslr_status_t mslr_pogm_init(mslr_pogm st, const void *y_dev, const void *X0_dev) {
+    return guarded([&] {
+        MSLR_REQUIRE(st && y_dev, MSLR_ERR_ARG, "null argument");
+        mslr_ctx_s *ctx = st->ctx;
+        ctx->activate();
+        const int64_t n = st->V * st->Nt;
+        sense_ref_to_internal(st->sense, (const c64 *)y_dev, st->y);
+        if (X0_dev) {
+            HIP_CHECK(hipMemcpyAsync(st->X, X0_dev, (size_t)(n * st->S) * sizeof(c64),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            sense_adjoint_internal(st->sense, st->y, st->X);
+            fill_zero(ctx, st->X + n, n * (st->S - 1));
+        }
+        HIP_CHECK(hipMemcpyAsync(st->U, st->X, (size_t)(n * st->S) * sizeof(c64),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+        // Reset the scalar recurrence (told = sig = 1, src/mirt_mod.jl:100).
+        PogmDev cur{};
+        HIP_CHECK(hipMemcpyAsync(&cur, st->dev, sizeof(cur), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        cur.told = 1.0;
+        cur.sig = 1.0;
+        cur.tnew = 1.0;
+        cur.iter = 0;
+        cur.err = 0;
+        HIP_CHECK(hipMemcpyAsync(st->dev, &cur, sizeof(cur), hipMemcpyHostToDevice, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        st->iters_done = 0;
+        st->initialised = true;
+    });
+}
+
+mslr_status_t mslr_pogm_run(mslr_pogm st, int n) {
+    return guarded([&] {
+        MSLR_REQUIRE(st, MSLR_ERR_ARG, "null state");
+        MSLR_REQUIRE(st->initialised, MSLR_ERR_ARG, "mslr_pogm_init has not been called");
+        MSLR_REQUIRE(n >= 0 && st->iters_done + n <= st->niter, MSLR_ERR_ARG,
+                     "more iterations than niter");
+        mslr_ctx_s *ctx = st->ctx;
+        ctx->activate();
+        for (int i = 0; i < n; ++i) {
+            const bool use_graph = st->form != MSLR_FORM_PLAIN;
+            if (use_graph && st->exec) {
+                HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
+            } else if (use_graph && st->iters_done >= 1) {
+                // Scratch is sized by the eager first iteration; capture the next.
+                HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    pogm_iteration(st);
+                } catch (...) {
+                    hipGraph_t gtmp = nullptr;
+                    hipStreamEndCapture(ctx->stream, &gtmp);
+                    if (gtmp) hipGraphDestroy(gtmp);
+                    throw;
+                }
+                HIP_CHECK(hipStreamEndCapture(ctx->stream, &st->graph));
+                HIP_CHECK(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0));
+                HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
+            } else {
+                pogm_iteration(st);
+            }
+            st->iters_done++;
+        }
+    });
+}
+
+void *mslr_pogm_x(mslr_pogm st) { return st ? (void *)st->X : nullptr; }
+
+mslr_status_t mslr_pogm_log(mslr_pogm st, int *iters_done, int *restarts) {
+    return guarded([&] {
+        MSLR_REQUIRE(st, MSLR_ERR_ARG, "null state");
+        mslr_ctx_s *ctx = st->ctx;
+        ctx->activate();
+        PogmDev cur{};
+        HIP_CHECK(hipMemcpyAsync(&cur, st->dev, sizeof(cur), hipMemcpyDeviceToHost, ctx->stream));
+        if (restarts && st->iters_done > 0)
+            HIP_CHECK(hipMemcpyAsync(restarts, st->log_dev, sizeof(int) * st->iters_done,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (iters_done) *iters_done = cur.iter;
+        MSLR_REQUIRE(cur.err == 0, MSLR_ERR_NUMERIC, "non-finite values in a patch matrix");
+    });
+}
+
+mslr_status_t mslr_pogm_thresholds(mslr_pogm st, double *beta) {
+    return guarded([&] {
+        MSLR_REQUIRE(st && beta, MSLR_ERR_ARG, "null argument");
+        st->ctx->activate();
+        HIP_CHECK(hipMemcpyAsync(beta, st->thr_dev, sizeof(double) * st->S, hipMemcpyDeviceToHost,
+                                 st->ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+    });
+}
+
+mslr_status_t mslr_patch_svst(mslr_ctx ctx, const void *img_dev, void *out_dev,
+                              const int64_t N[3], int64_t Nt, const int64_t ps[3],
+                              const int64_t ss[3], double beta) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(beta >= 0 && std::isfinite(beta), MSLR_ERR_ARG, "beta must be >= 0");
+        PatchGeom g = make_geom(N, Nt, ps, ss);
+        double *bd = (double *)c->scratch_get(SLOT_APPLY, 64);
+        int *err = (int *)(bd + 1);
+        HIP_CHECK(hipMemcpyAsync(bd, &beta, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+        prox_scale(c, g, (const c64 *)img_dev, (c64 *)out_dev, bd, err);
+        int herr = 0;
+        HIP_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        MSLR_REQUIRE(herr == 0, MSLR_ERR_NUMERIC, "non-finite values in a patch matrix");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,883 @@
+// Batched SVST (src/recon.jl:183-211, 217-241; src/sense_gpu.jl:122-128).
+//
+// For a patch matrix X (p x Nt) with m = min(p, Nt):
+//   1. Gram G = X^H X (p >= Nt) or X X^H (p < Nt): FP32 products on 64x64
+//      tiles, split over the contraction length, partial tiles summed in FP64.
+//   2. FP64 Hermitian eigensolve of G, restricted to eigenvalues above
+//      tau = beta^2: Householder reduction to a real tridiagonal T (one
+//      workgroup per patch), Sturm-count bisection for the eigenvalues above
+//      tau (the count is exact, so no singular value above beta is missed),
+//      inverse iteration with in-cluster re-orthogonalisation for their
+//      eigenvectors, and back-transformation through the reflectors with one
+//      wavefront per eigenvector.
+//   3. Y = X V_r diag(f) V_r^H (or V_r diag(f) V_r^H X), f = max(s - beta, 0)/s,
+//      fused per block of rows (columns) so X is read once and written in place.
+// SVST(X) = U max(S - beta, 0) V^H = X V diag(max(s-beta,0)/s) V^H, so the
+// output equals the reference's U*Diagonal(s_thresh)*Vt with only the
+// right (left) singular vectors formed.  A zero patch has no eigenvalue
+// above tau > 0 and maps to zero (src/recon.jl:185).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "reduce.hpp"
+#include "svst.hpp"
+
+namespace mslr {
+
+constexpr int GT = 64;       // Gram tile edge
+constexpr int GK = 16;       // Gram k-step
+constexpr int KCH = 4096;    // contraction chunk per workgroup
+constexpr int MAXM = 1024;   // largest Gram dimension
+constexpr int NSLOT = 64;    // concurrent inverse-iteration solves per patch
+constexpr int MAXQ = MAXM / 64;
+
+// ------------------------------------------------------------------ Gram
+
+// C[a,b] += sum_k conj(A(k,a)) A(k,b), A(k,a) = base[k*ks + a*cs], for the
+// upper tile pair (ti <= tj) of patch ip and contraction chunk kc.
+__global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, int64_t pstride,
+                                                    int64_t ks, int64_t cs, int m, int64_t Kc,
+                                                    int nkc, int ntile, c64 *__restrict__ part) {
+    __shared__ c64 As[GK][GT + 1];
+    __shared__ c64 Bs[GK][GT + 1];
+    const int ip = blockIdx.z;
+    const int kc = blockIdx.y;
+    // linear upper-triangular tile index -> (ti, tj), ti <= tj
+    int rem = blockIdx.x, ti = 0;
+    while (rem >= ntile - ti) {
+        rem -= ntile - ti;
+        ++ti;
+    }
+    const int tj = ti + rem;
+    const int a0 = ti * GT, b0 = tj * GT;
+    const c64 *A = P + ip * pstride;
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    c64 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = c64{0.f, 0.f};
+    const int64_t kbeg = (int64_t)kc * KCH;
+    const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
+    const bool kfast = ks == 1;
+    for (int64_t k0 = kbeg; k0 < kend; k0 += GK) {
+#pragma unroll
+        for (int r = 0; r < (GK * GT) / 256; ++r) {
+            const int idx = tid + 256 * r;
+            int kk, aa;
+            if (kfast) {
+                kk = idx % GK;
+                aa = idx / GK;
+            } else {
+                aa = idx % GT;
+                kk = idx / GT;
+            }
+            const int64_t k = k0 + kk;
+            c64 va = {0.f, 0.f}, vb = {0.f, 0.f};
+            if (k < kend) {
+                if (a0 + aa < m) va = A[k * ks + (int64_t)(a0 + aa) * cs];
+                if (b0 + aa < m) vb = A[k * ks + (int64_t)(b0 + aa) * cs];
+            }
+            As[kk][aa] = va;
+            Bs[kk][aa] = vb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk) {
+            c64 av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = As[kk][tx + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    // conj(a) * b
+                    acc[i][j].re += av[i].re * bv[j].re + av[i].im * bv[j].im;
+                    acc[i][j].im += av[i].re * bv[j].im - av[i].im * bv[j].re;
+                }
+        }
+        __syncthreads();
+    }
+    c64 *out = part + ((int64_t)ip * nkc + kc) * (int64_t)m * m;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int a = a0 + tx + 16 * i, b = b0 + ty + 16 * j;
+            if (a < m && b < m) out[a + (int64_t)m * b] = acc[i][j];
+        }
+}
+
+// G[a,b] = sum_kc part (FP64), Hermitian fill; conj_out flips to conj(C).
+__global__ void k_gram_reduce(const c64 *__restrict__ part, int m, int nkc, int np, int conj_out,
+                              c128 *__restrict__ G, int *err) {
+    const int64_t mm = (int64_t)m * m;
+    const int64_t n = mm * np;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ip = e / mm;
+        const int64_t r = e - ip * mm;
+        const int a = (int)(r % m), b = (int)(r / m);
+        if (a > b) continue;
+        double sr = 0.0, si = 0.0;
+        const c64 *pp = part + ip * nkc * mm + r;
+        for (int kc = 0; kc < nkc; ++kc) {
+            const c64 v = pp[kc * mm];
+            sr += (double)v.re;
+            si += (double)v.im;
+        }
+        if (conj_out) si = -si;
+        if (!isfinite(sr) || !isfinite(si)) {
+            if (err) *err = 1;
+            sr = 0.0;
+            si = 0.0;
+        }
+        c128 *g = G + ip * mm;
+        if (a == b) {
+            g[r] = c128{sr, 0.0};
+        } else {
+            g[r] = c128{sr, si};
+            g[b + (int64_t)m * a] = c128{sr, -si};
+        }
+    }
+}
+
+// ------------------------------------------------------------------ tridiagonalisation
+
+__device__ __forceinline__ c128 cdiv(c128 a, c128 b) {
+    const double den = b.re * b.re + b.im * b.im;
+    return c128{(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+}
+
+// Lower Householder reduction (the zhetd2 sequence): Q^H G Q = T with
+// Q = H(0) H(1) ... H(m-2), H(j) = I - tau_j v_j v_j^H, v_j(j+1) = 1,
+// v_j(i) stored in G[i, j] for i > j+1.  d, e real.
+__global__ __launch_bounds__(256) void k_tridiag(c128 *__restrict__ Gall, int m,
+                                                 double *__restrict__ dall,
+                                                 double *__restrict__ eall,
+                                                 c128 *__restrict__ tall) {
+    __shared__ c128 v[MAXM];
+    __shared__ c128 w[MAXM];
+    __shared__ double red[16];
+    __shared__ c128 s_tau;
+    __shared__ c128 s_scal;
+    __shared__ double s_beta;
+    __shared__ c128 s_dot;
+    const int ip = blockIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    c128 *A = Gall + (int64_t)ip * m * m;
+    double *d = dall + (int64_t)ip * m;
+    double *e = eall + (int64_t)ip * m;
+    c128 *tv = tall + (int64_t)ip * m;
+    for (int j = 0; j + 1 < m; ++j) {
+        const int n = m - j - 1;
+        c128 *col = A + (int64_t)m * j;
+        double xn = 0.0;
+        for (int i = j + 2 + tid; i < m; i += nt) xn += abs2(col[i]);
+        xn = block_sum(xn, red);
+        if (tid == 0) {
+            const c128 alpha = col[j + 1];
+            if (xn == 0.0 && alpha.im == 0.0) {
+                s_tau = c128{0.0, 0.0};
+                s_beta = alpha.re;
+            } else {
+                const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn);
+                const double beta = alpha.re >= 0.0 ? -nrm : nrm;
+                s_tau = c128{(beta - alpha.re) / beta, -alpha.im / beta};
+                s_scal = cdiv(c128{1.0, 0.0}, c128{alpha.re - beta, alpha.im});
+                s_beta = beta;
+            }
+            d[j] = A[j + (int64_t)m * j].re;
+            e[j] = s_beta;
+            tv[j] = s_tau;
+        }
+        __syncthreads();
+        const c128 tau = s_tau;
+        if (tau.re == 0.0 && tau.im == 0.0) continue;
+        const c128 scal = s_scal;
+        for (int i = tid; i < n; i += nt) {
+            c128 vi;
+            if (i == 0) {
+                vi = c128{1.0, 0.0};
+            } else {
+                vi = col[j + 1 + i] * scal;
+                col[j + 1 + i] = vi;
+            }
+            v[i] = vi;
+        }
+        __syncthreads();
+        // w = tau * A22 * v
+        const c128 *A22 = A + (j + 1) + (int64_t)m * (j + 1);
+        for (int i = tid; i < n; i += nt) {
+            double ar = 0.0, ai = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const c128 a = A22[i + (int64_t)m * k];
+                const c128 b = v[k];
+                ar += a.re * b.re - a.im * b.im;
+                ai += a.re * b.im + a.im * b.re;
+            }
+            w[i] = tau * c128{ar, ai};
+        }
+        __syncthreads();
+        // alpha2 = -1/2 tau (w^H v); w += alpha2 v
+        double dr = 0.0, di = 0.0;
+        for (int i = tid; i < n; i += nt) {
+            const c128 t = cmulc(w[i], v[i]);
+            dr += t.re;
+            di += t.im;
+        }
+        dr = block_sum(dr, red);
+        di = block_sum(di, red);
+        if (tid == 0) s_dot = c128{dr, di};
+        __syncthreads();
+        const c128 alpha2 = (-0.5) * (tau * s_dot);
+        for (int i = tid; i < n; i += nt) w[i] = w[i] + alpha2 * v[i];
+        __syncthreads();
+        // A22 -= v w^H + w v^H
+        c128 *B = A + (j + 1) + (int64_t)m * (j + 1);
+        const int64_t nn = (int64_t)n * n;
+        for (int64_t idx = tid; idx < nn; idx += nt) {
+            const int i = (int)(idx % n), k = (int)(idx / n);
+            const c128 vi = v[i], wi = w[i], vk = v[k], wk = w[k];
+            c128 t = B[i + (int64_t)m * k];
+            t.re -= vi.re * wk.re + vi.im * wk.im + wi.re * vk.re + wi.im * vk.im;
+            t.im -= vi.im * wk.re - vi.re * wk.im + wi.im * vk.re - wi.re * vk.im;
+            B[i + (int64_t)m * k] = t;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        d[m - 1] = A[(m - 1) + (int64_t)m * (m - 1)].re;
+        e[m - 1] = 0.0;
+        tv[m - 1] = c128{0.0, 0.0};
+    }
+}
+
+// ------------------------------------------------------------------ tridiagonal eigenpairs
+
+// Number of eigenvalues of T smaller than x.
+__device__ int sturm_count(const double *d, const double *e, int m, double x, double pivmin) {
+    int c = 0;
+    double q = d[0] - x;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    c += q < 0.0;
+    for (int i = 1; i < m; ++i) {
+        q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        c += q < 0.0;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double hash_unit(uint32_t a, uint32_t b) {
+    uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return (double)(h & 0xFFFFFF) / 8388608.0 - 1.0;  // [-1, 1)
+}
+
+// Solve (T - lam I) x = x in place by Gaussian elimination with partial
+// pivoting (row interchanges between i and i+1).  u0,u1,u2 hold the upper
+// factor diagonals; arrays are strided by `st` (interleaved slots).
+__device__ void tri_solve(const double *d, const double *e, int m, double lam, double tiny,
+                          double *x, double *u0, double *u1, double *u2, int st) {
+    // forward elimination applied to x
+    double a = d[0] - lam;            // current pivot row diagonal
+    double b = m > 1 ? e[0] : 0.0;    // current pivot row super-diagonal
+    double c = 0.0;                   // current pivot row second super-diagonal
+    double xi = x[0];
+    for (int i = 0; i + 1 < m; ++i) {
+        const double sub = e[i];          // T[i+1, i]
+        const double dn = d[i + 1] - lam; // T[i+1, i+1]
+        const double en = (i + 2 < m) ? e[i + 1] : 0.0;  // T[i+1, i+2]
+        const double xn = x[(i + 1) * st];
+        if (fabs(a) >= fabs(sub)) {
+            const double piv = a != 0.0 ? a : tiny;
+            const double l = sub / piv;
+            u0[i * st] = piv;
+            u1[i * st] = b;
+            u2[i * st] = c;
+            x[i * st] = xi;
+            a = dn - l * b;
+            b = en - l * c;
+            c = 0.0;
+            xi = xn - l * xi;
+        } else {
+            // swap rows i and i+1
+            const double l = a / sub;
+            u0[i * st] = sub;
+            u1[i * st] = dn;
+            u2[i * st] = en;
+            x[i * st] = xn;
+            const double na = b - l * dn;
+            const double nb = c - l * en;
+            a = na;
+            b = nb;
+            c = 0.0;
+            xi = xi - l * xn;
+        }
+    }
+    u0[(m - 1) * st] = a != 0.0 ? a : tiny;
+    x[(m - 1) * st] = xi;
+    // back substitution
+    for (int i = m - 1; i >= 0; --i) {
+        double s = x[i * st];
+        if (i + 1 < m) s -= u1[i * st] * x[(i + 1) * st];
+        if (i + 2 < m) s -= u2[i * st] * x[(i + 2) * st];
+        x[i * st] = s / u0[i * st];
+    }
+}
+
+__device__ __forceinline__ c128 wave_sum(c128 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        v.re += __shfl_xor(v.re, o, 64);
+        v.im += __shfl_xor(v.im, o, 64);
+    }
+    return v;
+}
+
+// Eigenpairs of G above tau = beta^2 -> U (m x r, ComplexF32), filter factors f, r.
+// r = -1 marks beta == 0 (identity).
+__global__ __launch_bounds__(256) void k_eigvec(
+    const c128 *__restrict__ Gall, const double *__restrict__ dall, const double *__restrict__ eall,
+    const c128 *__restrict__ tall, int m, const double *__restrict__ beta, int beta_per_patch,
+    double *__restrict__ lamall, double *__restrict__ Zall, double *__restrict__ wsall,
+    c64 *__restrict__ Uall, float *__restrict__ fall, int *__restrict__ rall) {
+    __shared__ int s_r;
+    __shared__ double s_tnorm, s_pivmin, s_gl, s_gu;
+    const int ip = blockIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double *d = dall + (int64_t)ip * m;
+    const double *e = eall + (int64_t)ip * m;
+    double *lam = lamall + (int64_t)ip * m;
+    double *Z = Zall + (int64_t)ip * m * m;
+    const float b32 = (float)beta[beta_per_patch ? ip : 0];
+    const double bd = (double)b32;
+    if (bd == 0.0) {
+        if (tid == 0) rall[ip] = -1;
+        return;
+    }
+    const double tau = bd * bd;
+    if (tid == 0) {
+        double tn = 0.0, emax = 0.0, gl = INFINITY, gu = -INFINITY;
+        for (int i = 0; i < m; ++i) {
+            const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
+            const double er = i + 1 < m ? fabs(e[i]) : 0.0;
+            tn = fmax(tn, fabs(d[i]) + el + er);
+            gl = fmin(gl, d[i] - el - er);
+            gu = fmax(gu, d[i] + el + er);
+            if (i + 1 < m) emax = fmax(emax, e[i] * e[i]);
+        }
+        const double pivmin = DBL_MIN * fmax(1.0, emax);
+        s_tnorm = tn;
+        s_pivmin = pivmin;
+        s_gl = gl;
+        s_gu = gu + 2.0 * DBL_EPSILON * fmax(fabs(gl), fabs(gu)) + pivmin;
+        int r = 0;
+        if (tau < s_gu) r = m - sturm_count(d, e, m, tau, pivmin);
+        s_r = r;
+        rall[ip] = r;
+    }
+    __syncthreads();
+    const int r = s_r;
+    if (r == 0) return;
+    const double pivmin = s_pivmin, tnorm = s_tnorm, gu = s_gu;
+    // bisection: lam[k] = k-th largest eigenvalue (k < r), all above tau
+    for (int k = tid; k < r; k += nt) {
+        const int idx = m - 1 - k;  // ascending index
+        double lo = tau, hi = gu;
+        for (int it = 0; it < 200; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+            if (mid <= lo || mid >= hi) break;
+            if (sturm_count(d, e, m, mid, pivmin) > idx)
+                hi = mid;
+            else
+                lo = mid;
+        }
+        lam[k] = 0.5 * (lo + hi);
+    }
+    __syncthreads();
+    // inverse iteration, one thread per cluster (clusters are runs of
+    // eigenvalues closer than ortol, processed in descending order)
+    const double ortol = 1e-3 * tnorm;
+    const double pertol = 10.0 * DBL_EPSILON * fmax(tnorm, DBL_MIN);
+    const double tiny = DBL_EPSILON * fmax(tnorm, DBL_MIN);
+    double *ws = wsall + (int64_t)ip * NSLOT * 4 * m;
+    // The first wavefront owns the NSLOT solve slots; thread tid takes the
+    // cluster starts tid, tid + NSLOT, ...
+    if (tid < NSLOT) {
+        double *xs = ws + tid;
+        double *u0 = ws + (int64_t)NSLOT * m + tid;
+        double *u1 = ws + (int64_t)2 * NSLOT * m + tid;
+        double *u2 = ws + (int64_t)3 * NSLOT * m + tid;
+        for (int k0 = tid; k0 < r; k0 += NSLOT) {
+            // this thread handles cluster starts at k0 only
+            if (k0 > 0 && lam[k0 - 1] - lam[k0] <= ortol) continue;
+            int k1 = k0 + 1;
+            while (k1 < r && lam[k1 - 1] - lam[k1] <= ortol) ++k1;
+            double lprev = 0.0;
+            for (int k = k0; k < k1; ++k) {
+                double lk = lam[k];
+                if (k > k0 && lprev - lk < pertol) lk = lprev - pertol;
+                lprev = lk;
+                for (int i = 0; i < m; ++i) xs[i * NSLOT] = hash_unit((uint32_t)i, (uint32_t)k);
+                for (int it = 0; it < 3; ++it) {
+                    tri_solve(d, e, m, lk, tiny, xs, u0, u1, u2, NSLOT);
+                    // re-orthogonalise against earlier members of the cluster
+                    for (int q = k0; q < k; ++q) {
+                        const double *zq = Z + (int64_t)q * m;
+                        double dot = 0.0;
+                        for (int i = 0; i < m; ++i) dot += zq[i] * xs[i * NSLOT];
+                        for (int i = 0; i < m; ++i) xs[i * NSLOT] -= dot * zq[i];
+                    }
+                    double nrm = 0.0, amax = 0.0;
+                    for (int i = 0; i < m; ++i) amax = fmax(amax, fabs(xs[i * NSLOT]));
+                    if (amax == 0.0) amax = 1.0;
+                    for (int i = 0; i < m; ++i) {
+                        const double t = xs[i * NSLOT] / amax;
+                        nrm += t * t;
+                    }
+                    nrm = amax * sqrt(nrm);
+                    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+                    for (int i = 0; i < m; ++i) xs[i * NSLOT] *= inv;
+                }
+                double *zk = Z + (int64_t)k * m;
+                for (int i = 0; i < m; ++i) zk[i] = xs[i * NSLOT];
+            }
+        }
+    }
+    __syncthreads();
+    // back-transformation u = Q z, one wavefront per eigenvector
+    const c128 *A = Gall + (int64_t)ip * m * m;
+    const c128 *tv = tall + (int64_t)ip * m;
+    const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+    c64 *U = Uall + (int64_t)ip * m * m;
+    float *f = fall + (int64_t)ip * m;
+    for (int k = wv; k < r; k += nw) {
+        c128 u[MAXQ];
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int i = lane + 64 * q;
+            u[q] = (64 * q < m && i < m) ? c128{Z[(int64_t)k * m + i], 0.0} : c128{0.0, 0.0};
+        }
+        for (int j = m - 2; j >= 0; --j) {
+            const c128 t = tv[j];
+            if (t.re == 0.0 && t.im == 0.0) continue;
+            const c128 *col = A + (int64_t)m * j;
+            c128 s = {0.0, 0.0};
+            c128 vq[MAXQ];
+#pragma unroll
+            for (int q = 0; q < MAXQ; ++q) {
+                vq[q] = c128{0.0, 0.0};
+                if (64 * q < m) {
+                    const int i = lane + 64 * q;
+                    if (i >= j + 1 && i < m) {
+                        vq[q] = (i == j + 1) ? c128{1.0, 0.0} : col[i];
+                        s = s + cmulc(vq[q], u[q]);
+                    }
+                }
+            }
+            s = t * wave_sum(s);
+#pragma unroll
+            for (int q = 0; q < MAXQ; ++q)
+                if (64 * q < m) u[q] = u[q] - s * vq[q];
+        }
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int i = lane + 64 * q;
+            if (64 * q < m && i < m) U[(int64_t)k * m + i] = c64{(float)u[q].re, (float)u[q].im};
+        }
+        if (lane == 0) {
+            const double sg = sqrt(fmax(lam[k], 0.0));
+            const double fk = sg > bd ? (sg - bd) / sg : 0.0;
+            f[k] = (float)fk;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ apply
+
+// right mode (p >= Nt, m = Nt): rows [i0, i0+RB) of X (p x m):
+//   Y = (X V_r) diag(f) V_r^H
+// left mode (p < Nt, m = p): columns [t0, t0+RB) of X (m x Nt):
+//   Y = V_r diag(f) (V_r^H X)
+__global__ __launch_bounds__(256) void k_apply(c64 *__restrict__ Xall, int64_t p, int Nt, int m,
+                                               int left, int RB, const c64 *__restrict__ Uall,
+                                               const float *__restrict__ fall,
+                                               const int *__restrict__ rall) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int ip = blockIdx.y;
+    const int r = rall[ip];
+    if (r < 0) return;  // beta == 0: identity
+    c64 *X = Xall + (int64_t)ip * p * Nt;
+    const c64 *U = Uall + (int64_t)ip * m * m;
+    const float *f = fall + (int64_t)ip * m;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    c64 *xs = reinterpret_cast<c64 *>(smem_raw);  // RB x m
+    c64 *t1 = xs + (int64_t)RB * m;                // RB x r
+    if (!left) {
+        const int64_t i0 = (int64_t)blockIdx.x * RB;
+        const int nr = (int)std::min<int64_t>(RB, p - i0);
+        if (r == 0) {
+            for (int idx = tid; idx < nr * m; idx += nt) {
+                const int ii = idx % nr, t = idx / nr;
+                X[i0 + ii + p * t] = c64{0.f, 0.f};
+            }
+            return;
+        }
+        for (int idx = tid; idx < nr * m; idx += nt) {
+            const int ii = idx % nr, t = idx / nr;
+            xs[ii * m + t] = X[i0 + ii + p * t];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nr * r; idx += nt) {
+            const int ii = idx % nr, k = idx / nr;
+            const c64 *uk = U + (int64_t)k * m;
+            const c64 *xr = xs + ii * m;
+            float ar = 0.f, ai = 0.f;
+            for (int t = 0; t < m; ++t) {
+                const c64 a = xr[t], b = uk[t];
+                ar += a.re * b.re - a.im * b.im;
+                ai += a.re * b.im + a.im * b.re;
+            }
+            const float fk = f[k];
+            t1[ii * r + k] = c64{ar * fk, ai * fk};
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nr * m; idx += nt) {
+            const int ii = idx % nr, t = idx / nr;
+            float ar = 0.f, ai = 0.f;
+            for (int k = 0; k < r; ++k) {
+                const c64 a = t1[ii * r + k], b = U[(int64_t)k * m + t];
+                // a * conj(b)
+                ar += a.re * b.re + a.im * b.im;
+                ai += a.im * b.re - a.re * b.im;
+            }
+            X[i0 + ii + p * t] = c64{ar, ai};
+        }
+    } else {
+        const int t0 = blockIdx.x * RB;
+        const int nc = std::min(RB, Nt - t0);
+        if (r == 0) {
+            for (int idx = tid; idx < nc * m; idx += nt) {
+                const int i = idx % m, tt = idx / m;
+                X[i + p * (t0 + tt)] = c64{0.f, 0.f};
+            }
+            return;
+        }
+        for (int idx = tid; idx < nc * m; idx += nt) {
+            const int i = idx % m, tt = idx / m;
+            xs[tt * m + i] = X[i + p * (t0 + tt)];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nc * r; idx += nt) {
+            const int tt = idx % nc, k = idx / nc;
+            const c64 *uk = U + (int64_t)k * m;
+            const c64 *xc = xs + tt * m;
+            float ar = 0.f, ai = 0.f;
+            for (int i = 0; i < m; ++i) {
+                const c64 a = uk[i], b = xc[i];
+                // conj(a) * b
+                ar += a.re * b.re + a.im * b.im;
+                ai += a.re * b.im - a.im * b.re;
+            }
+            const float fk = f[k];
+            t1[tt * r + k] = c64{ar * fk, ai * fk};
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nc * m; idx += nt) {
+            const int i = idx % m, tt = idx / m;
+            float ar = 0.f, ai = 0.f;
+            for (int k = 0; k < r; ++k) {
+                const c64 a = U[(int64_t)k * m + i], b = t1[tt * r + k];
+                ar += a.re * b.re - a.im * b.im;
+                ai += a.re * b.im + a.im * b.re;
+            }
+            X[i + p * (t0 + tt)] = c64{ar, ai};
+        }
+    }
+}
+
+// m == 1: each matrix is a vector v (len = p*Nt); SVST = v * max(|v| - beta, 0)/|v|.
+__global__ __launch_bounds__(256) void k_svst_m1(c64 *__restrict__ P, int64_t len, int64_t Np,
+                                                 const double *__restrict__ beta,
+                                                 int beta_per_patch) {
+    const int lane = threadIdx.x & 63;
+    const int64_t ip = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (ip >= Np) return;
+    c64 *v = P + ip * len;
+    double s = 0.0;
+    for (int64_t i = lane; i < len; i += 64) {
+        const c64 a = v[i];
+        s += (double)a.re * a.re + (double)a.im * a.im;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    const double bd = (double)(float)beta[beta_per_patch ? ip : 0];
+    if (bd == 0.0) return;
+    const double sg = sqrt(s);
+    const float fk = sg > bd ? (float)((sg - bd) / sg) : 0.f;
+    for (int64_t i = lane; i < len; i += 64) {
+        const c64 a = v[i];
+        v[i] = c64{a.re * fk, a.im * fk};
+    }
+}
+
+// Sum of sqrt(max(lambda,0)) over all eigenvalues of T (nuclear norm).
+__global__ __launch_bounds__(256) void k_nucnorm(const double *__restrict__ dall,
+                                                 const double *__restrict__ eall, int m,
+                                                 double *__restrict__ out) {
+    __shared__ double red[16];
+    __shared__ double s_gl, s_gu, s_pivmin;
+    const int ip = blockIdx.x;
+    const double *d = dall + (int64_t)ip * m;
+    const double *e = eall + (int64_t)ip * m;
+    if (threadIdx.x == 0) {
+        double emax = 0.0, gl = INFINITY, gu = -INFINITY;
+        for (int i = 0; i < m; ++i) {
+            const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
+            const double er = i + 1 < m ? fabs(e[i]) : 0.0;
+            gl = fmin(gl, d[i] - el - er);
+            gu = fmax(gu, d[i] + el + er);
+            if (i + 1 < m) emax = fmax(emax, e[i] * e[i]);
+        }
+        s_pivmin = DBL_MIN * fmax(1.0, emax);
+        const double pad = 2.0 * DBL_EPSILON * fmax(fabs(gl), fabs(gu)) + s_pivmin;
+        s_gl = gl - pad;
+        s_gu = gu + pad;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        double lo = s_gl, hi = s_gu;
+        for (int it = 0; it < 200; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + s_pivmin) break;
+            if (mid <= lo || mid >= hi) break;
+            if (sturm_count(d, e, m, mid, s_pivmin) > k)
+                hi = mid;
+            else
+                lo = mid;
+        }
+        acc += sqrt(fmax(0.5 * (lo + hi), 0.0));
+    }
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) out[ip] = s;
+}
+
+__global__ void k_m1_nucnorm(const c64 *__restrict__ P, int64_t len, int64_t Np,
+                             double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t ip = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (ip >= Np) return;
+    const c64 *v = P + ip * len;
+    double s = 0.0;
+    for (int64_t i = lane; i < len; i += 64) {
+        const c64 a = v[i];
+        s += (double)a.re * a.re + (double)a.im * a.im;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[ip] = sqrt(s);
+}
+
+// ------------------------------------------------------------------ host driver
+
+struct SvstWork {
+    int64_t m, nkc, chunk;
+    c64 *part;
+    c128 *G;
+    double *d, *e, *lam, *Z, *ws;
+    c128 *tau;
+    c64 *U;
+    float *f;
+    int *r;
+};
+
+static size_t svst_bytes_per_patch(int64_t m, int64_t nkc) {
+    return (size_t)(nkc * m * m * 8 + m * m * 16 + m * 8 * 3 + m * 16 + m * m * 8 +
+                    (int64_t)NSLOT * 4 * m * 8 + m * m * 8 + m * 4 + 16);
+}
+
+static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
+    SvstWork w{};
+    w.m = m;
+    w.nkc = cdiv(Kc, KCH);
+    size_t budget = (size_t)4 << 30;
+    if (const char *s = getenv("MSLR_SVST_WORK_MB")) {
+        long mb = atol(s);
+        if (mb > 0) budget = (size_t)mb << 20;
+    }
+    const size_t per = svst_bytes_per_patch(m, w.nkc);
+    int64_t chunk = (int64_t)(budget / per);
+    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, Np));
+    chunk = std::min<int64_t>(chunk, 65535);
+    w.chunk = chunk;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const int64_t mm = m * m;
+    size_t off = 0;
+    size_t o_part = off; off += al((size_t)chunk * w.nkc * mm * 8);
+    size_t o_G = off; off += al((size_t)chunk * mm * 16);
+    size_t o_d = off; off += al((size_t)chunk * m * 8);
+    size_t o_e = off; off += al((size_t)chunk * m * 8);
+    size_t o_lam = off; off += al((size_t)chunk * m * 8);
+    size_t o_tau = off; off += al((size_t)chunk * m * 16);
+    size_t o_Z = off; off += al((size_t)chunk * mm * 8);
+    size_t o_ws = off; off += al((size_t)chunk * NSLOT * 4 * m * 8);
+    size_t o_U = off; off += al((size_t)chunk * mm * 8);
+    size_t o_f = off; off += al((size_t)chunk * m * 4);
+    size_t o_r = off; off += al((size_t)chunk * 4);
+    char *base = (char *)ctx->scratch_get(SLOT_GRAM, off);
+    w.part = (c64 *)(base + o_part);
+    w.G = (c128 *)(base + o_G);
+    w.d = (double *)(base + o_d);
+    w.e = (double *)(base + o_e);
+    w.lam = (double *)(base + o_lam);
+    w.tau = (c128 *)(base + o_tau);
+    w.Z = (double *)(base + o_Z);
+    w.ws = (double *)(base + o_ws);
+    w.U = (c64 *)(base + o_U);
+    w.f = (float *)(base + o_f);
+    w.r = (int *)(base + o_r);
+    return w;
+}
+
+static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, int64_t p,
+                             int64_t Nt, int64_t nb, int *err_dev) {
+    const int m = (int)w.m;
+    const bool left = p < Nt;
+    const int64_t Kc = left ? Nt : p;
+    const int64_t ks = left ? p : 1;
+    const int64_t cs = left ? 1 : p;
+    const int ntile = (int)cdiv(m, GT);
+    const int npair = ntile * (ntile + 1) / 2;
+    dim3 gg((unsigned)npair, (unsigned)w.nkc, (unsigned)nb);
+    k_gram_tiles<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, ntile,
+                                              w.part);
+    HIP_LAUNCH_CHECK();
+    const int64_t ne = (int64_t)m * m * nb;
+    const unsigned gr = (unsigned)std::min<int64_t>(cdiv(ne, 256), 1 << 20);
+    k_gram_reduce<<<gr, 256, 0, ctx->stream>>>(w.part, m, (int)w.nkc, (int)nb, left ? 1 : 0, w.G,
+                                               err_dev);
+    HIP_LAUNCH_CHECK();
+    k_tridiag<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.G, m, w.d, w.e, w.tau);
+    HIP_LAUNCH_CHECK();
+}
+
+void svst_batched(mslr_ctx_s *ctx, c64 *P, int64_t p, int64_t Nt, int64_t Np,
+                  const double *beta_dev, int per_patch, int *err_dev) {
+    if (Np == 0 || p == 0 || Nt == 0) return;
+    const int64_t m = std::min(p, Nt);
+    MSLR_REQUIRE(m <= MAXM, MSLR_ERR_UNSUPPORTED,
+                 "min(patch voxels, Nt) must be <= " + std::to_string(MAXM));
+    const int64_t len = p * Nt;
+    if (m == 1) {
+        const unsigned nb = (unsigned)cdiv(Np, 4);
+        k_svst_m1<<<nb, 256, 0, ctx->stream>>>(P, len, Np, beta_dev, per_patch);
+        HIP_LAUNCH_CHECK();
+        return;
+    }
+    static bool attr = false;
+    if (!attr) {
+        HIP_CHECK(hipFuncSetAttribute((const void *)k_apply,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    const bool left = p < Nt;
+    const SvstWork w = svst_work(ctx, m, left ? Nt : p, Np);
+    int RB = (int)std::max<int64_t>(1, std::min<int64_t>(16, 8192 / m));
+    for (int64_t ip0 = 0; ip0 < Np; ip0 += w.chunk) {
+        const int64_t nb = std::min<int64_t>(w.chunk, Np - ip0);
+        c64 *Pc = P + ip0 * len;
+        gram_and_tridiag(ctx, w, Pc, p, Nt, nb, err_dev);
+        k_eigvec<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.G, w.d, w.e, w.tau, (int)m,
+                                                       beta_dev + (per_patch ? ip0 : 0), per_patch,
+                                                       w.lam, w.Z, w.ws, w.U, w.f, w.r);
+        HIP_LAUNCH_CHECK();
+        const int64_t nblk = left ? cdiv(Nt, RB) : cdiv(p, RB);
+        const size_t lds = (size_t)2 * RB * m * sizeof(c64);
+        for (int64_t b0 = 0; b0 < nb; b0 += 65535) {
+            const int64_t nbb = std::min<int64_t>(65535, nb - b0);
+            dim3 ga((unsigned)nblk, (unsigned)nbb);
+            k_apply<<<ga, 256, lds, ctx->stream>>>(Pc + b0 * len, p, (int)Nt, (int)m, left ? 1 : 0,
+                                                  RB, w.U + b0 * m * m, w.f + b0 * m, w.r + b0);
+            HIP_LAUNCH_CHECK();
+        }
+    }
+}
+
+void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,
+                     double *out_dev) {
+    if (Np == 0) {
+        HIP_CHECK(hipMemsetAsync(out_dev, 0, sizeof(double), ctx->stream));
+        return;
+    }
+    const int64_t m = std::min(p, Nt);
+    MSLR_REQUIRE(m <= MAXM, MSLR_ERR_UNSUPPORTED, "min(patch voxels, Nt) too large");
+    double *per = (double *)ctx->scratch_get(SLOT_EIG, (size_t)(Np + 1) * sizeof(double));
+    if (m == 1) {
+        k_m1_nucnorm<<<(unsigned)cdiv(Np, 4), 256, 0, ctx->stream>>>(P, p * Nt, Np, per);
+        HIP_LAUNCH_CHECK();
+    } else {
+        const bool left = p < Nt;
+        const SvstWork w = svst_work(ctx, m, left ? Nt : p, Np);
+        for (int64_t ip0 = 0; ip0 < Np; ip0 += w.chunk) {
+            const int64_t nb = std::min<int64_t>(w.chunk, Np - ip0);
+            gram_and_tridiag(ctx, w, P + ip0 * p * Nt, p, Nt, nb, nullptr);
+            k_nucnorm<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.d, w.e, (int)m, per + ip0);
+            HIP_LAUNCH_CHECK();
+        }
+    }
+    sum_partials(ctx, per, Np, out_dev);
+}
+
+}  // namespace mslr
+
+using namespace mslr;
+
+extern "C" {
+
+mslr_status_t mslr_svst_batched(mslr_ctx ctx, void *P_dev, int64_t p, int64_t Nt, int64_t Np,
+                                const double *beta_host, int per_patch) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(p >= 0 && Nt >= 0 && Np >= 0, MSLR_ERR_ARG, "negative size");
+        MSLR_REQUIRE(beta_host != nullptr, MSLR_ERR_ARG, "beta is null");
+        const int64_t nbeta = per_patch ? Np : 1;
+        for (int64_t i = 0; i < nbeta; ++i)
+            MSLR_REQUIRE(beta_host[i] >= 0.0 && std::isfinite(beta_host[i]), MSLR_ERR_ARG,
+                         "beta must be finite and >= 0");
+        if (Np == 0 || p == 0 || Nt == 0) return;
+        double *bd = (double *)c->scratch_get(SLOT_APPLY, (size_t)(nbeta + 1) * sizeof(double) + 64);
+        int *err = (int *)(bd + nbeta + 1);
+        HIP_CHECK(hipMemcpyAsync(bd, beta_host, (size_t)nbeta * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+        svst_batched(c, (c64 *)P_dev, p, Nt, Np, bd, per_patch, err);
+        int herr = 0;
+        HIP_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        MSLR_REQUIRE(herr == 0, MSLR_ERR_NUMERIC, "non-finite values in a patch matrix");
+    });
+}
+
+mslr_status_t mslr_patch_nucnorm(mslr_ctx ctx, const void *P_dev, int64_t p, int64_t Nt,
+                                 int64_t Np, double *out) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(out != nullptr, MSLR_ERR_ARG, "out is null");
+        MSLR_REQUIRE(p >= 1 && Nt >= 1 && Np >= 0, MSLR_ERR_ARG, "bad sizes");
+        double *od = (double *)c->scratch_get(SLOT_APPLY, 64);
+        nucnorm_batched(c, (const c64 *)P_dev, p, Nt, Np, od);
+        HIP_CHECK(hipMemcpyAsync(out, od, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,462 @@
+"""NumPy restatement of the reference CPU path (test oracle; see oracle/__init__.py).
+
+Arrays use the reference's Julia shapes and column-major semantics: a 4D image
+is (Nx, Ny, Nz, Nt); linear indices are Fortran-order (x fastest).
+
+Precision modes of the POGM driver (`run_recon_arrays(..., mode=)`):
+  "ref" : reference-faithful -- FP32 patches and SVD, POGM state promoted to
+          complex128 by Julia's Float64 alpha (scripts/reconstruct.jl:159,
+          src/mirt_mod.jl:112,131), i.e. the reference CPU path.
+  "f32" : everything complex64 (the arithmetic model of the GPU drop-in).
+  "f64" : everything complex128 (the FP64 oracle).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.fft as sfft
+
+AX3 = (0, 1, 2)
+EPS32 = float(np.finfo(np.float32).eps)
+EPS64 = float(np.finfo(np.float64).eps)
+
+PINS = {
+    "Asense_forward": "closed form vs numpy DFT (1 coil, S=1, full mask); brute-force DFT on tiny sizes",
+    "Asense_adjoint": "dot test <Ax,y> = <x,A'y>; A'A = I for full mask + unit-RSS smaps",
+    "img2patches/patches2img": "brute-force index loops; round trip; coverage-count closed form",
+    "SVST": "closed forms: rank-1, 1xNt group shrink, orthogonal columns, beta=0, beta>=s1, zero",
+    "patch_nucnorm": "sum of singular values vs closed forms",
+    "pogm_mod": "scalar-recurrence golden table; toy quadratic with identity prox vs hand trace",
+    "poweriter_mod": "diagonal operator with known sigma1; full-mask SENSE -> 1",
+    "lambdas": "closed-form values for the configs",
+}
+
+
+# --------------------------------------------------------------------------- FFT helpers
+
+def _workers():
+    import os
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cfft3(x, sign=-1):
+    """Unnormalised 3D DFT over axes 0..2 with exponent sign `sign`."""
+    if sign < 0:
+        return sfft.fftn(x, axes=AX3, workers=_workers())
+    return sfft.ifftn(x, axes=AX3, norm="forward", workers=_workers())  # unnormalised inverse
+
+
+# --------------------------------------------------------------------------- SENSE
+# src/sense_gpu.jl:56-108 (the GPU file's convention, which it states matches MIRT.Asense)
+
+def sample_index(samp):
+    """0-based linear indices of sampled voxels in column-major order (src/sense_gpu.jl:68)."""
+    return np.flatnonzero(np.asarray(samp).ravel(order="F"))
+
+
+def Asense_forward(x, samp, smaps, fft_forward=True, unitary=True):
+    """y (K, Nc) = scale * fftshift(fft(ifftshift(smaps .* x)))[samp, :]  (src/sense_gpu.jl:71-81)."""
+    N = smaps.shape[:3]
+    Nc = smaps.shape[3]
+    rdt = np.float64 if x.dtype == np.complex128 or smaps.dtype == np.complex128 else np.float32
+    scale = rdt(1.0 / math.sqrt(np.prod(N))) if unitary else rdt(1.0)
+    xc = smaps * x[..., None]
+    sh = sfft.ifftshift(xc, axes=AX3)
+    k = cfft3(sh, -1 if fft_forward else +1)
+    if not fft_forward:
+        k = k / np.prod(N)  # src/sense_gpu.jl:77 uses the normalised ifft
+    kc = scale * sfft.fftshift(k, axes=AX3)
+    idx = sample_index(samp)
+    return kc.reshape(-1, Nc, order="F")[idx, :]
+
+
+def Asense_adjoint(y, samp, smaps, fft_forward=True, unitary=True, exact=True):
+    """x (N) = sum_c conj(S_c) .* scale * ifftshift(IDFT(fftshift(zero-fill(y)))).
+
+    exact=True uses the unnormalised inverse DFT, which makes this the conjugate
+    transpose of Asense_forward.  exact=False reproduces src/sense_gpu.jl:93 as
+    written (normalised ifft, i.e. 1/prod(N) times the adjoint).
+    """
+    N = smaps.shape[:3]
+    Nc = smaps.shape[3]
+    V = int(np.prod(N))
+    dt = np.result_type(y.dtype, smaps.dtype)
+    rdt = np.float64 if dt == np.complex128 else np.float32
+    scale = rdt(1.0 / math.sqrt(V)) if unitary else rdt(1.0)
+    full = np.zeros((V, Nc), dtype=dt)
+    full[sample_index(samp), :] = y
+    full = full.reshape(*N, Nc, order="F")
+    if exact:
+        # (fftshift o F o ifftshift)^H = fftshift o F^H o ifftshift; for even sizes
+        # this equals the reference's ifftshift o F^H o fftshift.
+        xc = cfft3(sfft.ifftshift(full, axes=AX3), +1 if fft_forward else -1)
+        if not fft_forward:
+            xc = xc / V
+        xc = scale * sfft.fftshift(xc, axes=AX3)
+    else:
+        # src/sense_gpu.jl:92-96 as written
+        sh = sfft.fftshift(full, axes=AX3)
+        xc = cfft3(sh, +1) / V if fft_forward else cfft3(sh, -1)
+        xc = scale * sfft.ifftshift(xc, axes=AX3)
+    return np.sum(np.conj(smaps) * xc, axis=3)
+
+
+def A_forward(x4, mask4, smaps, **kw):
+    """block_diag over frames (scripts/reconstruct.jl:142): (Nx,Ny,Nz,Nt) -> (K,Nc,Nt)."""
+    Nt = x4.shape[3]
+    return np.stack([Asense_forward(x4[..., t], mask4[..., t], smaps, **kw) for t in range(Nt)], axis=2)
+
+
+def A_adjoint(y3, mask4, smaps, **kw):
+    Nt = y3.shape[2]
+    return np.stack([Asense_adjoint(y3[:, :, t], mask4[..., t], smaps, **kw) for t in range(Nt)], axis=3)
+
+
+# --------------------------------------------------------------------------- patches
+# src/recon.jl:53-126
+
+def patch_geometry(N, patch_size, stride_size):
+    ps = [min(int(patch_size[a]), int(N[a])) for a in range(3)]
+    nsteps = [-(-(int(N[a]) - ps[a]) // int(stride_size[a])) for a in range(3)]  # cld
+    Np = (nsteps[0] + 1) * (nsteps[1] + 1) * (nsteps[2] + 1)
+    return ps, nsteps, Np
+
+
+def _patch_starts(N, ps, ss, nsteps):
+    """0-based starts in patch order (x fastest): min(i*s, N - ps)  (src/recon.jl:69-72)."""
+    for iz in range(nsteps[2] + 1):
+        for iy in range(nsteps[1] + 1):
+            for ix in range(nsteps[0] + 1):
+                yield (min(ix * ss[0], N[0] - ps[0]), min(iy * ss[1], N[1] - ps[1]),
+                       min(iz * ss[2], N[2] - ps[2]))
+
+
+def img2patches(img, patch_size, stride_size, dtype=np.complex64):
+    """P (p, Nt, Np); src/recon.jl:53-78 (always ComplexF32 there: dtype=complex64)."""
+    N = img.shape[:3]
+    Nt = img.shape[3]
+    ps, nsteps, Np = patch_geometry(N, patch_size, stride_size)
+    p = ps[0] * ps[1] * ps[2]
+    P = np.zeros((p, Nt, Np), dtype=dtype)
+    for ip, (sx, sy, sz) in enumerate(_patch_starts(N, ps, stride_size, nsteps)):
+        patch = img[sx:sx + ps[0], sy:sy + ps[1], sz:sz + ps[2], :]
+        P[:, :, ip] = patch.reshape(p, Nt, order="F")
+    return P
+
+
+def patches2img(P, patch_size, stride_size, og_size, dtype=np.complex64):
+    """Sum of patches in patch order / max(count, 1)  (src/recon.jl:96-126)."""
+    Nt = P.shape[1]
+    N = tuple(int(n) for n in og_size)
+    ps, nsteps, Np = patch_geometry(N, patch_size, stride_size)
+    rdt = np.float64 if np.dtype(dtype) == np.complex128 else np.float32
+    img = np.zeros((*N, Nt), dtype=dtype)
+    cnt = np.zeros(N, dtype=rdt)
+    for ip, (sx, sy, sz) in enumerate(_patch_starts(N, ps, stride_size, nsteps)):
+        patch = P[:, :, ip].reshape(ps[0], ps[1], ps[2], Nt, order="F")
+        img[sx:sx + ps[0], sy:sy + ps[1], sz:sz + ps[2], :] += patch
+        cnt[sx:sx + ps[0], sy:sy + ps[1], sz:sz + ps[2]] += rdt(1)
+    cnt = np.maximum(cnt, rdt(1))
+    # Julia's ComplexF32 / Float32 divides the parts separately: Complex(re/x, im/x)
+    # (NumPy's complex / real division rounds differently).
+    img.real /= cnt[..., None]
+    img.imag /= cnt[..., None]
+    return img
+
+
+def coverage_count(N, patch_size, stride_size):
+    ps, nsteps, _ = patch_geometry(N, patch_size, stride_size)
+    cnt = np.zeros(tuple(N), dtype=np.int64)
+    for sx, sy, sz in _patch_starts(N, ps, stride_size, nsteps):
+        cnt[sx:sx + ps[0], sy:sy + ps[1], sz:sz + ps[2]] += 1
+    return cnt
+
+
+# --------------------------------------------------------------------------- SVST
+# src/recon.jl:183-269
+
+def SVST(X, beta):
+    """U * Diagonal(max(S - beta, 0)) * Vt with the zero shortcut (src/recon.jl:183-205)."""
+    if np.linalg.norm(X) == 0:
+        return np.zeros_like(X)
+    U, S, Vh = np.linalg.svd(X, full_matrices=False)  # LAPACK gesdd, as Julia's svd
+    bT = S.dtype.type(beta)
+    s = np.maximum(S - bT, S.dtype.type(0))
+    keep = s > 0
+    if not keep.any():
+        return np.zeros((U.shape[0], Vh.shape[1]), dtype=X.dtype)
+    return ((U[:, keep] * s[keep]) @ Vh[keep, :]).astype(X.dtype, copy=False)
+
+
+def svst_batched(P, beta):
+    out = np.empty_like(P)
+    b = np.broadcast_to(np.asarray(beta, dtype=np.float64), (P.shape[2],))
+    for i in range(P.shape[2]):
+        out[:, :, i] = SVST(np.array(P[:, :, i]), b[i])
+    return out
+
+
+def patchSVST(img, beta, patch_size, stride_size, dtype=np.complex64):
+    """src/recon.jl:251-269 (scalar or per-patch beta)."""
+    P = img2patches(img, patch_size, stride_size, dtype=dtype)
+    P = svst_batched(P, beta)
+    return patches2img(P, patch_size, stride_size, img.shape[:3], dtype=dtype)
+
+
+def patch_nucnorm(P, lambdas=None):
+    """src/recon.jl:138-166."""
+    Np = P.shape[2]
+    costs = np.zeros(Np)
+    for i in range(Np):
+        s = np.linalg.svd(P[:, :, i], compute_uv=False)
+        if lambdas is None:
+            costs[i] = float(np.sum(s, dtype=np.float64))
+        else:
+            costs[i] = lambdas[i] * float(np.sum(s, dtype=np.float64)) if s[0] > 0 else 0.0
+    return float(np.sum(costs)) if lambdas is None else float(np.sum(costs)) / Np
+
+
+# --------------------------------------------------------------------------- POGM
+# src/mirt_mod.jl:26-29, 74-188
+
+def _gr_restart(Fgrad, ynew_yold, restart_cutoff):
+    """sum(Float64, real(-Fgrad .* ynew_yold)) <= cutoff*norm(Fgrad)*norm(ynew_yold)  (no conjugate)."""
+    lhs = float(np.sum(np.real(-Fgrad * ynew_yold), dtype=np.float64))
+    return lhs <= restart_cutoff * float(np.linalg.norm(Fgrad)) * float(np.linalg.norm(ynew_yold))
+
+
+def pogm_mod(x0, Fcost, f_grad, f_L, *, f_mu=0.0, mom="pogm", restart="gr", restart_cutoff=0.0,
+             bsig=1, niter=10, g_prox=lambda z, c: z, fun=lambda it, xk, yk, r: None,
+             scalar=np.float64):
+    """Statement-by-statement restatement of pogm_mod (src/mirt_mod.jl:74-188).
+
+    `scalar` is the type of the step scalars: np.float64 reproduces Julia's
+    promotion of ComplexF32 arrays to ComplexF64 (Float64 alpha); np.float32
+    keeps complex64 state.  g_prox may mutate its argument (the reference's
+    closure does), which makes xnew and znew the same array as in the reference.
+    """
+    if mom not in ("pgm", "fpgm", "pogm"):
+        raise ValueError(f"mom={mom}")
+    if restart not in ("none", "gr", "fr"):
+        raise ValueError(f"restart={restart}")
+    if not f_L >= 0:
+        raise ValueError("f_L")
+    if not f_mu >= 0:
+        raise ValueError("f_mu")
+    if not bsig >= 0:
+        raise ValueError("bsig")
+    if not abs(restart_cutoff) < 1:
+        raise ValueError("restart_cutoff")
+    S = scalar
+    L = f_L
+    mu = f_mu
+    q = mu / L
+    told = 1.0
+    sig = 1.0
+    zetaold = 1.0
+    xold = x0
+    yold = x0
+    uold = x0
+    zold = x0
+    Fcostold = Fcost(x0)
+    Fgradold = np.zeros(x0.shape)
+    out = [None] * (niter + 1)
+    out[0] = fun(0, x0, x0, False)
+    xnew = x0
+    ynew = x0
+    tnew = told
+    for it in range(1, niter + 1):
+        alpha = 2 / (L + mu) if (mom == "pgm" and mu != 0) else 1 / L
+        a = S(alpha)
+        fgrad = f_grad(xold)
+        is_restart = False
+        if mom in ("pgm", "fpgm"):
+            ynew = g_prox(xold - a * fgrad, alpha)
+            Fgrad = -S(1 / alpha) * (ynew - xold)
+            Fcostnew = Fcost(ynew)
+            if restart != "none":
+                if (restart == "fr" and Fcostnew > Fcostold) or \
+                   (restart == "gr" and _gr_restart(Fgrad, ynew - yold, restart_cutoff)):
+                    told = 1.0
+                    is_restart = True
+                Fcostold = Fcostnew
+        else:
+            unew = xold - a * fgrad
+        if mom == "fpgm" and mu != 0:
+            beta = (1 - math.sqrt(q)) / (1 + math.sqrt(q))
+        elif mom == "pogm" and mu != 0:
+            beta = (2 + q - math.sqrt(q ** 2 + 8 * q)) ** 2 / 4 / (1 - q)
+        elif mom != "pgm":
+            tnew = 0.5 * (1 + math.sqrt(1 + 8 * told ** 2)) if (mom == "pogm" and it == niter) \
+                else 0.5 * (1 + math.sqrt(1 + 4 * told ** 2))
+            beta = (told - 1) / tnew
+        if mom == "pgm":
+            xnew = ynew
+        elif mom == "fpgm":
+            xnew = ynew + S(beta) * (ynew - yold)
+        else:
+            gamma = (2 + q - math.sqrt(q ** 2 + 8 * q)) / 2 if mu != 0 else sig * told / tnew
+            znew = (unew + S(beta) * (unew - uold) + S(gamma) * (unew - xold)
+                    - S(beta * alpha / zetaold) * (xold - zold))
+            zetanew = alpha * (1 + beta + gamma)
+            xnew = g_prox(znew, zetanew)
+            Fgrad = fgrad - S(1 / zetanew) * (xnew - znew)
+            ynew = xold - a * Fgrad
+            Fcostnew = Fcost(xnew)
+            if restart != "none":
+                if (restart == "fr" and Fcostnew > Fcostold) or \
+                   (restart == "gr" and _gr_restart(Fgrad, ynew - yold, restart_cutoff)):
+                    tnew = 1.0
+                    sig = 1.0
+                    is_restart = True
+                elif float(np.sum(np.real(Fgrad * Fgradold), dtype=np.float64)) < 0:
+                    sig = bsig * sig
+                Fcostold = Fcostnew
+                Fgradold = Fgrad
+            uold = unew
+            zold = znew
+            zetaold = zetanew
+        out[it] = fun(it, xnew, ynew, is_restart)
+        xold = xnew
+        yold = ynew
+        if mom != "pgm" and mu == 0:
+            told = tnew
+    return (xnew if mom == "pogm" else ynew), out
+
+
+def pogm_scalars(niter, restarts=()):
+    """(t_k, beta_k, gamma_k, zeta/alpha) of the aliased POGM with the given restart iterations
+    (src/mirt_mod.jl:140-156,166-168,184), for golden tables."""
+    told, sig = 1.0, 1.0
+    rows = []
+    rs = set(restarts)
+    for it in range(1, niter + 1):
+        tnew = 0.5 * (1 + math.sqrt(1 + 8 * told ** 2)) if it == niter else 0.5 * (1 + math.sqrt(1 + 4 * told ** 2))
+        beta = (told - 1) / tnew
+        gamma = sig * told / tnew
+        rows.append((tnew, beta, gamma, 1 + beta + gamma))
+        if it in rs:
+            tnew, sig = 1.0, 1.0
+        told = tnew
+    return rows
+
+
+def poweriter_mod(Afwd, Aadj, n, *, niter=200, tol=1e-6, x0=None, dtype=np.complex128):
+    """src/mirt_mod.jl:215-238 with A given as forward/adjoint callables."""
+    x = np.ones(n, dtype=dtype) if x0 is None else np.array(x0, dtype=dtype)
+    ratio_old = np.inf
+    it_done = niter
+    for it in range(1, niter + 1):
+        Ax = Afwd(x)
+        ratio = np.linalg.norm(Ax) / np.linalg.norm(x)
+        if abs(ratio - ratio_old) / ratio < tol:
+            it_done = it
+            break
+        ratio_old = ratio
+        x = Aadj(Ax)
+        x = x / np.linalg.norm(x)
+    return x, float(np.linalg.norm(Afwd(x)) / np.linalg.norm(x)), it_done
+
+
+# --------------------------------------------------------------------------- driver
+# scripts/reconstruct.jl:86-226 (without file I/O)
+
+def lambdas_ong_lustig(N, Nt, patch_sizes):
+    """scripts/reconstruct.jl:163-169."""
+    Nvox = int(np.prod(N))
+    return [math.sqrt(int(np.prod(ps))) + math.sqrt(Nt) + math.sqrt(math.log(Nvox * Nt / max(int(np.prod(ps)), Nt)))
+            for ps in patch_sizes]
+
+
+def normalise_smaps(smaps_raw):
+    """scripts/reconstruct.jl:88-89."""
+    s = np.asarray(smaps_raw).astype(np.complex64)
+    rss = np.sqrt(np.sum(np.abs(s) ** 2, axis=3, keepdims=True))
+    return (s / (rss + np.float32(EPS32))).astype(np.complex64)
+
+
+def sense_comb(ksp, smaps):
+    """src/recon.jl:312-323: fftshift(ifft(ifftshift(ksp), 1:3)) combined with conj(smaps)/(RSS^2+eps)."""
+    img_mc = sfft.fftshift(sfft.ifftn(sfft.ifftshift(ksp, axes=AX3), axes=AX3, workers=_workers()), axes=AX3)
+    denom = np.sum(np.abs(smaps) ** 2, axis=3).astype(np.float64) + EPS64
+    Nt = ksp.shape[4]
+    img = np.zeros((*ksp.shape[:3], Nt), dtype=ksp.dtype)
+    for t in range(Nt):
+        num = np.sum(np.conj(smaps) * img_mc[..., t], axis=3)
+        img[..., t] = num / denom
+    return img
+
+
+def quantile_linear(a, q):
+    """Statistics.quantile (type 7) == numpy 'linear'."""
+    return float(np.quantile(np.asarray(a, dtype=np.float64), q, method="linear"))
+
+
+def prepare(ksp0, smaps_raw, N, Nvc, Nt):
+    """Setup of scripts/reconstruct.jl:86-118,145-147: returns smaps, normalised ksp (K,Nvc,Nt), mask, R, sf."""
+    smaps = normalise_smaps(smaps_raw)
+    ksp0 = np.asarray(ksp0).astype(np.complex64)
+    assert ksp0.shape == (*N, Nvc, Nt), f"Unexpected k-space shape: {ksp0.shape}"
+    img0 = sense_comb(ksp0, smaps)
+    sf = quantile_linear(np.abs(img0).ravel(order="F"), 0.99)
+    ksp0 = (ksp0 / np.float32(max(sf, EPS32))).astype(np.complex64)
+    omega = ksp0[:, :, :, 0, :] != 0
+    R = float(np.prod(N)) / float(np.sum(omega[..., 0]))
+    for ic in range(1, Nvc):
+        assert np.array_equal(omega, ksp0[:, :, :, ic, :] != 0), f"Coil {ic + 1} has a different sampling pattern"
+    for it in range(1, Nt):
+        assert np.sum(omega[..., it]) == np.sum(omega[..., it - 1]), f"Frame {it + 1} has a different sample count"
+    flat = ksp0.reshape(-1, Nvc, Nt, order="F")
+    ksp = np.stack([flat[sample_index(omega[..., t]), :, t] for t in range(Nt)], axis=2)
+    return smaps, ksp, omega, R, sf
+
+
+def run_recon_arrays(ksp, omega, smaps, *, N, Nt, PATCH_SIZES, STRIDES, NITERS, sigma1A=1.0,
+                     mode="ref", exact_adjoint=True, fun=None):
+    """POGM part of run_recon (scripts/reconstruct.jl:159-226) on prepared inputs.
+
+    Returns (X (Nx,Ny,Nz,Nt,S), restarts list, extras dict).
+    """
+    S = len(PATCH_SIZES)
+    L = S * sigma1A ** 2
+    lams = lambdas_ong_lustig(N, Nt, PATCH_SIZES)
+    cdt = np.complex128 if mode == "f64" else np.complex64
+    pdt = np.complex128 if mode == "f64" else np.complex64
+    scalar = np.float32 if mode == "f32" else np.float64
+    sm = smaps.astype(cdt)
+    y = ksp.astype(cdt)
+
+    def Af(x4):
+        return A_forward(x4.astype(cdt, copy=False), omega, sm)
+
+    def Aa(y3):
+        return A_adjoint(y3, omega, sm, exact=exact_adjoint)
+
+    def image_sum(X):
+        s = X[..., 0].copy()
+        for k in range(1, S):
+            s = s + X[..., k]
+        return s
+
+    def dc_cost_grad(X):
+        g = Aa(Af(image_sum(X)) - y)
+        g = g.astype(X.dtype, copy=False) if mode != "ref" else g
+        return np.repeat(g[..., None], S, axis=4)
+
+    def g_prox(X, c):
+        for k in range(S):
+            X[..., k] = patchSVST(X[..., k], c * lams[k], PATCH_SIZES[k], STRIDES[k], dtype=pdt)
+        return X
+
+    X0 = np.zeros((*N, Nt, S), dtype=cdt)
+    X0[..., 0] = Aa(y)
+    X, out = pogm_mod(X0, lambda X: 0.0, dc_cost_grad, L, mom="pogm", niter=NITERS, g_prox=g_prox,
+                      fun=fun or (lambda it, xk, yk, r: r), scalar=scalar)
+    restarts = [bool(o) for o in out[1:]] if fun is None else out
+    return X, restarts, {"L": L, "lambdas": lams}
+
+
+def nrmse(x, ref):
+    ref = np.asarray(ref)
+    return float(np.linalg.norm(np.asarray(x) - ref) / np.linalg.norm(ref))

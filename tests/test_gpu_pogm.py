@@ -1,0 +1,83 @@
+"""The whole device POGM loop vs the oracle's run_recon restatement."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_err
+from problems import prepared
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_run(name, niter, form="plain"):
+    from mslr_hip.mirt_mod import pogm_mslr
+    from mslr_hip.sense import SenseOp
+    cfg, smaps, ksp, omega, sf = prepared(name)
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    S = len(cfg["patch_sizes"])
+    A = SenseOp(omega, smaps)
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    X, restarts, st = pogm_mslr(A, ksp, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"],
+                                lambdas=lams, L=S * 1.0, niter=niter, form=form)
+    return X.to_host(), restarts, st
+
+
+@pytest.mark.parametrize("name,niter", [("small", 6), ("c0", 4)])
+def test_pogm_matches_oracle(name, niter):
+    cfg, smaps, ksp, omega, sf = prepared(name)
+    X, restarts, _ = _gpu_run(name, niter)
+    Xo, ro, _ = O.run_recon_arrays(ksp, omega, smaps, N=tuple(cfg["N"]), Nt=cfg["Nt"],
+                                   PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"],
+                                   NITERS=niter, sigma1A=1.0, mode="f64")
+    assert restarts == ro
+    assert rel_err(X, Xo) < 1e-4
+    # final image (scale sum)
+    assert rel_err(X.sum(axis=4), Xo.sum(axis=4)) < 1e-4
+
+
+def test_pogm_reference_faithful_mode():
+    """Against the reference's own mixed precision (FP32 SVD, FP64 state)."""
+    name, niter = "small", 5
+    cfg, smaps, ksp, omega, sf = prepared(name)
+    X, restarts, _ = _gpu_run(name, niter)
+    Xo, ro, _ = O.run_recon_arrays(ksp, omega, smaps, N=tuple(cfg["N"]), Nt=cfg["Nt"],
+                                   PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"],
+                                   NITERS=niter, sigma1A=1.0, mode="ref")
+    assert restarts == ro
+    assert rel_err(X, Xo) < 1e-4
+
+
+def test_graph_form_matches_plain():
+    X1, r1, _ = _gpu_run("small", 6, form="plain")
+    X2, r2, _ = _gpu_run("small", 6, form="batched")
+    assert r1 == r2
+    assert rel_err(X2, X1) < 1e-6
+
+
+def test_thresholds_are_zeta_lambda():
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.sense import SenseOp
+    cfg, smaps, ksp, omega, sf = prepared("small")
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    A = SenseOp(omega, smaps)
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=2.0, niter=3)
+    st.init(ksp)
+    st.run(1)
+    rows = O.pogm_scalars(3)
+    zeta = 0.5 * rows[0][3]
+    np.testing.assert_allclose(st.thresholds(), [zeta * l for l in lams], rtol=1e-15)
+
+
+def test_invalid_options():
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.sense import SenseOp
+    cfg, smaps, ksp, omega, sf = prepared("small")
+    A = SenseOp(omega, smaps)
+    kw = dict(patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=[1.0, 1.0], niter=2)
+    with pytest.raises(ValueError):
+        POGM(A, L=-1.0, **kw)
+    with pytest.raises(ValueError):
+        POGM(A, L=1.0, mom="foo", **kw)
+    with pytest.raises(ValueError):
+        POGM(A, L=1.0, restart_cutoff=1.5, **kw)

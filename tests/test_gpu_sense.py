@@ -1,0 +1,158 @@
+"""SENSE A / A' on the GPU vs the oracle (src/sense_gpu.jl:56-108 convention)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import crandn, rel_err
+
+pytestmark = pytest.mark.gpu
+
+# (N, Nc, Nt) cases: config-like, odd and mixed-radix sizes
+CASES = [
+    ((12, 12, 10), 3, 4),
+    ((32, 32, 24), 8, 3),
+    ((9, 15, 5), 2, 2),     # odd sizes: complex pre/post phases
+    ((30, 20, 6), 4, 2),    # radix 2,3,5 mixtures
+    ((90, 90, 60), 2, 1),   # production plane shape
+]
+
+
+def _problem(rng, N, Nc, Nt, frac=0.4):
+    smaps = crandn(rng, (*N, Nc))
+    smaps /= np.sqrt(np.sum(np.abs(smaps) ** 2, axis=3, keepdims=True))
+    smaps = np.asfortranarray(smaps.astype(np.complex64))
+    # same count per frame: random permutation of a fixed number of voxels
+    V = int(np.prod(N))
+    K = max(1, int(frac * V))
+    mask = np.zeros((V, Nt), dtype=bool)
+    for t in range(Nt):
+        mask[rng.permutation(V)[:K], t] = True
+    mask = mask.reshape(*N, Nt, order="F")
+    return smaps, mask
+
+
+@pytest.mark.parametrize("N,Nc,Nt", CASES)
+def test_forward_adjoint_vs_oracle(ctx, rng, N, Nc, Nt):
+    from mslr_hip.sense import SenseOp
+    smaps, mask = _problem(rng, N, Nc, Nt)
+    A = SenseOp(mask, smaps, ctx=ctx)
+    x = crandn(rng, (*N, Nt))
+    y = A.forward(x).to_host()
+    yo = O.A_forward(x.astype(np.complex128), mask, smaps.astype(np.complex128))
+    assert y.shape == yo.shape
+    assert rel_err(y, yo) < 2e-6
+    r = crandn(rng, y.shape)
+    xa = A.adjoint(r).to_host()
+    xo = O.A_adjoint(r.astype(np.complex128), mask, smaps.astype(np.complex128))
+    assert rel_err(xa, xo) < 2e-6
+
+
+@pytest.mark.parametrize("N,Nc,Nt", CASES[:4])
+def test_f64_paths(ctx, rng, N, Nc, Nt):
+    from mslr_hip.sense import SenseOp
+    smaps, mask = _problem(rng, N, Nc, Nt)
+    A = SenseOp(mask, smaps, ctx=ctx)
+    x = crandn(rng, (*N, Nt), np.complex128)
+    y = A.forward(x, f64=True).to_host()
+    yo = O.A_forward(x, mask, smaps.astype(np.complex128))
+    assert rel_err(y, yo) < 1e-12
+    xa = A.adjoint(y, f64=True).to_host()
+    xo = O.A_adjoint(yo, mask, smaps.astype(np.complex128))
+    assert rel_err(xa, xo) < 1e-12
+
+
+def test_dot_test_exact_adjoint(ctx, rng):
+    from mslr_hip.sense import SenseOp
+    N, Nc, Nt = (16, 12, 10), 4, 3
+    smaps, mask = _problem(rng, N, Nc, Nt)
+    A = SenseOp(mask, smaps, ctx=ctx)
+    x = crandn(rng, (*N, Nt), np.complex128)
+    y = crandn(rng, (A.K, Nc, Nt), np.complex128)
+    lhs = np.vdot(A.forward(x, f64=True).to_host(), y)
+    rhs = np.vdot(x, A.adjoint(y, f64=True).to_host())
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+def test_full_mask_is_isometry(ctx, rng):
+    """Full sampling + unit-RSS smaps: A'A = I."""
+    from mslr_hip.sense import SenseOp
+    N, Nc, Nt = (10, 12, 6), 3, 2
+    smaps, _ = _problem(rng, N, Nc, Nt)
+    mask = np.ones((*N, Nt), dtype=bool)
+    A = SenseOp(mask, smaps, ctx=ctx)
+    x = crandn(rng, (*N, Nt), np.complex128)
+    back = A.adjoint(A.forward(x, f64=True), f64=True).to_host()
+    rss2 = np.sum(np.abs(smaps.astype(np.complex128)) ** 2, axis=3)[..., None]
+    assert rel_err(back, rss2 * x) < 1e-12
+
+
+def test_single_coil_closed_form(ctx, rng):
+    """S = 1, full mask: A is the centred unitary DFT."""
+    from mslr_hip.sense import SenseOp
+    N = (8, 6, 10)
+    smaps = np.ones((*N, 1), dtype=np.complex64, order="F")
+    mask = np.ones((*N, 1), dtype=bool)
+    A = SenseOp(mask, smaps, ctx=ctx)
+    x = crandn(rng, (*N, 1), np.complex128)
+    y = A.forward(x, f64=True).to_host()[:, 0, 0]
+    ref = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x[..., 0]))) / np.sqrt(np.prod(N))
+    assert rel_err(y, ref.ravel(order="F")) < 1e-12
+
+
+def test_grad_matches_oracle(ctx, rng):
+    from mslr_hip.sense import SenseOp
+    N, Nc, Nt, S = (12, 12, 10), 3, 4, 2
+    smaps, mask = _problem(rng, N, Nc, Nt)
+    A = SenseOp(mask, smaps, ctx=ctx)
+    X = crandn(rng, (*N, Nt, S))
+    y = crandn(rng, (A.K, Nc, Nt))
+    g, dc = A.grad(X, y, want_cost=True)
+    s = X.astype(np.complex128).sum(axis=4)
+    sm = smaps.astype(np.complex128)
+    r = O.A_forward(s, mask, sm) - y
+    go = O.A_adjoint(r, mask, sm)
+    assert rel_err(g.to_host(), go) < 3e-6
+    assert abs(dc - 0.5 * np.linalg.norm(r) ** 2) / dc < 1e-5
+
+
+def test_unequal_sample_counts_rejected(ctx, rng):
+    from mslr_hip.sense import SenseOp
+    N = (6, 6, 4)
+    smaps, mask = _problem(rng, N, 2, 2)
+    mask[0, 0, 0, 1] = not mask[0, 0, 0, 1]
+    with pytest.raises(AssertionError):
+        SenseOp(mask, smaps, ctx=ctx)
+
+
+def test_unsupported_size_rejected(ctx, rng):
+    from mslr_hip._lib import MSLRUnsupported
+    from mslr_hip.sense import SenseOp
+    N = (7, 6, 4)
+    smaps, mask = _problem(rng, N, 2, 1)
+    with pytest.raises(MSLRUnsupported):
+        SenseOp(mask, smaps, ctx=ctx)
+
+
+def test_poweriter_full_mask_is_one(ctx, rng):
+    from mslr_hip.sense import SenseOp
+    N, Nc, Nt = (8, 8, 6), 3, 3
+    smaps, _ = _problem(rng, N, Nc, Nt)
+    mask = np.ones((*N, Nt), dtype=bool)
+    A = SenseOp(mask, smaps, ctx=ctx)
+    s1, it = A.poweriter(50, 1e-10)
+    assert abs(s1 - 1.0) < 1e-6
+
+
+def test_poweriter_matches_oracle(ctx, rng):
+    from mslr_hip.sense import SenseOp
+    N, Nc, Nt = (8, 10, 6), 3, 2
+    smaps, mask = _problem(rng, N, Nc, Nt, frac=0.3)
+    A = SenseOp(mask, smaps, ctx=ctx)
+    s1, it = A.poweriter(30, 1e-6)
+    sm = smaps.astype(np.complex128)
+    n = int(np.prod(N)) * Nt
+    _, so, ito = O.poweriter_mod(lambda v: O.A_forward(v.reshape(*N, Nt, order="F"), mask, sm).ravel(order="F"),
+                                 lambda w: O.A_adjoint(w.reshape(A.K, Nc, Nt, order="F"), mask, sm).ravel(order="F"),
+                                 n, niter=30, tol=1e-6)
+    assert it == ito
+    assert abs(s1 - so) / so < 1e-10
