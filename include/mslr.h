@@ -63,6 +63,14 @@ const char *mslr_last_error(void);
 /* ABI version (major*10000 + minor*100 + patch). */
 int mslr_version(void);
 
+/* --------------------------------------------------------- instrumentation */
+/* No counterpart in the reference.  While enabled, every kernel launch of the
+ * context is bracketed by HIP events on the context stream (the POGM loop then
+ * runs without hipGraph capture).  Enabling or disabling clears the records. */
+mslr_status_t mslr_prof_enable(mslr_ctx ctx, int enable);
+/* Per-kernel lines "name launches total_ms\n" of the records so far (synchronises). */
+mslr_status_t mslr_prof_report(mslr_ctx ctx, char *buf, size_t len);
+
 /* ---------------------------------------------------------------- context */
 
 /* Replaces the reference's CUDA.functional() probe and device selection

@@ -41,9 +41,64 @@ void *mslr_ctx_s::scratch_get(size_t slot, size_t bytes) {
     return s.ptr;
 }
 
+int mslr_ctx_s::prof_tag(const char *name) {
+    for (size_t i = 0; i < prof_tags.size(); ++i)
+        if (prof_tags[i] == name) return (int)i;
+    prof_tags.emplace_back(name);
+    return (int)prof_tags.size() - 1;
+}
+
+hipEvent_t mslr_ctx_s::prof_event() {
+    if (!prof_pool.empty()) {
+        hipEvent_t e = prof_pool.back();
+        prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
 extern "C" {
 
 const char *mslr_last_error(void) { return g_last_error.c_str(); }
+
+mslr_status_t mslr_prof_enable(mslr_ctx ctx, int enable) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        for (auto &r : c->prof_recs) {
+            c->prof_pool.push_back(r.a);
+            c->prof_pool.push_back(r.b);
+        }
+        c->prof_recs.clear();
+        c->prof = enable != 0;
+    });
+}
+
+mslr_status_t mslr_prof_report(mslr_ctx ctx, char *buf, size_t len) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(buf && len > 0, MSLR_ERR_ARG, "null buffer");
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        std::vector<double> tot(c->prof_tags.size(), 0.0);
+        std::vector<long> cnt(c->prof_tags.size(), 0);
+        for (auto &r : c->prof_recs) {
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+            tot[r.tag] += ms;
+            cnt[r.tag] += 1;
+        }
+        std::string out;
+        for (size_t i = 0; i < tot.size(); ++i) {
+            if (!cnt[i]) continue;
+            char line[256];
+            std::snprintf(line, sizeof line, "%s %ld %.6f\n", c->prof_tags[i].c_str(), cnt[i], tot[i]);
+            out += line;
+        }
+        std::snprintf(buf, len, "%s", out.c_str());
+    });
+}
 
 int mslr_version(void) { return 100; }
 
@@ -80,6 +135,11 @@ mslr_status_t mslr_ctx_destroy(mslr_ctx ctx) {
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for (auto &s : ctx->scratch)
             if (s.ptr) hipFree(s.ptr);
+        for (auto &r : ctx->prof_recs) {
+            hipEventDestroy(r.a);
+            hipEventDestroy(r.b);
+        }
+        for (auto e : ctx->prof_pool) hipEventDestroy(e);
         hipStreamDestroy(ctx->stream);
         delete ctx;
     });

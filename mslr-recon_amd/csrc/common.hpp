@@ -109,9 +109,43 @@ struct mslr_ctx_s {
     int num_cus = 0;
     std::vector<mslr::Scratch> scratch;  // indexed slots, see ctx_scratch()
 
+    // Kernel timing with HIP events on `stream` (enabled by mslr_prof_enable).
+    struct ProfRec {
+        int tag;
+        hipEvent_t a, b;
+    };
+    bool prof = false;
+    std::vector<std::string> prof_tags;
+    std::vector<ProfRec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
+
     void *scratch_get(size_t slot, size_t bytes);
     void activate() const;
+    int prof_tag(const char *name);
+    hipEvent_t prof_event();
 };
+
+namespace mslr {
+// Records HIP events around the launches in its scope when profiling is on.
+struct ProfScope {
+    mslr_ctx_s *ctx;
+    int tag = -1;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(mslr_ctx_s *c, const char *name) : ctx(c) {
+        if (!ctx->prof) return;
+        tag = ctx->prof_tag(name);
+        a = ctx->prof_event();
+        b = ctx->prof_event();
+        hipEventRecord(a, ctx->stream);
+    }
+    ~ProfScope() {
+        if (tag < 0) return;
+        hipEventRecord(b, ctx->stream);
+        ctx->prof_recs.push_back({tag, a, b});
+    }
+};
+}  // namespace mslr
+#define MSLR_PROF(ctx, name) ::mslr::ProfScope prof_scope_##__LINE__((ctx), (name))
 
 namespace mslr {
 

@@ -134,14 +134,20 @@ void launch_img2patches(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64
                         int64_t np) {
     const int64_t n = g.p * g.Nt * np;
     if (n == 0) return;
-    k_img2patches<<<grid_for(n), 256, 0, ctx->stream>>>(img, P, g, ip0, np);
-    HIP_LAUNCH_CHECK();
+    {
+        MSLR_PROF(ctx, "img2patches");
+        k_img2patches<<<grid_for(n), 256, 0, ctx->stream>>>(img, P, g, ip0, np);
+        HIP_LAUNCH_CHECK();
+    }
 }
 
 void launch_patches2img(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *P, c64 *img) {
     const int64_t n = (int64_t)g.N[0] * g.N[1] * g.N[2] * g.Nt;
-    k_patches2img<<<grid_for(n), 256, 0, ctx->stream>>>(P, img, g);
-    HIP_LAUNCH_CHECK();
+    {
+        MSLR_PROF(ctx, "patches2img");
+        k_patches2img<<<grid_for(n), 256, 0, ctx->stream>>>(P, img, g);
+        HIP_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace mslr

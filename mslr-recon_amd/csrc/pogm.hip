@@ -184,13 +184,22 @@ static void pogm_iteration(mslr_pogm_s *st) {
     mslr_ctx_s *ctx = st->ctx;
     const int64_t n = st->V * st->Nt;
     sense_grad_internal(st->sense, st->X, (int)st->S, n, st->y, st->g, nullptr);
-    k_pogm_pre<<<1, 64, 0, ctx->stream>>>(st->dev, st->lam_dev, (int)st->S, st->thr_dev);
-    HIP_LAUNCH_CHECK();
-    k_pogm_update<<<UPD_BLOCKS, 256, 0, ctx->stream>>>(st->X, st->U, st->g, n, (int)st->S, n,
-                                                       st->dev, st->partials);
-    HIP_LAUNCH_CHECK();
-    k_pogm_post<<<1, 256, 0, ctx->stream>>>(st->dev, st->partials, UPD_BLOCKS, st->log_dev);
-    HIP_LAUNCH_CHECK();
+    {
+        MSLR_PROF(ctx, "pogm_pre");
+        k_pogm_pre<<<1, 64, 0, ctx->stream>>>(st->dev, st->lam_dev, (int)st->S, st->thr_dev);
+        HIP_LAUNCH_CHECK();
+    }
+    {
+        MSLR_PROF(ctx, "pogm_update");
+        k_pogm_update<<<UPD_BLOCKS, 256, 0, ctx->stream>>>(st->X, st->U, st->g, n, (int)st->S, n,
+                                                           st->dev, st->partials);
+        HIP_LAUNCH_CHECK();
+    }
+    {
+        MSLR_PROF(ctx, "pogm_post");
+        k_pogm_post<<<1, 256, 0, ctx->stream>>>(st->dev, st->partials, UPD_BLOCKS, st->log_dev);
+        HIP_LAUNCH_CHECK();
+    }
     int *err = (int *)((char *)st->dev + offsetof(PogmDev, err));
     for (int64_t k = 0; k < st->S; ++k) {
         c64 *Xk = st->X + k * n;
@@ -324,7 +333,8 @@ mslr_status_t mslr_pogm_run(mslr_pogm st, int n) {
         mslr_ctx_s *ctx = st->ctx;
         ctx->activate();
         for (int i = 0; i < n; ++i) {
-            const bool use_graph = st->form != MSLR_FORM_PLAIN;
+            // kernel timing needs eager launches
+            const bool use_graph = st->form != MSLR_FORM_PLAIN && !ctx->prof;
             if (use_graph && st->exec) {
                 HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
             } else if (use_graph && st->iters_done >= 1) {

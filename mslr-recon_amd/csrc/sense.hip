@@ -269,19 +269,25 @@ static void run_forward(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t ss
     for (int64_t f0 = 0; f0 < nframes; f0 += F) {
         const int nf = (int)std::min<int64_t>(F, nframes - f0);
         dim3 gx((unsigned)cdiv(P->NYZ, LB), (unsigned)nf);
-        k_fwd_x<T><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
-            X + f0 * P->V, nscales, sstride, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB,
-            P->d[0], Dev<T>::tw(P, 0), P->sgn, Tb);
-        HIP_LAUNCH_CHECK();
+        {
+            MSLR_PROF(ctx, "fwd_x");
+            k_fwd_x<T><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
+                X + f0 * P->V, nscales, sstride, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB,
+                P->d[0], Dev<T>::tw(P, 0), P->sgn, Tb);
+            HIP_LAUNCH_CHECK();
+        }
         const int64_t ta = t0 + f0;
         dim3 gyz((unsigned)Nx, (unsigned)P->Nc, (unsigned)nf);
-        k_fwd_yz<T><<<gyz, P->nt_yz, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
-            Tb, P->Nc, Nx, Ny, Nz, P->d[1], P->d[2], Dev<T>::tw(P, 1), Dev<T>::tw(P, 2), P->sgn,
-            P->ptr + ta * (Nx + 1), P->off + ta * P->K, ref_order ? P->perm + ta * P->K : nullptr,
-            P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1), Dev<T>::ph(P, 2), scale,
-            ysub ? ysub + f0 * P->Nc * P->K : nullptr, out + f0 * P->Nc * P->K,
-            partial ? partial + f0 * P->Nc * Nx : nullptr);
-        HIP_LAUNCH_CHECK();
+        {
+            MSLR_PROF(ctx, "fwd_yz");
+            k_fwd_yz<T><<<gyz, P->nt_yz, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
+                Tb, P->Nc, Nx, Ny, Nz, P->d[1], P->d[2], Dev<T>::tw(P, 1), Dev<T>::tw(P, 2), P->sgn,
+                P->ptr + ta * (Nx + 1), P->off + ta * P->K, ref_order ? P->perm + ta * P->K : nullptr,
+                P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1), Dev<T>::ph(P, 2), scale,
+                ysub ? ysub + f0 * P->Nc * P->K : nullptr, out + f0 * P->Nc * P->K,
+                partial ? partial + f0 * P->Nc * Nx : nullptr);
+            HIP_LAUNCH_CHECK();
+        }
     }
 }
 
@@ -300,17 +306,23 @@ static void run_adjoint(mslr_sense_s *P, const C2<T> *y, int64_t t0, int64_t nfr
         const int nf = (int)std::min<int64_t>(F, nframes - f0);
         const int64_t ta = t0 + f0;
         dim3 gyz((unsigned)Nx, (unsigned)P->Nc, (unsigned)nf);
-        k_adj_yz<T><<<gyz, P->nt_yz, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
-            y + f0 * P->Nc * P->K, P->Nc, Nx, Ny, Nz, P->d[1], P->d[2], Dev<T>::tw(P, 1),
-            Dev<T>::tw(P, 2), -P->sgn, P->ptr + ta * (Nx + 1), P->off + ta * P->K,
-            ref_order ? P->perm + ta * P->K : nullptr, P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1),
-            Dev<T>::ph(P, 2), Tb);
-        HIP_LAUNCH_CHECK();
+        {
+            MSLR_PROF(ctx, "adj_yz");
+            k_adj_yz<T><<<gyz, P->nt_yz, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
+                y + f0 * P->Nc * P->K, P->Nc, Nx, Ny, Nz, P->d[1], P->d[2], Dev<T>::tw(P, 1),
+                Dev<T>::tw(P, 2), -P->sgn, P->ptr + ta * (Nx + 1), P->off + ta * P->K,
+                ref_order ? P->perm + ta * P->K : nullptr, P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1),
+                Dev<T>::ph(P, 2), Tb);
+            HIP_LAUNCH_CHECK();
+        }
         dim3 gx((unsigned)cdiv(P->NYZ, LB), (unsigned)nf);
-        k_adj_x<T><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
-            Tb, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB, P->d[0], Dev<T>::tw(P, 0),
-            -P->sgn, scale, xout + f0 * P->V);
-        HIP_LAUNCH_CHECK();
+        {
+            MSLR_PROF(ctx, "adj_x");
+            k_adj_x<T><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
+                Tb, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB, P->d[0], Dev<T>::tw(P, 0),
+                -P->sgn, scale, xout + f0 * P->V);
+            HIP_LAUNCH_CHECK();
+        }
     }
 }
 

@@ -1,8 +1,9 @@
 // Batched SVST (src/recon.jl:183-211, 217-241; src/sense_gpu.jl:122-128).
 //
 // For a patch matrix X (p x Nt) with m = min(p, Nt):
-//   1. Gram G = X^H X (p >= Nt) or X X^H (p < Nt): FP32 products on 64x64
-//      tiles, split over the contraction length, partial tiles summed in FP64.
+//   1. Gram G = X^H X (p >= Nt) or X X^H (p < Nt): FP64 products of the FP32
+//      data on 64x64 tiles, split over the contraction length, partials summed
+//      in FP64.
 //   2. FP64 Hermitian eigensolve of G, restricted to eigenvalues above
 //      tau = beta^2: Householder reduction to a real tridiagonal T (one
 //      workgroup per patch), Sturm-count bisection for the eigenvalues above
@@ -31,16 +32,21 @@ constexpr int KCH = 4096;    // contraction chunk per workgroup
 constexpr int MAXM = 1024;   // largest Gram dimension
 constexpr int NSLOT = 64;    // concurrent inverse-iteration solves per patch
 constexpr int MAXQ = MAXM / 64;
+constexpr int WS_PER = 256;  // doubles per Gram row of eigen workspace per patch (>= 4*NSLOT and >= threads)
 
 // ------------------------------------------------------------------ Gram
 
 // C[a,b] += sum_k conj(A(k,a)) A(k,b), A(k,a) = base[k*ks + a*cs], for the
 // upper tile pair (ti <= tj) of patch ip and contraction chunk kc.
+// AT is the product / accumulation type: float for the SVST path, double for
+// the nuclear norm (whose small singular values need the extra precision).
+template <class AT>
 __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, int64_t pstride,
                                                     int64_t ks, int64_t cs, int m, int64_t Kc,
-                                                    int nkc, int ntile, c64 *__restrict__ part) {
-    __shared__ c64 As[GK][GT + 1];
-    __shared__ c64 Bs[GK][GT + 1];
+                                                    int nkc, int ntile, C2<AT> *__restrict__ part) {
+    using CA = C2<AT>;
+    __shared__ CA As[GK][GT + 1];
+    __shared__ CA Bs[GK][GT + 1];
     const int ip = blockIdx.z;
     const int kc = blockIdx.y;
     // linear upper-triangular tile index -> (ti, tj), ti <= tj
@@ -54,11 +60,11 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, i
     const c64 *A = P + ip * pstride;
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
-    c64 acc[4][4];
+    CA acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = c64{0.f, 0.f};
+        for (int j = 0; j < 4; ++j) acc[i][j] = CA{AT(0), AT(0)};
     const int64_t kbeg = (int64_t)kc * KCH;
     const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
     const bool kfast = ks == 1;
@@ -80,13 +86,13 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, i
                 if (a0 + aa < m) va = A[k * ks + (int64_t)(a0 + aa) * cs];
                 if (b0 + aa < m) vb = A[k * ks + (int64_t)(b0 + aa) * cs];
             }
-            As[kk][aa] = va;
-            Bs[kk][aa] = vb;
+            As[kk][aa] = CA{(AT)va.re, (AT)va.im};
+            Bs[kk][aa] = CA{(AT)vb.re, (AT)vb.im};
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < GK; ++kk) {
-            c64 av[4], bv[4];
+            CA av[4], bv[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) av[i] = As[kk][tx + 16 * i];
 #pragma unroll
@@ -102,7 +108,7 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, i
         }
         __syncthreads();
     }
-    c64 *out = part + ((int64_t)ip * nkc + kc) * (int64_t)m * m;
+    CA *out = part + ((int64_t)ip * nkc + kc) * (int64_t)m * m;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -113,7 +119,8 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, i
 }
 
 // G[a,b] = sum_kc part (FP64), Hermitian fill; conj_out flips to conj(C).
-__global__ void k_gram_reduce(const c64 *__restrict__ part, int m, int nkc, int np, int conj_out,
+template <class AT>
+__global__ void k_gram_reduce(const C2<AT> *__restrict__ part, int m, int nkc, int np, int conj_out,
                               c128 *__restrict__ G, int *err) {
     const int64_t mm = (int64_t)m * m;
     const int64_t n = mm * np;
@@ -124,9 +131,9 @@ __global__ void k_gram_reduce(const c64 *__restrict__ part, int m, int nkc, int 
         const int a = (int)(r % m), b = (int)(r / m);
         if (a > b) continue;
         double sr = 0.0, si = 0.0;
-        const c64 *pp = part + ip * nkc * mm + r;
+        const C2<AT> *pp = part + ip * nkc * mm + r;
         for (int kc = 0; kc < nkc; ++kc) {
-            const c64 v = pp[kc * mm];
+            const C2<AT> v = pp[kc * mm];
             sr += (double)v.re;
             si += (double)v.im;
         }
@@ -335,6 +342,57 @@ __device__ void tri_solve(const double *d, const double *e, int m, double lam, d
     }
 }
 
+// Eigenvector of the symmetric tridiagonal (d, e) for the eigenvalue lam by a
+// twisted factorisation: dp = pivots of T - lam I = L D L^T (top-down, kept in
+// z), dm = pivots of U D U^T (bottom-up, kept in dm with stride st), twist
+// index r = argmin |dp_r + dm_r - (d_r - lam)|, then z_r = 1 and
+// z_i = -(e_i/dp_i) z_{i+1} above r, z_{i+1} = -(e_i/dm_{i+1}) z_i below r.
+__device__ void twisted_vector(const double *d, const double *e, int m, double lam, double tiny,
+                               double *z, double *dm, int st) {
+    double p = d[0] - lam;
+    if (p == 0.0) p = tiny;
+    z[0] = p;
+    for (int i = 0; i + 1 < m; ++i) {
+        p = (d[i + 1] - lam) - e[i] * e[i] / p;
+        if (p == 0.0) p = tiny;
+        z[i + 1] = p;
+    }
+    double q = d[m - 1] - lam;
+    if (q == 0.0) q = tiny;
+    dm[(int64_t)(m - 1) * st] = q;
+    int best = m - 1;
+    double gbest = fabs(z[m - 1]);
+    for (int i = m - 2; i >= 0; --i) {
+        q = (d[i] - lam) - e[i] * e[i] / q;
+        if (q == 0.0) q = tiny;
+        dm[(int64_t)i * st] = q;
+        const double g = fabs(z[i] + q - (d[i] - lam));
+        if (g < gbest) {
+            gbest = g;
+            best = i;
+        }
+    }
+    z[best] = 1.0;
+    double amax = 1.0;
+    for (int i = best - 1; i >= 0; --i) {
+        const double v = -(e[i] / z[i]) * z[i + 1];
+        z[i] = v;
+        amax = fmax(amax, fabs(v));
+    }
+    for (int i = best; i + 1 < m; ++i) {
+        const double v = -(e[i] / dm[(int64_t)(i + 1) * st]) * z[i];
+        z[i + 1] = v;
+        amax = fmax(amax, fabs(v));
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double t = z[i] / amax;
+        nrm += t * t;
+    }
+    const double inv = 1.0 / (amax * sqrt(nrm));
+    for (int i = 0; i < m; ++i) z[i] *= inv;
+}
+
 __device__ __forceinline__ c128 wave_sum(c128 v) {
     for (int o = 32; o > 0; o >>= 1) {
         v.re += __shfl_xor(v.re, o, 64);
@@ -405,42 +463,48 @@ __global__ __launch_bounds__(256) void k_eigvec(
         lam[k] = 0.5 * (lo + hi);
     }
     __syncthreads();
-    // inverse iteration, one thread per cluster (clusters are runs of
-    // eigenvalues closer than ortol, processed in descending order)
-    const double ortol = 1e-3 * tnorm;
+    // Eigenvectors of T.  A twisted factorisation T - lam I = N_r D N_r^T
+    // (top-down and bottom-up LDL^T, twist where |gamma_r| is smallest) gives
+    // each eigenvector in one O(m) solve, one thread per eigenvector.  Runs of
+    // eigenvalues closer than cltol (tight clusters) are then redone: every
+    // member after the first by inverse iteration from a distinct start with
+    // re-orthogonalisation against the earlier members (one thread per cluster).
+    const double cltol = 1e-12 * tnorm;
     const double pertol = 10.0 * DBL_EPSILON * fmax(tnorm, DBL_MIN);
     const double tiny = DBL_EPSILON * fmax(tnorm, DBL_MIN);
-    double *ws = wsall + (int64_t)ip * NSLOT * 4 * m;
-    // The first wavefront owns the NSLOT solve slots; thread tid takes the
-    // cluster starts tid, tid + NSLOT, ...
+    double *ws = wsall + (int64_t)ip * WS_PER * m;
+    for (int k = tid; k < r; k += nt) {
+        double *z = Z + (int64_t)k * m;
+        twisted_vector(d, e, m, lam[k], tiny, z, ws + tid, nt);
+    }
+    __syncthreads();
     if (tid < NSLOT) {
         double *xs = ws + tid;
         double *u0 = ws + (int64_t)NSLOT * m + tid;
         double *u1 = ws + (int64_t)2 * NSLOT * m + tid;
         double *u2 = ws + (int64_t)3 * NSLOT * m + tid;
         for (int k0 = tid; k0 < r; k0 += NSLOT) {
-            // this thread handles cluster starts at k0 only
-            if (k0 > 0 && lam[k0 - 1] - lam[k0] <= ortol) continue;
+            if (k0 > 0 && lam[k0 - 1] - lam[k0] <= cltol) continue;  // not a cluster start
             int k1 = k0 + 1;
-            while (k1 < r && lam[k1 - 1] - lam[k1] <= ortol) ++k1;
-            double lprev = 0.0;
-            for (int k = k0; k < k1; ++k) {
+            while (k1 < r && lam[k1 - 1] - lam[k1] <= cltol) ++k1;
+            double lprev = lam[k0];
+            for (int k = k0 + 1; k < k1; ++k) {
                 double lk = lam[k];
-                if (k > k0 && lprev - lk < pertol) lk = lprev - pertol;
+                if (lprev - lk < pertol) lk = lprev - pertol;
                 lprev = lk;
                 for (int i = 0; i < m; ++i) xs[i * NSLOT] = hash_unit((uint32_t)i, (uint32_t)k);
                 for (int it = 0; it < 3; ++it) {
                     tri_solve(d, e, m, lk, tiny, xs, u0, u1, u2, NSLOT);
-                    // re-orthogonalise against earlier members of the cluster
                     for (int q = k0; q < k; ++q) {
                         const double *zq = Z + (int64_t)q * m;
                         double dot = 0.0;
                         for (int i = 0; i < m; ++i) dot += zq[i] * xs[i * NSLOT];
                         for (int i = 0; i < m; ++i) xs[i * NSLOT] -= dot * zq[i];
                     }
-                    double nrm = 0.0, amax = 0.0;
+                    double amax = 0.0;
                     for (int i = 0; i < m; ++i) amax = fmax(amax, fabs(xs[i * NSLOT]));
                     if (amax == 0.0) amax = 1.0;
+                    double nrm = 0.0;
                     for (int i = 0; i < m; ++i) {
                         const double t = xs[i * NSLOT] / amax;
                         nrm += t * t;
@@ -691,7 +755,7 @@ __global__ void k_m1_nucnorm(const c64 *__restrict__ P, int64_t len, int64_t Np,
 
 struct SvstWork {
     int64_t m, nkc, chunk;
-    c64 *part;
+    void *part;  // partial Gram tiles, C2<AT>
     c128 *G;
     double *d, *e, *lam, *Z, *ws;
     c128 *tau;
@@ -701,8 +765,8 @@ struct SvstWork {
 };
 
 static size_t svst_bytes_per_patch(int64_t m, int64_t nkc) {
-    return (size_t)(nkc * m * m * 8 + m * m * 16 + m * 8 * 3 + m * 16 + m * m * 8 +
-                    (int64_t)NSLOT * 4 * m * 8 + m * m * 8 + m * 4 + 16);
+    return (size_t)(nkc * m * m * 16 + m * m * 16 + m * 8 * 3 + m * 16 + m * m * 8 +
+                    (int64_t)WS_PER * m * 8 + m * m * 8 + m * 4 + 16);
 }
 
 static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
@@ -722,19 +786,19 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const int64_t mm = m * m;
     size_t off = 0;
-    size_t o_part = off; off += al((size_t)chunk * w.nkc * mm * 8);
+    size_t o_part = off; off += al((size_t)chunk * w.nkc * mm * 16);
     size_t o_G = off; off += al((size_t)chunk * mm * 16);
     size_t o_d = off; off += al((size_t)chunk * m * 8);
     size_t o_e = off; off += al((size_t)chunk * m * 8);
     size_t o_lam = off; off += al((size_t)chunk * m * 8);
     size_t o_tau = off; off += al((size_t)chunk * m * 16);
     size_t o_Z = off; off += al((size_t)chunk * mm * 8);
-    size_t o_ws = off; off += al((size_t)chunk * NSLOT * 4 * m * 8);
+    size_t o_ws = off; off += al((size_t)chunk * WS_PER * m * 8);
     size_t o_U = off; off += al((size_t)chunk * mm * 8);
     size_t o_f = off; off += al((size_t)chunk * m * 4);
     size_t o_r = off; off += al((size_t)chunk * 4);
     char *base = (char *)ctx->scratch_get(SLOT_GRAM, off);
-    w.part = (c64 *)(base + o_part);
+    w.part = (void *)(base + o_part);
     w.G = (c128 *)(base + o_G);
     w.d = (double *)(base + o_d);
     w.e = (double *)(base + o_e);
@@ -748,6 +812,7 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     return w;
 }
 
+template <class AT>
 static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, int64_t p,
                              int64_t Nt, int64_t nb, int *err_dev) {
     const int m = (int)w.m;
@@ -758,16 +823,25 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
     const int ntile = (int)cdiv(m, GT);
     const int npair = ntile * (ntile + 1) / 2;
     dim3 gg((unsigned)npair, (unsigned)w.nkc, (unsigned)nb);
-    k_gram_tiles<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, ntile,
-                                              w.part);
-    HIP_LAUNCH_CHECK();
+    {
+        MSLR_PROF(ctx, "gram_tiles");
+        k_gram_tiles<AT><<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, ntile,
+                                                      (C2<AT> *)w.part);
+        HIP_LAUNCH_CHECK();
+    }
     const int64_t ne = (int64_t)m * m * nb;
     const unsigned gr = (unsigned)std::min<int64_t>(cdiv(ne, 256), 1 << 20);
-    k_gram_reduce<<<gr, 256, 0, ctx->stream>>>(w.part, m, (int)w.nkc, (int)nb, left ? 1 : 0, w.G,
-                                               err_dev);
-    HIP_LAUNCH_CHECK();
-    k_tridiag<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.G, m, w.d, w.e, w.tau);
-    HIP_LAUNCH_CHECK();
+    {
+        MSLR_PROF(ctx, "gram_reduce");
+        k_gram_reduce<AT><<<gr, 256, 0, ctx->stream>>>((const C2<AT> *)w.part, m, (int)w.nkc, (int)nb,
+                                                       left ? 1 : 0, w.G, err_dev);
+        HIP_LAUNCH_CHECK();
+    }
+    {
+        MSLR_PROF(ctx, "tridiag");
+        k_tridiag<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.G, m, w.d, w.e, w.tau);
+        HIP_LAUNCH_CHECK();
+    }
 }
 
 void svst_batched(mslr_ctx_s *ctx, c64 *P, int64_t p, int64_t Nt, int64_t Np,
@@ -779,8 +853,11 @@ void svst_batched(mslr_ctx_s *ctx, c64 *P, int64_t p, int64_t Nt, int64_t Np,
     const int64_t len = p * Nt;
     if (m == 1) {
         const unsigned nb = (unsigned)cdiv(Np, 4);
-        k_svst_m1<<<nb, 256, 0, ctx->stream>>>(P, len, Np, beta_dev, per_patch);
-        HIP_LAUNCH_CHECK();
+        {
+            MSLR_PROF(ctx, "svst_m1");
+            k_svst_m1<<<nb, 256, 0, ctx->stream>>>(P, len, Np, beta_dev, per_patch);
+            HIP_LAUNCH_CHECK();
+        }
         return;
     }
     static bool attr = false;
@@ -795,19 +872,28 @@ void svst_batched(mslr_ctx_s *ctx, c64 *P, int64_t p, int64_t Nt, int64_t Np,
     for (int64_t ip0 = 0; ip0 < Np; ip0 += w.chunk) {
         const int64_t nb = std::min<int64_t>(w.chunk, Np - ip0);
         c64 *Pc = P + ip0 * len;
-        gram_and_tridiag(ctx, w, Pc, p, Nt, nb, err_dev);
-        k_eigvec<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.G, w.d, w.e, w.tau, (int)m,
-                                                       beta_dev + (per_patch ? ip0 : 0), per_patch,
-                                                       w.lam, w.Z, w.ws, w.U, w.f, w.r);
-        HIP_LAUNCH_CHECK();
+        // FP64 products: the Gram squares the dynamic range of X, and at production
+        // shapes (global patch s1/s_noise ~ 1e4) FP32 products would bury the
+        // singular values near the threshold under rounding error.
+        gram_and_tridiag<double>(ctx, w, Pc, p, Nt, nb, err_dev);
+        {
+            MSLR_PROF(ctx, "eigvec");
+            k_eigvec<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.G, w.d, w.e, w.tau, (int)m,
+                                                           beta_dev + (per_patch ? ip0 : 0), per_patch,
+                                                           w.lam, w.Z, w.ws, w.U, w.f, w.r);
+            HIP_LAUNCH_CHECK();
+        }
         const int64_t nblk = left ? cdiv(Nt, RB) : cdiv(p, RB);
         const size_t lds = (size_t)2 * RB * m * sizeof(c64);
         for (int64_t b0 = 0; b0 < nb; b0 += 65535) {
             const int64_t nbb = std::min<int64_t>(65535, nb - b0);
             dim3 ga((unsigned)nblk, (unsigned)nbb);
-            k_apply<<<ga, 256, lds, ctx->stream>>>(Pc + b0 * len, p, (int)Nt, (int)m, left ? 1 : 0,
-                                                  RB, w.U + b0 * m * m, w.f + b0 * m, w.r + b0);
-            HIP_LAUNCH_CHECK();
+            {
+                MSLR_PROF(ctx, "apply");
+                k_apply<<<ga, 256, lds, ctx->stream>>>(Pc + b0 * len, p, (int)Nt, (int)m, left ? 1 : 0,
+                                                      RB, w.U + b0 * m * m, w.f + b0 * m, w.r + b0);
+                HIP_LAUNCH_CHECK();
+            }
         }
     }
 }
@@ -829,7 +915,7 @@ void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64
         const SvstWork w = svst_work(ctx, m, left ? Nt : p, Np);
         for (int64_t ip0 = 0; ip0 < Np; ip0 += w.chunk) {
             const int64_t nb = std::min<int64_t>(w.chunk, Np - ip0);
-            gram_and_tridiag(ctx, w, P + ip0 * p * Nt, p, Nt, nb, nullptr);
+            gram_and_tridiag<double>(ctx, w, P + ip0 * p * Nt, p, Nt, nb, nullptr);
             k_nucnorm<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.d, w.e, (int)m, per + ip0);
             HIP_LAUNCH_CHECK();
         }

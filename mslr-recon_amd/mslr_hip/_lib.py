@@ -87,6 +87,8 @@ _pi = C.POINTER(C.c_int)
 SIGNATURES: dict[str, tuple] = {
     "mslr_last_error": (C.c_char_p, []),
     "mslr_version": (C.c_int, []),
+    "mslr_prof_enable": (C.c_int, [_p, C.c_int]),
+    "mslr_prof_report": (C.c_int, [_p, C.c_char_p, C.c_size_t]),
     "mslr_ctx_create": (C.c_int, [C.c_int, C.POINTER(_p)]),
     "mslr_ctx_destroy": (C.c_int, [_p]),
     "mslr_ctx_sync": (C.c_int, [_p]),

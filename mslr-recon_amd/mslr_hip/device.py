@@ -37,6 +37,20 @@ class Context:
         _lib.check(self._lib.mslr_ctx_info(self.handle, name, 256, C.byref(tot), C.byref(fr)))
         return {"name": name.value.decode(), "total_bytes": tot.value, "free_bytes": fr.value}
 
+    def prof_enable(self, on: bool = True) -> None:
+        """Time every kernel launch with HIP events (clears earlier records)."""
+        _lib.check(self._lib.mslr_prof_enable(self.handle, int(bool(on))))
+
+    def prof_report(self) -> dict:
+        """{kernel: (launches, total_ms)} of the launches recorded since prof_enable."""
+        buf = C.create_string_buffer(1 << 16)
+        _lib.check(self._lib.mslr_prof_report(self.handle, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             self._lib.mslr_ctx_destroy(self.handle)

@@ -10,3 +10,4 @@ function against something other than itself) are listed in mslr_ref.PINS and
 exercised by tests/test_oracle_*.py.
 """
 from .mslr_ref import *  # noqa: F401,F403
+from .mslr_ref import _gr_restart, _patch_starts, _workers  # noqa: F401,E402

@@ -125,6 +125,29 @@ def test_clustered_singular_values(ctx, rng):
     assert rel_err(Y, ref) < 1e-5
 
 
+@pytest.mark.parametrize("p,n", [(4000, 60), (40, 500)])
+def test_high_dynamic_range_full_rank(ctx, rng, p, n):
+    """A dominant rank-1 term (s1/s_noise ~ 1e4, as the production global patch) with every
+    noise singular value above beta: all eigenpairs are kept, and the noise subspace must
+    still be resolved."""
+    from mslr_hip import recon
+    u = crandn(rng, (p, 1), np.complex128)
+    v = crandn(rng, (n, 1), np.complex128)
+    noise = crandn(rng, (p, n), np.complex128)
+    X = (1e4 * (u @ v.conj().T) / np.sqrt(p * n) * np.sqrt(p + n) + noise).astype(np.complex64)
+    s = np.linalg.svd(X.astype(np.complex128), compute_uv=False)
+    beta = 0.5 * s[-1]
+    Y = recon.SVST(X, beta).to_host()
+    Yo = O.SVST(X.astype(np.complex128), np.float32(beta))
+    assert rel_err(Y, Yo) < 2e-5
+    # The part orthogonal to the dominant direction: an FP32 Gram would leave it O(1)
+    # wrong; FP32 data and the FP32 apply bound it at ~eps32 * s1/s_noise (as the
+    # reference's FP32 SVD).
+    U, S, Vh = np.linalg.svd(X.astype(np.complex128), full_matrices=False)
+    Pn = np.eye(p) - np.outer(U[:, 0], U[:, 0].conj())
+    assert rel_err(Pn @ Y, Pn @ Yo) < 5e-3
+
+
 def test_patch_nucnorm(ctx, rng):
     from mslr_hip import recon
     P = np.stack([lowrank(rng, 40, 15, 2) for _ in range(4)], axis=2)

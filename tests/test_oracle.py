@@ -1,0 +1,295 @@
+"""Pins of the CPU oracle against closed forms, brute force and hand-traced values.
+
+None of these retypes the oracle's own formula: each compares it with an
+independent statement of the same quantity (explicit DFT sums, per-element
+index loops, textbook SVD identities, hand-derived POGM numbers).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import crandn, rel_err
+
+
+# ----------------------------------------------------------------- SENSE
+def brute_centered_dft3(x, sign=-1):
+    """out[k] = sum_n x[n] exp(sign 2 pi i sum_a (k_a - c_a)(n_a - c_a)/N_a), c = N//2."""
+    N = x.shape
+    out = np.zeros(N, dtype=np.complex128)
+    grids = [np.arange(n) - n // 2 for n in N]
+    E = [np.exp(sign * 2j * np.pi * np.outer(g, g) / n) for g, n in zip(grids, N)]
+    # separable explicit sums
+    out = np.einsum("ai,bj,ck,ijk->abc", E[0], E[1], E[2], x)
+    return out
+
+
+@pytest.mark.parametrize("N", [(4, 6, 2), (5, 3, 4), (6, 5, 3)])
+def test_forward_equals_bruteforce_dft(rng, N):
+    Nc = 2
+    smaps = crandn(rng, (*N, Nc), np.complex128)
+    x = crandn(rng, N, np.complex128)
+    mask = rng.random(N) < 0.5
+    y = O.Asense_forward(x, mask, smaps)
+    ref = np.stack([brute_centered_dft3(smaps[..., c] * x) for c in range(Nc)], axis=-1)
+    ref = ref.reshape(-1, Nc, order="F")[np.flatnonzero(mask.ravel(order="F"))] / math.sqrt(np.prod(N))
+    assert rel_err(y, ref) < 1e-13
+
+
+@pytest.mark.parametrize("N", [(4, 6, 2), (5, 3, 4)])
+def test_adjoint_is_conjugate_transpose_bruteforce(rng, N):
+    """Build A as a dense matrix from the brute-force DFT; the oracle adjoint equals A^H."""
+    Nc = 2
+    smaps = crandn(rng, (*N, Nc), np.complex128)
+    mask = rng.random(N) < 0.6
+    V = int(np.prod(N))
+    cols = []
+    for j in range(V):
+        e = np.zeros(V, dtype=np.complex128)
+        e[j] = 1
+        cols.append(O.Asense_forward(e.reshape(N, order="F"), mask, smaps).ravel(order="F"))
+    Am = np.stack(cols, axis=1)
+    y = crandn(rng, (int(mask.sum()), Nc), np.complex128)
+    xa = O.Asense_adjoint(y, mask, smaps).ravel(order="F")
+    assert rel_err(xa, Am.conj().T @ y.ravel(order="F")) < 1e-12
+
+
+def test_single_coil_full_mask_is_unitary_centred_dft(rng):
+    N = (8, 6, 4)
+    x = crandn(rng, N, np.complex128)
+    y = O.Asense_forward(x, np.ones(N, bool), np.ones((*N, 1), np.complex128))[:, 0]
+    ref = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x))) / math.sqrt(np.prod(N))
+    assert rel_err(y, ref.ravel(order="F")) < 1e-13
+
+
+def test_even_size_sign_modulation_identity(rng):
+    """For even N, fftshift(fft(ifftshift(x))) = (-1)^(k+N/2) fft((-1)^n x) per axis."""
+    N = (8, 6, 4)
+    x = crandn(rng, N, np.complex128)
+    lhs = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x)))
+    n = np.indices(N).sum(axis=0)
+    half = sum(s // 2 for s in N)
+    rhs = (-1.0) ** (n + half) * np.fft.fftn((-1.0) ** n * x)
+    assert rel_err(lhs, rhs) < 1e-13
+
+
+def test_full_mask_unit_rss_gives_identity(rng):
+    N, Nc, Nt = (6, 4, 4), 3, 2
+    smaps = crandn(rng, (*N, Nc), np.complex128)
+    smaps /= np.sqrt(np.sum(np.abs(smaps) ** 2, axis=3, keepdims=True))
+    mask = np.ones((*N, Nt), bool)
+    x = crandn(rng, (*N, Nt), np.complex128)
+    assert rel_err(O.A_adjoint(O.A_forward(x, mask, smaps), mask, smaps), x) < 1e-13
+
+
+def test_reference_gpu_adjoint_scaling():
+    """exact=False reproduces src/sense_gpu.jl:93: 1/prod(N) times the exact adjoint (even N)."""
+    rng = np.random.default_rng(0)
+    N = (4, 4, 2)
+    smaps = crandn(rng, (*N, 2), np.complex128)
+    mask = rng.random(N) < 0.5
+    y = crandn(rng, (int(mask.sum()), 2), np.complex128)
+    a = O.Asense_adjoint(y, mask, smaps, exact=True)
+    b = O.Asense_adjoint(y, mask, smaps, exact=False)
+    assert rel_err(b * np.prod(N), a) < 1e-13
+
+
+# ----------------------------------------------------------------- patches
+def brute_patches(img, ps, ss):
+    N = img.shape[:3]
+    psc = [min(ps[a], N[a]) for a in range(3)]
+    ns = [-(-(N[a] - psc[a]) // ss[a]) + 1 for a in range(3)]
+    Np = ns[0] * ns[1] * ns[2]
+    p = psc[0] * psc[1] * psc[2]
+    P = np.zeros((p, img.shape[3], Np), dtype=img.dtype)
+    for ip in range(Np):
+        ix, iy, iz = ip % ns[0], (ip // ns[0]) % ns[1], ip // (ns[0] * ns[1])
+        for i in range(p):
+            dx, dy, dz = i % psc[0], (i // psc[0]) % psc[1], i // (psc[0] * psc[1])
+            x = min(ix * ss[0], N[0] - psc[0]) + dx
+            y = min(iy * ss[1], N[1] - psc[1]) + dy
+            z = min(iz * ss[2], N[2] - psc[2]) + dz
+            P[i, :, ip] = img[x, y, z, :]
+    return P
+
+
+GEOMS = [((6, 5, 4), (2, 2, 2), (2, 2, 2)), ((6, 5, 4), (3, 3, 3), (2, 1, 2)),
+         ((6, 5, 4), (9, 2, 4), (1, 2, 3)), ((7, 5, 3), (1, 1, 1), (1, 1, 1))]
+
+
+@pytest.mark.parametrize("N,ps,ss", GEOMS)
+def test_img2patches_bruteforce(rng, N, ps, ss):
+    img = crandn(rng, (*N, 3))
+    assert np.array_equal(O.img2patches(img, ps, ss), brute_patches(img, ps, ss))
+
+
+@pytest.mark.parametrize("N,ps,ss", GEOMS)
+def test_patches2img_adjoint_relation(rng, N, ps, ss):
+    """patches2img = D^-1 E^T with E the extraction: <E x, P> = <x, D patches2img(P)>."""
+    img = crandn(rng, (*N, 2), np.complex128)
+    P = crandn(rng, brute_patches(img, ps, ss).shape, np.complex128)
+    cnt = O.coverage_count(N, ps, ss)
+    back = O.patches2img(P, ps, ss, N, dtype=np.complex128) * np.maximum(cnt, 1)[..., None]
+    lhs = np.vdot(brute_patches(img, ps, ss), P)
+    rhs = np.vdot(img, back)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+def test_coverage_closed_form():
+    """Non-overlapping exact tiling covers once; half overlap covers interior voxels 2^3 times."""
+    assert np.all(O.coverage_count((12, 12, 10), (4, 4, 5), (4, 4, 5)) == 1)
+    c = O.coverage_count((12, 12, 12), (6, 6, 6), (3, 3, 3))
+    assert c[3:9, 3:9, 3:9].min() == 8 and c[0, 0, 0] == 1
+
+
+# ----------------------------------------------------------------- SVST
+def test_svst_rank1(rng):
+    u = crandn(rng, (30, 1), np.complex128)
+    v = crandn(rng, (12, 1), np.complex128)
+    X = u @ v.conj().T
+    s = np.linalg.norm(u) * np.linalg.norm(v)
+    assert rel_err(O.SVST(X, 0.25 * s), 0.75 * X) < 1e-13
+
+
+def test_svst_vector_group_shrink(rng):
+    v = crandn(rng, (1, 40), np.complex128)
+    n = np.linalg.norm(v)
+    assert rel_err(O.SVST(v, 0.4 * n), v * 0.6) < 1e-13
+
+
+def test_svst_orthogonal_columns(rng):
+    Q, _ = np.linalg.qr(crandn(rng, (20, 4), np.complex128))
+    d = np.array([5.0, 3.0, 2.0, 0.5])
+    X = Q * d
+    assert rel_err(O.SVST(X, 2.5), Q * np.maximum(d - 2.5, 0)) < 1e-13
+
+
+def test_svst_limits(rng):
+    X = crandn(rng, (9, 7), np.complex128)
+    assert rel_err(O.SVST(X, 0.0), X) < 1e-13
+    s1 = np.linalg.norm(X, 2)
+    assert np.all(O.SVST(X, 1.001 * s1) == 0)
+    assert np.all(O.SVST(np.zeros((5, 3), np.complex64), 1.0) == 0)
+
+
+def test_nucnorm_closed_forms(rng):
+    Q, _ = np.linalg.qr(crandn(rng, (10, 3), np.complex128))
+    P = np.stack([Q * np.array([3.0, 2.0, 1.0]), Q * np.array([1.0, 1.0, 1.0])], axis=2)
+    assert abs(O.patch_nucnorm(P) - 9.0) < 1e-12
+    assert abs(O.patch_nucnorm(P, [2.0, 1.0]) - (12.0 + 3.0) / 2) < 1e-12
+
+
+# ----------------------------------------------------------------- POGM
+def test_pogm_scalar_sequence_hand_values():
+    """t_1 = (1+sqrt5)/2, beta_1 = 0, gamma_1 = 1/t_1; t_2 = (1+sqrt(1+4 t_1^2))/2 ...; the last
+    iteration uses (1+sqrt(1+8 t^2))/2 (src/mirt_mod.jl:140-142)."""
+    phi = (1 + math.sqrt(5)) / 2
+    rows = O.pogm_scalars(3)
+    assert rows[0][0] == pytest.approx(phi, rel=1e-15)
+    assert rows[0][1] == 0.0
+    assert rows[0][2] == pytest.approx(1 / phi, rel=1e-15)
+    t2 = (1 + math.sqrt(1 + 4 * phi ** 2)) / 2
+    assert rows[1][0] == pytest.approx(t2, rel=1e-15)
+    assert rows[1][1] == pytest.approx((phi - 1) / t2, rel=1e-15)
+    t3 = (1 + math.sqrt(1 + 8 * t2 ** 2)) / 2
+    assert rows[2][0] == pytest.approx(t3, rel=1e-15)
+    # a restart after iteration 1 resets t to 1: iteration 2 repeats iteration 1's coefficients
+    rr = O.pogm_scalars(3, restarts=[1])
+    assert rr[1][:3] == pytest.approx(rows[0][:3], rel=1e-15)
+
+
+def test_pogm_aliased_toy_quadratic_hand_trace():
+    """f(x) = 0.5 (x - 1)^2, L = 1, identity prox, x0 = 0: traced by hand.
+    it1: g=-1, u1=1, t1=phi, beta=0, gamma=1/phi, z1 = 1 + 1/phi, no restart (dot = 1 > 0).
+    it2: g = 1/phi, u2 = z1 - g = 1, t2, beta2=(phi-1)/t2, gamma2=phi/t2,
+         z2 = 1 + beta2*(1-1) + gamma2*(1 - z1) = 1 - gamma2/phi; dot = -(1/phi)*(u2-u1)=0 -> restart."""
+    phi = (1 + math.sqrt(5)) / 2
+    x0 = np.zeros(1, dtype=np.complex128)
+
+    def grad(x):
+        return x - 1.0
+
+    def prox(z, c):
+        return z  # mutating identity
+
+    x, out = O.pogm_mod(x0, lambda x: 0.0, grad, 1.0, niter=2, g_prox=prox,
+                        fun=lambda it, xk, yk, r: (complex(xk[0]), r))
+    assert out[1][0] == pytest.approx(1 + 1 / phi, rel=1e-15)
+    assert out[1][1] is False
+    t2 = (1 + math.sqrt(1 + 8 * phi ** 2)) / 2  # iteration 2 is the last one
+    gamma2 = phi / t2
+    assert out[2][0] == pytest.approx(1 - gamma2 / phi, rel=1e-14)
+    assert out[2][1] is True  # equality (dot == 0) counts as a restart
+
+
+def test_aliased_prox_differs_from_textbook():
+    """With a non-mutating prox the zeta_old term survives and the iterates differ."""
+    rng = np.random.default_rng(3)
+    b = crandn(rng, (20,), np.complex128)
+
+    def grad(x):
+        return x - b
+
+    def prox_inplace(z, c):
+        z *= np.maximum(1 - 0.1 * c / np.maximum(np.abs(z), 1e-30), 0)
+        return z
+
+    def prox_copy(z, c):
+        return z * np.maximum(1 - 0.1 * c / np.maximum(np.abs(z), 1e-30), 0)
+
+    xa, _ = O.pogm_mod(np.zeros(20, np.complex128), lambda x: 0.0, grad, 2.0, niter=6, g_prox=prox_inplace)
+    xb, _ = O.pogm_mod(np.zeros(20, np.complex128), lambda x: 0.0, grad, 2.0, niter=6, g_prox=prox_copy)
+    assert rel_err(xa, xb) > 1e-6
+
+
+def test_restart_is_unconjugated():
+    Fg = np.array([1j])
+    d = np.array([1j])
+    # Re(-(1j)(1j)) = 1 > 0 -> no restart; the conjugated product would give -1 -> restart
+    assert O._gr_restart(Fg, d, 0.0) is False
+
+
+def test_pogm_argument_errors():
+    x0 = np.zeros(2)
+    with pytest.raises(ValueError):
+        O.pogm_mod(x0, lambda x: 0, lambda x: x, 1.0, mom="foo")
+    with pytest.raises(ValueError):
+        O.pogm_mod(x0, lambda x: 0, lambda x: x, -1.0)
+    with pytest.raises(ValueError):
+        O.pogm_mod(x0, lambda x: 0, lambda x: x, 1.0, restart_cutoff=1.0)
+
+
+# ----------------------------------------------------------------- power iteration, lambdas
+def test_poweriter_diagonal():
+    d = np.array([0.5, 3.0, 1.0, 2.0])
+    _, s, _ = O.poweriter_mod(lambda x: d * x, lambda y: d * y, 4, niter=500, tol=1e-12)
+    assert s == pytest.approx(3.0, rel=1e-9)
+
+
+def test_lambdas_closed_form():
+    # sqrt(p) + sqrt(Nt) + sqrt(log(V Nt / max(p, Nt))), computed by hand for 90x90x60, Nt=300
+    V, Nt = 486000, 300
+    for ps, lam in zip([[90, 90, 60], [10, 10, 10], [1, 1, 1]], O.lambdas_ong_lustig((90, 90, 60), 300, [[90, 90, 60], [10, 10, 10], [1, 1, 1]])):
+        p = ps[0] * ps[1] * ps[2]
+        assert lam == pytest.approx(math.sqrt(p) + math.sqrt(Nt) + math.sqrt(math.log(V * Nt / max(p, Nt))), rel=1e-15)
+    assert O.lambdas_ong_lustig((90, 90, 60), 300, [[1, 1, 1]])[0] == pytest.approx(21.9391, abs=1e-4)
+    assert O.lambdas_ong_lustig((32, 32, 24), 40, [[32, 32, 24]])[0] == pytest.approx(165.0125, abs=1e-4)
+
+
+def test_quantile_is_type7():
+    a = np.array([1.0, 2.0, 3.0, 4.0])
+    # type 7: h = (n-1) q + 1 -> for q=0.99: 3.97 -> 3 + 0.97*(4-3)
+    assert O.quantile_linear(a, 0.99) == pytest.approx(3.97, rel=1e-15)
+
+
+def test_small_end_to_end_modes_agree():
+    """The reference-faithful (FP32 SVD, FP64 state) and FP64 runs agree closely on a small problem."""
+    from problems import prepared
+    cfg, smaps, ksp, omega, sf = prepared("small")
+    kw = dict(N=tuple(cfg["N"]), Nt=cfg["Nt"], PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"],
+              NITERS=4, sigma1A=1.0)
+    Xa, ra, _ = O.run_recon_arrays(ksp, omega, smaps, mode="f64", **kw)
+    Xb, rb, _ = O.run_recon_arrays(ksp, omega, smaps, mode="ref", **kw)
+    assert ra == rb
+    assert rel_err(Xb, Xa) < 1e-5
