@@ -1,0 +1,269 @@
+#=
+MSLRHip.jl
+Julia binding of libmslr_hip.so (the C ABI in include/mslr.h) for the
+reference's driver: the same call sites as Asense_gpu / patchSVST_gpu /
+pogm_mod / poweriter_mod (src/sense_gpu.jl, src/recon.jl, src/mirt_mod.jl,
+scripts/reconstruct.jl), with device buffers owned by the library.
+
+Load with:  include("MSLRHip.jl"); using .MSLRHip
+Set ENV["MSLR_HIP_LIB"] to the path of libmslr_hip.so if it is not next to
+this file's package directory (../mslr_hip/libmslr_hip.so).
+=#
+module MSLRHip
+
+export HipContext, HipArray, Asense_hip, img2patches_hip, patches2img_hip, SVST_hip,
+       patchSVST_hip, patch_nucnorm_hip, pogm_hip, poweriter_hip, upload, download
+
+const LIB = get(ENV, "MSLR_HIP_LIB", joinpath(@__DIR__, "..", "mslr_hip", "libmslr_hip.so"))
+
+# status codes -> the reference's exception kinds
+function check(code::Cint)
+    code == 0 && return nothing
+    msg = unsafe_string(ccall((:mslr_last_error, LIB), Cstring, ()))
+    code == 1 && throw(ArgumentError(msg))
+    code == 2 && throw(AssertionError(msg))
+    code == 4 && throw(OutOfMemoryError())
+    code == 6 && throw(ErrorException("numeric failure: " * msg))
+    error(msg)   # 3: device, 5: unsupported size
+end
+
+# ---------------------------------------------------------------- context + buffers
+mutable struct HipContext
+    h::Ptr{Cvoid}
+    function HipContext(device::Integer = 0)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:mslr_ctx_create, LIB), Cint, (Cint, Ref{Ptr{Cvoid}}), device, r))
+        c = new(r[])
+        finalizer(c) do x
+            x.h != C_NULL && ccall((:mslr_ctx_destroy, LIB), Cint, (Ptr{Cvoid},), x.h)
+            x.h = C_NULL
+        end
+        return c
+    end
+end
+
+const _CTX = Ref{Union{Nothing,HipContext}}(nothing)
+default_context() = (_CTX[] === nothing && (_CTX[] = HipContext(0)); _CTX[])
+
+"Column-major device array owned by the library (ComplexF32 unless stated)."
+mutable struct HipArray{T,N}
+    ptr::Ptr{Cvoid}
+    dims::NTuple{N,Int}
+    ctx::HipContext
+    function HipArray{T}(dims::NTuple{N,Int}; ctx = default_context()) where {T,N}
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:mslr_malloc, LIB), Cint, (Ptr{Cvoid}, Csize_t, Ref{Ptr{Cvoid}}),
+                    ctx.h, max(1, prod(dims)) * sizeof(T), r))
+        a = new{T,N}(r[], dims, ctx)
+        finalizer(a) do x
+            x.ptr != C_NULL && ccall((:mslr_free, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), x.ctx.h, x.ptr)
+            x.ptr = C_NULL
+        end
+        return a
+    end
+end
+Base.size(a::HipArray) = a.dims
+Base.eltype(::HipArray{T}) where {T} = T
+
+function upload(x::Array{T,N}; ctx = default_context()) where {T,N}
+    a = HipArray{T}(size(x); ctx)
+    check(ccall((:mslr_upload, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{T}, Csize_t),
+                ctx.h, a.ptr, x, sizeof(x)))
+    return a
+end
+
+function download(a::HipArray{T,N}) where {T,N}
+    x = Array{T,N}(undef, a.dims)
+    check(ccall((:mslr_download, LIB), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{Cvoid}, Csize_t),
+                a.ctx.h, x, a.ptr, sizeof(x)))
+    return x
+end
+Base.Array(a::HipArray) = download(a)
+
+# ---------------------------------------------------------------- SENSE (src/sense_gpu.jl:56-108)
+"""
+    Asense_hip(samp, smaps; fft_forward=true, unitary=true)
+
+All-frames SENSE operator for samp::AbstractArray{Bool,4} (Nx,Ny,Nz,Nt) (a 3D
+mask gives one frame).  `A * x` and `A' * y` act on HipArrays (or host Arrays)
+of shape idim = (Nx,Ny,Nz,Nt) and odim = (K,Nc,Nt).  This replaces
+`block_diag([Asense_gpu(s, smaps) for s in eachslice(Ω; dims=4)]...)`
+(scripts/reconstruct.jl:142).  The adjoint is the exact conjugate transpose.
+"""
+mutable struct Asense_hip
+    h::Ptr{Cvoid}
+    N::NTuple{3,Int}
+    Nc::Int
+    Nt::Int
+    K::Int
+    ctx::HipContext
+end
+
+function Asense_hip(samp::AbstractArray{Bool}, smaps::AbstractArray{<:Complex,4};
+                    fft_forward::Bool = true, unitary::Bool = true, ctx = default_context())
+    m = ndims(samp) == 3 ? reshape(samp, size(samp)..., 1) : samp
+    N = size(smaps)[1:3]
+    Nc = size(smaps, 4)
+    Nt = size(m, 4)
+    s32 = ComplexF32.(collect(smaps))
+    m8 = UInt8.(collect(m))
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:mslr_sense_create, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Int64, Int64, Ptr{ComplexF32}, Ptr{UInt8}, Cint, Cint, Ref{Ptr{Cvoid}}),
+                ctx.h, Int64[N...], Nc, Nt, s32, m8, fft_forward, unitary, r))
+    K = ccall((:mslr_sense_nsamples, LIB), Int64, (Ptr{Cvoid},), r[])
+    A = Asense_hip(r[], N, Nc, Nt, K, ctx)
+    finalizer(A) do x
+        x.h != C_NULL && ccall((:mslr_sense_destroy, LIB), Cint, (Ptr{Cvoid},), x.h)
+        x.h = C_NULL
+    end
+    return A
+end
+
+struct AdjointHip
+    A::Asense_hip
+end
+Base.adjoint(A::Asense_hip) = AdjointHip(A)
+idim(A::Asense_hip) = (A.N..., A.Nt)
+odim(A::Asense_hip) = (A.K, A.Nc, A.Nt)
+
+function Base.:*(A::Asense_hip, x::HipArray{ComplexF32})
+    y = HipArray{ComplexF32}((A.K, A.Nc, A.Nt); ctx = A.ctx)
+    check(ccall((:mslr_sense_forward, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64),
+                A.h, x.ptr, y.ptr, 0, A.Nt))
+    return y
+end
+Base.:*(A::Asense_hip, x::Array) = download(A * upload(ComplexF32.(x); ctx = A.ctx))
+
+function Base.:*(Ah::AdjointHip, y::HipArray{ComplexF32})
+    A = Ah.A
+    x = HipArray{ComplexF32}((A.N..., A.Nt); ctx = A.ctx)
+    check(ccall((:mslr_sense_adjoint, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64),
+                A.h, y.ptr, x.ptr, 0, A.Nt))
+    return x
+end
+Base.:*(Ah::AdjointHip, y::Array) = download(Ah * upload(ComplexF32.(y); ctx = Ah.A.ctx))
+
+"poweriter_mod(undim(A)) in FP64 on the device (src/mirt_mod.jl:215-238); returns (nothing, σ1)."
+function poweriter_hip(A::Asense_hip; niter::Int = 200, tol::Real = 1e-6, chat::Bool = true)
+    s = Ref{Cdouble}(0); it = Ref{Cint}(0)
+    check(ccall((:mslr_sense_poweriter, LIB), Cint, (Ptr{Cvoid}, Cint, Cdouble, Ref{Cdouble}, Ref{Cint}),
+                A.h, niter, tol, s, it))
+    chat && it[] < niter && @info "Power iteration converged at iter $(it[])"
+    return nothing, s[]
+end
+
+# ---------------------------------------------------------------- patches + SVST (src/recon.jl)
+function patch_geometry(N, ps, ss)
+    pc = zeros(Int64, 3); ns = zeros(Int64, 3); Np = Ref{Int64}(0)
+    check(ccall((:mslr_patch_geometry, LIB), Cint,
+                (Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ref{Int64}),
+                Int64[N...], Int64[ps...], Int64[ss...], pc, ns, Np))
+    return Tuple(pc), Tuple(ns), Np[]
+end
+
+function img2patches_hip(img::HipArray{ComplexF32,4}, patch_size, stride_size)
+    N = img.dims[1:3]; Nt = img.dims[4]
+    pc, _, Np = patch_geometry(N, patch_size, stride_size)
+    P = HipArray{ComplexF32}((prod(pc), Nt, Np); ctx = img.ctx)
+    check(ccall((:mslr_img2patches, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}),
+                img.ctx.h, img.ptr, Int64[N...], Nt, Int64[patch_size...], Int64[stride_size...], P.ptr))
+    return P
+end
+
+function patches2img_hip(P::HipArray{ComplexF32,3}, patch_size, stride_size, og_size)
+    Nt = P.dims[2]
+    img = HipArray{ComplexF32}((og_size..., Nt); ctx = P.ctx)
+    check(ccall((:mslr_patches2img, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}),
+                P.ctx.h, P.ptr, Int64[og_size...], Nt, Int64[patch_size...], Int64[stride_size...], img.ptr))
+    return img
+end
+
+"SVST(X, β) (src/recon.jl:183-198) in place on a device matrix; returns X."
+function SVST_hip(X::HipArray{ComplexF32,2}, β::Real)
+    b = Cdouble[β]
+    check(ccall((:mslr_svst_batched, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cdouble}, Cint),
+                X.ctx.h, X.ptr, X.dims[1], X.dims[2], 1, b, 0))
+    return X
+end
+
+"patchSVST(img, β | λs, patch_size, stride_size) (src/recon.jl:251-269)."
+function patchSVST_hip(img::HipArray{ComplexF32,4}, β, patch_size, stride_size)
+    if β isa AbstractVector
+        P = img2patches_hip(img, patch_size, stride_size)
+        b = Cdouble.(β)
+        check(ccall((:mslr_svst_batched, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cdouble}, Cint),
+                    P.ctx.h, P.ptr, P.dims[1], P.dims[2], P.dims[3], b, 1))
+        return patches2img_hip(P, patch_size, stride_size, img.dims[1:3])
+    end
+    out = HipArray{ComplexF32}(img.dims; ctx = img.ctx)
+    check(ccall((:mslr_patch_svst, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Int64, Ptr{Int64}, Ptr{Int64}, Cdouble),
+                img.ctx.h, img.ptr, out.ptr, Int64[img.dims[1:3]...], img.dims[4],
+                Int64[patch_size...], Int64[stride_size...], β))
+    return out
+end
+
+function patch_nucnorm_hip(P::HipArray{ComplexF32,3})
+    r = Ref{Cdouble}(0)
+    check(ccall((:mslr_patch_nucnorm, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Ref{Cdouble}),
+                P.ctx.h, P.ptr, P.dims[1], P.dims[2], P.dims[3], r))
+    return r[]
+end
+
+# ---------------------------------------------------------------- POGM (src/mirt_mod.jl:74-188)
+struct PogmDesc
+    N::NTuple{3,Int64}
+    Nt::Int64
+    nscales::Int64
+    patch_sizes::Ptr{Int64}
+    strides::Ptr{Int64}
+    lambdas::Ptr{Cdouble}
+    L::Cdouble
+    niter::Cint
+    mom::Cint
+    restart::Cint
+    restart_cutoff::Cdouble
+    bsig::Cdouble
+    form::Cint
+end
+
+"""
+    pogm_hip(A, ksp, L, λs, PATCH_SIZES, STRIDES; niter, form=0) -> (X, restarts)
+
+The device form of
+    pogm_mod(X0, comp_cost, dc_cost_grad, L; mom=:pogm, niter, g_prox, fun)
+as called by run_recon (scripts/reconstruct.jl:196-226), X0[..,1] = A'ksp.
+`ksp` is (K,Nc,Nt) in the reference's sample order.  form: 0 plain, 1 hipGraph.
+"""
+function pogm_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES, STRIDES;
+                  niter::Int, form::Integer = 0)
+    S = length(PATCH_SIZES)
+    ps = Int64[p[a] for p in PATCH_SIZES for a in 1:3]
+    ss = Int64[s[a] for s in STRIDES for a in 1:3]
+    yd = ksp isa HipArray ? ksp : upload(ComplexF32.(ksp); ctx = A.ctx)
+    st = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve ps ss λs begin
+        d = PogmDesc(Int64.(A.N), A.Nt, S, pointer(ps), pointer(ss), pointer(λs), L, niter, 0, 0, 0.0, 1.0, form)
+        check(ccall((:mslr_pogm_create, LIB), Cint, (Ptr{Cvoid}, Ref{PogmDesc}, Ref{Ptr{Cvoid}}), A.h, d, st))
+    end
+    try
+        check(ccall((:mslr_pogm_init, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), st[], yd.ptr, C_NULL))
+        check(ccall((:mslr_pogm_run, LIB), Cint, (Ptr{Cvoid}, Cint), st[], niter))
+        it = Ref{Cint}(0); rs = zeros(Cint, max(1, niter))
+        check(ccall((:mslr_pogm_log, LIB), Cint, (Ptr{Cvoid}, Ref{Cint}, Ptr{Cint}), st[], it, rs))
+        xp = ccall((:mslr_pogm_x, LIB), Ptr{Cvoid}, (Ptr{Cvoid},), st[])
+        X = Array{ComplexF32}(undef, A.N..., A.Nt, S)
+        check(ccall((:mslr_download, LIB), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{Cvoid}, Csize_t),
+                    A.ctx.h, X, xp, sizeof(X)))
+        return X, Bool.(rs[1:it[]])
+    finally
+        ccall((:mslr_pogm_destroy, LIB), Cint, (Ptr{Cvoid},), st[])
+    end
+end
+
+end # module MSLRHip
