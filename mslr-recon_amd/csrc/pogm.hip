@@ -133,11 +133,12 @@ static void fill_zero(mslr_ctx_s *ctx, c64 *x, int64_t n) {
 // patchSVST for one scale: out <- patches2img(SVST(img2patches(img))).
 static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *out,
                        const double *beta_dev, int *err) {
-    c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, (size_t)std::max<int64_t>(1, g.p * g.Nt * g.Np) *
-                                                     sizeof(c64));
+    const size_t bytes = (size_t)std::max<int64_t>(1, g.p * g.Nt * g.Np) * sizeof(c64);
+    c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, bytes);
+    c64 *Y = (c64 *)ctx->scratch_get(SLOT_PATCH_OUT, bytes);
     launch_img2patches(ctx, g, img, P, 0, g.Np);
-    svst_batched(ctx, P, g.p, g.Nt, g.Np, beta_dev, 0, err);
-    launch_patches2img(ctx, g, P, out);
+    svst_batched(ctx, P, Y, g.p, g.Nt, g.Np, beta_dev, 0, err);
+    launch_patches2img(ctx, g, Y, out);
 }
 
 }  // namespace mslr

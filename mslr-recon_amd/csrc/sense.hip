@@ -13,8 +13,10 @@
 // -> conj-coil combine) and is the exact conjugate transpose of the forward.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "fft.hpp"
+#include "fft_fixed.hpp"
 #include "reduce.hpp"
 #include "sense.hpp"
 
@@ -26,13 +28,34 @@ constexpr int MAXE_YZ = 16;  // elements per thread in the yz-plane passes
 constexpr int NTYZ_MAX = 512;  // threads of the yz-plane passes (at most)
 constexpr int MAX_NYZ = MAXE_YZ * NTYZ_MAX;
 
+// ------------------------------------------------------------------ FFT policies
+// A policy transforms `nlines` lines in LDS: runtime lengths (any 2/3/5 length)
+// or one compile-time length.
+template <class T, int MAXE>
+struct FftRt {
+    FftDesc d;
+    const C2<T> *tw;
+    int sgn;
+    __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride,
+                                               int estride) const {
+        fft_lines<T, MAXE>(buf, d, nlines, lstride, estride, tw, sgn);
+    }
+};
+template <class T, int N, int SGN, int MAXLINES, int NT>
+struct FftFix {
+    const C2<T> *tw;
+    __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride,
+                                               int estride) const {
+        fix_fft<T, N, SGN, MAXLINES, NT>(buf, nlines, lstride, estride, tw);
+    }
+};
+
 // ------------------------------------------------------------------ kernels
 
-template <class T>
+template <class T, class FX>
 __global__ __launch_bounds__(NTX) void k_fwd_x(const C2<T> *__restrict__ X, int nscales,
                                                int64_t sstride, const C2<T> *__restrict__ smaps,
-                                               int Nc, int64_t V, int Nx, int NYZ, int LB,
-                                               FftDesc dx, const C2<T> *__restrict__ twx, int sgn,
+                                               int Nc, int64_t V, int Nx, int NYZ, int LB, FX fx,
                                                C2<T> *__restrict__ Tb) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
@@ -61,7 +84,7 @@ __global__ __launch_bounds__(NTX) void k_fwd_x(const C2<T> *__restrict__ X, int 
             if (e < nel) buf[e] = s[q] * sm[e];
         }
         __syncthreads();
-        fft_lines<T, MAXE_X>(buf, dx, nl, Nx, 1, twx, sgn);
+        fx(buf, nl, Nx, 1);
         C2<T> *out = Tb + ((int64_t)f * Nc + c) * V + l0;
         for (int idx = tid; idx < nel; idx += NTX) {
             const int kx = idx / nl;
@@ -72,10 +95,9 @@ __global__ __launch_bounds__(NTX) void k_fwd_x(const C2<T> *__restrict__ X, int 
     }
 }
 
-template <class T>
+template <class T, class FY, class FZ>
 __global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
-    const C2<T> *__restrict__ Tb, int Nc, int Nx, int Ny, int Nz, FftDesc dy, FftDesc dz,
-    const C2<T> *__restrict__ twy, const C2<T> *__restrict__ twz, int sgn,
+    const C2<T> *__restrict__ Tb, int Nc, int Nx, int Ny, int Nz, FY fy, FZ fz,
     const int *__restrict__ ptr, const int *__restrict__ off, const int *__restrict__ perm,
     int64_t K, const C2<T> *__restrict__ phx, const C2<T> *__restrict__ phy,
     const C2<T> *__restrict__ phz, T scale, const C2<T> *__restrict__ ysub,
@@ -89,8 +111,8 @@ __global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
     const C2<T> *src = Tb + (((int64_t)f * Nc + c) * Nx + kx) * NYZ;
     for (int i = tid; i < NYZ; i += nthr) buf[i] = src[i];
     __syncthreads();
-    fft_lines<T, MAXE_YZ>(buf, dy, Nz, Ny, 1, twy, sgn);
-    fft_lines<T, MAXE_YZ>(buf, dz, Ny, 1, Ny, twz, sgn);
+    fy(buf, Nz, Ny, 1);
+    fz(buf, Ny, 1, Ny);
     const int *pf = ptr + (int64_t)f * (Nx + 1);
     const int j0 = pf[kx], j1 = pf[kx + 1];
     const int *of = off + (int64_t)f * K;
@@ -115,10 +137,9 @@ __global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
     }
 }
 
-template <class T>
+template <class T, class FY, class FZ>
 __global__ __launch_bounds__(NTYZ_MAX) void k_adj_yz(
-    const C2<T> *__restrict__ yin, int Nc, int Nx, int Ny, int Nz, FftDesc dy, FftDesc dz,
-    const C2<T> *__restrict__ twy, const C2<T> *__restrict__ twz, int sgn,
+    const C2<T> *__restrict__ yin, int Nc, int Nx, int Ny, int Nz, FY fy, FZ fz,
     const int *__restrict__ ptr, const int *__restrict__ off, const int *__restrict__ perm,
     int64_t K, const C2<T> *__restrict__ phx, const C2<T> *__restrict__ phy,
     const C2<T> *__restrict__ phz, C2<T> *__restrict__ Tb) {
@@ -143,17 +164,16 @@ __global__ __launch_bounds__(NTYZ_MAX) void k_adj_yz(
         buf[o] = cmulc(ph, yin[ibase + (pm ? pm[j] : j)]);
     }
     __syncthreads();
-    fft_lines<T, MAXE_YZ>(buf, dz, Ny, 1, Ny, twz, sgn);
-    fft_lines<T, MAXE_YZ>(buf, dy, Nz, Ny, 1, twy, sgn);
+    fz(buf, Ny, 1, Ny);
+    fy(buf, Nz, Ny, 1);
     C2<T> *dst = Tb + (((int64_t)f * Nc + c) * Nx + kx) * NYZ;
     for (int i = tid; i < NYZ; i += nthr) dst[i] = buf[i];
 }
 
-template <class T>
+template <class T, class FX>
 __global__ __launch_bounds__(NTX) void k_adj_x(const C2<T> *__restrict__ Tb,
                                                const C2<T> *__restrict__ smaps, int Nc, int64_t V,
-                                               int Nx, int NYZ, int LB, FftDesc dx,
-                                               const C2<T> *__restrict__ twx, int sgn, T scale,
+                                               int Nx, int NYZ, int LB, FX fx, T scale,
                                                C2<T> *__restrict__ xout) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
@@ -173,7 +193,7 @@ __global__ __launch_bounds__(NTX) void k_adj_x(const C2<T> *__restrict__ Tb,
             buf[l * Nx + kx] = src[(int64_t)kx * NYZ + l];
         }
         __syncthreads();
-        fft_lines<T, MAXE_X>(buf, dx, nl, Nx, 1, twx, sgn);
+        fx(buf, nl, Nx, 1);
         const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
 #pragma unroll
         for (int q = 0; q < MAXE_X; ++q) {
@@ -220,20 +240,13 @@ struct Dev<double> {
     static c128 *ph(mslr_sense_s *P, int a) { return P->ph64[a]; }
 };
 
-template <class T>
-static void set_lds_limits() {
-    static bool done = false;
-    if (done) return;
-    const int lim = 160 * 1024;
-    HIP_CHECK(hipFuncSetAttribute((const void *)k_fwd_x<T>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIP_CHECK(hipFuncSetAttribute((const void *)k_adj_x<T>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIP_CHECK(hipFuncSetAttribute((const void *)k_fwd_yz<T>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lim - 256));
-    HIP_CHECK(hipFuncSetAttribute((const void *)k_adj_yz<T>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    done = true;
+// Kernels with more than 64 KB of dynamic LDS need the limit raised once.
+static void allow_lds(const void *fn) {
+    static std::vector<const void *> done;
+    for (const void *d : done)
+        if (d == fn) return;
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256));
+    done.push_back(fn);
 }
 
 template <class T>
@@ -253,6 +266,161 @@ static void check_lds(mslr_sense_s *P) {
                  "Ny*Nz plane does not fit in LDS at this precision");
 }
 
+// ---- launch wrappers (one chunk of nf frames; ta = absolute first frame)
+template <class T, class FX>
+static void launch_fwd_x(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride, int nf,
+                         C2<T> *Tb, FX fx) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int LB = P->LBx, Nx = P->N[0];
+    allow_lds((const void *)k_fwd_x<T, FX>);
+    dim3 gx((unsigned)cdiv(P->NYZ, LB), (unsigned)nf);
+    MSLR_PROF(ctx, "fwd_x");
+    k_fwd_x<T, FX><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
+        X, nscales, sstride, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB, fx, Tb);
+    HIP_LAUNCH_CHECK();
+}
+
+template <class T, class FY, class FZ>
+static void launch_fwd_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *ysub, C2<T> *out,
+                          bool ref_order, double *partial, const C2<T> *Tb, FY fy, FZ fz, int nt) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int Nx = P->N[0];
+    allow_lds((const void *)k_fwd_yz<T, FY, FZ>);
+    dim3 gyz((unsigned)Nx, (unsigned)P->Nc, (unsigned)nf);
+    MSLR_PROF(ctx, "fwd_yz");
+    k_fwd_yz<T, FY, FZ><<<gyz, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
+        Tb, P->Nc, Nx, P->N[1], P->N[2], fy, fz, P->ptr + ta * (Nx + 1), P->off + ta * P->K,
+        ref_order ? P->perm + ta * P->K : nullptr, P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1),
+        Dev<T>::ph(P, 2), (T)P->scale, ysub, out, partial);
+    HIP_LAUNCH_CHECK();
+}
+
+template <class T, class FY, class FZ>
+static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, bool ref_order,
+                          C2<T> *Tb, FY fy, FZ fz, int nt) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int Nx = P->N[0];
+    allow_lds((const void *)k_adj_yz<T, FY, FZ>);
+    dim3 gyz((unsigned)Nx, (unsigned)P->Nc, (unsigned)nf);
+    MSLR_PROF(ctx, "adj_yz");
+    k_adj_yz<T, FY, FZ><<<gyz, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
+        y, P->Nc, Nx, P->N[1], P->N[2], fy, fz, P->ptr + ta * (Nx + 1), P->off + ta * P->K,
+        ref_order ? P->perm + ta * P->K : nullptr, P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1),
+        Dev<T>::ph(P, 2), Tb);
+    HIP_LAUNCH_CHECK();
+}
+
+template <class T, class FX>
+static void launch_adj_x(mslr_sense_s *P, int nf, const C2<T> *Tb, C2<T> *xout, FX fx) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int LB = P->LBx, Nx = P->N[0];
+    allow_lds((const void *)k_adj_x<T, FX>);
+    dim3 gx((unsigned)cdiv(P->NYZ, LB), (unsigned)nf);
+    MSLR_PROF(ctx, "adj_x");
+    k_adj_x<T, FX><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
+        Tb, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB, fx, (T)P->scale, xout);
+    HIP_LAUNCH_CHECK();
+}
+
+// ---- dispatch: compile-time FFT lengths for the production / test sizes in
+// FP32 with the default exponent sign, runtime lengths otherwise.
+constexpr int fix_nt_yz(int ny, int nz) { return ny * nz > 2048 ? 512 : 256; }
+
+template <int NX, int SGN>
+using FixX = FftFix<float, NX, SGN, (MAXE_X * NTX) / NX, NTX>;
+template <int NY, int NZ, int SGN>
+using FixY = FftFix<float, NY, SGN, NZ, fix_nt_yz(NY, NZ)>;
+template <int NY, int NZ, int SGN>
+using FixZ = FftFix<float, NZ, SGN, NY, fix_nt_yz(NY, NZ)>;
+
+#define MSLR_X_SIZES(M) M(90) M(64) M(32) M(12)
+#define MSLR_YZ_SIZES(M) M(90, 60) M(64, 40) M(32, 24) M(12, 10)
+
+template <class T>
+static void fwd_x_any(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride, int nf,
+                      C2<T> *Tb) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (P->sgn == -1 && P->use_fixed) {
+            switch (P->N[0]) {
+#define CASE_X(n)                                                              \
+    case n:                                                                    \
+        launch_fwd_x<float>(P, X, nscales, sstride, nf, Tb, FixX<n, -1>{P->tw32[0]}); \
+        return;
+                MSLR_X_SIZES(CASE_X)
+#undef CASE_X
+                default: break;
+            }
+        }
+    }
+    launch_fwd_x<T>(P, X, nscales, sstride, nf, Tb,
+                    FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), P->sgn});
+}
+
+template <class T>
+static void fwd_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *ysub, C2<T> *out,
+                       bool ref_order, double *partial, const C2<T> *Tb) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (P->sgn == -1 && P->use_fixed) {
+            const int key = P->N[1] * 10000 + P->N[2];
+            switch (key) {
+#define CASE_YZ(ny, nz)                                                                   \
+    case ny * 10000 + nz:                                                                 \
+        launch_fwd_yz<float>(P, ta, nf, ysub, out, ref_order, partial, Tb,                \
+                             FixY<ny, nz, -1>{P->tw32[1]}, FixZ<ny, nz, -1>{P->tw32[2]},   \
+                             fix_nt_yz(ny, nz));                                          \
+        return;
+                MSLR_YZ_SIZES(CASE_YZ)
+#undef CASE_YZ
+                default: break;
+            }
+        }
+    }
+    launch_fwd_yz<T>(P, ta, nf, ysub, out, ref_order, partial, Tb,
+                     FftRt<T, MAXE_YZ>{P->d[1], Dev<T>::tw(P, 1), P->sgn},
+                     FftRt<T, MAXE_YZ>{P->d[2], Dev<T>::tw(P, 2), P->sgn}, P->nt_yz);
+}
+
+template <class T>
+static void adj_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, bool ref_order,
+                       C2<T> *Tb) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (P->sgn == -1 && P->use_fixed) {
+            const int key = P->N[1] * 10000 + P->N[2];
+            switch (key) {
+#define CASE_YZ(ny, nz)                                                                   \
+    case ny * 10000 + nz:                                                                 \
+        launch_adj_yz<float>(P, ta, nf, y, ref_order, Tb, FixY<ny, nz, 1>{P->tw32[1]},    \
+                             FixZ<ny, nz, 1>{P->tw32[2]}, fix_nt_yz(ny, nz));              \
+        return;
+                MSLR_YZ_SIZES(CASE_YZ)
+#undef CASE_YZ
+                default: break;
+            }
+        }
+    }
+    launch_adj_yz<T>(P, ta, nf, y, ref_order, Tb,
+                     FftRt<T, MAXE_YZ>{P->d[1], Dev<T>::tw(P, 1), -P->sgn},
+                     FftRt<T, MAXE_YZ>{P->d[2], Dev<T>::tw(P, 2), -P->sgn}, P->nt_yz);
+}
+
+template <class T>
+static void adj_x_any(mslr_sense_s *P, int nf, const C2<T> *Tb, C2<T> *xout) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (P->sgn == -1 && P->use_fixed) {
+            switch (P->N[0]) {
+#define CASE_X(n)                                                 \
+    case n:                                                       \
+        launch_adj_x<float>(P, nf, Tb, xout, FixX<n, 1>{P->tw32[0]}); \
+        return;
+                MSLR_X_SIZES(CASE_X)
+#undef CASE_X
+                default: break;
+            }
+        }
+    }
+    launch_adj_x<T>(P, nf, Tb, xout, FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), -P->sgn});
+}
+
 // Forward over frames [t0, t0+nframes) of the plan; X holds local frames.
 template <class T>
 static void run_forward(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride,
@@ -260,34 +428,14 @@ static void run_forward(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t ss
                         bool ref_order, double *partial) {
     mslr_ctx_s *ctx = P->ctx;
     check_lds<T>(P);
-    set_lds_limits<T>();
-    const int Nx = P->N[0], Ny = P->N[1], Nz = P->N[2];
     const int64_t F = frames_per_chunk<T>(P, nframes);
     C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, (size_t)F * P->Nc * P->V * sizeof(C2<T>));
-    const int LB = P->LBx;
-    const T scale = (T)P->scale;
     for (int64_t f0 = 0; f0 < nframes; f0 += F) {
         const int nf = (int)std::min<int64_t>(F, nframes - f0);
-        dim3 gx((unsigned)cdiv(P->NYZ, LB), (unsigned)nf);
-        {
-            MSLR_PROF(ctx, "fwd_x");
-            k_fwd_x<T><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
-                X + f0 * P->V, nscales, sstride, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB,
-                P->d[0], Dev<T>::tw(P, 0), P->sgn, Tb);
-            HIP_LAUNCH_CHECK();
-        }
-        const int64_t ta = t0 + f0;
-        dim3 gyz((unsigned)Nx, (unsigned)P->Nc, (unsigned)nf);
-        {
-            MSLR_PROF(ctx, "fwd_yz");
-            k_fwd_yz<T><<<gyz, P->nt_yz, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
-                Tb, P->Nc, Nx, Ny, Nz, P->d[1], P->d[2], Dev<T>::tw(P, 1), Dev<T>::tw(P, 2), P->sgn,
-                P->ptr + ta * (Nx + 1), P->off + ta * P->K, ref_order ? P->perm + ta * P->K : nullptr,
-                P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1), Dev<T>::ph(P, 2), scale,
-                ysub ? ysub + f0 * P->Nc * P->K : nullptr, out + f0 * P->Nc * P->K,
-                partial ? partial + f0 * P->Nc * Nx : nullptr);
-            HIP_LAUNCH_CHECK();
-        }
+        fwd_x_any<T>(P, X + f0 * P->V, nscales, sstride, nf, Tb);
+        fwd_yz_any<T>(P, t0 + f0, nf, ysub ? ysub + f0 * P->Nc * P->K : nullptr,
+                      out + f0 * P->Nc * P->K, ref_order,
+                      partial ? partial + f0 * P->Nc * P->N[0] : nullptr, Tb);
     }
 }
 
@@ -296,33 +444,12 @@ static void run_adjoint(mslr_sense_s *P, const C2<T> *y, int64_t t0, int64_t nfr
                         C2<T> *xout, bool ref_order) {
     mslr_ctx_s *ctx = P->ctx;
     check_lds<T>(P);
-    set_lds_limits<T>();
-    const int Nx = P->N[0], Ny = P->N[1], Nz = P->N[2];
     const int64_t F = frames_per_chunk<T>(P, nframes);
     C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, (size_t)F * P->Nc * P->V * sizeof(C2<T>));
-    const int LB = P->LBx;
-    const T scale = (T)P->scale;
     for (int64_t f0 = 0; f0 < nframes; f0 += F) {
         const int nf = (int)std::min<int64_t>(F, nframes - f0);
-        const int64_t ta = t0 + f0;
-        dim3 gyz((unsigned)Nx, (unsigned)P->Nc, (unsigned)nf);
-        {
-            MSLR_PROF(ctx, "adj_yz");
-            k_adj_yz<T><<<gyz, P->nt_yz, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
-                y + f0 * P->Nc * P->K, P->Nc, Nx, Ny, Nz, P->d[1], P->d[2], Dev<T>::tw(P, 1),
-                Dev<T>::tw(P, 2), -P->sgn, P->ptr + ta * (Nx + 1), P->off + ta * P->K,
-                ref_order ? P->perm + ta * P->K : nullptr, P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1),
-                Dev<T>::ph(P, 2), Tb);
-            HIP_LAUNCH_CHECK();
-        }
-        dim3 gx((unsigned)cdiv(P->NYZ, LB), (unsigned)nf);
-        {
-            MSLR_PROF(ctx, "adj_x");
-            k_adj_x<T><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
-                Tb, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB, P->d[0], Dev<T>::tw(P, 0),
-                -P->sgn, scale, xout + f0 * P->V);
-            HIP_LAUNCH_CHECK();
-        }
+        adj_yz_any<T>(P, t0 + f0, nf, y + f0 * P->Nc * P->K, ref_order, Tb);
+        adj_x_any<T>(P, nf, Tb, xout + f0 * P->V);
     }
 }
 
@@ -429,6 +556,7 @@ mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, in
             nt = (int)(cdiv(nt, 64) * 64);
             P->nt_yz = std::max(64, std::min(NTYZ_MAX, nt));
             P->chunk_bytes = (size_t)512 << 20;
+            P->use_fixed = getenv("MSLR_FFT_GENERIC") == nullptr;
             if (const char *e = getenv("MSLR_FFT_CHUNK_MB")) {
                 long mb = atol(e);
                 if (mb > 0) P->chunk_bytes = (size_t)mb << 20;

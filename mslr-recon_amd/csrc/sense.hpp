@@ -18,6 +18,7 @@ struct mslr_sense_s {
     int LBx = 1;      // yz-lines per workgroup in the x-axis passes
     int nt_yz = 64;   // threads per workgroup in the yz-plane passes
     size_t chunk_bytes = 0;
+    bool use_fixed = true;  // compile-time FFT lengths where available
 
     // device arrays
     mslr::c64 *smaps32 = nullptr;   // smaps * pre-phase, (V, Nc)

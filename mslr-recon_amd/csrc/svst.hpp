@@ -5,12 +5,12 @@
 
 namespace mslr {
 
-// SVST of Np matrices X_i (p x Nt, ComplexF32, consecutive) in place:
+// SVST of Np matrices X_i (p x Nt, ComplexF32, consecutive), Pin -> Pout (distinct):
 //   X_i <- U diag(max(s - beta_i, 0)) V^H.
 // beta_dev: device doubles, beta_i = beta_dev[per_patch ? i : 0] (rounded to
 // Float32 inside, src/recon.jl:195).  err_dev (device int, nullable) is set to
 // 1 when a Gram matrix holds non-finite values.
-void svst_batched(mslr_ctx_s *ctx, c64 *P, int64_t p, int64_t Nt, int64_t Np,
+void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
                   const double *beta_dev, int per_patch, int *err_dev);
 
 // Sum of singular values of each matrix, accumulated into *out_dev (device double).

@@ -156,3 +156,21 @@ def test_poweriter_matches_oracle(ctx, rng):
                                  n, niter=30, tol=1e-6)
     assert it == ito
     assert abs(s1 - so) / so < 1e-10
+
+
+@pytest.mark.parametrize("N", [(90, 90, 60), (32, 32, 24), (12, 12, 10)])
+def test_fixed_and_runtime_fft_paths_agree(ctx, rng, N, monkeypatch):
+    """The compile-time-length kernels and the runtime-length kernels compute the same operator."""
+    from mslr_hip.sense import SenseOp
+    Nc, Nt = 2, 2
+    smaps, mask = _problem(rng, N, Nc, Nt)
+    x = crandn(rng, (*N, Nt))
+    A1 = SenseOp(mask, smaps, ctx=ctx)
+    y1 = A1.forward(x).to_host()
+    x1 = A1.adjoint(y1).to_host()
+    monkeypatch.setenv("MSLR_FFT_GENERIC", "1")
+    A2 = SenseOp(mask, smaps, ctx=ctx)
+    y2 = A2.forward(x).to_host()
+    x2 = A2.adjoint(y2).to_host()
+    assert rel_err(y1, y2) < 2e-6
+    assert rel_err(x1, x2) < 2e-6
