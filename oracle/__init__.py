@@ -7,7 +7,7 @@ CPU fallback.
 
 Each function cites the reference file:line it restates.  Pins (what checks each
 function against something other than itself) are listed in mslr_ref.PINS and
-exercised by tests/test_oracle_*.py.
+exercised by tests/test_oracle.py.
 """
 from .mslr_ref import *  # noqa: F401,F403
 from .mslr_ref import _gr_restart, _patch_starts, _workers  # noqa: F401,E402
