@@ -220,7 +220,13 @@ def main() -> int:
         chunk = max(1, int((512 << 20) // (Nc * int(np.prod(N)) * 8)))
         by = kernel_bytes_per_iter(cfg, A.K, chunk)
         cand = {k: v for k, v in kern.items() if k in by}
-        dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
+        # per-scale tags ("tridiag@s2") summed per kernel for the dominant-kernel figure
+        per_kernel = {}
+        for kname, (cnt, ms) in kern.items():
+            base = kname.split("@")[0]
+            c0, m0 = per_kernel.get(base, (0, 0.0))
+            per_kernel[base] = (c0 + cnt, m0 + ms)
+        dom = max(per_kernel.items(), key=lambda kv: kv[1][1])[0]
         hb = max(cand.items(), key=lambda kv: kv[1][1])[0] if cand else None
         if hb is not None:
             cnt, ms = kern[hb]
@@ -230,7 +236,7 @@ def main() -> int:
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                     "bytes_per_launch": int(per_launch), "avg_launch_ms": round(ms / cnt, 4),
                     "dominant_kernel": dom,
-                    "dominant_share": round(kern[dom][1] / (elapsed * 1e3), 3)}
+                    "dominant_share": round(per_kernel[dom][1] / (elapsed * 1e3), 3)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -257,7 +263,7 @@ def main() -> int:
                    "form": args.form, "parallelism": "single-gpu", "restarts_timed": sum(restarts[W:])},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels_ms": {k: round(v[1], 3) for k, v in kern.items()},
+        "kernels_ms_per_iter": {k: round(v[1] / K, 3) for k, v in kern.items()},
     }
     print(json.dumps(line), flush=True)
     return 0

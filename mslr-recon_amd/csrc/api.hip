@@ -21,12 +21,13 @@ void mslr_ctx_s::activate() const {
 }
 
 void *mslr_ctx_s::scratch_get(size_t slot, size_t bytes) {
+    slot += slot_base;
     if (slot >= scratch.size()) scratch.resize(slot + 1);
     Scratch &s = scratch[slot];
     if (bytes == 0) bytes = 256;
     if (s.bytes >= bytes) return s.ptr;
-    // Growing: work queued on the old buffer must finish before it is freed.
-    HIP_CHECK(hipStreamSynchronize(stream));
+    // Growing: work queued on the old buffer (on any stream) must finish first.
+    HIP_CHECK(hipDeviceSynchronize());
     if (s.ptr) HIP_CHECK(hipFree(s.ptr));
     s.ptr = nullptr;
     s.bytes = 0;

@@ -107,7 +107,8 @@ struct mslr_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 0;
-    std::vector<mslr::Scratch> scratch;  // indexed slots, see ctx_scratch()
+    std::vector<mslr::Scratch> scratch;  // indexed slots, see ScratchSlot
+    size_t slot_base = 0;                // added to every slot (per-stream scratch sets)
 
     // Kernel timing with HIP events on `stream` (enabled by mslr_prof_enable).
     struct ProfRec {
@@ -115,6 +116,7 @@ struct mslr_ctx_s {
         hipEvent_t a, b;
     };
     bool prof = false;
+    const char *prof_suffix = "";  // appended to kernel names (e.g. the scale being processed)
     std::vector<std::string> prof_tags;
     std::vector<ProfRec> prof_recs;
     std::vector<hipEvent_t> prof_pool;
@@ -133,7 +135,7 @@ struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     ProfScope(mslr_ctx_s *c, const char *name) : ctx(c) {
         if (!ctx->prof) return;
-        tag = ctx->prof_tag(name);
+        tag = ctx->prof_tag((std::string(name) + ctx->prof_suffix).c_str());
         a = ctx->prof_event();
         b = ctx->prof_event();
         hipEventRecord(a, ctx->stream);

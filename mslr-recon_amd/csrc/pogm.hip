@@ -162,6 +162,10 @@ struct mslr_pogm_s {
     bool initialised = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // batched forms: one stream per scale for the prox (fork / join with events)
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> ev_join;
+    hipEvent_t ev_fork = nullptr;
     std::vector<void *> allocs;
 
     void *alloc(size_t b) {
@@ -176,6 +180,9 @@ static void pogm_free(mslr_pogm_s *st) {
     if (!st) return;
     if (st->exec) hipGraphExecDestroy(st->exec);
     if (st->graph) hipGraphDestroy(st->graph);
+    for (auto s2 : st->aux) hipStreamDestroy(s2);
+    for (auto e : st->ev_join) hipEventDestroy(e);
+    if (st->ev_fork) hipEventDestroy(st->ev_fork);
     for (void *p : st->allocs) hipFree(p);
     delete st;
 }
@@ -202,10 +209,39 @@ static void pogm_iteration(mslr_pogm_s *st) {
         HIP_LAUNCH_CHECK();
     }
     int *err = (int *)((char *)st->dev + offsetof(PogmDev, err));
-    for (int64_t k = 0; k < st->S; ++k) {
-        c64 *Xk = st->X + k * n;
-        prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err);
+    static const char *sfx[] = {"@s1", "@s2", "@s3", "@s4", "@s5", "@s6", "@s7", "@s8"};
+    if (st->aux.empty()) {
+        for (int64_t k = 0; k < st->S; ++k) {
+            c64 *Xk = st->X + k * n;
+            ctx->prof_suffix = k < 8 ? sfx[k] : "@s9+";
+            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err);
+        }
+        ctx->prof_suffix = "";
+        return;
     }
+    // Scales are independent: each runs on its own stream with its own scratch.
+    hipStream_t main = ctx->stream;
+    HIP_CHECK(hipEventRecord(st->ev_fork, main));
+    try {
+        for (int64_t k = 0; k < st->S; ++k) {
+            c64 *Xk = st->X + k * n;
+            HIP_CHECK(hipStreamWaitEvent(st->aux[k], st->ev_fork, 0));
+            ctx->stream = st->aux[k];
+            ctx->slot_base = (size_t)SLOT_COUNT * (size_t)(k + 1);
+            ctx->prof_suffix = k < 8 ? sfx[k] : "@s9+";
+            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err);
+            HIP_CHECK(hipEventRecord(st->ev_join[k], st->aux[k]));
+        }
+    } catch (...) {
+        ctx->stream = main;
+        ctx->slot_base = 0;
+        ctx->prof_suffix = "";
+        throw;
+    }
+    ctx->stream = main;
+    ctx->slot_base = 0;
+    ctx->prof_suffix = "";
+    for (int64_t k = 0; k < st->S; ++k) HIP_CHECK(hipStreamWaitEvent(main, st->ev_join[k], 0));
 }
 
 extern "C" {
@@ -276,6 +312,17 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
             h.restart = d.restart;
             HIP_CHECK(hipMemcpy(st->dev, &h, sizeof(h), hipMemcpyHostToDevice));
             HIP_CHECK(hipMemset(st->log_dev, 0, sizeof(int) * std::max(1, d.niter)));
+            if (d.form != MSLR_FORM_PLAIN && d.nscales > 1) {
+                for (int64_t k = 0; k < d.nscales; ++k) {
+                    hipStream_t s2 = nullptr;
+                    HIP_CHECK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+                    st->aux.push_back(s2);
+                    hipEvent_t e = nullptr;
+                    HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                    st->ev_join.push_back(e);
+                }
+                HIP_CHECK(hipEventCreateWithFlags(&st->ev_fork, hipEventDisableTiming));
+            }
             *out = st;
         } catch (...) {
             pogm_free(st);

@@ -277,10 +277,23 @@ __global__ __launch_bounds__(TNT) void k_tridiag_blk(c128 *__restrict__ Gall, in
             }
             const c128 *A22 = A + (j + 1) + (int64_t)m * (j + 1);
             for (int i = tid; i < n; i += TNT) {
+                // 8 independent loads in flight per thread before their FMAs
                 double ar = 0.0, ai = 0.0;
-#pragma unroll 4
-                for (int k = 0; k < n; ++k) {
-                    const c128 a = A22[i + (int64_t)m * k];
+                const c128 *row = A22 + i;
+                int k = 0;
+                for (; k + 8 <= n; k += 8) {
+                    c128 a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = row[(int64_t)m * (k + u)];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const c128 b = v[k + u];
+                        ar += a[u].re * b.re - a[u].im * b.im;
+                        ai += a[u].re * b.im + a[u].im * b.re;
+                    }
+                }
+                for (; k < n; ++k) {
+                    const c128 a = row[(int64_t)m * k];
                     const c128 b = v[k];
                     ar += a.re * b.re - a.im * b.im;
                     ai += a.re * b.im + a.im * b.re;
