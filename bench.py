@@ -144,6 +144,22 @@ def cpu_baseline(cfg, pb, y_norm, smaps, budget_s=30.0):
     return 1.0 / tot, cores, sample
 
 
+# --------------------------------------------------------------------------- counter calibration
+def copy_calibration(mib: int) -> int:
+    """A plain elementwise copy (dst = src * 1) of a known size: each launch reads and
+    writes exactly mib MiB, so its FETCH_SIZE / WRITE_SIZE give the counters' scale."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "launches": 5, "bytes_read_per_launch": n * 4,
+                      "bytes_written_per_launch": n * 4}), flush=True)
+    return 0
+
+
 # --------------------------------------------------------------------------- main
 def main() -> int:
     ap = argparse.ArgumentParser()
@@ -154,7 +170,12 @@ def main() -> int:
     ap.add_argument("--form", default="plain", choices=["plain", "batched"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-prof", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read + write) five times, to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters, and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py: this build runs the MSLR hot path on one GPU; --gpus {args.gpus} is not "
               f"supported (the path would shard frames for A/A' and space for the prox).", file=sys.stderr)

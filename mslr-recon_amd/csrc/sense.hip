@@ -33,6 +33,7 @@ constexpr int MAX_NYZ = MAXE_YZ * NTYZ_MAX;
 // or one compile-time length.
 template <class T, int MAXE>
 struct FftRt {
+    static constexpr bool kPipe = false;  // x kernels: no next-coil register prefetch
     FftDesc d;
     const C2<T> *tw;
     int sgn;
@@ -43,6 +44,7 @@ struct FftRt {
 };
 template <class T, int N, int SGN, int MAXLINES, int NT>
 struct FftFix {
+    static constexpr bool kPipe = true;   // x kernels: next-coil register prefetch
     const C2<T> *tw;
     __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride,
                                                int estride) const {
@@ -76,12 +78,39 @@ __global__ __launch_bounds__(NTX) void k_fwd_x(const C2<T> *__restrict__ X, int 
         }
         s[q] = acc;
     }
-    for (int c = 0; c < Nc; ++c) {
-        const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
+    // coil maps of the next coil are loaded before this coil's FFT (latency overlap)
+    constexpr int PX = FX::kPipe ? MAXE_X : 1;
+    C2<T> smn[PX];
+    if constexpr (FX::kPipe) {
+        const C2<T> *sm = smaps + (int64_t)l0 * Nx;
 #pragma unroll
-        for (int q = 0; q < MAXE_X; ++q) {
+        for (int q = 0; q < PX; ++q) {
             const int e = tid + q * NTX;
-            if (e < nel) buf[e] = s[q] * sm[e];
+            smn[q] = e < nel ? sm[e] : C2<T>{T(0), T(0)};
+        }
+    }
+    for (int c = 0; c < Nc; ++c) {
+        if constexpr (FX::kPipe) {
+#pragma unroll
+            for (int q = 0; q < PX; ++q) {
+                const int e = tid + q * NTX;
+                if (e < nel) buf[e] = s[q] * smn[q];
+            }
+            if (c + 1 < Nc) {
+                const C2<T> *sm = smaps + (int64_t)(c + 1) * V + (int64_t)l0 * Nx;
+#pragma unroll
+                for (int q = 0; q < PX; ++q) {
+                    const int e = tid + q * NTX;
+                    if (e < nel) smn[q] = sm[e];
+                }
+            }
+        } else {
+            const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
+#pragma unroll
+            for (int q = 0; q < MAXE_X; ++q) {
+                const int e = tid + q * NTX;
+                if (e < nel) buf[e] = s[q] * sm[e];
+            }
         }
         __syncthreads();
         fx(buf, nl, Nx, 1);
@@ -95,6 +124,8 @@ __global__ __launch_bounds__(NTX) void k_fwd_x(const C2<T> *__restrict__ X, int 
     }
 }
 
+// One workgroup per (kx, coil, frame): 2D FFT of one plane, then gather of the
+// sampled (ky, kz) with the post-phases, optional residual and its squared norm.
 template <class T, class FY, class FZ>
 __global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
     const C2<T> *__restrict__ Tb, int Nc, int Nx, int Ny, int Nz, FY fy, FZ fz,
@@ -108,16 +139,16 @@ __global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
     const int kx = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
     const int NYZ = Ny * Nz;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const C2<T> *src = Tb + (((int64_t)f * Nc + c) * Nx + kx) * NYZ;
-    for (int i = tid; i < NYZ; i += nthr) buf[i] = src[i];
-    __syncthreads();
-    fy(buf, Nz, Ny, 1);
-    fz(buf, Ny, 1, Ny);
     const int *pf = ptr + (int64_t)f * (Nx + 1);
     const int j0 = pf[kx], j1 = pf[kx + 1];
     const int *of = off + (int64_t)f * K;
     const int *pm = perm ? perm + (int64_t)f * K : nullptr;
     const C2<T> px = phx[kx];
+    const C2<T> *src = Tb + (((int64_t)f * Nc + c) * Nx + kx) * NYZ;
+    for (int i = tid; i < NYZ; i += nthr) buf[i] = src[i];
+    __syncthreads();
+    fy(buf, Nz, Ny, 1);
+    fz(buf, Ny, 1, Ny);
     const int64_t obase = ((int64_t)f * Nc + c) * K;
     double acc = 0.0;
     for (int j = j0 + tid; j < j1; j += nthr) {
@@ -132,11 +163,13 @@ __global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
         acc += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
     }
     if (partial) {
-        const double s = block_sum(acc, red);
-        if (tid == 0) partial[((int64_t)f * Nc + c) * Nx + kx] = s;
+        const double sres = block_sum(acc, red);
+        if (tid == 0) partial[((int64_t)f * Nc + c) * Nx + kx] = sres;
     }
 }
 
+// One workgroup per (kx, coil, frame): scatter of the sampled (ky, kz) times the
+// conjugate post-phases into a zeroed plane, then its inverse-direction 2D FFT.
 template <class T, class FY, class FZ>
 __global__ __launch_bounds__(NTYZ_MAX) void k_adj_yz(
     const C2<T> *__restrict__ yin, int Nc, int Nx, int Ny, int Nz, FY fy, FZ fz,
@@ -148,13 +181,13 @@ __global__ __launch_bounds__(NTYZ_MAX) void k_adj_yz(
     const int kx = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
     const int NYZ = Ny * Nz;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int i = tid; i < NYZ; i += nthr) buf[i] = C2<T>{T(0), T(0)};
-    __syncthreads();
     const int *pf = ptr + (int64_t)f * (Nx + 1);
     const int j0 = pf[kx], j1 = pf[kx + 1];
     const int *of = off + (int64_t)f * K;
     const int *pm = perm ? perm + (int64_t)f * K : nullptr;
     const C2<T> px = phx[kx];
+    for (int i = tid; i < NYZ; i += nthr) buf[i] = C2<T>{T(0), T(0)};
+    __syncthreads();
     const int64_t ibase = ((int64_t)f * Nc + c) * K;
     for (int j = j0 + tid; j < j1; j += nthr) {
         const int o = of[j];
@@ -185,20 +218,76 @@ __global__ __launch_bounds__(NTX) void k_adj_x(const C2<T> *__restrict__ Tb,
     C2<T> acc[MAXE_X];
 #pragma unroll
     for (int q = 0; q < MAXE_X; ++q) acc[q] = C2<T>{T(0), T(0)};
+    // plane block of the next coil (transposed read) is loaded before this coil's FFT
+    constexpr int PX = FX::kPipe ? MAXE_X : 1;
+    C2<T> tn[PX];
+    if constexpr (FX::kPipe) {
+        const C2<T> *src = Tb + (int64_t)f * Nc * V + l0;
+#pragma unroll
+        for (int q = 0; q < PX; ++q) {
+            const int idx = tid + q * NTX;
+            if (idx < nel) {
+                const int kx = idx / nl;
+                const int l = idx - kx * nl;
+                tn[q] = src[(int64_t)kx * NYZ + l];
+            }
+        }
+    }
     for (int c = 0; c < Nc; ++c) {
-        const C2<T> *src = Tb + ((int64_t)f * Nc + c) * V + l0;
-        for (int idx = tid; idx < nel; idx += NTX) {
-            const int kx = idx / nl;
-            const int l = idx - kx * nl;
-            buf[l * Nx + kx] = src[(int64_t)kx * NYZ + l];
+        if constexpr (FX::kPipe) {
+#pragma unroll
+            for (int q = 0; q < PX; ++q) {
+                const int idx = tid + q * NTX;
+                if (idx < nel) {
+                    const int kx = idx / nl;
+                    const int l = idx - kx * nl;
+                    buf[l * Nx + kx] = tn[q];
+                }
+            }
+        } else {
+            const C2<T> *src = Tb + ((int64_t)f * Nc + c) * V + l0;
+            for (int idx = tid; idx < nel; idx += NTX) {
+                const int kx = idx / nl;
+                const int l = idx - kx * nl;
+                buf[l * Nx + kx] = src[(int64_t)kx * NYZ + l];
+            }
+        }
+        C2<T> smc[PX];
+        if constexpr (FX::kPipe) {
+            const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
+#pragma unroll
+            for (int q = 0; q < PX; ++q) {
+                const int e = tid + q * NTX;
+                if (e < nel) smc[q] = sm[e];
+            }
+            if (c + 1 < Nc) {
+                const C2<T> *src = Tb + ((int64_t)f * Nc + c + 1) * V + l0;
+#pragma unroll
+                for (int q = 0; q < PX; ++q) {
+                    const int idx = tid + q * NTX;
+                    if (idx < nel) {
+                        const int kx = idx / nl;
+                        const int l = idx - kx * nl;
+                        tn[q] = src[(int64_t)kx * NYZ + l];
+                    }
+                }
+            }
         }
         __syncthreads();
         fx(buf, nl, Nx, 1);
-        const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
+        if constexpr (FX::kPipe) {
 #pragma unroll
-        for (int q = 0; q < MAXE_X; ++q) {
-            const int e = tid + q * NTX;
-            if (e < nel) acc[q] = acc[q] + cmulc(sm[e], buf[e]);
+            for (int q = 0; q < PX; ++q) {
+                const int e = tid + q * NTX;
+                if (e < nel) acc[q] = acc[q] + cmulc(smc[q], buf[e]);
+            }
+        } else {
+            const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
+#pragma unroll
+            for (int q = 0; q < MAXE_X; ++q) {
+                const int e = tid + q * NTX;
+                if (e < nel) acc[q] = acc[q] + cmulc(sm[e], buf[e]);
+            }
         }
         __syncthreads();
     }
