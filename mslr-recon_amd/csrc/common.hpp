@@ -162,7 +162,7 @@ inline mslr_ctx_s *check_ctx(mslr_ctx c) {
 
 // Scratch slots used by the library (one arena per slot per context).
 enum ScratchSlot : size_t {
-    SLOT_FFT = 0,       // SENSE intermediate (coil k-space planes)
+    SLOT_FFT = 0,       // SENSE intermediate (sampled k-space lines per coil)
     SLOT_RESID = 1,     // SENSE residual / staging
     SLOT_REDUCE = 2,    // FP64 partial sums
     SLOT_GRAM_PART = 3, // split-K partial Gram tiles (FP32)
@@ -172,7 +172,8 @@ enum ScratchSlot : size_t {
     SLOT_PATCH = 7,     // materialised patches
     SLOT_PATCH_OUT = 8, // image staging for patch operators
     SLOT_PWR = 9,       // power-iteration vectors
-    SLOT_COUNT = 10
+    SLOT_PLANES = 10,   // SENSE plane-major image frames
+    SLOT_COUNT = 11
 };
 
 }  // namespace mslr

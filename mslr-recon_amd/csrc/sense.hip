@@ -4,13 +4,17 @@
 //   y = scale * (fftshift o FFT3 o ifftshift)(S_c .* x_t) restricted to mask_t.
 // The centred DFT is written as post_phase .* DFT(pre_phase .* v): the
 // pre-phase is folded into the stored coil maps and the post-phase into the
-// k-space gather, so no shift copies are made.  Two passes per frame chunk:
-//   pass X : scale sum + coil multiply + x-axis FFT of a block of yz-lines,
-//            written as per-coil planes T[f][c][kx][y + Ny*z];
-//   pass YZ: one (kx, coil, frame) plane in LDS, y- and z-axis FFTs, gather
-//            of the sampled (ky,kz) points of that plane (+ residual).
-// The adjoint runs the mirror image (scatter -> inverse yz FFT -> inverse x FFT
-// -> conj-coil combine) and is the exact conjugate transpose of the forward.
+// k-space gather, so no shift copies are made.  The mask samples whole or
+// partial lines along kx at a subset of (ky, kz); only those lines are carried
+// between the passes.  Per frame chunk:
+//   sum_planes: scale sum, image -> plane-major [f][x][y + Ny*z];
+//   fwd_yz    : per (x, frame), for each coil: map multiply, y- and z-axis FFT
+//               of the plane in LDS, values on the sampled lines kept;
+//   fwd_x     : per block of lines: x-axis FFT, gather of the sampled kx with
+//               the post-phases (+ residual and its squared norm).
+// The adjoint runs the mirror image (scatter -> inverse x FFT -> inverse yz
+// FFT -> conj-coil combine -> usual layout) and is the exact conjugate
+// transpose of the forward.
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -33,282 +37,323 @@ constexpr int MAX_NYZ = MAXE_YZ * NTYZ_MAX;
 // or one compile-time length.
 template <class T, int MAXE>
 struct FftRt {
-    static constexpr bool kPipe = false;  // x kernels: no next-coil register prefetch
+    static constexpr bool kPipe = false;  // no per-thread register copy of a plane
+    static constexpr int kPlane = 1;
+    static constexpr int kNT = 0;
     FftDesc d;
     const C2<T> *tw;
     int sgn;
+    __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride, int estride,
+                                               const C2<T> *twp) const {
+        fft_lines<T, MAXE>(buf, d, nlines, lstride, estride, twp, sgn);
+    }
     __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride,
                                                int estride) const {
-        fft_lines<T, MAXE>(buf, d, nlines, lstride, estride, tw, sgn);
+        (*this)(buf, nlines, lstride, estride, tw);
     }
 };
 template <class T, int N, int SGN, int MAXLINES, int NT>
 struct FftFix {
-    static constexpr bool kPipe = true;   // x kernels: next-coil register prefetch
+    static constexpr bool kPipe = true;   // plane kept in registers across coils
+    static constexpr int kPlane = (N * MAXLINES + NT - 1) / NT;  // plane elements per thread
+    static constexpr int kNT = NT;                                // threads of the workgroup
     const C2<T> *tw;
+    __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride, int estride,
+                                               const C2<T> *twp) const {
+        fix_fft<T, N, SGN, MAXLINES, NT>(buf, nlines, lstride, estride, twp);
+    }
     __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride,
                                                int estride) const {
-        fix_fft<T, N, SGN, MAXLINES, NT>(buf, nlines, lstride, estride, tw);
+        (*this)(buf, nlines, lstride, estride, tw);
     }
 };
 
+// The twiddle table's address, hidden from loop-invariant code motion: in a
+// loop over coils the per-thread twiddle loads then stay inside the loop
+// instead of occupying registers across it.
+template <class T>
+__device__ __forceinline__ const C2<T> *opaque_ptr(const C2<T> *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+// Per-lane address hidden the same way.
+template <class T>
+__device__ __forceinline__ const C2<T> *opaque_lane_ptr(const C2<T> *p) {
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
 // ------------------------------------------------------------------ kernels
 
-template <class T, class FX>
-__global__ __launch_bounds__(NTX) void k_fwd_x(const C2<T> *__restrict__ X, int nscales,
-                                               int64_t sstride, const C2<T> *__restrict__ smaps,
-                                               int Nc, int64_t V, int Nx, int NYZ, int LB, FX fx,
-                                               C2<T> *__restrict__ Tb) {
+// Image frames in the usual layout (x fastest) -> plane-major layout
+// P[f][x][y + Ny*z], summing nscales images spaced sstride apart.  One
+// workgroup per (block of TB yz-lines, frame), tile transposed through LDS.
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_planes(const C2<T> *__restrict__ X, int nscales,
+                                                    int64_t sstride, int Nx, int NYZ, int TB,
+                                                    C2<T> *__restrict__ Pm) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T> *tile = reinterpret_cast<C2<T> *>(smem_raw);
+    const int f = blockIdx.y;
+    const int l0 = blockIdx.x * TB;
+    const int nl = min(TB, NYZ - l0);
+    const int64_t V = (int64_t)Nx * NYZ;
+    const C2<T> *src = X + (int64_t)f * V + (int64_t)l0 * Nx;
+    for (int e = threadIdx.x; e < nl * Nx; e += blockDim.x) {
+        C2<T> s = src[e];
+        for (int k = 1; k < nscales; ++k) s = s + src[(int64_t)k * sstride + e];
+        const int l = e / Nx;
+        const int x = e - l * Nx;
+        tile[x * (TB + 1) + l] = s;
+    }
+    __syncthreads();
+    C2<T> *dst = Pm + (int64_t)f * V + l0;
+    for (int e = threadIdx.x; e < nl * Nx; e += blockDim.x) {
+        const int x = e / nl;
+        const int l = e - x * nl;
+        dst[(int64_t)x * NYZ + l] = tile[x * (TB + 1) + l];
+    }
+}
+
+// Plane-major frames -> usual layout (the inverse of k_sum_planes, one scale).
+template <class T>
+__global__ __launch_bounds__(256) void k_planes_to_image(const C2<T> *__restrict__ Pm, int Nx,
+                                                         int NYZ, int TB, C2<T> *__restrict__ X) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T> *tile = reinterpret_cast<C2<T> *>(smem_raw);
+    const int f = blockIdx.y;
+    const int l0 = blockIdx.x * TB;
+    const int nl = min(TB, NYZ - l0);
+    const int64_t V = (int64_t)Nx * NYZ;
+    const C2<T> *src = Pm + (int64_t)f * V + l0;
+    for (int e = threadIdx.x; e < nl * Nx; e += blockDim.x) {
+        const int x = e / nl;
+        const int l = e - x * nl;
+        tile[x * (TB + 1) + l] = src[(int64_t)x * NYZ + l];
+    }
+    __syncthreads();
+    C2<T> *dst = X + (int64_t)f * V + (int64_t)l0 * Nx;
+    for (int e = threadIdx.x; e < nl * Nx; e += blockDim.x) {
+        const int l = e / Nx;
+        const int x = e - l * Nx;
+        dst[e] = tile[x * (TB + 1) + l];
+    }
+}
+
+// Sampling tables of a frame chunk (absolute frame ta + f).
+struct LineTabs {
+    const int *lptr;     // (Nt+1) first global line of each frame
+    const int *line_yz;  // (lines) ky + Ny*kz of each line, ascending within a frame
+    const int *kbeg;     // (lines) first internal sample of the line within its frame
+    const int *lk;       // (K, Nt) l*Nx + kx of sample j, l = line within frame
+};
+
+// Forward pass 1: one workgroup per (x, coil group, frame).  The summed image
+// plane x is multiplied by each coil's map, transformed along y and z in LDS,
+// and the values on the frame's sampled (ky, kz) lines are kept:
+//   T[f][c][x][l] (l = line of the frame), the only part of k-space pass 2 needs.
+// The image plane is re-read per coil (an L2 hit): holding it in registers
+// would halve the workgroups resident per CU.  Compile-time FFT lengths loop
+// over all coils, runtime lengths take one coil per workgroup.
+template <class T, class FY, class FZ>
+__global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
+    const C2<T> *__restrict__ Pm, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny,
+    int Nz, int cpw, FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz,
+    int64_t ta, C2<T> *__restrict__ Tb) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
-    const int f = blockIdx.y;
-    const int l0 = blockIdx.x * LB;
-    const int nl = min(LB, NYZ - l0);
-    const int nel = nl * Nx;
-    const int tid = threadIdx.x;
-    const int64_t base = (int64_t)f * V + (int64_t)l0 * Nx;
-    C2<T> s[MAXE_X];
-#pragma unroll
-    for (int q = 0; q < MAXE_X; ++q) {
-        const int e = tid + q * NTX;
-        C2<T> acc = {T(0), T(0)};
-        if (e < nel) {
-            acc = X[base + e];
-            for (int k = 1; k < nscales; ++k) acc = acc + X[(int64_t)k * sstride + base + e];
-        }
-        s[q] = acc;
-    }
-    // coil maps of the next coil are loaded before this coil's FFT (latency overlap)
-    constexpr int PX = FX::kPipe ? MAXE_X : 1;
-    C2<T> smn[PX];
-    if constexpr (FX::kPipe) {
-        const C2<T> *sm = smaps + (int64_t)l0 * Nx;
-#pragma unroll
-        for (int q = 0; q < PX; ++q) {
-            const int e = tid + q * NTX;
-            smn[q] = e < nel ? sm[e] : C2<T>{T(0), T(0)};
-        }
-    }
-    for (int c = 0; c < Nc; ++c) {
-        if constexpr (FX::kPipe) {
-#pragma unroll
-            for (int q = 0; q < PX; ++q) {
-                const int e = tid + q * NTX;
-                if (e < nel) buf[e] = s[q] * smn[q];
-            }
-            if (c + 1 < Nc) {
-                const C2<T> *sm = smaps + (int64_t)(c + 1) * V + (int64_t)l0 * Nx;
-#pragma unroll
-                for (int q = 0; q < PX; ++q) {
-                    const int e = tid + q * NTX;
-                    if (e < nel) smn[q] = sm[e];
-                }
-            }
-        } else {
-            const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
-#pragma unroll
-            for (int q = 0; q < MAXE_X; ++q) {
-                const int e = tid + q * NTX;
-                if (e < nel) buf[e] = s[q] * sm[e];
-            }
-        }
+    const int x = blockIdx.x, f = blockIdx.z;
+    const int NYZ = Ny * Nz;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int g0 = lptr[ta + f];
+    const int Lf = lptr[ta + f + 1] - g0;
+    const int *lines = line_yz + g0;
+    C2<T> *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
+    const C2<T> *pl = Pm + ((int64_t)f * Nx + x) * NYZ;
+    // runtime lengths: one coil, no loop (a loop around the runtime FFT spills)
+    const int c0 = FY::kPipe ? (int)blockIdx.y * cpw : (int)blockIdx.y;
+    const int c1 = FY::kPipe ? min(Nc, c0 + cpw) : c0 + 1;
+#pragma unroll 1
+    for (int c = c0; c < c1; ++c) {
+        const C2<T> *twy = FY::kPipe ? opaque_ptr(fy.tw) : fy.tw;
+        const C2<T> *twz = FY::kPipe ? opaque_ptr(fz.tw) : fz.tw;
+        const C2<T> *sm = smaps + ((int64_t)c * Nx + x) * NYZ;
+        for (int i = tid; i < NYZ; i += nthr) buf[i] = pl[i] * sm[i];
         __syncthreads();
-        fx(buf, nl, Nx, 1);
-        C2<T> *out = Tb + ((int64_t)f * Nc + c) * V + l0;
-        for (int idx = tid; idx < nel; idx += NTX) {
-            const int kx = idx / nl;
-            const int l = idx - kx * nl;
-            out[(int64_t)kx * NYZ + l] = buf[l * Nx + kx];
-        }
+        fy(buf, Nz, Ny, 1, twy);
+        fz(buf, Ny, 1, Ny, twz);
+        C2<T> *dst = Tf + ((int64_t)c * Nx + x) * Lf;
+        for (int l = tid; l < Lf; l += nthr) dst[l] = buf[lines[l]];
+        if constexpr (!FY::kPipe) break;
         __syncthreads();
     }
 }
 
-// One workgroup per (kx, coil, frame): 2D FFT of one plane, then gather of the
-// sampled (ky, kz) with the post-phases, optional residual and its squared norm.
-template <class T, class FY, class FZ>
-__global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
-    const C2<T> *__restrict__ Tb, int Nc, int Nx, int Ny, int Nz, FY fy, FZ fz,
-    const int *__restrict__ ptr, const int *__restrict__ off, const int *__restrict__ perm,
-    int64_t K, const C2<T> *__restrict__ phx, const C2<T> *__restrict__ phy,
+// Forward pass 2: one workgroup per (block of LB lines, coil, frame).  The
+// lines are transformed along x in LDS and the sampled kx of each line are
+// gathered with the post-phases and the scale (+ residual and its squared norm).
+template <class T, class FX>
+__global__ __launch_bounds__(NTX) void k_fwd_x(
+    const C2<T> *__restrict__ Tb, int Nc, int Nx, int Ny, int LB, int nlb, FX fx, LineTabs tb,
+    int64_t K, int64_t ta, const C2<T> *__restrict__ phx, const C2<T> *__restrict__ phy,
     const C2<T> *__restrict__ phz, T scale, const C2<T> *__restrict__ ysub,
     C2<T> *__restrict__ out, double *__restrict__ partial) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
     __shared__ double red[16];
-    const int kx = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
-    const int NYZ = Ny * Nz;
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int *pf = ptr + (int64_t)f * (Nx + 1);
-    const int j0 = pf[kx], j1 = pf[kx + 1];
-    const int *of = off + (int64_t)f * K;
-    const int *pm = perm ? perm + (int64_t)f * K : nullptr;
-    const C2<T> px = phx[kx];
-    const C2<T> *src = Tb + (((int64_t)f * Nc + c) * Nx + kx) * NYZ;
-    for (int i = tid; i < NYZ; i += nthr) buf[i] = src[i];
+    const int lb = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
+    const int tid = threadIdx.x;
+    const int g0 = tb.lptr[ta + f];
+    const int Lf = tb.lptr[ta + f + 1] - g0;
+    const int l0 = lb * LB;
+    if (l0 >= Lf) {  // uniform per workgroup
+        if (partial && tid == 0) partial[((int64_t)f * Nc + c) * nlb + lb] = 0.0;
+        return;
+    }
+    const int nl = min(LB, Lf - l0);
+    const C2<T> *src = Tb + (int64_t)Nc * Nx * (g0 - tb.lptr[ta]) + (int64_t)c * Nx * Lf + l0;
+    for (int e = tid; e < nl * Nx; e += NTX) {
+        const int kx = e / nl;
+        const int l = e - kx * nl;
+        buf[l * Nx + kx] = src[(int64_t)kx * Lf + l];
+    }
     __syncthreads();
-    fy(buf, Nz, Ny, 1);
-    fz(buf, Ny, 1, Ny);
+    fx(buf, nl, Nx, 1);
+    __syncthreads();
+    const int *kb = tb.kbeg + g0;
+    const int jb = kb[l0];
+    const int je = l0 + nl < Lf ? kb[l0 + nl] : (int)K;
+    const int64_t fa = (ta + f) * K;
+    const int *lines = tb.line_yz + g0;
     const int64_t obase = ((int64_t)f * Nc + c) * K;
     double acc = 0.0;
-    for (int j = j0 + tid; j < j1; j += nthr) {
-        const int o = of[j];
-        const int ky = o % Ny;
-        const int kz = o / Ny;
-        const C2<T> ph = px * phy[ky] * phz[kz];
-        C2<T> v = scale * (buf[o] * ph);
-        const int64_t oi = obase + (pm ? pm[j] : j);
-        if (ysub) v = v - ysub[oi];
-        out[oi] = v;
-        acc += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
+    for (int j = jb + tid; j < je; j += NTX) {
+        const int v = tb.lk[fa + j];
+        const int l = v / Nx;
+        const int kx = v - l * Nx;
+        const int yz = lines[l];
+        const int ky = yz % Ny;
+        const int kz = yz / Ny;
+        const C2<T> ph = phx[kx] * phy[ky] * phz[kz];
+        C2<T> val = scale * (buf[(l - l0) * Nx + kx] * ph);
+        const int64_t oi = obase + j;
+        if (ysub) val = val - ysub[oi];
+        out[oi] = val;
+        acc += (double)val.re * (double)val.re + (double)val.im * (double)val.im;
     }
     if (partial) {
-        const double sres = block_sum(acc, red);
-        if (tid == 0) partial[((int64_t)f * Nc + c) * Nx + kx] = sres;
+        const double s = block_sum(acc, red);
+        if (tid == 0) partial[((int64_t)f * Nc + c) * nlb + lb] = s;
     }
 }
 
-// One workgroup per (kx, coil, frame): scatter of the sampled (ky, kz) times the
-// conjugate post-phases into a zeroed plane, then its inverse-direction 2D FFT.
-template <class T, class FY, class FZ>
-__global__ __launch_bounds__(NTYZ_MAX) void k_adj_yz(
-    const C2<T> *__restrict__ yin, int Nc, int Nx, int Ny, int Nz, FY fy, FZ fz,
-    const int *__restrict__ ptr, const int *__restrict__ off, const int *__restrict__ perm,
-    int64_t K, const C2<T> *__restrict__ phx, const C2<T> *__restrict__ phy,
+// Adjoint pass A: one workgroup per (block of LB lines, coil, frame).  The
+// samples of the lines (times the conjugate post-phases) are scattered into
+// zeroed lines, transformed along x, and written as T[f][c][x][l].
+template <class T, class FX>
+__global__ __launch_bounds__(NTX) void k_adj_x(
+    const C2<T> *__restrict__ yin, int Nc, int Nx, int Ny, int LB, FX fx, LineTabs tb, int64_t K,
+    int64_t ta, const C2<T> *__restrict__ phx, const C2<T> *__restrict__ phy,
     const C2<T> *__restrict__ phz, C2<T> *__restrict__ Tb) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
-    const int kx = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
-    const int NYZ = Ny * Nz;
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int *pf = ptr + (int64_t)f * (Nx + 1);
-    const int j0 = pf[kx], j1 = pf[kx + 1];
-    const int *of = off + (int64_t)f * K;
-    const int *pm = perm ? perm + (int64_t)f * K : nullptr;
-    const C2<T> px = phx[kx];
-    for (int i = tid; i < NYZ; i += nthr) buf[i] = C2<T>{T(0), T(0)};
+    const int lb = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
+    const int tid = threadIdx.x;
+    const int g0 = tb.lptr[ta + f];
+    const int Lf = tb.lptr[ta + f + 1] - g0;
+    const int l0 = lb * LB;
+    if (l0 >= Lf) return;  // uniform per workgroup
+    const int nl = min(LB, Lf - l0);
+    for (int e = tid; e < nl * Nx; e += NTX) buf[e] = C2<T>{T(0), T(0)};
     __syncthreads();
+    const int *kb = tb.kbeg + g0;
+    const int jb = kb[l0];
+    const int je = l0 + nl < Lf ? kb[l0 + nl] : (int)K;
+    const int64_t fa = (ta + f) * K;
+    const int *lines = tb.line_yz + g0;
     const int64_t ibase = ((int64_t)f * Nc + c) * K;
-    for (int j = j0 + tid; j < j1; j += nthr) {
-        const int o = of[j];
-        const int ky = o % Ny;
-        const int kz = o / Ny;
-        const C2<T> ph = px * phy[ky] * phz[kz];
-        buf[o] = cmulc(ph, yin[ibase + (pm ? pm[j] : j)]);
+    for (int j = jb + tid; j < je; j += NTX) {
+        const int v = tb.lk[fa + j];
+        const int l = v / Nx;
+        const int kx = v - l * Nx;
+        const int yz = lines[l];
+        const int ky = yz % Ny;
+        const int kz = yz / Ny;
+        const C2<T> ph = phx[kx] * phy[ky] * phz[kz];
+        buf[(l - l0) * Nx + kx] = cmulc(ph, yin[ibase + j]);
     }
     __syncthreads();
-    fz(buf, Ny, 1, Ny);
-    fy(buf, Nz, Ny, 1);
-    C2<T> *dst = Tb + (((int64_t)f * Nc + c) * Nx + kx) * NYZ;
-    for (int i = tid; i < NYZ; i += nthr) dst[i] = buf[i];
+    fx(buf, nl, Nx, 1);
+    __syncthreads();
+    C2<T> *dst = Tb + (int64_t)Nc * Nx * (g0 - tb.lptr[ta]) + (int64_t)c * Nx * Lf + l0;
+    for (int e = tid; e < nl * Nx; e += NTX) {
+        const int x = e / nl;
+        const int l = e - x * nl;
+        dst[(int64_t)x * Lf + l] = buf[l * Nx + x];
+    }
 }
 
-template <class T, class FX>
-__global__ __launch_bounds__(NTX) void k_adj_x(const C2<T> *__restrict__ Tb,
-                                               const C2<T> *__restrict__ smaps, int Nc, int64_t V,
-                                               int Nx, int NYZ, int LB, FX fx, T scale,
-                                               C2<T> *__restrict__ xout) {
+// Adjoint pass B: one workgroup per (x, frame).  For each coil the frame's
+// lines at x are scattered into a zeroed (y, z) plane, transformed along z and
+// y, and accumulated times the conjugate coil map; the scaled sum is written
+// as the plane-major image plane.  Compile-time FFT lengths accumulate in
+// registers (addresses kept out of loop-invariant code motion so that two
+// workgroups stay resident per CU), runtime lengths in the output plane itself.
+template <class T, class FY, class FZ>
+__global__ __launch_bounds__(NTYZ_MAX) void k_adj_yz(
+    const C2<T> *__restrict__ Tb, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny, int Nz,
+    FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz, int64_t ta,
+    T scale, C2<T> *__restrict__ Pm) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
-    const int f = blockIdx.y;
-    const int l0 = blockIdx.x * LB;
-    const int nl = min(LB, NYZ - l0);
-    const int nel = nl * Nx;
-    const int tid = threadIdx.x;
-    C2<T> acc[MAXE_X];
+    const int x = blockIdx.x, f = blockIdx.y;
+    const int NYZ = Ny * Nz;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int g0 = lptr[ta + f];
+    const int Lf = lptr[ta + f + 1] - g0;
+    const int *lines = line_yz + g0;
+    const C2<T> *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
+    C2<T> *pl = Pm + ((int64_t)f * Nx + x) * NYZ;
+    constexpr bool REG = FY::kPipe;
+    constexpr int PE = REG ? FY::kPlane : 1;
+    C2<T> acc[PE];
 #pragma unroll
-    for (int q = 0; q < MAXE_X; ++q) acc[q] = C2<T>{T(0), T(0)};
-    // plane block of the next coil (transposed read) is loaded before this coil's FFT
-    constexpr int PX = FX::kPipe ? MAXE_X : 1;
-    C2<T> tn[PX];
-    if constexpr (FX::kPipe) {
-        const C2<T> *src = Tb + (int64_t)f * Nc * V + l0;
-#pragma unroll
-        for (int q = 0; q < PX; ++q) {
-            const int idx = tid + q * NTX;
-            if (idx < nel) {
-                const int kx = idx / nl;
-                const int l = idx - kx * nl;
-                tn[q] = src[(int64_t)kx * NYZ + l];
-            }
-        }
-    }
+    for (int q = 0; q < PE; ++q) acc[q] = C2<T>{T(0), T(0)};
+#pragma unroll 1
     for (int c = 0; c < Nc; ++c) {
-        if constexpr (FX::kPipe) {
-#pragma unroll
-            for (int q = 0; q < PX; ++q) {
-                const int idx = tid + q * NTX;
-                if (idx < nel) {
-                    const int kx = idx / nl;
-                    const int l = idx - kx * nl;
-                    buf[l * Nx + kx] = tn[q];
-                }
-            }
-        } else {
-            const C2<T> *src = Tb + ((int64_t)f * Nc + c) * V + l0;
-            for (int idx = tid; idx < nel; idx += NTX) {
-                const int kx = idx / nl;
-                const int l = idx - kx * nl;
-                buf[l * Nx + kx] = src[(int64_t)kx * NYZ + l];
-            }
-        }
-        C2<T> smc[PX];
-        if constexpr (FX::kPipe) {
-            const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
-#pragma unroll
-            for (int q = 0; q < PX; ++q) {
-                const int e = tid + q * NTX;
-                if (e < nel) smc[q] = sm[e];
-            }
-            if (c + 1 < Nc) {
-                const C2<T> *src = Tb + ((int64_t)f * Nc + c + 1) * V + l0;
-#pragma unroll
-                for (int q = 0; q < PX; ++q) {
-                    const int idx = tid + q * NTX;
-                    if (idx < nel) {
-                        const int kx = idx / nl;
-                        const int l = idx - kx * nl;
-                        tn[q] = src[(int64_t)kx * NYZ + l];
-                    }
-                }
-            }
-        }
+        const C2<T> *twy = opaque_ptr(fy.tw), *twz = opaque_ptr(fz.tw);
+        for (int i = tid; i < NYZ; i += nthr) buf[i] = C2<T>{T(0), T(0)};
         __syncthreads();
-        fx(buf, nl, Nx, 1);
-        if constexpr (FX::kPipe) {
+        const C2<T> *src = Tf + ((int64_t)c * Nx + x) * Lf;
+        for (int l = tid; l < Lf; l += nthr) buf[lines[l]] = src[l];
+        __syncthreads();
+        fz(buf, Ny, 1, Ny, twz);
+        fy(buf, Nz, Ny, 1, twy);
+        const C2<T> *sm = smaps + ((int64_t)c * Nx + x) * NYZ;
+        if constexpr (REG) {
+            const C2<T> *smt = opaque_lane_ptr(sm + tid);
 #pragma unroll
-            for (int q = 0; q < PX; ++q) {
-                const int e = tid + q * NTX;
-                if (e < nel) acc[q] = acc[q] + cmulc(smc[q], buf[e]);
+            for (int q = 0; q < PE; ++q) {
+                const int i = tid + q * FY::kNT;
+                if (q < PE - 1 || i < NYZ) acc[q] = acc[q] + cmulc(smt[q * FY::kNT], buf[i]);
             }
         } else {
-            const C2<T> *sm = smaps + (int64_t)c * V + (int64_t)l0 * Nx;
-#pragma unroll
-            for (int q = 0; q < MAXE_X; ++q) {
-                const int e = tid + q * NTX;
-                if (e < nel) acc[q] = acc[q] + cmulc(sm[e], buf[e]);
+            // each thread owns the same plane elements for every coil
+            for (int i = tid; i < NYZ; i += nthr) {
+                const C2<T> v = cmulc(sm[i], buf[i]);
+                pl[i] = c == 0 ? v : pl[i] + v;
             }
         }
         __syncthreads();
     }
-    C2<T> *xo = xout + (int64_t)f * V + (int64_t)l0 * Nx;
+    if constexpr (REG) {
 #pragma unroll
-    for (int q = 0; q < MAXE_X; ++q) {
-        const int e = tid + q * NTX;
-        if (e < nel) xo[e] = scale * acc[q];
-    }
-}
-
-// y_int[f][c][j] = y_ref[f][c][perm[f][j]]
-__global__ void k_perm_gather(const c64 *__restrict__ yref, const int *__restrict__ perm,
-                              int64_t K, int Nc, int Nt, c64 *__restrict__ yint) {
-    const int64_t n = K * Nc * (int64_t)Nt;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = i % K;
-        const int64_t fc = i / K;
-        const int64_t f = fc / Nc;
-        yint[i] = yref[fc * K + perm[f * K + j]];
+        for (int q = 0; q < PE; ++q) {
+            const int i = tid + q * nthr;
+            if (i < NYZ) pl[i] = scale * acc[q];
+        }
+    } else {
+        for (int i = tid; i < NYZ; i += nthr) pl[i] = scale * pl[i];
     }
 }
 
@@ -338,14 +383,37 @@ static void allow_lds(const void *fn) {
     done.push_back(fn);
 }
 
+// Frame chunks [t, t+n) of [t0, t0+nframes): the line buffer of a chunk
+// (Nc*Nx*lines elements) and its plane-major image (nf*V) each fit chunk_bytes.
 template <class T>
-static int64_t frames_per_chunk(mslr_sense_s *P, int64_t nframes) {
-    const int64_t per = (int64_t)P->Nc * P->V * (int64_t)sizeof(C2<T>);
-    int64_t F = (int64_t)(P->chunk_bytes / (size_t)per);
-    if (F < 1) F = 1;
-    if (F > nframes) F = nframes;
-    if (F > 65535) F = 65535;
-    return F;
+static std::vector<std::pair<int64_t, int64_t>> frame_chunks(mslr_sense_s *P, int64_t t0,
+                                                             int64_t nframes) {
+    std::vector<std::pair<int64_t, int64_t>> ch;
+    const size_t el = sizeof(C2<T>);
+    const size_t per_line = (size_t)P->Nc * P->N[0] * el;
+    const size_t per_plane = (size_t)P->V * el;
+    int64_t s = t0, n = 0;
+    size_t lb = 0, pb = 0;
+    for (int64_t t = t0; t < t0 + nframes; ++t) {
+        const size_t l = per_line * (size_t)(P->lptr_h[t + 1] - P->lptr_h[t]);
+        if (n > 0 && (lb + l > P->chunk_bytes || pb + per_plane > P->chunk_bytes || n >= 65535)) {
+            ch.push_back({s, n});
+            s = t;
+            n = 0;
+            lb = pb = 0;
+        }
+        lb += l;
+        pb += per_plane;
+        ++n;
+    }
+    if (n > 0) ch.push_back({s, n});
+    return ch;
+}
+
+template <class T>
+static size_t chunk_line_bytes(mslr_sense_s *P, int64_t t, int64_t n) {
+    return std::max<size_t>(1, (size_t)P->Nc * P->N[0] * sizeof(C2<T>) *
+                                   (size_t)(P->lptr_h[t + n] - P->lptr_h[t]));
 }
 
 template <class T>
@@ -355,59 +423,96 @@ static void check_lds(mslr_sense_s *P) {
                  "Ny*Nz plane does not fit in LDS at this precision");
 }
 
-// ---- launch wrappers (one chunk of nf frames; ta = absolute first frame)
-template <class T, class FX>
-static void launch_fwd_x(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride, int nf,
-                         C2<T> *Tb, FX fx) {
+// yz-lines per workgroup of the transposes (LDS tile Nx x (TB+1))
+template <class T>
+static int transpose_tb(mslr_sense_s *P) {
+    const int cap = (int)((150 * 1024) / ((size_t)P->N[0] * sizeof(C2<T>))) - 1;
+    return std::max(1, std::min(64, std::min(cap, P->NYZ)));
+}
+
+template <class T>
+static void launch_sum_planes(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride,
+                              int nf, C2<T> *Pm) {
     mslr_ctx_s *ctx = P->ctx;
-    const int LB = P->LBx, Nx = P->N[0];
-    allow_lds((const void *)k_fwd_x<T, FX>);
-    dim3 gx((unsigned)cdiv(P->NYZ, LB), (unsigned)nf);
-    MSLR_PROF(ctx, "fwd_x");
-    k_fwd_x<T, FX><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
-        X, nscales, sstride, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB, fx, Tb);
+    const int TB = transpose_tb<T>(P), Nx = P->N[0];
+    allow_lds((const void *)k_sum_planes<T>);
+    dim3 g((unsigned)cdiv(P->NYZ, TB), (unsigned)nf);
+    MSLR_PROF(ctx, "sum_planes");
+    k_sum_planes<T><<<g, 256, (size_t)Nx * (TB + 1) * sizeof(C2<T>), ctx->stream>>>(
+        X, nscales, sstride, Nx, P->NYZ, TB, Pm);
     HIP_LAUNCH_CHECK();
 }
 
+template <class T>
+static void launch_planes_to_image(mslr_sense_s *P, const C2<T> *Pm, int nf, C2<T> *X) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int TB = transpose_tb<T>(P), Nx = P->N[0];
+    allow_lds((const void *)k_planes_to_image<T>);
+    dim3 g((unsigned)cdiv(P->NYZ, TB), (unsigned)nf);
+    MSLR_PROF(ctx, "planes_to_image");
+    k_planes_to_image<T><<<g, 256, (size_t)Nx * (TB + 1) * sizeof(C2<T>), ctx->stream>>>(
+        Pm, Nx, P->NYZ, TB, X);
+    HIP_LAUNCH_CHECK();
+}
+
+static LineTabs line_tabs(mslr_sense_s *P) {
+    return LineTabs{P->lptr, P->line_yz, P->kbeg, P->lk};
+}
+
+// ---- launch wrappers (one chunk of nf frames; ta = absolute first frame)
 template <class T, class FY, class FZ>
-static void launch_fwd_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *ysub, C2<T> *out,
-                          bool ref_order, double *partial, const C2<T> *Tb, FY fy, FZ fz, int nt) {
+static void launch_fwd_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Pm, C2<T> *Tb,
+                          FY fy, FZ fz, int nt) {
     mslr_ctx_s *ctx = P->ctx;
     const int Nx = P->N[0];
+    const int cpw = FY::kPipe ? P->Nc : 1;
     allow_lds((const void *)k_fwd_yz<T, FY, FZ>);
-    dim3 gyz((unsigned)Nx, (unsigned)P->Nc, (unsigned)nf);
+    dim3 g((unsigned)Nx, (unsigned)cdiv(P->Nc, cpw), (unsigned)nf);
     MSLR_PROF(ctx, "fwd_yz");
-    k_fwd_yz<T, FY, FZ><<<gyz, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
-        Tb, P->Nc, Nx, P->N[1], P->N[2], fy, fz, P->ptr + ta * (Nx + 1), P->off + ta * P->K,
-        ref_order ? P->perm + ta * P->K : nullptr, P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1),
-        Dev<T>::ph(P, 2), (T)P->scale, ysub, out, partial);
+    k_fwd_yz<T, FY, FZ><<<g, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
+        Pm, Dev<T>::smaps(P), P->Nc, Nx, P->N[1], P->N[2], cpw, fy, fz, P->lptr, P->line_yz, ta,
+        Tb);
+    HIP_LAUNCH_CHECK();
+}
+
+template <class T, class FX>
+static void launch_fwd_x(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, const C2<T> *ysub,
+                         C2<T> *out, double *partial, FX fx) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int Nx = P->N[0];
+    allow_lds((const void *)k_fwd_x<T, FX>);
+    dim3 g((unsigned)P->nlb, (unsigned)P->Nc, (unsigned)nf);
+    MSLR_PROF(ctx, "fwd_x");
+    k_fwd_x<T, FX><<<g, NTX, (size_t)P->LBx * Nx * sizeof(C2<T>), ctx->stream>>>(
+        Tb, P->Nc, Nx, P->N[1], P->LBx, P->nlb, fx, line_tabs(P), P->K, ta,
+        Dev<T>::ph(P, 0), Dev<T>::ph(P, 1), Dev<T>::ph(P, 2), (T)P->scale, ysub, out, partial);
+    HIP_LAUNCH_CHECK();
+}
+
+template <class T, class FX>
+static void launch_adj_x(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, C2<T> *Tb, FX fx) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int Nx = P->N[0];
+    allow_lds((const void *)k_adj_x<T, FX>);
+    dim3 g((unsigned)P->nlb, (unsigned)P->Nc, (unsigned)nf);
+    MSLR_PROF(ctx, "adj_x");
+    k_adj_x<T, FX><<<g, NTX, (size_t)P->LBx * Nx * sizeof(C2<T>), ctx->stream>>>(
+        y, P->Nc, Nx, P->N[1], P->LBx, fx, line_tabs(P), P->K, ta, Dev<T>::ph(P, 0),
+        Dev<T>::ph(P, 1), Dev<T>::ph(P, 2), Tb);
     HIP_LAUNCH_CHECK();
 }
 
 template <class T, class FY, class FZ>
-static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, bool ref_order,
-                          C2<T> *Tb, FY fy, FZ fz, int nt) {
+static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, C2<T> *Pm, FY fy,
+                          FZ fz, int nt) {
     mslr_ctx_s *ctx = P->ctx;
     const int Nx = P->N[0];
     allow_lds((const void *)k_adj_yz<T, FY, FZ>);
-    dim3 gyz((unsigned)Nx, (unsigned)P->Nc, (unsigned)nf);
+    dim3 g((unsigned)Nx, (unsigned)nf);
     MSLR_PROF(ctx, "adj_yz");
-    k_adj_yz<T, FY, FZ><<<gyz, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
-        y, P->Nc, Nx, P->N[1], P->N[2], fy, fz, P->ptr + ta * (Nx + 1), P->off + ta * P->K,
-        ref_order ? P->perm + ta * P->K : nullptr, P->K, Dev<T>::ph(P, 0), Dev<T>::ph(P, 1),
-        Dev<T>::ph(P, 2), Tb);
-    HIP_LAUNCH_CHECK();
-}
-
-template <class T, class FX>
-static void launch_adj_x(mslr_sense_s *P, int nf, const C2<T> *Tb, C2<T> *xout, FX fx) {
-    mslr_ctx_s *ctx = P->ctx;
-    const int LB = P->LBx, Nx = P->N[0];
-    allow_lds((const void *)k_adj_x<T, FX>);
-    dim3 gx((unsigned)cdiv(P->NYZ, LB), (unsigned)nf);
-    MSLR_PROF(ctx, "adj_x");
-    k_adj_x<T, FX><<<gx, NTX, (size_t)LB * Nx * sizeof(C2<T>), ctx->stream>>>(
-        Tb, Dev<T>::smaps(P), P->Nc, P->V, Nx, P->NYZ, LB, fx, (T)P->scale, xout);
+    k_adj_yz<T, FY, FZ><<<g, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
+        Tb, Dev<T>::smaps(P), P->Nc, Nx, P->N[1], P->N[2], fy, fz, P->lptr, P->line_yz, ta,
+        (T)P->scale, Pm);
     HIP_LAUNCH_CHECK();
 }
 
@@ -426,37 +531,19 @@ using FixZ = FftFix<float, NZ, SGN, NY, fix_nt_yz(NY, NZ)>;
 #define MSLR_YZ_SIZES(M) M(90, 60) M(64, 40) M(32, 24) M(12, 10)
 
 template <class T>
-static void fwd_x_any(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride, int nf,
-                      C2<T> *Tb) {
-    if constexpr (std::is_same<T, float>::value) {
-        if (P->sgn == -1 && P->use_fixed) {
-            switch (P->N[0]) {
-#define CASE_X(n)                                                              \
-    case n:                                                                    \
-        launch_fwd_x<float>(P, X, nscales, sstride, nf, Tb, FixX<n, -1>{P->tw32[0]}); \
-        return;
-                MSLR_X_SIZES(CASE_X)
-#undef CASE_X
-                default: break;
-            }
-        }
-    }
-    launch_fwd_x<T>(P, X, nscales, sstride, nf, Tb,
-                    FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), P->sgn});
+static bool fixed_ok(mslr_sense_s *P) {
+    return std::is_same<T, float>::value && P->sgn == -1 && P->use_fixed;
 }
 
 template <class T>
-static void fwd_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *ysub, C2<T> *out,
-                       bool ref_order, double *partial, const C2<T> *Tb) {
+static void fwd_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Pm, C2<T> *Tb) {
     if constexpr (std::is_same<T, float>::value) {
-        if (P->sgn == -1 && P->use_fixed) {
-            const int key = P->N[1] * 10000 + P->N[2];
-            switch (key) {
-#define CASE_YZ(ny, nz)                                                                   \
-    case ny * 10000 + nz:                                                                 \
-        launch_fwd_yz<float>(P, ta, nf, ysub, out, ref_order, partial, Tb,                \
-                             FixY<ny, nz, -1>{P->tw32[1]}, FixZ<ny, nz, -1>{P->tw32[2]},   \
-                             fix_nt_yz(ny, nz));                                          \
+        if (fixed_ok<T>(P)) {
+            switch (P->N[1] * 10000 + P->N[2]) {
+#define CASE_YZ(ny, nz)                                                                  \
+    case ny * 10000 + nz:                                                                \
+        launch_fwd_yz<float>(P, ta, nf, Pm, Tb, FixY<ny, nz, -1>{P->tw32[1]},             \
+                             FixZ<ny, nz, -1>{P->tw32[2]}, fix_nt_yz(ny, nz));            \
         return;
                 MSLR_YZ_SIZES(CASE_YZ)
 #undef CASE_YZ
@@ -464,42 +551,19 @@ static void fwd_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *ysub, C
             }
         }
     }
-    launch_fwd_yz<T>(P, ta, nf, ysub, out, ref_order, partial, Tb,
-                     FftRt<T, MAXE_YZ>{P->d[1], Dev<T>::tw(P, 1), P->sgn},
+    launch_fwd_yz<T>(P, ta, nf, Pm, Tb, FftRt<T, MAXE_YZ>{P->d[1], Dev<T>::tw(P, 1), P->sgn},
                      FftRt<T, MAXE_YZ>{P->d[2], Dev<T>::tw(P, 2), P->sgn}, P->nt_yz);
 }
 
 template <class T>
-static void adj_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, bool ref_order,
-                       C2<T> *Tb) {
+static void fwd_x_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, const C2<T> *ysub,
+                      C2<T> *out, double *partial) {
     if constexpr (std::is_same<T, float>::value) {
-        if (P->sgn == -1 && P->use_fixed) {
-            const int key = P->N[1] * 10000 + P->N[2];
-            switch (key) {
-#define CASE_YZ(ny, nz)                                                                   \
-    case ny * 10000 + nz:                                                                 \
-        launch_adj_yz<float>(P, ta, nf, y, ref_order, Tb, FixY<ny, nz, 1>{P->tw32[1]},    \
-                             FixZ<ny, nz, 1>{P->tw32[2]}, fix_nt_yz(ny, nz));              \
-        return;
-                MSLR_YZ_SIZES(CASE_YZ)
-#undef CASE_YZ
-                default: break;
-            }
-        }
-    }
-    launch_adj_yz<T>(P, ta, nf, y, ref_order, Tb,
-                     FftRt<T, MAXE_YZ>{P->d[1], Dev<T>::tw(P, 1), -P->sgn},
-                     FftRt<T, MAXE_YZ>{P->d[2], Dev<T>::tw(P, 2), -P->sgn}, P->nt_yz);
-}
-
-template <class T>
-static void adj_x_any(mslr_sense_s *P, int nf, const C2<T> *Tb, C2<T> *xout) {
-    if constexpr (std::is_same<T, float>::value) {
-        if (P->sgn == -1 && P->use_fixed) {
+        if (fixed_ok<T>(P)) {
             switch (P->N[0]) {
-#define CASE_X(n)                                                 \
-    case n:                                                       \
-        launch_adj_x<float>(P, nf, Tb, xout, FixX<n, 1>{P->tw32[0]}); \
+#define CASE_X(n)                                                                          \
+    case n:                                                                                \
+        launch_fwd_x<float>(P, ta, nf, Tb, ysub, out, partial, FixX<n, -1>{P->tw32[0]});   \
         return;
                 MSLR_X_SIZES(CASE_X)
 #undef CASE_X
@@ -507,68 +571,122 @@ static void adj_x_any(mslr_sense_s *P, int nf, const C2<T> *Tb, C2<T> *xout) {
             }
         }
     }
-    launch_adj_x<T>(P, nf, Tb, xout, FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), -P->sgn});
+    launch_fwd_x<T>(P, ta, nf, Tb, ysub, out, partial,
+                    FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), P->sgn});
+}
+
+template <class T>
+static void adj_x_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, C2<T> *Tb) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (fixed_ok<T>(P)) {
+            switch (P->N[0]) {
+#define CASE_X(n)                                                                       \
+    case n:                                                                             \
+        launch_adj_x<float>(P, ta, nf, y, Tb, FixX<n, 1>{P->tw32[0]});                   \
+        return;
+                MSLR_X_SIZES(CASE_X)
+#undef CASE_X
+                default: break;
+            }
+        }
+    }
+    launch_adj_x<T>(P, ta, nf, y, Tb, FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), -P->sgn});
+}
+
+template <class T>
+static void adj_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, C2<T> *Pm) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (fixed_ok<T>(P)) {
+            switch (P->N[1] * 10000 + P->N[2]) {
+#define CASE_YZ(ny, nz)                                                                  \
+    case ny * 10000 + nz:                                                                \
+        launch_adj_yz<float>(P, ta, nf, Tb, Pm, FixY<ny, nz, 1>{P->tw32[1]},              \
+                             FixZ<ny, nz, 1>{P->tw32[2]}, fix_nt_yz(ny, nz));             \
+        return;
+                MSLR_YZ_SIZES(CASE_YZ)
+#undef CASE_YZ
+                default: break;
+            }
+        }
+    }
+    launch_adj_yz<T>(P, ta, nf, Tb, Pm, FftRt<T, MAXE_YZ>{P->d[1], Dev<T>::tw(P, 1), -P->sgn},
+                     FftRt<T, MAXE_YZ>{P->d[2], Dev<T>::tw(P, 2), -P->sgn}, P->nt_yz);
+}
+
+// Forward of one chunk [ta, ta+nf): X holds the chunk's frames (nscales images
+// spaced sstride apart), ysub/out/partial start at the chunk.
+template <class T>
+static void forward_chunk(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride,
+                          int64_t ta, int nf, const C2<T> *ysub, C2<T> *out, double *partial) {
+    mslr_ctx_s *ctx = P->ctx;
+    C2<T> *Pm = (C2<T> *)ctx->scratch_get(SLOT_PLANES, (size_t)nf * P->V * sizeof(C2<T>));
+    C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, chunk_line_bytes<T>(P, ta, nf));
+    launch_sum_planes<T>(P, X, nscales, sstride, nf, Pm);
+    fwd_yz_any<T>(P, ta, nf, Pm, Tb);
+    fwd_x_any<T>(P, ta, nf, Tb, ysub, out, partial);
+}
+
+template <class T>
+static void adjoint_chunk(mslr_sense_s *P, const C2<T> *y, int64_t ta, int nf, C2<T> *xout) {
+    mslr_ctx_s *ctx = P->ctx;
+    C2<T> *Pm = (C2<T> *)ctx->scratch_get(SLOT_PLANES, (size_t)nf * P->V * sizeof(C2<T>));
+    C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, chunk_line_bytes<T>(P, ta, nf));
+    adj_x_any<T>(P, ta, nf, y, Tb);
+    adj_yz_any<T>(P, ta, nf, Tb, Pm);
+    launch_planes_to_image<T>(P, Pm, nf, xout);
 }
 
 // Forward over frames [t0, t0+nframes) of the plan; X holds local frames.
 template <class T>
 static void run_forward(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride,
                         int64_t t0, int64_t nframes, const C2<T> *ysub, C2<T> *out,
-                        bool ref_order, double *partial) {
-    mslr_ctx_s *ctx = P->ctx;
+                        double *partial) {
     check_lds<T>(P);
-    const int64_t F = frames_per_chunk<T>(P, nframes);
-    C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, (size_t)F * P->Nc * P->V * sizeof(C2<T>));
-    for (int64_t f0 = 0; f0 < nframes; f0 += F) {
-        const int nf = (int)std::min<int64_t>(F, nframes - f0);
-        fwd_x_any<T>(P, X + f0 * P->V, nscales, sstride, nf, Tb);
-        fwd_yz_any<T>(P, t0 + f0, nf, ysub ? ysub + f0 * P->Nc * P->K : nullptr,
-                      out + f0 * P->Nc * P->K, ref_order,
-                      partial ? partial + f0 * P->Nc * P->N[0] : nullptr, Tb);
+    for (auto [ta, nf] : frame_chunks<T>(P, t0, nframes)) {
+        const int64_t f0 = ta - t0;
+        forward_chunk<T>(P, X + f0 * P->V, nscales, sstride, ta, (int)nf,
+                         ysub ? ysub + f0 * P->Nc * P->K : nullptr, out + f0 * P->Nc * P->K,
+                         partial ? partial + f0 * P->Nc * P->nlb : nullptr);
     }
 }
 
 template <class T>
 static void run_adjoint(mslr_sense_s *P, const C2<T> *y, int64_t t0, int64_t nframes,
-                        C2<T> *xout, bool ref_order) {
-    mslr_ctx_s *ctx = P->ctx;
+                        C2<T> *xout) {
     check_lds<T>(P);
-    const int64_t F = frames_per_chunk<T>(P, nframes);
-    C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, (size_t)F * P->Nc * P->V * sizeof(C2<T>));
-    for (int64_t f0 = 0; f0 < nframes; f0 += F) {
-        const int nf = (int)std::min<int64_t>(F, nframes - f0);
-        adj_yz_any<T>(P, t0 + f0, nf, y + f0 * P->Nc * P->K, ref_order, Tb);
-        adj_x_any<T>(P, nf, Tb, xout + f0 * P->V);
+    for (auto [ta, nf] : frame_chunks<T>(P, t0, nframes)) {
+        const int64_t f0 = ta - t0;
+        adjoint_chunk<T>(P, y + f0 * P->Nc * P->K, ta, (int)nf, xout + f0 * P->V);
     }
 }
 
 void sense_ref_to_internal(mslr_sense_s *P, const c64 *y_ref, c64 *y_int) {
     const int64_t n = P->K * P->Nc * (int64_t)P->Nt;
-    if (n == 0) return;
-    int64_t nb = std::min<int64_t>(cdiv(n, 256), 65536);
-    k_perm_gather<<<(unsigned)nb, 256, 0, P->ctx->stream>>>(y_ref, P->perm, P->K, P->Nc, P->Nt,
-                                                            y_int);
-    HIP_LAUNCH_CHECK();
+    if (n == 0 || y_ref == y_int) return;
+    HIP_CHECK(hipMemcpyAsync(y_int, y_ref, (size_t)n * sizeof(c64), hipMemcpyDeviceToDevice,
+                             P->ctx->stream));
 }
 
 void sense_grad_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
                          const c64 *y_int, c64 *g, double *dc_partial) {
     mslr_ctx_s *ctx = P->ctx;
+    check_lds<float>(P);
     // Residuals of one frame chunk live in scratch; A and A' alternate per chunk.
-    const int64_t F = frames_per_chunk<float>(P, P->Nt);
-    c64 *r = (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, F * P->Nc * P->K) *
+    const auto chunks = frame_chunks<float>(P, 0, P->Nt);
+    int64_t fmax = 1;
+    for (auto [ta, nf] : chunks) fmax = std::max(fmax, nf);
+    c64 *r = (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, fmax * P->Nc * P->K) *
                                                      sizeof(c64));
-    for (int64_t f0 = 0; f0 < P->Nt; f0 += F) {
-        const int64_t nf = std::min<int64_t>(F, P->Nt - f0);
-        run_forward<float>(P, X + f0 * P->V, nscales, sstride, f0, nf,
-                           y_int + f0 * P->Nc * P->K, r, false,
-                           dc_partial ? dc_partial + f0 * P->Nc * P->N[0] : nullptr);
-        run_adjoint<float>(P, r, f0, nf, g + f0 * P->V, false);
+    for (auto [ta, nf] : chunks) {
+        forward_chunk<float>(P, X + ta * P->V, nscales, sstride, ta, (int)nf,
+                             y_int + ta * P->Nc * P->K, r,
+                             dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr);
+        adjoint_chunk<float>(P, r, ta, (int)nf, g + ta * P->V);
     }
 }
 
 void sense_adjoint_internal(mslr_sense_s *P, const c64 *y_int, c64 *x) {
-    run_adjoint<float>(P, y_int, 0, P->Nt, x, false);
+    run_adjoint<float>(P, y_int, 0, P->Nt, x);
 }
 
 // exp(2 pi i num/den) with exact values on quarter turns.
@@ -640,58 +758,64 @@ mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, in
             // fft_forward = false: the forward applies the normalised inverse DFT
             // (src/sense_gpu.jl:77), so the 1/prod(N) joins the scale.
             if (!fft_forward) P->scale /= (double)P->V;
-            P->LBx = std::max(1, std::min(P->NYZ, (MAXE_X * NTX) / Nx));
             int nt = (int)cdiv(P->NYZ, MAXE_YZ);
             nt = (int)(cdiv(nt, 64) * 64);
             P->nt_yz = std::max(64, std::min(NTYZ_MAX, nt));
-            P->chunk_bytes = (size_t)512 << 20;
+            P->chunk_bytes = (size_t)4 << 30;
             P->use_fixed = getenv("MSLR_FFT_GENERIC") == nullptr;
             if (const char *e = getenv("MSLR_FFT_CHUNK_MB")) {
                 long mb = atol(e);
                 if (mb > 0) P->chunk_bytes = (size_t)mb << 20;
             }
 
-            // ---- mask -> sample tables
+            // ---- mask -> sampled lines and sample tables.  Samples of a frame are
+            // ordered by line (ascending ky + Ny*kz) then kx, which is the column-major
+            // order of findall(vec(mask_t)) (scripts/reconstruct.jl:135-142).
             const int64_t V = P->V;
             int64_t K = -1;
-            std::vector<int> ptr((size_t)(Nx + 1) * Nt);
-            std::vector<int64_t> counts(Nt, 0);
             for (int64_t t = 0; t < Nt; ++t) {
                 const uint8_t *m = mask_host + t * V;
                 int64_t k = 0;
                 for (int64_t i = 0; i < V; ++i) k += m[i] != 0;
-                counts[t] = k;
                 if (t == 0) K = k;
                 MSLR_REQUIRE(k == K, MSLR_ERR_SHAPE,
                              "Frame " + std::to_string(t + 1) + " has a different sample count");
             }
             MSLR_REQUIRE(K <= INT32_MAX, MSLR_ERR_UNSUPPORTED, "too many samples per frame");
+            MSLR_REQUIRE((int64_t)P->NYZ * Nt <= INT32_MAX, MSLR_ERR_UNSUPPORTED,
+                         "too many k-space lines");
             P->K = K;
-            std::vector<int> off((size_t)std::max<int64_t>(1, K * Nt));
-            std::vector<int> perm((size_t)std::max<int64_t>(1, K * Nt));
-            std::vector<int> cnt(Nx);
+            std::vector<int> lptr((size_t)Nt + 1, 0), line_yz, kbeg;
+            std::vector<int> lk((size_t)std::max<int64_t>(1, K * Nt));
+            int lmax = 1;
             for (int64_t t = 0; t < Nt; ++t) {
                 const uint8_t *m = mask_host + t * V;
-                std::fill(cnt.begin(), cnt.end(), 0);
-                for (int64_t i = 0; i < V; ++i)
-                    if (m[i]) cnt[i % Nx]++;
-                int *pt = ptr.data() + t * (Nx + 1);
-                pt[0] = 0;
-                for (int x = 0; x < Nx; ++x) pt[x + 1] = pt[x] + cnt[x];
-                std::vector<int> fill(pt, pt + Nx);
-                int kref = 0;
-                for (int64_t i = 0; i < V; ++i) {
-                    if (!m[i]) continue;
-                    const int x = (int)(i % Nx);
-                    const int o = (int)(i / Nx);
-                    const int j = fill[x]++;
-                    off[t * K + j] = o;
-                    perm[t * K + j] = kref++;
+                int j = 0, l = 0;
+                for (int yz = 0; yz < P->NYZ; ++yz) {
+                    const uint8_t *row = m + (int64_t)yz * Nx;
+                    bool any = false;
+                    for (int x = 0; x < Nx; ++x) {
+                        if (!row[x]) continue;
+                        if (!any) {
+                            line_yz.push_back(yz);
+                            kbeg.push_back(j);
+                            any = true;
+                        }
+                        lk[t * K + j] = l * Nx + x;
+                        ++j;
+                    }
+                    l += any;
                 }
+                lptr[t + 1] = lptr[t] + l;
+                lmax = std::max(lmax, l);
             }
-            P->ptr = upload_vec(P, ptr);
-            P->off = upload_vec(P, off);
-            P->perm = upload_vec(P, perm);
+            P->LBx = std::max(1, std::min(lmax, (MAXE_X * NTX) / Nx));
+            P->nlb = (int)cdiv(lmax, P->LBx);
+            P->lptr_h = lptr;
+            P->lptr = upload_vec(P, lptr);
+            P->line_yz = upload_vec(P, line_yz);
+            P->kbeg = upload_vec(P, kbeg);
+            P->lk = upload_vec(P, lk);
 
             // ---- twiddles and phases
             const int s = P->sgn;
@@ -717,19 +841,22 @@ mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, in
                 P->tw32[a] = upload_vec(P, tw32);
                 P->ph32[a] = upload_vec(P, ph32);
             }
+            // coil maps times the pre-phase, plane-major: [c][x][y + Ny*z]
             std::vector<c128> sm64((size_t)V * Nc);
             std::vector<c64> sm32((size_t)V * Nc);
             for (int64_t ci = 0; ci < Nc; ++ci)
                 for (int z = 0; z < Nz; ++z)
                     for (int y = 0; y < Ny; ++y) {
                         const c128 pyz = pre[1][y] * pre[2][z];
+                        const int64_t yz = y + (int64_t)Ny * z;
                         for (int x = 0; x < Nx; ++x) {
-                            const int64_t i = x + (int64_t)Nx * (y + (int64_t)Ny * z);
+                            const int64_t i = x + (int64_t)Nx * yz;
                             const c128 ph = pre[0][x] * pyz;
                             const float *sp = smaps_host + 2 * (ci * V + i);
                             const c128 v = c128{(double)sp[0], (double)sp[1]} * ph;
-                            sm64[ci * V + i] = v;
-                            sm32[ci * V + i] = {(float)v.re, (float)v.im};
+                            const int64_t o = (ci * Nx + x) * (int64_t)P->NYZ + yz;
+                            sm64[o] = v;
+                            sm32[o] = {(float)v.re, (float)v.im};
                         }
                     }
             P->smaps64 = upload_vec(P, sm64);
@@ -766,7 +893,7 @@ mslr_status_t mslr_sense_forward(mslr_sense plan, const void *x_dev, void *y_dev
         check_frames(plan, t0, nframes);
         if (nframes == 0) return;
         run_forward<float>(plan, (const c64 *)x_dev, 1, 0, t0, nframes, nullptr, (c64 *)y_dev,
-                           true, nullptr);
+                           nullptr);
     });
 }
 
@@ -775,7 +902,7 @@ mslr_status_t mslr_sense_adjoint(mslr_sense plan, const void *y_dev, void *x_dev
     return guarded([&] {
         check_frames(plan, t0, nframes);
         if (nframes == 0) return;
-        run_adjoint<float>(plan, (const c64 *)y_dev, t0, nframes, (c64 *)x_dev, true);
+        run_adjoint<float>(plan, (const c64 *)y_dev, t0, nframes, (c64 *)x_dev);
     });
 }
 
@@ -785,7 +912,7 @@ mslr_status_t mslr_sense_forward_f64(mslr_sense plan, const void *x_dev, void *y
         check_frames(plan, t0, nframes);
         if (nframes == 0) return;
         run_forward<double>(plan, (const c128 *)x_dev, 1, 0, t0, nframes, nullptr,
-                            (c128 *)y_dev, true, nullptr);
+                            (c128 *)y_dev, nullptr);
     });
 }
 
@@ -794,7 +921,7 @@ mslr_status_t mslr_sense_adjoint_f64(mslr_sense plan, const void *y_dev, void *x
     return guarded([&] {
         check_frames(plan, t0, nframes);
         if (nframes == 0) return;
-        run_adjoint<double>(plan, (const c128 *)y_dev, t0, nframes, (c128 *)x_dev, true);
+        run_adjoint<double>(plan, (const c128 *)y_dev, t0, nframes, (c128 *)x_dev);
     });
 }
 
@@ -808,11 +935,11 @@ mslr_status_t mslr_sense_grad(mslr_sense plan, const void *X_dev, int64_t nscale
         const int64_t ny = P->K * P->Nc * (int64_t)P->Nt;
         // Residual in reference order for the whole run (public entry point).
         c64 *r = (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, ny) * sizeof(c64));
-        const int64_t npart = (int64_t)P->Nt * P->Nc * P->N[0];
+        const int64_t npart = (int64_t)P->Nt * P->Nc * P->nlb;
         double *part = (double *)ctx->scratch_get(SLOT_REDUCE, (size_t)(npart + 1) * sizeof(double));
         run_forward<float>(P, (const c64 *)X_dev, (int)nscales, P->V * (int64_t)P->Nt, 0, P->Nt,
-                           (const c64 *)y_dev, r, true, part);
-        run_adjoint<float>(P, r, 0, P->Nt, (c64 *)g_dev, true);
+                           (const c64 *)y_dev, r, part);
+        run_adjoint<float>(P, r, 0, P->Nt, (c64 *)g_dev);
         if (dc_cost) {
             sum_partials(ctx, part, npart, part + npart);
             double h = 0.0;
@@ -844,12 +971,12 @@ mslr_status_t mslr_sense_poweriter(mslr_sense plan, int niter, double tol, doubl
         int it = 0;
         double xn2 = (double)nx;  // ||ones||^2
         for (it = 1; it <= niter; ++it) {
-            run_forward<double>(P, x, 1, 0, 0, P->Nt, nullptr, Ax, true, nullptr);
+            run_forward<double>(P, x, 1, 0, 0, P->Nt, nullptr, Ax, nullptr);
             const double an = std::sqrt(device_norm2<double>(ctx, Ax, ny));
             const double ratio = an / std::sqrt(xn2);
             if (std::fabs(ratio - ratio_old) / ratio < tol) break;
             ratio_old = ratio;
-            run_adjoint<double>(P, Ax, 0, P->Nt, x, true);
+            run_adjoint<double>(P, Ax, 0, P->Nt, x);
             const double n2 = device_norm2<double>(ctx, x, nx);
             MSLR_REQUIRE(n2 > 0 && std::isfinite(n2), MSLR_ERR_NUMERIC,
                          "power iteration produced a zero or non-finite vector");
@@ -858,7 +985,7 @@ mslr_status_t mslr_sense_poweriter(mslr_sense plan, int niter, double tol, doubl
         }
         if (it > niter) it = niter;
         // sigma1 = ||A x|| / ||x|| at the returned x.
-        run_forward<double>(P, x, 1, 0, 0, P->Nt, nullptr, Ax, true, nullptr);
+        run_forward<double>(P, x, 1, 0, 0, P->Nt, nullptr, Ax, nullptr);
         const double an = std::sqrt(device_norm2<double>(ctx, Ax, ny));
         *sigma1 = an / std::sqrt(device_norm2<double>(ctx, x, nx));
         if (iters) *iters = it;

@@ -15,32 +15,37 @@ struct mslr_sense_s {
     int sgn = -1;     // exponent sign of the forward DFT
     double scale = 1.0;
     mslr::FftDesc d[3];
-    int LBx = 1;      // yz-lines per workgroup in the x-axis passes
+    int LBx = 1;      // sampled lines per workgroup in the x-axis passes
+    int nlb = 1;      // line blocks per (coil, frame): cdiv(max lines per frame, LBx)
     int nt_yz = 64;   // threads per workgroup in the yz-plane passes
     size_t chunk_bytes = 0;
     bool use_fixed = true;  // compile-time FFT lengths where available
 
     // device arrays
-    mslr::c64 *smaps32 = nullptr;   // smaps * pre-phase, (V, Nc)
+    mslr::c64 *smaps32 = nullptr;   // smaps * pre-phase, plane-major (Nyz, Nx, Nc)
     mslr::c128 *smaps64 = nullptr;
     mslr::c64 *tw32[3] = {nullptr, nullptr, nullptr};   // exp(+2 pi i m / n)
     mslr::c128 *tw64[3] = {nullptr, nullptr, nullptr};
     mslr::c64 *ph32[3] = {nullptr, nullptr, nullptr};   // post-phase per axis
     mslr::c128 *ph64[3] = {nullptr, nullptr, nullptr};
-    int *ptr = nullptr;    // (Nx+1, Nt): first internal sample of plane kx in frame t
-    int *off = nullptr;    // (K, Nt): ky + Ny*kz of internal sample j
-    int *perm = nullptr;   // (K, Nt): reference index of internal sample j
+    // Sampled lines: the distinct (ky, kz) of a frame having at least one sample.
+    int *lptr = nullptr;     // (Nt+1): first global line of frame t
+    int *line_yz = nullptr;  // (lines): ky + Ny*kz, ascending within a frame
+    int *kbeg = nullptr;     // (lines): first internal sample of the line in its frame
+    int *lk = nullptr;       // (K, Nt): l*Nx + kx of internal sample j (l: line of the frame)
+    std::vector<int> lptr_h; // host copy of lptr
     std::vector<void *> allocs;
 };
 
 namespace mslr {
 
-// Internal sample order: frame-major, then kx plane, then ky + Ny*kz.
-// y_int = y_ref permuted, both (K, Nc, Nt).
+// Sample order: frame-major, then sampled line (ascending ky + Ny*kz), then kx,
+// which is the reference's column-major order; kept as an explicit step
+// (a copy when the buffers differ) for callers holding reference-order data.
 void sense_ref_to_internal(mslr_sense_s *P, const c64 *y_ref, c64 *y_int);
 
 // g = A'(A sum_k X_k - y_int), frames 0..Nt-1, everything in internal sample
-// order.  dc_partial (device, nullable) receives Nt*Nc*Nx partial ||r||^2.
+// order.  dc_partial (device, nullable) receives Nt*Nc*nlb partial ||r||^2.
 void sense_grad_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
                          const c64 *y_int, c64 *g, double *dc_partial);
 

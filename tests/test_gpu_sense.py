@@ -174,3 +174,69 @@ def test_fixed_and_runtime_fft_paths_agree(ctx, rng, N, monkeypatch):
     x2 = A2.adjoint(y2).to_host()
     assert rel_err(y1, y2) < 2e-6
     assert rel_err(x1, x2) < 2e-6
+
+
+def _line_mask(rng, N, Nt, nlines):
+    """Fully sampled kx lines at nlines random (ky, kz) per frame."""
+    Nx, Ny, Nz = N
+    mask = np.zeros((Nx, Ny * Nz, Nt), dtype=bool)
+    for t in range(Nt):
+        mask[:, rng.permutation(Ny * Nz)[:nlines], t] = True
+    return mask.reshape(Nx, Ny, Nz, Nt, order="F")
+
+
+@pytest.mark.parametrize("N", [(90, 90, 60), (12, 12, 10), (9, 15, 5)])
+def test_line_mask_vs_oracle(ctx, rng, N):
+    """Readout-line masks (the acquisition's own pattern): every sampled line is whole."""
+    from mslr_hip.sense import SenseOp
+    Nc, Nt = 3, 2
+    smaps, _ = _problem(rng, N, Nc, Nt)
+    mask = _line_mask(rng, N, Nt, max(1, N[1] * N[2] // 8))
+    A = SenseOp(mask, smaps, ctx=ctx)
+    x = crandn(rng, (*N, Nt))
+    sm = smaps.astype(np.complex128)
+    assert rel_err(A.forward(x).to_host(), O.A_forward(x.astype(np.complex128), mask, sm)) < 2e-6
+    r = crandn(rng, (A.K, Nc, Nt))
+    assert rel_err(A.adjoint(r).to_host(), O.A_adjoint(r.astype(np.complex128), mask, sm)) < 2e-6
+
+
+def test_frame_chunks_agree(ctx, rng, monkeypatch):
+    """A 1 MB chunk budget (about one frame per chunk) gives the single-chunk operator."""
+    from mslr_hip.sense import SenseOp
+    N, Nc, Nt, S = (32, 32, 24), 3, 5, 2
+    smaps, mask = _problem(rng, N, Nc, Nt, frac=0.3)
+    X = crandn(rng, (*N, Nt, S))
+    y = crandn(rng, (int(mask[..., 0].sum()), Nc, Nt))
+    A1 = SenseOp(mask, smaps, ctx=ctx)
+    g1, dc1 = A1.grad(X, y, want_cost=True)
+    y1 = A1.forward(X[..., 0]).to_host()
+    monkeypatch.setenv("MSLR_FFT_CHUNK_MB", "1")
+    A2 = SenseOp(mask, smaps, ctx=ctx)
+    g2, dc2 = A2.grad(X, y, want_cost=True)
+    y2 = A2.forward(X[..., 0]).to_host()
+    assert np.array_equal(y1, y2)
+    assert np.array_equal(g1.to_host(), g2.to_host())
+    assert dc1 == dc2
+
+
+def test_frames_with_different_line_counts(ctx, rng):
+    """Equal sample counts per frame spread over different numbers of lines."""
+    from mslr_hip.sense import SenseOp
+    N, Nc, Nt = (12, 12, 10), 2, 3
+    Nx = N[0]
+    smaps, _ = _problem(rng, N, Nc, Nt)
+    K = 4 * Nx
+    mask = np.zeros((Nx, N[1] * N[2], Nt), dtype=bool)
+    mask[:, rng.permutation(N[1] * N[2])[:4], 0] = True          # 4 whole lines
+    flat = np.zeros(Nx * N[1] * N[2], dtype=bool)                 # K scattered samples
+    flat[rng.permutation(flat.size)[:K]] = True
+    mask[..., 1] = flat.reshape(Nx, N[1] * N[2], order="F")
+    mask[: Nx // 2, rng.permutation(N[1] * N[2])[:8], 2] = True   # 8 half lines
+    mask = mask.reshape(*N, Nt, order="F")
+    assert all(int(mask[..., t].sum()) == K for t in range(Nt))
+    A = SenseOp(mask, smaps, ctx=ctx)
+    x = crandn(rng, (*N, Nt))
+    sm = smaps.astype(np.complex128)
+    assert rel_err(A.forward(x).to_host(), O.A_forward(x.astype(np.complex128), mask, sm)) < 2e-6
+    r = crandn(rng, (K, Nc, Nt))
+    assert rel_err(A.adjoint(r).to_host(), O.A_adjoint(r.astype(np.complex128), mask, sm)) < 2e-6
