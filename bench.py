@@ -35,20 +35,27 @@ def log(msg: str) -> None:
 
 
 # --------------------------------------------------------------------------- algorithmic work
-def kernel_bytes_per_iter(cfg, K, chunk_frames):
-    """Algorithmic HBM bytes of each kernel per POGM iteration (inputs read once, outputs written once)."""
+def kernel_bytes_per_iter(cfg, K, lines_total, nchunk=1):
+    """Algorithmic HBM bytes of each kernel per POGM iteration (inputs read once, outputs
+    written once).  lines_total: sampled (ky, kz) lines summed over frames; nchunk: frame
+    chunks of the SENSE passes (coil maps are read once per chunk)."""
     N, Nt, Nc = cfg["N"], cfg["Nt"], cfg["Nc"]
     V = int(np.prod(N))
     S = len(cfg["patch_sizes"])
     b = 8
-    nchunk = math.ceil(Nt / chunk_frames)
+    lines_b = Nc * N[0] * lines_total * b      # sampled-line intermediate, all frames
     out = {
-        # scale sum + coil multiply + x-FFT: X (S scales) in, smaps in (once per launch), planes out
-        "fwd_x": S * V * Nt * b + nchunk * Nc * V * b + Nc * V * Nt * b,
-        # planes in, y in, residual out, sample offsets in
-        "fwd_yz": Nc * V * Nt * b + 2 * K * Nc * Nt * b + K * Nt * 4,
-        "adj_yz": K * Nc * Nt * b + K * Nt * 4 + Nc * V * Nt * b,
-        "adj_x": Nc * V * Nt * b + nchunk * Nc * V * b + V * Nt * b,
+        # S scale images in, plane-major sum out
+        "sum_planes": S * V * Nt * b + V * Nt * b,
+        # image planes in, coil maps in, sampled lines out
+        "fwd_yz": V * Nt * b + nchunk * Nc * V * b + lines_b,
+        # lines in, y in, residual out, sample table in
+        "fwd_x": lines_b + 2 * K * Nc * Nt * b + K * Nt * 4,
+        # residual in, sample table in, lines out
+        "adj_x": K * Nc * Nt * b + K * Nt * 4 + lines_b,
+        # lines in, coil maps in, image planes out
+        "adj_yz": lines_b + nchunk * Nc * V * b + V * Nt * b,
+        "planes_to_image": 2 * V * Nt * b,
         # X, U in and out (all scales), g in
         "pogm_update": 4 * S * V * Nt * b + V * Nt * b,
     }
@@ -238,8 +245,11 @@ def main() -> int:
 
     roof = None
     if kern:
-        chunk = max(1, int((512 << 20) // (Nc * int(np.prod(N)) * 8)))
-        by = kernel_bytes_per_iter(cfg, A.K, chunk)
+        lines_total = int(pb.mask.reshape(N[0], N[1] * N[2], Nt, order="F").any(axis=0).sum())
+        budget = 4 << 30
+        nchunk = max(math.ceil(Nc * N[0] * lines_total * 8 / budget),
+                      math.ceil(int(np.prod(N)) * Nt * 8 / budget))
+        by = kernel_bytes_per_iter(cfg, A.K, lines_total, nchunk)
         cand = {k: v for k, v in kern.items() if k in by}
         # per-scale tags ("tridiag@s2") summed per kernel for the dominant-kernel figure
         per_kernel = {}

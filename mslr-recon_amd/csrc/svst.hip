@@ -491,134 +491,166 @@ __device__ __forceinline__ c128 wave_sum(c128 v) {
     return v;
 }
 
-// Eigenpairs of G above tau = beta^2 -> U (m x r, ComplexF32), filter factors f, r.
+// Eigenpairs of the tridiagonal T of each patch above tau = beta^2, in three
+// launches so that large problems spread over many workgroups:
+//   k_eig_count   : rank r = #eigenvalues > tau (Sturm count), Gershgorin bounds;
+//   k_eig_vectors : one thread per eigenvalue: bisection, then its eigenvector
+//                   by a twisted factorisation;
+//   k_eig_clusters: tight clusters redone by inverse iteration, filter factors.
 // r = -1 marks beta == 0 (identity).
-__global__ __launch_bounds__(256) void k_eigvec(
-    const c128 *__restrict__ Gall, const double *__restrict__ dall, const double *__restrict__ eall,
-    const c128 *__restrict__ tall, int m, const double *__restrict__ beta, int beta_per_patch,
-    double *__restrict__ lamall, double *__restrict__ Zall, double *__restrict__ wsall,
-    c64 *__restrict__ Uall, float *__restrict__ fall, int *__restrict__ rall) {
-    __shared__ int s_r;
-    __shared__ double s_tnorm, s_pivmin, s_gl, s_gu;
-    const int ip = blockIdx.x;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const double *d = dall + (int64_t)ip * m;
-    const double *e = eall + (int64_t)ip * m;
-    double *lam = lamall + (int64_t)ip * m;
-    double *Z = Zall + (int64_t)ip * m * m;
+constexpr int EIG_INFO = 4;  // per patch: tnorm, pivmin, upper Gershgorin bound
+
+__global__ __launch_bounds__(64) void k_eig_count(const double *__restrict__ dall,
+                                                  const double *__restrict__ eall, int m, int64_t nb,
+                                                  const double *__restrict__ beta,
+                                                  int beta_per_patch, double *__restrict__ info,
+                                                  int *__restrict__ rall) {
+    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= nb) return;
+    const double *d = dall + ip * m;
+    const double *e = eall + ip * m;
     const float b32 = (float)beta[beta_per_patch ? ip : 0];
     const double bd = (double)b32;
     if (bd == 0.0) {
-        if (tid == 0) rall[ip] = -1;
+        rall[ip] = -1;
         return;
     }
     const double tau = bd * bd;
-    if (tid == 0) {
-        double tn = 0.0, emax = 0.0, gl = INFINITY, gu = -INFINITY;
-        for (int i = 0; i < m; ++i) {
-            const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
-            const double er = i + 1 < m ? fabs(e[i]) : 0.0;
-            tn = fmax(tn, fabs(d[i]) + el + er);
-            gl = fmin(gl, d[i] - el - er);
-            gu = fmax(gu, d[i] + el + er);
-            if (i + 1 < m) emax = fmax(emax, e[i] * e[i]);
-        }
-        const double pivmin = DBL_MIN * fmax(1.0, emax);
-        s_tnorm = tn;
-        s_pivmin = pivmin;
-        s_gl = gl;
-        s_gu = gu + 2.0 * DBL_EPSILON * fmax(fabs(gl), fabs(gu)) + pivmin;
-        int r = 0;
-        if (tau < s_gu) r = m - sturm_count(d, e, m, tau, pivmin);
-        s_r = r;
-        rall[ip] = r;
+    double tn = 0.0, emax = 0.0, gl = INFINITY, gu = -INFINITY;
+    for (int i = 0; i < m; ++i) {
+        const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
+        const double er = i + 1 < m ? fabs(e[i]) : 0.0;
+        tn = fmax(tn, fabs(d[i]) + el + er);
+        gl = fmin(gl, d[i] - el - er);
+        gu = fmax(gu, d[i] + el + er);
+        if (i + 1 < m) emax = fmax(emax, e[i] * e[i]);
     }
-    __syncthreads();
-    const int r = s_r;
-    if (r == 0) return;
-    const double pivmin = s_pivmin, tnorm = s_tnorm, gu = s_gu;
-    // bisection: lam[k] = k-th largest eigenvalue (k < r), all above tau
-    for (int k = tid; k < r; k += nt) {
-        const int idx = m - 1 - k;  // ascending index
-        double lo = tau, hi = gu;
-        for (int it = 0; it < 200; ++it) {
-            const double mid = 0.5 * (lo + hi);
-            if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
-            if (mid <= lo || mid >= hi) break;
-            if (sturm_count(d, e, m, mid, pivmin) > idx)
-                hi = mid;
-            else
-                lo = mid;
-        }
-        lam[k] = 0.5 * (lo + hi);
+    const double pivmin = DBL_MIN * fmax(1.0, emax);
+    gu = gu + 2.0 * DBL_EPSILON * fmax(fabs(gl), fabs(gu)) + pivmin;
+    int r = 0;
+    if (tau < gu) r = m - sturm_count(d, e, m, tau, pivmin);
+    double *inf = info + ip * EIG_INFO;
+    inf[0] = tn;
+    inf[1] = pivmin;
+    inf[2] = gu;
+    rall[ip] = r;
+}
+
+// grid (patches, cdiv(m, 64)); thread k = blockIdx.y*64 + tid < r.  lam[k] is
+// the k-th largest eigenvalue; its eigenvector goes to Z column k, with the
+// bottom-up pivots kept in U's column k (free until the back-transformation).
+__global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ dall,
+                                                    const double *__restrict__ eall, int m,
+                                                    const double *__restrict__ beta,
+                                                    int beta_per_patch,
+                                                    const double *__restrict__ info,
+                                                    const int *__restrict__ rall,
+                                                    double *__restrict__ lamall,
+                                                    double *__restrict__ Zall,
+                                                    c64 *__restrict__ Uall) {
+    const int ip = blockIdx.x;
+    const int r = rall[ip];
+    const int k = blockIdx.y * 64 + threadIdx.x;
+    if (k >= r) return;
+    const double *d = dall + (int64_t)ip * m;
+    const double *e = eall + (int64_t)ip * m;
+    const double *inf = info + (int64_t)ip * EIG_INFO;
+    const double tnorm = inf[0], pivmin = inf[1], gu = inf[2];
+    const double bd = (double)(float)beta[beta_per_patch ? ip : 0];
+    const double tau = bd * bd;
+    const int idx = m - 1 - k;  // ascending index
+    double lo = tau, hi = gu;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+        if (mid <= lo || mid >= hi) break;
+        if (sturm_count(d, e, m, mid, pivmin) > idx)
+            hi = mid;
+        else
+            lo = mid;
     }
-    __syncthreads();
-    // Eigenvectors of T.  A twisted factorisation T - lam I = N_r D N_r^T
-    // (top-down and bottom-up LDL^T, twist where |gamma_r| is smallest) gives
-    // each eigenvector in one O(m) solve, one thread per eigenvector.  Runs of
-    // eigenvalues closer than cltol (tight clusters) are then redone: every
-    // member after the first by inverse iteration from a distinct start with
-    // re-orthogonalisation against the earlier members (one thread per cluster).
+    const double lk = 0.5 * (lo + hi);
+    lamall[(int64_t)ip * m + k] = lk;
+    const double tiny = DBL_EPSILON * fmax(tnorm, DBL_MIN);
+    double *z = Zall + (int64_t)ip * m * m + (int64_t)k * m;
+    double *dm = reinterpret_cast<double *>(Uall + (int64_t)ip * m * m + (int64_t)k * m);
+    twisted_vector(d, e, m, lk, tiny, z, dm, 1);
+}
+
+// One workgroup of NSLOT threads per patch.  Runs of eigenvalues closer than
+// cltol (tight clusters) are redone: every member after the first by inverse
+// iteration from a distinct start with re-orthogonalisation against the
+// earlier members (one thread per cluster).  Then f_k = max(s_k - beta, 0)/s_k
+// (src/recon.jl:195-196).
+__global__ __launch_bounds__(NSLOT) void k_eig_clusters(
+    const double *__restrict__ dall, const double *__restrict__ eall, int m,
+    const double *__restrict__ beta, int beta_per_patch, const double *__restrict__ info,
+    const int *__restrict__ rall, const double *__restrict__ lamall, double *__restrict__ Zall,
+    double *__restrict__ wsall, float *__restrict__ fall) {
+    const int ip = blockIdx.x;
+    const int r = rall[ip];
+    if (r <= 0) return;
+    const int tid = threadIdx.x;
+    const double *d = dall + (int64_t)ip * m;
+    const double *e = eall + (int64_t)ip * m;
+    const double *lam = lamall + (int64_t)ip * m;
+    double *Z = Zall + (int64_t)ip * m * m;
+    const double tnorm = info[(int64_t)ip * EIG_INFO];
+    const double bd = (double)(float)beta[beta_per_patch ? ip : 0];
     const double cltol = 1e-12 * tnorm;
     const double pertol = 10.0 * DBL_EPSILON * fmax(tnorm, DBL_MIN);
     const double tiny = DBL_EPSILON * fmax(tnorm, DBL_MIN);
     double *ws = wsall + (int64_t)ip * WS_PER * m;
-    for (int k = tid; k < r; k += nt) {
-        double *z = Z + (int64_t)k * m;
-        twisted_vector(d, e, m, lam[k], tiny, z, ws + tid, nt);
-    }
-    __syncthreads();
-    if (tid < NSLOT) {
-        double *xs = ws + tid;
-        double *u0 = ws + (int64_t)NSLOT * m + tid;
-        double *u1 = ws + (int64_t)2 * NSLOT * m + tid;
-        double *u2 = ws + (int64_t)3 * NSLOT * m + tid;
-        for (int k0 = tid; k0 < r; k0 += NSLOT) {
-            if (k0 > 0 && lam[k0 - 1] - lam[k0] <= cltol) continue;  // not a cluster start
-            int k1 = k0 + 1;
-            while (k1 < r && lam[k1 - 1] - lam[k1] <= cltol) ++k1;
-            double lprev = lam[k0];
-            for (int k = k0 + 1; k < k1; ++k) {
-                double lk = lam[k];
-                if (lprev - lk < pertol) lk = lprev - pertol;
-                lprev = lk;
-                for (int i = 0; i < m; ++i) xs[i * NSLOT] = hash_unit((uint32_t)i, (uint32_t)k);
-                for (int it = 0; it < 3; ++it) {
-                    tri_solve(d, e, m, lk, tiny, xs, u0, u1, u2, NSLOT);
-                    for (int q = k0; q < k; ++q) {
-                        const double *zq = Z + (int64_t)q * m;
-                        double dot = 0.0;
-                        for (int i = 0; i < m; ++i) dot += zq[i] * xs[i * NSLOT];
-                        for (int i = 0; i < m; ++i) xs[i * NSLOT] -= dot * zq[i];
-                    }
-                    double amax = 0.0;
-                    for (int i = 0; i < m; ++i) amax = fmax(amax, fabs(xs[i * NSLOT]));
-                    if (amax == 0.0) amax = 1.0;
-                    double nrm = 0.0;
-                    for (int i = 0; i < m; ++i) {
-                        const double t = xs[i * NSLOT] / amax;
-                        nrm += t * t;
-                    }
-                    nrm = amax * sqrt(nrm);
-                    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-                    for (int i = 0; i < m; ++i) xs[i * NSLOT] *= inv;
+    double *xs = ws + tid;
+    double *u0 = ws + (int64_t)NSLOT * m + tid;
+    double *u1 = ws + (int64_t)2 * NSLOT * m + tid;
+    double *u2 = ws + (int64_t)3 * NSLOT * m + tid;
+    for (int k0 = tid; k0 < r; k0 += NSLOT) {
+        if (k0 > 0 && lam[k0 - 1] - lam[k0] <= cltol) continue;  // not a cluster start
+        int k1 = k0 + 1;
+        while (k1 < r && lam[k1 - 1] - lam[k1] <= cltol) ++k1;
+        double lprev = lam[k0];
+        for (int k = k0 + 1; k < k1; ++k) {
+            double lk = lam[k];
+            if (lprev - lk < pertol) lk = lprev - pertol;
+            lprev = lk;
+            for (int i = 0; i < m; ++i) xs[i * NSLOT] = hash_unit((uint32_t)i, (uint32_t)k);
+            for (int it = 0; it < 3; ++it) {
+                tri_solve(d, e, m, lk, tiny, xs, u0, u1, u2, NSLOT);
+                for (int q = k0; q < k; ++q) {
+                    const double *zq = Z + (int64_t)q * m;
+                    double dot = 0.0;
+                    for (int i = 0; i < m; ++i) dot += zq[i] * xs[i * NSLOT];
+                    for (int i = 0; i < m; ++i) xs[i * NSLOT] -= dot * zq[i];
                 }
-                double *zk = Z + (int64_t)k * m;
-                for (int i = 0; i < m; ++i) zk[i] = xs[i * NSLOT];
+                double amax = 0.0;
+                for (int i = 0; i < m; ++i) amax = fmax(amax, fabs(xs[i * NSLOT]));
+                if (amax == 0.0) amax = 1.0;
+                double nrm = 0.0;
+                for (int i = 0; i < m; ++i) {
+                    const double t = xs[i * NSLOT] / amax;
+                    nrm += t * t;
+                }
+                nrm = amax * sqrt(nrm);
+                const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+                for (int i = 0; i < m; ++i) xs[i * NSLOT] *= inv;
             }
+            double *zk = Z + (int64_t)k * m;
+            for (int i = 0; i < m; ++i) zk[i] = xs[i * NSLOT];
         }
     }
+    // cluster members may be rewritten by other threads: all are done before f
     __syncthreads();
-    // filter factors f_k = max(s_k - beta, 0)/s_k (src/recon.jl:195-196)
     float *f = fall + (int64_t)ip * m;
-    for (int k = tid; k < r; k += nt) {
+    for (int k = tid; k < r; k += NSLOT) {
         const double sg = sqrt(fmax(lam[k], 0.0));
         f[k] = (float)(sg > bd ? (sg - bd) / sg : 0.0);
     }
 }
 
 // Back-transformation U = Q Z (Q = H(0) ... H(m-2)) for the r eigenvectors of
-// each patch.  G lanes own one eigenvector (element i at lane i mod G, in
+// each patch; grid (patches, cdiv(m, 256/G)), one chunk of eigenvectors per
+// workgroup.  G lanes own one eigenvector (element i at lane i mod G, in
 // registers); blocks of BTR reflectors are staged in LDS, and each reflector
 // costs one G-lane shuffle reduction -- no workgroup barrier per reflector.
 constexpr int BT_MAXE = 40;  // eigenvector elements per lane
@@ -637,13 +669,15 @@ __global__ __launch_bounds__(256) void k_backtransform(const c128 *__restrict__ 
     const int ip = blockIdx.x;
     const int r = rall[ip];
     if (r <= 0) return;
+    const int c0 = blockIdx.y * CH;  // this workgroup's eigenvectors c0 .. c0+CH-1
+    if (c0 >= r) return;
     const int tid = threadIdx.x;
     const int kl = tid / G, t = tid % G;
     const c128 *A = Gall + (int64_t)ip * m * m;
     const c128 *tv = tall + (int64_t)ip * m;
     const double *Z = Zall + (int64_t)ip * m * m;
     c64 *U = Uall + (int64_t)ip * m * m;
-    for (int c0 = 0; c0 < r; c0 += CH) {
+    {
         const int k = c0 + kl;
         const bool active = k < r;
         c128 z[BT_MAXE];
@@ -898,11 +932,12 @@ struct SvstWork {
     c64 *U;
     float *f;
     int *r;
+    double *info;  // EIG_INFO doubles per patch
 };
 
 static size_t svst_bytes_per_patch(int64_t m, int64_t nkc) {
     return (size_t)(2 * TNB * m * 16 + nkc * m * m * 16 + m * m * 16 + m * 8 * 3 + m * 16 + m * m * 8 +
-                    (int64_t)WS_PER * m * 8 + m * m * 8 + m * 4 + 16);
+                    (int64_t)WS_PER * m * 8 + m * m * 8 + m * 4 + 16 + EIG_INFO * 8);
 }
 
 static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
@@ -934,6 +969,7 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     size_t o_f = off; off += al((size_t)chunk * m * 4);
     size_t o_r = off; off += al((size_t)chunk * 4);
     size_t o_VW = off; off += al((size_t)chunk * 2 * TNB * m * 16);
+    size_t o_info = off; off += al((size_t)chunk * EIG_INFO * 8);
     char *base = (char *)ctx->scratch_get(SLOT_GRAM, off);
     w.part = (void *)(base + o_part);
     w.G = (c128 *)(base + o_G);
@@ -947,6 +983,7 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     w.f = (float *)(base + o_f);
     w.r = (int *)(base + o_r);
     w.VW = (c128 *)(base + o_VW);
+    w.info = (double *)(base + o_info);
     return w;
 }
 
@@ -993,7 +1030,8 @@ static void launch_backtransform(mslr_ctx_s *ctx, const SvstWork &w, int m, int6
                                           hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024)); \
             attr = true;                                                                       \
         }                                                                                      \
-        k_backtransform<G_><<<(unsigned)nb, 256, lds, ctx->stream>>>(w.G, w.tau, w.Z, w.r, m, w.U); \
+        dim3 g_((unsigned)nb, (unsigned)cdiv(m, 256 / G_));                                    \
+        k_backtransform<G_><<<g_, 256, lds, ctx->stream>>>(w.G, w.tau, w.Z, w.r, m, w.U);       \
     }
     if (m <= 8 * BT_MAXE) BT_LAUNCH(8)
     else if (m <= 16 * BT_MAXE) BT_LAUNCH(16)
@@ -1047,10 +1085,18 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         // singular values near the threshold under rounding error.
         gram_and_tridiag<double>(ctx, w, Pc, p, Nt, nb, err_dev);
         {
+            const double *bp = beta_dev + (per_patch ? ip0 : 0);
             MSLR_PROF(ctx, "eigvec");
-            k_eigvec<<<(unsigned)nb, 256, 0, ctx->stream>>>(w.G, w.d, w.e, w.tau, (int)m,
-                                                           beta_dev + (per_patch ? ip0 : 0), per_patch,
-                                                           w.lam, w.Z, w.ws, w.U, w.f, w.r);
+            k_eig_count<<<(unsigned)cdiv(nb, 64), 64, 0, ctx->stream>>>(w.d, w.e, (int)m, nb, bp,
+                                                                         per_patch, w.info, w.r);
+            HIP_LAUNCH_CHECK();
+            dim3 ge((unsigned)nb, (unsigned)cdiv(m, 64));
+            k_eig_vectors<<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch, w.info, w.r,
+                                                      w.lam, w.Z, w.U);
+            HIP_LAUNCH_CHECK();
+            k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
+                                                                    w.info, w.r, w.lam, w.Z, w.ws,
+                                                                    w.f);
             HIP_LAUNCH_CHECK();
         }
         launch_backtransform(ctx, w, (int)m, nb);
