@@ -733,11 +733,18 @@ __global__ __launch_bounds__(256) void k_backtransform(const c128 *__restrict__ 
     }
 }
 
-// Batched complex GEMM C = A * B (FP32) on 64x64 tiles, 4x4 outputs per thread.
+// Batched complex GEMM C = A * B (FP32) on the matrix cores.
 // A(i,k) = A[z*bA + i*ai + k*ak] * fs[z*bfs + k] (fs optional),
-// B(k,j) = B[z*bB + k*bk + j*bj] (conjugated when conjB).
+// B(k,j) = B[z*bB + k*bk + j*bj] (conjugated when conjB), C(i,j) = C[z*bC + i*ci + j*cj].
 // K = kvar[z] when kvar is given (kvar[z] < 0 writes the identity), else Kfix.
+// 64x64 output tile per workgroup of four waves, 32x32 per wave as 2x2 blocks of
+// 16x16; each complex block is four v_mfma_f32_16x16x4f32 per k-step of 4 (real
+// and imaginary parts as separate real products).  The MFMA operands are fed
+// as (B^T, A^T) so that the accumulator lanes run down the columns of C.
+// Lane layout of v_mfma_f32_16x16x4f32 (D = X Y, X 16x4, Y 4x16): lane l holds
+// X[l%16][l/16], Y[l/16][l%16] and D[4*(l/16)+v][l%16], v = 0..3.
 constexpr int MT = 64, MK = 16;
+typedef float mf4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_t bA, int64_t ai,
                                                int64_t ak, const c64 *__restrict__ B, int64_t bB,
@@ -746,12 +753,12 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
                                                c64 *__restrict__ C, int64_t bC, int64_t ci,
                                                int64_t cj, int M, int N, int Kfix,
                                                const int *__restrict__ kvar) {
-    __shared__ c64 As[MK][MT + 1];
-    __shared__ c64 Bs[MK][MT + 1];
+    __shared__ float Ar[MK][MT + 4], Ai[MK][MT + 4];
+    __shared__ float Br[MK][MT + 4], Bi[MK][MT + 4];
     const int z = blockIdx.z;
     const int i0 = blockIdx.x * MT, j0 = blockIdx.y * MT;
     const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
+    const int lane = tid & 63, wave = tid >> 6;
     c64 *Cz = C + (int64_t)z * bC;
     int K = Kfix;
     if (kvar) {
@@ -767,13 +774,18 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
     const c64 *Az = A + (int64_t)z * bA;
     const c64 *Bz = B + (int64_t)z * bB;
     const float *fz = fs ? fs + (int64_t)z * bfs : nullptr;
-    c64 acc[4][4];
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;  // wave's sub-tile
+    mf4 accr[2][2], acci[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = c64{0.f, 0.f};
+        for (int bj2 = 0; bj2 < 2; ++bj2) {
+            accr[bi][bj2] = mf4{0.f, 0.f, 0.f, 0.f};
+            acci[bi][bj2] = mf4{0.f, 0.f, 0.f, 0.f};
+        }
     const bool afast_i = ai == 1;
     const bool bfast_k = bk == 1;
+    const int r16 = lane & 15, k4 = lane >> 4;
     for (int k0 = 0; k0 < K; k0 += MK) {
 #pragma unroll
         for (int r = 0; r < (MK * MT) / 256; ++r) {
@@ -795,7 +807,8 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
                     va = c64{va.re * f, va.im * f};
                 }
             }
-            As[kk][ii] = va;
+            Ar[kk][ii] = va.re;
+            Ai[kk][ii] = va.im;
             int kb, jb;
             if (bfast_k) {
                 kb = idx % MK;
@@ -810,33 +823,51 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
                 vb = Bz[kq * bk + j * bj];
                 if (conjB) vb.im = -vb.im;
             }
-            Bs[kb][jb] = vb;
+            Br[kb][jb] = vb.re;
+            Bi[kb][jb] = vb.im;
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < MK; ++kk) {
-            c64 av[4], bv[4];
+        for (int ks = 0; ks < MK; ks += 4) {
+            const int kr = ks + k4;
+            float xr[2], xi[2], xn[2], yr[2], yi[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[kk][tx + 16 * i];
+            for (int bj2 = 0; bj2 < 2; ++bj2) {  // X = B^T block rows j
+                xr[bj2] = Br[kr][wj + 16 * bj2 + r16];
+                xi[bj2] = Bi[kr][wj + 16 * bj2 + r16];
+                xn[bj2] = -xi[bj2];
+            }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][ty + 16 * j];
+            for (int bi = 0; bi < 2; ++bi) {  // Y = A^T block columns i
+                yr[bi] = Ar[kr][wi + 16 * bi + r16];
+                yi[bi] = Ai[kr][wi + 16 * bi + r16];
+            }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    acc[i][j].re += av[i].re * bv[j].re - av[i].im * bv[j].im;
-                    acc[i][j].im += av[i].re * bv[j].im + av[i].im * bv[j].re;
+                for (int bj2 = 0; bj2 < 2; ++bj2) {
+                    mf4 cr = accr[bi][bj2], cim = acci[bi][bj2];
+                    cr = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[bj2], yr[bi], cr, 0, 0, 0);
+                    cr = __builtin_amdgcn_mfma_f32_16x16x4f32(xn[bj2], yi[bi], cr, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[bj2], yi[bi], cim, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[bj2], yr[bi], cim, 0, 0, 0);
+                    accr[bi][bj2] = cr;
+                    acci[bi][bj2] = cim;
                 }
         }
         __syncthreads();
     }
+    // D[j][i] at lane (4*(l/16)+v, l%16): i = l%16 runs down the column of C
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int ii = i0 + tx + 16 * i, jj = j0 + ty + 16 * j;
-            if (ii < M && jj < N) Cz[ii * ci + jj * cj] = acc[i][j];
-        }
+        for (int bj2 = 0; bj2 < 2; ++bj2)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wi + 16 * bi + r16;
+                const int j = j0 + wj + 16 * bj2 + 4 * k4 + v;
+                if (i < M && j < N) Cz[i * ci + j * cj] = c64{accr[bi][bj2][v], acci[bi][bj2][v]};
+            }
 }
 
 // m == 1: each matrix is a vector v (len = p*Nt); SVST = v * max(|v| - beta, 0)/|v|.
