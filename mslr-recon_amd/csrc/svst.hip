@@ -786,18 +786,16 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
     const bool afast_i = ai == 1;
     const bool bfast_k = bk == 1;
     const int r16 = lane & 15, k4 = lane >> 4;
-    for (int k0 = 0; k0 < K; k0 += MK) {
+    constexpr int NR = (MK * MT) / 256;
+    // global -> registers for one k-tile (the next tile is fetched while the
+    // current one is multiplied)
+    c64 ra[NR], rb[NR];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int r = 0; r < (MK * MT) / 256; ++r) {
+        for (int r = 0; r < NR; ++r) {
             const int idx = tid + 256 * r;
-            int kk, ii;
-            if (afast_i) {
-                ii = idx % MT;
-                kk = idx / MT;
-            } else {
-                kk = idx % MK;
-                ii = idx / MK;
-            }
+            const int kk = afast_i ? idx / MT : idx % MK;
+            const int ii = afast_i ? idx % MT : idx / MK;
             const int k = k0 + kk, i = i0 + ii;
             c64 va = {0.f, 0.f};
             if (k < K && i < M) {
@@ -807,26 +805,34 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
                     va = c64{va.re * f, va.im * f};
                 }
             }
-            Ar[kk][ii] = va.re;
-            Ai[kk][ii] = va.im;
-            int kb, jb;
-            if (bfast_k) {
-                kb = idx % MK;
-                jb = idx / MK;
-            } else {
-                jb = idx % MT;
-                kb = idx / MT;
-            }
+            ra[r] = va;
+            const int kb = bfast_k ? idx % MK : idx / MT;
+            const int jb = bfast_k ? idx / MK : idx % MT;
             const int kq = k0 + kb, j = j0 + jb;
             c64 vb = {0.f, 0.f};
             if (kq < K && j < N) {
                 vb = Bz[kq * bk + j * bj];
                 if (conjB) vb.im = -vb.im;
             }
-            Br[kb][jb] = vb.re;
-            Bi[kb][jb] = vb.im;
+            rb[r] = vb;
+        }
+    };
+    if (K > 0) fetch(0);
+    for (int k0 = 0; k0 < K; k0 += MK) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int idx = tid + 256 * r;
+            const int kk = afast_i ? idx / MT : idx % MK;
+            const int ii = afast_i ? idx % MT : idx / MK;
+            Ar[kk][ii] = ra[r].re;
+            Ai[kk][ii] = ra[r].im;
+            const int kb = bfast_k ? idx % MK : idx / MT;
+            const int jb = bfast_k ? idx / MK : idx % MT;
+            Br[kb][jb] = rb[r].re;
+            Bi[kb][jb] = rb[r].im;
         }
         __syncthreads();
+        if (k0 + MK < K) fetch(k0 + MK);
 #pragma unroll
         for (int ks = 0; ks < MK; ks += 4) {
             const int kr = ks + k4;
