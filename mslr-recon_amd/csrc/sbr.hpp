@@ -1,0 +1,45 @@
+// Two-stage reduction of batched Hermitian matrices (FP64) to real tridiagonal
+// form: dense -> band of width SBR_B (block Householder, GEMM-shaped updates),
+// then band -> tridiagonal (bulge chasing in LDS), and the back-transformation
+// of tridiagonal eigenvectors through both stages.
+#pragma once
+
+#include "common.hpp"
+
+namespace mslr {
+
+constexpr int SBR_B = 16;       // bandwidth of the intermediate band matrix
+constexpr int SBR_MINM = 64;    // smaller matrices use the one-stage reduction
+constexpr int SBR_MAXM = 300;   // the band (2*SBR_B diagonals) must fit in LDS
+
+inline bool sbr_eligible(int64_t m) { return m >= SBR_MINM && m <= SBR_MAXM; }
+// panels of the dense -> band stage
+inline int sbr_panels(int m) {
+    int n = 0;
+    for (int c0 = 0; m - c0 - SBR_B >= 2; c0 += SBR_B) ++n;
+    return n;
+}
+// reflector slots per sweep of the band -> tridiagonal stage
+inline int sbr_steps(int m) { return (m - 1 + SBR_B - 1) / SBR_B; }
+
+struct SbrWork {
+    c128 *T1 = nullptr;    // (sbr_panels(m), B, B) per matrix: compact-WY T of each panel
+    c128 *V2 = nullptr;    // (m, sbr_steps(m), B) per matrix: bulge-chasing reflectors
+    c128 *tau2 = nullptr;  // (m, sbr_steps(m)) per matrix
+};
+
+inline size_t sbr_bytes_per_matrix(int m) {
+    return (size_t)sbr_panels(m) * SBR_B * SBR_B * 16 + (size_t)m * sbr_steps(m) * SBR_B * 16 +
+           (size_t)m * sbr_steps(m) * 16;
+}
+
+// G (nb matrices, m x m, full Hermitian storage) -> d, e (real tridiagonal).
+// G keeps the panel reflectors below the band.
+void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, double *d,
+                double *e);
+
+// U[:, k] = Q1 Q2 Z[:, k] for k < r (per matrix), U as ComplexF32 (m x m per matrix).
+void sbr_backtransform(mslr_ctx_s *ctx, const c128 *G, int m, int64_t nb, const SbrWork &w,
+                       const double *Z, const int *r, c64 *U);
+
+}  // namespace mslr

@@ -20,8 +20,10 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 
 #include "reduce.hpp"
+#include "sbr.hpp"
 #include "svst.hpp"
 
 namespace mslr {
@@ -970,11 +972,21 @@ struct SvstWork {
     float *f;
     int *r;
     double *info;  // EIG_INFO doubles per patch
+    bool sbr;      // two-stage tridiagonalisation (sbr.hip)
+    SbrWork sw;
 };
+
+// Tridiagonalisation: MSLR_SVST_REDUCTION = "two" selects the two-stage path
+// (sbr.hip) where the size allows, anything else the one-stage path.
+static bool use_sbr(int64_t m) {
+    const char *e = getenv("MSLR_SVST_REDUCTION");
+    return sbr_eligible(m) && e != nullptr && strcmp(e, "two") == 0;
+}
 
 static size_t svst_bytes_per_patch(int64_t m, int64_t nkc) {
     return (size_t)(2 * TNB * m * 16 + nkc * m * m * 16 + m * m * 16 + m * 8 * 3 + m * 16 + m * m * 8 +
-                    (int64_t)WS_PER * m * 8 + m * m * 8 + m * 4 + 16 + EIG_INFO * 8);
+                    (int64_t)WS_PER * m * 8 + m * m * 8 + m * 4 + 16 + EIG_INFO * 8) +
+           (use_sbr(m) ? sbr_bytes_per_matrix((int)m) : 0);
 }
 
 static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
@@ -1007,6 +1019,14 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     size_t o_r = off; off += al((size_t)chunk * 4);
     size_t o_VW = off; off += al((size_t)chunk * 2 * TNB * m * 16);
     size_t o_info = off; off += al((size_t)chunk * EIG_INFO * 8);
+    w.sbr = use_sbr(m);
+    size_t o_T1 = off, o_V2 = off, o_tau2 = off;
+    if (w.sbr) {
+        const int mi = (int)m;
+        o_T1 = off; off += al((size_t)chunk * sbr_panels(mi) * SBR_B * SBR_B * 16);
+        o_V2 = off; off += al((size_t)chunk * m * sbr_steps(mi) * SBR_B * 16);
+        o_tau2 = off; off += al((size_t)chunk * m * sbr_steps(mi) * 16);
+    }
     char *base = (char *)ctx->scratch_get(SLOT_GRAM, off);
     w.part = (void *)(base + o_part);
     w.G = (c128 *)(base + o_G);
@@ -1021,6 +1041,11 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     w.r = (int *)(base + o_r);
     w.VW = (c128 *)(base + o_VW);
     w.info = (double *)(base + o_info);
+    if (w.sbr) {
+        w.sw.T1 = (c128 *)(base + o_T1);
+        w.sw.V2 = (c128 *)(base + o_V2);
+        w.sw.tau2 = (c128 *)(base + o_tau2);
+    }
     return w;
 }
 
@@ -1049,7 +1074,9 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
                                                        left ? 1 : 0, w.G, err_dev);
         HIP_LAUNCH_CHECK();
     }
-    {
+    if (w.sbr) {
+        sbr_reduce(ctx, w.G, m, nb, w.sw, w.d, w.e);
+    } else {
         MSLR_PROF(ctx, "tridiag");
         k_tridiag_blk<<<(unsigned)nb, TNT, 0, ctx->stream>>>(w.G, m, w.d, w.e, w.tau, w.VW);
         HIP_LAUNCH_CHECK();
@@ -1136,7 +1163,10 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
                                                                     w.f);
             HIP_LAUNCH_CHECK();
         }
-        launch_backtransform(ctx, w, (int)m, nb);
+        if (w.sbr)
+            sbr_backtransform(ctx, w.G, (int)m, nb, w.sw, w.Z, w.r, w.U);
+        else
+            launch_backtransform(ctx, w, (int)m, nb);
         const int mi = (int)m;
         // W = (U diag f) U^H ; r < 0 (beta = 0) gives the identity, r = 0 gives 0
         cgemm(ctx, "form_w", w.U, m * m, 1, m, w.U, m * m, m, 1, 1, w.f, m, Wm, m * m, 1, m, mi, mi,

@@ -171,3 +171,66 @@ def test_patch_svst_vs_oracle(ctx, rng, N, ps, ss):
     out = recon.patchSVST(img, beta, ps, ss).to_host()
     ref = O.patchSVST(img.astype(np.complex128), beta, ps, ss, dtype=np.complex128)
     assert rel_err(out, ref) < 2e-5
+
+
+# Gram sizes handled by the two-stage (dense -> band -> tridiagonal) reduction:
+# 64 <= min(p, Nt) <= 300; the production scales use m = Nt = 300.
+@pytest.fixture
+def two_stage(monkeypatch):
+    monkeypatch.setenv("MSLR_SVST_REDUCTION", "two")
+
+
+@pytest.mark.parametrize("p,n,r", [(64, 200, 3), (100, 77, 4), (1000, 299, 7), (300, 1000, 5),
+                                   (4000, 300, 12), (300, 300, 150)])
+def test_two_stage_sizes_match_lapack(ctx, rng, two_stage, p, n, r):
+    from mslr_hip import recon
+    X = lowrank(rng, p, n, r, noise=1e-2)
+    beta = _beta_between(X, r)
+    Y = recon.SVST(X, beta).to_host()
+    Yo = O.SVST(X.astype(np.complex128), np.float32(beta))
+    assert rel_err(Y, Yo) < 2e-5
+
+
+def test_two_stage_batch_with_mixed_ranks(ctx, rng, two_stage):
+    """A batch of production-shaped patches (1000 x 300) with different ranks kept."""
+    from mslr_hip import recon
+    p, n, Np = 1000, 300, 5
+    ranks = [0, 1, 3, 40, 299]
+    P = np.stack([lowrank(rng, p, n, max(1, k), noise=1e-2) for k in ranks], axis=2)
+    P = np.asfortranarray(P)
+    betas = []
+    for i, k in enumerate(ranks):
+        s = np.linalg.svd(P[:, :, i].astype(np.complex128), compute_uv=False)
+        betas.append(2.0 * s[0] if k == 0 else 0.5 * (s[k - 1] + s[k]))
+    Y = recon.svst_batched(P, np.array(betas)).to_host()
+    for i in range(Np):
+        Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
+        assert rel_err(Y[:, :, i], Yo) < 3e-5, (i, ranks[i])
+
+
+@pytest.mark.parametrize("p,n", [(4000, 300), (500, 120)])
+def test_two_stage_and_one_stage_agree(ctx, rng, p, n, monkeypatch):
+    """Both tridiagonalisations give the same SVST (high dynamic range input)."""
+    from mslr_hip import recon
+    u = crandn(rng, (p, 1), np.complex128)
+    v = crandn(rng, (n, 1), np.complex128)
+    X = (1e4 * (u @ v.conj().T) / np.sqrt(p * n) * np.sqrt(p + n)
+         + crandn(rng, (p, n), np.complex128)).astype(np.complex64)
+    s = np.linalg.svd(X.astype(np.complex128), compute_uv=False)
+    beta = s[n // 3]
+    monkeypatch.setenv("MSLR_SVST_REDUCTION", "two")
+    Y2 = recon.SVST(X, beta).to_host()
+    monkeypatch.setenv("MSLR_SVST_REDUCTION", "one")
+    Y1 = recon.SVST(X, beta).to_host()
+    Yo = O.SVST(X.astype(np.complex128), np.float32(beta))
+    assert rel_err(Y2, Yo) < 2e-5
+    assert rel_err(Y1, Yo) < 2e-5
+    assert rel_err(Y2, Y1) < 2e-5
+
+
+def test_two_stage_nucnorm(ctx, rng, two_stage):
+    from mslr_hip import recon
+    P = np.asfortranarray(np.stack([lowrank(rng, 400, 150, 5, noise=1e-2) for _ in range(3)], axis=2))
+    got = recon.patch_nucnorm(P)
+    ref = sum(np.linalg.svd(P[:, :, i].astype(np.complex128), compute_uv=False).sum() for i in range(3))
+    assert abs(got - ref) / ref < 1e-6
