@@ -130,12 +130,55 @@ static void fill_zero(mslr_ctx_s *ctx, c64 *x, int64_t n) {
     HIP_LAUNCH_CHECK();
 }
 
+// patchSVST of 1x1x1 patches tiling the image (stride 1): each voxel's row over
+// time x_v (1 x Nt) is one patch, its only singular value is ||x_v||, so
+// out_v = x_v max(||x_v|| - beta, 0) / ||x_v|| (beta rounded to Float32 as in
+// src/recon.jl:195).  Equals img2patches -> SVST -> patches2img without the two
+// transposing copies; in place allowed.
+__global__ __launch_bounds__(256) void k_prox_voxel(const c64 *img, c64 *out, int64_t V, int Nt,
+                                                    const double *__restrict__ beta) {
+    const double bd = (double)(float)beta[0];
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        float fk = 1.f;
+        if (bd != 0.0) {
+            double s = 0.0;
+            for (int t = 0; t < Nt; ++t) {
+                const c64 a = img[v + V * t];
+                s += (double)a.re * a.re + (double)a.im * a.im;
+            }
+            const double sg = sqrt(s);
+            fk = sg > bd ? (float)((sg - bd) / sg) : 0.f;
+        }
+        for (int t = 0; t < Nt; ++t) {
+            const c64 a = img[v + V * t];
+            out[v + V * t] = c64{a.re * fk, a.im * fk};
+        }
+    }
+}
+
 // patchSVST for one scale: out <- patches2img(SVST(img2patches(img))).
+// Two layouts need no patch copies: 1x1x1 patches with stride 1 (one voxel per
+// patch), and a single patch covering the image (the patch matrix is the image).
 static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *out,
                        const double *beta_dev, int *err) {
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    if (g.p == 1 && g.Np == V && g.exact_tiling()) {
+        const unsigned nb = (unsigned)std::min<int64_t>(cdiv(V, 256), 1 << 16);
+        MSLR_PROF(ctx, "prox_voxel");
+        k_prox_voxel<<<nb, 256, 0, ctx->stream>>>(img, out, V, g.Nt, beta_dev);
+        HIP_LAUNCH_CHECK();
+        return;
+    }
     const size_t bytes = (size_t)std::max<int64_t>(1, g.p * g.Nt * g.Np) * sizeof(c64);
-    c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, bytes);
     c64 *Y = (c64 *)ctx->scratch_get(SLOT_PATCH_OUT, bytes);
+    if (g.Np == 1 && g.p == V) {
+        svst_batched(ctx, img, Y, g.p, g.Nt, 1, beta_dev, 0, err);
+        MSLR_PROF(ctx, "patch_copy");
+        HIP_CHECK(hipMemcpyAsync(out, Y, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return;
+    }
+    c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, bytes);
     launch_img2patches(ctx, g, img, P, 0, g.Np);
     svst_batched(ctx, P, Y, g.p, g.Nt, g.Np, beta_dev, 0, err);
     launch_patches2img(ctx, g, Y, out);

@@ -10,8 +10,11 @@
 //     columns instead of one per column.
 //   stage 2 (k_sb2st): band -> real tridiagonal by bulge chasing, the band
 //     (2B diagonals) held in LDS; each wave runs one sweep at a time, sweeps
-//     start STAGGER time steps apart so that concurrent tasks touch disjoint
-//     index ranges (one workgroup barrier per time step).
+//     start STAGGER = 2 time steps apart (one workgroup barrier per time step).
+//     Task (s, k) touches entries whose column lies in I_k(s) (plus column s at
+//     k = 0) and reads what tasks (s-1, k) and (s-1, k+1) wrote; the task of
+//     sweep s-1 running beside it, (s-1, k+2), has columns I_{k+2}(s-1), which
+//     begin after I_k(s) ends, so concurrent tasks never share an entry.
 //   back-transformation (k_sbr_bt): U = Q1 Q2 Z for the wanted eigenvectors.
 // Reflectors follow zlarfg: H = I - tau v v^H, H^H x = beta e1, beta real.
 #include <algorithm>
@@ -19,36 +22,24 @@
 #include <cmath>
 
 #include "sbr.hpp"
+#include "wave.hpp"
 
 namespace mslr {
 
 constexpr int B = SBR_B;
 constexpr int S1T = 512;       // threads of the dense -> band kernel
-constexpr int S2T = 512;       // threads of the band -> tridiagonal kernel
+constexpr int S2T = 768;       // threads of the band -> tridiagonal kernel
 constexpr int S2W = S2T / 64;  // concurrent sweeps
-constexpr int STAGGER = 3;     // time steps between the starts of consecutive sweeps
+constexpr int STAGGER = 2;     // time steps between the starts of consecutive sweeps
 constexpr int BTT = 256;       // threads of the back-transformation
 constexpr int BTC = BTT / B;   // eigenvectors per back-transformation workgroup
+constexpr int KG = 8;          // reflectors of one sweep applied together
 
 __device__ __forceinline__ c128 c_div(c128 a, c128 b) {
     const double den = b.re * b.re + b.im * b.im;
     return c128{(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
 }
 
-__device__ __forceinline__ double wsum_d(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ c128 wsum_c(c128 v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        v.re += __shfl_xor(v.re, o, 64);
-        v.im += __shfl_xor(v.im, o, 64);
-    }
-    return v;
-}
-__device__ __forceinline__ c128 shfl_c(c128 v, int src) {
-    return c128{__shfl(v.re, src, 64), __shfl(v.im, src, 64)};
-}
 // An LDS table's address hidden from loop-invariant code motion, so that its
 // entries are re-read per loop iteration instead of held in registers.
 template <class P>
@@ -86,6 +77,18 @@ __device__ __forceinline__ Hh householder(c128 alpha, double xn) {
 }
 __device__ __forceinline__ bool nonzero(c128 t) { return t.re != 0.0 || t.im != 0.0; }
 
+// Value of *p when c, else 0, through scalar loads (a conditionally assigned
+// struct otherwise ends up in private memory).
+__device__ __forceinline__ c128 ld_if(bool c, const c128 *p) {
+    double r = 0.0, i = 0.0;
+    if (c) {
+        r = p->re;
+        i = p->im;
+    }
+    return c128{r, i};
+}
+
+
 // ------------------------------------------------------------------ stage 1
 // One workgroup per matrix.  LDS: V and W, (m - B) x B each, column-major.
 __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, int npanel,
@@ -112,53 +115,54 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             V[j * L + i] = A[(r0 + i) + (int64_t)m * (c0 + j)];
         }
         __syncthreads();
-        // 2. Householder QR of the panel by wave 0; lane l owns rows l + 64t of
-        //    every column, so lanes exchange values only through shuffles.
-        if (wave == 0) {
-            for (int j = 0; j < B; ++j) {
-                c128 *vj = V + j * L;
-                if (j >= kq) {
-                    if (lane == 0) taus[j] = c128{0.0, 0.0};
-                    continue;
-                }
+        // 2. Householder QR of the panel.  Per column j: wave 0 forms the
+        //    reflector (lane l owns rows l + 64t, so it needs only shuffles), then
+        //    all waves apply it to the later columns, one column per wave.
+        for (int j = 0; j < kq; ++j) {
+            c128 *vj = V + j * L;
+            if (wave == 0) {
                 double xn = 0.0;
                 c128 own = {0.0, 0.0};
                 for (int i = lane; i < L; i += 64) {
                     if (i > j) xn += abs2(vj[i]);
                     if (i == j) own = vj[i];
                 }
-                xn = wsum_d(xn);
-                const c128 alpha = shfl_c(own, j & 63);
+                xn = wave_sum64(xn);
+                const c128 alpha = {readlane_d(own.re, j & 63), readlane_d(own.im, j & 63)};
                 const Hh h = householder(alpha, xn);
-                const bool refl = nonzero(h.tau);
-                if (refl)
+                if (nonzero(h.tau))
                     for (int i = lane; i < L; i += 64)
                         if (i > j) vj[i] = vj[i] * h.scal;
                 if (lane == 0) {
                     taus[j] = h.tau;
                     betas[j] = h.beta;
                 }
-                if (!refl) continue;
-                // columns c > j: x <- H^H x = x - conj(tau) v (v^H x)
-                const c128 ct = conj(h.tau);
-                for (int c = j + 1; c < B; ++c) {
+            }
+            __syncthreads();
+            const c128 tj = taus[j];
+            if (nonzero(tj)) {
+                // column c > j: x <- H^H x = x - conj(tau) v (v^H x)
+                const c128 ct = conj(tj);
+                for (int c = j + 1 + wave; c < B; c += S1T / 64) {
                     c128 *pc = V + c * L;
-                    c128 s = {0.0, 0.0};
-                    for (int i = lane; i < L; i += 64) {
-                        if (i < j) continue;
-                        const c128 vi = i == j ? c128{1.0, 0.0} : vj[i];
-                        s = s + cmulc(vi, pc[i]);
+                    c128 sacc = {0.0, 0.0};
+                    for (int i = j + lane; i < L; i += 64) {
+                        c128 vi = ld_if(i != j, vj + i);
+                        if (i == j) vi.re = 1.0;
+                        sacc = sacc + cmulc(vi, pc[i]);
                     }
-                    s = wsum_c(s);
-                    const c128 f = ct * s;
-                    for (int i = lane; i < L; i += 64) {
-                        if (i < j) continue;
-                        const c128 vi = i == j ? c128{1.0, 0.0} : vj[i];
+                    sacc = wave_sum64(sacc);
+                    const c128 f = ct * sacc;
+                    for (int i = j + lane; i < L; i += 64) {
+                        c128 vi = ld_if(i != j, vj + i);
+                        if (i == j) vi.re = 1.0;
                         pc[i] = pc[i] - vi * f;
                     }
                 }
             }
+            __syncthreads();
         }
+        for (int j = kq + tid; j < B; j += S1T) taus[j] = c128{0.0, 0.0};
         __syncthreads();
         // 3. R -> the band, reflector tails -> below the band; V -> unit lower form
         for (int idx = tid; idx < L * B; idx += S1T) {
@@ -185,9 +189,9 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
         for (int pr = tid; pr < B * B; pr += S1T) {
             const int a = pr % B, c = pr / B;
             if (a < c) {
-                c128 s = {0.0, 0.0};
-                for (int i = 0; i < L; ++i) s = s + cmulc(V[a * L + i], V[c * L + i]);
-                Ms[a][c] = s;
+                c128 sacc = {0.0, 0.0};
+                for (int i = 0; i < L; ++i) sacc = sacc + cmulc(V[a * L + i], V[c * L + i]);
+                Ms[a][c] = sacc;
             }
         }
         __syncthreads();
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
                 const c128 tmp = lane < j ? (-1.0) * (tj * Ms[lane][j]) : c128{0.0, 0.0};
                 c128 acc = {0.0, 0.0};
                 for (int q = 0; q < j; ++q) {
-                    const c128 tq = shfl_c(tmp, q);
+                    const c128 tq = {readlane_d(tmp.re, q), readlane_d(tmp.im, q)};
                     if (lane <= q) acc = acc + Ts[lane][q] * tq;
                 }
                 if (lane < j) Ts[lane][j] = acc;
@@ -225,27 +229,33 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             for (int t = 0; t < 8; ++t) W[(8 * h + t) * L + i] = acc[t];
         }
         __syncthreads();
-        // 6. Y <- Y T (row by row, T upper)
-        for (int i = tid; i < L; i += S1T) {
-            c128 (*T_)[B + 1] = lds_opaque(Ts);
-            c128 y[B];
+        // 6. Y <- Y T (T upper); outputs computed first, written after a barrier
+        {
+            constexpr int NTASK = (SBR_MAXM * B + S1T - 1) / S1T;
+            c128 res[NTASK];
 #pragma unroll
-            for (int a = 0; a < B; ++a) y[a] = W[a * L + i];
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
+            for (int u = 0; u < NTASK; ++u) {
+                const int task = tid + u * S1T;
+                const int i = task % L, c = task / L;
                 c128 acc = {0.0, 0.0};
+                if (task < L * B)
+                    for (int a = 0; a <= c; ++a) acc = acc + W[a * L + i] * Ts[a][c];
+                res[u] = acc;
+            }
+            __syncthreads();
 #pragma unroll
-                for (int a = 0; a <= c; ++a) acc = acc + y[a] * T_[a][c];
-                W[c * L + i] = acc;
+            for (int u = 0; u < NTASK; ++u) {
+                const int task = tid + u * S1T;
+                if (task < L * B) W[(task / L) * L + task % L] = res[u];
             }
         }
         __syncthreads();
         // 7. X = V^H Y, M = T^H X (Hermitian)
         for (int pr = tid; pr < B * B; pr += S1T) {
             const int a = pr % B, c = pr / B;
-            c128 s = {0.0, 0.0};
-            for (int i = 0; i < L; ++i) s = s + cmulc(V[a * L + i], W[c * L + i]);
-            Xs[a][c] = s;
+            c128 sacc = {0.0, 0.0};
+            for (int i = 0; i < L; ++i) sacc = sacc + cmulc(V[a * L + i], W[c * L + i]);
+            Xs[a][c] = sacc;
         }
         __syncthreads();
         for (int pr = tid; pr < B * B; pr += S1T) {
@@ -255,19 +265,13 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             Ms[a][c] = s;
         }
         __syncthreads();
-        // 8. W = Y - 1/2 V M
-        for (int i = tid; i < L; i += S1T) {
-            c128 (*M_)[B + 1] = lds_opaque(Ms);
-            c128 v[B];
+        // 8. W = Y - 1/2 V M (one output per thread and step)
+        for (int task = tid; task < L * B; task += S1T) {
+            const int i = task % L, c = task / L;
+            c128 acc = {0.0, 0.0};
 #pragma unroll
-            for (int a = 0; a < B; ++a) v[a] = V[a * L + i];
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
-                c128 acc = {0.0, 0.0};
-#pragma unroll
-                for (int a = 0; a < B; ++a) acc = acc + v[a] * M_[a][c];
-                W[c * L + i] = W[c * L + i] - 0.5 * acc;
-            }
+            for (int a = 0; a < B; ++a) acc = acc + V[a * L + i] * Ms[a][c];
+            W[c * L + i] = W[c * L + i] - 0.5 * acc;
         }
         __syncthreads();
         // 9. A22 <- A22 - V W^H - W V^H (all L x L entries, 4x4 per thread)
@@ -280,8 +284,7 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const int i = i0 + a, k = k0 + c;
-                    acc[a][c] = (i < L && k < L) ? A[(r0 + i) + (int64_t)m * (r0 + k)]
-                                                 : c128{0.0, 0.0};
+                    acc[a][c] = ld_if(i < L && k < L, A + (r0 + i) + (int64_t)m * (r0 + k));
                 }
             for (int j = 0; j < B; ++j) {
                 c128 vi[4], wi[4], vk[4], wk[4];
@@ -312,103 +315,123 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
 }
 
 // ------------------------------------------------------------------ stage 2
-// Band element (i, j), i >= j, i - j < 2B, stored at Bd[(i - j) * m + j].
-__device__ __forceinline__ c128 &bref(c128 *Bd, int m, int i, int j) { return Bd[(i - j) * m + j]; }
+// Band element (i, j), i >= j, i - j < 2B, stored column by column at
+// Bd[j * 2B + (i - j)]: along a row the stride is 2B - 1 entries, down a column
+// 1, both free of LDS bank conflicts for 16 consecutive lanes.
+__device__ __forceinline__ c128 &bref(c128 *Bd, int m, int i, int j) { return Bd[j * (2 * B) + (i - j)]; }
 // Element (q, r) of the Hermitian block starting at (st, st).
 __device__ __forceinline__ c128 bherm(c128 *Bd, int m, int st, int q, int r) {
-    return q >= r ? bref(Bd, m, st + q, st + r) : conj(bref(Bd, m, st + r, st + q));
+    if (q >= r) return bref(Bd, m, st + q, st + r);
+    return conj(bref(Bd, m, st + r, st + q));
 }
 
-// One task of sweep s at step k on the wave's lanes.  H_k (reflector on rows
-// st .. st+len-1, st = s + 1 + k B) is in sv / tau (k > 0) or is made here from
+// One task of sweep s at step k on the wave's lanes; returns the tau of H_{k+1}.
+// H_k (reflector on rows st .. st+len-1, st = s + 1 + k B) is in sv / tau (k > 0) or is made here from
 // column s (k = 0).  Then: two-sided H_k on the diagonal block, right H_k on the
 // block below (E, rows st+B ..), a new reflector H_{k+1} from E's first column,
 // and its left application to the rest of E.
-__device__ void sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw, c128 &tau,
+__device__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw, c128 *sx, c128 tau,
                            c128 *V2, c128 *tau2, int ks) {
     const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, g = lane >> 4;  // row of 16 lanes g, term index r16
     const int st = s + 1 + k * B;
     const int len = min(B, m - st);
     if (k == 0) {
-        const c128 x = lane < len ? bref(Bd, m, st + lane, s) : c128{0.0, 0.0};
-        const double xn = wsum_d(lane >= 1 && lane < len ? abs2(x) : 0.0);
-        const c128 alpha = shfl_c(x, 0);
+        const c128 x = ld_if(lane < len, &bref(Bd, m, st + lane, s));
+        const double xn = wave_sum64(lane >= 1 && lane < len ? abs2(x) : 0.0);
+        const c128 alpha = {readlane_d(x.re, 0), readlane_d(x.im, 0)};
         const Hh h = householder(alpha, xn);
         tau = h.tau;
         const bool refl = nonzero(h.tau);
-        const c128 v = lane == 0 ? c128{1.0, 0.0} : (refl ? x * h.scal : c128{0.0, 0.0});
+        const c128 xs = x * h.scal;
+        const c128 v = {lane == 0 ? 1.0 : (refl ? xs.re : 0.0), lane == 0 ? 0.0 : (refl ? xs.im : 0.0)};
         wave_lds_sync();
-        if (lane < B) sv[lane] = lane < len ? v : c128{0.0, 0.0};
+        const c128 vl = {lane < len ? v.re : 0.0, lane < len ? v.im : 0.0};
+        if (lane < B) sv[lane] = vl;
         if (lane == 0) bref(Bd, m, st, s) = c128{h.beta, 0.0};
         else if (lane < len && refl) bref(Bd, m, st + lane, s) = c128{0.0, 0.0};
-        if (lane < B) V2[((int64_t)s * ks) * B + lane] = lane < len ? v : c128{0.0, 0.0};
+        if (lane < B) V2[((int64_t)s * ks) * B + lane] = vl;
         if (lane == 0) tau2[(int64_t)s * ks] = h.tau;
         wave_lds_sync();
     }
     const int st2 = st + B;
     const int len2 = min(B, m - st2);
     if (nonzero(tau)) {
-        // (a) D <- H^H D H on the diagonal block: p = tau D v, w = p - 1/2 tau (p^H v) v
-        c128 pq = {0.0, 0.0};
-        if (lane < len)
-            for (int r = 0; r < len; ++r) pq = pq + bherm(Bd, m, st, lane, r) * sv[r];
-        pq = tau * pq;
-        const c128 vq = lane < len ? sv[lane] : c128{0.0, 0.0};
-        const c128 dot = wsum_c(cmulc(pq, vq));
-        const c128 alpha2 = (-0.5) * (tau * dot);
-        const c128 wq = pq + alpha2 * vq;
-        if (lane < B) sw[lane] = wq;
-        wave_lds_sync();
-        for (int idx = lane; idx < B * B; idx += 64) {
-            const int q = idx / B, r = idx % B;
-            if (q < len && r <= q) {
-                c128 &d = bref(Bd, m, st + q, st + r);
-                d = d - sv[q] * conj(sw[r]) - sw[q] * conj(sv[r]);
+        const c128 vr = ld_if(r16 < len, sv + r16);
+        // (a)+(b): outputs o = 4t + g; o < 16: tau (D v)_o, o >= 16: tau (E v)_(o-16),
+        // E = A[st2 .., st ..]; each a product over the row's 16 lanes and one row sum.
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int o = 4 * t + g;
+            c128 prod = c128{0.0, 0.0};
+            if (o < 16) {
+                if (o < len && r16 < len) prod = bherm(Bd, m, st, o, r16) * vr;
+            } else if (o - 16 < len2 && r16 < len) {
+                prod = bref(Bd, m, st2 + o - 16, st + r16) * vr;
+            }
+            prod = tau * sum16(prod);
+            if (r16 == 0) {
+                if (o < 16) sw[o] = prod;
+                else sx[o - 16] = prod;
             }
         }
         wave_lds_sync();
-        // (b) E <- E H = E - tau (E v) v^H, E = A[st2 .., st ..]
-        if (len2 > 0) {
-            c128 qi = {0.0, 0.0};
-            if (lane < len2)
-                for (int r = 0; r < len; ++r) qi = qi + bref(Bd, m, st2 + lane, st + r) * sv[r];
-            qi = tau * qi;
-            if (lane < B) sw[lane] = qi;
-            wave_lds_sync();
-            for (int idx = lane; idx < B * B; idx += 64) {
-                const int i = idx / B, r = idx % B;
+        // D <- H^H D H: w = p - 1/2 tau (p^H v) v
+        const c128 pq = ld_if(r16 < len, sw + r16);
+        const c128 dot = sum16(cmulc(pq, vr));
+        const c128 alpha2 = (-0.5) * (tau * dot);
+        wave_lds_sync();
+        if (lane < B) sw[lane] = pq + alpha2 * vr;
+        wave_lds_sync();
+        // D -= v w^H + w v^H (lower part); E <- E H = E - (tau E v) v^H
+        for (int idx = lane; idx < 2 * B * B; idx += 64) {
+            if (idx < B * B) {
+                const int q = idx / B, r = idx % B;
+                if (q < len && r <= q) {
+                    c128 &d = bref(Bd, m, st + q, st + r);
+                    d = d - sv[q] * conj(sw[r]) - sw[q] * conj(sv[r]);
+                }
+            } else {
+                const int i = (idx - B * B) / B, r = idx % B;
                 if (i < len2 && r < len) {
                     c128 &e = bref(Bd, m, st2 + i, st + r);
-                    e = e - sw[i] * conj(sv[r]);
+                    e = e - sx[i] * conj(sv[r]);
                 }
             }
-            wave_lds_sync();
         }
+        wave_lds_sync();
     }
-    if (len2 <= 0) return;
+    if (len2 <= 0) return tau;
     // (c) H_{k+1} from E's first column
-    const c128 x = lane < len2 ? bref(Bd, m, st2 + lane, st) : c128{0.0, 0.0};
-    const double xn = wsum_d(lane >= 1 && lane < len2 ? abs2(x) : 0.0);
-    const c128 alpha = shfl_c(x, 0);
+    const c128 x = ld_if(lane < len2, &bref(Bd, m, st2 + lane, st));
+    const double xn = wave_sum64(lane >= 1 && lane < len2 ? abs2(x) : 0.0);
+    const c128 alpha = {readlane_d(x.re, 0), readlane_d(x.im, 0)};
     const Hh h = householder(alpha, xn);
     const bool refl = nonzero(h.tau);
-    const c128 v = lane == 0 ? c128{1.0, 0.0} : (refl ? x * h.scal : c128{0.0, 0.0});
+    const c128 xs = x * h.scal;
+    const c128 v = {lane == 0 ? 1.0 : (refl ? xs.re : 0.0), lane == 0 ? 0.0 : (refl ? xs.im : 0.0)};
     wave_lds_sync();
-    if (lane < B) sv[lane] = lane < len2 ? v : c128{0.0, 0.0};
+    const c128 vl = {lane < len2 ? v.re : 0.0, lane < len2 ? v.im : 0.0};
+    if (lane < B) sv[lane] = vl;
     if (lane == 0) bref(Bd, m, st2, st) = c128{h.beta, 0.0};
     else if (lane < len2 && refl) bref(Bd, m, st2 + lane, st) = c128{0.0, 0.0};
     const int64_t slot = (int64_t)s * ks + k + 1;
-    if (lane < B) V2[slot * B + lane] = lane < len2 ? v : c128{0.0, 0.0};
+    if (lane < B) V2[slot * B + lane] = vl;
     if (lane == 0) tau2[slot] = h.tau;
     tau = h.tau;
     wave_lds_sync();
-    // (d) E[:, 1..] <- H_{k+1}^H E[:, 1..] = E - conj(tau') v (v^H E)
+    // (d) E[:, 1..] <- H_{k+1}^H E[:, 1..] = E - conj(tau') v (v^H E): output o = 4t + g
     if (refl) {
-        c128 sr = {0.0, 0.0};
-        if (lane >= 1 && lane < len)
-            for (int i = 0; i < len2; ++i) sr = sr + cmulc(sv[i], bref(Bd, m, st2 + i, st + lane));
-        sr = conj(h.tau) * sr;
-        if (lane < B) sw[lane] = sr;
+        const c128 vi = ld_if(r16 < len2, sv + r16);
+        const c128 ct = conj(h.tau);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int o = 4 * t + g;
+            c128 prod = c128{0.0, 0.0};
+            if (o >= 1 && o < len && r16 < len2) prod = cmulc(vi, bref(Bd, m, st2 + r16, st + o));
+            prod = ct * sum16(prod);
+            if (r16 == 0) sw[o] = prod;
+        }
         wave_lds_sync();
         for (int idx = lane; idx < B * B; idx += 64) {
             const int i = idx / B, r = idx % B;
@@ -419,6 +442,7 @@ __device__ void sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw, c1
         }
         wave_lds_sync();
     }
+    return tau;
 }
 
 __global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, int m, int ks,
@@ -427,17 +451,18 @@ __global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, in
                                                c128 *__restrict__ tau2all) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c128 *Bd = reinterpret_cast<c128 *>(smem_raw);  // 2B x m
-    c128 *scr = Bd + (size_t)2 * B * m;              // per wave: sv[B], sw[B]
+    c128 *scr = Bd + (size_t)2 * B * m;              // per wave: sv[B], sw[B], sx[B]
     const int ip = blockIdx.x;
     const int tid = threadIdx.x, wave = tid >> 6;
     const c128 *A = Gall + (int64_t)ip * m * m;
     for (int idx = tid; idx < 2 * B * m; idx += S2T) {
-        const int dg = idx / m, j = idx % m;
-        Bd[idx] = (dg <= B && j + dg < m) ? A[(j + dg) + (int64_t)m * j] : c128{0.0, 0.0};
+        const int j = idx / (2 * B), dg = idx % (2 * B);
+        Bd[idx] = ld_if(dg <= B && j + dg < m, A + (j + dg) + (int64_t)m * j);
     }
     __syncthreads();
-    c128 *sv = scr + wave * 2 * B;
+    c128 *sv = scr + wave * 3 * B;
     c128 *sw = sv + B;
+    c128 *sx = sw + B;
     c128 *V2 = V2all + (int64_t)ip * m * ks * B;
     c128 *tau2 = tau2all + (int64_t)ip * m * ks;
     c128 tau = {0.0, 0.0};
@@ -449,7 +474,7 @@ __global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, in
             const int k = t - STAGGER * s;
             const int nst = (m - s - 1 + B - 1) / B;
             if (k >= 0 && k < nst) {
-                sb2st_task(Bd, m, s, k, sv, sw, tau, V2, tau2, ks);
+                tau = sb2st_task(Bd, m, s, k, sv, sw, sx, tau, V2, tau2, ks);
                 if (k == nst - 1) s += S2W;
             }
         }
@@ -458,8 +483,8 @@ __global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, in
     double *d = dall + (int64_t)ip * m;
     double *e = eall + (int64_t)ip * m;
     for (int i = tid; i < m; i += S2T) {
-        d[i] = Bd[i].re;
-        e[i] = i + 1 < m ? Bd[m + i].re : 0.0;
+        d[i] = Bd[i * 2 * B].re;
+        e[i] = i + 1 < m ? Bd[i * 2 * B + 1].re : 0.0;
     }
 }
 
@@ -490,32 +515,52 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
     const double *Z = Zall + (int64_t)ip * m * m;
     for (int idx = tid; idx < BTC * m; idx += BTT) {
         const int c = idx / m, i = idx % m;
-        Uc[idx] = c < ncol ? c128{Z[(int64_t)(c0 + c) * m + i], 0.0} : c128{0.0, 0.0};
+        c128 z = {0.0, 0.0};
+        if (c < ncol) z.re = Z[(int64_t)(c0 + c) * m + i];
+        Uc[idx] = z;
     }
     __syncthreads();
-    // Q2 (sweep-major product) applied in reverse: z <- z - tau v (v^H z)
+    // Q2 (sweep-major product) applied in reverse: z <- z - tau v (v^H z).
+    // The reflectors of nsw sweeps at a time are staged in the (free) panel area.
     {
         const int q = tid % B, c = tid / B;
         c128 *u = Uc + c * m;
         const c128 *V2 = V2all + (int64_t)ip * m * ks * B;
         const c128 *tau2 = tau2all + (int64_t)ip * m * ks;
-        for (int s = m - 2; s >= 0; --s) {
-            const int nst = (m - s - 1 + B - 1) / B;
-            for (int k = nst - 1; k >= 0; --k) {
-                const int64_t slot = (int64_t)s * ks + k;
-                const c128 tau = tau2[slot];
-                if (!nonzero(tau)) continue;
-                const int st = s + 1 + k * B;
-                const int len = min(B, m - st);
-                const c128 v = q < len ? V2[slot * B + q] : c128{0.0, 0.0};
-                const c128 x = q < len ? u[st + q] : c128{0.0, 0.0};
-                c128 dot = cmulc(v, x);
+        const int nsw = max(1, ((m - B) * B) / (ks * (B + 1)));
+        c128 *vb = Vs;
+        c128 *tb = Vs + (size_t)nsw * ks * B;
+        for (int shi = m - 2; shi >= 0; shi -= nsw) {
+            const int slo = max(0, shi - nsw + 1);
+            const int nv = (shi - slo + 1) * ks;
+            __syncthreads();
+            for (int idx = tid; idx < nv * B; idx += BTT) vb[idx] = V2[(int64_t)slo * ks * B + idx];
+            for (int idx = tid; idx < nv; idx += BTT) tb[idx] = tau2[(int64_t)slo * ks + idx];
+            __syncthreads();
+            // The reflectors of one sweep act on disjoint row windows and commute:
+            // they are applied together, KG at a time with independent reductions.
+            for (int s = shi; s >= slo; --s) {
+                const int nst = (m - s - 1 + B - 1) / B;
+                for (int k0 = 0; k0 < nst; k0 += KG) {
+                    c128 v[KG], x[KG], tk[KG];
 #pragma unroll
-                for (int o = B / 2; o >= 1; o >>= 1) {
-                    dot.re += __shfl_xor(dot.re, o, 64);
-                    dot.im += __shfl_xor(dot.im, o, 64);
+                    for (int g = 0; g < KG; ++g) {
+                        const int k = k0 + g;
+                        const int slot = (s - slo) * ks + k;
+                        const int st = s + 1 + k * B;
+                        const bool on = k < nst && q < m - st;
+                        tk[g] = ld_if(k < nst, tb + slot);
+                        v[g] = ld_if(on, vb + slot * B + q);
+                        x[g] = ld_if(on, u + st + q);
+                    }
+#pragma unroll
+                    for (int g = 0; g < KG; ++g) {
+                        const c128 dot = sum16(cmulc(v[g], x[g]));
+                        const int k = k0 + g;
+                        const int st = s + 1 + k * B;
+                        if (k < nst && q < m - st) u[st + q] = x[g] - (tk[g] * dot) * v[g];
+                    }
                 }
-                if (q < len) u[st + q] = x - (tau * dot) * v;
                 wave_lds_sync();
             }
         }
@@ -588,6 +633,8 @@ static void raise_lds(const void *fn, size_t need) {
 void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, double *d,
                 double *e) {
     MSLR_REQUIRE(sbr_eligible(m), MSLR_ERR_UNSUPPORTED, "two-stage reduction size");
+    // a wave must finish its sweep before its next one (S2W sweeps later) starts
+    MSLR_REQUIRE(sbr_steps(m) <= STAGGER * S2W, MSLR_ERR_UNSUPPORTED, "sweep longer than the schedule");
     const int npanel = sbr_panels(m), ks = sbr_steps(m);
     {
         const size_t lds = (size_t)2 * (m - B) * B * sizeof(c128);
@@ -597,7 +644,7 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
         HIP_LAUNCH_CHECK();
     }
     {
-        const size_t lds = ((size_t)2 * B * m + (size_t)S2W * 2 * B) * sizeof(c128);
+        const size_t lds = ((size_t)2 * B * m + (size_t)S2W * 3 * B) * sizeof(c128);
         raise_lds((const void *)k_sb2st, lds);
         MSLR_PROF(ctx, "sb2st");
         k_sb2st<<<(unsigned)nb, S2T, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2);

@@ -1,0 +1,51 @@
+// Cross-lane reductions on CDNA wavefronts (64 lanes) without the LDS crossbar:
+// DPP row permutations inside each row of 16 lanes, readlane across the rows.
+#pragma once
+
+#include "common.hpp"
+
+namespace mslr {
+
+// DPP controls (row = 16 lanes): quad_perm patterns, row_half_mirror (lane i of
+// each 8 reads lane 7-i), row_mirror (lane i of each 16 reads lane 15-i).
+constexpr int DPP_QP_1032 = 0xB1;  // quad_perm [1,0,3,2]: lane i reads i^1
+constexpr int DPP_QP_2301 = 0x4E;  // quad_perm [2,3,0,1]: lane i reads i^2
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;
+constexpr int DPP_ROW_MIRROR = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each row of 16 lanes; every lane of the row receives the row sum.
+// (pairs i^1, then i^2, then the mirrored half, then the mirrored row)
+__device__ __forceinline__ double sum16(double v) {
+    v += dpp_d<DPP_QP_1032>(v);
+    v += dpp_d<DPP_QP_2301>(v);
+    v += dpp_d<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_d<DPP_ROW_MIRROR>(v);
+    return v;
+}
+__device__ __forceinline__ C2<double> sum16(C2<double> v) { return {sum16(v.re), sum16(v.im)}; }
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the wave (uniform result): row sums by DPP, then the four rows.
+__device__ __forceinline__ double wave_sum64(double v) {
+    v = sum16(v);
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+__device__ __forceinline__ C2<double> wave_sum64(C2<double> v) {
+    return {wave_sum64(v.re), wave_sum64(v.im)};
+}
+
+}  // namespace mslr
