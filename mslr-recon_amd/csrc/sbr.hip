@@ -326,14 +326,17 @@ __device__ __forceinline__ c128 bherm(c128 *Bd, int m, int st, int q, int r) {
 }
 
 // One task of sweep s at step k on the wave's lanes; returns the tau of H_{k+1}.
-// H_k (reflector on rows st .. st+len-1, st = s + 1 + k B) is in sv / tau (k > 0) or is made here from
-// column s (k = 0).  Then: two-sided H_k on the diagonal block, right H_k on the
-// block below (E, rows st+B ..), a new reflector H_{k+1} from E's first column,
-// and its left application to the rest of E.
+// H_k (reflector on rows st .. st+len-1, st = s + 1 + k B) is in sv / tau
+// (k > 0) or is made here from column s (k = 0).  Then: two-sided H_k on the
+// diagonal block D, right H_k on the block below (E, rows st+B ..), a new
+// reflector H_{k+1} from E's first column and its left application to the
+// rest of E.  D and E stay in registers for the whole task: lane l holds
+// column r = l % 16 of rows q = l / 16 + 4t, t = 0..3; row products are DPP
+// row sums, column products sums over the four rows of lanes.
 __device__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw, c128 *sx, c128 tau,
                            c128 *V2, c128 *tau2, int ks) {
     const int lane = threadIdx.x & 63;
-    const int r16 = lane & 15, g = lane >> 4;  // row of 16 lanes g, term index r16
+    const int r = lane & 15, g = lane >> 4;
     const int st = s + 1 + k * B;
     const int len = min(B, m - st);
     if (k == 0) {
@@ -345,8 +348,8 @@ __device__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw, c1
         const bool refl = nonzero(h.tau);
         const c128 xs = x * h.scal;
         const c128 v = {lane == 0 ? 1.0 : (refl ? xs.re : 0.0), lane == 0 ? 0.0 : (refl ? xs.im : 0.0)};
-        wave_lds_sync();
         const c128 vl = {lane < len ? v.re : 0.0, lane < len ? v.im : 0.0};
+        wave_lds_sync();
         if (lane < B) sv[lane] = vl;
         if (lane == 0) bref(Bd, m, st, s) = c128{h.beta, 0.0};
         else if (lane < len && refl) bref(Bd, m, st + lane, s) = c128{0.0, 0.0};
@@ -356,92 +359,116 @@ __device__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw, c1
     }
     const int st2 = st + B;
     const int len2 = min(B, m - st2);
-    if (nonzero(tau)) {
-        const c128 vr = ld_if(r16 < len, sv + r16);
-        // (a)+(b): outputs o = 4t + g; o < 16: tau (D v)_o, o >= 16: tau (E v)_(o-16),
-        // E = A[st2 .., st ..]; each a product over the row's 16 lanes and one row sum.
+    // D (Hermitian, upper part from the lower) and E into registers
+    c128 Dr[4], Er[4], vq[4];
+    const c128 vr = ld_if(r < len, sv + r);
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int o = 4 * t + g;
-            c128 prod = c128{0.0, 0.0};
-            if (o < 16) {
-                if (o < len && r16 < len) prod = bherm(Bd, m, st, o, r16) * vr;
-            } else if (o - 16 < len2 && r16 < len) {
-                prod = bref(Bd, m, st2 + o - 16, st + r16) * vr;
-            }
-            prod = tau * sum16(prod);
-            if (r16 == 0) {
-                if (o < 16) sw[o] = prod;
-                else sx[o - 16] = prod;
-            }
+    for (int t = 0; t < 4; ++t) {
+        const int q = g + 4 * t;
+        vq[t] = ld_if(q < len, sv + q);
+        Dr[t] = c128{0.0, 0.0};
+        if (q < len && r < len) {
+            if (q >= r) Dr[t] = bref(Bd, m, st + q, st + r);
+            else Dr[t] = conj(bref(Bd, m, st + r, st + q));
         }
-        wave_lds_sync();
-        // D <- H^H D H: w = p - 1/2 tau (p^H v) v
-        const c128 pq = ld_if(r16 < len, sw + r16);
-        const c128 dot = sum16(cmulc(pq, vr));
-        const c128 alpha2 = (-0.5) * (tau * dot);
-        wave_lds_sync();
-        if (lane < B) sw[lane] = pq + alpha2 * vr;
-        wave_lds_sync();
-        // D -= v w^H + w v^H (lower part); E <- E H = E - (tau E v) v^H
-        for (int idx = lane; idx < 2 * B * B; idx += 64) {
-            if (idx < B * B) {
-                const int q = idx / B, r = idx % B;
-                if (q < len && r <= q) {
-                    c128 &d = bref(Bd, m, st + q, st + r);
-                    d = d - sv[q] * conj(sw[r]) - sw[q] * conj(sv[r]);
-                }
-            } else {
-                const int i = (idx - B * B) / B, r = idx % B;
-                if (i < len2 && r < len) {
-                    c128 &e = bref(Bd, m, st2 + i, st + r);
-                    e = e - sx[i] * conj(sv[r]);
-                }
-            }
-        }
-        wave_lds_sync();
+        Er[t] = ld_if(q < len2 && r < len, &bref(Bd, m, st2 + q, st + r));
     }
-    if (len2 <= 0) return tau;
-    // (c) H_{k+1} from E's first column
-    const c128 x = ld_if(lane < len2, &bref(Bd, m, st2 + lane, st));
-    const double xn = wave_sum64(lane >= 1 && lane < len2 ? abs2(x) : 0.0);
-    const c128 alpha = {readlane_d(x.re, 0), readlane_d(x.im, 0)};
-    const Hh h = householder(alpha, xn);
-    const bool refl = nonzero(h.tau);
-    const c128 xs = x * h.scal;
-    const c128 v = {lane == 0 ? 1.0 : (refl ? xs.re : 0.0), lane == 0 ? 0.0 : (refl ? xs.im : 0.0)};
-    wave_lds_sync();
-    const c128 vl = {lane < len2 ? v.re : 0.0, lane < len2 ? v.im : 0.0};
-    if (lane < B) sv[lane] = vl;
-    if (lane == 0) bref(Bd, m, st2, st) = c128{h.beta, 0.0};
-    else if (lane < len2 && refl) bref(Bd, m, st2 + lane, st) = c128{0.0, 0.0};
-    const int64_t slot = (int64_t)s * ks + k + 1;
-    if (lane < B) V2[slot * B + lane] = vl;
-    if (lane == 0) tau2[slot] = h.tau;
-    tau = h.tau;
-    wave_lds_sync();
-    // (d) E[:, 1..] <- H_{k+1}^H E[:, 1..] = E - conj(tau') v (v^H E): output o = 4t + g
-    if (refl) {
-        const c128 vi = ld_if(r16 < len2, sv + r16);
-        const c128 ct = conj(h.tau);
+    if (nonzero(tau)) {
+        // p = tau D v and tau E v (row sums)
+        c128 pd[4], pe[4];
+        c128 part = {0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const int o = 4 * t + g;
-            c128 prod = c128{0.0, 0.0};
-            if (o >= 1 && o < len && r16 < len2) prod = cmulc(vi, bref(Bd, m, st2 + r16, st + o));
-            prod = ct * sum16(prod);
-            if (r16 == 0) sw[o] = prod;
+            pd[t] = tau * sum16(Dr[t] * vr);
+            pe[t] = tau * sum16(Er[t] * vr);
+            part = part + cmulc(pd[t], vq[t]);
+        }
+        // p^H v: the rows of lanes hold disjoint q; add the four row results
+        const c128 dot = {(readlane_d(part.re, 0) + readlane_d(part.re, 16)) +
+                              (readlane_d(part.re, 32) + readlane_d(part.re, 48)),
+                          (readlane_d(part.im, 0) + readlane_d(part.im, 16)) +
+                              (readlane_d(part.im, 32) + readlane_d(part.im, 48))};
+        const c128 alpha2 = (-0.5) * (tau * dot);
+        // w_q for the own rows; w_r for the own column through LDS
+        wave_lds_sync();
+        if (r == 0) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sw[g + 4 * t] = pd[t];
         }
         wave_lds_sync();
-        for (int idx = lane; idx < B * B; idx += 64) {
-            const int i = idx / B, r = idx % B;
-            if (i < len2 && r >= 1 && r < len) {
-                c128 &e = bref(Bd, m, st2 + i, st + r);
-                e = e - sv[i] * sw[r];
+        const c128 wr = ld_if(r < len, sw + r) + alpha2 * vr;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const c128 wq = pd[t] + alpha2 * vq[t];
+            Dr[t] = Dr[t] - vq[t] * conj(wr) - wq * conj(vr);   // D <- H^H D H
+            Er[t] = Er[t] - pe[t] * conj(vr);                   // E <- E H
+        }
+        // D back to the band (lower part)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = g + 4 * t;
+            if (q < len && r <= q) bref(Bd, m, st + q, st + r) = Dr[t];
+        }
+    }
+    if (len2 > 0) {
+        // (c) H_{k+1} from E's first column (exchanged through LDS)
+        wave_lds_sync();
+        if (r == 0) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sx[g + 4 * t] = Er[t];
+        }
+        wave_lds_sync();
+        const c128 x = ld_if(lane < len2, sx + lane);
+        const double xn = wave_sum64(lane >= 1 && lane < len2 ? abs2(x) : 0.0);
+        const c128 alpha = {readlane_d(x.re, 0), readlane_d(x.im, 0)};
+        const Hh h = householder(alpha, xn);
+        const bool refl = nonzero(h.tau);
+        const c128 xs = x * h.scal;
+        const c128 v = {lane == 0 ? 1.0 : (refl ? xs.re : 0.0), lane == 0 ? 0.0 : (refl ? xs.im : 0.0)};
+        const c128 vl = {lane < len2 ? v.re : 0.0, lane < len2 ? v.im : 0.0};
+        if (lane < B) sv[lane] = vl;
+        const int64_t slot = (int64_t)s * ks + k + 1;
+        if (lane < B) V2[slot * B + lane] = vl;
+        if (lane == 0) tau2[slot] = h.tau;
+        tau = h.tau;
+        if (r == 0) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int q = g + 4 * t;
+                if (q == 0) Er[t] = c128{h.beta, 0.0};
+                else if (refl) Er[t] = c128{0.0, 0.0};
             }
         }
         wave_lds_sync();
+        // (d) E[:, 1..] <- H_{k+1}^H E = E - conj(tau') v (v^H E) (column sums)
+        if (refl) {
+            c128 vn[4];
+            c128 cs = {0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                vn[t] = ld_if(g + 4 * t < len2, sv + g + 4 * t);
+                cs = cs + cmulc(vn[t], Er[t]);
+            }
+            cs.re += __shfl_xor(cs.re, 16, 64);
+            cs.im += __shfl_xor(cs.im, 16, 64);
+            cs.re += __shfl_xor(cs.re, 32, 64);
+            cs.im += __shfl_xor(cs.im, 32, 64);
+            cs = conj(h.tau) * cs;
+            if (r >= 1) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) Er[t] = Er[t] - vn[t] * cs;
+            }
+        }
+        // E back to the band
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = g + 4 * t;
+            if (q < len2 && r < len) bref(Bd, m, st2 + q, st + r) = Er[t];
+        }
+    } else if (nonzero(tau)) {
+        // no block below: nothing else to store
     }
+    wave_lds_sync();
     return tau;
 }
 

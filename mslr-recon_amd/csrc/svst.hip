@@ -976,11 +976,11 @@ struct SvstWork {
     SbrWork sw;
 };
 
-// Tridiagonalisation: MSLR_SVST_REDUCTION = "two" selects the two-stage path
-// (sbr.hip) where the size allows, anything else the one-stage path.
+// Tridiagonalisation: the two-stage path (sbr.hip) where the size allows;
+// MSLR_SVST_REDUCTION = "one" selects the one-stage path everywhere.
 static bool use_sbr(int64_t m) {
     const char *e = getenv("MSLR_SVST_REDUCTION");
-    return sbr_eligible(m) && e != nullptr && strcmp(e, "two") == 0;
+    return sbr_eligible(m) && !(e != nullptr && strcmp(e, "one") == 0);
 }
 
 static size_t svst_bytes_per_patch(int64_t m, int64_t nkc) {

@@ -234,3 +234,16 @@ def test_two_stage_nucnorm(ctx, rng, two_stage):
     got = recon.patch_nucnorm(P)
     ref = sum(np.linalg.svd(P[:, :, i].astype(np.complex128), compute_uv=False).sum() for i in range(3))
     assert abs(got - ref) / ref < 1e-6
+
+
+@pytest.mark.parametrize("reduction", ["one", "two"])
+@pytest.mark.parametrize("p,n,r", [(300, 300, 6), (1000, 300, 20), (216, 387, 5)])
+def test_both_reductions_match_lapack(ctx, rng, monkeypatch, reduction, p, n, r):
+    """The one-stage (column by column) and two-stage (band) tridiagonalisations."""
+    from mslr_hip import recon
+    monkeypatch.setenv("MSLR_SVST_REDUCTION", reduction)
+    X = lowrank(rng, p, n, r, noise=1e-2)
+    beta = _beta_between(X, r)
+    Y = recon.SVST(X, beta).to_host()
+    Yo = O.SVST(X.astype(np.complex128), np.float32(beta))
+    assert rel_err(Y, Yo) < 2e-5
