@@ -21,6 +21,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "reduce.hpp"
 #include "sbr.hpp"
@@ -118,6 +119,114 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, i
             const int a = a0 + tx + 16 * i, b = b0 + ty + 16 * j;
             if (a < m && b < m) out[a + (int64_t)m * b] = acc[i][j];
         }
+}
+
+// FP64 Gram tiles on the matrix cores (v_mfma_f64_16x16x4f64): same tiling,
+// staging and output as k_gram_tiles<double>.  Per wave a 32x32 sub-tile as
+// 2x2 blocks of 16x16; each complex block takes four real MFMAs per k-step of
+// 4.  Operands are fed as (X(k, b), conj X(k, a)) so that the accumulator
+// lanes run down the columns of the output.  Lane layout (D = P Q, P 16x4,
+// Q 4x16): lane l holds P[l%16][l/16], Q[l/16][l%16] and D[l/16 + 4v][l%16].
+typedef double md4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_gram_mfma(const c64 *__restrict__ P, int64_t pstride,
+                                                   int64_t ks, int64_t cs, int m, int64_t Kc,
+                                                   int nkc, int ntile, c128 *__restrict__ part) {
+    __shared__ double Ar[GK][GT + 2], Ai[GK][GT + 2];
+    __shared__ double Br[GK][GT + 2], Bi[GK][GT + 2];
+    const int ip = blockIdx.z;
+    const int kc = blockIdx.y;
+    int rem = blockIdx.x, ti = 0;
+    while (rem >= ntile - ti) {
+        rem -= ntile - ti;
+        ++ti;
+    }
+    const int tj = ti + rem;
+    const int a0 = ti * GT, b0 = tj * GT;
+    const c64 *X = P + ip * pstride;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wa = (wave & 1) * 32, wb = (wave >> 1) * 32;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    md4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            accr[i][j] = md4{0.0, 0.0, 0.0, 0.0};
+            acci[i][j] = md4{0.0, 0.0, 0.0, 0.0};
+        }
+    const int64_t kbeg = (int64_t)kc * KCH;
+    const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
+    const bool kfast = ks == 1;
+    for (int64_t k0 = kbeg; k0 < kend; k0 += GK) {
+#pragma unroll
+        for (int r = 0; r < (GK * GT) / 256; ++r) {
+            const int idx = tid + 256 * r;
+            const int kk = kfast ? idx % GK : idx / GT;
+            const int aa = kfast ? idx / GK : idx % GT;
+            const int64_t k = k0 + kk;
+            float var = 0.f, vai = 0.f, vbr = 0.f, vbi = 0.f;
+            if (k < kend) {
+                if (a0 + aa < m) {
+                    const c64 v = X[k * ks + (int64_t)(a0 + aa) * cs];
+                    var = v.re;
+                    vai = v.im;
+                }
+                if (b0 + aa < m) {
+                    const c64 v = X[k * ks + (int64_t)(b0 + aa) * cs];
+                    vbr = v.re;
+                    vbi = v.im;
+                }
+            }
+            Ar[kk][aa] = var;
+            Ai[kk][aa] = vai;
+            Br[kk][aa] = vbr;
+            Bi[kk][aa] = vbi;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kq = 0; kq < GK; kq += 4) {
+            const int kr = kq + k4;
+            double pr[2], pi[2], pn[2], qr[2], qi[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {  // P = X(k, b) block rows b
+                pr[j] = Br[kr][wb + 16 * j + r16];
+                pi[j] = Bi[kr][wb + 16 * j + r16];
+                pn[j] = -pr[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {  // Q = conj X(k, a) block columns a
+                qr[i] = Ar[kr][wa + 16 * i + r16];
+                qi[i] = Ai[kr][wa + 16 * i + r16];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    // x_b conj(x_a): re = br ar + bi ai, im = bi ar - br ai
+                    md4 cr = accr[i][j], ci = acci[i][j];
+                    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[j], qr[i], cr, 0, 0, 0);
+                    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[j], qi[i], cr, 0, 0, 0);
+                    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[j], qr[i], ci, 0, 0, 0);
+                    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(pn[j], qi[i], ci, 0, 0, 0);
+                    accr[i][j] = cr;
+                    acci[i][j] = ci;
+                }
+        }
+        __syncthreads();
+    }
+    c128 *out = part + ((int64_t)ip * nkc + kc) * (int64_t)m * m;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int a = a0 + wa + 16 * i + r16;
+                const int b = b0 + wb + 16 * j + k4 + 4 * v;
+                if (a < m && b < m) out[a + (int64_t)m * b] = c128{accr[i][j][v], acci[i][j][v]};
+            }
 }
 
 // G[a,b] = sum_kc part (FP64), Hermitian fill; conj_out flips to conj(C).
@@ -1062,8 +1171,12 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
     dim3 gg((unsigned)npair, (unsigned)w.nkc, (unsigned)nb);
     {
         MSLR_PROF(ctx, "gram_tiles");
-        k_gram_tiles<AT><<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, ntile,
-                                                      (C2<AT> *)w.part);
+        if constexpr (std::is_same<AT, double>::value)
+            k_gram_mfma<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, ntile,
+                                                     (c128 *)w.part);
+        else
+            k_gram_tiles<AT><<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc,
+                                                          ntile, (C2<AT> *)w.part);
         HIP_LAUNCH_CHECK();
     }
     const int64_t ne = (int64_t)m * m * nb;
