@@ -27,6 +27,9 @@ sys.path.insert(0, str(ROOT))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the kernel
+# families, from the committed rocprofv3 counter summary of this workload
+TRAFFIC_BYTES = {}
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -74,38 +77,58 @@ def kernel_bytes_per_iter(cfg, K, lines_total, nchunk=1):
     return out
 
 
-def gram_flops_per_iter(cfg):
-    """Complex Gram products (full-matrix convention 8*max(p,Nt)*m^2 per patch)."""
+def kernel_flops_per_iter(cfg):
+    """Algorithmic floating-point work (real flops) of the dense SVST kernels per
+    POGM iteration, per kernel family, with the precision they run in."""
     N, Nt = cfg["N"], cfg["Nt"]
-    tot = 0
+    out = {}
     for ps, ss in zip(cfg["patch_sizes"], cfg["strides"]):
         psz = [min(ps[a], N[a]) for a in range(3)]
         ns = [math.ceil((N[a] - psz[a]) / ss[a]) + 1 for a in range(3)]
         Np = ns[0] * ns[1] * ns[2]
         p = psz[0] * psz[1] * psz[2]
         m = min(p, Nt)
-        if m > 1:
-            tot += 8 * max(p, Nt) * m * m * Np
-    return tot
+        if m <= 1:
+            continue
+        K = max(p, Nt)
+        def add(k, v):
+            out[k] = out.get(k, 0) + v
+        add("gram_tiles", 8 * K * m * (m + 1) // 2 * Np)          # Hermitian X^H X, FP64
+        add("sy2sb", 16 * m ** 3 // 3 * Np)                        # Hermitian reduction to band, FP64
+        add("apply", 8 * p * Nt * m * Np)                          # Y = X W, FP32
+    return out
+
+
+FP64_PEAK_TFLOPS = 78.6
+# kernel family -> (bound, unit, peak) of its roofline
+BOUNDS = {
+    "gram_tiles": ("mfma", "TFLOP/s", FP64_PEAK_TFLOPS),
+    "sy2sb": ("mfma", "TFLOP/s", FP64_PEAK_TFLOPS),
+    "apply": ("mfma", "TFLOP/s", FP32_PEAK_TFLOPS),
+}
 
 
 # --------------------------------------------------------------------------- CPU baseline
-def cpu_baseline(cfg, pb, y_norm, smaps, budget_s=30.0):
+def cpu_baseline(cfg, pb, y_norm, smaps, part_s=4.0):
     """The oracle (a restatement of the reference CPU path) timed on samples of one
-    POGM iteration and scaled to a full iteration.  Returns (iter/s, cores, sample text)."""
+    POGM iteration and scaled to a full iteration; each part is sampled for about
+    part_s seconds.  Returns (iter/s, cores, sample text)."""
     import oracle as O
     N, Nt, Nc = tuple(cfg["N"]), cfg["Nt"], cfg["Nc"]
     V = int(np.prod(N))
     cores = O._workers()
-    parts = {}
+    parts, notes = {}, []
     mask = pb.mask
-    # SENSE A and A' for a few frames, all coils
-    fs = 2
-    x = pb.x_true[..., :fs].astype(np.complex64)
+    # SENSE A and A' (residual), frame by frame until part_s
     t = time.perf_counter()
-    r = O.A_forward(x, mask[..., :fs], smaps) - y_norm[:, :, :fs]
-    O.A_adjoint(r, mask[..., :fs], smaps)
+    fs = 0
+    while fs < Nt and (fs == 0 or time.perf_counter() - t < part_s):
+        x = pb.x_true[..., fs:fs + 1].astype(np.complex64)
+        r = O.A_forward(x, mask[..., fs:fs + 1], smaps) - y_norm[:, :, fs:fs + 1]
+        O.A_adjoint(r, mask[..., fs:fs + 1], smaps)
+        fs += 1
     parts["sense"] = (time.perf_counter() - t) * Nt / fs
+    notes.append(f"A+A' on {fs}/{Nt} frames")
     lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
     img = pb.x_true
     for k, (ps, ss) in enumerate(zip(cfg["patch_sizes"], cfg["strides"])):
@@ -114,22 +137,27 @@ def cpu_baseline(cfg, pb, y_norm, smaps, budget_s=30.0):
         beta = 0.5 * lams[k]
         if Np == 1:
             # one global patch: gesdd cost is linear in the rows of a tall matrix
-            rows = max(Nt, p // 10)
+            rows = max(Nt, p // 8)
             X = img.reshape(V, Nt, order="F")[:rows]
             t = time.perf_counter()
             O.SVST(X, beta)
             parts[f"svst_scale{k + 1}"] = (time.perf_counter() - t) * p / rows
+            notes.append(f"scale {k + 1}: gesdd of {rows}/{p} rows")
         else:
-            ns = min(Np, 8 if p > 1 else 4000)
-            # sample patches: first ns patches of the scale
-            t = time.perf_counter()
-            Psub = np.empty((p, Nt, ns), dtype=np.complex64)
+            chunk = 4 if p > 1 else 4000
             it = O._patch_starts(N, psc, ss, nst)
-            for i in range(ns):
-                sx, sy, sz = next(it)
-                Psub[:, :, i] = img[sx:sx + psc[0], sy:sy + psc[1], sz:sz + psc[2], :].reshape(p, Nt, order="F")
-            O.svst_batched(Psub, beta)
-            parts[f"svst_scale{k + 1}"] = (time.perf_counter() - t) * Np / ns
+            t = time.perf_counter()
+            done = 0
+            while done < Np and (done == 0 or time.perf_counter() - t < part_s):
+                n = min(chunk, Np - done)
+                Psub = np.empty((p, Nt, n), dtype=np.complex64)
+                for i in range(n):
+                    sx, sy, sz = next(it)
+                    Psub[:, :, i] = img[sx:sx + psc[0], sy:sy + psc[1], sz:sz + psc[2], :].reshape(p, Nt, order="F")
+                O.svst_batched(Psub, beta)
+                done += n
+            parts[f"svst_scale{k + 1}"] = (time.perf_counter() - t) * Np / done
+            notes.append(f"scale {k + 1}: {done}/{Np} patches")
     # POGM update on a tenth of the state
     S = len(cfg["patch_sizes"])
     n = V * Nt * S // 10
@@ -142,12 +170,11 @@ def cpu_baseline(cfg, pb, y_norm, smaps, budget_s=30.0):
     float(np.sum(np.real(-g * (u - us)), dtype=np.float64))
     del z
     parts["pogm_update"] = (time.perf_counter() - t) * 10
+    notes.append("POGM update on 1/10 of the state")
     tot = sum(parts.values())
-    sample = (f"oracle (NumPy/SciPy restatement of the reference CPU path) on samples of one iteration, "
-              f"scaled: A+A' on {fs}/{Nt} frames; global-patch SVD on 1/10 of the rows; "
-              f"8 of the 10^3 patches and 4000 of the voxel patches (LAPACK gesdd per patch, as the reference); "
-              f"POGM update on 1/10 of the state; parts (s/iter): "
-              + ", ".join(f"{k}={v:.2f}" for k, v in parts.items()))
+    sample = ("oracle (NumPy/SciPy restatement of the reference CPU path, LAPACK gesdd per patch as "
+              "the reference) on samples of one iteration, scaled: " + "; ".join(notes) +
+              "; parts (s/iter): " + ", ".join(f"{k}={v:.2f}" for k, v in parts.items()))
     return 1.0 / tot, cores, sample
 
 
@@ -174,9 +201,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--form", default="plain", choices=["plain", "batched"])
+    ap.add_argument("--form", default="batched", choices=["plain", "batched"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-prof", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--no-prof", action="store_true", help="skip the plain pass with per-kernel event timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) five times, to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -217,57 +244,95 @@ def main() -> int:
     lams = lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
     L = S * 1.0
     W, K = args.warmup, args.steps
-    st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=L,
-              niter=W + K, form=args.form)
-    st.init(yd)
-    ctx.sync()
+    V = int(np.prod(N))
+
+    def make(form, niter):
+        st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=L,
+                  niter=niter, form=form)
+        st.init(yd)
+        ctx.sync()
+        return st
+
+    def timed(st, n):
+        ctx.sync()
+        t0 = time.perf_counter()
+        st.run(n)
+        ctx.sync()
+        return time.perf_counter() - t0
+
+    # the measured form: W untimed iterations, then K timed ones
+    st = make(args.form, W + K)
     t = time.time()
     st.run(W)
     ctx.sync()
-    log(f"warmup {W} iterations in {time.time() - t:.2f}s")
-    prof = not args.no_prof
-    if prof:
-        ctx.prof_enable(True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    st.run(K)
-    ctx.sync()
-    t1 = time.perf_counter()
-    kern = ctx.prof_report() if prof else {}
-    if prof:
-        ctx.prof_enable(False)
-    elapsed = t1 - t0
+    log(f"{args.form}: warmup {W} iterations in {time.time() - t:.2f}s")
+    elapsed = timed(st, K)
     value = K / elapsed
     _, restarts = st.log()
-    log(f"timed {K} iterations in {elapsed:.3f}s -> {value:.3f} iter/s; restarts {restarts}")
-    for kname, (cnt, ms) in sorted(kern.items(), key=lambda kv: -kv[1][1]):
-        log(f"  {kname:14s} launches={cnt:6d} total={ms:9.2f} ms  ({100 * ms / (elapsed * 1e3):5.1f}%)")
+    x_form = st.X.to_host().sum(axis=-1)
+    log(f"{args.form}: timed {K} iterations in {elapsed:.3f}s -> {value:.3f} iter/s; restarts {restarts}")
+    st.close()
+
+    # the plain form on the same problem with the same iteration count (POGM's
+    # last iteration differs): its rate and the difference of the results; then
+    # a second plain run with per-kernel event timing (one stream, so each
+    # kernel's events bracket it alone)
+    plain = None
+    kern = {}
+    if args.form != "plain":
+        sp = make("plain", W + K)
+        sp.run(W)
+        el_p = timed(sp, K)
+        x_plain = sp.X.to_host().sum(axis=-1)
+        sp.close()
+        dn = float(np.linalg.norm(x_form - x_plain) / np.linalg.norm(x_plain))
+        plain = {"value": round(K / el_p, 4), "ms_per_step": round(1e3 * el_p / K, 3),
+                 "delta_nrmse": dn}
+        log(f"plain: {K / el_p:.3f} iter/s; NRMSE({args.form}, plain) = {dn:.3e}")
+    if not args.no_prof:
+        sp = make("plain", W + K)
+        sp.run(W)
+        ctx.prof_enable(True)
+        el_k = timed(sp, K)
+        kern = ctx.prof_report()
+        ctx.prof_enable(False)
+        sp.close()
+        for kname, (cnt, ms) in sorted(kern.items(), key=lambda kv: -kv[1][1]):
+            log(f"  {kname:18s} launches={cnt:6d} total={ms:9.2f} ms  ({100 * ms / (el_k * 1e3):5.1f}%)")
 
     roof = None
     if kern:
         lines_total = int(pb.mask.reshape(N[0], N[1] * N[2], Nt, order="F").any(axis=0).sum())
         budget = 4 << 30
-        nchunk = max(math.ceil(Nc * N[0] * lines_total * 8 / budget),
-                      math.ceil(int(np.prod(N)) * Nt * 8 / budget))
+        nchunk = max(math.ceil(Nc * N[0] * lines_total * 8 / budget), math.ceil(V * Nt * 8 / budget))
         by = kernel_bytes_per_iter(cfg, A.K, lines_total, nchunk)
-        cand = {k: v for k, v in kern.items() if k in by}
-        # per-scale tags ("tridiag@s2") summed per kernel for the dominant-kernel figure
-        per_kernel = {}
+        fl = kernel_flops_per_iter(cfg)
+        # per-scale tags ("sy2sb@s2") summed per kernel family
+        fam = {}
         for kname, (cnt, ms) in kern.items():
             base = kname.split("@")[0]
-            c0, m0 = per_kernel.get(base, (0, 0.0))
-            per_kernel[base] = (c0 + cnt, m0 + ms)
-        dom = max(per_kernel.items(), key=lambda kv: kv[1][1])[0]
-        hb = max(cand.items(), key=lambda kv: kv[1][1])[0] if cand else None
-        if hb is not None:
-            cnt, ms = kern[hb]
-            per_launch = by[hb] * K / cnt
-            ach = per_launch / (ms / cnt / 1e3) / 1e9
-            roof = {"bound": "hbm", "kernel": hb, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                    "bytes_per_launch": int(per_launch), "avg_launch_ms": round(ms / cnt, 4),
-                    "dominant_kernel": dom,
-                    "dominant_share": round(per_kernel[dom][1] / (elapsed * 1e3), 3)}
+            c0, m0 = fam.get(base, (0, 0.0))
+            fam[base] = (c0 + cnt, m0 + ms)
+        total_ms = sum(ms for _, ms in fam.values())
+        dom = max(fam.items(), key=lambda kv: kv[1][1])[0]
+        cnt, ms = fam[dom]
+        per_launch_ms = ms / cnt
+        if dom in fl:
+            bound, unit, peak = BOUNDS[dom]
+            work = fl[dom] * K / cnt
+            ach = work / (per_launch_ms / 1e3) / 1e12
+            roof = {"bound": bound, "achieved": round(ach, 2), "peak": peak, "unit": unit,
+                    "frac": round(ach / peak, 4), "traffic": TRAFFIC_BYTES.get(dom),
+                    "kernel": dom, "flops_per_launch": int(work)}
+        elif dom in by:
+            work = by[dom] * K / cnt
+            ach = work / (per_launch_ms / 1e3) / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES.get(dom),
+                    "kernel": dom, "bytes_per_launch": int(work)}
+        if roof is not None:
+            roof.update({"avg_launch_ms": round(per_launch_ms, 4), "launches_per_iter": cnt / K,
+                         "share_of_kernel_time": round(ms / total_ms, 3)})
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -288,10 +353,12 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64",
+        "dtype": "c64/f64",
         "data": "synthetic",
         "config": {"workload": f"{name}: {N[0]}x{N[1]}x{N[2]} Nt={Nt} {Nc} coils R={cfg['R']} scales={cfg['patch_sizes']}",
                    "form": args.form, "parallelism": "single-gpu", "restarts_timed": sum(restarts[W:])},
+        "plain": plain,
+        "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_per_iter": {k: round(v[1] / K, 3) for k, v in kern.items()},
