@@ -77,6 +77,8 @@ __device__ __forceinline__ Hh householder(c128 alpha, double xn) {
 }
 __device__ __forceinline__ bool nonzero(c128 t) { return t.re != 0.0 || t.im != 0.0; }
 
+typedef double md4 __attribute__((ext_vector_type(4)));
+
 // Value of *p when c, else 0, through scalar loads (a conditionally assigned
 // struct otherwise ends up in private memory).
 __device__ __forceinline__ c128 ld_if(bool c, const c128 *p) {
@@ -211,22 +213,34 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
         }
         __syncthreads();
         for (int pr = tid; pr < B * B; pr += S1T) Tg[(int64_t)p * B * B + pr] = Ts[pr % B][pr / B];
-        // 5. Y = A22 V -> W (two threads per row, eight columns each)
-        for (int task = tid; task < 2 * L; task += S1T) {
-            const int i = task % L, h = task / L;
-            c128 acc[8];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) acc[t] = c128{0.0, 0.0};
-            const c128 *arow = A + (r0 + i) + (int64_t)m * r0;
-            const c128 *vb = V + (8 * h) * L;
+        // 5. Y = A22 V -> W on the FP64 matrix cores: one 16-row block per wave at a
+        //    time; the A22 operand is read from global memory once.
+        //    v_mfma_f64_16x16x4f64 (D = P Q): lane l holds P[l%16][l/16],
+        //    Q[l/16][l%16] and D[l/16 + 4v][l%16].
+        {
+            const int nrb = (L + 15) / 16;
+            const int rl = lane & 15, kl = lane >> 4;
+            for (int rb = wave; rb < nrb; rb += S1T / 64) {
+                md4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+                const int i = rb * 16 + rl;
+                const c128 *arow = A + (r0 + i) + (int64_t)m * r0;
 #pragma unroll 4
-            for (int k = 0; k < L; ++k) {
-                const c128 a = arow[(int64_t)m * k];
+                for (int ks = 0; ks < L; ks += 4) {
+                    const int k = ks + kl;
+                    const c128 a = ld_if(i < L && k < L, arow + (int64_t)m * k);
+                    const c128 v = ld_if(k < L, V + rl * L + k);
+                    // operands swapped: the block computed is Y^T, so the output
+                    // lanes run down the columns of W
+                    yr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, a.re, yr, 0, 0, 0);
+                    yr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, -a.im, yr, 0, 0, 0);
+                    yi = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, a.re, yi, 0, 0, 0);
+                    yi = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, a.im, yi, 0, 0, 0);
+                }
+                if (i < L) {
 #pragma unroll
-                for (int t = 0; t < 8; ++t) acc[t] = acc[t] + a * vb[t * L + k];
+                    for (int q = 0; q < 4; ++q) W[(kl + 4 * q) * L + i] = c128{yr[q], yi[q]};
+                }
             }
-#pragma unroll
-            for (int t = 0; t < 8; ++t) W[(8 * h + t) * L + i] = acc[t];
         }
         __syncthreads();
         // 6. Y <- Y T (T upper); outputs computed first, written after a barrier
@@ -274,41 +288,45 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             W[c * L + i] = W[c * L + i] - 0.5 * acc;
         }
         __syncthreads();
-        // 9. A22 <- A22 - V W^H - W V^H (all L x L entries, 4x4 per thread)
-        const int nbl = (L + 3) / 4;
-        for (int blk = tid; blk < nbl * nbl; blk += S1T) {
-            const int i0 = 4 * (blk % nbl), k0 = 4 * (blk / nbl);
-            c128 acc[4][4];
+        // 9. A22 <- A22 - V W^H - W V^H (all L x L entries) on the FP64 matrix cores,
+        //    one 16x16 block per wave at a time, K = 2B complex.
+        {
+            const int nb = (L + 15) / 16;
+            const int rl = lane & 15, kl = lane >> 4;
+            for (int blk = wave; blk < nb * nb; blk += S1T / 64) {
+                const int bi = blk % nb, bk = blk / nb;
+                md4 dr, di;
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int i = i0 + a, k = k0 + c;
-                    acc[a][c] = ld_if(i < L && k < L, A + (r0 + i) + (int64_t)m * (r0 + k));
+                for (int q = 0; q < 4; ++q) {
+                    const int i = bi * 16 + rl, k = bk * 16 + kl + 4 * q;
+                    const c128 a = ld_if(i < L && k < L, A + (r0 + i) + (int64_t)m * (r0 + k));
+                    dr[q] = a.re;
+                    di[q] = a.im;
                 }
-            for (int j = 0; j < B; ++j) {
-                c128 vi[4], wi[4], vk[4], wk[4];
+                const int ii = bi * 16 + rl, kk = bk * 16 + rl;
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int i = min(i0 + a, L - 1), k = min(k0 + a, L - 1);
-                    vi[a] = V[j * L + i];
-                    wi[a] = W[j * L + i];
-                    vk[a] = V[j * L + k];
-                    wk[a] = W[j * L + k];
+                for (int js = 0; js < B; js += 4) {
+                    const int j = js + kl;
+                    const c128 vI = ld_if(ii < L, V + j * L + ii), wI = ld_if(ii < L, W + j * L + ii);
+                    const c128 vK = ld_if(kk < L, V + j * L + kk), wK = ld_if(kk < L, W + j * L + kk);
+                    // V_I W_K^H + W_I V_K^H: re = vr wr + vi wi, im = vi wr - vr wi.
+                    // Operands swapped (K-side first): the block computed is the
+                    // transpose, so the output lanes run down the columns of A.
+                    dr = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.re, -vI.re, dr, 0, 0, 0);
+                    dr = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.im, -vI.im, dr, 0, 0, 0);
+                    dr = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.re, -wI.re, dr, 0, 0, 0);
+                    dr = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.im, -wI.im, dr, 0, 0, 0);
+                    di = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.re, -vI.im, di, 0, 0, 0);
+                    di = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.im, vI.re, di, 0, 0, 0);
+                    di = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.re, -wI.im, di, 0, 0, 0);
+                    di = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.im, wI.re, di, 0, 0, 0);
                 }
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        acc[a][c] = acc[a][c] - vi[a] * conj(wk[c]) - wi[a] * conj(vk[c]);
+                for (int q = 0; q < 4; ++q) {
+                    const int i = bi * 16 + rl, k = bk * 16 + kl + 4 * q;
+                    if (i < L && k < L) A[(r0 + i) + (int64_t)m * (r0 + k)] = c128{dr[q], di[q]};
+                }
             }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int i = i0 + a, k = k0 + c;
-                    if (i < L && k < L) A[(r0 + i) + (int64_t)m * (r0 + k)] = acc[a][c];
-                }
         }
         __syncthreads();
     }
