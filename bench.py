@@ -92,9 +92,10 @@ def kernel_bytes_per_iter(cfg, K, lines_total, nchunk=1):
     return out
 
 
-def kernel_flops_per_iter(cfg):
+def kernel_flops_per_iter(cfg, split=False):
     """Algorithmic floating-point work (real flops) of the dense SVST kernels per
-    POGM iteration, per kernel family, with the precision they run in."""
+    POGM iteration, per kernel family.  split: the Gram and apply products run as
+    split BF16, three BF16 products per useful product, and are counted as issued."""
     N, Nt = cfg["N"], cfg["Nt"]
     out = {}
     for ps, ss in zip(cfg["patch_sizes"], cfg["strides"]):
@@ -108,19 +109,23 @@ def kernel_flops_per_iter(cfg):
         K = max(p, Nt)
         def add(k, v):
             out[k] = out.get(k, 0) + v
-        add("gram_tiles", 8 * K * m * (m + 1) // 2 * Np)          # Hermitian X^H X, FP64
+        f = 3 if split else 1
+        add("gram_tiles", f * 8 * K * m * (m + 1) // 2 * Np)      # Hermitian X^H X (FP64 or split BF16)
         add("sy2sb", 16 * m ** 3 // 3 * Np)                        # Hermitian reduction to band, FP64
-        add("apply", 8 * p * Nt * m * Np)                          # Y = X W, FP32
+        add("apply", f * 8 * p * Nt * m * Np)                      # Y = X W (FP32 or split BF16)
     return out
 
 
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
 # kernel family -> (bound, unit, peak) of its roofline
 BOUNDS = {
     "gram_tiles": ("mfma", "TFLOP/s", FP64_PEAK_TFLOPS),
     "sy2sb": ("mfma", "TFLOP/s", FP64_PEAK_TFLOPS),
     "apply": ("mfma", "TFLOP/s", FP32_PEAK_TFLOPS),
 }
+BOUNDS_SPLIT = dict(BOUNDS, gram_tiles=("mfma", "TFLOP/s", BF16_PEAK_TFLOPS),
+                    apply=("mfma", "TFLOP/s", BF16_PEAK_TFLOPS))
 
 
 # --------------------------------------------------------------------------- CPU baseline
@@ -216,7 +221,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--form", default="batched", choices=["plain", "batched"])
+    ap.add_argument("--form", default="mixed", choices=["plain", "batched", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-prof", action="store_true", help="skip the plain pass with per-kernel event timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -275,6 +280,13 @@ def main() -> int:
         ctx.sync()
         return time.perf_counter() - t0
 
+    xt_norm = float(np.linalg.norm(pb.x_true))
+
+    def nrmse_truth(x):
+        """NRMSE of a reconstruction (scale sum) against the synthetic truth, in the
+        truth's units (the k-space was divided by sf)."""
+        return float(np.linalg.norm(x * np.float32(sf) - pb.x_true) / xt_norm)
+
     # the measured form: W untimed iterations, then K timed ones
     st = make(args.form, W + K)
     t = time.time()
@@ -288,24 +300,31 @@ def main() -> int:
     log(f"{args.form}: timed {K} iterations in {elapsed:.3f}s -> {value:.3f} iter/s; restarts {restarts}")
     st.close()
 
-    # the plain form on the same problem with the same iteration count (POGM's
-    # last iteration differs): its rate and the difference of the results; then
-    # a second plain run with per-kernel event timing (one stream, so each
-    # kernel's events bracket it alone)
+    # the other forms on the same problem with the same iteration count (POGM's
+    # last iteration differs): their rates and the difference of the results;
+    # then a run of the measured form with per-kernel event timing (scales run
+    # one after another on one stream, so each kernel's events bracket it alone)
     plain = None
+    others = {}
     kern = {}
+    nr_form = nrmse_truth(x_form)
     if args.form != "plain":
-        sp = make("plain", W + K)
-        sp.run(W)
-        el_p = timed(sp, K)
-        x_plain = sp.X.to_host().sum(axis=-1)
-        sp.close()
-        dn = float(np.linalg.norm(x_form - x_plain) / np.linalg.norm(x_plain))
-        plain = {"value": round(K / el_p, 4), "ms_per_step": round(1e3 * el_p / K, 3),
-                 "delta_nrmse": dn}
-        log(f"plain: {K / el_p:.3f} iter/s; NRMSE({args.form}, plain) = {dn:.3e}")
+        for other in (["batched"] if args.form == "mixed" else []) + ["plain"]:
+            sp = make(other, W + K)
+            sp.run(W)
+            el_o = timed(sp, K)
+            x_o = sp.X.to_host().sum(axis=-1)
+            sp.close()
+            rd = float(np.linalg.norm(x_form - x_o) / np.linalg.norm(x_o))
+            nr_o = nrmse_truth(x_o)
+            others[other] = {"value": round(K / el_o, 4), "ms_per_step": round(1e3 * el_o / K, 3),
+                             "nrmse_vs_truth": nr_o, "rel_diff_of_measured_form": rd}
+            log(f"{other}: {K / el_o:.3f} iter/s; NRMSE vs truth {nr_o:.5e}; "
+                f"||x_{args.form} - x_{other}|| / ||x_{other}|| = {rd:.3e}")
+        plain = others["plain"]
+    log(f"{args.form}: NRMSE vs truth {nr_form:.5e}")
     if not args.no_prof:
-        sp = make("plain", W + K)
+        sp = make(args.form, W + K)
         sp.run(W)
         ctx.prof_enable(True)
         el_k = timed(sp, K)
@@ -321,7 +340,9 @@ def main() -> int:
         budget = 4 << 30
         nchunk = max(math.ceil(Nc * N[0] * lines_total * 8 / budget), math.ceil(V * Nt * 8 / budget))
         by = kernel_bytes_per_iter(cfg, A.K, lines_total, nchunk)
-        fl = kernel_flops_per_iter(cfg)
+        split = args.form == "mixed"
+        fl = kernel_flops_per_iter(cfg, split)
+        bounds = BOUNDS_SPLIT if split else BOUNDS
         # per-scale tags ("sy2sb@s2") summed per kernel family
         fam = {}
         for kname, (cnt, ms) in kern.items():
@@ -333,7 +354,7 @@ def main() -> int:
         cnt, ms = fam[dom]
         per_launch_ms = ms / cnt
         if dom in fl:
-            bound, unit, peak = BOUNDS[dom]
+            bound, unit, peak = bounds[dom]
             work = fl[dom] * K / cnt
             ach = work / (per_launch_ms / 1e3) / 1e12
             roof = {"bound": bound, "achieved": round(ach, 2), "peak": peak, "unit": unit,
@@ -368,11 +389,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64/f64",
+        "dtype": "c64/f64" + ("/bf16x2" if args.form == "mixed" else ""),
         "data": "synthetic",
         "config": {"workload": f"{name}: {N[0]}x{N[1]}x{N[2]} Nt={Nt} {Nc} coils R={cfg['R']} scales={cfg['patch_sizes']}",
                    "form": args.form, "parallelism": "single-gpu", "restarts_timed": sum(restarts[W:])},
+        "nrmse_vs_truth": nr_form,
+        "delta_nrmse": abs(nr_form - plain["nrmse_vs_truth"]) if plain else None,
         "plain": plain,
+        "batched_fp32": others.get("batched"),
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,

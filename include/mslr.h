@@ -82,6 +82,18 @@ mslr_status_t mslr_ctx_sync(mslr_ctx ctx);
 mslr_status_t mslr_ctx_info(mslr_ctx ctx, char *name, size_t name_len,
                             size_t *total_bytes, size_t *free_bytes);
 
+/* Products of the SVST Gram matrix and reconstruction GEMM on this context
+ * (mslr_svst_batched, mslr_patch_svst).  No counterpart in the reference, whose
+ * patch SVD runs in FP32 LAPACK (src/recon.jl:183-198): this selects the
+ * mixed-precision form.  FP32 (default): FP64 Gram products of the FP32 data,
+ * FP32 reconstruction.  SPLIT_BF16: each FP32 operand as hi + lo BF16 on the
+ * matrix cores (hi*hi + hi*lo + lo*hi, FP32 accumulation; Gram sums in FP64);
+ * the eigensolve stays FP64.  The POGM MSLR_FORM_MIXED form uses SPLIT_BF16 for
+ * its prox whatever this setting is. */
+#define MSLR_PRODUCTS_FP32       0
+#define MSLR_PRODUCTS_SPLIT_BF16 1
+mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
+
 /* ------------------------------------------------------- device buffers */
 /* Replaces cu(...) / Array(...) transfers (scripts/reconstruct.jl:122-130, 233-238). */
 mslr_status_t mslr_malloc(mslr_ctx ctx, size_t bytes, void **dev_ptr);
@@ -171,7 +183,8 @@ mslr_status_t mslr_patches2img(mslr_ctx ctx, const void *P_dev, const int64_t N[
  *   X_i <- U diag(max(s - beta_i, 0)) V^H,     X_i == 0 -> 0.
  * beta_host: one value (per_patch = 0) or Np values (per_patch = 1).  Each
  * beta is rounded to Float32 first (src/recon.jl:195).  beta < 0 -> MSLR_ERR_ARG.
- * Method: Gram matrix (FP32 products, FP64 sum), FP64 Hermitian eigensolve,
+ * Method: Gram matrix (FP64 products of the FP32 data, or split-BF16 products
+ * with FP64 sums, see mslr_ctx_set_products), FP64 Hermitian eigensolve,
  * reconstruction X_i V_r diag(f) V_r^H.
  */
 mslr_status_t mslr_svst_batched(mslr_ctx ctx, void *P_dev, int64_t p, int64_t Nt, int64_t Np,

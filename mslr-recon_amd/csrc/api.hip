@@ -146,6 +146,15 @@ mslr_status_t mslr_ctx_destroy(mslr_ctx ctx) {
     });
 }
 
+mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(products == MSLR_PRODUCTS_FP32 || products == MSLR_PRODUCTS_SPLIT_BF16,
+                     MSLR_ERR_ARG, "products must be MSLR_PRODUCTS_FP32 or MSLR_PRODUCTS_SPLIT_BF16");
+        c->products = products;
+    });
+}
+
 mslr_status_t mslr_ctx_sync(mslr_ctx ctx) {
     return guarded([&] {
         mslr_ctx_s *c = check_ctx(ctx);

@@ -253,7 +253,20 @@ static void pogm_iteration(mslr_pogm_s *st) {
     }
     int *err = (int *)((char *)st->dev + offsetof(PogmDev, err));
     static const char *sfx[] = {"@s1", "@s2", "@s3", "@s4", "@s5", "@s6", "@s7", "@s8"};
-    if (st->aux.empty()) {
+    // the mixed form runs the SVST products in split BF16; the context setting
+    // is restored after the prox
+    struct ProductsScope {
+        mslr_ctx_s *c;
+        int saved;
+        ProductsScope(mslr_ctx_s *c_, int form) : c(c_), saved(c_->products) {
+            if (form == MSLR_FORM_MIXED) c->products = MSLR_PRODUCTS_SPLIT_BF16;
+            else c->products = MSLR_PRODUCTS_FP32;
+        }
+        ~ProductsScope() { c->products = saved; }
+    } products_scope(ctx, st->form);
+    // serial scales without aux streams, and whenever kernels are being timed (so
+    // each kernel's events bracket it alone)
+    if (st->aux.empty() || ctx->prof) {
         for (int64_t k = 0; k < st->S; ++k) {
             c64 *Xk = st->X + k * n;
             ctx->prof_suffix = k < 8 ? sfx[k] : "@s9+";

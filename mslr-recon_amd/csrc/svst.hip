@@ -987,6 +987,312 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
             }
 }
 
+// ------------------------------------------------------------------ split-BF16 products
+// A FP32 value v is carried as hi + lo, hi = bf16(v), lo = bf16(v - hi), so
+// |v - hi - lo| <= 2^-18 |v|; a product in the reconstruction GEMM is
+// hi*hi' + hi*lo' + lo*hi' (the lo*lo' term is below the split error there; the
+// Gram keeps it, below).  bf16 x bf16 products are exact in FP32.
+// A complex value is one 32-bit word per part (bf16 re in the low half, bf16 im
+// in the high half), so a run of K complex values is the run of 2K interleaved
+// reals that v_mfma_f32_16x16x32_bf16 contracts: one instruction covers 16
+// complex k.  Its lane layout (D = P Q, P 16x32, Q 32x16): lane l holds
+// P[l%16][8(l/16) .. 8(l/16)+7], Q[8(l/16) .. +7][l%16] and D[4(l/16)+v][l%16].
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
+
+constexpr int BKS = 32;         // complex k per LDS stage
+constexpr int BLD = BKS + 4;    // LDS row pitch in words: 16-byte reads of 16 rows hit distinct banks
+
+__device__ __forceinline__ void split_c64(c64 v, unsigned &hi, unsigned &lo) {
+    const bf2v h = {(__bf16)v.re, (__bf16)v.im};
+    const bf2v l = {(__bf16)(v.re - (float)h[0]), (__bf16)(v.im - (float)h[1])};
+    hi = __builtin_bit_cast(unsigned, h);
+    lo = __builtin_bit_cast(unsigned, l);
+}
+// (re, im) -> (re, -im)
+__device__ __forceinline__ u4 w_conj(u4 w) { return w ^ 0x80000000u; }
+// (re, im) -> (im, re)
+__device__ __forceinline__ u4 w_swap(u4 w) { return (w >> 16) | (w << 16); }
+// (re, im) -> (im, -re)
+__device__ __forceinline__ u4 w_swapneg(u4 w) { return w_swap(w) ^ 0x80000000u; }
+
+__device__ __forceinline__ mf4 mfma_bf16(u4 p, u4 q, mf4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, p),
+                                                    __builtin_bit_cast(bf8, q), c, 0, 0, 0);
+}
+// c += P Q in split form
+__device__ __forceinline__ mf4 mfma_split(u4 ph, u4 pl, u4 qh, u4 ql, mf4 c) {
+    c = mfma_bf16(pl, qh, c);
+    c = mfma_bf16(ph, ql, c);
+    return mfma_bf16(ph, qh, c);
+}
+
+// Eight complex values of one row (column) of a tile into LDS as hi and lo words.
+__device__ __forceinline__ void stage8(const c64 (&r)[8], unsigned *sh, unsigned *sl) {
+    unsigned h[8], l[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) split_c64(r[e], h[e], l[e]);
+    *(u4 *)sh = u4{h[0], h[1], h[2], h[3]};
+    *(u4 *)(sh + 4) = u4{h[4], h[5], h[6], h[7]};
+    *(u4 *)sl = u4{l[0], l[1], l[2], l[3]};
+    *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
+}
+
+// Split-BF16 Gram tiles with error-free sums: same tiling, contraction chunks and
+// FP64 partial output as k_gram_mfma, for row-contiguous patches (ks == 1).
+// Per LDS stage (BKS rows) each column a gets one exponent E_a, 2^E_a > max |x|
+// over the stage's rows, and every value is cut into two fixed-point slices
+// x = s1 + s2 + r, s1 a multiple of 2^(E-8) (|s1| <= 2^8 of those), s2 a multiple
+// of 2^(E-16) (<= 2^7 of those), |r| <= 2^(E-17); both slices are exact in BF16
+// (the Ozaki split).  For an entry (a, b) all s1*s1 products of a stage share one
+// quantum and stay below 2^16 of it, so the MFMA sums of a stage (64 products)
+// are exact in FP32, and so are those of the s1*s2 + s2*s1 terms (a quantum 2^8
+// smaller, 128 products) and of the s2*s2 terms (2^16 smaller, 64 products); each
+// stage's three sums go into FP64 accumulators.  The result is the exact Gram of
+// s1 + s2: the FP32 data perturbed by at most 2^-17 of each column's maximum,
+// with no rounding of long sums (which would put eps32 of the dominant
+// eigenvalue onto every small one).  The s2*s2 terms are kept because dropping
+// them is a one-signed bias of the diagonal, ~2^-20 of the dominant eigenvalue,
+// which would bury eigenvalues 1e8 below it.
+// Operands are (X(k, b), X(k, a)): G_re = [br, bi].[ar, ai],
+// G_im = [bi, -br].[ar, ai] over interleaved k.
+// (s1, s2) of x for the quanta set by sg1 = 3*2^(E+14), sg2 = 3*2^(E+6)
+__device__ __forceinline__ float2 ozaki_split(float x, float sg1, float sg2) {
+    const float s1 = (x + sg1) - sg1;
+    const float r1 = x - s1;
+    const float s2 = (r1 + sg2) - sg2;
+    return make_float2(s1, s2);
+}
+
+__global__ __launch_bounds__(256) void k_gram_bf16(const c64 *__restrict__ P, int64_t pstride,
+                                                   int64_t cs, int m, int64_t Kc, int nkc,
+                                                   int ntile, c128 *__restrict__ part) {
+    __shared__ __align__(16) unsigned Sh[2][GT][BLD];  // [A cols a, B cols b][col][k]
+    __shared__ __align__(16) unsigned Sl[2][GT][BLD];
+    const int ip = blockIdx.z;
+    const int kc = blockIdx.y;
+    int rem = blockIdx.x, ti = 0;
+    while (rem >= ntile - ti) {
+        rem -= ntile - ti;
+        ++ti;
+    }
+    const int tj = ti + rem;
+    const int a0 = ti * GT, b0 = tj * GT;
+    const c64 *X = P + ip * pstride;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wa = (wave & 1) * 32, wb = (wave >> 1) * 32;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    // load slot: one column, eight consecutive rows; the four slots of a column
+    // are adjacent lanes of one wave
+    const int lc = tid >> 2;
+    const int lk = (tid & 3) * 8;
+    const bool va = a0 + lc < m, vb = b0 + lc < m;
+    const c64 *xa = X + (int64_t)(a0 + lc) * cs;
+    const c64 *xb = X + (int64_t)(b0 + lc) * cs;
+    mf4 hr[2][2], hi[2][2], xr[2][2], xi[2][2], lr[2][2], li[2][2];
+    md4 dr[2][2], di[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            dr[i][j] = md4{0.0, 0.0, 0.0, 0.0};
+            di[i][j] = md4{0.0, 0.0, 0.0, 0.0};
+        }
+    const int64_t kbeg = (int64_t)kc * KCH;
+    const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
+    c64 ra[8], rb[8];
+    auto fetch = [&](int64_t k0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int64_t k = k0 + lk + e;
+            const bool kin = k < kend;
+            ra[e] = (kin && va) ? xa[k] : c64{0.f, 0.f};
+            rb[e] = (kin && vb) ? xb[k] : c64{0.f, 0.f};
+        }
+    };
+    // slices of one column's eight values into LDS
+    auto stage = [&](const c64 (&r)[8], unsigned *sh, unsigned *sl) {
+        float mx = 0.f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fmaxf(fabsf(r[e].re), fabsf(r[e].im)));
+        mx = fmaxf(mx, __shfl_xor(mx, 1, 64));
+        mx = fmaxf(mx, __shfl_xor(mx, 2, 64));
+        int E;
+        (void)frexpf(mx, &E);  // mx < 2^E (E = 0 for a zero column: all slices 0)
+        const float sg1 = ldexpf(3.f, E + 14), sg2 = ldexpf(3.f, E + 6);
+        unsigned h[8], l[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float2 sr = ozaki_split(r[e].re, sg1, sg2);
+            const float2 si = ozaki_split(r[e].im, sg1, sg2);
+            const bf2v hv = {(__bf16)sr.x, (__bf16)si.x};
+            const bf2v lv = {(__bf16)sr.y, (__bf16)si.y};
+            h[e] = __builtin_bit_cast(unsigned, hv);
+            l[e] = __builtin_bit_cast(unsigned, lv);
+        }
+        *(u4 *)sh = u4{h[0], h[1], h[2], h[3]};
+        *(u4 *)(sh + 4) = u4{h[4], h[5], h[6], h[7]};
+        *(u4 *)sl = u4{l[0], l[1], l[2], l[3]};
+        *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
+    };
+    fetch(kbeg);
+    for (int64_t k0 = kbeg; k0 < kend; k0 += BKS) {
+        stage(ra, &Sh[0][lc][lk], &Sl[0][lc][lk]);
+        stage(rb, &Sh[1][lc][lk], &Sl[1][lc][lk]);
+        __syncthreads();
+        if (k0 + BKS < kend) fetch(k0 + BKS);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                hr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                hi[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                xr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                xi[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                lr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                li[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll
+        for (int s = 0; s < BKS / 16; ++s) {
+            const int kw = 16 * s + 4 * g4;
+            u4 qh[2], ql[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                qh[i] = *(const u4 *)&Sh[0][wa + 16 * i + r16][kw];
+                ql[i] = *(const u4 *)&Sl[0][wa + 16 * i + r16][kw];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const u4 ph = *(const u4 *)&Sh[1][wb + 16 * j + r16][kw];
+                const u4 pl = *(const u4 *)&Sl[1][wb + 16 * j + r16][kw];
+                const u4 ph2 = w_swapneg(ph), pl2 = w_swapneg(pl);
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    hr[i][j] = mfma_bf16(ph, qh[i], hr[i][j]);
+                    xr[i][j] = mfma_bf16(pl, qh[i], xr[i][j]);
+                    xr[i][j] = mfma_bf16(ph, ql[i], xr[i][j]);
+                    lr[i][j] = mfma_bf16(pl, ql[i], lr[i][j]);
+                    hi[i][j] = mfma_bf16(ph2, qh[i], hi[i][j]);
+                    xi[i][j] = mfma_bf16(pl2, qh[i], xi[i][j]);
+                    xi[i][j] = mfma_bf16(ph2, ql[i], xi[i][j]);
+                    li[i][j] = mfma_bf16(pl2, ql[i], li[i][j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    dr[i][j][v] += ((double)hr[i][j][v] + (double)xr[i][j][v]) + (double)lr[i][j][v];
+                    di[i][j][v] += ((double)hi[i][j][v] + (double)xi[i][j][v]) + (double)li[i][j][v];
+                }
+        __syncthreads();
+    }
+    c128 *out = part + ((int64_t)ip * nkc + kc) * (int64_t)m * m;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int a = a0 + wa + 16 * i + r16;
+                const int b = b0 + wb + 16 * j + 4 * g4 + v;
+                if (a < m && b < m) out[a + (int64_t)m * b] = c128{dr[i][j][v], di[i][j][v]};
+            }
+}
+
+// Split-BF16 complex GEMM C = A B (the SVST apply), FP32 accumulation.  Same
+// tiling and output layout as k_cgemm (64x64 per workgroup, 32x32 per wave),
+// operands (B^T, A^T): C_re = [br, bi].[ar, -ai], C_im = [br, bi].[ai, ar].
+// blockIdx.x runs over the column tiles of C so the tiles sharing a block of
+// rows of A are dispatched together.
+__global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, int64_t bA,
+                                                    int64_t ai, int64_t ak,
+                                                    const c64 *__restrict__ B, int64_t bB,
+                                                    int64_t bk, int64_t bj,
+                                                    c64 *__restrict__ C, int64_t bC, int64_t ci,
+                                                    int64_t cj, int M, int N, int K) {
+    __shared__ __align__(16) unsigned Sh[2][MT][BLD];  // [A rows i, B cols j][row][k]
+    __shared__ __align__(16) unsigned Sl[2][MT][BLD];
+    const int z = blockIdx.z;
+    const int j0 = blockIdx.x * MT, i0 = blockIdx.y * MT;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const c64 *Az = A + (int64_t)z * bA;
+    const c64 *Bz = B + (int64_t)z * bB;
+    // load slots: one row of A (column of B), eight consecutive k
+    const bool afast = ai == 1, bfast = bk == 1;
+    const int ar_ = afast ? tid & 63 : tid >> 2;
+    const int ak_ = afast ? (tid >> 6) * 8 : (tid & 3) * 8;
+    const int bc_ = bfast ? tid >> 2 : tid & 63;
+    const int bk_ = bfast ? (tid & 3) * 8 : (tid >> 6) * 8;
+    const bool av = i0 + ar_ < M, bv = j0 + bc_ < N;
+    const c64 *pa = Az + (int64_t)(i0 + ar_) * ai;
+    const c64 *pb = Bz + (int64_t)(j0 + bc_) * bj;
+    mf4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            accr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+            acci[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+        }
+    c64 ra[8], rb[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int ka = k0 + ak_ + e, kb = k0 + bk_ + e;
+            ra[e] = (av && ka < K) ? pa[(int64_t)ka * ak] : c64{0.f, 0.f};
+            rb[e] = (bv && kb < K) ? pb[(int64_t)kb * bk] : c64{0.f, 0.f};
+        }
+    };
+    if (K > 0) fetch(0);
+    for (int k0 = 0; k0 < K; k0 += BKS) {
+        stage8(ra, &Sh[0][ar_][ak_], &Sl[0][ar_][ak_]);
+        stage8(rb, &Sh[1][bc_][bk_], &Sl[1][bc_][bk_]);
+        __syncthreads();
+        if (k0 + BKS < K) fetch(k0 + BKS);
+#pragma unroll
+        for (int s = 0; s < BKS / 16; ++s) {
+            const int kw = 16 * s + 4 * g4;
+            u4 qh[2], ql[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                qh[i] = *(const u4 *)&Sh[0][wi + 16 * i + r16][kw];
+                ql[i] = *(const u4 *)&Sl[0][wi + 16 * i + r16][kw];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const u4 ph = *(const u4 *)&Sh[1][wj + 16 * j + r16][kw];
+                const u4 pl = *(const u4 *)&Sl[1][wj + 16 * j + r16][kw];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    accr[i][j] = mfma_split(ph, pl, w_conj(qh[i]), w_conj(ql[i]), accr[i][j]);
+                    acci[i][j] = mfma_split(ph, pl, w_swap(qh[i]), w_swap(ql[i]), acci[i][j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    c64 *Cz = C + (int64_t)z * bC;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ii = i0 + wi + 16 * i + r16;
+                const int jj = j0 + wj + 16 * j + 4 * g4 + v;
+                if (ii < M && jj < N) Cz[ii * ci + jj * cj] = c64{accr[i][j][v], acci[i][j][v]};
+            }
+}
+
 // m == 1: each matrix is a vector v (len = p*Nt); SVST = v * max(|v| - beta, 0)/|v|.
 __global__ __launch_bounds__(256) void k_svst_m1(c64 *__restrict__ P, int64_t len, int64_t Np,
                                                  const double *__restrict__ beta,
@@ -1160,7 +1466,7 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
 
 template <class AT>
 static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, int64_t p,
-                             int64_t Nt, int64_t nb, int *err_dev) {
+                             int64_t Nt, int64_t nb, int *err_dev, bool split = false) {
     const int m = (int)w.m;
     const bool left = p < Nt;
     const int64_t Kc = left ? Nt : p;
@@ -1171,9 +1477,14 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
     dim3 gg((unsigned)npair, (unsigned)w.nkc, (unsigned)nb);
     {
         MSLR_PROF(ctx, "gram_tiles");
-        if constexpr (std::is_same<AT, double>::value)
-            k_gram_mfma<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, ntile,
-                                                     (c128 *)w.part);
+        if constexpr (std::is_same<AT, double>::value) {
+            if (split && ks == 1)  // the split Gram reads row-contiguous patches
+                k_gram_bf16<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, cs, m, Kc, (int)w.nkc, ntile,
+                                                         (c128 *)w.part);
+            else
+                k_gram_mfma<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc,
+                                                         ntile, (c128 *)w.part);
+        }
         else
             k_gram_tiles<AT><<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc,
                                                           ntile, (C2<AT> *)w.part);
@@ -1232,6 +1543,19 @@ static void cgemm(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, in
     }
 }
 
+static void cgemm_bf16(mslr_ctx_s *ctx, const c64 *A, int64_t bA, int64_t ai, int64_t ak,
+                       const c64 *B, int64_t bB, int64_t bk, int64_t bj, c64 *C, int64_t bC,
+                       int64_t ci, int64_t cj, int M, int N, int K, int64_t nbatch) {
+    for (int64_t z0 = 0; z0 < nbatch; z0 += 65535) {
+        const int64_t nz = std::min<int64_t>(65535, nbatch - z0);
+        dim3 g((unsigned)cdiv(N, MT), (unsigned)cdiv(M, MT), (unsigned)nz);
+        MSLR_PROF(ctx, "apply");
+        k_cgemm_bf16<<<g, 256, 0, ctx->stream>>>(A + z0 * bA, bA, ai, ak, B + z0 * bB, bB, bk, bj,
+                                                 C + z0 * bC, bC, ci, cj, M, N, K);
+        HIP_LAUNCH_CHECK();
+    }
+}
+
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
                   const double *beta_dev, int per_patch, int *err_dev) {
     if (Np == 0 || p == 0 || Nt == 0) return;
@@ -1251,6 +1575,8 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
     }
     const bool left = p < Nt;
     const SvstWork w = svst_work(ctx, m, left ? Nt : p, Np);
+    // split-BF16 Gram and apply products (the mixed-precision form)
+    const bool split = ctx->products == MSLR_PRODUCTS_SPLIT_BF16;
     // W = V_r diag(f) V_r^H per patch (m x m, FP32), reuses the Gram partial space
     c64 *Wm = (c64 *)w.part;
     for (int64_t ip0 = 0; ip0 < Np; ip0 += w.chunk) {
@@ -1260,7 +1586,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         // FP64 products: the Gram squares the dynamic range of X, and at production
         // shapes (global patch s1/s_noise ~ 1e4) FP32 products would bury the
         // singular values near the threshold under rounding error.
-        gram_and_tridiag<double>(ctx, w, Pc, p, Nt, nb, err_dev);
+        gram_and_tridiag<double>(ctx, w, Pc, p, Nt, nb, err_dev, split);
         {
             const double *bp = beta_dev + (per_patch ? ip0 : 0);
             MSLR_PROF(ctx, "eigvec");
@@ -1286,12 +1612,19 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
               0, w.r, nb);
         if (!left) {
             // Y (p x m) = X (p x m) W
-            cgemm(ctx, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p,
-                  (int)p, mi, mi, nullptr, nb);
+            if (split)
+                cgemm_bf16(ctx, Pc, len, 1, p, Wm, m * m, 1, m, Yc, len, 1, p, (int)p, mi, mi, nb);
+            else
+                cgemm(ctx, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p,
+                      (int)p, mi, mi, nullptr, nb);
         } else {
             // Y (p x Nt) = W (p x p) X
-            cgemm(ctx, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1, p,
-                  (int)p, (int)Nt, mi, nullptr, nb);
+            if (split)
+                cgemm_bf16(ctx, Wm, m * m, 1, m, Pc, len, 1, p, Yc, len, 1, p, (int)p, (int)Nt, mi,
+                           nb);
+            else
+                cgemm(ctx, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1, p,
+                      (int)p, (int)Nt, mi, nullptr, nb);
         }
     }
 }

@@ -23,6 +23,8 @@ MSLR_ERR_NUMERIC = 6
 FORM_PLAIN = 0
 FORM_BATCHED = 1
 FORM_MIXED = 2
+PRODUCTS_FP32 = 0
+PRODUCTS_SPLIT_BF16 = 1
 MOM_POGM, MOM_FPGM, MOM_PGM = 0, 1, 2
 RESTART_GR, RESTART_NONE = 0, 1
 
@@ -92,6 +94,7 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_ctx_create": (C.c_int, [C.c_int, C.POINTER(_p)]),
     "mslr_ctx_destroy": (C.c_int, [_p]),
     "mslr_ctx_sync": (C.c_int, [_p]),
+    "mslr_ctx_set_products": (C.c_int, [_p, C.c_int]),
     "mslr_ctx_info": (C.c_int, [_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "mslr_malloc": (C.c_int, [_p, C.c_size_t, C.POINTER(_p)]),
     "mslr_free": (C.c_int, [_p, _p]),

@@ -37,6 +37,13 @@ class Context:
         _lib.check(self._lib.mslr_ctx_info(self.handle, name, 256, C.byref(tot), C.byref(fr)))
         return {"name": name.value.decode(), "total_bytes": tot.value, "free_bytes": fr.value}
 
+    def set_products(self, products: str) -> None:
+        """SVST Gram / reconstruction products: "fp32" (FP64 Gram products of the
+        FP32 data, the default) or "split_bf16" (hi + lo BF16 on the matrix cores,
+        the mixed-precision form)."""
+        code = {"fp32": _lib.PRODUCTS_FP32, "split_bf16": _lib.PRODUCTS_SPLIT_BF16}[products]
+        _lib.check(self._lib.mslr_ctx_set_products(self.handle, code))
+
     def prof_enable(self, on: bool = True) -> None:
         """Time every kernel launch with HIP events (clears earlier records)."""
         _lib.check(self._lib.mslr_prof_enable(self.handle, int(bool(on))))
