@@ -31,6 +31,22 @@ constexpr int MAXE_X = 16;   // elements per thread in the x-axis passes
 constexpr int MAXE_YZ = 16;  // elements per thread in the yz-plane passes
 constexpr int NTYZ_MAX = 512;  // threads of the yz-plane passes (at most)
 constexpr int MAX_NYZ = MAXE_YZ * NTYZ_MAX;
+// threads of the compile-time yz passes for planes above 2048 points.  Forward:
+// 1024 (a 90 x 60 plane then takes one round of butterflies per pass, at 60
+// VGPRs, two workgroups per CU; 8.4 ms per A at the 90x90x60x300x18 shape
+// against 10.7 ms with 512).  Adjoint: 512 (its plane accumulator in registers
+// needs fewer, larger threads; 8.1 ms against 9.8-13.8 ms with 256-1024).
+#ifndef MSLR_NTYZ_FWD
+#define MSLR_NTYZ_FWD 1024
+#endif
+#ifndef MSLR_NTYZ_ADJ
+#define MSLR_NTYZ_ADJ 512
+#endif
+#ifdef MSLR_YZ_WPE
+#define MSLR_YZ_ATTR __attribute__((amdgpu_waves_per_eu(MSLR_YZ_WPE)))
+#else
+#define MSLR_YZ_ATTR
+#endif
 
 // ------------------------------------------------------------------ FFT policies
 // A policy transforms `nlines` lines in LDS: runtime lengths (any 2/3/5 length)
@@ -156,7 +172,7 @@ struct LineTabs {
 // would halve the workgroups resident per CU.  Compile-time FFT lengths loop
 // over all coils, runtime lengths take one coil per workgroup.
 template <class T, class FY, class FZ>
-__global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
+__global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) MSLR_YZ_ATTR void k_fwd_yz(
     const C2<T> *__restrict__ Pm, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny,
     int Nz, int cpw, FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz,
     int64_t ta, C2<T> *__restrict__ Tb) {
@@ -300,7 +316,7 @@ __global__ __launch_bounds__(NTX) void k_adj_x(
 // registers (addresses kept out of loop-invariant code motion so that two
 // workgroups stay resident per CU), runtime lengths in the output plane itself.
 template <class T, class FY, class FZ>
-__global__ __launch_bounds__(NTYZ_MAX) void k_adj_yz(
+__global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) MSLR_YZ_ATTR void k_adj_yz(
     const C2<T> *__restrict__ Tb, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny, int Nz,
     FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz, int64_t ta,
     T scale, C2<T> *__restrict__ Pm) {
@@ -518,14 +534,16 @@ static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, 
 
 // ---- dispatch: compile-time FFT lengths for the production / test sizes in
 // FP32 with the default exponent sign, runtime lengths otherwise.
-constexpr int fix_nt_yz(int ny, int nz) { return ny * nz > 2048 ? 512 : 256; }
+constexpr int fix_nt_yz(int ny, int nz, int sgn) {
+    return ny * nz > 2048 ? (sgn < 0 ? MSLR_NTYZ_FWD : MSLR_NTYZ_ADJ) : 256;
+}
 
 template <int NX, int SGN>
 using FixX = FftFix<float, NX, SGN, (MAXE_X * NTX) / NX, NTX>;
 template <int NY, int NZ, int SGN>
-using FixY = FftFix<float, NY, SGN, NZ, fix_nt_yz(NY, NZ)>;
+using FixY = FftFix<float, NY, SGN, NZ, fix_nt_yz(NY, NZ, SGN)>;
 template <int NY, int NZ, int SGN>
-using FixZ = FftFix<float, NZ, SGN, NY, fix_nt_yz(NY, NZ)>;
+using FixZ = FftFix<float, NZ, SGN, NY, fix_nt_yz(NY, NZ, SGN)>;
 
 #define MSLR_X_SIZES(M) M(90) M(64) M(32) M(12)
 #define MSLR_YZ_SIZES(M) M(90, 60) M(64, 40) M(32, 24) M(12, 10)
@@ -543,7 +561,7 @@ static void fwd_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Pm, C2<
 #define CASE_YZ(ny, nz)                                                                  \
     case ny * 10000 + nz:                                                                \
         launch_fwd_yz<float>(P, ta, nf, Pm, Tb, FixY<ny, nz, -1>{P->tw32[1]},             \
-                             FixZ<ny, nz, -1>{P->tw32[2]}, fix_nt_yz(ny, nz));            \
+                             FixZ<ny, nz, -1>{P->tw32[2]}, fix_nt_yz(ny, nz, -1));            \
         return;
                 MSLR_YZ_SIZES(CASE_YZ)
 #undef CASE_YZ
@@ -601,7 +619,7 @@ static void adj_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, C2<
 #define CASE_YZ(ny, nz)                                                                  \
     case ny * 10000 + nz:                                                                \
         launch_adj_yz<float>(P, ta, nf, Tb, Pm, FixY<ny, nz, 1>{P->tw32[1]},              \
-                             FixZ<ny, nz, 1>{P->tw32[2]}, fix_nt_yz(ny, nz));             \
+                             FixZ<ny, nz, 1>{P->tw32[2]}, fix_nt_yz(ny, nz, 1));             \
         return;
                 MSLR_YZ_SIZES(CASE_YZ)
 #undef CASE_YZ
