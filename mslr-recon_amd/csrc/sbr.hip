@@ -187,14 +187,30 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             }
         }
         __syncthreads();
-        // 4. T (upper triangular, Q = I - V T V^H): S = V^H V, then the zlarft recurrence
-        for (int pr = tid; pr < B * B; pr += S1T) {
-            const int a = pr % B, c = pr / B;
-            if (a < c) {
-                c128 sacc = {0.0, 0.0};
-                for (int i = 0; i < L; ++i) sacc = sacc + cmulc(V[a * L + i], V[c * L + i]);
-                Ms[a][c] = sacc;
+        // 4. T (upper triangular, Q = I - V T V^H): S = V^H V (entries a < c, four
+        //    threads each over interleaved rows, two partial sums per thread; V_c is
+        //    zero above row c), then the zlarft recurrence
+        {
+            const int pr = tid >> 2, part = tid & 3;
+            int c = 1;
+            while (c * (c + 1) / 2 <= pr) ++c;
+            const int a = pr - c * (c - 1) / 2;
+            c128 s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+            if (c < B) {
+                const c128 *va = V + a * L, *vc = V + c * L;
+                int i = c + part;
+                for (; i + 4 < L; i += 8) {
+                    s0 = s0 + cmulc(va[i], vc[i]);
+                    s1 = s1 + cmulc(va[i + 4], vc[i + 4]);
+                }
+                if (i < L) s0 = s0 + cmulc(va[i], vc[i]);
             }
+            s0 = s0 + s1;
+            s0.re += __shfl_xor(s0.re, 1, 64);
+            s0.im += __shfl_xor(s0.im, 1, 64);
+            s0.re += __shfl_xor(s0.re, 2, 64);
+            s0.im += __shfl_xor(s0.im, 2, 64);
+            if (c < B && part == 0) Ms[a][c] = s0;
         }
         __syncthreads();
         if (wave == 0) {
@@ -214,7 +230,9 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
         __syncthreads();
         for (int pr = tid; pr < B * B; pr += S1T) Tg[(int64_t)p * B * B + pr] = Ts[pr % B][pr / B];
         // 5. Y = A22 V -> W on the FP64 matrix cores: one 16-row block per wave at a
-        //    time; the A22 operand is read from global memory once.
+        //    time; the A22 operand is read from global memory once.  Only the lower
+        //    triangle of A22 is kept up to date: an entry above the diagonal is read
+        //    as the conjugate of its mirror.
         //    v_mfma_f64_16x16x4f64 (D = P Q): lane l holds P[l%16][l/16],
         //    Q[l/16][l%16] and D[l/16 + 4v][l%16].
         {
@@ -223,11 +241,13 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             for (int rb = wave; rb < nrb; rb += S1T / 64) {
                 md4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
                 const int i = rb * 16 + rl;
-                const c128 *arow = A + (r0 + i) + (int64_t)m * r0;
 #pragma unroll 4
                 for (int ks = 0; ks < L; ks += 4) {
                     const int k = ks + kl;
-                    const c128 a = ld_if(i < L && k < L, arow + (int64_t)m * k);
+                    const bool lo = i >= k;
+                    c128 a = ld_if(i < L && k < L, lo ? A + (r0 + i) + (int64_t)m * (r0 + k)
+                                                      : A + (r0 + k) + (int64_t)m * (r0 + i));
+                    if (!lo) a.im = -a.im;
                     const c128 v = ld_if(k < L, V + rl * L + k);
                     // operands swapped: the block computed is Y^T, so the output
                     // lanes run down the columns of W
@@ -264,12 +284,23 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             }
         }
         __syncthreads();
-        // 7. X = V^H Y, M = T^H X (Hermitian)
-        for (int pr = tid; pr < B * B; pr += S1T) {
+        // 7. X = V^H Y (two threads per entry over interleaved rows, two partial sums
+        //    each; V_a is zero above row a), M = T^H X (Hermitian)
+        {
+            const int pr = tid >> 1, part = tid & 1;
             const int a = pr % B, c = pr / B;
-            c128 sacc = {0.0, 0.0};
-            for (int i = 0; i < L; ++i) sacc = sacc + cmulc(V[a * L + i], W[c * L + i]);
-            Xs[a][c] = sacc;
+            const c128 *va = V + a * L, *wc = W + c * L;
+            c128 s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+            int i = a + part;
+            for (; i + 2 < L; i += 4) {
+                s0 = s0 + cmulc(va[i], wc[i]);
+                s1 = s1 + cmulc(va[i + 2], wc[i + 2]);
+            }
+            if (i < L) s0 = s0 + cmulc(va[i], wc[i]);
+            s0 = s0 + s1;
+            s0.re += __shfl_xor(s0.re, 1, 64);
+            s0.im += __shfl_xor(s0.im, 1, 64);
+            if (part == 0) Xs[a][c] = s0;
         }
         __syncthreads();
         for (int pr = tid; pr < B * B; pr += S1T) {
@@ -279,22 +310,48 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             Ms[a][c] = s;
         }
         __syncthreads();
-        // 8. W = Y - 1/2 V M (one output per thread and step)
-        for (int task = tid; task < L * B; task += S1T) {
-            const int i = task % L, c = task / L;
-            c128 acc = {0.0, 0.0};
+        // 8. W = Y - 1/2 V M on the matrix cores, one 16-row block per wave at a time
+        //    (the block computed is W^T: lane (g, rl) holds W[rl][g + 4q])
+        {
+            const int nrb = (L + 15) / 16;
+            const int rl = lane & 15, kl = lane >> 4;
+            for (int rb = wave; rb < nrb; rb += S1T / 64) {
+                const int i = rb * 16 + rl;
+                md4 wr, wi;
 #pragma unroll
-            for (int a = 0; a < B; ++a) acc = acc + V[a * L + i] * Ms[a][c];
-            W[c * L + i] = W[c * L + i] - 0.5 * acc;
+                for (int q = 0; q < 4; ++q) {
+                    const c128 w0 = ld_if(i < L, W + (kl + 4 * q) * L + i);
+                    wr[q] = w0.re;
+                    wi[q] = w0.im;
+                }
+#pragma unroll
+                for (int as = 0; as < B; as += 4) {
+                    const int a = as + kl;
+                    const c128 mm = Ms[a][rl];  // P[c][a] = M[a][c], c = rl
+                    const c128 v = ld_if(i < L, V + a * L + i);
+                    // V[i][a] M[a][c]: re = vr mr - vi mi, im = vr mi + vi mr
+                    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.re, v.re, wr, 0, 0, 0);
+                    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(0.5 * mm.im, v.im, wr, 0, 0, 0);
+                    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.im, v.re, wi, 0, 0, 0);
+                    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.re, v.im, wi, 0, 0, 0);
+                }
+                if (i < L) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) W[(kl + 4 * q) * L + i] = c128{wr[q], wi[q]};
+                }
+            }
         }
         __syncthreads();
-        // 9. A22 <- A22 - V W^H - W V^H (all L x L entries) on the FP64 matrix cores,
-        //    one 16x16 block per wave at a time, K = 2B complex.
+        // 9. A22 <- A22 - V W^H - W V^H on the FP64 matrix cores, the 16x16 blocks on
+        //    and below the diagonal (the lower triangle is what later steps read),
+        //    one block per wave at a time, K = 2B complex.
         {
             const int nb = (L + 15) / 16;
             const int rl = lane & 15, kl = lane >> 4;
-            for (int blk = wave; blk < nb * nb; blk += S1T / 64) {
-                const int bi = blk % nb, bk = blk / nb;
+            for (int blk = wave; blk < nb * (nb + 1) / 2; blk += S1T / 64) {
+                int bi = 0;
+                while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
+                const int bk = blk - bi * (bi + 1) / 2;
                 md4 dr, di;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
