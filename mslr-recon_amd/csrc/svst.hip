@@ -481,6 +481,32 @@ __device__ int sturm_count(const double *d, const double *e, int m, double x, do
     return c;
 }
 
+// 1/q from the hardware estimate and two Newton steps (correct to rounding for
+// finite nonzero q; the callers keep |q| >= pivmin or tiny, so q is never 0 or inf).
+__device__ __forceinline__ double fast_rcp(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    double t = __builtin_fma(-q, r, 1.0);
+    r = __builtin_fma(r, t, r);
+    t = __builtin_fma(-q, r, 1.0);
+    return __builtin_fma(r, t, r);
+}
+
+// sturm_count with the division replaced by fast_rcp (the bisection steps;
+// k_eig_count keeps the IEEE division for the count that fixes r).
+__device__ int sturm_count_fast(const double *d, const double *e, int m, double x, double pivmin) {
+    int c = 0;
+    double q = d[0] - x;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    c += q < 0.0;
+    for (int i = 1; i < m; ++i) {
+        const double el = e[i - 1];
+        q = (d[i] - x) - (el * el) * fast_rcp(q);
+        if (fabs(q) <= pivmin) q = -pivmin;
+        c += q < 0.0;
+    }
+    return c;
+}
+
 __device__ __forceinline__ double hash_unit(uint32_t a, uint32_t b) {
     uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
     h ^= h >> 16;
@@ -554,7 +580,7 @@ __device__ void twisted_vector(const double *d, const double *e, int m, double l
     if (p == 0.0) p = tiny;
     z[0] = p;
     for (int i = 0; i + 1 < m; ++i) {
-        p = (d[i + 1] - lam) - e[i] * e[i] / p;
+        p = (d[i + 1] - lam) - (e[i] * e[i]) * fast_rcp(p);
         if (p == 0.0) p = tiny;
         z[i + 1] = p;
     }
@@ -564,7 +590,7 @@ __device__ void twisted_vector(const double *d, const double *e, int m, double l
     int best = m - 1;
     double gbest = fabs(z[m - 1]);
     for (int i = m - 2; i >= 0; --i) {
-        q = (d[i] - lam) - e[i] * e[i] / q;
+        q = (d[i] - lam) - (e[i] * e[i]) * fast_rcp(q);
         if (q == 0.0) q = tiny;
         dm[(int64_t)i * st] = q;
         const double g = fabs(z[i] + q - (d[i] - lam));
@@ -605,8 +631,8 @@ __device__ __forceinline__ c128 wave_sum(c128 v) {
 // Eigenpairs of the tridiagonal T of each patch above tau = beta^2, in three
 // launches so that large problems spread over many workgroups:
 //   k_eig_count   : rank r = #eigenvalues > tau (Sturm count), Gershgorin bounds;
-//   k_eig_vectors : one thread per eigenvalue: bisection, then its eigenvector
-//                   by a twisted factorisation;
+//   k_eig_vectors : a group of lanes per eigenvalue: multisection, then its
+//                   eigenvector by a twisted factorisation;
 //   k_eig_clusters: tight clusters redone by inverse iteration, filter factors.
 // r = -1 marks beta == 0 (identity).
 constexpr int EIG_INFO = 4;  // per patch: tnorm, pivmin, upper Gershgorin bound
@@ -647,9 +673,14 @@ __global__ __launch_bounds__(64) void k_eig_count(const double *__restrict__ dal
     rall[ip] = r;
 }
 
-// grid (patches, cdiv(m, 64)); thread k = blockIdx.y*64 + tid < r.  lam[k] is
-// the k-th largest eigenvalue; its eigenvector goes to Z column k, with the
-// bottom-up pivots kept in U's column k (free until the back-transformation).
+// grid (patches, cdiv(m * G, 64)); a group of G lanes per eigenvalue k < r.
+// lam[k] is the k-th largest eigenvalue, found by multisection: each step the
+// G lanes take Sturm counts at G interior points of [lo, hi] and the group keeps
+// the sub-interval where the count passes the eigenvalue's index (log2(G+1)
+// bits per step instead of one; the counts are monotone in the point).  Its
+// eigenvector goes to Z column k, with the bottom-up pivots kept in U's column
+// k (free until the back-transformation).
+template <int G>
 __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ dall,
                                                     const double *__restrict__ eall, int m,
                                                     const double *__restrict__ beta,
@@ -659,27 +690,37 @@ __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ d
                                                     double *__restrict__ lamall,
                                                     double *__restrict__ Zall,
                                                     c64 *__restrict__ Uall) {
+    static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "group size");
     const int ip = blockIdx.x;
     const int r = rall[ip];
-    const int k = blockIdx.y * 64 + threadIdx.x;
-    if (k >= r) return;
+    const int t = blockIdx.y * 64 + threadIdx.x;
+    const int k = t / G, j = t % G;
+    if (k >= r) return;  // whole groups leave together
+    const int gbase = (threadIdx.x & 63) - j;
     const double *d = dall + (int64_t)ip * m;
     const double *e = eall + (int64_t)ip * m;
     const double *inf = info + (int64_t)ip * EIG_INFO;
     const double tnorm = inf[0], pivmin = inf[1], gu = inf[2];
     const double bd = (double)(float)beta[beta_per_patch ? ip : 0];
     const double tau = bd * bd;
-    const int idx = m - 1 - k;  // ascending index
+    const int idx = m - 1 - k;  // ascending index: count(lo) <= idx < count(hi)
     double lo = tau, hi = gu;
     for (int it = 0; it < 200; ++it) {
-        const double mid = 0.5 * (lo + hi);
         if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
-        if (mid <= lo || mid >= hi) break;
-        if (sturm_count(d, e, m, mid, pivmin) > idx)
-            hi = mid;
-        else
-            lo = mid;
+        const double w = (hi - lo) / (double)(G + 1);
+        if (!(lo + w > lo) || !(hi - w < hi)) break;
+        const double xj = lo + (double)(j + 1) * w;
+        const bool above = sturm_count_fast(d, e, m, xj, pivmin) > idx;
+        const unsigned long long bal = __ballot(above);
+        const unsigned long long gm = (bal >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1));
+        const int f = gm ? __builtin_ctzll(gm) : G;  // first point above the index
+        const double nlo = f == 0 ? lo : lo + (double)f * w;
+        const double nhi = f == G ? hi : lo + (double)(f + 1) * w;
+        if (!(nlo < nhi)) break;
+        lo = nlo;
+        hi = nhi;
     }
+    if (j != 0) return;
     const double lk = 0.5 * (lo + hi);
     lamall[(int64_t)ip * m + k] = lk;
     const double tiny = DBL_EPSILON * fmax(tnorm, DBL_MIN);
@@ -1593,9 +1634,17 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
             k_eig_count<<<(unsigned)cdiv(nb, 64), 64, 0, ctx->stream>>>(w.d, w.e, (int)m, nb, bp,
                                                                          per_patch, w.info, w.r);
             HIP_LAUNCH_CHECK();
-            dim3 ge((unsigned)nb, (unsigned)cdiv(m, 64));
-            k_eig_vectors<<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch, w.info, w.r,
-                                                      w.lam, w.Z, w.U);
+            // lanes per eigenvalue: many when the eigenvalues are few (the bisection
+            // latency is the cost), few when the batch already fills the device
+            if (nb * m <= 64 * 1024) {
+                dim3 ge((unsigned)nb, (unsigned)cdiv(m * 16, 64));
+                k_eig_vectors<16><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
+                                                              w.info, w.r, w.lam, w.Z, w.U);
+            } else {
+                dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
+                k_eig_vectors<4><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
+                                                             w.info, w.r, w.lam, w.Z, w.U);
+            }
             HIP_LAUNCH_CHECK();
             k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
                                                                     w.info, w.r, w.lam, w.Z, w.ws,
