@@ -6,8 +6,9 @@
 //   stage 1 (k_sy2sb): dense -> band of width B.  Per panel of B columns a
 //     Householder QR (one wave, in LDS) gives Q = I - V T V^H, and the trailing
 //     matrix gets A <- Q^H A Q = A - V W^H - W V^H with Y = A V T,
-//     W = Y - 1/2 V (T^H V^H Y): three passes over the trailing matrix per B
-//     columns instead of one per column.
+//     W = Y - 1/2 V (T^H V^H Y): per B columns one read of the trailing matrix
+//     for Y and a read and write of its lower triangle for the update (the
+//     upper triangle is never read again), instead of a read per column.
 //   stage 2 (k_sb2st): band -> real tridiagonal by bulge chasing, the band
 //     (2B diagonals) held in LDS; each wave runs one sweep at a time, sweeps
 //     start STAGGER = 2 time steps apart (one workgroup barrier per time step).
@@ -32,8 +33,7 @@ constexpr int S2T = 768;       // threads of the band -> tridiagonal kernel
 constexpr int S2W = S2T / 64;  // concurrent sweeps
 constexpr int STAGGER = 2;     // time steps between the starts of consecutive sweeps
 constexpr int BTT = 256;       // threads of the back-transformation
-constexpr int BTC = BTT / B;   // eigenvectors per back-transformation workgroup
-constexpr int KG = 8;          // reflectors of one sweep applied together
+constexpr int BTC = 14;        // eigenvectors per back-transformation workgroup (LDS for two per CU)
 
 __device__ __forceinline__ c128 c_div(c128 a, c128 b) {
     const double den = b.re * b.re + b.im * b.im;
@@ -590,11 +590,15 @@ __global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, in
     }
 }
 
-// ------------------------------------------------------------------ back-transformation
-// grid (matrices, cdiv(m, BTC)); BTC eigenvectors per workgroup in LDS (column
-// c at Uc[c*m ..]).  Q2: B lanes per eigenvector, one reflector at a time in
-// reverse order (no workgroup barrier).  Q1: per panel in reverse order,
-// U <- U - V (T (V^H U)) with the panel's V staged in LDS.
+// Back-transformation U[:, c0 + c] = Q1 Q2 Z[:, c0 + c] for BTC eigenvectors per
+// workgroup, accumulated in FP64 in LDS (BTC x m: two workgroups per CU).
+// Q2, the sweep-major product of the bulge-chasing reflectors, is applied in
+// reverse sweep order, z <- z - tau v (v^H z); the reflectors of one sweep act
+// on disjoint row windows and are applied together, one thread per (reflector,
+// eigenvector) with its 16-term product in registers (the reflector is read from
+// global memory: the BTC threads of a reflector share its 256 bytes).  Q1, the
+// panels in reverse order, as U <- U - V (T (V^H U)) with V read from the
+// reduced matrix (unit lower form; the tails stored below the band).
 __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, int m, int npanel,
                                                 int ks, const c128 *__restrict__ Tall,
                                                 const c128 *__restrict__ V2all,
@@ -612,7 +616,6 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
     const int ncol = min(BTC, r - c0);
     const int tid = threadIdx.x;
     c128 *Uc = reinterpret_cast<c128 *>(smem_raw);  // BTC x m
-    c128 *Vs = Uc + (size_t)BTC * m;                 // (m - B) x B
     const c128 *A = Gall + (int64_t)ip * m * m;
     const double *Z = Zall + (int64_t)ip * m * m;
     for (int idx = tid; idx < BTC * m; idx += BTT) {
@@ -622,90 +625,91 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         Uc[idx] = z;
     }
     __syncthreads();
-    // Q2 (sweep-major product) applied in reverse: z <- z - tau v (v^H z).
-    // The reflectors of nsw sweeps at a time are staged in the (free) panel area.
+    // Q2
     {
-        const int q = tid % B, c = tid / B;
-        c128 *u = Uc + c * m;
         const c128 *V2 = V2all + (int64_t)ip * m * ks * B;
         const c128 *tau2 = tau2all + (int64_t)ip * m * ks;
-        const int nsw = max(1, ((m - B) * B) / (ks * (B + 1)));
-        c128 *vb = Vs;
-        c128 *tb = Vs + (size_t)nsw * ks * B;
-        for (int shi = m - 2; shi >= 0; shi -= nsw) {
-            const int slo = max(0, shi - nsw + 1);
-            const int nv = (shi - slo + 1) * ks;
-            __syncthreads();
-            for (int idx = tid; idx < nv * B; idx += BTT) vb[idx] = V2[(int64_t)slo * ks * B + idx];
-            for (int idx = tid; idx < nv; idx += BTT) tb[idx] = tau2[(int64_t)slo * ks + idx];
-            __syncthreads();
-            // The reflectors of one sweep act on disjoint row windows and commute:
-            // they are applied together, KG at a time with independent reductions.
-            for (int s = shi; s >= slo; --s) {
-                const int nst = (m - s - 1 + B - 1) / B;
-                for (int k0 = 0; k0 < nst; k0 += KG) {
-                    c128 v[KG], x[KG], tk[KG];
+        for (int s = m - 2; s >= 0; --s) {
+            const int nst = (m - s - 1 + B - 1) / B;
+            for (int t = tid; t < nst * BTC; t += BTT) {
+                const int k = t / BTC, c = t - k * BTC;
+                const int64_t slot = (int64_t)s * ks + k;
+                const c128 tk = tau2[slot];
+                if (c < ncol && nonzero(tk)) {
+                    const int st = s + 1 + k * B;
+                    const int len = min(B, m - st);
+                    const c128 *v = V2 + slot * B;
+                    c128 *u = Uc + c * m + st;
+                    c128 vv[B], uu[B];
 #pragma unroll
-                    for (int g = 0; g < KG; ++g) {
-                        const int k = k0 + g;
-                        const int slot = (s - slo) * ks + k;
-                        const int st = s + 1 + k * B;
-                        const bool on = k < nst && q < m - st;
-                        tk[g] = ld_if(k < nst, tb + slot);
-                        v[g] = ld_if(on, vb + slot * B + q);
-                        x[g] = ld_if(on, u + st + q);
+                    for (int q = 0; q < B; ++q) {
+                        vv[q] = ld_if(q < len, v + q);
+                        uu[q] = ld_if(q < len, u + q);
                     }
+                    c128 d0 = {0.0, 0.0}, d1 = {0.0, 0.0};
 #pragma unroll
-                    for (int g = 0; g < KG; ++g) {
-                        const c128 dot = sum16(cmulc(v[g], x[g]));
-                        const int k = k0 + g;
-                        const int st = s + 1 + k * B;
-                        if (k < nst && q < m - st) u[st + q] = x[g] - (tk[g] * dot) * v[g];
+                    for (int q = 0; q < B; q += 2) {
+                        d0 = d0 + cmulc(vv[q], uu[q]);
+                        d1 = d1 + cmulc(vv[q + 1], uu[q + 1]);
                     }
+                    const c128 f = tk * (d0 + d1);
+#pragma unroll
+                    for (int q = 0; q < B; ++q)
+                        if (q < len) u[q] = uu[q] - f * vv[q];
                 }
-                wave_lds_sync();
             }
+            __syncthreads();
         }
     }
-    __syncthreads();
-    // Q1: panels in reverse order
+    // Q1
     for (int p = npanel - 1; p >= 0; --p) {
         const int pc0 = p * B, r0 = pc0 + B, L = m - r0;
         const int kq = min(B, L - 1);
-        for (int idx = tid; idx < L * B; idx += BTT) {
-            const int i = idx % L, j = idx / L;
-            c128 v = {0.0, 0.0};
-            if (j < kq) {
-                if (i == j) v = c128{1.0, 0.0};
-                else if (i > j) v = A[(r0 + i) + (int64_t)m * (pc0 + j)];
-            }
-            Vs[j * L + i] = v;
-        }
         for (int pr = tid; pr < B * B; pr += BTT)
             Ts[pr % B][pr / B] = Tall[((int64_t)ip * npanel + p) * B * B + pr];
-        __syncthreads();
-        {   // X = V^H U (B x BTC), one entry per thread
+        {   // X = V^H U, one entry per thread, four partial sums
             const int a = tid % B, c = tid / B;
-            c128 s = {0.0, 0.0};
-            const c128 *uc = Uc + c * m + r0;
-            for (int i = 0; i < L; ++i) s = s + cmulc(Vs[a * L + i], uc[i]);
-            XY[a][c] = s;
+            if (c < BTC) {
+                c128 x = {0.0, 0.0};
+                if (a < kq) {
+                    const c128 *va = A + r0 + (int64_t)m * (pc0 + a);
+                    const c128 *uc = Uc + c * m + r0;
+                    c128 s0 = uc[a], s1 = {0.0, 0.0}, s2 = {0.0, 0.0}, s3 = {0.0, 0.0};
+                    int i = a + 1;
+                    for (; i + 3 < L; i += 4) {
+                        s0 = s0 + cmulc(va[i], uc[i]);
+                        s1 = s1 + cmulc(va[i + 1], uc[i + 1]);
+                        s2 = s2 + cmulc(va[i + 2], uc[i + 2]);
+                        s3 = s3 + cmulc(va[i + 3], uc[i + 3]);
+                    }
+                    for (; i < L; ++i) s0 = s0 + cmulc(va[i], uc[i]);
+                    x = (s0 + s1) + (s2 + s3);
+                }
+                XY[a][c] = x;
+            }
         }
         __syncthreads();
         {   // Y = T X (T upper)
             const int a = tid % B, c = tid / B;
-            c128 s = {0.0, 0.0};
-            for (int q = a; q < B; ++q) s = s + Ts[a][q] * XY[q][c];
+            c128 y = {0.0, 0.0};
+            if (c < BTC)
+                for (int q = a; q < B; ++q) y = y + Ts[a][q] * XY[q][c];
             __syncthreads();
-            XY[a][c] = s;
+            if (c < BTC) XY[a][c] = y;
         }
         __syncthreads();
+        // U <- U - V Y
         for (int idx = tid; idx < L * BTC; idx += BTT) {
             const int i = idx % L, c = idx / L;
-            c128 s = {0.0, 0.0};
+            const c128 *vrow = A + (r0 + i) + (int64_t)m * pc0;
+            c128 y = {0.0, 0.0};
 #pragma unroll
-            for (int a = 0; a < B; ++a) s = s + Vs[a * L + i] * XY[a][c];
-            Uc[c * m + r0 + i] = Uc[c * m + r0 + i] - s;
+            for (int a = 0; a < B; ++a) {
+                c128 v = ld_if(a < kq && i > a, vrow + (int64_t)m * a);
+                if (a < kq && i == a) v.re = 1.0;
+                y = y + v * XY[a][c];
+            }
+            Uc[c * m + r0 + i] = Uc[c * m + r0 + i] - y;
         }
         __syncthreads();
     }
@@ -757,7 +761,7 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
 void sbr_backtransform(mslr_ctx_s *ctx, const c128 *G, int m, int64_t nb, const SbrWork &w,
                        const double *Z, const int *r, c64 *U) {
     const int npanel = sbr_panels(m), ks = sbr_steps(m);
-    const size_t lds = ((size_t)BTC * m + (size_t)(m - B) * B) * sizeof(c128);
+    const size_t lds = (size_t)BTC * m * sizeof(c128);
     raise_lds((const void *)k_sbr_bt, lds);
     dim3 g((unsigned)nb, (unsigned)cdiv(m, BTC));
     MSLR_PROF(ctx, "backtransform");
