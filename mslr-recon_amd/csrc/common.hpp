@@ -174,7 +174,8 @@ enum ScratchSlot : size_t {
     SLOT_PATCH_OUT = 8, // image staging for patch operators
     SLOT_PWR = 9,       // power-iteration vectors
     SLOT_PLANES = 10,   // SENSE plane-major image frames
-    SLOT_COUNT = 11
+    SLOT_SPLIT = 11,    // patch data as split-BF16 words (mixed-precision products)
+    SLOT_COUNT = 12
 };
 
 }  // namespace mslr

@@ -52,6 +52,26 @@ __global__ void k_pogm_pre(PogmDev *s, const double *__restrict__ lambdas, int n
 }
 
 // X <- z, U <- u; block partials of (restart dot, ||g||^2, ||u - u_old||^2).
+// One element of one scale (FP64 arithmetic on the FP32 state).
+__device__ __forceinline__ void pogm_elem(float xr, float xi, float uor, float uoi, double gr,
+                                          double gi, double alpha, double beta, double gamma,
+                                          float &zr_out, float &zi_out, float &ur_out,
+                                          float &ui_out, double &dot, double &gg, double &dd) {
+    const double ur = xr - alpha * gr, ui = xi - alpha * gi;
+    const double dr = ur - uor, di = ui - uoi;
+    const double zr = ur + beta * dr + gamma * (ur - xr);
+    const double zi = ui + beta * di + gamma * (ui - xi);
+    zr_out = (float)zr;
+    zi_out = (float)zi;
+    ur_out = (float)ur;
+    ui_out = (float)ui;
+    dot -= gr * dr - gi * di;  // Re(-g .* d), no conjugate
+    gg += gr * gr + gi * gi;
+    dd += dr * dr + di * di;
+}
+
+// Two consecutive elements per thread (16-byte loads) when n and the scale
+// stride are even, the X and U of up to four scales loaded before any store.
 __global__ __launch_bounds__(256) void k_pogm_update(c64 *__restrict__ X, c64 *__restrict__ U,
                                                      const c64 *__restrict__ g, int64_t n,
                                                      int nscales, int64_t sstride,
@@ -60,22 +80,49 @@ __global__ __launch_bounds__(256) void k_pogm_update(c64 *__restrict__ X, c64 *_
     __shared__ double red[16];
     const double alpha = s->alpha, beta = s->beta, gamma = s->gamma;
     double dot = 0.0, gg = 0.0, dd = 0.0;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const c64 gv = g[e];
-        const double gr = gv.re, gi = gv.im;
-        for (int k = 0; k < nscales; ++k) {
-            const int64_t i = (int64_t)k * sstride + e;
-            const c64 xv = X[i], uo = U[i];
-            const double ur = xv.re - alpha * gr, ui = xv.im - alpha * gi;
-            const double dr = ur - uo.re, di = ui - uo.im;
-            const double zr = ur + beta * dr + gamma * (ur - xv.re);
-            const double zi = ui + beta * di + gamma * (ui - xv.im);
-            X[i] = c64{(float)zr, (float)zi};
-            U[i] = c64{(float)ur, (float)ui};
-            dot -= gr * dr - gi * di;  // Re(-g .* d), no conjugate
-            gg += gr * gr + gi * gi;
-            dd += dr * dr + di * di;
+    const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((n & 1) == 0 && (sstride & 1) == 0) {
+        const int64_t n2 = n / 2, s2 = sstride / 2;
+        float4 *X4 = reinterpret_cast<float4 *>(X);
+        float4 *U4 = reinterpret_cast<float4 *>(U);
+        const float4 *g4 = reinterpret_cast<const float4 *>(g);
+        for (int64_t e = t0; e < n2; e += gstride) {
+            const float4 gv = g4[e];
+            for (int k0 = 0; k0 < nscales; k0 += 4) {
+                float4 xv[4], uv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (k0 + q < nscales) {
+                        xv[q] = X4[(int64_t)(k0 + q) * s2 + e];
+                        uv[q] = U4[(int64_t)(k0 + q) * s2 + e];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (k0 + q < nscales) {
+                        float4 zo, uo;
+                        pogm_elem(xv[q].x, xv[q].y, uv[q].x, uv[q].y, gv.x, gv.y, alpha, beta,
+                                  gamma, zo.x, zo.y, uo.x, uo.y, dot, gg, dd);
+                        pogm_elem(xv[q].z, xv[q].w, uv[q].z, uv[q].w, gv.z, gv.w, alpha, beta,
+                                  gamma, zo.z, zo.w, uo.z, uo.w, dot, gg, dd);
+                        X4[(int64_t)(k0 + q) * s2 + e] = zo;
+                        U4[(int64_t)(k0 + q) * s2 + e] = uo;
+                    }
+                }
+            }
+        }
+    } else {  // odd sizes: one element at a time
+        for (int64_t e = t0; e < n; e += gstride) {
+            const c64 gv = g[e];
+            for (int k = 0; k < nscales; ++k) {
+                const int64_t i = (int64_t)k * sstride + e;
+                float zr, zi, ur, ui;
+                pogm_elem(X[i].re, X[i].im, U[i].re, U[i].im, gv.re, gv.im, alpha, beta, gamma,
+                          zr, zi, ur, ui, dot, gg, dd);
+                X[i] = c64{zr, zi};
+                U[i] = c64{ur, ui};
+            }
         }
     }
     const double a = block_sum(dot, red);

@@ -33,7 +33,8 @@ constexpr int S2T = 768;       // threads of the band -> tridiagonal kernel
 constexpr int S2W = S2T / 64;  // concurrent sweeps
 constexpr int STAGGER = 2;     // time steps between the starts of consecutive sweeps
 constexpr int BTT = 256;       // threads of the back-transformation
-constexpr int BTC = 14;        // eigenvectors per back-transformation workgroup (LDS for two per CU)
+constexpr int BTC = 13;        // eigenvectors per back-transformation workgroup: one thread per
+                               // (reflector, eigenvector) of a sweep, LDS for two workgroups per CU
 
 __device__ __forceinline__ c128 c_div(c128 a, c128 b) {
     const double den = b.re * b.re + b.im * b.im;
@@ -595,10 +596,13 @@ __global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, in
 // Q2, the sweep-major product of the bulge-chasing reflectors, is applied in
 // reverse sweep order, z <- z - tau v (v^H z); the reflectors of one sweep act
 // on disjoint row windows and are applied together, one thread per (reflector,
-// eigenvector) with its 16-term product in registers (the reflector is read from
-// global memory: the BTC threads of a reflector share its 256 bytes).  Q1, the
-// panels in reverse order, as U <- U - V (T (V^H U)) with V read from the
-// reduced matrix (unit lower form; the tails stored below the band).
+// eigenvector) with its 16-term product in registers.  A sweep's reflectors are
+// staged in LDS, the next sweep's loaded while the current one is applied.  Q1,
+// the panels in reverse order, as U <- U - V (T (V^H U)) with V (unit lower
+// form, tails stored below the band of the reduced matrix) passed through LDS
+// in 16 x 16 tiles, again loading the next tile during the current one.
+constexpr int BT_KS = (SBR_MAXM - 1 + B - 1) / B;  // reflectors per sweep, at most
+
 __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, int m, int npanel,
                                                 int ks, const c128 *__restrict__ Tall,
                                                 const c128 *__restrict__ V2all,
@@ -609,6 +613,8 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ c128 Ts[B][B + 1];
     __shared__ c128 XY[B][BTC + 1];
+    __shared__ c128 Vb[2][BT_KS * B];  // a sweep's reflectors / a 16 x 16 tile of V
+    __shared__ c128 Tb[2][BT_KS];
     const int ip = blockIdx.x;
     const int r = rall[ip];
     const int c0 = blockIdx.y * BTC;
@@ -624,92 +630,116 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         if (c < ncol) z.re = Z[(int64_t)(c0 + c) * m + i];
         Uc[idx] = z;
     }
-    __syncthreads();
     // Q2
     {
         const c128 *V2 = V2all + (int64_t)ip * m * ks * B;
         const c128 *tau2 = tau2all + (int64_t)ip * m * ks;
-        for (int s = m - 2; s >= 0; --s) {
-            const int nst = (m - s - 1 + B - 1) / B;
-            for (int t = tid; t < nst * BTC; t += BTT) {
-                const int k = t / BTC, c = t - k * BTC;
-                const int64_t slot = (int64_t)s * ks + k;
-                const c128 tk = tau2[slot];
-                if (c < ncol && nonzero(tk)) {
-                    const int st = s + 1 + k * B;
-                    const int len = min(B, m - st);
-                    const c128 *v = V2 + slot * B;
-                    c128 *u = Uc + c * m + st;
-                    c128 vv[B], uu[B];
+        auto nsweep_refl = [&](int s) { return (m - s - 1 + B - 1) / B; };
+        // this thread's share of a sweep's reflectors (<= 2 values) and tau
+        c128 pv0, pv1, pt;
+        auto fetch = [&](int s) {
+            const int n = nsweep_refl(s) * B;
+            const c128 *src = V2 + (int64_t)s * ks * B;
+            pv0 = ld_if(tid < n, src + tid);
+            pv1 = ld_if(tid + BTT < n, src + tid + BTT);
+            pt = ld_if(tid < nsweep_refl(s), tau2 + (int64_t)s * ks + tid);
+        };
+        auto put = [&](int buf) {
+            Vb[buf][tid] = pv0;
+            if (tid + BTT < BT_KS * B) Vb[buf][tid + BTT] = pv1;
+            if (tid < BT_KS) Tb[buf][tid] = pt;
+        };
+        const int k = tid / BTC, c = tid - k * BTC;  // (reflector, eigenvector) of this thread
+        if (m >= 2) {
+            fetch(m - 2);
+            put(0);
+        }
+        __syncthreads();
+        for (int s = m - 2, buf = 0; s >= 0; --s, buf ^= 1) {
+            if (s > 0) fetch(s - 1);
+            const int nst = nsweep_refl(s);
+            const c128 tk = Tb[buf][k < BT_KS ? k : 0];
+            if (k < nst && c < ncol && nonzero(tk)) {
+                const int st = s + 1 + k * B;
+                const int len = min(B, m - st);
+                const c128 *v = &Vb[buf][k * B];
+                c128 *u = Uc + c * m + st;
+                c128 vv[B], uu[B];
 #pragma unroll
-                    for (int q = 0; q < B; ++q) {
-                        vv[q] = ld_if(q < len, v + q);
-                        uu[q] = ld_if(q < len, u + q);
-                    }
-                    c128 d0 = {0.0, 0.0}, d1 = {0.0, 0.0};
-#pragma unroll
-                    for (int q = 0; q < B; q += 2) {
-                        d0 = d0 + cmulc(vv[q], uu[q]);
-                        d1 = d1 + cmulc(vv[q + 1], uu[q + 1]);
-                    }
-                    const c128 f = tk * (d0 + d1);
-#pragma unroll
-                    for (int q = 0; q < B; ++q)
-                        if (q < len) u[q] = uu[q] - f * vv[q];
+                for (int q = 0; q < B; ++q) {
+                    vv[q] = ld_if(q < len, v + q);
+                    uu[q] = ld_if(q < len, u + q);
                 }
+                c128 d0 = {0.0, 0.0}, d1 = {0.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < B; q += 2) {
+                    d0 = d0 + cmulc(vv[q], uu[q]);
+                    d1 = d1 + cmulc(vv[q + 1], uu[q + 1]);
+                }
+                const c128 f = tk * (d0 + d1);
+#pragma unroll
+                for (int q = 0; q < B; ++q)
+                    if (q < len) u[q] = uu[q] - f * vv[q];
             }
+            if (s > 0) put(buf ^ 1);
             __syncthreads();
         }
     }
     // Q1
+    const int ta = tid % B, tr = tid / B;  // tile slot: V[i0 + tr][ta]; also (a, c) / (row, c)
     for (int p = npanel - 1; p >= 0; --p) {
         const int pc0 = p * B, r0 = pc0 + B, L = m - r0;
         const int kq = min(B, L - 1);
         for (int pr = tid; pr < B * B; pr += BTT)
             Ts[pr % B][pr / B] = Tall[((int64_t)ip * npanel + p) * B * B + pr];
-        {   // X = V^H U, one entry per thread, four partial sums
-            const int a = tid % B, c = tid / B;
-            if (c < BTC) {
-                c128 x = {0.0, 0.0};
-                if (a < kq) {
-                    const c128 *va = A + r0 + (int64_t)m * (pc0 + a);
-                    const c128 *uc = Uc + c * m + r0;
-                    c128 s0 = uc[a], s1 = {0.0, 0.0}, s2 = {0.0, 0.0}, s3 = {0.0, 0.0};
-                    int i = a + 1;
-                    for (; i + 3 < L; i += 4) {
-                        s0 = s0 + cmulc(va[i], uc[i]);
-                        s1 = s1 + cmulc(va[i + 1], uc[i + 1]);
-                        s2 = s2 + cmulc(va[i + 2], uc[i + 2]);
-                        s3 = s3 + cmulc(va[i + 3], uc[i + 3]);
-                    }
-                    for (; i < L; ++i) s0 = s0 + cmulc(va[i], uc[i]);
-                    x = (s0 + s1) + (s2 + s3);
+        auto vtile = [&](int i0) {
+            const int i = i0 + tr;
+            c128 v = ld_if(ta < kq && i > ta && i < L, A + (r0 + i) + (int64_t)m * (pc0 + ta));
+            if (ta < kq && i == ta) v.re = 1.0;
+            return v;
+        };
+        // X = V^H U: thread (a, c) = (ta, tr)
+        c128 x0 = {0.0, 0.0}, x1 = {0.0, 0.0};
+        c128 pv = vtile(0);
+        for (int i0 = 0, buf = 0; i0 < L; i0 += B, buf ^= 1) {
+            Vb[buf][tr * B + ta] = pv;
+            __syncthreads();
+            if (i0 + B < L) pv = vtile(i0 + B);
+            if (tr < BTC) {
+                const c128 *uc = Uc + tr * m + r0 + i0;
+                const int n = min(B, L - i0);
+#pragma unroll
+                for (int ii = 0; ii < B; ii += 2) {
+                    if (ii < n) x0 = x0 + cmulc(Vb[buf][ii * B + ta], uc[ii]);
+                    if (ii + 1 < n) x1 = x1 + cmulc(Vb[buf][(ii + 1) * B + ta], uc[ii + 1]);
                 }
-                XY[a][c] = x;
             }
         }
+        if (tr < BTC) XY[ta][tr] = x0 + x1;
         __syncthreads();
         {   // Y = T X (T upper)
-            const int a = tid % B, c = tid / B;
             c128 y = {0.0, 0.0};
-            if (c < BTC)
-                for (int q = a; q < B; ++q) y = y + Ts[a][q] * XY[q][c];
+            if (tr < BTC)
+                for (int q = ta; q < B; ++q) y = y + Ts[ta][q] * XY[q][tr];
             __syncthreads();
-            if (c < BTC) XY[a][c] = y;
+            if (tr < BTC) XY[ta][tr] = y;
         }
-        __syncthreads();
-        // U <- U - V Y
-        for (int idx = tid; idx < L * BTC; idx += BTT) {
-            const int i = idx % L, c = idx / L;
-            const c128 *vrow = A + (r0 + i) + (int64_t)m * pc0;
-            c128 y = {0.0, 0.0};
+        // U <- U - V Y: thread (row i0 + ta, c = tr)
+        pv = vtile(0);
+        for (int i0 = 0, buf = 0; i0 < L; i0 += B, buf ^= 1) {
+            Vb[buf][tr * B + ta] = pv;
+            __syncthreads();
+            if (i0 + B < L) pv = vtile(i0 + B);
+            const int i = i0 + ta;
+            if (tr < BTC && i < L) {
+                c128 y0 = {0.0, 0.0}, y1 = {0.0, 0.0};
 #pragma unroll
-            for (int a = 0; a < B; ++a) {
-                c128 v = ld_if(a < kq && i > a, vrow + (int64_t)m * a);
-                if (a < kq && i == a) v.re = 1.0;
-                y = y + v * XY[a][c];
+                for (int a2 = 0; a2 < B; a2 += 2) {
+                    y0 = y0 + Vb[buf][ta * B + a2] * XY[a2][tr];
+                    y1 = y1 + Vb[buf][ta * B + a2 + 1] * XY[a2 + 1][tr];
+                }
+                Uc[tr * m + r0 + i] = Uc[tr * m + r0 + i] - (y0 + y1);
             }
-            Uc[c * m + r0 + i] = Uc[c * m + r0 + i] - y;
         }
         __syncthreads();
     }

@@ -121,6 +121,34 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, i
         }
 }
 
+// Epilogue of the Gram tile kernels.  With one contraction chunk the sums are
+// final: entries a <= b go straight into the Hermitian matrix with their mirror
+// (as k_gram_reduce writes them: real diagonal, conjugate for conj_out, a
+// non-finite sum flagged and zeroed); otherwise into the partial tiles.
+__device__ __forceinline__ void gram_store(c128 *__restrict__ part, c128 *__restrict__ G, int ip,
+                                           int nkc, int kc, int m, int a, int b, double sr,
+                                           double si, int conj_out, int *err) {
+    if (a >= m || b >= m) return;
+    if (G == nullptr) {
+        part[((int64_t)ip * nkc + kc) * (int64_t)m * m + a + (int64_t)m * b] = c128{sr, si};
+        return;
+    }
+    if (a > b) return;
+    if (conj_out) si = -si;
+    if (!isfinite(sr) || !isfinite(si)) {
+        if (err) *err = 1;
+        sr = 0.0;
+        si = 0.0;
+    }
+    c128 *g = G + (int64_t)ip * m * m;
+    if (a == b) {
+        g[a + (int64_t)m * b] = c128{sr, 0.0};
+    } else {
+        g[a + (int64_t)m * b] = c128{sr, si};
+        g[b + (int64_t)m * a] = c128{sr, -si};
+    }
+}
+
 // FP64 Gram tiles on the matrix cores (v_mfma_f64_16x16x4f64): same tiling,
 // staging and output as k_gram_tiles<double>.  Per wave a 32x32 sub-tile as
 // 2x2 blocks of 16x16; each complex block takes four real MFMAs per k-step of
@@ -131,7 +159,9 @@ typedef double md4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_gram_mfma(const c64 *__restrict__ P, int64_t pstride,
                                                    int64_t ks, int64_t cs, int m, int64_t Kc,
-                                                   int nkc, int ntile, c128 *__restrict__ part) {
+                                                   int nkc, int ntile, c128 *__restrict__ part,
+                                                   c128 *__restrict__ G, int conj_out,
+                                                   int *__restrict__ err) {
     __shared__ double Ar[GK][GT + 2], Ai[GK][GT + 2];
     __shared__ double Br[GK][GT + 2], Bi[GK][GT + 2];
     const int ip = blockIdx.z;
@@ -216,7 +246,6 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const c64 *__restrict__ P, in
         }
         __syncthreads();
     }
-    c128 *out = part + ((int64_t)ip * nkc + kc) * (int64_t)m * m;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -225,7 +254,8 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const c64 *__restrict__ P, in
             for (int v = 0; v < 4; ++v) {
                 const int a = a0 + wa + 16 * i + r16;
                 const int b = b0 + wb + 16 * j + k4 + 4 * v;
-                if (a < m && b < m) out[a + (int64_t)m * b] = c128{accr[i][j][v], acci[i][j][v]};
+                gram_store(part, G, ip, nkc, kc, m, a, b, accr[i][j][v], acci[i][j][v], conj_out,
+                           err);
             }
 }
 
@@ -1080,24 +1110,15 @@ __device__ __forceinline__ void stage8(const c64 (&r)[8], unsigned *sh, unsigned
     *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
 }
 
-// Split-BF16 Gram tiles with error-free sums: same tiling, contraction chunks and
-// FP64 partial output as k_gram_mfma, for row-contiguous patches (ks == 1).
-// Per LDS stage (BKS rows) each column a gets one exponent E_a, 2^E_a > max |x|
-// over the stage's rows, and every value is cut into two fixed-point slices
-// x = s1 + s2 + r, s1 a multiple of 2^(E-8) (|s1| <= 2^8 of those), s2 a multiple
-// of 2^(E-16) (<= 2^7 of those), |r| <= 2^(E-17); both slices are exact in BF16
-// (the Ozaki split).  For an entry (a, b) all s1*s1 products of a stage share one
-// quantum and stay below 2^16 of it, so the MFMA sums of a stage (64 products)
-// are exact in FP32, and so are those of the s1*s2 + s2*s1 terms (a quantum 2^8
-// smaller, 128 products) and of the s2*s2 terms (2^16 smaller, 64 products); each
-// stage's three sums go into FP64 accumulators.  The result is the exact Gram of
-// s1 + s2: the FP32 data perturbed by at most 2^-17 of each column's maximum,
-// with no rounding of long sums (which would put eps32 of the dominant
-// eigenvalue onto every small one).  The s2*s2 terms are kept because dropping
-// them is a one-signed bias of the diagonal, ~2^-20 of the dominant eigenvalue,
-// which would bury eigenvalues 1e8 below it.
-// Operands are (X(k, b), X(k, a)): G_re = [br, bi].[ar, ai],
-// G_im = [bi, -br].[ar, ai] over interleaved k.
+// Ozaki slices of row-contiguous patch matrices (rows of each column contiguous,
+// the columns of all patches one after another): per column and block of
+// SPL_ROWS rows one exponent E, 2^E > max |x| over the block (both parts), and
+// x = s1 + s2 + r with s1 a multiple of 2^(E-8) (|s1| <= 2^8 of those), s2 a
+// multiple of 2^(E-16) (<= 2^7 of those), |r| <= 2^(E-17); both slices exact in
+// BF16.  Written as two words per element (BF16 re in the low half, im in the
+// high half): wh = s1, wl = s2.  One wave per (column, block).
+constexpr int SPL_ROWS = 128;
+
 // (s1, s2) of x for the quanta set by sg1 = 3*2^(E+14), sg2 = 3*2^(E+6)
 __device__ __forceinline__ float2 ozaki_split(float x, float sg1, float sg2) {
     const float s1 = (x + sg1) - sg1;
@@ -1106,9 +1127,66 @@ __device__ __forceinline__ float2 ozaki_split(float x, float sg1, float sg2) {
     return make_float2(s1, s2);
 }
 
-__global__ __launch_bounds__(256) void k_gram_bf16(const c64 *__restrict__ P, int64_t pstride,
-                                                   int64_t cs, int m, int64_t Kc, int nkc,
-                                                   int ntile, c128 *__restrict__ part) {
+__device__ __forceinline__ void ozaki_words(c64 v, float sg1, float sg2, unsigned &wh, unsigned &wl) {
+    const float2 sr = ozaki_split(v.re, sg1, sg2);
+    const float2 si = ozaki_split(v.im, sg1, sg2);
+    const bf2v hv = {(__bf16)sr.x, (__bf16)si.x};
+    const bf2v lv = {(__bf16)sr.y, (__bf16)si.y};
+    wh = __builtin_bit_cast(unsigned, hv);
+    wl = __builtin_bit_cast(unsigned, lv);
+}
+
+__global__ __launch_bounds__(256) void k_split_rows(const c64 *__restrict__ X, int64_t p,
+                                                    int64_t ncol, unsigned *__restrict__ Wh,
+                                                    unsigned *__restrict__ Wl) {
+    const int64_t nblk = (p + SPL_ROWS - 1) / SPL_ROWS;
+    const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= nblk * ncol) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t col = wid / nblk, blk = wid - col * nblk;
+    const int64_t off = col * p + blk * SPL_ROWS;
+    const int n = (int)std::min<int64_t>(SPL_ROWS, p - blk * SPL_ROWS);
+    const c64 v0 = lane < n ? X[off + lane] : c64{0.f, 0.f};
+    const c64 v1 = lane + 64 < n ? X[off + lane + 64] : c64{0.f, 0.f};
+    float mx = fmaxf(fmaxf(fabsf(v0.re), fabsf(v0.im)), fmaxf(fabsf(v1.re), fabsf(v1.im)));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    int E;
+    (void)frexpf(mx, &E);  // mx < 2^E (E = 0 for a zero block: all slices 0)
+    const float sg1 = ldexpf(3.f, E + 14), sg2 = ldexpf(3.f, E + 6);
+    unsigned h, l;
+    if (lane < n) {
+        ozaki_words(v0, sg1, sg2, h, l);
+        Wh[off + lane] = h;
+        Wl[off + lane] = l;
+    }
+    if (lane + 64 < n) {
+        ozaki_words(v1, sg1, sg2, h, l);
+        Wh[off + lane + 64] = h;
+        Wl[off + lane + 64] = l;
+    }
+}
+
+// Split-BF16 Gram tiles with error-free sums, from the Ozaki words of
+// k_split_rows (row-contiguous patches): same tiling, contraction chunks and
+// FP64 partial output as k_gram_mfma.  For an entry (a, b) all s1*s1 products of
+// one SPL_ROWS-row block share one quantum and stay below 2^16 of it, so the
+// MFMA sums over a block (256 products) are exact in FP32; so are those of the
+// s1*s2 + s2*s1 terms (a quantum 2^8 smaller, 512 products below 2^15) and of
+// the s2*s2 terms (2^16 smaller).  After each block the three sums go into FP64
+// accumulators.  The result is the exact Gram of s1 + s2: the FP32 data
+// perturbed by at most 2^-17 of each block's maximum, with no rounding of long
+// sums (which would put eps32 of the dominant eigenvalue onto every small one).
+// The s2*s2 terms are kept because dropping them is a one-signed bias of the
+// diagonal, ~2^-20 of the dominant eigenvalue, which would bury eigenvalues 1e8
+// below it.  Operands are (X(k, b), X(k, a)): G_re = [br, bi].[ar, ai],
+// G_im = [bi, -br].[ar, ai] over interleaved k.
+__global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ Wh,
+                                                   const unsigned *__restrict__ Wl,
+                                                   int64_t pstride, int64_t cs, int m, int64_t Kc,
+                                                   int nkc, int ntile, int vec,
+                                                   c128 *__restrict__ part, c128 *__restrict__ G,
+                                                   int *__restrict__ err) {
     __shared__ __align__(16) unsigned Sh[2][GT][BLD];  // [A cols a, B cols b][col][k]
     __shared__ __align__(16) unsigned Sl[2][GT][BLD];
     const int ip = blockIdx.z;
@@ -1120,70 +1198,19 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const c64 *__restrict__ P, in
     }
     const int tj = ti + rem;
     const int a0 = ti * GT, b0 = tj * GT;
-    const c64 *X = P + ip * pstride;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wa = (wave & 1) * 32, wb = (wave >> 1) * 32;
     const int r16 = lane & 15, g4 = lane >> 4;
-    // load slot: one column, eight consecutive rows; the four slots of a column
-    // are adjacent lanes of one wave
+    // load slot: one column, eight consecutive rows
     const int lc = tid >> 2;
     const int lk = (tid & 3) * 8;
     const bool va = a0 + lc < m, vb = b0 + lc < m;
-    const c64 *xa = X + (int64_t)(a0 + lc) * cs;
-    const c64 *xb = X + (int64_t)(b0 + lc) * cs;
+    const int64_t oa = ip * pstride + (int64_t)(a0 + lc) * cs;
+    const int64_t ob = ip * pstride + (int64_t)(b0 + lc) * cs;
     mf4 hr[2][2], hi[2][2], xr[2][2], xi[2][2], lr[2][2], li[2][2];
     md4 dr[2][2], di[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            dr[i][j] = md4{0.0, 0.0, 0.0, 0.0};
-            di[i][j] = md4{0.0, 0.0, 0.0, 0.0};
-        }
-    const int64_t kbeg = (int64_t)kc * KCH;
-    const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
-    c64 ra[8], rb[8];
-    auto fetch = [&](int64_t k0) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int64_t k = k0 + lk + e;
-            const bool kin = k < kend;
-            ra[e] = (kin && va) ? xa[k] : c64{0.f, 0.f};
-            rb[e] = (kin && vb) ? xb[k] : c64{0.f, 0.f};
-        }
-    };
-    // slices of one column's eight values into LDS
-    auto stage = [&](const c64 (&r)[8], unsigned *sh, unsigned *sl) {
-        float mx = 0.f;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fmaxf(fabsf(r[e].re), fabsf(r[e].im)));
-        mx = fmaxf(mx, __shfl_xor(mx, 1, 64));
-        mx = fmaxf(mx, __shfl_xor(mx, 2, 64));
-        int E;
-        (void)frexpf(mx, &E);  // mx < 2^E (E = 0 for a zero column: all slices 0)
-        const float sg1 = ldexpf(3.f, E + 14), sg2 = ldexpf(3.f, E + 6);
-        unsigned h[8], l[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float2 sr = ozaki_split(r[e].re, sg1, sg2);
-            const float2 si = ozaki_split(r[e].im, sg1, sg2);
-            const bf2v hv = {(__bf16)sr.x, (__bf16)si.x};
-            const bf2v lv = {(__bf16)sr.y, (__bf16)si.y};
-            h[e] = __builtin_bit_cast(unsigned, hv);
-            l[e] = __builtin_bit_cast(unsigned, lv);
-        }
-        *(u4 *)sh = u4{h[0], h[1], h[2], h[3]};
-        *(u4 *)(sh + 4) = u4{h[4], h[5], h[6], h[7]};
-        *(u4 *)sl = u4{l[0], l[1], l[2], l[3]};
-        *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
-    };
-    fetch(kbeg);
-    for (int64_t k0 = kbeg; k0 < kend; k0 += BKS) {
-        stage(ra, &Sh[0][lc][lk], &Sl[0][lc][lk]);
-        stage(rb, &Sh[1][lc][lk], &Sl[1][lc][lk]);
-        __syncthreads();
-        if (k0 + BKS < kend) fetch(k0 + BKS);
+    auto zero32 = [&]() {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -1195,6 +1222,62 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const c64 *__restrict__ P, in
                 lr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
                 li[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
             }
+    };
+    zero32();
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            dr[i][j] = md4{0.0, 0.0, 0.0, 0.0};
+            di[i][j] = md4{0.0, 0.0, 0.0, 0.0};
+        }
+    const int64_t kbeg = (int64_t)kc * KCH;
+    const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
+    u4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
+    auto fetch = [&](int64_t k0) {
+        const int64_t k = k0 + lk;
+        if (vec && k + 8 <= kend) {  // 16-byte aligned rows (p % 4 == 0)
+            const u4 z = {0u, 0u, 0u, 0u};
+            ah0 = va ? *(const u4 *)(Wh + oa + k) : z;
+            ah1 = va ? *(const u4 *)(Wh + oa + k + 4) : z;
+            al0 = va ? *(const u4 *)(Wl + oa + k) : z;
+            al1 = va ? *(const u4 *)(Wl + oa + k + 4) : z;
+            bh0 = vb ? *(const u4 *)(Wh + ob + k) : z;
+            bh1 = vb ? *(const u4 *)(Wh + ob + k + 4) : z;
+            bl0 = vb ? *(const u4 *)(Wl + ob + k) : z;
+            bl1 = vb ? *(const u4 *)(Wl + ob + k + 4) : z;
+        } else {
+            unsigned t[4][8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const bool kin = k + e < kend;
+                t[0][e] = (kin && va) ? Wh[oa + k + e] : 0u;
+                t[1][e] = (kin && va) ? Wl[oa + k + e] : 0u;
+                t[2][e] = (kin && vb) ? Wh[ob + k + e] : 0u;
+                t[3][e] = (kin && vb) ? Wl[ob + k + e] : 0u;
+            }
+            ah0 = u4{t[0][0], t[0][1], t[0][2], t[0][3]};
+            ah1 = u4{t[0][4], t[0][5], t[0][6], t[0][7]};
+            al0 = u4{t[1][0], t[1][1], t[1][2], t[1][3]};
+            al1 = u4{t[1][4], t[1][5], t[1][6], t[1][7]};
+            bh0 = u4{t[2][0], t[2][1], t[2][2], t[2][3]};
+            bh1 = u4{t[2][4], t[2][5], t[2][6], t[2][7]};
+            bl0 = u4{t[3][0], t[3][1], t[3][2], t[3][3]};
+            bl1 = u4{t[3][4], t[3][5], t[3][6], t[3][7]};
+        }
+    };
+    fetch(kbeg);
+    for (int64_t k0 = kbeg; k0 < kend; k0 += BKS) {
+        *(u4 *)&Sh[0][lc][lk] = ah0;
+        *(u4 *)&Sh[0][lc][lk + 4] = ah1;
+        *(u4 *)&Sl[0][lc][lk] = al0;
+        *(u4 *)&Sl[0][lc][lk + 4] = al1;
+        *(u4 *)&Sh[1][lc][lk] = bh0;
+        *(u4 *)&Sh[1][lc][lk + 4] = bh1;
+        *(u4 *)&Sl[1][lc][lk] = bl0;
+        *(u4 *)&Sl[1][lc][lk + 4] = bl1;
+        __syncthreads();
+        if (k0 + BKS < kend) fetch(k0 + BKS);
 #pragma unroll
         for (int s = 0; s < BKS / 16; ++s) {
             const int kw = 16 * s + 4 * g4;
@@ -1222,18 +1305,22 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const c64 *__restrict__ P, in
                 }
             }
         }
+        // end of a SPL_ROWS-row block (the chunk starts on a block boundary) or of
+        // the chunk: the exact FP32 sums into FP64
+        if (((k0 + BKS - kbeg) % SPL_ROWS) == 0 || k0 + BKS >= kend) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    dr[i][j][v] += ((double)hr[i][j][v] + (double)xr[i][j][v]) + (double)lr[i][j][v];
-                    di[i][j][v] += ((double)hi[i][j][v] + (double)xi[i][j][v]) + (double)li[i][j][v];
-                }
+                    for (int v = 0; v < 4; ++v) {
+                        dr[i][j][v] += ((double)hr[i][j][v] + (double)xr[i][j][v]) + (double)lr[i][j][v];
+                        di[i][j][v] += ((double)hi[i][j][v] + (double)xi[i][j][v]) + (double)li[i][j][v];
+                    }
+            zero32();
+        }
         __syncthreads();
     }
-    c128 *out = part + ((int64_t)ip * nkc + kc) * (int64_t)m * m;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -1242,19 +1329,26 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const c64 *__restrict__ P, in
             for (int v = 0; v < 4; ++v) {
                 const int a = a0 + wa + 16 * i + r16;
                 const int b = b0 + wb + 16 * j + 4 * g4 + v;
-                if (a < m && b < m) out[a + (int64_t)m * b] = c128{dr[i][j][v], di[i][j][v]};
+                gram_store(part, G, ip, nkc, kc, m, a, b, dr[i][j][v], di[i][j][v], 0, err);
             }
 }
 
 // Split-BF16 complex GEMM C = A B (the SVST apply), FP32 accumulation.  Same
 // tiling and output layout as k_cgemm (64x64 per workgroup, 32x32 per wave),
 // operands (B^T, A^T): C_re = [br, bi].[ar, -ai], C_im = [br, bi].[ai, ar].
+// An operand is either FP32 (split here into hi + lo) or given as the words of
+// k_split_rows (Awh/Awl or Bwh/Bwl non-null, same element strides).
 // blockIdx.x runs over the column tiles of C so the tiles sharing a block of
 // rows of A are dispatched together.
+template <bool AW, bool BW>
 __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, int64_t bA,
                                                     int64_t ai, int64_t ak,
                                                     const c64 *__restrict__ B, int64_t bB,
                                                     int64_t bk, int64_t bj,
+                                                    const unsigned *__restrict__ Awh,
+                                                    const unsigned *__restrict__ Awl,
+                                                    const unsigned *__restrict__ Bwh,
+                                                    const unsigned *__restrict__ Bwl,
                                                     c64 *__restrict__ C, int64_t bC, int64_t ci,
                                                     int64_t cj, int M, int N, int K) {
     __shared__ __align__(16) unsigned Sh[2][MT][BLD];  // [A rows i, B cols j][row][k]
@@ -1265,8 +1359,6 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
     const int lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
     const int r16 = lane & 15, g4 = lane >> 4;
-    const c64 *Az = A + (int64_t)z * bA;
-    const c64 *Bz = B + (int64_t)z * bB;
     // load slots: one row of A (column of B), eight consecutive k
     const bool afast = ai == 1, bfast = bk == 1;
     const int ar_ = afast ? tid & 63 : tid >> 2;
@@ -1274,8 +1366,11 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
     const int bc_ = bfast ? tid >> 2 : tid & 63;
     const int bk_ = bfast ? (tid & 3) * 8 : (tid >> 6) * 8;
     const bool av = i0 + ar_ < M, bv = j0 + bc_ < N;
-    const c64 *pa = Az + (int64_t)(i0 + ar_) * ai;
-    const c64 *pb = Bz + (int64_t)(j0 + bc_) * bj;
+    const int64_t oa = (int64_t)z * bA + (int64_t)(i0 + ar_) * ai;
+    const int64_t ob = (int64_t)z * bB + (int64_t)(j0 + bc_) * bj;
+    const c64 *pa = A + oa;
+    const c64 *pb = B + ob;
+    constexpr bool aw = AW, bw = BW;
     mf4 accr[2][2], acci[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -1284,19 +1379,40 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
             accr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
             acci[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
         }
-    c64 ra[8], rb[8];
+    // per operand: FP32 values (split when staged) or the hi / lo words
+    c64 ra[AW ? 1 : 8], rb[BW ? 1 : 8];
+    unsigned wah[AW ? 8 : 1], wal[AW ? 8 : 1], wbh[BW ? 8 : 1], wbl[BW ? 8 : 1];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int ka = k0 + ak_ + e, kb = k0 + bk_ + e;
-            ra[e] = (av && ka < K) ? pa[(int64_t)ka * ak] : c64{0.f, 0.f};
-            rb[e] = (bv && kb < K) ? pb[(int64_t)kb * bk] : c64{0.f, 0.f};
+            const bool ina = av && ka < K, inb = bv && kb < K;
+            if constexpr (AW) {
+                wah[e] = ina ? Awh[oa + (int64_t)ka * ak] : 0u;
+                wal[e] = ina ? Awl[oa + (int64_t)ka * ak] : 0u;
+            } else {
+                ra[e] = ina ? pa[(int64_t)ka * ak] : c64{0.f, 0.f};
+            }
+            if constexpr (BW) {
+                wbh[e] = inb ? Bwh[ob + (int64_t)kb * bk] : 0u;
+                wbl[e] = inb ? Bwl[ob + (int64_t)kb * bk] : 0u;
+            } else {
+                rb[e] = inb ? pb[(int64_t)kb * bk] : c64{0.f, 0.f};
+            }
         }
+    };
+    auto putw = [&](const unsigned (&h)[8], const unsigned (&l)[8], unsigned *sh, unsigned *sl) {
+        *(u4 *)sh = u4{h[0], h[1], h[2], h[3]};
+        *(u4 *)(sh + 4) = u4{h[4], h[5], h[6], h[7]};
+        *(u4 *)sl = u4{l[0], l[1], l[2], l[3]};
+        *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
     };
     if (K > 0) fetch(0);
     for (int k0 = 0; k0 < K; k0 += BKS) {
-        stage8(ra, &Sh[0][ar_][ak_], &Sl[0][ar_][ak_]);
-        stage8(rb, &Sh[1][bc_][bk_], &Sl[1][bc_][bk_]);
+        if constexpr (AW) putw(wah, wal, &Sh[0][ar_][ak_], &Sl[0][ar_][ak_]);
+        else stage8(ra, &Sh[0][ar_][ak_], &Sl[0][ar_][ak_]);
+        if constexpr (BW) putw(wbh, wbl, &Sh[1][bc_][bk_], &Sl[1][bc_][bk_]);
+        else stage8(rb, &Sh[1][bc_][bk_], &Sl[1][bc_][bk_]);
         __syncthreads();
         if (k0 + BKS < K) fetch(k0 + BKS);
 #pragma unroll
@@ -1507,7 +1623,8 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
 
 template <class AT>
 static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, int64_t p,
-                             int64_t Nt, int64_t nb, int *err_dev, bool split = false) {
+                             int64_t Nt, int64_t nb, int *err_dev, const unsigned *Wh = nullptr,
+                             const unsigned *Wl = nullptr) {
     const int m = (int)w.m;
     const bool left = p < Nt;
     const int64_t Kc = left ? Nt : p;
@@ -1519,12 +1636,16 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
     {
         MSLR_PROF(ctx, "gram_tiles");
         if constexpr (std::is_same<AT, double>::value) {
-            if (split && ks == 1)  // the split Gram reads row-contiguous patches
-                k_gram_bf16<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, cs, m, Kc, (int)w.nkc, ntile,
-                                                         (c128 *)w.part);
+            // one contraction chunk: the tiles write the Hermitian matrices directly
+            c128 *Gd = w.nkc == 1 ? w.G : nullptr;
+            if (Wh && ks == 1)  // split words of row-contiguous patches
+                k_gram_bf16<<<gg, 256, 0, ctx->stream>>>(Wh, Wl, p * Nt, cs, m, Kc, (int)w.nkc,
+                                                         ntile, (p % 4) == 0 ? 1 : 0,
+                                                         (c128 *)w.part, Gd, err_dev);
             else
                 k_gram_mfma<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc,
-                                                         ntile, (c128 *)w.part);
+                                                         ntile, (c128 *)w.part, Gd, left ? 1 : 0,
+                                                         err_dev);
         }
         else
             k_gram_tiles<AT><<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc,
@@ -1533,7 +1654,7 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
     }
     const int64_t ne = (int64_t)m * m * nb;
     const unsigned gr = (unsigned)std::min<int64_t>(cdiv(ne, 256), 1 << 20);
-    {
+    if (!(std::is_same<AT, double>::value && w.nkc == 1)) {
         MSLR_PROF(ctx, "gram_reduce");
         k_gram_reduce<AT><<<gr, 256, 0, ctx->stream>>>((const C2<AT> *)w.part, m, (int)w.nkc, (int)nb,
                                                        left ? 1 : 0, w.G, err_dev);
@@ -1585,14 +1706,27 @@ static void cgemm(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, in
 }
 
 static void cgemm_bf16(mslr_ctx_s *ctx, const c64 *A, int64_t bA, int64_t ai, int64_t ak,
-                       const c64 *B, int64_t bB, int64_t bk, int64_t bj, c64 *C, int64_t bC,
-                       int64_t ci, int64_t cj, int M, int N, int K, int64_t nbatch) {
+                       const c64 *B, int64_t bB, int64_t bk, int64_t bj, const unsigned *Awh,
+                       const unsigned *Awl, const unsigned *Bwh, const unsigned *Bwl, c64 *C,
+                       int64_t bC, int64_t ci, int64_t cj, int M, int N, int K, int64_t nbatch) {
     for (int64_t z0 = 0; z0 < nbatch; z0 += 65535) {
         const int64_t nz = std::min<int64_t>(65535, nbatch - z0);
         dim3 g((unsigned)cdiv(N, MT), (unsigned)cdiv(M, MT), (unsigned)nz);
         MSLR_PROF(ctx, "apply");
-        k_cgemm_bf16<<<g, 256, 0, ctx->stream>>>(A + z0 * bA, bA, ai, ak, B + z0 * bB, bB, bk, bj,
-                                                 C + z0 * bC, bC, ci, cj, M, N, K);
+        const unsigned *ah = Awh ? Awh + z0 * bA : nullptr, *al = Awl ? Awl + z0 * bA : nullptr;
+        const unsigned *bh = Bwh ? Bwh + z0 * bB : nullptr, *bl = Bwl ? Bwl + z0 * bB : nullptr;
+        const c64 *Az = A + z0 * bA, *Bz = B + z0 * bB;
+        c64 *Cz = C + z0 * bC;
+        if (Awh && !Bwh)
+            k_cgemm_bf16<true, false><<<g, 256, 0, ctx->stream>>>(Az, bA, ai, ak, Bz, bB, bk, bj, ah,
+                                                                  al, bh, bl, Cz, bC, ci, cj, M, N, K);
+        else if (!Awh && Bwh)
+            k_cgemm_bf16<false, true><<<g, 256, 0, ctx->stream>>>(Az, bA, ai, ak, Bz, bB, bk, bj, ah,
+                                                                  al, bh, bl, Cz, bC, ci, cj, M, N, K);
+        else
+            k_cgemm_bf16<false, false><<<g, 256, 0, ctx->stream>>>(Az, bA, ai, ak, Bz, bB, bk, bj, ah,
+                                                                   al, bh, bl, Cz, bC, ci, cj, M, N,
+                                                                   K);
         HIP_LAUNCH_CHECK();
     }
 }
@@ -1624,10 +1758,23 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         const int64_t nb = std::min<int64_t>(w.chunk, Np - ip0);
         const c64 *Pc = Pin + ip0 * len;
         c64 *Yc = Pout + ip0 * len;
+        // split products: the chunk's patches once as split-BF16 words for the Gram
+        // (the apply splits the FP32 data per element: no block-relative residual)
+        const unsigned *Wh = nullptr, *Wl = nullptr;
+        if (split && !left) {
+            unsigned *wb = (unsigned *)ctx->scratch_get(SLOT_SPLIT, (size_t)len * nb * 8);
+            const int64_t nw = cdiv(p, SPL_ROWS) * Nt * nb;
+            MSLR_PROF(ctx, "split");
+            k_split_rows<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(Pc, p, Nt * nb, wb,
+                                                                         wb + len * nb);
+            HIP_LAUNCH_CHECK();
+            Wh = wb;
+            Wl = wb + len * nb;
+        }
         // FP64 products: the Gram squares the dynamic range of X, and at production
         // shapes (global patch s1/s_noise ~ 1e4) FP32 products would bury the
         // singular values near the threshold under rounding error.
-        gram_and_tridiag<double>(ctx, w, Pc, p, Nt, nb, err_dev, split);
+        gram_and_tridiag<double>(ctx, w, Pc, p, Nt, nb, err_dev, Wh, Wl);
         {
             const double *bp = beta_dev + (per_patch ? ip0 : 0);
             MSLR_PROF(ctx, "eigvec");
@@ -1662,15 +1809,16 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         if (!left) {
             // Y (p x m) = X (p x m) W
             if (split)
-                cgemm_bf16(ctx, Pc, len, 1, p, Wm, m * m, 1, m, Yc, len, 1, p, (int)p, mi, mi, nb);
+                cgemm_bf16(ctx, Pc, len, 1, p, Wm, m * m, 1, m, nullptr, nullptr, nullptr, nullptr,
+                           Yc, len, 1, p, (int)p, mi, mi, nb);
             else
                 cgemm(ctx, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p,
                       (int)p, mi, mi, nullptr, nb);
         } else {
             // Y (p x Nt) = W (p x p) X
             if (split)
-                cgemm_bf16(ctx, Wm, m * m, 1, m, Pc, len, 1, p, Yc, len, 1, p, (int)p, (int)Nt, mi,
-                           nb);
+                cgemm_bf16(ctx, Wm, m * m, 1, m, Pc, len, 1, p, nullptr, nullptr, nullptr, nullptr,
+                           Yc, len, 1, p, (int)p, (int)Nt, mi, nb);
             else
                 cgemm(ctx, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1, p,
                       (int)p, (int)Nt, mi, nullptr, nb);
