@@ -72,6 +72,59 @@ __global__ void k_img2patches(const c64 *__restrict__ img, c64 *__restrict__ P, 
     }
 }
 
+// Image offset (frame 0) of element i of patch ip.
+__device__ __forceinline__ int64_t patch_voxel(const PatchGeom &g, int64_t ip, int i) {
+    const int ix = (int)(ip % g.ns[0]);
+    const int iy = (int)((ip / g.ns[0]) % g.ns[1]);
+    const int iz = (int)(ip / ((int64_t)g.ns[0] * g.ns[1]));
+    const int dx = i % g.ps[0];
+    const int dy = (i / g.ps[0]) % g.ps[1];
+    const int dz = i / (g.ps[0] * g.ps[1]);
+    const int x = g.start(0, ix) + dx;
+    const int y = g.start(1, iy) + dy;
+    const int z = g.start(2, iz) + dz;
+    return x + (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z);
+}
+
+// One thread per (patch element i, patch), looping over the frames: the index
+// arithmetic once per thread, then strided copies (P rows coalesced, image
+// reads in runs of ps[0] voxels).
+__global__ __launch_bounds__(256) void k_img2patches_fr(const c64 *__restrict__ img,
+                                                        c64 *__restrict__ P, PatchGeom g,
+                                                        int64_t ip0, int64_t np) {
+    const int64_t nrc = (g.p + 255) / 256;
+    const int64_t ipl = (int64_t)blockIdx.x / nrc;
+    const int64_t i = ((int64_t)blockIdx.x - ipl * nrc) * 256 + threadIdx.x;
+    if (i >= g.p || ipl >= np) return;
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    const c64 *src = img + patch_voxel(g, ip0 + ipl, (int)i);
+    c64 *dst = P + i + g.p * g.Nt * ipl;
+#pragma unroll 4
+    for (int t = 0; t < g.Nt; ++t) dst[g.p * t] = src[V * t];
+}
+
+// The inverse for an exact tiling (every voxel in exactly one patch): a copy,
+// one thread per image voxel looping over the frames (image writes coalesced,
+// patch reads in runs of ps[0]).
+__global__ __launch_bounds__(256) void k_patches2img_tiled(const c64 *__restrict__ P,
+                                                           c64 *__restrict__ img, PatchGeom g) {
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= V) return;
+    const int x = (int)(v % g.N[0]);
+    const int y = (int)((v / g.N[0]) % g.N[1]);
+    const int z = (int)(v / ((int64_t)g.N[0] * g.N[1]));
+    // exact tiling: patch starts are multiples of the patch size
+    const int ix = x / g.ps[0], iy = y / g.ps[1], iz = z / g.ps[2];
+    const int dx = x - ix * g.ps[0], dy = y - iy * g.ps[1], dz = z - iz * g.ps[2];
+    const int64_t ip = ix + (int64_t)g.ns[0] * (iy + (int64_t)g.ns[1] * iz);
+    const int64_t i = dx + (int64_t)g.ps[0] * (dy + (int64_t)g.ps[1] * dz);
+    const c64 *src = P + i + g.p * g.Nt * ip;
+    c64 *dst = img + v;
+#pragma unroll 4
+    for (int t = 0; t < g.Nt; ++t) dst[V * t] = src[g.p * t];
+}
+
 // Range of patch indices along axis a covering coordinate x: [lo, hi].
 __device__ __forceinline__ void cover_range(const PatchGeom &g, int a, int x, int &lo, int &hi) {
     int l = x - g.ps[a] + 1;
@@ -136,7 +189,11 @@ void launch_img2patches(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64
     if (n == 0) return;
     {
         MSLR_PROF(ctx, "img2patches");
-        k_img2patches<<<grid_for(n), 256, 0, ctx->stream>>>(img, P, g, ip0, np);
+        const int64_t nb = cdiv(g.p, 256) * np;
+        if (nb < (int64_t)1 << 31)
+            k_img2patches_fr<<<(unsigned)nb, 256, 0, ctx->stream>>>(img, P, g, ip0, np);
+        else
+            k_img2patches<<<grid_for(n), 256, 0, ctx->stream>>>(img, P, g, ip0, np);
         HIP_LAUNCH_CHECK();
     }
 }
@@ -145,7 +202,11 @@ void launch_patches2img(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *P, c64 *
     const int64_t n = (int64_t)g.N[0] * g.N[1] * g.N[2] * g.Nt;
     {
         MSLR_PROF(ctx, "patches2img");
-        k_patches2img<<<grid_for(n), 256, 0, ctx->stream>>>(P, img, g);
+        const int64_t nb = cdiv((int64_t)g.N[0] * g.N[1] * g.N[2], 256);
+        if (g.exact_tiling() && nb < (int64_t)1 << 31)
+            k_patches2img_tiled<<<(unsigned)nb, 256, 0, ctx->stream>>>(P, img, g);
+        else
+            k_patches2img<<<grid_for(n), 256, 0, ctx->stream>>>(P, img, g);
         HIP_LAUNCH_CHECK();
     }
 }
