@@ -118,35 +118,38 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             V[j * L + i] = A[(r0 + i) + (int64_t)m * (c0 + j)];
         }
         __syncthreads();
-        // 2. Householder QR of the panel.  Per column j: wave 0 forms the
-        //    reflector (lane l owns rows l + 64t, so it needs only shuffles), then
-        //    all waves apply it to the later columns, one column per wave.
-        for (int j = 0; j < kq; ++j) {
+        // 2. Householder QR of the panel.  Per column j all waves apply reflector j
+        //    to the later columns, one column per wave; wave 0 takes column j + 1 and
+        //    forms reflector j + 1 right after updating it (look-ahead), so a column
+        //    costs one workgroup barrier.  Lane l owns rows l + 64t of a column.
+        auto make_reflector = [&](int j) {  // wave 0
             c128 *vj = V + j * L;
-            if (wave == 0) {
-                double xn = 0.0;
-                c128 own = {0.0, 0.0};
-                for (int i = lane; i < L; i += 64) {
-                    if (i > j) xn += abs2(vj[i]);
-                    if (i == j) own = vj[i];
-                }
-                xn = wave_sum64(xn);
-                const c128 alpha = {readlane_d(own.re, j & 63), readlane_d(own.im, j & 63)};
-                const Hh h = householder(alpha, xn);
-                if (nonzero(h.tau))
-                    for (int i = lane; i < L; i += 64)
-                        if (i > j) vj[i] = vj[i] * h.scal;
-                if (lane == 0) {
-                    taus[j] = h.tau;
-                    betas[j] = h.beta;
-                }
+            double xn = 0.0;
+            c128 own = {0.0, 0.0};
+            for (int i = lane; i < L; i += 64) {
+                if (i > j) xn += abs2(vj[i]);
+                if (i == j) own = vj[i];
             }
-            __syncthreads();
+            xn = wave_sum64(xn);
+            const c128 alpha = {readlane_d(own.re, j & 63), readlane_d(own.im, j & 63)};
+            const Hh h = householder(alpha, xn);
+            if (nonzero(h.tau))
+                for (int i = lane; i < L; i += 64)
+                    if (i > j) vj[i] = vj[i] * h.scal;
+            if (lane == 0) {
+                taus[j] = h.tau;
+                betas[j] = h.beta;
+            }
+        };
+        if (wave == 0 && kq > 0) make_reflector(0);
+        __syncthreads();
+        for (int j = 0; j < kq; ++j) {
+            const c128 *vj = V + j * L;
             const c128 tj = taus[j];
-            if (nonzero(tj)) {
-                // column c > j: x <- H^H x = x - conj(tau) v (v^H x)
-                const c128 ct = conj(tj);
-                for (int c = j + 1 + wave; c < B; c += S1T / 64) {
+            // column c > j: x <- H^H x = x - conj(tau) v (v^H x)
+            const c128 ct = conj(tj);
+            for (int c = j + 1 + wave; c < B; c += S1T / 64) {
+                if (nonzero(tj)) {
                     c128 *pc = V + c * L;
                     c128 sacc = {0.0, 0.0};
                     for (int i = j + lane; i < L; i += 64) {
@@ -161,6 +164,10 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
                         if (i == j) vi.re = 1.0;
                         pc[i] = pc[i] - vi * f;
                     }
+                }
+                if (c == j + 1 && j + 1 < kq) {  // wave 0: the next reflector
+                    wave_lds_sync();
+                    make_reflector(j + 1);
                 }
             }
             __syncthreads();
@@ -215,17 +222,25 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
         }
         __syncthreads();
         if (wave == 0) {
+            // T[0:j, j] = -tau_j T[0:j, 0:j] S[0:j, j], column by column; row `lane` of
+            // T is read back from LDS (zero left of the diagonal), the S entries are
+            // broadcasts, two partial sums
             for (int j = 0; j < B; ++j) {
                 const c128 tj = taus[j];
-                const c128 tmp = lane < j ? (-1.0) * (tj * Ms[lane][j]) : c128{0.0, 0.0};
-                c128 acc = {0.0, 0.0};
+                c128 a0 = {0.0, 0.0}, a1 = {0.0, 0.0};
+                const int row = lane < B ? lane : 0;
+#pragma unroll 4
                 for (int q = 0; q < j; ++q) {
-                    const c128 tq = {readlane_d(tmp.re, q), readlane_d(tmp.im, q)};
-                    if (lane <= q) acc = acc + Ts[lane][q] * tq;
+                    const c128 tq = (-1.0) * (tj * Ms[q][j]);
+                    const c128 t = Ts[row][q];
+                    if (q & 1) a1 = a1 + t * tq;
+                    else a0 = a0 + t * tq;
                 }
+                const c128 acc = a0 + a1;
                 if (lane < j) Ts[lane][j] = acc;
                 else if (lane == j) Ts[j][j] = tj;
                 else if (lane < B) Ts[lane][j] = c128{0.0, 0.0};
+                wave_lds_sync();
             }
         }
         __syncthreads();
