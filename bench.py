@@ -223,7 +223,9 @@ def main() -> int:
     ap.add_argument("--config", default="c1")
     ap.add_argument("--form", default="mixed", choices=["plain", "batched", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-prof", action="store_true", help="skip the plain pass with per-kernel event timing")
+    ap.add_argument("--no-prof", action="store_true", help="skip the pass with per-kernel event timing")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the batched FP32 and plain runs (rates and result differences)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) five times, to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -308,7 +310,7 @@ def main() -> int:
     others = {}
     kern = {}
     nr_form = nrmse_truth(x_form)
-    if args.form != "plain":
+    if args.form != "plain" and not args.no_compare:
         for other in (["batched"] if args.form == "mixed" else []) + ["plain"]:
             sp = make(other, W + K)
             sp.run(W)

@@ -1110,6 +1110,17 @@ __device__ __forceinline__ void stage8(const c64 (&r)[8], unsigned *sh, unsigned
     *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
 }
 
+// Work-group index renumbered so that consecutive numbers run on one XCD: the
+// dispatcher hands consecutive work-groups to the 8 XCDs in turn, each with its
+// own L2, so tiles that share operands and are numbered consecutively would
+// otherwise fetch the shared data once per XCD.  The last n % 8 keep their order.
+__device__ __forceinline__ int64_t xcd_order(int64_t w, int64_t n) {
+    constexpr int NXCD = 8;
+    const int64_t per = n / NXCD;
+    if (w >= per * NXCD) return w;
+    return (w % NXCD) * per + w / NXCD;
+}
+
 // Ozaki slices of row-contiguous patch matrices (rows of each column contiguous,
 // the columns of all patches one after another): per column and block of
 // SPL_ROWS rows one exponent E, 2^E > max |x| over the block (both parts), and
@@ -1189,9 +1200,12 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ 
                                                    int *__restrict__ err) {
     __shared__ __align__(16) unsigned Sh[2][GT][BLD];  // [A cols a, B cols b][col][k]
     __shared__ __align__(16) unsigned Sl[2][GT][BLD];
-    const int ip = blockIdx.z;
-    const int kc = blockIdx.y;
-    int rem = blockIdx.x, ti = 0;
+    // the tile pairs of one (patch, chunk) share column tiles: keep them on one XCD
+    const int64_t nwg = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t wl = xcd_order(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z), nwg);
+    const int ip = (int)(wl / ((int64_t)gridDim.x * gridDim.y));
+    const int kc = (int)((wl / gridDim.x) % gridDim.y);
+    int rem = (int)(wl % gridDim.x), ti = 0;
     while (rem >= ntile - ti) {
         rem -= ntile - ti;
         ++ti;
@@ -1353,8 +1367,11 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
                                                     int64_t cj, int M, int N, int K) {
     __shared__ __align__(16) unsigned Sh[2][MT][BLD];  // [A rows i, B cols j][row][k]
     __shared__ __align__(16) unsigned Sl[2][MT][BLD];
-    const int z = blockIdx.z;
-    const int j0 = blockIdx.x * MT, i0 = blockIdx.y * MT;
+    // the column tiles of one block of rows share A: keep them on one XCD
+    const int64_t nwg = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t wl = xcd_order(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z), nwg);
+    const int z = (int)(wl / ((int64_t)gridDim.x * gridDim.y));
+    const int j0 = (int)(wl % gridDim.x) * MT, i0 = (int)((wl / gridDim.x) % gridDim.y) * MT;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
