@@ -39,6 +39,18 @@ constexpr int WS_PER = 256;  // doubles per Gram row of eigen workspace per patc
 
 // ------------------------------------------------------------------ Gram
 
+// Work-group index renumbered so that consecutive numbers run on one XCD: the
+// dispatcher hands consecutive work-groups to the 8 XCDs in turn, each with its
+// own L2, so tiles that share operands and are numbered consecutively would
+// otherwise fetch the shared data once per XCD.  The last n % 8 keep their order.
+__device__ __forceinline__ int64_t xcd_order(int64_t w, int64_t n) {
+    constexpr int NXCD = 8;
+    const int64_t per = n / NXCD;
+    if (w >= per * NXCD) return w;
+    return (w % NXCD) * per + w / NXCD;
+}
+
+
 // C[a,b] += sum_k conj(A(k,a)) A(k,b), A(k,a) = base[k*ks + a*cs], for the
 // upper tile pair (ti <= tj) of patch ip and contraction chunk kc.
 // AT is the product / accumulation type: float for the SVST path, double for
@@ -164,9 +176,12 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const c64 *__restrict__ P, in
                                                    int *__restrict__ err) {
     __shared__ double Ar[GK][GT + 2], Ai[GK][GT + 2];
     __shared__ double Br[GK][GT + 2], Bi[GK][GT + 2];
-    const int ip = blockIdx.z;
-    const int kc = blockIdx.y;
-    int rem = blockIdx.x, ti = 0;
+    // the tile pairs of one (patch, chunk) share column tiles: keep them on one XCD
+    const int64_t nwg = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t wl = xcd_order(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z), nwg);
+    const int ip = (int)(wl / ((int64_t)gridDim.x * gridDim.y));
+    const int kc = (int)((wl / gridDim.x) % gridDim.y);
+    int rem = (int)(wl % gridDim.x), ti = 0;
     while (rem >= ntile - ti) {
         rem -= ntile - ti;
         ++ti;
@@ -919,6 +934,7 @@ __global__ __launch_bounds__(256) void k_backtransform(const c128 *__restrict__ 
 // A(i,k) = A[z*bA + i*ai + k*ak] * fs[z*bfs + k] (fs optional),
 // B(k,j) = B[z*bB + k*bk + j*bj] (conjugated when conjB), C(i,j) = C[z*bC + i*ci + j*cj].
 // K = kvar[z] when kvar is given (kvar[z] < 0 writes the identity), else Kfix.
+// Grid (column tiles, row tiles, batch), renumbered onto XCDs (xcd_order).
 // 64x64 output tile per workgroup of four waves, 32x32 per wave as 2x2 blocks of
 // 16x16; each complex block is four v_mfma_f32_16x16x4f32 per k-step of 4 (real
 // and imaginary parts as separate real products).  The MFMA operands are fed
@@ -937,8 +953,11 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
                                                const int *__restrict__ kvar) {
     __shared__ float Ar[MK][MT + 4], Ai[MK][MT + 4];
     __shared__ float Br[MK][MT + 4], Bi[MK][MT + 4];
-    const int z = blockIdx.z;
-    const int i0 = blockIdx.x * MT, j0 = blockIdx.y * MT;
+    // the column tiles of one block of rows share A: keep them on one XCD
+    const int64_t nwg = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t wl = xcd_order(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z), nwg);
+    const int z = (int)(wl / ((int64_t)gridDim.x * gridDim.y));
+    const int j0 = (int)(wl % gridDim.x) * MT, i0 = (int)((wl / gridDim.x) % gridDim.y) * MT;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     c64 *Cz = C + (int64_t)z * bC;
@@ -1108,17 +1127,6 @@ __device__ __forceinline__ void stage8(const c64 (&r)[8], unsigned *sh, unsigned
     *(u4 *)(sh + 4) = u4{h[4], h[5], h[6], h[7]};
     *(u4 *)sl = u4{l[0], l[1], l[2], l[3]};
     *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
-}
-
-// Work-group index renumbered so that consecutive numbers run on one XCD: the
-// dispatcher hands consecutive work-groups to the 8 XCDs in turn, each with its
-// own L2, so tiles that share operands and are numbered consecutively would
-// otherwise fetch the shared data once per XCD.  The last n % 8 keep their order.
-__device__ __forceinline__ int64_t xcd_order(int64_t w, int64_t n) {
-    constexpr int NXCD = 8;
-    const int64_t per = n / NXCD;
-    if (w >= per * NXCD) return w;
-    return (w % NXCD) * per + w / NXCD;
 }
 
 // Ozaki slices of row-contiguous patch matrices (rows of each column contiguous,
@@ -1713,7 +1721,7 @@ static void cgemm(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, in
                   int N, int Kfix, const int *kvar, int64_t nbatch) {
     for (int64_t z0 = 0; z0 < nbatch; z0 += 65535) {
         const int64_t nz = std::min<int64_t>(65535, nbatch - z0);
-        dim3 g((unsigned)cdiv(M, MT), (unsigned)cdiv(N, MT), (unsigned)nz);
+        dim3 g((unsigned)cdiv(N, MT), (unsigned)cdiv(M, MT), (unsigned)nz);
         MSLR_PROF(ctx, tag);
         k_cgemm<<<g, 256, 0, ctx->stream>>>(A + z0 * bA, bA, ai, ak, B + z0 * bB, bB, bk, bj, conjB,
                                             fs ? fs + z0 * bfs : nullptr, bfs, C + z0 * bC, bC, ci,
