@@ -28,22 +28,25 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the kernel
-# families, from the committed rocprofv3 counter summary of this workload
+# families, from the committed rocprofv3 counter summary of this workload (mixed
+# form; the SENSE, update and patch kernels are the same in every form)
 TRAFFIC_BYTES = {
-    "fwd_yz": 42593873720,
-    "adj_yz": 24837016474,
-    "fwd_x": 8711234328,
-    "adj_x": 6046133958,
-    "sy2sb": 6484327317,
-    "sb2st": 224863632,
-    "gram_tiles": 5894903605,
-    "backtransform": 1138798587,
-    "pogm_update": 15163579349,
-    "sum_planes": 4082476160,
-    "planes_to_image": 2479730483,
-    "patches2img": 3000814720,
-    "img2patches": 3750132181,
-    "prox_voxel": 3499237291,
+    "fwd_yz": 42422000000,
+    "adj_yz": 24838500000,
+    "pogm_update": 15163600000,
+    "fwd_x": 8843400000,
+    "adj_x": 6046700000,
+    "sum_planes": 4199100000,
+    "sy2sb": 3757800000,
+    "img2patches": 3675500000,
+    "prox_voxel": 3499200000,
+    "patches2img": 3150000000,
+    "apply": 2522200000,
+    "planes_to_image": 2479700000,
+    "split": 2342200000,
+    "gram_tiles": 1865300000,
+    "backtransform": 946200000,
+    "sb2st": 224800000,
 }
 FP32_PEAK_TFLOPS = 157.3
 
