@@ -12,7 +12,8 @@ this file's package directory (../mslr_hip/libmslr_hip.so).
 module MSLRHip
 
 export HipContext, HipArray, Asense_hip, img2patches_hip, patches2img_hip, SVST_hip,
-       patchSVST_hip, patch_nucnorm_hip, pogm_hip, poweriter_hip, upload, download
+       patchSVST_hip, patch_nucnorm_hip, pogm_hip, poweriter_hip, upload, download,
+       set_products!
 
 const LIB = get(ENV, "MSLR_HIP_LIB", joinpath(@__DIR__, "..", "mslr_hip", "libmslr_hip.so"))
 
@@ -44,6 +45,20 @@ end
 
 const _CTX = Ref{Union{Nothing,HipContext}}(nothing)
 default_context() = (_CTX[] === nothing && (_CTX[] = HipContext(0)); _CTX[])
+
+"""
+    set_products!(ctx, products::Symbol)
+
+SVST Gram / reconstruction products of `SVST_hip` and `patchSVST_hip` on `ctx`:
+`:fp32` (FP64 Gram products of the FP32 data, the default) or `:split_bf16` (the
+mixed-precision form: hi + lo BF16 on the matrix cores).
+"""
+function set_products!(ctx::HipContext, products::Symbol)
+    code = products === :fp32 ? 0 : products === :split_bf16 ? 1 :
+           throw(ArgumentError("products must be :fp32 or :split_bf16"))
+    check(ccall((:mslr_ctx_set_products, LIB), Cint, (Ptr{Cvoid}, Cint), ctx.h, code))
+    return ctx
+end
 
 "Column-major device array owned by the library (ComplexF32 unless stated)."
 mutable struct HipArray{T,N}
@@ -238,7 +253,8 @@ end
 The device form of
     pogm_mod(X0, comp_cost, dc_cost_grad, L; mom=:pogm, niter, g_prox, fun)
 as called by run_recon (scripts/reconstruct.jl:196-226), X0[..,1] = A'ksp.
-`ksp` is (K,Nc,Nt) in the reference's sample order.  form: 0 plain, 1 hipGraph.
+`ksp` is (K,Nc,Nt) in the reference's sample order.  form: 0 plain, 1 hipGraph,
+2 hipGraph with split-BF16 SVST products (mixed precision).
 """
 function pogm_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES, STRIDES;
                   niter::Int, form::Integer = 0)
