@@ -255,7 +255,10 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             const int nrb = (L + 15) / 16;
             const int rl = lane & 15, kl = lane >> 4;
             for (int rb = wave; rb < nrb; rb += S1T / 64) {
+                // four independent accumulator chains (summed below) keep the
+                // matrix pipe busy between dependent MFMAs
                 md4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+                md4 yr2 = {0.0, 0.0, 0.0, 0.0}, yi2 = {0.0, 0.0, 0.0, 0.0};
                 const int i = rb * 16 + rl;
 #pragma unroll 4
                 for (int ks = 0; ks < L; ks += 4) {
@@ -268,10 +271,12 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
                     // operands swapped: the block computed is Y^T, so the output
                     // lanes run down the columns of W
                     yr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, a.re, yr, 0, 0, 0);
-                    yr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, -a.im, yr, 0, 0, 0);
                     yi = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, a.re, yi, 0, 0, 0);
-                    yi = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, a.im, yi, 0, 0, 0);
+                    yr2 = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, -a.im, yr2, 0, 0, 0);
+                    yi2 = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, a.im, yi2, 0, 0, 0);
                 }
+                yr += yr2;
+                yi += yi2;
                 if (i < L) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) W[(kl + 4 * q) * L + i] = c128{yr[q], yi[q]};
@@ -377,6 +382,10 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
                     di[q] = a.im;
                 }
                 const int ii = bi * 16 + rl, kk = bk * 16 + rl;
+                // the update as four independent accumulator chains (the product
+                // terms split two ways), summed into the block below
+                md4 er = {0.0, 0.0, 0.0, 0.0}, ei = {0.0, 0.0, 0.0, 0.0};
+                md4 fr = {0.0, 0.0, 0.0, 0.0}, fi = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int js = 0; js < B; js += 4) {
                     const int j = js + kl;
@@ -385,15 +394,17 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
                     // V_I W_K^H + W_I V_K^H: re = vr wr + vi wi, im = vi wr - vr wi.
                     // Operands swapped (K-side first): the block computed is the
                     // transpose, so the output lanes run down the columns of A.
-                    dr = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.re, -vI.re, dr, 0, 0, 0);
-                    dr = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.im, -vI.im, dr, 0, 0, 0);
-                    dr = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.re, -wI.re, dr, 0, 0, 0);
-                    dr = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.im, -wI.im, dr, 0, 0, 0);
-                    di = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.re, -vI.im, di, 0, 0, 0);
-                    di = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.im, vI.re, di, 0, 0, 0);
-                    di = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.re, -wI.im, di, 0, 0, 0);
-                    di = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.im, wI.re, di, 0, 0, 0);
+                    er = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.re, vI.re, er, 0, 0, 0);
+                    ei = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.re, vI.im, ei, 0, 0, 0);
+                    fr = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.re, wI.re, fr, 0, 0, 0);
+                    fi = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.re, wI.im, fi, 0, 0, 0);
+                    er = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.im, vI.im, er, 0, 0, 0);
+                    ei = __builtin_amdgcn_mfma_f64_16x16x4f64(wK.im, -vI.re, ei, 0, 0, 0);
+                    fr = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.im, wI.im, fr, 0, 0, 0);
+                    fi = __builtin_amdgcn_mfma_f64_16x16x4f64(vK.im, -wI.re, fi, 0, 0, 0);
                 }
+                dr -= er + fr;
+                di -= ei + fi;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int i = bi * 16 + rl, k = bk * 16 + kl + 4 * q;
