@@ -161,7 +161,7 @@ struct LineTabs {
     const int *lptr;     // (Nt+1) first global line of each frame
     const int *line_yz;  // (lines) ky + Ny*kz of each line, ascending within a frame
     const int *kbeg;     // (lines) first internal sample of the line within its frame
-    const int *lk;       // (K, Nt) l*Nx + kx of sample j, l = line within frame
+    const int *lk;       // (K, Nt) (l << 16) | kx of sample j, l = line within frame
 };
 
 // Forward pass 1: one workgroup per (x, coil group, frame).  The summed image
@@ -227,11 +227,19 @@ __global__ __launch_bounds__(NTX) void k_fwd_x(
         return;
     }
     const int nl = min(LB, Lf - l0);
+    C2<T> *phl = buf + LB * Nx;  // per line of the block: y and z post-phases times scale
+    const int *lines = tb.line_yz + g0;
+    for (int li = tid; li < nl; li += NTX) {
+        const int yz = lines[l0 + li];
+        phl[li] = scale * (phy[yz % Ny] * phz[yz / Ny]);
+    }
     const C2<T> *src = Tb + (int64_t)Nc * Nx * (g0 - tb.lptr[ta]) + (int64_t)c * Nx * Lf + l0;
-    for (int e = tid; e < nl * Nx; e += NTX) {
-        const int kx = e / nl;
-        const int l = e - kx * nl;
-        buf[l * Nx + kx] = src[(int64_t)kx * Lf + l];
+    {   // thread (line lq, x = x0, x0 + xs, ...): independent loads, one division
+        const int lq = tid % nl, x0 = tid / nl, xs = NTX / nl;
+        if (x0 < xs) {
+#pragma unroll 4
+            for (int x = x0; x < Nx; x += xs) buf[lq * Nx + x] = src[(int64_t)x * Lf + lq];
+        }
     }
     __syncthreads();
     fx(buf, nl, Nx, 1);
@@ -240,18 +248,14 @@ __global__ __launch_bounds__(NTX) void k_fwd_x(
     const int jb = kb[l0];
     const int je = l0 + nl < Lf ? kb[l0 + nl] : (int)K;
     const int64_t fa = (ta + f) * K;
-    const int *lines = tb.line_yz + g0;
     const int64_t obase = ((int64_t)f * Nc + c) * K;
     double acc = 0.0;
     for (int j = jb + tid; j < je; j += NTX) {
-        const int v = tb.lk[fa + j];
-        const int l = v / Nx;
-        const int kx = v - l * Nx;
-        const int yz = lines[l];
-        const int ky = yz % Ny;
-        const int kz = yz / Ny;
-        const C2<T> ph = phx[kx] * phy[ky] * phz[kz];
-        C2<T> val = scale * (buf[(l - l0) * Nx + kx] * ph);
+        const int v = tb.lk[fa + j];  // (line << 16) | kx
+        const int l = v >> 16;
+        const int kx = v & 0xFFFF;
+        const C2<T> ph = phx[kx] * phl[l - l0];
+        C2<T> val = buf[(l - l0) * Nx + kx] * ph;
         const int64_t oi = obase + j;
         if (ysub) val = val - ysub[oi];
         out[oi] = val;
@@ -280,32 +284,36 @@ __global__ __launch_bounds__(NTX) void k_adj_x(
     const int l0 = lb * LB;
     if (l0 >= Lf) return;  // uniform per workgroup
     const int nl = min(LB, Lf - l0);
+    C2<T> *phl = buf + LB * Nx;  // per line of the block: y and z post-phases
+    const int *lines = tb.line_yz + g0;
+    for (int li = tid; li < nl; li += NTX) {
+        const int yz = lines[l0 + li];
+        phl[li] = phy[yz % Ny] * phz[yz / Ny];
+    }
     for (int e = tid; e < nl * Nx; e += NTX) buf[e] = C2<T>{T(0), T(0)};
     __syncthreads();
     const int *kb = tb.kbeg + g0;
     const int jb = kb[l0];
     const int je = l0 + nl < Lf ? kb[l0 + nl] : (int)K;
     const int64_t fa = (ta + f) * K;
-    const int *lines = tb.line_yz + g0;
     const int64_t ibase = ((int64_t)f * Nc + c) * K;
     for (int j = jb + tid; j < je; j += NTX) {
-        const int v = tb.lk[fa + j];
-        const int l = v / Nx;
-        const int kx = v - l * Nx;
-        const int yz = lines[l];
-        const int ky = yz % Ny;
-        const int kz = yz / Ny;
-        const C2<T> ph = phx[kx] * phy[ky] * phz[kz];
+        const int v = tb.lk[fa + j];  // (line << 16) | kx
+        const int l = v >> 16;
+        const int kx = v & 0xFFFF;
+        const C2<T> ph = phx[kx] * phl[l - l0];
         buf[(l - l0) * Nx + kx] = cmulc(ph, yin[ibase + j]);
     }
     __syncthreads();
     fx(buf, nl, Nx, 1);
     __syncthreads();
     C2<T> *dst = Tb + (int64_t)Nc * Nx * (g0 - tb.lptr[ta]) + (int64_t)c * Nx * Lf + l0;
-    for (int e = tid; e < nl * Nx; e += NTX) {
-        const int x = e / nl;
-        const int l = e - x * nl;
-        dst[(int64_t)x * Lf + l] = buf[l * Nx + x];
+    {   // thread (line lq, x = x0, x0 + xs, ...), one division
+        const int lq = tid % nl, x0 = tid / nl, xs = NTX / nl;
+        if (x0 < xs) {
+#pragma unroll 4
+            for (int x = x0; x < Nx; x += xs) dst[(int64_t)x * Lf + lq] = buf[lq * Nx + x];
+        }
     }
 }
 
@@ -499,7 +507,7 @@ static void launch_fwd_x(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, c
     allow_lds((const void *)k_fwd_x<T, FX>);
     dim3 g((unsigned)P->nlb, (unsigned)P->Nc, (unsigned)nf);
     MSLR_PROF(ctx, "fwd_x");
-    k_fwd_x<T, FX><<<g, NTX, (size_t)P->LBx * Nx * sizeof(C2<T>), ctx->stream>>>(
+    k_fwd_x<T, FX><<<g, NTX, (size_t)P->LBx * (Nx + 1) * sizeof(C2<T>), ctx->stream>>>(
         Tb, P->Nc, Nx, P->N[1], P->LBx, P->nlb, fx, line_tabs(P), P->K, ta,
         Dev<T>::ph(P, 0), Dev<T>::ph(P, 1), Dev<T>::ph(P, 2), (T)P->scale, ysub, out, partial);
     HIP_LAUNCH_CHECK();
@@ -512,7 +520,7 @@ static void launch_adj_x(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, C2
     allow_lds((const void *)k_adj_x<T, FX>);
     dim3 g((unsigned)P->nlb, (unsigned)P->Nc, (unsigned)nf);
     MSLR_PROF(ctx, "adj_x");
-    k_adj_x<T, FX><<<g, NTX, (size_t)P->LBx * Nx * sizeof(C2<T>), ctx->stream>>>(
+    k_adj_x<T, FX><<<g, NTX, (size_t)P->LBx * (Nx + 1) * sizeof(C2<T>), ctx->stream>>>(
         y, P->Nc, Nx, P->N[1], P->LBx, fx, line_tabs(P), P->K, ta, Dev<T>::ph(P, 0),
         Dev<T>::ph(P, 1), Dev<T>::ph(P, 2), Tb);
     HIP_LAUNCH_CHECK();
@@ -800,6 +808,8 @@ mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, in
                              "Frame " + std::to_string(t + 1) + " has a different sample count");
             }
             MSLR_REQUIRE(K <= INT32_MAX, MSLR_ERR_UNSUPPORTED, "too many samples per frame");
+            MSLR_REQUIRE(Nx < 65536 && P->NYZ < 32768, MSLR_ERR_UNSUPPORTED,
+                         "sample table packs (line, kx) in 15 + 16 bits");
             MSLR_REQUIRE((int64_t)P->NYZ * Nt <= INT32_MAX, MSLR_ERR_UNSUPPORTED,
                          "too many k-space lines");
             P->K = K;
@@ -819,7 +829,7 @@ mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, in
                             kbeg.push_back(j);
                             any = true;
                         }
-                        lk[t * K + j] = l * Nx + x;
+                        lk[t * K + j] = (l << 16) | x;
                         ++j;
                     }
                     l += any;

@@ -32,7 +32,7 @@ struct mslr_sense_s {
     int *lptr = nullptr;     // (Nt+1): first global line of frame t
     int *line_yz = nullptr;  // (lines): ky + Ny*kz, ascending within a frame
     int *kbeg = nullptr;     // (lines): first internal sample of the line in its frame
-    int *lk = nullptr;       // (K, Nt): l*Nx + kx of internal sample j (l: line of the frame)
+    int *lk = nullptr;       // (K, Nt): (l << 16) | kx of internal sample j (l: line of the frame)
     std::vector<int> lptr_h; // host copy of lptr
     std::vector<void *> allocs;
 };
