@@ -229,6 +229,9 @@ def main() -> int:
     ap.add_argument("--no-prof", action="store_true", help="skip the pass with per-kernel event timing")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the batched FP32 and plain runs (rates and result differences)")
+    ap.add_argument("--logger", default="on", choices=["on", "off"],
+                    help="keep run_recon's logger (dc_cost, reg_cost per iterate) on the device "
+                         "inside the timed loop, as the reference's loop calls it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) five times, to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -273,7 +276,7 @@ def main() -> int:
 
     def make(form, niter):
         st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=L,
-                  niter=niter, form=form)
+                  niter=niter, form=form, log_costs=args.logger == "on")
         st.init(yd)
         ctx.sync()
         return st
@@ -303,6 +306,11 @@ def main() -> int:
     _, restarts = st.log()
     x_form = st.X.to_host().sum(axis=-1)
     log(f"{args.form}: timed {K} iterations in {elapsed:.3f}s -> {value:.3f} iter/s; restarts {restarts}")
+    costs = None
+    if args.logger == "on":
+        dc, reg = st.costs()
+        costs = {"dc_cost": [round(v, 6) for v in dc], "reg_cost": [round(v, 6) for v in reg]}
+        log(f"{args.form}: dc_cost {dc[0]:.6g} -> {dc[-1]:.6g}, reg_cost {reg[0]:.6g} -> {reg[-1]:.6g}")
     st.close()
 
     # the other forms on the same problem with the same iteration count (POGM's
@@ -397,7 +405,8 @@ def main() -> int:
         "dtype": "c64/f64" + ("/bf16x2" if args.form == "mixed" else ""),
         "data": "synthetic",
         "config": {"workload": f"{name}: {N[0]}x{N[1]}x{N[2]} Nt={Nt} {Nc} coils R={cfg['R']} scales={cfg['patch_sizes']}",
-                   "form": args.form, "parallelism": "single-gpu", "restarts_timed": sum(restarts[W:])},
+                   "form": args.form, "parallelism": "single-gpu", "restarts_timed": sum(restarts[W:]),
+                   "logger": args.logger},
         "nrmse_vs_truth": nr_form,
         "delta_nrmse": abs(nr_form - plain["nrmse_vs_truth"]) if plain else None,
         "plain": plain,
@@ -406,6 +415,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_per_iter": {k: round(v[1] / K, 3) for k, v in kern.items()},
+        "costs": costs,
     }
     print(json.dumps(line), flush=True)
     return 0

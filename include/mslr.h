@@ -224,6 +224,7 @@ typedef struct {
     double restart_cutoff;      /* must satisfy |restart_cutoff| < 1 */
     double bsig;                /* gamma decrease factor, >= 0 */
     int form;                   /* MSLR_FORM_* */
+    int log_costs;              /* 1: keep dc_cost and reg_cost of every iterate (the logger) */
 } mslr_pogm_desc;
 
 /*
@@ -245,6 +246,14 @@ void *mslr_pogm_x(mslr_pogm st);
 /* Iterations done so far and the restart flags (one per iteration, host array
  * of length >= iterations done). */
 mslr_status_t mslr_pogm_log(mslr_pogm st, int *iters_done, int *restarts);
+/* The logger's costs of x_0 .. x_k after k iterations (k + 1 values each, host
+ * arrays, either may be NULL), as run_recon collects them
+ * (scripts/reconstruct.jl:176-191,205,228-229 with src/mirt_mod.jl:106,181):
+ *   dc_costs[i]  = 0.5 ||A sum_s X_s - y||^2           (FP32 residual, FP64 sum)
+ *   reg_costs[i] = sum_s lambda_s patch_nucnorm(img2patches(X_s))   (FP64)
+ * The restart flags are mslr_pogm_log's (x_0's is false).  Needs log_costs = 1
+ * at create; reading the log after the last iteration costs one forward A. */
+mslr_status_t mslr_pogm_costs(mslr_pogm st, int *n, double *dc_costs, double *reg_costs);
 /* Per-scale SVST thresholds zeta*lambda_k used in the last iteration (host, nscales). */
 mslr_status_t mslr_pogm_thresholds(mslr_pogm st, double *beta);
 

@@ -39,6 +39,7 @@ void *mslr_ctx_s::scratch_get(size_t slot, size_t bytes) {
                                         " bytes failed: " + hipGetErrorString(e));
     }
     s.bytes = want;
+    ++scratch_gen;  // captured graphs holding the old address are stale now
     return s.ptr;
 }
 

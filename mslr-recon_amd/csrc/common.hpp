@@ -109,6 +109,7 @@ struct mslr_ctx_s {
     int num_cus = 0;
     std::vector<mslr::Scratch> scratch;  // indexed slots, see ScratchSlot
     size_t slot_base = 0;                // added to every slot (per-stream scratch sets)
+    uint64_t scratch_gen = 0;            // bumped whenever a slot is reallocated
     int products = 0;                    // MSLR_PRODUCTS_*: SVST Gram / apply products
 
     // Kernel timing with HIP events on `stream` (enabled by mslr_prof_enable).

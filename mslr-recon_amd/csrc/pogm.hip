@@ -11,6 +11,18 @@
 // The scalar recurrence (t, beta, gamma, zeta, sigma) lives in device memory
 // and is advanced by single-workgroup kernels, so an iteration has no host
 // synchronisation and can be captured in a hipGraph.
+//
+// Cost log (the run_recon logger, scripts/reconstruct.jl:176-191,205 with
+// src/mirt_mod.jl:106,181): for every iterate x_k, k = 0..niter,
+//   dc_cost(x_k)  = 0.5 ||A sum_s X_s - y||^2
+//   reg_cost(x_k) = sum_s lambda_s sum_patches ||patch of X_s||_*
+// dc_cost(x_{k-1}) is the squared norm of the residual iteration k forms for its
+// gradient anyway (partials from the x-pass of A); the last iterate's takes one
+// more A when the log is read.  reg_cost(x_k): with disjoint patches the prox
+// output's patches are the SVST outputs, whose singular values are
+// max(s - beta, 0) of the eigenvalues the thresholding already found; with
+// overlapping patches the new X_s is cut into patches again and their
+// singular values taken from Gram eigenvalues (no vectors).
 #include <algorithm>
 #include <cmath>
 
@@ -164,6 +176,25 @@ __global__ void k_pogm_post(PogmDev *s, const double *__restrict__ partials, int
     }
 }
 
+// costs[2 iter] = 0.5 sum(part): dc_cost of the iterate the gradient was taken at.
+__global__ __launch_bounds__(1024) void k_cost_dc(const double *__restrict__ part, int64_t n,
+                                                  const PogmDev *s, double *__restrict__ costs) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];  // fixed order
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) costs[2 * s->iter] = 0.5 * acc;
+}
+
+// costs[2 iter + 1] = sum_s lambda_s nuc_s: reg_cost of the newest iterate.
+__global__ void k_cost_reg(const double *__restrict__ nuc, const double *__restrict__ lambdas,
+                           int nscales, const PogmDev *s, double *__restrict__ costs) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double r = 0.0;
+    for (int k = 0; k < nscales; ++k) r += lambdas[k] * nuc[k];
+    costs[2 * s->iter + 1] = r;
+}
+
 __global__ void k_fill_zero(c64 *__restrict__ x, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -181,54 +212,105 @@ static void fill_zero(mslr_ctx_s *ctx, c64 *x, int64_t n) {
 // time x_v (1 x Nt) is one patch, its only singular value is ||x_v||, so
 // out_v = x_v max(||x_v|| - beta, 0) / ||x_v|| (beta rounded to Float32 as in
 // src/recon.jl:195).  Equals img2patches -> SVST -> patches2img without the two
-// transposing copies; in place allowed.
+// transposing copies; in place allowed.  nuc_part (nullable): per-workgroup sums
+// of the outputs' singular values max(||x_v|| - beta, 0).
 __global__ __launch_bounds__(256) void k_prox_voxel(const c64 *img, c64 *out, int64_t V, int Nt,
-                                                    const double *__restrict__ beta) {
+                                                    const double *__restrict__ beta,
+                                                    double *__restrict__ nuc_part) {
+    __shared__ double red[4];
     const double bd = (double)(float)beta[0];
+    double nacc = 0.0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
          v += (int64_t)gridDim.x * blockDim.x) {
         float fk = 1.f;
-        if (bd != 0.0) {
+        if (bd != 0.0 || nuc_part) {
             double s = 0.0;
             for (int t = 0; t < Nt; ++t) {
                 const c64 a = img[v + V * t];
                 s += (double)a.re * a.re + (double)a.im * a.im;
             }
             const double sg = sqrt(s);
-            fk = sg > bd ? (float)((sg - bd) / sg) : 0.f;
+            if (bd != 0.0) fk = sg > bd ? (float)((sg - bd) / sg) : 0.f;
+            nacc += fmax(sg - bd, 0.0);
         }
         for (int t = 0; t < Nt; ++t) {
             const c64 a = img[v + V * t];
             out[v + V * t] = c64{a.re * fk, a.im * fk};
         }
     }
+    if (nuc_part) {
+        nacc = block_sum(nacc, red);
+        if (threadIdx.x == 0) nuc_part[blockIdx.x] = nacc;
+    }
+}
+
+static unsigned prox_voxel_blocks(int64_t V) {
+    return (unsigned)std::min<int64_t>(cdiv(V, 256), 1 << 16);
+}
+
+// Per-unit partial count prox_scale writes for the nuclear norm of its output.
+static int64_t nuc_units(const PatchGeom &g) {
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    if (g.p == 1 && g.Np == V && g.exact_tiling()) return prox_voxel_blocks(V);
+    return std::max<int64_t>(1, g.Np);
+}
+
+// *nuc_tot = sum over patches of the nuclear norm of img's patches (eigenvalues
+// of the Gram matrices only).
+static void scale_nucnorm(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, double *nuc_tot) {
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    MSLR_PROF(ctx, "nucnorm");
+    if (g.Np == 1 && g.p == V) {
+        nucnorm_batched(ctx, img, g.p, g.Nt, 1, nuc_tot);
+        return;
+    }
+    const size_t bytes = (size_t)std::max<int64_t>(1, g.p * g.Nt * g.Np) * sizeof(c64);
+    c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, bytes);
+    launch_img2patches(ctx, g, img, P, 0, g.Np);
+    nucnorm_batched(ctx, P, g.p, g.Nt, g.Np, nuc_tot);
 }
 
 // patchSVST for one scale: out <- patches2img(SVST(img2patches(img))).
 // Two layouts need no patch copies: 1x1x1 patches with stride 1 (one voxel per
 // patch), and a single patch covering the image (the patch matrix is the image).
+// nuc_part/nuc_tot (device, nullable together): *nuc_tot <- sum over patches of
+// the nuclear norm of out's patches; nuc_part holds nuc_units(g) partials.
 static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *out,
-                       const double *beta_dev, int *err) {
+                       const double *beta_dev, int *err, double *nuc_part, double *nuc_tot) {
     const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
     if (g.p == 1 && g.Np == V && g.exact_tiling()) {
-        const unsigned nb = (unsigned)std::min<int64_t>(cdiv(V, 256), 1 << 16);
-        MSLR_PROF(ctx, "prox_voxel");
-        k_prox_voxel<<<nb, 256, 0, ctx->stream>>>(img, out, V, g.Nt, beta_dev);
-        HIP_LAUNCH_CHECK();
+        const unsigned nb = prox_voxel_blocks(V);
+        {
+            MSLR_PROF(ctx, "prox_voxel");
+            k_prox_voxel<<<nb, 256, 0, ctx->stream>>>(img, out, V, g.Nt, beta_dev, nuc_part);
+            HIP_LAUNCH_CHECK();
+        }
+        if (nuc_part) sum_partials(ctx, nuc_part, nb, nuc_tot);
         return;
     }
     const size_t bytes = (size_t)std::max<int64_t>(1, g.p * g.Nt * g.Np) * sizeof(c64);
     c64 *Y = (c64 *)ctx->scratch_get(SLOT_PATCH_OUT, bytes);
     if (g.Np == 1 && g.p == V) {
-        svst_batched(ctx, img, Y, g.p, g.Nt, 1, beta_dev, 0, err);
-        MSLR_PROF(ctx, "patch_copy");
-        HIP_CHECK(hipMemcpyAsync(out, Y, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        svst_batched(ctx, img, Y, g.p, g.Nt, 1, beta_dev, 0, err, nuc_part);
+        {
+            MSLR_PROF(ctx, "patch_copy");
+            HIP_CHECK(hipMemcpyAsync(out, Y, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        if (nuc_part) sum_partials(ctx, nuc_part, 1, nuc_tot);
         return;
     }
+    // disjoint patches come back unchanged from X_s: their singular values are
+    // the SVST's outputs
+    const bool disjoint = g.disjoint();
     c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, bytes);
     launch_img2patches(ctx, g, img, P, 0, g.Np);
-    svst_batched(ctx, P, Y, g.p, g.Nt, g.Np, beta_dev, 0, err);
+    svst_batched(ctx, P, Y, g.p, g.Nt, g.Np, beta_dev, 0, err, disjoint ? nuc_part : nullptr);
     launch_patches2img(ctx, g, Y, out);
+    if (!nuc_part) return;
+    if (disjoint)
+        sum_partials(ctx, nuc_part, g.Np, nuc_tot);
+    else
+        scale_nucnorm(ctx, g, out, nuc_tot);
 }
 
 }  // namespace mslr
@@ -250,8 +332,18 @@ struct mslr_pogm_s {
     int *log_dev = nullptr;
     int iters_done = 0;
     bool initialised = false;
+    // cost log: costs[2k], costs[2k+1] = dc_cost, reg_cost of x_k (k = 0..niter);
+    // dc_part: residual partials; nuc_tot[s] and nuc_part[s]: per-scale nuclear norms
+    bool log_costs = false;
+    double *costs = nullptr, *dc_part = nullptr, *nuc_tot = nullptr;
+    int64_t n_dc_part = 0;
+    std::vector<double *> nuc_part;
+    int dc_final_at = -1;  // iterate whose dc_cost was last completed by an extra A
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // scratch generation the graph was captured at / of the last eager iteration
+    uint64_t graph_gen = 0, warm_gen = 0;
+    bool warm = false;
     // batched forms: one stream per scale for the prox (fork / join with events)
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> ev_join;
@@ -266,10 +358,16 @@ struct mslr_pogm_s {
     }
 };
 
-static void pogm_free(mslr_pogm_s *st) {
-    if (!st) return;
+static void drop_graph(mslr_pogm_s *st) {
     if (st->exec) hipGraphExecDestroy(st->exec);
     if (st->graph) hipGraphDestroy(st->graph);
+    st->exec = nullptr;
+    st->graph = nullptr;
+}
+
+static void pogm_free(mslr_pogm_s *st) {
+    if (!st) return;
+    drop_graph(st);
     for (auto s2 : st->aux) hipStreamDestroy(s2);
     for (auto e : st->ev_join) hipEventDestroy(e);
     if (st->ev_fork) hipEventDestroy(st->ev_fork);
@@ -277,11 +375,27 @@ static void pogm_free(mslr_pogm_s *st) {
     delete st;
 }
 
+// reg_cost of the newest iterate from the per-scale nuclear norms.
+static void log_reg_cost(mslr_pogm_s *st) {
+    if (!st->log_costs) return;
+    mslr_ctx_s *ctx = st->ctx;
+    MSLR_PROF(ctx, "cost_reg");
+    k_cost_reg<<<1, 64, 0, ctx->stream>>>(st->nuc_tot, st->lam_dev, (int)st->S, st->dev,
+                                          st->costs);
+    HIP_LAUNCH_CHECK();
+}
+
 // Enqueue one iteration on the context stream.
 static void pogm_iteration(mslr_pogm_s *st) {
     mslr_ctx_s *ctx = st->ctx;
     const int64_t n = st->V * st->Nt;
-    sense_grad_internal(st->sense, st->X, (int)st->S, n, st->y, st->g, nullptr);
+    sense_grad_internal(st->sense, st->X, (int)st->S, n, st->y, st->g,
+                        st->log_costs ? st->dc_part : nullptr);
+    if (st->log_costs) {
+        MSLR_PROF(ctx, "cost_dc");
+        k_cost_dc<<<1, 1024, 0, ctx->stream>>>(st->dc_part, st->n_dc_part, st->dev, st->costs);
+        HIP_LAUNCH_CHECK();
+    }
     {
         MSLR_PROF(ctx, "pogm_pre");
         k_pogm_pre<<<1, 64, 0, ctx->stream>>>(st->dev, st->lam_dev, (int)st->S, st->thr_dev);
@@ -313,13 +427,16 @@ static void pogm_iteration(mslr_pogm_s *st) {
     } products_scope(ctx, st->form);
     // serial scales without aux streams, and whenever kernels are being timed (so
     // each kernel's events bracket it alone)
+    auto nuc_part = [&](int64_t k) { return st->log_costs ? st->nuc_part[k] : nullptr; };
+    auto nuc_tot = [&](int64_t k) { return st->log_costs ? st->nuc_tot + k : nullptr; };
     if (st->aux.empty() || ctx->prof) {
         for (int64_t k = 0; k < st->S; ++k) {
             c64 *Xk = st->X + k * n;
             ctx->prof_suffix = k < 8 ? sfx[k] : "@s9+";
-            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err);
+            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err, nuc_part(k), nuc_tot(k));
         }
         ctx->prof_suffix = "";
+        log_reg_cost(st);
         return;
     }
     // Scales are independent: each runs on its own stream with its own scratch.
@@ -332,7 +449,7 @@ static void pogm_iteration(mslr_pogm_s *st) {
             ctx->stream = st->aux[k];
             ctx->slot_base = (size_t)SLOT_COUNT * (size_t)(k + 1);
             ctx->prof_suffix = k < 8 ? sfx[k] : "@s9+";
-            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err);
+            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err, nuc_part(k), nuc_tot(k));
             HIP_CHECK(hipEventRecord(st->ev_join[k], st->aux[k]));
         }
     } catch (...) {
@@ -345,6 +462,7 @@ static void pogm_iteration(mslr_pogm_s *st) {
     ctx->slot_base = 0;
     ctx->prof_suffix = "";
     for (int64_t k = 0; k < st->S; ++k) HIP_CHECK(hipStreamWaitEvent(main, st->ev_join[k], 0));
+    log_reg_cost(st);
 }
 
 extern "C" {
@@ -415,6 +533,16 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
             h.restart = d.restart;
             HIP_CHECK(hipMemcpy(st->dev, &h, sizeof(h), hipMemcpyHostToDevice));
             HIP_CHECK(hipMemset(st->log_dev, 0, sizeof(int) * std::max(1, d.niter)));
+            MSLR_REQUIRE(d.log_costs == 0 || d.log_costs == 1, MSLR_ERR_ARG, "log_costs must be 0 or 1");
+            st->log_costs = d.log_costs == 1;
+            if (st->log_costs) {
+                st->costs = (double *)st->alloc(sizeof(double) * 2 * (size_t)(d.niter + 1));
+                st->n_dc_part = st->Nt * st->Nc * (int64_t)sense->nlb;
+                st->dc_part = (double *)st->alloc(sizeof(double) * (size_t)st->n_dc_part);
+                st->nuc_tot = (double *)st->alloc(sizeof(double) * (size_t)st->S);
+                for (const PatchGeom &g : st->geoms)
+                    st->nuc_part.push_back((double *)st->alloc(sizeof(double) * (size_t)nuc_units(g)));
+            }
             if (d.form != MSLR_FORM_PLAIN && d.nscales > 1) {
                 for (int64_t k = 0; k < d.nscales; ++k) {
                     hipStream_t s2 = nullptr;
@@ -469,8 +597,15 @@ mslr_status_t mslr_pogm_init(mslr_pogm st, const void *y_dev, const void *X0_dev
         cur.iter = 0;
         cur.err = 0;
         HIP_CHECK(hipMemcpyAsync(st->dev, &cur, sizeof(cur), hipMemcpyHostToDevice, ctx->stream));
+        if (st->log_costs) {
+            // reg_cost(x0) (the logger's call at iteration 0, src/mirt_mod.jl:106)
+            for (int64_t k = 0; k < st->S; ++k)
+                scale_nucnorm(ctx, st->geoms[k], st->X + k * n, st->nuc_tot + k);
+            log_reg_cost(st);
+        }
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         st->iters_done = 0;
+        st->dc_final_at = -1;
         st->initialised = true;
     });
 }
@@ -486,10 +621,13 @@ mslr_status_t mslr_pogm_run(mslr_pogm st, int n) {
         for (int i = 0; i < n; ++i) {
             // kernel timing needs eager launches
             const bool use_graph = st->form != MSLR_FORM_PLAIN && !ctx->prof;
+            // a scratch slot reallocated since the capture (by any call on the
+            // context) leaves the graph pointing at freed memory
+            if (st->exec && st->graph_gen != ctx->scratch_gen) drop_graph(st);
             if (use_graph && st->exec) {
                 HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
-            } else if (use_graph && st->iters_done >= 1) {
-                // Scratch is sized by the eager first iteration; capture the next.
+            } else if (use_graph && st->warm && st->warm_gen == ctx->scratch_gen) {
+                // Scratch is sized by an eager iteration; capture the next.
                 HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
                 try {
                     pogm_iteration(st);
@@ -501,9 +639,12 @@ mslr_status_t mslr_pogm_run(mslr_pogm st, int n) {
                 }
                 HIP_CHECK(hipStreamEndCapture(ctx->stream, &st->graph));
                 HIP_CHECK(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0));
+                st->graph_gen = ctx->scratch_gen;
                 HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
             } else {
                 pogm_iteration(st);
+                st->warm = true;
+                st->warm_gen = ctx->scratch_gen;
             }
             st->iters_done++;
         }
@@ -528,6 +669,35 @@ mslr_status_t mslr_pogm_log(mslr_pogm st, int *iters_done, int *restarts) {
     });
 }
 
+mslr_status_t mslr_pogm_costs(mslr_pogm st, int *n_out, double *dc_costs, double *reg_costs) {
+    return guarded([&] {
+        MSLR_REQUIRE(st && n_out, MSLR_ERR_ARG, "null argument");
+        MSLR_REQUIRE(st->log_costs, MSLR_ERR_ARG, "the state was created with log_costs = 0");
+        MSLR_REQUIRE(st->initialised, MSLR_ERR_ARG, "mslr_pogm_init has not been called");
+        mslr_ctx_s *ctx = st->ctx;
+        ctx->activate();
+        const int k = st->iters_done;
+        if (st->dc_final_at != k) {
+            // dc_cost of the newest iterate: its residual has not been formed yet
+            const int64_t n = st->V * st->Nt;
+            sense_resid_internal(st->sense, st->X, (int)st->S, n, st->y, st->dc_part);
+            MSLR_PROF(ctx, "cost_dc");
+            k_cost_dc<<<1, 1024, 0, ctx->stream>>>(st->dc_part, st->n_dc_part, st->dev, st->costs);
+            HIP_LAUNCH_CHECK();
+            st->dc_final_at = k;
+        }
+        std::vector<double> h(2 * (size_t)(k + 1));
+        HIP_CHECK(hipMemcpyAsync(h.data(), st->costs, sizeof(double) * h.size(),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        *n_out = k + 1;
+        for (int i = 0; i <= k; ++i) {
+            if (dc_costs) dc_costs[i] = h[2 * i];
+            if (reg_costs) reg_costs[i] = h[2 * i + 1];
+        }
+    });
+}
+
 mslr_status_t mslr_pogm_thresholds(mslr_pogm st, double *beta) {
     return guarded([&] {
         MSLR_REQUIRE(st && beta, MSLR_ERR_ARG, "null argument");
@@ -549,7 +719,7 @@ mslr_status_t mslr_patch_svst(mslr_ctx ctx, const void *img_dev, void *out_dev,
         int *err = (int *)(bd + 1);
         HIP_CHECK(hipMemcpyAsync(bd, &beta, sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
-        prox_scale(c, g, (const c64 *)img_dev, (c64 *)out_dev, bd, err);
+        prox_scale(c, g, (const c64 *)img_dev, (c64 *)out_dev, bd, err, nullptr, nullptr);
         int herr = 0;
         HIP_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));

@@ -711,6 +711,20 @@ void sense_grad_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sst
     }
 }
 
+void sense_resid_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
+                          const c64 *y_int, double *dc_partial) {
+    mslr_ctx_s *ctx = P->ctx;
+    check_lds<float>(P);
+    const auto chunks = frame_chunks<float>(P, 0, P->Nt);
+    int64_t fmax = 1;
+    for (auto [ta, nf] : chunks) fmax = std::max(fmax, nf);
+    c64 *r = (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, fmax * P->Nc * P->K) *
+                                                     sizeof(c64));
+    for (auto [ta, nf] : chunks)
+        forward_chunk<float>(P, X + ta * P->V, nscales, sstride, ta, (int)nf,
+                             y_int + ta * P->Nc * P->K, r, dc_partial + ta * P->Nc * P->nlb);
+}
+
 void sense_adjoint_internal(mslr_sense_s *P, const c64 *y_int, c64 *x) {
     run_adjoint<float>(P, y_int, 0, P->Nt, x);
 }

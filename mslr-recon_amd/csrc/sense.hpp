@@ -49,6 +49,11 @@ void sense_ref_to_internal(mslr_sense_s *P, const c64 *y_ref, c64 *y_int);
 void sense_grad_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
                          const c64 *y_int, c64 *g, double *dc_partial);
 
+// Partial ||A sum_k X_k - y_int||^2 (Nt*Nc*nlb doubles into dc_partial), the
+// forward half of sense_grad_internal.
+void sense_resid_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
+                          const c64 *y_int, double *dc_partial);
+
 // x = A' y for all frames, y in internal order.
 void sense_adjoint_internal(mslr_sense_s *P, const c64 *y_int, c64 *x);
 

@@ -1476,9 +1476,10 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
 }
 
 // m == 1: each matrix is a vector v (len = p*Nt); SVST = v * max(|v| - beta, 0)/|v|.
+// nuc (nullable): the output's singular value max(|v| - beta, 0) per matrix.
 __global__ __launch_bounds__(256) void k_svst_m1(c64 *__restrict__ P, int64_t len, int64_t Np,
                                                  const double *__restrict__ beta,
-                                                 int beta_per_patch) {
+                                                 int beta_per_patch, double *__restrict__ nuc) {
     const int lane = threadIdx.x & 63;
     const int64_t ip = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (ip >= Np) return;
@@ -1490,8 +1491,9 @@ __global__ __launch_bounds__(256) void k_svst_m1(c64 *__restrict__ P, int64_t le
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     const double bd = (double)(float)beta[beta_per_patch ? ip : 0];
-    if (bd == 0.0) return;
     const double sg = sqrt(s);
+    if (nuc && lane == 0) nuc[ip] = fmax(sg - bd, 0.0);
+    if (bd == 0.0) return;
     const float fk = sg > bd ? (float)((sg - bd) / sg) : 0.f;
     for (int64_t i = lane; i < len; i += 64) {
         const c64 a = v[i];
@@ -1499,15 +1501,14 @@ __global__ __launch_bounds__(256) void k_svst_m1(c64 *__restrict__ P, int64_t le
     }
 }
 
-// Sum of sqrt(max(lambda,0)) over all eigenvalues of T (nuclear norm).
-__global__ __launch_bounds__(256) void k_nucnorm(const double *__restrict__ dall,
-                                                 const double *__restrict__ eall, int m,
-                                                 double *__restrict__ out) {
+// Sum of sqrt(max(lambda,0)) over all eigenvalues of the symmetric tridiagonal
+// (d, e) of order m (the nuclear norm of a matrix whose Gram matrix it is), by
+// bisection on Sturm counts; every thread of the workgroup calls it, the sum is
+// valid in thread 0.
+__device__ double tri_sqrt_eig_sum(const double *__restrict__ d, const double *__restrict__ e,
+                                   int m) {
     __shared__ double red[16];
     __shared__ double s_gl, s_gu, s_pivmin;
-    const int ip = blockIdx.x;
-    const double *d = dall + (int64_t)ip * m;
-    const double *e = eall + (int64_t)ip * m;
     if (threadIdx.x == 0) {
         double emax = 0.0, gl = INFINITY, gu = -INFINITY;
         for (int i = 0; i < m; ++i) {
@@ -1537,6 +1538,40 @@ __global__ __launch_bounds__(256) void k_nucnorm(const double *__restrict__ dall
         }
         acc += sqrt(fmax(0.5 * (lo + hi), 0.0));
     }
+    return block_sum(acc, red);
+}
+
+__global__ __launch_bounds__(256) void k_nucnorm(const double *__restrict__ dall,
+                                                 const double *__restrict__ eall, int m,
+                                                 double *__restrict__ out) {
+    const int ip = blockIdx.x;
+    const double s = tri_sqrt_eig_sum(dall + (int64_t)ip * m, eall + (int64_t)ip * m, m);
+    if (threadIdx.x == 0) out[ip] = s;
+}
+
+// Nuclear norm of each SVST output, sum_k max(sigma_k - beta, 0), from the
+// eigenvalues the thresholding found: lam[k] (k < r) are the r Gram eigenvalues
+// above beta^2.  r < 0 (beta = 0: the output is the input) takes all
+// eigenvalues of the tridiagonal by bisection.  One workgroup per patch.
+__global__ __launch_bounds__(64) void k_out_nucnorm(const double *__restrict__ dall,
+                                                    const double *__restrict__ eall, int m,
+                                                    const double *__restrict__ beta,
+                                                    int beta_per_patch,
+                                                    const int *__restrict__ rall,
+                                                    const double *__restrict__ lamall,
+                                                    double *__restrict__ out) {
+    __shared__ double red[16];
+    const int ip = blockIdx.x;
+    const int r = rall[ip];
+    if (r < 0) {
+        const double s = tri_sqrt_eig_sum(dall + (int64_t)ip * m, eall + (int64_t)ip * m, m);
+        if (threadIdx.x == 0) out[ip] = s;
+        return;
+    }
+    const double bd = (double)(float)beta[beta_per_patch ? ip : 0];
+    const double *lam = lamall + (int64_t)ip * m;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < r; k += blockDim.x) acc += fmax(sqrt(fmax(lam[k], 0.0)) - bd, 0.0);
     const double s = block_sum(acc, red);
     if (threadIdx.x == 0) out[ip] = s;
 }
@@ -1757,7 +1792,7 @@ static void cgemm_bf16(mslr_ctx_s *ctx, const c64 *A, int64_t bA, int64_t ai, in
 }
 
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
-                  const double *beta_dev, int per_patch, int *err_dev) {
+                  const double *beta_dev, int per_patch, int *err_dev, double *nuc_out) {
     if (Np == 0 || p == 0 || Nt == 0) return;
     const int64_t m = std::min(p, Nt);
     MSLR_REQUIRE(m <= MAXM, MSLR_ERR_UNSUPPORTED,
@@ -1769,7 +1804,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
                                  hipMemcpyDeviceToDevice, ctx->stream));
         const unsigned nb = (unsigned)cdiv(Np, 4);
         MSLR_PROF(ctx, "svst_m1");
-        k_svst_m1<<<nb, 256, 0, ctx->stream>>>(Pout, len, Np, beta_dev, per_patch);
+        k_svst_m1<<<nb, 256, 0, ctx->stream>>>(Pout, len, Np, beta_dev, per_patch, nuc_out);
         HIP_LAUNCH_CHECK();
         return;
     }
@@ -1822,6 +1857,12 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
                                                                     w.info, w.r, w.lam, w.Z, w.ws,
                                                                     w.f);
             HIP_LAUNCH_CHECK();
+            if (nuc_out) {
+                MSLR_PROF(ctx, "out_nucnorm");
+                k_out_nucnorm<<<(unsigned)nb, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
+                                                                    w.r, w.lam, nuc_out + ip0);
+                HIP_LAUNCH_CHECK();
+            }
         }
         if (w.sbr)
             sbr_backtransform(ctx, w.G, (int)m, nb, w.sw, w.Z, w.r, w.U);

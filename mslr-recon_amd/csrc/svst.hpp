@@ -9,9 +9,10 @@ namespace mslr {
 //   X_i <- U diag(max(s - beta_i, 0)) V^H.
 // beta_dev: device doubles, beta_i = beta_dev[per_patch ? i : 0] (rounded to
 // Float32 inside, src/recon.jl:195).  err_dev (device int, nullable) is set to
-// 1 when a Gram matrix holds non-finite values.
+// 1 when a Gram matrix holds non-finite values.  nuc_out (device, Np doubles,
+// nullable) receives each output's nuclear norm sum_k max(s_k - beta_i, 0).
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
-                  const double *beta_dev, int per_patch, int *err_dev);
+                  const double *beta_dev, int per_patch, int *err_dev, double *nuc_out = nullptr);
 
 // Sum of singular values of each matrix, accumulated into *out_dev (device double).
 void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,

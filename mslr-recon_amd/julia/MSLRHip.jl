@@ -245,26 +245,31 @@ struct PogmDesc
     restart_cutoff::Cdouble
     bsig::Cdouble
     form::Cint
+    log_costs::Cint
 end
 
 """
-    pogm_hip(A, ksp, L, λs, PATCH_SIZES, STRIDES; niter, form=0) -> (X, restarts)
+    pogm_hip(A, ksp, L, λs, PATCH_SIZES, STRIDES; niter, form=0, log_costs=false)
+        -> (X, restarts)  or, with log_costs, (X, costs)
 
 The device form of
     pogm_mod(X0, comp_cost, dc_cost_grad, L; mom=:pogm, niter, g_prox, fun)
 as called by run_recon (scripts/reconstruct.jl:196-226), X0[..,1] = A'ksp.
 `ksp` is (K,Nc,Nt) in the reference's sample order.  form: 0 plain, 1 hipGraph,
-2 hipGraph with split-BF16 SVST products (mixed precision).
+2 hipGraph with split-BF16 SVST products (mixed precision).  With log_costs the
+second value is the logger's `costs` (niter + 1 tuples (dc_cost, reg_cost,
+is_restart), scripts/reconstruct.jl:205,228-230), kept on the device.
 """
 function pogm_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES, STRIDES;
-                  niter::Int, form::Integer = 0)
+                  niter::Int, form::Integer = 0, log_costs::Bool = false)
     S = length(PATCH_SIZES)
     ps = Int64[p[a] for p in PATCH_SIZES for a in 1:3]
     ss = Int64[s[a] for s in STRIDES for a in 1:3]
     yd = ksp isa HipArray ? ksp : upload(ComplexF32.(ksp); ctx = A.ctx)
     st = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve ps ss λs begin
-        d = PogmDesc(Int64.(A.N), A.Nt, S, pointer(ps), pointer(ss), pointer(λs), L, niter, 0, 0, 0.0, 1.0, form)
+        d = PogmDesc(Int64.(A.N), A.Nt, S, pointer(ps), pointer(ss), pointer(λs), L, niter, 0, 0, 0.0, 1.0, form,
+                     log_costs ? 1 : 0)
         check(ccall((:mslr_pogm_create, LIB), Cint, (Ptr{Cvoid}, Ref{PogmDesc}, Ref{Ptr{Cvoid}}), A.h, d, st))
     end
     try
@@ -276,7 +281,13 @@ function pogm_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES
         X = Array{ComplexF32}(undef, A.N..., A.Nt, S)
         check(ccall((:mslr_download, LIB), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{Cvoid}, Csize_t),
                     A.ctx.h, X, xp, sizeof(X)))
-        return X, Bool.(rs[1:it[]])
+        restarts = Bool.(rs[1:it[]])
+        log_costs || return X, restarts
+        n = Ref{Cint}(0); dc = zeros(Cdouble, niter + 1); reg = zeros(Cdouble, niter + 1)
+        check(ccall((:mslr_pogm_costs, LIB), Cint, (Ptr{Cvoid}, Ref{Cint}, Ptr{Cdouble}, Ptr{Cdouble}),
+                    st[], n, dc, reg))
+        flags = [false; restarts]
+        return X, [(dc[i], reg[i], flags[i]) for i in 1:n[]]
     finally
         ccall((:mslr_pogm_destroy, LIB), Cint, (Ptr{Cvoid},), st[])
     end

@@ -76,6 +76,7 @@ class mslr_pogm_desc(C.Structure):
         ("restart_cutoff", C.c_double),
         ("bsig", C.c_double),
         ("form", C.c_int),
+        ("log_costs", C.c_int),
     ]
 
 
@@ -124,6 +125,7 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_pogm_x": (_p, [_p]),
     "mslr_pogm_log": (C.c_int, [_p, _pi, _pi]),
     "mslr_pogm_thresholds": (C.c_int, [_p, _pd]),
+    "mslr_pogm_costs": (C.c_int, [_p, _pi, _pd, _pd]),
 }
 
 _lib: C.CDLL | None = None

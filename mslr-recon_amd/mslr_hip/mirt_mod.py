@@ -32,7 +32,7 @@ class POGM:
 
     def __init__(self, A: SenseOp, *, patch_sizes, strides, lambdas, L: float, niter: int,
                  mom="pogm", restart="gr", restart_cutoff: float = 0.0, bsig: float = 1.0,
-                 f_mu: float = 0.0, form="plain"):
+                 f_mu: float = 0.0, form="plain", log_costs: bool = False):
         mom, restart = _sym(mom), _sym(restart)
         if mom not in _MOM:
             raise ValueError(f"mom={mom}")
@@ -76,6 +76,7 @@ class POGM:
         d.restart_cutoff = float(restart_cutoff)
         d.bsig = float(bsig)
         d.form = _FORM[form] if isinstance(form, str) else int(form)
+        d.log_costs = 1 if log_costs else 0
         h = C.c_void_p()
         _lib.check(self.ctx._lib.mslr_pogm_create(A.handle, C.byref(d), C.byref(h)))
         self.handle = h
@@ -102,6 +103,23 @@ class POGM:
         buf = (C.c_int * max(1, self.niter))()
         _lib.check(self.ctx._lib.mslr_pogm_log(self.handle, C.byref(it), buf))
         return it.value, [bool(buf[i]) for i in range(it.value)]
+
+    def costs(self):
+        """The logger's (dc_costs, reg_costs) of x_0 .. x_k after k iterations
+        (scripts/reconstruct.jl:176-191,205,228-229); needs log_costs=True."""
+        n = C.c_int()
+        dc = (C.c_double * (self.niter + 1))()
+        reg = (C.c_double * (self.niter + 1))()
+        _lib.check(self.ctx._lib.mslr_pogm_costs(self.handle, C.byref(n), dc, reg))
+        return [dc[i] for i in range(n.value)], [reg[i] for i in range(n.value)]
+
+    def logger_out(self):
+        """pogm_mod's `out` for run_recon's logger: [(dc_cost, reg_cost, is_restart)]
+        for iterations 0..k (src/mirt_mod.jl:106,181)."""
+        dc, reg = self.costs()
+        _, rs = self.log()
+        flags = [False] + rs
+        return [(dc[i], reg[i], flags[i]) for i in range(len(dc))]
 
     def thresholds(self):
         b = (C.c_double * self.S)()

@@ -108,17 +108,20 @@ def run_recon_arrays(ksp0, smaps_raw, *, N, Nvc, Nt, PATCH_SIZES, STRIDES, NITER
         s1 = float(σ1A_PRECOMPUTED)
     L = S * s1 ** 2
     lams = lambdas_ong_lustig(N, Nt, PATCH_SIZES)
-    st = POGM(A, patch_sizes=PATCH_SIZES, strides=STRIDES, lambdas=lams, L=L, niter=NITERS, form=form)
+    st = POGM(A, patch_sizes=PATCH_SIZES, strides=STRIDES, lambdas=lams, L=L, niter=NITERS, form=form,
+              log_costs=True)
     st.init(yd)
     t0 = time.time()
     st.run(NITERS)
     ctx.sync()
     if verbose:
         print(f"POGM: {NITERS} iterations in {time.time() - t0:.2f} s")
-    _, restarts = st.log()
+    costs = st.logger_out()
     X = st.X.to_host()
     return {
-        "X": X, "X_recon": X.sum(axis=4), "omega": omega, "restarts": restarts, "R": R,
+        "X": X, "X_recon": X.sum(axis=4), "omega": omega,
+        "dc_costs": np.array([c[0] for c in costs]), "reg_costs": np.array([c[1] for c in costs]),
+        "restarts": np.array([c[2] for c in costs]), "R": R,
         "sigma1A": s1, "L": L, "Nscales": S, "patch_sizes": PATCH_SIZES, "strides": STRIDES,
         "lambdas": lams, "Niters": NITERS, "scale_factor": sf, "used_gpu": True,
     }

@@ -30,6 +30,8 @@ PINS = {
     "pogm_mod": "scalar-recurrence golden table; toy quadratic with identity prox vs hand trace",
     "poweriter_mod": "diagonal operator with known sigma1; full-mask SENSE -> 1",
     "lambdas": "closed-form values for the configs",
+    "recon_costs": "X = 0 -> (||y||^2/2, 0); noise-free data at the truth -> dc 0; rank-1 single-patch "
+                   "scale -> lambda |u||v|; dc quadratic in X; reg positively homogeneous",
 }
 
 
@@ -455,6 +457,24 @@ def run_recon_arrays(ksp, omega, smaps, *, N, Nt, PATCH_SIZES, STRIDES, NITERS, 
                       fun=fun or (lambda it, xk, yk, r: r), scalar=scalar)
     restarts = [bool(o) for o in out[1:]] if fun is None else out
     return X, restarts, {"L": L, "lambdas": lams}
+
+
+def recon_costs(X, y, omega, smaps, lambdas, PATCH_SIZES, STRIDES, dtype=np.complex128):
+    """run_recon's logger on one iterate X (Nx,Ny,Nz,Nt,S), scripts/reconstruct.jl:174-191,205:
+    dc_cost = 0.5 ||A image_sum(X) - y||^2,
+    reg_cost = sum_k lambda_k patch_nucnorm(img2patches(X[..., k], PATCH_SIZES[k], STRIDES[k])).
+    Evaluated in `dtype` throughout (the reference: FP32 arrays, Float64 sums)."""
+    S = X.shape[4]
+    img = X[..., 0].astype(dtype)
+    for k in range(1, S):
+        img = img + X[..., k].astype(dtype)
+    r = A_forward(img, omega, smaps.astype(dtype)) - y.astype(dtype)
+    dc = 0.5 * float(np.vdot(r.ravel(), r.ravel()).real)
+    reg = 0.0
+    for k in range(S):
+        P = img2patches(X[..., k].astype(dtype), PATCH_SIZES[k], STRIDES[k], dtype=dtype)
+        reg += float(lambdas[k]) * patch_nucnorm(P)
+    return dc, reg
 
 
 def nrmse(x, ref):

@@ -7,11 +7,19 @@ import oracle as O
 from mslr_hip import synth
 
 
+SMALL = {
+    "small": {},
+    # overlapping 4^3 patches (stride 2) and the 1x1x1 voxel scale
+    "small_overlap": dict(patch_sizes=([12, 12, 10], [4, 4, 4], [1, 1, 1]),
+                          strides=([12, 12, 10], [2, 2, 2], [1, 1, 1])),
+}
+
+
 @functools.lru_cache(maxsize=8)
-def prepared(name: str = "small", seed: int = 7):
-    """(cfg, smaps, ksp (K,Nc,Nt) normalised, omega, sf) following scripts/reconstruct.jl:86-147."""
-    cfg = synth.small_config() if name == "small" else synth.CONFIGS[name]
-    if name != "small":
+def raw(name: str = "small", seed: int = 7):
+    """(cfg, ksp0 (Nx,Ny,Nz,Nc,Nt) zero-filled, raw smaps): the run_recon inputs."""
+    cfg = synth.small_config(**SMALL[name]) if name in SMALL else synth.CONFIGS[name]
+    if name not in SMALL:
         seed = synth.SEEDS[name]
     pb = synth.generate(cfg, seed=seed)
     N, Nt, Nc = tuple(cfg["N"]), cfg["Nt"], cfg["Nc"]
@@ -21,5 +29,13 @@ def prepared(name: str = "small", seed: int = 7):
     flat = ksp0.reshape(-1, Nc, Nt, order="F")
     for t in range(Nt):
         flat[O.sample_index(pb.mask[..., t]), :, t] = y[:, :, t]
-    smaps, ksp, omega, R, sf = O.prepare(ksp0, pb.smaps, N, Nc, Nt)
+    return cfg, ksp0, pb.smaps
+
+
+@functools.lru_cache(maxsize=8)
+def prepared(name: str = "small", seed: int = 7):
+    """(cfg, smaps, ksp (K,Nc,Nt) normalised, omega, sf) following scripts/reconstruct.jl:86-147."""
+    cfg, ksp0, smaps_raw = raw(name, seed)
+    N, Nt, Nc = tuple(cfg["N"]), cfg["Nt"], cfg["Nc"]
+    smaps, ksp, omega, R, sf = O.prepare(ksp0, smaps_raw, N, Nc, Nt)
     return cfg, smaps, ksp, omega, sf

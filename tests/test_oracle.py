@@ -293,3 +293,65 @@ def test_small_end_to_end_modes_agree():
     Xb, rb, _ = O.run_recon_arrays(ksp, omega, smaps, mode="ref", **kw)
     assert ra == rb
     assert rel_err(Xb, Xa) < 1e-5
+
+
+# ----------------------------------------------------------------- logger costs
+def _cost_problem(rng, N=(4, 6, 2), Nt=3, Nc=2):
+    smaps = O.normalise_smaps(crandn(rng, (*N, Nc), np.complex128))
+    omega = rng.random((*N, Nt)) < 0.6
+    omega[..., :] = omega[..., :1]  # same sample count per frame
+    return smaps, omega
+
+
+def test_recon_costs_at_zero_and_at_truth(rng):
+    N, Nt = (4, 6, 2), 3
+    smaps, omega = _cost_problem(rng, N, Nt)
+    ps, ss = [[4, 6, 2], [2, 2, 2]], [[4, 6, 2], [2, 2, 2]]
+    lams = [1.5, 0.25]
+    x = crandn(rng, (*N, Nt), np.complex128)
+    y = O.A_forward(x, omega, smaps)
+    X = np.zeros((*N, Nt, 2), np.complex128)
+    dc, reg = O.recon_costs(X, y, omega, smaps, lams, ps, ss)
+    half_sq = 0.5 * sum(abs(v) ** 2 for v in y.ravel())
+    assert dc == pytest.approx(half_sq, rel=1e-13)
+    assert reg == 0.0
+    # the truth split over the two scales: the data term only sees the sum
+    X[..., 0] = 0.3 * x
+    X[..., 1] = 0.7 * x
+    dc, _ = O.recon_costs(X, y, omega, smaps, lams, ps, ss)
+    assert dc < 1e-20 * half_sq
+
+
+def test_recon_costs_rank1_global_patch(rng):
+    """One global patch holding u v^T: its nuclear norm is |u| |v|; the 2^3 scale holds
+    one voxel per patch non-zero, each patch a 1 x Nt row c_i v^T (norm |c_i| |v|)."""
+    N, Nt = (4, 6, 2), 3
+    smaps, omega = _cost_problem(rng, N, Nt)
+    ps, ss = [[4, 6, 2], [2, 2, 2]], [[4, 6, 2], [2, 2, 2]]
+    lams = [1.5, 0.25]
+    u = crandn(rng, N, np.complex128)
+    v = crandn(rng, (Nt,), np.complex128)
+    X = np.zeros((*N, Nt, 2), np.complex128)
+    X[..., 0] = u[..., None] * v
+    c = crandn(rng, (2, 3, 1), np.complex128)
+    X[0::2, 0::2, 0::2, :, 1] = c[..., None] * v
+    y = np.zeros((int(omega[..., 0].sum()), 2, Nt), np.complex128)
+    _, reg = O.recon_costs(X, y, omega, smaps, lams, ps, ss)
+    nv = math.sqrt(sum(abs(t) ** 2 for t in v))
+    nu = math.sqrt(sum(abs(t) ** 2 for t in u.ravel()))
+    want = 1.5 * nu * nv + 0.25 * nv * sum(abs(t) for t in c.ravel())
+    assert reg == pytest.approx(want, rel=1e-12)
+
+
+def test_recon_costs_scaling(rng):
+    """dc is quadratic in X at y = 0, reg positively homogeneous (overlapping patches)."""
+    N, Nt = (4, 6, 2), 3
+    smaps, omega = _cost_problem(rng, N, Nt)
+    ps, ss = [[4, 6, 2], [2, 3, 2]], [[4, 6, 2], [1, 2, 1]]
+    lams = [1.0, 2.0]
+    X = crandn(rng, (*N, Nt, 2), np.complex128)
+    y = np.zeros((int(omega[..., 0].sum()), 2, Nt), np.complex128)
+    d1, r1 = O.recon_costs(X, y, omega, smaps, lams, ps, ss)
+    d3, r3 = O.recon_costs(3.0 * X, y, omega, smaps, lams, ps, ss)
+    assert d3 == pytest.approx(9.0 * d1, rel=1e-12)
+    assert r3 == pytest.approx(3.0 * r1, rel=1e-12)
