@@ -155,6 +155,17 @@ namespace mslr {
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Work-group index renumbered so that consecutive numbers run on one XCD: the
+// dispatcher hands consecutive work-groups to the 8 XCDs in turn, each with its
+// own L2, so work-groups that share operands and are numbered consecutively would
+// otherwise fetch the shared data once per XCD.  The last n % 8 keep their order.
+__device__ __forceinline__ int64_t xcd_order(int64_t w, int64_t n) {
+    constexpr int NXCD = 8;
+    const int64_t per = n / NXCD;
+    if (w >= per * NXCD) return w;
+    return (w % NXCD) * per + w / NXCD;
+}
+
 // Lookup of the context behind a handle with validation.
 inline mslr_ctx_s *check_ctx(mslr_ctx c) {
     if (!c) throw Error(MSLR_ERR_ARG, "null context");

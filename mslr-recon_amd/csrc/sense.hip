@@ -175,10 +175,15 @@ template <class T, class FY, class FZ>
 __global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) MSLR_YZ_ATTR void k_fwd_yz(
     const C2<T> *__restrict__ Pm, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny,
     int Nz, int cpw, FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz,
-    int64_t ta, C2<T> *__restrict__ Tb) {
+    int64_t ta, int nf, C2<T> *__restrict__ Tb) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
-    const int x = blockIdx.x, f = blockIdx.z;
+    // (frame, x, coil group) with the frame fastest, numbered onto XCDs: the
+    // work-groups resident on one XCD share one or two x-planes of coil maps
+    const int64_t wid = xcd_order(blockIdx.x, gridDim.x);
+    const int f = (int)(wid % nf);
+    const int x = (int)((wid / nf) % Nx);
+    const int cg = (int)(wid / ((int64_t)nf * Nx));
     const int NYZ = Ny * Nz;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int g0 = lptr[ta + f];
@@ -187,7 +192,7 @@ __global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) MSLR_YZ_ATTR void
     C2<T> *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
     const C2<T> *pl = Pm + ((int64_t)f * Nx + x) * NYZ;
     // runtime lengths: one coil, no loop (a loop around the runtime FFT spills)
-    const int c0 = FY::kPipe ? (int)blockIdx.y * cpw : (int)blockIdx.y;
+    const int c0 = FY::kPipe ? cg * cpw : cg;
     const int c1 = FY::kPipe ? min(Nc, c0 + cpw) : c0 + 1;
 #pragma unroll 1
     for (int c = c0; c < c1; ++c) {
@@ -327,10 +332,13 @@ template <class T, class FY, class FZ>
 __global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) MSLR_YZ_ATTR void k_adj_yz(
     const C2<T> *__restrict__ Tb, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny, int Nz,
     FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz, int64_t ta,
-    T scale, C2<T> *__restrict__ Pm) {
+    int nf, T scale, C2<T> *__restrict__ Pm) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
-    const int x = blockIdx.x, f = blockIdx.y;
+    // (frame, x) with the frame fastest, numbered onto XCDs (coil-map reuse in L2)
+    const int64_t wid = xcd_order(blockIdx.x, gridDim.x);
+    const int f = (int)(wid % nf);
+    const int x = (int)(wid / nf);
     const int NYZ = Ny * Nz;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int g0 = lptr[ta + f];
@@ -491,11 +499,13 @@ static void launch_fwd_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Pm, 
     const int Nx = P->N[0];
     const int cpw = FY::kPipe ? P->Nc : 1;
     allow_lds((const void *)k_fwd_yz<T, FY, FZ>);
-    dim3 g((unsigned)Nx, (unsigned)cdiv(P->Nc, cpw), (unsigned)nf);
+    const int64_t nwg = (int64_t)Nx * cdiv(P->Nc, cpw) * nf;
+    MSLR_REQUIRE(nwg * nt < ((int64_t)1 << 32), MSLR_ERR_UNSUPPORTED, "too many frames per chunk");
+    const unsigned g = (unsigned)nwg;
     MSLR_PROF(ctx, "fwd_yz");
     k_fwd_yz<T, FY, FZ><<<g, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
         Pm, Dev<T>::smaps(P), P->Nc, Nx, P->N[1], P->N[2], cpw, fy, fz, P->lptr, P->line_yz, ta,
-        Tb);
+        nf, Tb);
     HIP_LAUNCH_CHECK();
 }
 
@@ -532,10 +542,12 @@ static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, 
     mslr_ctx_s *ctx = P->ctx;
     const int Nx = P->N[0];
     allow_lds((const void *)k_adj_yz<T, FY, FZ>);
-    dim3 g((unsigned)Nx, (unsigned)nf);
+    const int64_t nwg = (int64_t)Nx * nf;
+    MSLR_REQUIRE(nwg * nt < ((int64_t)1 << 32), MSLR_ERR_UNSUPPORTED, "too many frames per chunk");
+    const unsigned g = (unsigned)nwg;
     MSLR_PROF(ctx, "adj_yz");
     k_adj_yz<T, FY, FZ><<<g, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
-        Tb, Dev<T>::smaps(P), P->Nc, Nx, P->N[1], P->N[2], fy, fz, P->lptr, P->line_yz, ta,
+        Tb, Dev<T>::smaps(P), P->Nc, Nx, P->N[1], P->N[2], fy, fz, P->lptr, P->line_yz, ta, nf,
         (T)P->scale, Pm);
     HIP_LAUNCH_CHECK();
 }

@@ -39,17 +39,6 @@ constexpr int WS_PER = 256;  // doubles per Gram row of eigen workspace per patc
 
 // ------------------------------------------------------------------ Gram
 
-// Work-group index renumbered so that consecutive numbers run on one XCD: the
-// dispatcher hands consecutive work-groups to the 8 XCDs in turn, each with its
-// own L2, so tiles that share operands and are numbered consecutively would
-// otherwise fetch the shared data once per XCD.  The last n % 8 keep their order.
-__device__ __forceinline__ int64_t xcd_order(int64_t w, int64_t n) {
-    constexpr int NXCD = 8;
-    const int64_t per = n / NXCD;
-    if (w >= per * NXCD) return w;
-    return (w % NXCD) * per + w / NXCD;
-}
-
 
 // C[a,b] += sum_k conj(A(k,a)) A(k,b), A(k,a) = base[k*ks + a*cs], for the
 // upper tile pair (ti <= tj) of patch ip and contraction chunk kc.
