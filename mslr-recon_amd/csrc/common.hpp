@@ -187,7 +187,8 @@ enum ScratchSlot : size_t {
     SLOT_PWR = 9,       // power-iteration vectors
     SLOT_PLANES = 10,   // SENSE plane-major image frames
     SLOT_SPLIT = 11,    // patch data as split-BF16 words (mixed-precision products)
-    SLOT_COUNT = 12
+    SLOT_LOWRANK = 12,  // X U_r of the low-rank SVST apply
+    SLOT_COUNT = 13
 };
 
 }  // namespace mslr

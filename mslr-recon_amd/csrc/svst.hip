@@ -933,13 +933,38 @@ __global__ __launch_bounds__(256) void k_backtransform(const c128 *__restrict__ 
 constexpr int MT = 64, MK = 16;
 typedef float mf4 __attribute__((ext_vector_type(4)));
 
+// Per-patch choice between the dense apply (Y = X W, W = U_r diag(f) U_r^H formed
+// first) and the low-rank one (Y = (X U_r diag(f)) U_r^H, two skinny products):
+// a patch with 0 <= r and 2r <= m takes the low-rank form.  mode 0: every patch
+// (no selection); 1: dense patches only; 2: low-rank patches, N = r; 3: low-rank
+// patches, K = r.  Decided per workgroup from r (uniform), so one launch of each
+// form covers a batch with both kinds.
+struct RankSel {
+    const int *r = nullptr;
+    int m = 0;
+    int mode = 0;
+};
+
+// Applies the selection for batch entry z: false when the workgroup has nothing
+// to do; N or K narrowed to r for the low-rank products.
+__device__ __forceinline__ bool rank_select(const RankSel &rs, int z, int &N, int &K) {
+    if (rs.mode == 0) return true;
+    const int r = rs.r[z];
+    const bool low = r >= 0 && 2 * r <= rs.m;
+    if (rs.mode == 1) return !low;
+    if (!low) return false;
+    if (rs.mode == 2) N = min(N, r);
+    else K = r;
+    return true;
+}
+
 __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_t bA, int64_t ai,
                                                int64_t ak, const c64 *__restrict__ B, int64_t bB,
                                                int64_t bk, int64_t bj, int conjB,
                                                const float *__restrict__ fs, int64_t bfs,
                                                c64 *__restrict__ C, int64_t bC, int64_t ci,
                                                int64_t cj, int M, int N, int Kfix,
-                                               const int *__restrict__ kvar) {
+                                               const int *__restrict__ kvar, RankSel rs) {
     __shared__ float Ar[MK][MT + 4], Ai[MK][MT + 4];
     __shared__ float Br[MK][MT + 4], Bi[MK][MT + 4];
     // the column tiles of one block of rows share A: keep them on one XCD
@@ -951,6 +976,7 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
     const int lane = tid & 63, wave = tid >> 6;
     c64 *Cz = C + (int64_t)z * bC;
     int K = Kfix;
+    if (!rank_select(rs, z, N, K) || j0 >= N) return;  // uniform per workgroup
     if (kvar) {
         K = kvar[z];
         if (K < 0) {  // identity
@@ -979,7 +1005,11 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
     constexpr int NR = (MK * MT) / 256;
     // global -> registers for one k-tile (the next tile is fetched while the
     // current one is multiplied)
+    // (the loads of a tile are issued together; the scaling by fs and the
+    // conjugation wait until the tile is staged)
     c64 ra[NR], rb[NR];
+    float fa[NR];
+    const float bsgn = conjB ? -1.f : 1.f;
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
@@ -987,24 +1017,13 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
             const int kk = afast_i ? idx / MT : idx % MK;
             const int ii = afast_i ? idx % MT : idx / MK;
             const int k = k0 + kk, i = i0 + ii;
-            c64 va = {0.f, 0.f};
-            if (k < K && i < M) {
-                va = Az[i * ai + k * ak];
-                if (fz) {
-                    const float f = fz[k];
-                    va = c64{va.re * f, va.im * f};
-                }
-            }
-            ra[r] = va;
+            const bool ina = k < K && i < M;
+            ra[r] = ina ? Az[i * ai + k * ak] : c64{0.f, 0.f};
+            fa[r] = (fz && ina) ? fz[k] : 1.f;
             const int kb = bfast_k ? idx % MK : idx / MT;
             const int jb = bfast_k ? idx / MK : idx % MT;
             const int kq = k0 + kb, j = j0 + jb;
-            c64 vb = {0.f, 0.f};
-            if (kq < K && j < N) {
-                vb = Bz[kq * bk + j * bj];
-                if (conjB) vb.im = -vb.im;
-            }
-            rb[r] = vb;
+            rb[r] = (kq < K && j < N) ? Bz[kq * bk + j * bj] : c64{0.f, 0.f};
         }
     };
     if (K > 0) fetch(0);
@@ -1014,12 +1033,12 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
             const int idx = tid + 256 * r;
             const int kk = afast_i ? idx / MT : idx % MK;
             const int ii = afast_i ? idx % MT : idx / MK;
-            Ar[kk][ii] = ra[r].re;
-            Ai[kk][ii] = ra[r].im;
+            Ar[kk][ii] = ra[r].re * fa[r];
+            Ai[kk][ii] = ra[r].im * fa[r];
             const int kb = bfast_k ? idx % MK : idx / MT;
             const int jb = bfast_k ? idx / MK : idx % MT;
             Br[kb][jb] = rb[r].re;
-            Bi[kb][jb] = rb[r].im;
+            Bi[kb][jb] = rb[r].im * bsgn;
         }
         __syncthreads();
         if (k0 + MK < K) fetch(k0 + MK);
@@ -1345,23 +1364,20 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ 
 }
 
 // Split-BF16 complex GEMM C = A B (the SVST apply), FP32 accumulation.  Same
-// tiling and output layout as k_cgemm (64x64 per workgroup, 32x32 per wave),
-// operands (B^T, A^T): C_re = [br, bi].[ar, -ai], C_im = [br, bi].[ai, ar].
-// An operand is either FP32 (split here into hi + lo) or given as the words of
-// k_split_rows (Awh/Awl or Bwh/Bwl non-null, same element strides).
-// blockIdx.x runs over the column tiles of C so the tiles sharing a block of
-// rows of A are dispatched together.
-template <bool AW, bool BW>
+// tiling, operand conventions and output layout as k_cgemm (64x64 per
+// workgroup, 32x32 per wave; A(i,k) scaled by fs[k] when given, B conjugated when
+// conjB), operands (B^T, A^T): C_re = [br, bi].[ar, -ai], C_im = [br, bi].[ai, ar].
+// Each FP32 operand is split into hi + lo BF16 words when staged.  blockIdx.x runs
+// over the column tiles of C so the tiles sharing a block of rows of A are
+// dispatched together.
+template <bool SCALED>
 __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, int64_t bA,
                                                     int64_t ai, int64_t ak,
                                                     const c64 *__restrict__ B, int64_t bB,
-                                                    int64_t bk, int64_t bj,
-                                                    const unsigned *__restrict__ Awh,
-                                                    const unsigned *__restrict__ Awl,
-                                                    const unsigned *__restrict__ Bwh,
-                                                    const unsigned *__restrict__ Bwl,
+                                                    int64_t bk, int64_t bj, int conjB,
+                                                    const float *__restrict__ fs, int64_t bfs,
                                                     c64 *__restrict__ C, int64_t bC, int64_t ci,
-                                                    int64_t cj, int M, int N, int K) {
+                                                    int64_t cj, int M, int N, int K, RankSel rs) {
     __shared__ __align__(16) unsigned Sh[2][MT][BLD];  // [A rows i, B cols j][row][k]
     __shared__ __align__(16) unsigned Sl[2][MT][BLD];
     // the column tiles of one block of rows share A: keep them on one XCD
@@ -1369,6 +1385,7 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
     const int64_t wl = xcd_order(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z), nwg);
     const int z = (int)(wl / ((int64_t)gridDim.x * gridDim.y));
     const int j0 = (int)(wl % gridDim.x) * MT, i0 = (int)((wl / gridDim.x) % gridDim.y) * MT;
+    if (!rank_select(rs, z, N, K) || j0 >= N) return;  // uniform per workgroup
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
@@ -1380,11 +1397,9 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
     const int bc_ = bfast ? tid >> 2 : tid & 63;
     const int bk_ = bfast ? (tid & 3) * 8 : (tid >> 6) * 8;
     const bool av = i0 + ar_ < M, bv = j0 + bc_ < N;
-    const int64_t oa = (int64_t)z * bA + (int64_t)(i0 + ar_) * ai;
-    const int64_t ob = (int64_t)z * bB + (int64_t)(j0 + bc_) * bj;
-    const c64 *pa = A + oa;
-    const c64 *pb = B + ob;
-    constexpr bool aw = AW, bw = BW;
+    const c64 *pa = A + (int64_t)z * bA + (int64_t)(i0 + ar_) * ai;
+    const c64 *pb = B + (int64_t)z * bB + (int64_t)(j0 + bc_) * bj;
+    const float *fz = fs ? fs + (int64_t)z * bfs : nullptr;
     mf4 accr[2][2], acci[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -1393,40 +1408,32 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
             accr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
             acci[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
         }
-    // per operand: FP32 values (split when staged) or the hi / lo words
-    c64 ra[AW ? 1 : 8], rb[BW ? 1 : 8];
-    unsigned wah[AW ? 8 : 1], wal[AW ? 8 : 1], wbh[BW ? 8 : 1], wbl[BW ? 8 : 1];
+    // (the loads of a tile are issued together; with SCALED the scaling by fs and
+    // the conjugation of B wait until the tile is staged)
+    c64 ra[8], rb[8];
+    float fa[SCALED ? 8 : 1];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int ka = k0 + ak_ + e, kb = k0 + bk_ + e;
-            const bool ina = av && ka < K, inb = bv && kb < K;
-            if constexpr (AW) {
-                wah[e] = ina ? Awh[oa + (int64_t)ka * ak] : 0u;
-                wal[e] = ina ? Awl[oa + (int64_t)ka * ak] : 0u;
-            } else {
-                ra[e] = ina ? pa[(int64_t)ka * ak] : c64{0.f, 0.f};
-            }
-            if constexpr (BW) {
-                wbh[e] = inb ? Bwh[ob + (int64_t)kb * bk] : 0u;
-                wbl[e] = inb ? Bwl[ob + (int64_t)kb * bk] : 0u;
-            } else {
-                rb[e] = inb ? pb[(int64_t)kb * bk] : c64{0.f, 0.f};
-            }
+            const bool ina = av && ka < K;
+            ra[e] = ina ? pa[(int64_t)ka * ak] : c64{0.f, 0.f};
+            rb[e] = (bv && kb < K) ? pb[(int64_t)kb * bk] : c64{0.f, 0.f};
+            if constexpr (SCALED) fa[e] = (fz && ina) ? fz[ka] : 1.f;
         }
-    };
-    auto putw = [&](const unsigned (&h)[8], const unsigned (&l)[8], unsigned *sh, unsigned *sl) {
-        *(u4 *)sh = u4{h[0], h[1], h[2], h[3]};
-        *(u4 *)(sh + 4) = u4{h[4], h[5], h[6], h[7]};
-        *(u4 *)sl = u4{l[0], l[1], l[2], l[3]};
-        *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
     };
     if (K > 0) fetch(0);
     for (int k0 = 0; k0 < K; k0 += BKS) {
-        if constexpr (AW) putw(wah, wal, &Sh[0][ar_][ak_], &Sl[0][ar_][ak_]);
-        else stage8(ra, &Sh[0][ar_][ak_], &Sl[0][ar_][ak_]);
-        if constexpr (BW) putw(wbh, wbl, &Sh[1][bc_][bk_], &Sl[1][bc_][bk_]);
-        else stage8(rb, &Sh[1][bc_][bk_], &Sl[1][bc_][bk_]);
+        if constexpr (SCALED) {
+            const float bsgn = conjB ? -1.f : 1.f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                ra[e] = c64{ra[e].re * fa[e], ra[e].im * fa[e]};
+                rb[e].im *= bsgn;
+            }
+        }
+        stage8(ra, &Sh[0][ar_][ak_], &Sl[0][ar_][ak_]);
+        stage8(rb, &Sh[1][bc_][bk_], &Sl[1][bc_][bk_]);
         __syncthreads();
         if (k0 + BKS < K) fetch(k0 + BKS);
 #pragma unroll
@@ -1739,45 +1746,56 @@ static void launch_backtransform(mslr_ctx_s *ctx, const SvstWork &w, int m, int6
     HIP_LAUNCH_CHECK();
 }
 
+static RankSel rank_sel_at(RankSel rs, int64_t z0) {
+    if (rs.r) rs.r += z0;
+    return rs;
+}
+
 static void cgemm(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, int64_t ai,
                   int64_t ak, const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB,
                   const float *fs, int64_t bfs, c64 *C, int64_t bC, int64_t ci, int64_t cj, int M,
-                  int N, int Kfix, const int *kvar, int64_t nbatch) {
+                  int N, int Kfix, const int *kvar, int64_t nbatch, RankSel rs = RankSel{}) {
     for (int64_t z0 = 0; z0 < nbatch; z0 += 65535) {
         const int64_t nz = std::min<int64_t>(65535, nbatch - z0);
         dim3 g((unsigned)cdiv(N, MT), (unsigned)cdiv(M, MT), (unsigned)nz);
         MSLR_PROF(ctx, tag);
         k_cgemm<<<g, 256, 0, ctx->stream>>>(A + z0 * bA, bA, ai, ak, B + z0 * bB, bB, bk, bj, conjB,
                                             fs ? fs + z0 * bfs : nullptr, bfs, C + z0 * bC, bC, ci,
-                                            cj, M, N, Kfix, kvar ? kvar + z0 : nullptr);
+                                            cj, M, N, Kfix, kvar ? kvar + z0 : nullptr,
+                                            rank_sel_at(rs, z0));
         HIP_LAUNCH_CHECK();
     }
 }
 
-static void cgemm_bf16(mslr_ctx_s *ctx, const c64 *A, int64_t bA, int64_t ai, int64_t ak,
-                       const c64 *B, int64_t bB, int64_t bk, int64_t bj, const unsigned *Awh,
-                       const unsigned *Awl, const unsigned *Bwh, const unsigned *Bwl, c64 *C,
-                       int64_t bC, int64_t ci, int64_t cj, int M, int N, int K, int64_t nbatch) {
+static void cgemm_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, int64_t ai,
+                       int64_t ak, const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB,
+                       const float *fs, int64_t bfs, c64 *C, int64_t bC, int64_t ci, int64_t cj,
+                       int M, int N, int K, int64_t nbatch, RankSel rs = RankSel{}) {
     for (int64_t z0 = 0; z0 < nbatch; z0 += 65535) {
         const int64_t nz = std::min<int64_t>(65535, nbatch - z0);
         dim3 g((unsigned)cdiv(N, MT), (unsigned)cdiv(M, MT), (unsigned)nz);
-        MSLR_PROF(ctx, "apply");
-        const unsigned *ah = Awh ? Awh + z0 * bA : nullptr, *al = Awl ? Awl + z0 * bA : nullptr;
-        const unsigned *bh = Bwh ? Bwh + z0 * bB : nullptr, *bl = Bwl ? Bwl + z0 * bB : nullptr;
-        const c64 *Az = A + z0 * bA, *Bz = B + z0 * bB;
-        c64 *Cz = C + z0 * bC;
-        if (Awh && !Bwh)
-            k_cgemm_bf16<true, false><<<g, 256, 0, ctx->stream>>>(Az, bA, ai, ak, Bz, bB, bk, bj, ah,
-                                                                  al, bh, bl, Cz, bC, ci, cj, M, N, K);
-        else if (!Awh && Bwh)
-            k_cgemm_bf16<false, true><<<g, 256, 0, ctx->stream>>>(Az, bA, ai, ak, Bz, bB, bk, bj, ah,
-                                                                  al, bh, bl, Cz, bC, ci, cj, M, N, K);
+        MSLR_PROF(ctx, tag);
+        if (fs || conjB)
+            k_cgemm_bf16<true><<<g, 256, 0, ctx->stream>>>(
+                A + z0 * bA, bA, ai, ak, B + z0 * bB, bB, bk, bj, conjB,
+                fs ? fs + z0 * bfs : nullptr, bfs, C + z0 * bC, bC, ci, cj, M, N, K,
+                rank_sel_at(rs, z0));
         else
-            k_cgemm_bf16<false, false><<<g, 256, 0, ctx->stream>>>(Az, bA, ai, ak, Bz, bB, bk, bj, ah,
-                                                                   al, bh, bl, Cz, bC, ci, cj, M, N,
-                                                                   K);
+            k_cgemm_bf16<false><<<g, 256, 0, ctx->stream>>>(A + z0 * bA, bA, ai, ak, B + z0 * bB,
+                                                            bB, bk, bj, 0, nullptr, 0, C + z0 * bC,
+                                                            bC, ci, cj, M, N, K,
+                                                            rank_sel_at(rs, z0));
         HIP_LAUNCH_CHECK();
     }
+}
+
+// MSLR_SVST_APPLY=dense turns the low-rank apply off (every patch forms W).
+static bool svst_lowrank_apply() {
+    static const bool on = [] {
+        const char *e = std::getenv("MSLR_SVST_APPLY");
+        return !(e && std::string(e) == "dense");
+    }();
+    return on;
 }
 
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
@@ -1858,25 +1876,49 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         else
             launch_backtransform(ctx, w, (int)m, nb);
         const int mi = (int)m;
+        // p >= Nt: patches keeping at most half their singular values take the
+        // low-rank form Y = (X U_r) diag(f) U_r^H (4 p m r flops instead of
+        // 2 p m^2 + m^2 r); the others W = U_r diag(f) U_r^H, then Y = X W.  The
+        // choice is per patch, on the device, from r.
+        const int half = mi / 2;
+        const bool lowrank = !left && half >= 1 && svst_lowrank_apply();
+        const RankSel dense = lowrank ? RankSel{w.r, mi, 1} : RankSel{};
         // W = (U diag f) U^H ; r < 0 (beta = 0) gives the identity, r = 0 gives 0
         cgemm(ctx, "form_w", w.U, m * m, 1, m, w.U, m * m, m, 1, 1, w.f, m, Wm, m * m, 1, m, mi, mi,
-              0, w.r, nb);
+              0, w.r, nb, dense);
         if (!left) {
             // Y (p x m) = X (p x m) W
             if (split)
-                cgemm_bf16(ctx, Pc, len, 1, p, Wm, m * m, 1, m, nullptr, nullptr, nullptr, nullptr,
-                           Yc, len, 1, p, (int)p, mi, mi, nb);
+                cgemm_bf16(ctx, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1,
+                           p, (int)p, mi, mi, nb, dense);
             else
                 cgemm(ctx, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p,
-                      (int)p, mi, mi, nullptr, nb);
+                      (int)p, mi, mi, nullptr, nb, dense);
         } else {
             // Y (p x Nt) = W (p x p) X
             if (split)
-                cgemm_bf16(ctx, Wm, m * m, 1, m, Pc, len, 1, p, nullptr, nullptr, nullptr, nullptr,
-                           Yc, len, 1, p, (int)p, (int)Nt, mi, nb);
+                cgemm_bf16(ctx, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1,
+                           p, (int)p, (int)Nt, mi, nb);
             else
                 cgemm(ctx, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1, p,
                       (int)p, (int)Nt, mi, nullptr, nb);
+        }
+        if (lowrank) {
+            // T (p x r) = X U_r, then Y = (T diag f) U_r^H; T holds at most m/2 columns
+            const int64_t bT = p * half;
+            c64 *T = (c64 *)ctx->scratch_get(SLOT_LOWRANK, (size_t)(bT * nb) * sizeof(c64));
+            const RankSel s1{w.r, mi, 2}, s2{w.r, mi, 3};
+            if (split) {
+                cgemm_bf16(ctx, "apply_xu", Pc, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1,
+                           p, (int)p, half, mi, nb, s1);
+                cgemm_bf16(ctx, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p,
+                           (int)p, mi, half, nb, s2);
+            } else {
+                cgemm(ctx, "apply_xu", Pc, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1, p,
+                      (int)p, half, mi, nullptr, nb, s1);
+                cgemm(ctx, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p,
+                      (int)p, mi, half, nullptr, nb, s2);
+            }
         }
     }
 }
