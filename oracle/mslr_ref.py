@@ -30,6 +30,10 @@ PINS = {
     "pogm_mod": "scalar-recurrence golden table; toy quadratic with identity prox vs hand trace",
     "poweriter_mod": "diagonal operator with known sigma1; full-mask SENSE -> 1",
     "lambdas": "closed-form values for the configs",
+    "svst_jvp": "Y equals the SVD route (SVST); tangents vs FP64 central differences; rank-1 closed "
+                "forms (radial direction, beta direction); divided differences vs (g_i - g_j)/(l_i - l_j)",
+    "pogm_lambda_jvp": "X equals run_recon_arrays(mode f64); dloss/dlambda vs FP64 central differences "
+                       "of the same loss (restart sequence unchanged)",
     "recon_costs": "X = 0 -> (||y||^2/2, 0); noise-free data at the truth -> dc 0; rank-1 single-patch "
                    "scale -> lambda |u||v|; dc quadratic in X; reg positively homogeneous",
 }
@@ -415,14 +419,15 @@ def prepare(ksp0, smaps_raw, N, Nvc, Nt):
 
 
 def run_recon_arrays(ksp, omega, smaps, *, N, Nt, PATCH_SIZES, STRIDES, NITERS, sigma1A=1.0,
-                     mode="ref", exact_adjoint=True, fun=None):
+                     mode="ref", exact_adjoint=True, fun=None, lambdas=None):
     """POGM part of run_recon (scripts/reconstruct.jl:159-226) on prepared inputs.
+    lambdas (optional) replaces the Ong-Lustig weights (for derivative checks).
 
     Returns (X (Nx,Ny,Nz,Nt,S), restarts list, extras dict).
     """
     S = len(PATCH_SIZES)
     L = S * sigma1A ** 2
-    lams = lambdas_ong_lustig(N, Nt, PATCH_SIZES)
+    lams = list(lambdas) if lambdas is not None else lambdas_ong_lustig(N, Nt, PATCH_SIZES)
     cdt = np.complex128 if mode == "f64" else np.complex64
     pdt = np.complex128 if mode == "f64" else np.complex64
     scalar = np.float32 if mode == "f32" else np.float64
@@ -475,6 +480,127 @@ def recon_costs(X, y, omega, smaps, lambdas, PATCH_SIZES, STRIDES, dtype=np.comp
         P = img2patches(X[..., k].astype(dtype), PATCH_SIZES[k], STRIDES[k], dtype=dtype)
         reg += float(lambdas[k]) * patch_nucnorm(P)
     return dc, reg
+
+
+# --------------------------------------------------------------------------- derivatives
+# The reference has no autodiff; these are the FP64 forward-mode references of the
+# differentiation form: directional derivatives of SVST (src/recon.jl:183-205) and
+# of the unrolled aliased POGM (src/mirt_mod.jl:131-184 with the in-place prox of
+# scripts/reconstruct.jl:196-202) with respect to the per-scale lambda_k.
+
+def svst_divdiff(lam, beta):
+    """Divided differences D_ij = (g(l_i) - g(l_j)) / (l_i - l_j) of the Gram spectral
+    function g(l) = max(1 - beta / sqrt(l), 0) (D_ii = g'(l_i)), in forms without the
+    difference in a denominator: both above beta^2: beta / (s_i s_j (s_i + s_j));
+    one above: g(l_i) / (l_i - l_j) with l_i - l_j >= (s_i - beta)(s_i + beta); both
+    below: 0.  A kink at l = beta^2 (the derivative there is one-sided)."""
+    lam = np.maximum(np.asarray(lam, dtype=np.float64), 0.0)
+    s = np.sqrt(lam)
+    up = s > beta
+    g = np.where(up, 1.0 - beta / np.where(up, s, 1.0), 0.0)
+    si, sj = s[:, None], s[None, :]
+    ui, uj = up[:, None], up[None, :]
+    both = beta / np.where(ui & uj, si * sj * (si + sj), 1.0)
+    dl = lam[:, None] - lam[None, :]
+    one_i = g[:, None] / np.where(ui & ~uj, dl, 1.0)
+    one_j = g[None, :] / np.where(~ui & uj, -dl, 1.0)
+    return np.where(ui & uj, both, np.where(ui & ~uj, one_i, np.where(~ui & uj, one_j, 0.0)))
+
+
+def svst_jvp(Z, beta, Zdot, bdot=0.0):
+    """(Y, Ydot) for Y = SVST(Z, beta) = Z g(Z^H Z) and a direction (Zdot, bdot), FP64:
+    G = V diag(l) V^H, W = V diag(g(l)) V^H, Y = Z W (= U diag(max(s - beta, 0)) V^H),
+    Wdot = V (D o (V^H Gdot V) + diag(dg/dbeta) bdot) V^H with Gdot = Zdot^H Z + Z^H Zdot,
+    Ydot = Zdot W + Z Wdot.  Uses the Gram of Z^H Z (p >= Nt) or of Z Z^H."""
+    Z = np.asarray(Z, dtype=np.complex128)
+    Zd = np.asarray(Zdot, dtype=np.complex128)
+    left = Z.shape[0] < Z.shape[1]
+    if left:  # Y^H = SVST(Z^H)
+        Y, Yd = svst_jvp(Z.conj().T, beta, Zd.conj().T, bdot)
+        return Y.conj().T, Yd.conj().T
+    G = Z.conj().T @ Z
+    lam, V = np.linalg.eigh(G)
+    lam = np.maximum(lam, 0.0)
+    s = np.sqrt(lam)
+    up = s > beta
+    g = np.where(up, 1.0 - beta / np.where(up, s, 1.0), 0.0)
+    W = (V * g) @ V.conj().T
+    Gd = Zd.conj().T @ Z + Z.conj().T @ Zd
+    M = V.conj().T @ Gd @ V
+    dgb = np.where(up, -1.0 / np.where(up, s, 1.0), 0.0)
+    Wd = V @ (svst_divdiff(lam, beta) * M + np.diag(dgb * bdot)) @ V.conj().T
+    return Z @ W, Zd @ W + Z @ Wd
+
+
+def pogm_lambda_jvp(ksp, omega, smaps, x_ref, *, N, Nt, PATCH_SIZES, STRIDES, NITERS, lambdas=None,
+                    sigma1A=1.0):
+    """Loss = mean |sum_s X_s - x_ref|^2 after NITERS iterations of the aliased POGM
+    (the iteration of run_recon_arrays in mode f64: g = A'(A sum X - y), u = x - alpha g,
+    z = u + beta (u - u_old) + gamma (u - x), x_s = patchSVST(z_s, zeta lambda_s);
+    src/mirt_mod.jl:131-184, scripts/reconstruct.jl:174-226) and its derivative with
+    respect to each lambda_k by forward-mode (one tangent per scale), FP64.
+    The restart decisions are those of the primal run (piecewise constant in lambda).
+    Returns (loss, [dloss/dlambda_k], restarts, X)."""
+    S = len(PATCH_SIZES)
+    L = S * sigma1A ** 2
+    lams = list(lambdas) if lambdas is not None else lambdas_ong_lustig(N, Nt, PATCH_SIZES)
+    sm = smaps.astype(np.complex128)
+    y = ksp.astype(np.complex128)
+
+    def AtA(x4):
+        return A_adjoint(A_forward(x4, omega, sm), omega, sm)
+
+    def prox_jvp(z, zd_list, thr, s):
+        """patchSVST of one scale and its tangents (the patch maps are linear)."""
+        ps, ss = PATCH_SIZES[s], STRIDES[s]
+        P = img2patches(z, ps, ss, dtype=np.complex128)
+        Pd = [img2patches(zd, ps, ss, dtype=np.complex128) for zd in zd_list]
+        Y = np.empty_like(P)
+        Yd = [np.empty_like(P) for _ in zd_list]
+        for ip in range(P.shape[2]):
+            for j in range(len(zd_list)):
+                Yi, Ydj = svst_jvp(P[:, :, ip], thr, Pd[j][:, :, ip], zeta_now * (1.0 if j == s else 0.0))
+                Yd[j][:, :, ip] = Ydj
+            Y[:, :, ip] = Yi
+        back = lambda Q: patches2img(Q, ps, ss, z.shape[:3], dtype=np.complex128)
+        return back(Y), [back(Q) for Q in Yd]
+
+    alpha = 1.0 / L
+    X = np.zeros((*N, Nt, S), dtype=np.complex128)
+    X[..., 0] = A_adjoint(y, omega, sm)
+    Xd = [np.zeros_like(X) for _ in range(S)]  # d X / d lambda_j
+    U, Ud = X.copy(), [np.zeros_like(X) for _ in range(S)]
+    told, sig = 1.0, 1.0
+    restarts = []
+    for it in range(1, NITERS + 1):
+        g = A_adjoint(A_forward(X.sum(axis=4), omega, sm) - y, omega, sm)
+        gd = [AtA(Xd[j].sum(axis=4)) for j in range(S)]
+        Un = X - alpha * g[..., None]
+        Und = [Xd[j] - alpha * gd[j][..., None] for j in range(S)]
+        tnew = 0.5 * (1 + math.sqrt(1 + 8 * told ** 2)) if it == NITERS else 0.5 * (1 + math.sqrt(1 + 4 * told ** 2))
+        beta = (told - 1) / tnew
+        gamma = sig * told / tnew
+        zeta_now = alpha * (1 + beta + gamma)
+        Z = Un + beta * (Un - U) + gamma * (Un - X)
+        Zd = [Und[j] + beta * (Und[j] - Ud[j]) + gamma * (Und[j] - Xd[j]) for j in range(S)]
+        rs = float(np.sum(np.real(-g[..., None] * (Un - U)), dtype=np.float64)) <= 0.0
+        restarts.append(bool(rs))
+        if rs:
+            tnew, sig = 1.0, 1.0
+        Xn = np.empty_like(X)
+        Xnd = [np.empty_like(X) for _ in range(S)]
+        for s in range(S):
+            xs, xds = prox_jvp(Z[..., s], [Zd[j][..., s] for j in range(S)], zeta_now * lams[s], s)
+            Xn[..., s] = xs
+            for j in range(S):
+                Xnd[j][..., s] = xds[j]
+        X, Xd, U, Ud = Xn, Xnd, Un, Und
+        told = tnew
+    r = X.sum(axis=4) - x_ref
+    n = r.size
+    loss = float(np.vdot(r.ravel(), r.ravel()).real) / n
+    grads = [2.0 * float(np.vdot(r.ravel(), Xd[j].sum(axis=4).ravel()).real) / n for j in range(S)]
+    return loss, grads, restarts, X
 
 
 def nrmse(x, ref):

@@ -140,3 +140,13 @@ def test_mixed_form_vs_plain():
     X2, r2 = _gpu_run("small", 6, "mixed")
     assert r1 == r2
     assert rel_err(X2.sum(axis=4), X1.sum(axis=4)) < 1e-4
+
+
+def test_split_apply_forms_at_the_boundary(split, rng):
+    from mslr_hip import recon
+    from test_gpu_svst import APPLY_RANKS, mixed_rank_batch
+    P, betas = mixed_rank_batch(rng, APPLY_RANKS)
+    Y = recon.svst_batched(P, betas).to_host()
+    for i, k in enumerate(APPLY_RANKS):
+        Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
+        assert rel_err(Y[:, :, i], Yo) < 1e-4, (i, k)

@@ -247,3 +247,26 @@ def test_both_reductions_match_lapack(ctx, rng, monkeypatch, reduction, p, n, r)
     Y = recon.SVST(X, beta).to_host()
     Yo = O.SVST(X.astype(np.complex128), np.float32(beta))
     assert rel_err(Y, Yo) < 2e-5
+
+
+def mixed_rank_batch(rng, ranks, p=1000, n=300):
+    """Patches with the given numbers of singular values above their thresholds."""
+    P = np.asfortranarray(np.stack([lowrank(rng, p, n, max(1, k), noise=1e-2) for k in ranks], axis=2))
+    betas = []
+    for i, k in enumerate(ranks):
+        s = np.linalg.svd(P[:, :, i].astype(np.complex128), compute_uv=False)
+        betas.append(2.0 * s[0] if k == 0 else 0.5 * (s[k - 1] + s[k]))
+    return P, np.array(betas)
+
+
+# r <= m/2 takes Y = (X U_r f) U_r^H, r > m/2 forms W: both sides of the boundary
+APPLY_RANKS = [0, 2, 149, 150, 151, 299]
+
+
+def test_apply_forms_at_the_boundary(ctx, rng):
+    from mslr_hip import recon
+    P, betas = mixed_rank_batch(rng, APPLY_RANKS)
+    Y = recon.svst_batched(P, betas).to_host()
+    for i, k in enumerate(APPLY_RANKS):
+        Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
+        assert rel_err(Y[:, :, i], Yo) < 3e-5, (i, k)

@@ -355,3 +355,87 @@ def test_recon_costs_scaling(rng):
     d3, r3 = O.recon_costs(3.0 * X, y, omega, smaps, lams, ps, ss)
     assert d3 == pytest.approx(9.0 * d1, rel=1e-12)
     assert r3 == pytest.approx(3.0 * r1, rel=1e-12)
+
+
+# ----------------------------------------------------------------- derivatives
+def _lr(rng, p, n, r, noise=1e-2):
+    A = crandn(rng, (p, r), np.complex128) @ crandn(rng, (r, n), np.complex128)
+    return A * np.linspace(3.0, 1.0, r).mean() + noise * crandn(rng, (p, n), np.complex128)
+
+
+@pytest.mark.parametrize("p,n", [(40, 12), (12, 40)])
+def test_svst_jvp_value_is_svst(rng, p, n):
+    Z = _lr(rng, p, n, 3)
+    s = np.linalg.svd(Z, compute_uv=False)
+    for beta in [0.0, 0.5 * (s[2] + s[3]), 0.3 * s[0], 2 * s[0]]:
+        Y, _ = O.svst_jvp(Z, beta, np.zeros_like(Z))
+        assert rel_err(Y, O.SVST(Z, beta)) < 1e-12 or np.abs(Y).max() < 1e-12
+
+
+@pytest.mark.parametrize("p,n", [(40, 12), (12, 40), (30, 30)])
+def test_svst_jvp_central_differences(rng, p, n):
+    Z = _lr(rng, p, n, 4, noise=0.3)
+    s = np.linalg.svd(Z, compute_uv=False)
+    beta = 0.5 * (s[3] + s[4])
+    Zd = crandn(rng, (p, n), np.complex128)
+    h = 1e-5
+    for bd in [0.0, 1.0]:
+        _, Yd = O.svst_jvp(Z, beta, Zd, bd)
+        fd = (O.SVST(Z + h * Zd, beta + h * bd) - O.SVST(Z - h * Zd, beta - h * bd)) / (2 * h)
+        assert rel_err(Yd, fd) < 1e-7, bd
+
+
+def test_svst_jvp_rank1_closed_forms(rng):
+    u = crandn(rng, (30, 1), np.complex128)
+    v = crandn(rng, (8, 1), np.complex128)
+    u /= np.linalg.norm(u)
+    v /= np.linalg.norm(v)
+    Z = 5.0 * (u @ v.conj().T)
+    _, Yd = O.svst_jvp(Z, 2.0, Z)  # radial: d/dt ((1+t) 5 - 2) u v^H = 5 u v^H
+    assert rel_err(Yd, Z) < 1e-12
+    _, Yb = O.svst_jvp(Z, 2.0, np.zeros_like(Z), 1.0)  # d/dbeta = -u v^H
+    assert rel_err(Yb, -(u @ v.conj().T)) < 1e-12
+
+
+def test_svst_divdiff_explicit():
+    lam = np.array([9.0, 4.0, 1.0, 0.25])
+    beta = 1.5  # above: 9, 4 ; below: 1, 0.25
+    g = lambda l: max(1 - beta / math.sqrt(l), 0.0)
+    D = O.svst_divdiff(lam, beta)
+    for i in range(4):
+        for j in range(4):
+            if i == j:
+                want = beta / (2 * lam[i] ** 1.5) if math.sqrt(lam[i]) > beta else 0.0
+            else:
+                want = (g(lam[i]) - g(lam[j])) / (lam[i] - lam[j])
+            assert D[i, j] == pytest.approx(want, rel=1e-13, abs=1e-15)
+
+
+def test_pogm_lambda_jvp_central_differences():
+    """d loss / d lambda_k against central differences of the FP64 POGM run (small
+    problem, 3 iterations, restart sequence unchanged by the step)."""
+    from problems import prepared
+    cfg, smaps, ksp, omega, sf = prepared("small")
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    kw = dict(N=N, Nt=Nt, PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"], NITERS=3)
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    X0, r0, _ = O.run_recon_arrays(ksp, omega, smaps, mode="f64", **kw)
+    rng = np.random.default_rng(3)
+    x_ref = X0.sum(axis=4) + 0.1 * np.abs(X0).max() * crandn(rng, X0.shape[:4], np.complex128)
+    loss, grads, rs, X = O.pogm_lambda_jvp(ksp, omega, smaps, x_ref, **kw)
+    assert rs == r0
+    assert rel_err(X, X0) < 1e-12
+
+    def lossf(lm):
+        Xl, rl, _ = O.run_recon_arrays(ksp, omega, smaps, mode="f64", lambdas=lm, **kw)
+        assert rl == r0
+        r = Xl.sum(axis=4) - x_ref
+        return float(np.vdot(r.ravel(), r.ravel()).real) / r.size
+
+    assert lossf(lams) == pytest.approx(loss, rel=1e-12)
+    for k in range(len(lams)):
+        h = 1e-4 * lams[k]
+        lp = list(lams); lp[k] += h
+        lm = list(lams); lm[k] -= h
+        fd = (lossf(lp) - lossf(lm)) / (2 * h)
+        assert grads[k] == pytest.approx(fd, rel=1e-5), k
