@@ -189,6 +189,16 @@ mslr_status_t mslr_patches2img(mslr_ctx ctx, const void *P_dev, const int64_t N[
  */
 mslr_status_t mslr_svst_batched(mslr_ctx ctx, void *P_dev, int64_t p, int64_t Nt, int64_t Np,
                                 const double *beta_host, int per_patch);
+/* Vector-Jacobian product of the batched SVST above (no counterpart in the
+ * reference; the SVST pullback an rrule of SVST, src/recon.jl:183-205, would call):
+ * for Y_i = SVST(Z_i, beta_i) and cotangents Ybar_i, writes Zbar_i (device, distinct
+ * from Z and Ybar) and betabar = d<Ybar, Y>/d beta (host: Np values with per_patch,
+ * else their sum).  Z, Ybar, Zbar: ComplexF32 (p, Nt, Np); p >= Nt >= 2.  Uses
+ * every eigenpair of Z_i^H Z_i (FP64) and safeguarded divided differences of
+ * g(l) = max(1 - beta / sqrt(l), 0) (no 1/(s_i^2 - s_j^2) is formed). */
+mslr_status_t mslr_svst_vjp(mslr_ctx ctx, const void *Z_dev, const void *Ybar_dev, void *Zbar_dev,
+                            int64_t p, int64_t Nt, int64_t Np, const double *beta_host,
+                            int per_patch, double *betabar_host);
 /* Sum over patches of the nuclear norms (patch_nucnorm, src/recon.jl:138-146). */
 mslr_status_t mslr_patch_nucnorm(mslr_ctx ctx, const void *P_dev, int64_t p, int64_t Nt,
                                  int64_t Np, double *out);
@@ -225,6 +235,7 @@ typedef struct {
     double bsig;                /* gamma decrease factor, >= 0 */
     int form;                   /* MSLR_FORM_* */
     int log_costs;              /* 1: keep dc_cost and reg_cost of every iterate (the logger) */
+    int checkpoint;             /* 1: keep each iteration's prox input for mslr_pogm_lambda_vjp */
 } mslr_pogm_desc;
 
 /*
@@ -254,6 +265,16 @@ mslr_status_t mslr_pogm_log(mslr_pogm st, int *iters_done, int *restarts);
  * The restart flags are mslr_pogm_log's (x_0's is false).  Needs log_costs = 1
  * at create; reading the log after the last iteration costs one forward A. */
 mslr_status_t mslr_pogm_costs(mslr_pogm st, int *n, double *dc_costs, double *reg_costs);
+/* Derivative form (no counterpart in the reference, which has no autodiff; the
+ * pullback a ChainRulesCore rrule of run_recon's POGM loop,
+ * scripts/reconstruct.jl:220-226, would call).  After k iterations of a state
+ * created with checkpoint = 1: lambda_bar[s] = d<xbar, sum_s X_s>/d lambda_s through
+ * the k unrolled iterations, xbar (device, ComplexF32 (Nx,Ny,Nz,Nt)) the cotangent
+ * of the final image sum (for the MSE to x_ref: 2 (sum_s X_s - x_ref) / numel).
+ * The restart decisions are held fixed.  Scales of disjoint patches with at least
+ * Nt voxels, or the whole-image patch; others give MSLR_ERR_UNSUPPORTED.
+ * Allocates its backward state for the call. */
+mslr_status_t mslr_pogm_lambda_vjp(mslr_pogm st, const void *xbar_dev, double *lambda_bar);
 /* Per-scale SVST thresholds zeta*lambda_k used in the last iteration (host, nscales). */
 mslr_status_t mslr_pogm_thresholds(mslr_pogm st, double *beta);
 

@@ -188,7 +188,9 @@ enum ScratchSlot : size_t {
     SLOT_PLANES = 10,   // SENSE plane-major image frames
     SLOT_SPLIT = 11,    // patch data as split-BF16 words (mixed-precision products)
     SLOT_LOWRANK = 12,  // X U_r of the low-rank SVST apply
-    SLOT_COUNT = 13
+    SLOT_VJP = 13,      // m x m intermediates of the SVST derivative
+    SLOT_VJP_P = 14,    // patch cotangents of the prox derivative
+    SLOT_COUNT = 15
 };
 
 }  // namespace mslr

@@ -141,8 +141,10 @@ __device__ __forceinline__ void cover_range(const PatchGeom &g, int a, int x, in
     }
 }
 
-// One thread per output element (voxel, t).
-__global__ void k_patches2img(const c64 *__restrict__ P, c64 *__restrict__ img, PatchGeom g) {
+// One thread per output element (voxel, t).  divide = false leaves the sum of the
+// covering patches (the adjoint of img2patches).
+__global__ void k_patches2img(const c64 *__restrict__ P, c64 *__restrict__ img, PatchGeom g,
+                              int divide) {
     const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
     const int64_t n = V * g.Nt;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
@@ -171,8 +173,30 @@ __global__ void k_patches2img(const c64 *__restrict__ P, c64 *__restrict__ img, 
                 }
             }
         }
-        const float d = cnt > 1.f ? cnt : 1.f;
+        const float d = (divide && cnt > 1.f) ? cnt : 1.f;
         img[e] = c64{acc.re / d, acc.im / d};
+    }
+}
+
+// out = img / max(coverage count, 1) per voxel (the adjoint of the division in
+// patches2img); in place allowed.
+__global__ void k_div_coverage(const c64 *img, c64 *out, PatchGeom g) {
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    const int64_t n = V * g.Nt;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = e % V;
+        const int x = (int)(v % g.N[0]);
+        const int y = (int)((v / g.N[0]) % g.N[1]);
+        const int z = (int)(v / ((int64_t)g.N[0] * g.N[1]));
+        int xl, xh, yl, yh, zl, zh;
+        cover_range(g, 0, x, xl, xh);
+        cover_range(g, 1, y, yl, yh);
+        cover_range(g, 2, z, zl, zh);
+        const float cnt = (float)(max(xh - xl + 1, 0) * max(yh - yl + 1, 0) * max(zh - zl + 1, 0));
+        const float d = cnt > 1.f ? cnt : 1.f;
+        const c64 a = img[e];
+        out[e] = c64{a.re / d, a.im / d};
     }
 }
 
@@ -206,9 +230,25 @@ void launch_patches2img(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *P, c64 *
         if (g.exact_tiling() && nb < (int64_t)1 << 31)
             k_patches2img_tiled<<<(unsigned)nb, 256, 0, ctx->stream>>>(P, img, g);
         else
-            k_patches2img<<<grid_for(n), 256, 0, ctx->stream>>>(P, img, g);
+            k_patches2img<<<grid_for(n), 256, 0, ctx->stream>>>(P, img, g, 1);
         HIP_LAUNCH_CHECK();
     }
+}
+
+void launch_patches2img_sum(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *P, c64 *img) {
+    const int64_t n = (int64_t)g.N[0] * g.N[1] * g.N[2] * g.Nt;
+    if (n == 0) return;
+    MSLR_PROF(ctx, "patches2img");
+    k_patches2img<<<grid_for(n), 256, 0, ctx->stream>>>(P, img, g, 0);
+    HIP_LAUNCH_CHECK();
+}
+
+void launch_div_coverage(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *out) {
+    const int64_t n = (int64_t)g.N[0] * g.N[1] * g.N[2] * g.Nt;
+    if (n == 0) return;
+    MSLR_PROF(ctx, "div_coverage");
+    k_div_coverage<<<grid_for(n), 256, 0, ctx->stream>>>(img, out, g);
+    HIP_LAUNCH_CHECK();
 }
 
 }  // namespace mslr

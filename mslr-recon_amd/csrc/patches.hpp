@@ -33,5 +33,9 @@ void launch_img2patches(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64
                         int64_t ip0, int64_t np);
 // img <- sum of covering patches / max(count, 1), summed in patch order.
 void launch_patches2img(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *P, c64 *img);
+// img <- sum of covering patches (the adjoint of launch_img2patches).
+void launch_patches2img_sum(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *P, c64 *img);
+// out <- img / max(count, 1) per voxel (the adjoint of the division); in place allowed.
+void launch_div_coverage(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *out);
 
 }  // namespace mslr

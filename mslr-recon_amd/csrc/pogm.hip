@@ -46,8 +46,10 @@ struct PogmDev {
 
 constexpr int UPD_BLOCKS = 2048;
 
+// scal_log (nullable): alpha, beta, gamma, zeta of each iteration, for the
+// derivative's backward pass.
 __global__ void k_pogm_pre(PogmDev *s, const double *__restrict__ lambdas, int nscales,
-                           double *__restrict__ thresholds) {
+                           double *__restrict__ thresholds, double *__restrict__ scal_log) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int k = s->iter + 1;
     const double told = s->told;
@@ -61,6 +63,68 @@ __global__ void k_pogm_pre(PogmDev *s, const double *__restrict__ lambdas, int n
     s->gamma = gamma;
     s->zeta = zeta;
     for (int i = 0; i < nscales; ++i) thresholds[i] = zeta * lambdas[i];
+    if (scal_log) {
+        double *o = scal_log + 4 * (k - 1);
+        o[0] = s->alpha;
+        o[1] = beta;
+        o[2] = gamma;
+        o[3] = zeta;
+    }
+}
+
+// dst[iter] <- X (the prox input z of the running iteration), iter from the device
+// counter so that graph replays fill their own slots.
+__global__ __launch_bounds__(256) void k_ckpt_copy(const float4 *__restrict__ X,
+                                                   float4 *__restrict__ dst, int64_t n4,
+                                                   const PogmDev *s) {
+    float4 *d = dst + (int64_t)s->iter * n4;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = X[i];
+}
+
+// Backward of z = (1 + b + c) u - b u_old - c x and u = x - alpha g for one
+// iteration, per element over all scales (zb holds the prox input cotangent):
+//   ut = ub + (1 + b + c) zb ;  ub <- -b zb ;  xb <- ut - c zb ;  gs = -alpha sum_s ut
+__global__ __launch_bounds__(256) void k_vjp_state(const c64 *__restrict__ zb, c64 *__restrict__ ub,
+                                                   c64 *__restrict__ xb, c64 *__restrict__ gs,
+                                                   int64_t n, int nscales, float alpha, float b,
+                                                   float c) {
+    const float c1 = 1.f + b + c;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float sr = 0.f, si = 0.f;
+        for (int k = 0; k < nscales; ++k) {
+            const int64_t o = i + k * n;
+            const c64 z = zb[o], u = ub[o];
+            const float tr = u.re + c1 * z.re, ti = u.im + c1 * z.im;
+            ub[o] = c64{-b * z.re, -b * z.im};
+            xb[o] = c64{tr - c * z.re, ti - c * z.im};
+            sr += tr;
+            si += ti;
+        }
+        gs[i] = c64{-alpha * sr, -alpha * si};
+    }
+}
+
+// xb_s += h for every scale.
+__global__ __launch_bounds__(256) void k_vjp_add(c64 *__restrict__ xb, const c64 *__restrict__ h,
+                                                 int64_t n, int nscales) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const c64 v = h[i];
+        for (int k = 0; k < nscales; ++k) xb[i + k * n] = xb[i + k * n] + v;
+    }
+}
+
+// lam_bar[s] += zeta * sum(betabar[0..np)) (fixed order, one workgroup).
+__global__ __launch_bounds__(256) void k_vjp_lambda(const double *__restrict__ betabar, int64_t np,
+                                                    double zeta, double *__restrict__ lam_bar) {
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < np; i += blockDim.x) acc += betabar[i];
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) *lam_bar += zeta * acc;
 }
 
 // X <- z, U <- u; block partials of (restart dot, ||g||^2, ||u - u_old||^2).
@@ -313,6 +377,38 @@ static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 
         scale_nucnorm(ctx, g, out, nuc_tot);
 }
 
+// Backward of x_s = patchSVST(z_s, beta) = p2i(SVST(i2p(z_s))) for one scale:
+// zb <- i2p^T(SVST'^T(p2i^T(xb))), betabar per patch into bb (device), where
+// p2i^T(xb) = i2p(xb / count) and i2p^T sums the covering patches.  tmp: an image
+// of scratch.  Patches need at least Nt voxels (1x1x1 patches are not covered).
+static void prox_vjp_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *z, const c64 *xb,
+                           c64 *zb, const double *beta_dev, double *bb, int *err, c64 *tmp) {
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    MSLR_REQUIRE(g.p >= g.Nt, MSLR_ERR_UNSUPPORTED,
+                 "the lambda derivative needs patches with at least Nt voxels");
+    if (g.Np == 1 && g.p == V) {
+        svst_vjp_batched(ctx, z, xb, zb, g.p, g.Nt, 1, beta_dev, 0, bb, err);
+        return;
+    }
+    const bool disjoint = g.disjoint();
+    const size_t bytes = (size_t)(g.p * g.Nt * g.Np) * sizeof(c64);
+    c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, bytes);
+    c64 *Pb = (c64 *)ctx->scratch_get(SLOT_PATCH_OUT, bytes);
+    c64 *Qb = (c64 *)ctx->scratch_get(SLOT_VJP_P, bytes);
+    launch_img2patches(ctx, g, z, P, 0, g.Np);
+    if (disjoint) {
+        launch_img2patches(ctx, g, xb, Pb, 0, g.Np);
+    } else {
+        launch_div_coverage(ctx, g, xb, tmp);
+        launch_img2patches(ctx, g, tmp, Pb, 0, g.Np);
+    }
+    svst_vjp_batched(ctx, P, Pb, Qb, g.p, g.Nt, g.Np, beta_dev, 0, bb, err);
+    if (disjoint)
+        launch_patches2img(ctx, g, Qb, zb);  // every covered voxel once: a copy
+    else
+        launch_patches2img_sum(ctx, g, Qb, zb);
+}
+
 }  // namespace mslr
 
 using namespace mslr;
@@ -339,6 +435,10 @@ struct mslr_pogm_s {
     int64_t n_dc_part = 0;
     std::vector<double *> nuc_part;
     int dc_final_at = -1;  // iterate whose dc_cost was last completed by an extra A
+    // derivative form: the prox inputs z_k of every iteration and the scalars
+    bool checkpoint = false;
+    c64 *ckpt = nullptr;
+    double *scal_log = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // scratch generation the graph was captured at / of the last eager iteration
@@ -398,13 +498,21 @@ static void pogm_iteration(mslr_pogm_s *st) {
     }
     {
         MSLR_PROF(ctx, "pogm_pre");
-        k_pogm_pre<<<1, 64, 0, ctx->stream>>>(st->dev, st->lam_dev, (int)st->S, st->thr_dev);
+        k_pogm_pre<<<1, 64, 0, ctx->stream>>>(st->dev, st->lam_dev, (int)st->S, st->thr_dev,
+                                              st->scal_log);
         HIP_LAUNCH_CHECK();
     }
     {
         MSLR_PROF(ctx, "pogm_update");
         k_pogm_update<<<UPD_BLOCKS, 256, 0, ctx->stream>>>(st->X, st->U, st->g, n, (int)st->S, n,
                                                            st->dev, st->partials);
+        HIP_LAUNCH_CHECK();
+    }
+    if (st->checkpoint) {
+        MSLR_PROF(ctx, "checkpoint");
+        const int64_t n4 = n * st->S / 2;
+        k_ckpt_copy<<<2048, 256, 0, ctx->stream>>>((const float4 *)st->X, (float4 *)st->ckpt, n4,
+                                                   st->dev);
         HIP_LAUNCH_CHECK();
     }
     {
@@ -535,6 +643,16 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
             HIP_CHECK(hipMemset(st->log_dev, 0, sizeof(int) * std::max(1, d.niter)));
             MSLR_REQUIRE(d.log_costs == 0 || d.log_costs == 1, MSLR_ERR_ARG, "log_costs must be 0 or 1");
             st->log_costs = d.log_costs == 1;
+            MSLR_REQUIRE(d.checkpoint == 0 || d.checkpoint == 1, MSLR_ERR_ARG,
+                         "checkpoint must be 0 or 1");
+            st->checkpoint = d.checkpoint == 1;
+            if (st->checkpoint) {
+                MSLR_REQUIRE((n * st->S) % 2 == 0, MSLR_ERR_UNSUPPORTED,
+                             "checkpointing needs an even element count");
+                st->ckpt = (c64 *)st->alloc((size_t)std::max(1, d.niter) * (size_t)(n * st->S) *
+                                            sizeof(c64));
+                st->scal_log = (double *)st->alloc(sizeof(double) * 4 * (size_t)std::max(1, d.niter));
+            }
             if (st->log_costs) {
                 st->costs = (double *)st->alloc(sizeof(double) * 2 * (size_t)(d.niter + 1));
                 st->n_dc_part = st->Nt * st->Nc * (int64_t)sense->nlb;
@@ -694,6 +812,98 @@ mslr_status_t mslr_pogm_costs(mslr_pogm st, int *n_out, double *dc_costs, double
         for (int i = 0; i <= k; ++i) {
             if (dc_costs) dc_costs[i] = h[2 * i];
             if (reg_costs) reg_costs[i] = h[2 * i + 1];
+        }
+    });
+}
+
+mslr_status_t mslr_pogm_lambda_vjp(mslr_pogm st, const void *xbar_dev, double *lambda_bar) {
+    return guarded([&] {
+        MSLR_REQUIRE(st && xbar_dev && lambda_bar, MSLR_ERR_ARG, "null argument");
+        MSLR_REQUIRE(st->checkpoint, MSLR_ERR_ARG, "the state was created with checkpoint = 0");
+        MSLR_REQUIRE(st->initialised, MSLR_ERR_ARG, "mslr_pogm_init has not been called");
+        mslr_ctx_s *ctx = st->ctx;
+        ctx->activate();
+        const int T = st->iters_done;
+        const int64_t S = st->S, n = st->V * st->Nt;
+        for (int64_t k = 0; k < S; ++k) lambda_bar[k] = 0.0;
+        if (T == 0) return;
+        std::vector<double> sc(4 * (size_t)T);
+        HIP_CHECK(hipMemcpyAsync(sc.data(), st->scal_log, sizeof(double) * sc.size(),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        int64_t npmax = 1;
+        for (const PatchGeom &g : st->geoms) npmax = std::max<int64_t>(npmax, g.Np);
+        // backward state (allocated once per call; the forward's buffers stay intact)
+        std::vector<void *> tmp;
+        auto dalloc = [&](size_t b) {
+            void *p = nullptr;
+            HIP_CHECK(hipMalloc(&p, std::max<size_t>(b, 256)));
+            tmp.push_back(p);
+            return p;
+        };
+        try {
+            c64 *xb = (c64 *)dalloc((size_t)(n * S) * sizeof(c64));
+            c64 *ub = (c64 *)dalloc((size_t)(n * S) * sizeof(c64));
+            c64 *zb = (c64 *)dalloc((size_t)(n * S) * sizeof(c64));
+            c64 *gs = (c64 *)dalloc((size_t)n * sizeof(c64));
+            c64 *h = (c64 *)dalloc((size_t)n * sizeof(c64));
+            c64 *timg = (c64 *)dalloc((size_t)n * sizeof(c64));
+            c64 *y0 = (c64 *)dalloc((size_t)std::max<int64_t>(1, st->K * st->Nc * st->Nt) *
+                                    sizeof(c64));
+            double *bb = (double *)dalloc(sizeof(double) * (size_t)npmax);
+            double *lb = (double *)dalloc(sizeof(double) * (size_t)S);
+            double *thr = (double *)dalloc(sizeof(double) * (size_t)S);
+            int *err = (int *)dalloc(sizeof(int));
+            fill_zero(ctx, y0, st->K * st->Nc * st->Nt);
+            fill_zero(ctx, ub, n * S);
+            HIP_CHECK(hipMemsetAsync(lb, 0, sizeof(double) * S, ctx->stream));
+            HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+            for (int64_t k = 0; k < S; ++k)
+                HIP_CHECK(hipMemcpyAsync(xb + k * n, xbar_dev, (size_t)n * sizeof(c64),
+                                         hipMemcpyDeviceToDevice, ctx->stream));
+            const unsigned nbl = (unsigned)std::min<int64_t>(cdiv(n, 256), 1 << 16);
+            std::vector<double> thr_h((size_t)S);
+            for (int it = T; it >= 1; --it) {
+                const double alpha = sc[4 * (it - 1)], beta = sc[4 * (it - 1) + 1];
+                const double gamma = sc[4 * (it - 1) + 2], zeta = sc[4 * (it - 1) + 3];
+                for (int64_t k = 0; k < S; ++k) thr_h[k] = zeta * st->lambdas[k];
+                HIP_CHECK(hipMemcpyAsync(thr, thr_h.data(), sizeof(double) * S,
+                                         hipMemcpyHostToDevice, ctx->stream));
+                const c64 *zk = st->ckpt + (int64_t)(it - 1) * n * S;
+                for (int64_t k = 0; k < S; ++k) {
+                    prox_vjp_scale(ctx, st->geoms[k], zk + k * n, xb + k * n, zb + k * n, thr + k, bb,
+                                   err, timg);
+                    MSLR_PROF(ctx, "vjp_lambda");
+                    k_vjp_lambda<<<1, 256, 0, ctx->stream>>>(bb, st->geoms[k].Np, zeta, lb + k);
+                    HIP_LAUNCH_CHECK();
+                }
+                {
+                    MSLR_PROF(ctx, "vjp_state");
+                    k_vjp_state<<<nbl, 256, 0, ctx->stream>>>(zb, ub, xb, gs, n, (int)S,
+                                                              (float)alpha, (float)beta,
+                                                              (float)gamma);
+                    HIP_LAUNCH_CHECK();
+                }
+                // h = A'A gs (the gradient map's derivative, the same for every scale)
+                sense_grad_internal(st->sense, gs, 1, n, y0, h, nullptr);
+                {
+                    MSLR_PROF(ctx, "vjp_state");
+                    k_vjp_add<<<nbl, 256, 0, ctx->stream>>>(xb, h, n, (int)S);
+                    HIP_LAUNCH_CHECK();
+                }
+            }
+            int herr = 0;
+            HIP_CHECK(hipMemcpyAsync(lambda_bar, lb, sizeof(double) * S, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            for (void *p : tmp) hipFree(p);
+            tmp.clear();
+            MSLR_REQUIRE(herr == 0, MSLR_ERR_NUMERIC, "non-finite values in a patch matrix");
+        } catch (...) {
+            hipStreamSynchronize(ctx->stream);
+            for (void *p : tmp) hipFree(p);
+            throw;
         }
     });
 }

@@ -669,20 +669,21 @@ __device__ __forceinline__ c128 wave_sum(c128 v) {
 //                   eigenvector by a twisted factorisation;
 //   k_eig_clusters: tight clusters redone by inverse iteration, filter factors.
 // r = -1 marks beta == 0 (identity).
-constexpr int EIG_INFO = 4;  // per patch: tnorm, pivmin, upper Gershgorin bound
+constexpr int EIG_INFO = 4;  // per patch: tnorm, pivmin, upper and lower search bounds
 
+// all != 0: every eigenpair is wanted (r = m), for the SVST derivative.
 __global__ __launch_bounds__(64) void k_eig_count(const double *__restrict__ dall,
                                                   const double *__restrict__ eall, int m, int64_t nb,
                                                   const double *__restrict__ beta,
                                                   int beta_per_patch, double *__restrict__ info,
-                                                  int *__restrict__ rall) {
+                                                  int *__restrict__ rall, int all) {
     const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= nb) return;
     const double *d = dall + ip * m;
     const double *e = eall + ip * m;
     const float b32 = (float)beta[beta_per_patch ? ip : 0];
     const double bd = (double)b32;
-    if (bd == 0.0) {
+    if (bd == 0.0 && !all) {
         rall[ip] = -1;
         return;
     }
@@ -697,13 +698,16 @@ __global__ __launch_bounds__(64) void k_eig_count(const double *__restrict__ dal
         if (i + 1 < m) emax = fmax(emax, e[i] * e[i]);
     }
     const double pivmin = DBL_MIN * fmax(1.0, emax);
-    gu = gu + 2.0 * DBL_EPSILON * fmax(fabs(gl), fabs(gu)) + pivmin;
+    const double pad = 2.0 * DBL_EPSILON * fmax(fabs(gl), fabs(gu)) + pivmin;
+    gu = gu + pad;
     int r = 0;
-    if (tau < gu) r = m - sturm_count(d, e, m, tau, pivmin);
+    if (all) r = m;
+    else if (tau < gu) r = m - sturm_count(d, e, m, tau, pivmin);
     double *inf = info + ip * EIG_INFO;
     inf[0] = tn;
     inf[1] = pivmin;
     inf[2] = gu;
+    inf[3] = all ? gl - pad : tau;  // every wanted eigenvalue lies above it
     rall[ip] = r;
 }
 
@@ -735,10 +739,8 @@ __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ d
     const double *e = eall + (int64_t)ip * m;
     const double *inf = info + (int64_t)ip * EIG_INFO;
     const double tnorm = inf[0], pivmin = inf[1], gu = inf[2];
-    const double bd = (double)(float)beta[beta_per_patch ? ip : 0];
-    const double tau = bd * bd;
     const int idx = m - 1 - k;  // ascending index: count(lo) <= idx < count(hi)
-    double lo = tau, hi = gu;
+    double lo = inf[3], hi = gu;
     for (int it = 0; it < 200; ++it) {
         if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
         const double w = (hi - lo) / (double)(G + 1);
@@ -964,7 +966,8 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
                                                const float *__restrict__ fs, int64_t bfs,
                                                c64 *__restrict__ C, int64_t bC, int64_t ci,
                                                int64_t cj, int M, int N, int Kfix,
-                                               const int *__restrict__ kvar, RankSel rs) {
+                                               const int *__restrict__ kvar, RankSel rs,
+                                               int accum) {
     __shared__ float Ar[MK][MT + 4], Ai[MK][MT + 4];
     __shared__ float Br[MK][MT + 4], Bi[MK][MT + 4];
     // the column tiles of one block of rows share A: keep them on one XCD
@@ -1081,7 +1084,11 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + wi + 16 * bi + r16;
                 const int j = j0 + wj + 16 * bj2 + 4 * k4 + v;
-                if (i < M && j < N) Cz[i * ci + j * cj] = c64{accr[bi][bj2][v], acci[bi][bj2][v]};
+                if (i < M && j < N) {
+                    c64 o = c64{accr[bi][bj2][v], acci[bi][bj2][v]};
+                    if (accum) o = o + Cz[i * ci + j * cj];
+                    Cz[i * ci + j * cj] = o;
+                }
             }
 }
 
@@ -1754,7 +1761,8 @@ static RankSel rank_sel_at(RankSel rs, int64_t z0) {
 static void cgemm(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, int64_t ai,
                   int64_t ak, const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB,
                   const float *fs, int64_t bfs, c64 *C, int64_t bC, int64_t ci, int64_t cj, int M,
-                  int N, int Kfix, const int *kvar, int64_t nbatch, RankSel rs = RankSel{}) {
+                  int N, int Kfix, const int *kvar, int64_t nbatch, RankSel rs = RankSel{},
+                  int accum = 0) {
     for (int64_t z0 = 0; z0 < nbatch; z0 += 65535) {
         const int64_t nz = std::min<int64_t>(65535, nbatch - z0);
         dim3 g((unsigned)cdiv(N, MT), (unsigned)cdiv(M, MT), (unsigned)nz);
@@ -1762,7 +1770,7 @@ static void cgemm(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, in
         k_cgemm<<<g, 256, 0, ctx->stream>>>(A + z0 * bA, bA, ai, ak, B + z0 * bB, bB, bk, bj, conjB,
                                             fs ? fs + z0 * bfs : nullptr, bfs, C + z0 * bC, bC, ci,
                                             cj, M, N, Kfix, kvar ? kvar + z0 : nullptr,
-                                            rank_sel_at(rs, z0));
+                                            rank_sel_at(rs, z0), accum);
         HIP_LAUNCH_CHECK();
     }
 }
@@ -1846,7 +1854,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
             const double *bp = beta_dev + (per_patch ? ip0 : 0);
             MSLR_PROF(ctx, "eigvec");
             k_eig_count<<<(unsigned)cdiv(nb, 64), 64, 0, ctx->stream>>>(w.d, w.e, (int)m, nb, bp,
-                                                                         per_patch, w.info, w.r);
+                                                                         per_patch, w.info, w.r, 0);
             HIP_LAUNCH_CHECK();
             // lanes per eigenvalue: many when the eigenvalues are few (the bisection
             // latency is the cost), few when the batch already fills the device
@@ -1923,6 +1931,137 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
     }
 }
 
+// ------------------------------------------------------------------ derivative
+// Vector-Jacobian product of Y = SVST(Z, beta) = Z g(G), G = Z^H Z = V diag(l) V^H,
+// g(l) = max(1 - beta / sqrt(l), 0), for p >= Nt.  With Wb = Z^H Ybar,
+// S = (Wb + Wb^H) / 2, M = V^H S V and the divided differences D of g:
+//   Zbar = Ybar W + 2 Z V (D o M) V^H,   betabar = -sum_{s_k > beta} M_kk / s_k.
+
+// S = (Wb + Wb^H) / 2 per m x m matrix.
+__global__ __launch_bounds__(256) void k_vjp_sym(const c64 *__restrict__ Wb, c64 *__restrict__ S,
+                                                 int m) {
+    const int64_t mm = (int64_t)m * m;
+    const c64 *w = Wb + (int64_t)blockIdx.y * mm;
+    c64 *o = S + (int64_t)blockIdx.y * mm;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < mm;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % m), j = (int)(t / m);
+        const c64 a = w[t], b = w[(int64_t)i * m + j];
+        o[t] = c64{0.5f * (a.re + b.re), 0.5f * (a.im - b.im)};
+    }
+}
+
+// M <- 2 D o M (in place) and betabar of each patch; one workgroup per patch.
+// lam: the patch's Gram eigenvalues (V's column order), beta cast to Float32 as in
+// the forward.  D in forms without the eigenvalue difference in a denominator:
+// both above beta^2: beta / (s_i s_j (s_i + s_j)); one above: g(l_i) / (l_i - l_j),
+// where l_i - l_j >= (s_i - beta)(s_i + beta) > 0; both below: 0.
+__global__ __launch_bounds__(256) void k_vjp_divdiff(c64 *__restrict__ Mall,
+                                                     const double *__restrict__ lamall, int m,
+                                                     const double *__restrict__ beta,
+                                                     int beta_per_patch,
+                                                     double *__restrict__ betabar) {
+    __shared__ double red[4];
+    const int ip = blockIdx.x;
+    c64 *M = Mall + (int64_t)ip * m * m;
+    const double *lam = lamall + (int64_t)ip * m;
+    const double bd = (double)(float)beta[beta_per_patch ? ip : 0];
+    double bb = 0.0;
+    for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+        const int i = t % m, j = t / m;
+        const double li = fmax(lam[i], 0.0), lj = fmax(lam[j], 0.0);
+        const double si = sqrt(li), sj = sqrt(lj);
+        const bool ui = si > bd, uj = sj > bd;
+        double dd = 0.0;
+        if (ui && uj) dd = bd / (si * sj * (si + sj));
+        else if (ui) dd = (1.0 - bd / si) / (li - lj);
+        else if (uj) dd = (1.0 - bd / sj) / (lj - li);
+        const c64 v = M[t];
+        if (i == j && ui) bb -= (double)v.re / si;
+        const float f = (float)(2.0 * dd);
+        M[t] = c64{v.re * f, v.im * f};
+    }
+    bb = block_sum(bb, red);
+    if (threadIdx.x == 0) betabar[ip] = bb;
+}
+
+void svst_vjp_batched(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int64_t p, int64_t Nt,
+                      int64_t Np, const double *beta_dev, int per_patch, double *betabar_dev,
+                      int *err_dev) {
+    if (Np == 0) return;
+    MSLR_REQUIRE(p >= Nt && Nt >= 2, MSLR_ERR_UNSUPPORTED,
+                 "the SVST derivative needs patch voxels >= Nt >= 2");
+    const int64_t m = Nt;
+    MSLR_REQUIRE(m <= MAXM, MSLR_ERR_UNSUPPORTED, "Nt must be <= " + std::to_string(MAXM));
+    const int64_t len = p * Nt, mm = m * m;
+    const int mi = (int)m;
+    const SvstWork w = svst_work(ctx, m, p, Np);
+    c64 *Wm = (c64 *)w.part;
+    c64 *buf = (c64 *)ctx->scratch_get(SLOT_VJP, (size_t)(5 * mm * w.chunk) * sizeof(c64));
+    c64 *Wb = buf, *Sx = buf + mm * w.chunk, *T1 = buf + 2 * mm * w.chunk;
+    c64 *M1 = buf + 3 * mm * w.chunk, *T2 = buf + 4 * mm * w.chunk;
+    c64 *Gb = Wb;  // Wb is dead once S is formed
+    for (int64_t ip0 = 0; ip0 < Np; ip0 += w.chunk) {
+        const int64_t nb = std::min<int64_t>(w.chunk, Np - ip0);
+        const c64 *Zc = Z + ip0 * len, *Ybc = Yb + ip0 * len;
+        c64 *Zbc = Zb + ip0 * len;
+        const double *bp = beta_dev + (per_patch ? ip0 : 0);
+        // every eigenpair of G
+        gram_and_tridiag<double>(ctx, w, Zc, p, Nt, nb, err_dev);
+        {
+            MSLR_PROF(ctx, "vjp_eig");
+            k_eig_count<<<(unsigned)cdiv(nb, 64), 64, 0, ctx->stream>>>(w.d, w.e, mi, nb, bp,
+                                                                         per_patch, w.info, w.r, 1);
+            HIP_LAUNCH_CHECK();
+            dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
+            k_eig_vectors<4><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, mi, bp, per_patch, w.info, w.r,
+                                                         w.lam, w.Z, w.U);
+            HIP_LAUNCH_CHECK();
+            k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, mi, bp, per_patch,
+                                                                    w.info, w.r, w.lam, w.Z, w.ws,
+                                                                    w.f);
+            HIP_LAUNCH_CHECK();
+        }
+        if (w.sbr)
+            sbr_backtransform(ctx, w.G, mi, nb, w.sw, w.Z, w.r, w.U);
+        else
+            launch_backtransform(ctx, w, mi, nb);
+        // W = V diag(g) V^H
+        cgemm(ctx, "vjp_w", w.U, mm, 1, m, w.U, mm, m, 1, 1, w.f, m, Wm, mm, 1, m, mi, mi, 0, w.r,
+              nb);
+        // Wb = Z^H Ybar, formed as its transpose Ybar^T conj(Z) written column-major
+        cgemm(ctx, "vjp_zy", Ybc, len, p, 1, Zc, len, 1, p, 1, nullptr, 0, Wb, mm, m, 1, mi, mi,
+              (int)p, nullptr, nb);
+        {
+            MSLR_PROF(ctx, "vjp_sym");
+            dim3 gs((unsigned)cdiv(mm, 256), (unsigned)nb);
+            k_vjp_sym<<<gs, 256, 0, ctx->stream>>>(Wb, Sx, mi);
+            HIP_LAUNCH_CHECK();
+        }
+        // M = V^H S V (V^H T1 as the transpose T1^T conj(V))
+        cgemm(ctx, "vjp_rot", Sx, mm, 1, m, w.U, mm, 1, m, 0, nullptr, 0, T1, mm, 1, m, mi, mi, mi,
+              nullptr, nb);
+        cgemm(ctx, "vjp_rot", T1, mm, m, 1, w.U, mm, 1, m, 1, nullptr, 0, M1, mm, m, 1, mi, mi, mi,
+              nullptr, nb);
+        {
+            MSLR_PROF(ctx, "vjp_divdiff");
+            k_vjp_divdiff<<<(unsigned)nb, 256, 0, ctx->stream>>>(M1, w.lam, mi, bp, per_patch,
+                                                                 betabar_dev + ip0);
+            HIP_LAUNCH_CHECK();
+        }
+        // 2 Gbar = V (2 D o M) V^H
+        cgemm(ctx, "vjp_rot", M1, mm, 1, m, w.U, mm, m, 1, 1, nullptr, 0, T2, mm, 1, m, mi, mi, mi,
+              nullptr, nb);
+        cgemm(ctx, "vjp_rot", w.U, mm, 1, m, T2, mm, 1, m, 0, nullptr, 0, Gb, mm, 1, m, mi, mi, mi,
+              nullptr, nb);
+        // Zbar = Ybar W + Z (2 Gbar)
+        cgemm(ctx, "vjp_out", Ybc, len, 1, p, Wm, mm, 1, m, 0, nullptr, 0, Zbc, len, 1, p, (int)p,
+              mi, mi, nullptr, nb);
+        cgemm(ctx, "vjp_out", Zc, len, 1, p, Gb, mm, 1, m, 0, nullptr, 0, Zbc, len, 1, p, (int)p, mi,
+              mi, nullptr, nb, RankSel{}, 1);
+    }
+}
+
 void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,
                      double *out_dev) {
     if (Np == 0) {
@@ -1978,6 +2117,41 @@ mslr_status_t mslr_svst_batched(mslr_ctx ctx, void *P_dev, int64_t p, int64_t Nt
         HIP_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
         MSLR_REQUIRE(herr == 0, MSLR_ERR_NUMERIC, "non-finite values in a patch matrix");
+    });
+}
+
+mslr_status_t mslr_svst_vjp(mslr_ctx ctx, const void *Z_dev, const void *Ybar_dev, void *Zbar_dev,
+                            int64_t p, int64_t Nt, int64_t Np, const double *beta_host,
+                            int per_patch, double *betabar_host) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(Z_dev && Ybar_dev && Zbar_dev && beta_host && betabar_host, MSLR_ERR_ARG,
+                     "null argument");
+        MSLR_REQUIRE(p >= 0 && Nt >= 0 && Np >= 0, MSLR_ERR_ARG, "negative size");
+        MSLR_REQUIRE(Zbar_dev != Z_dev && Zbar_dev != Ybar_dev, MSLR_ERR_ARG,
+                     "Zbar must not alias the inputs");
+        const int64_t nbeta = per_patch ? Np : 1;
+        for (int64_t i = 0; i < nbeta; ++i)
+            MSLR_REQUIRE(beta_host[i] >= 0.0 && std::isfinite(beta_host[i]), MSLR_ERR_ARG,
+                         "beta must be finite and >= 0");
+        for (int64_t i = 0; i < nbeta; ++i) betabar_host[i] = 0.0;
+        if (Np == 0 || p == 0 || Nt == 0) return;
+        double *bd = (double *)c->scratch_get(SLOT_APPLY, (size_t)(nbeta + Np + 2) * sizeof(double) + 64);
+        double *bb = bd + nbeta;
+        int *err = (int *)(bb + Np + 1);
+        HIP_CHECK(hipMemcpyAsync(bd, beta_host, (size_t)nbeta * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+        svst_vjp_batched(c, (const c64 *)Z_dev, (const c64 *)Ybar_dev, (c64 *)Zbar_dev, p, Nt, Np,
+                         bd, per_patch, bb, err);
+        std::vector<double> h((size_t)Np);
+        int herr = 0;
+        HIP_CHECK(hipMemcpyAsync(h.data(), bb, (size_t)Np * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIP_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        MSLR_REQUIRE(herr == 0, MSLR_ERR_NUMERIC, "non-finite values in a patch matrix");
+        for (int64_t i = 0; i < Np; ++i) betabar_host[per_patch ? i : 0] += h[(size_t)i];
     });
 }
 

@@ -14,6 +14,13 @@ namespace mslr {
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
                   const double *beta_dev, int per_patch, int *err_dev, double *nuc_out = nullptr);
 
+// Vector-Jacobian product of the batched SVST (p >= Nt >= 2): Zbar (distinct from
+// the inputs) <- the cotangent of Z, betabar_dev[i] <- d<Ybar, Y>/d beta_i per
+// matrix (device, Np doubles).  Takes every eigenpair of each Gram matrix.
+void svst_vjp_batched(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int64_t p, int64_t Nt,
+                      int64_t Np, const double *beta_dev, int per_patch, double *betabar_dev,
+                      int *err_dev);
+
 // Sum of singular values of each matrix, accumulated into *out_dev (device double).
 void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,
                      double *out_dev);

@@ -77,6 +77,7 @@ class mslr_pogm_desc(C.Structure):
         ("bsig", C.c_double),
         ("form", C.c_int),
         ("log_costs", C.c_int),
+        ("checkpoint", C.c_int),
     ]
 
 
@@ -117,6 +118,7 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_patches2img": (C.c_int, [_p, _p, _pi64, _i64, _pi64, _pi64, _p]),
     "mslr_svst_batched": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, C.c_int]),
     "mslr_patch_nucnorm": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd]),
+    "mslr_svst_vjp": (C.c_int, [_p, _p, _p, _p, _i64, _i64, _i64, _pd, C.c_int, _pd]),
     "mslr_patch_svst": (C.c_int, [_p, _p, _p, _pi64, _i64, _pi64, _pi64, C.c_double]),
     "mslr_pogm_create": (C.c_int, [_p, C.POINTER(mslr_pogm_desc), C.POINTER(_p)]),
     "mslr_pogm_destroy": (C.c_int, [_p]),
@@ -126,6 +128,7 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_pogm_log": (C.c_int, [_p, _pi, _pi]),
     "mslr_pogm_thresholds": (C.c_int, [_p, _pd]),
     "mslr_pogm_costs": (C.c_int, [_p, _pi, _pd, _pd]),
+    "mslr_pogm_lambda_vjp": (C.c_int, [_p, _p, _pd]),
 }
 
 _lib: C.CDLL | None = None

@@ -32,7 +32,8 @@ class POGM:
 
     def __init__(self, A: SenseOp, *, patch_sizes, strides, lambdas, L: float, niter: int,
                  mom="pogm", restart="gr", restart_cutoff: float = 0.0, bsig: float = 1.0,
-                 f_mu: float = 0.0, form="plain", log_costs: bool = False):
+                 f_mu: float = 0.0, form="plain", log_costs: bool = False,
+                 checkpoint: bool = False):
         mom, restart = _sym(mom), _sym(restart)
         if mom not in _MOM:
             raise ValueError(f"mom={mom}")
@@ -77,6 +78,7 @@ class POGM:
         d.bsig = float(bsig)
         d.form = _FORM[form] if isinstance(form, str) else int(form)
         d.log_costs = 1 if log_costs else 0
+        d.checkpoint = 1 if checkpoint else 0
         h = C.c_void_p()
         _lib.check(self.ctx._lib.mslr_pogm_create(A.handle, C.byref(d), C.byref(h)))
         self.handle = h
@@ -121,6 +123,17 @@ class POGM:
         flags = [False] + rs
         return [(dc[i], reg[i], flags[i]) for i in range(len(dc))]
 
+    def lambda_vjp(self, xbar):
+        """d<xbar, sum_s X_s>/d lambda_s through the iterations run so far (the pullback
+        of the unrolled POGM with respect to the per-scale weights); needs checkpoint=True.
+        xbar: cotangent of the final image sum, (Nx, Ny, Nz, Nt)."""
+        xb = as_device(xbar, np.complex64, self.ctx)
+        if tuple(xb.shape) != (*self.A.N, self.A.Nt):
+            raise AssertionError("xbar must be (Nx, Ny, Nz, Nt)")
+        out = (C.c_double * self.S)()
+        _lib.check(self.ctx._lib.mslr_pogm_lambda_vjp(self.handle, xb.ptr, out))
+        return [out[i] for i in range(self.S)]
+
     def thresholds(self):
         b = (C.c_double * self.S)()
         _lib.check(self.ctx._lib.mslr_pogm_thresholds(self.handle, b))
@@ -147,6 +160,22 @@ def pogm_mslr(A: SenseOp, y, *, patch_sizes, strides, lambdas, L: float, niter: 
     st.run(niter)
     _, restarts = st.log()
     return st.X, restarts, st
+
+
+def pogm_lambda_grad(A: SenseOp, y, x_ref, *, patch_sizes, strides, lambdas, L: float, niter: int,
+                     form="plain"):
+    """The differentiation form: loss = mean |sum_s X_s - x_ref|^2 after `niter` unrolled
+    POGM iterations and d loss / d lambda_s (per-iteration checkpoints, backward pass on
+    the device, restart decisions held fixed).  Returns (loss, grads, X DeviceArray, state)."""
+    st = POGM(A, patch_sizes=patch_sizes, strides=strides, lambdas=lambdas, L=L, niter=niter,
+              form=form, checkpoint=True)
+    st.init(y)
+    st.run(niter)
+    xs = st.X.to_host().sum(axis=4)
+    r = xs.astype(np.complex128) - np.asarray(x_ref, dtype=np.complex128)
+    loss = float(np.vdot(r.ravel(), r.ravel()).real) / r.size
+    xbar = np.asfortranarray((2.0 / r.size) * r).astype(np.complex64)
+    return loss, st.lambda_vjp(xbar), st.X, st
 
 
 def poweriter_mod(A: SenseOp, *, niter: int = 200, tol: float = 1e-6, x0=None, chat: bool = True):

@@ -78,6 +78,30 @@ def svst_batched(P, beta, ctx: Context | None = None, inplace: bool = False) -> 
     return d
 
 
+def svst_vjp(Z, Ybar, beta, ctx: Context | None = None):
+    """Pullback of svst_batched: (Zbar DeviceArray, betabar) for Y = SVST(Z, beta) and the
+    cotangent Ybar (same shape as Z: (p, Nt) or (p, Nt, Np), p >= Nt).  betabar has one
+    value per patch when beta does, else it is the scalar d<Ybar, Y>/d beta."""
+    ctx = ctx or default_context()
+    z = as_device(Z, np.complex64, ctx)
+    yb = as_device(Ybar, np.complex64, ctx)
+    if z.shape != yb.shape:
+        raise AssertionError("Z and Ybar differ in shape")
+    p, Nt = z.shape[0], z.shape[1]
+    Np = 1 if len(z.shape) == 2 else z.shape[2]
+    b = np.atleast_1d(np.asarray(beta, dtype=np.float64))
+    per = int(b.size > 1 or (np.ndim(beta) > 0 and Np > 1))
+    if per and b.size != Np:
+        raise AssertionError("need one threshold per patch")
+    b = np.ascontiguousarray(b)
+    out = DeviceArray(z.shape, np.complex64, ctx)
+    bb = np.zeros(Np if per else 1)
+    _lib.check(ctx._lib.mslr_svst_vjp(ctx.handle, z.ptr, yb.ptr, out.ptr, p, Nt, Np,
+                                      b.ctypes.data_as(C.POINTER(C.c_double)), per,
+                                      bb.ctypes.data_as(C.POINTER(C.c_double))))
+    return out, (bb if per else float(bb[0]))
+
+
 def SVST(X, beta, ctx: Context | None = None) -> DeviceArray:
     """Singular value soft-thresholding of one matrix X (p x Nt)."""
     d = as_device(X, np.complex64, ctx)

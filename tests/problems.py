@@ -15,13 +15,23 @@ SMALL = {
 }
 
 
-@functools.lru_cache(maxsize=8)
-def raw(name: str = "small", seed: int = 7):
-    """(cfg, ksp0 (Nx,Ny,Nz,Nc,Nt) zero-filled, raw smaps): the run_recon inputs."""
+@functools.lru_cache(maxsize=4)
+def _problem(name: str, seed: int):
     cfg = synth.small_config(**SMALL[name]) if name in SMALL else synth.CONFIGS[name]
     if name not in SMALL:
         seed = synth.SEEDS[name]
-    pb = synth.generate(cfg, seed=seed)
+    return cfg, synth.generate(cfg, seed=seed), seed
+
+
+def truth(name: str = "small", seed: int = 7):
+    """The synthetic image (Nx,Ny,Nz,Nt) the k-space was made from."""
+    return _problem(name, seed)[1].x_true
+
+
+@functools.lru_cache(maxsize=8)
+def raw(name: str = "small", seed: int = 7):
+    """(cfg, ksp0 (Nx,Ny,Nz,Nc,Nt) zero-filled, raw smaps): the run_recon inputs."""
+    cfg, pb, seed = _problem(name, seed)
     N, Nt, Nc = tuple(cfg["N"]), cfg["Nt"], cfg["Nc"]
     y = O.A_forward(pb.x_true, pb.mask, pb.smaps)
     y = synth.add_noise(y, seed)
