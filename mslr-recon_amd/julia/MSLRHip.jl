@@ -13,7 +13,7 @@ module MSLRHip
 
 export HipContext, HipArray, Asense_hip, img2patches_hip, patches2img_hip, SVST_hip,
        patchSVST_hip, patch_nucnorm_hip, pogm_hip, poweriter_hip, upload, download,
-       set_products!
+       set_products!, SVST_vjp_hip, pogm_checkpointed_hip, lambda_vjp_hip, pogm_lambda_grad_hip
 
 const LIB = get(ENV, "MSLR_HIP_LIB", joinpath(@__DIR__, "..", "mslr_hip", "libmslr_hip.so"))
 
@@ -223,6 +223,27 @@ function patchSVST_hip(img::HipArray{ComplexF32,4}, β, patch_size, stride_size)
     return out
 end
 
+"""
+    SVST_vjp_hip(Z, Ybar, β) -> (Zbar, βbar)
+
+Pullback of `Y = SVST(Z, β)` (src/recon.jl:183-205) for device matrices Z, Ybar of
+size (p, Nt) or (p, Nt, Np) with p ≥ Nt: Zbar and βbar = ∂⟨Ybar, Y⟩/∂β (one per
+patch when β is a vector).  The rrule of SVST calls it.
+"""
+function SVST_vjp_hip(Z::HipArray{ComplexF32}, Ybar::HipArray{ComplexF32}, β)
+    Z.dims == Ybar.dims || throw(AssertionError("Z and Ybar differ in shape"))
+    p, Nt = Z.dims[1], Z.dims[2]
+    Np = length(Z.dims) == 3 ? Z.dims[3] : 1
+    per = β isa AbstractVector
+    b = per ? Cdouble.(β) : Cdouble[β]
+    bb = zeros(Cdouble, per ? Np : 1)
+    Zbar = HipArray{ComplexF32}(Z.dims; ctx = Z.ctx)
+    check(ccall((:mslr_svst_vjp, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cdouble}, Cint,
+                 Ptr{Cdouble}), Z.ctx.h, Z.ptr, Ybar.ptr, Zbar.ptr, p, Nt, Np, b, per, bb))
+    return Zbar, per ? bb : bb[1]
+end
+
 function patch_nucnorm_hip(P::HipArray{ComplexF32,3})
     r = Ref{Cdouble}(0)
     check(ccall((:mslr_patch_nucnorm, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Ref{Cdouble}),
@@ -246,6 +267,7 @@ struct PogmDesc
     bsig::Cdouble
     form::Cint
     log_costs::Cint
+    checkpoint::Cint
 end
 
 """
@@ -269,7 +291,7 @@ function pogm_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES
     st = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve ps ss λs begin
         d = PogmDesc(Int64.(A.N), A.Nt, S, pointer(ps), pointer(ss), pointer(λs), L, niter, 0, 0, 0.0, 1.0, form,
-                     log_costs ? 1 : 0)
+                     log_costs ? 1 : 0, 0)
         check(ccall((:mslr_pogm_create, LIB), Cint, (Ptr{Cvoid}, Ref{PogmDesc}, Ref{Ptr{Cvoid}}), A.h, d, st))
     end
     try
@@ -291,6 +313,70 @@ function pogm_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES
     finally
         ccall((:mslr_pogm_destroy, LIB), Cint, (Ptr{Cvoid},), st[])
     end
+end
+
+"A POGM state kept on the device with its per-iteration checkpoints (the derivative form)."
+mutable struct PogmCheckpointed
+    h::Ptr{Cvoid}
+    A::Asense_hip
+    S::Int
+end
+
+"""
+    pogm_checkpointed_hip(A, ksp, L, λs, PATCH_SIZES, STRIDES; niter, form=0) -> (X_recon, state)
+
+Runs `niter` POGM iterations as `pogm_hip` does, keeping each iteration's prox
+input on the device; `lambda_vjp_hip(state, xbar)` then pulls a cotangent of the
+image sum X_recon = Σ_s X_s back to the per-scale weights.
+"""
+function pogm_checkpointed_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES, STRIDES;
+                               niter::Int, form::Integer = 0)
+    S = length(PATCH_SIZES)
+    ps = Int64[p[a] for p in PATCH_SIZES for a in 1:3]
+    ss = Int64[s[a] for s in STRIDES for a in 1:3]
+    yd = ksp isa HipArray ? ksp : upload(ComplexF32.(ksp); ctx = A.ctx)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve ps ss λs begin
+        d = PogmDesc(Int64.(A.N), A.Nt, S, pointer(ps), pointer(ss), pointer(λs), L, niter, 0, 0, 0.0, 1.0, form,
+                     0, 1)
+        check(ccall((:mslr_pogm_create, LIB), Cint, (Ptr{Cvoid}, Ref{PogmDesc}, Ref{Ptr{Cvoid}}), A.h, d, r))
+    end
+    st = PogmCheckpointed(r[], A, S)
+    finalizer(st) do x
+        x.h != C_NULL && ccall((:mslr_pogm_destroy, LIB), Cint, (Ptr{Cvoid},), x.h)
+        x.h = C_NULL
+    end
+    check(ccall((:mslr_pogm_init, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), st.h, yd.ptr, C_NULL))
+    check(ccall((:mslr_pogm_run, LIB), Cint, (Ptr{Cvoid}, Cint), st.h, niter))
+    xp = ccall((:mslr_pogm_x, LIB), Ptr{Cvoid}, (Ptr{Cvoid},), st.h)
+    X = Array{ComplexF32}(undef, A.N..., A.Nt, S)
+    check(ccall((:mslr_download, LIB), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{Cvoid}, Csize_t),
+                A.ctx.h, X, xp, sizeof(X)))
+    return dropdims(sum(X; dims = 5); dims = 5), st
+end
+
+"λbar[k] = ∂⟨xbar, Σ_s X_s⟩/∂λ_k through the checkpointed iterations (restarts held fixed)."
+function lambda_vjp_hip(st::PogmCheckpointed, xbar)
+    xb = xbar isa HipArray ? xbar : upload(ComplexF32.(xbar); ctx = st.A.ctx)
+    λbar = zeros(Cdouble, st.S)
+    check(ccall((:mslr_pogm_lambda_vjp, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cdouble}),
+                st.h, xb.ptr, λbar))
+    return λbar
+end
+
+"""
+    pogm_lambda_grad_hip(A, ksp, L, λs, PATCH_SIZES, STRIDES, x_ref; niter, form=0)
+        -> (loss, ∇λ)
+
+MSE of the image sum after `niter` unrolled POGM iterations to `x_ref` and its
+gradient with respect to the per-scale weights.
+"""
+function pogm_lambda_grad_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES, STRIDES,
+                              x_ref; niter::Int, form::Integer = 0)
+    Xr, st = pogm_checkpointed_hip(A, ksp, L, λs, PATCH_SIZES, STRIDES; niter = niter, form = form)
+    r = ComplexF64.(Xr) .- ComplexF64.(x_ref)
+    loss = sum(abs2, r) / length(r)
+    return loss, lambda_vjp_hip(st, (2 / length(r)) .* r)
 end
 
 end # module MSLRHip
