@@ -202,6 +202,57 @@ def cpu_baseline(cfg, pb, y_norm, smaps, part_s=4.0):
 
 
 # --------------------------------------------------------------------------- counter calibration
+def differentiation_run(ctx, niter: int = 10, name: str = "c4") -> dict:
+    """The differentiation workload (64x64x40, Nt=120, 12 coils, scales
+    [[64,64,40],[8,8,8]]): MSE of the image sum after `niter` unrolled POGM
+    iterations to the synthetic truth and its gradient with respect to the per-scale
+    lambda_k (checkpointed forward, backward pass on the device); timed separately
+    from the headline loop."""
+    from mslr_hip import synth
+    from mslr_hip.device import DeviceArray
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.reconstruct import lambdas_ong_lustig, normalise_smaps, prepare_flat
+    from mslr_hip.sense import SenseOp
+    cfg = synth.CONFIGS[name]
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    S = len(cfg["patch_sizes"])
+    pb = synth.generate(name)
+    smaps = normalise_smaps(pb.smaps)
+    A0 = SenseOp(pb.mask, smaps, ctx=ctx)
+    y = synth.add_noise(A0.forward(DeviceArray.from_host(pb.x_true, ctx)).to_host(), synth.SEEDS[name])
+    A0.close()
+    A, _, yd, sf, _ = prepare_flat(pb.smaps, pb.mask, y, ctx)
+    lams = lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    x_ref = (pb.x_true / np.float32(sf)).astype(np.complex64)
+    xr_d = DeviceArray.from_host(x_ref, ctx)
+    times = []
+    for rep in range(2):  # the first run sizes the scratch
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=S * 1.0,
+                  niter=niter, checkpoint=True)
+        st.init(yd)
+        st.run(niter)
+        ctx.sync()
+        t1 = time.perf_counter()
+        xs = st.X.to_host().sum(axis=4)
+        r = xs.astype(np.complex128) - x_ref.astype(np.complex128)
+        loss = float(np.vdot(r.ravel(), r.ravel()).real) / r.size
+        xr_d.upload(np.asfortranarray((2.0 / r.size) * r).astype(np.complex64))
+        t2 = time.perf_counter()
+        grads = st.lambda_vjp(xr_d)
+        ctx.sync()
+        t3 = time.perf_counter()
+        _, restarts = st.log()
+        st.close()
+        times.append((t1 - t0, t3 - t2))
+    fwd, bwd = times[-1]
+    return {"workload": f"{name}: {N[0]}x{N[1]}x{N[2]} Nt={Nt} {cfg['Nc']} coils scales={cfg['patch_sizes']}",
+            "niter": niter, "loss": loss, "dloss_dlambda": grads, "restarts": restarts,
+            "forward_checkpointed_s": round(fwd, 4), "backward_s": round(bwd, 4),
+            "unrolled_iter_per_s_fwd_bwd": round(niter / (fwd + bwd), 3)}
+
+
 def copy_calibration(mib: int) -> int:
     """A plain elementwise copy (dst = src * 1) of a known size: each launch reads and
     writes exactly mib MiB, so its FETCH_SIZE / WRITE_SIZE give the counters' scale."""
@@ -229,6 +280,8 @@ def main() -> int:
     ap.add_argument("--no-prof", action="store_true", help="skip the pass with per-kernel event timing")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the batched FP32 and plain runs (rates and result differences)")
+    ap.add_argument("--no-grad", action="store_true",
+                    help="skip the differentiation workload (c4 lambda-gradient, timed apart)")
     ap.add_argument("--logger", default="on", choices=["on", "off"],
                     help="keep run_recon's logger (dc_cost, reg_cost per iterate) on the device "
                          "inside the timed loop, as the reference's loop calls it")
@@ -383,6 +436,13 @@ def main() -> int:
             roof.update({"avg_launch_ms": round(per_launch_ms, 4), "launches_per_iter": cnt / K,
                          "share_of_kernel_time": round(ms / total_ms, 3)})
 
+    grad = None
+    if not args.no_grad:
+        t = time.time()
+        grad = differentiation_run(ctx)
+        log(f"differentiation (c4, {grad['niter']} iterations): forward {grad['forward_checkpointed_s']:.3f}s "
+            f"backward {grad['backward_s']:.3f}s dloss/dlambda {grad['dloss_dlambda']} ({time.time() - t:.1f}s)")
+
     cpu = None
     if not args.no_cpu_baseline:
         t = time.time()
@@ -416,6 +476,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "kernels_ms_per_iter": {k: round(v[1] / K, 3) for k, v in kern.items()},
         "costs": costs,
+        "differentiation": grad,
     }
     print(json.dumps(line), flush=True)
     return 0
