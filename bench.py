@@ -31,21 +31,21 @@ HBM_PEAK_GBS = 8000.0
 # families, from the committed rocprofv3 counter summary of this workload (mixed
 # form; the SENSE, update and patch kernels are the same in every form)
 TRAFFIC_BYTES = {
-    "fwd_yz": 42422000000,
-    "adj_yz": 24838500000,
+    "fwd_yz": 6994400000,
+    "adj_yz": 4406400000,
     "pogm_update": 15163600000,
-    "fwd_x": 8843400000,
-    "adj_x": 6046700000,
-    "sum_planes": 4199100000,
-    "sy2sb": 3757800000,
-    "img2patches": 3675500000,
-    "prox_voxel": 3499200000,
-    "patches2img": 3150000000,
-    "apply": 2522200000,
+    "fwd_x": 8933700000,
+    "adj_x": 6115300000,
+    "sum_planes": 4276900000,
+    "sy2sb": 3764600000,
+    "img2patches": 5286600000,
+    "prox_voxel": 3499400000,
+    "patches2img": 3149800000,
+    "apply": 908900000,
     "planes_to_image": 2479700000,
     "split": 2342200000,
-    "gram_tiles": 1865300000,
-    "backtransform": 946200000,
+    "gram_tiles": 1864500000,
+    "backtransform": 946800000,
     "sb2st": 224800000,
 }
 FP32_PEAK_TFLOPS = 157.3
