@@ -727,12 +727,12 @@ __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ d
                                                     const int *__restrict__ rall,
                                                     double *__restrict__ lamall,
                                                     double *__restrict__ Zall,
-                                                    c64 *__restrict__ Uall) {
+                                                    c64 *__restrict__ Uall, int k0) {
     static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "group size");
     const int ip = blockIdx.x;
     const int r = rall[ip];
     const int t = blockIdx.y * 64 + threadIdx.x;
-    const int k = t / G, j = t % G;
+    const int k = k0 + t / G, j = t % G;
     if (k >= r) return;  // whole groups leave together
     const int gbase = (threadIdx.x & 63) - j;
     const double *d = dall + (int64_t)ip * m;
@@ -1857,17 +1857,24 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
                                                                          per_patch, w.info, w.r, 0);
             HIP_LAUNCH_CHECK();
             // lanes per eigenvalue: many when the eigenvalues are few (the bisection
-            // latency is the cost), few when the batch already fills the device
-            if (nb * m <= 64 * 1024) {
-                dim3 ge((unsigned)nb, (unsigned)cdiv(m * 16, 64));
+            // latency is the cost), few when they fill the device.  The number kept
+            // per patch is known only on the device, so a large batch takes its
+            // first EIG_FEW eigenvalues with 16 lanes each and the rest with 4
+            // (workgroups past a patch's r leave at once).
+            constexpr int EIG_FEW = 32;
+            const int mfew = (int)std::min<int64_t>(m, nb * m <= 64 * 1024 ? m : EIG_FEW);
+            {
+                dim3 ge((unsigned)nb, (unsigned)cdiv(mfew * 16, 64));
                 k_eig_vectors<16><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
-                                                              w.info, w.r, w.lam, w.Z, w.U);
-            } else {
-                dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
-                k_eig_vectors<4><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
-                                                             w.info, w.r, w.lam, w.Z, w.U);
+                                                              w.info, w.r, w.lam, w.Z, w.U, 0);
+                HIP_LAUNCH_CHECK();
             }
-            HIP_LAUNCH_CHECK();
+            if (mfew < m) {
+                dim3 ge((unsigned)nb, (unsigned)cdiv((m - mfew) * 4, 64));
+                k_eig_vectors<4><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
+                                                             w.info, w.r, w.lam, w.Z, w.U, mfew);
+                HIP_LAUNCH_CHECK();
+            }
             k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
                                                                     w.info, w.r, w.lam, w.Z, w.ws,
                                                                     w.f);
@@ -2015,7 +2022,7 @@ void svst_vjp_batched(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int
             HIP_LAUNCH_CHECK();
             dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
             k_eig_vectors<4><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, mi, bp, per_patch, w.info, w.r,
-                                                         w.lam, w.Z, w.U);
+                                                         w.lam, w.Z, w.U, 0);
             HIP_LAUNCH_CHECK();
             k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, mi, bp, per_patch,
                                                                     w.info, w.r, w.lam, w.Z, w.ws,

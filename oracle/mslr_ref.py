@@ -533,7 +533,7 @@ def svst_jvp(Z, beta, Zdot, bdot=0.0):
 
 
 def pogm_lambda_jvp(ksp, omega, smaps, x_ref, *, N, Nt, PATCH_SIZES, STRIDES, NITERS, lambdas=None,
-                    sigma1A=1.0):
+                    sigma1A=1.0, progress=None):
     """Loss = mean |sum_s X_s - x_ref|^2 after NITERS iterations of the aliased POGM
     (the iteration of run_recon_arrays in mode f64: g = A'(A sum X - y), u = x - alpha g,
     z = u + beta (u - u_old) + gamma (u - x), x_s = patchSVST(z_s, zeta lambda_s);
@@ -596,6 +596,8 @@ def pogm_lambda_jvp(ksp, omega, smaps, x_ref, *, N, Nt, PATCH_SIZES, STRIDES, NI
                 Xnd[j][..., s] = xds[j]
         X, Xd, U, Ud = Xn, Xnd, Un, Und
         told = tnew
+        if progress:
+            progress(it)
     r = X.sum(axis=4) - x_ref
     n = r.size
     loss = float(np.vdot(r.ravel(), r.ravel()).real) / n

@@ -26,7 +26,9 @@ def main(name: str, niter: int) -> None:
     t = time.time()
     loss, grads, rs, _ = O.pogm_lambda_jvp(ksp, omega, smaps, x_ref, N=tuple(cfg["N"]), Nt=cfg["Nt"],
                                            PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"],
-                                           NITERS=niter)
+                                           NITERS=niter,
+                                           progress=lambda it: print(f"iteration {it}: {time.time() - t:.0f} s",
+                                                                     flush=True))
     out = {"config": name, "niter": niter, "sigma1A": 1.0, "loss": loss, "grads": grads,
            "restarts": rs, "scale_factor": sf,
            "lambdas": O.lambdas_ong_lustig(tuple(cfg["N"]), cfg["Nt"], cfg["patch_sizes"]),
