@@ -93,18 +93,22 @@ __device__ __forceinline__ c128 ld_if(bool c, const c128 *p) {
 
 
 // ------------------------------------------------------------------ stage 1
-// One workgroup per matrix.  LDS: V and W, (m - B) x B each, column-major.
-__global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, int npanel,
-                                               c128 *__restrict__ Tall) {
+// One workgroup per matrix, two per CU (a batch of a few hundred matrices runs in
+// one round): V ((m - B) x B, column-major), T and one B x B block (S = V^H V, then
+// X = V^H W turned into M = T^H X in place) in LDS, at most 80 KB; Y and W in global
+// memory (L2); at most 128 VGPRs (four waves per SIMD).
+__global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m, int npanel,
+                                                  c128 *__restrict__ Tall,
+                                                  c128 *__restrict__ W1all) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ c128 Ts[B][B + 1];
     __shared__ c128 Ms[B][B + 1];
-    __shared__ c128 Xs[B][B + 1];
     __shared__ c128 taus[B];
     __shared__ double betas[B];
     const int Lmax = m - B;
     c128 *V = reinterpret_cast<c128 *>(smem_raw);
-    c128 *W = V + (size_t)Lmax * B;
+    c128 *Y = W1all + (int64_t)blockIdx.x * sbr_w1_elems(m);
+    c128 *W = Y + (size_t)Lmax * B;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     c128 *A = Gall + (int64_t)blockIdx.x * m * m;
     c128 *Tg = Tall + (int64_t)blockIdx.x * npanel * B * B;
@@ -279,30 +283,17 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
                 yi += yi2;
                 if (i < L) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) W[(kl + 4 * q) * L + i] = c128{yr[q], yi[q]};
+                    for (int q = 0; q < 4; ++q) Y[(kl + 4 * q) * L + i] = c128{yr[q], yi[q]};
                 }
             }
         }
         __syncthreads();
-        // 6. Y <- Y T (T upper); outputs computed first, written after a barrier
-        {
-            constexpr int NTASK = (SBR_MAXM * B + S1T - 1) / S1T;
-            c128 res[NTASK];
-#pragma unroll
-            for (int u = 0; u < NTASK; ++u) {
-                const int task = tid + u * S1T;
-                const int i = task % L, c = task / L;
-                c128 acc = {0.0, 0.0};
-                if (task < L * B)
-                    for (int a = 0; a <= c; ++a) acc = acc + W[a * L + i] * Ts[a][c];
-                res[u] = acc;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < NTASK; ++u) {
-                const int task = tid + u * S1T;
-                if (task < L * B) W[(task / L) * L + task % L] = res[u];
-            }
+        // 6. W <- Y T (T upper)
+        for (int task = tid; task < L * B; task += S1T) {
+            const int i = task % L, c = task / L;
+            c128 acc = {0.0, 0.0};
+            for (int a = 0; a <= c; ++a) acc = acc + Y[a * L + i] * Ts[a][c];
+            W[c * L + i] = acc;
         }
         __syncthreads();
         // 7. X = V^H Y (two threads per entry over interleaved rows, two partial sums
@@ -321,14 +312,18 @@ __global__ __launch_bounds__(S1T) void k_sy2sb(c128 *__restrict__ Gall, int m, i
             s0 = s0 + s1;
             s0.re += __shfl_xor(s0.re, 1, 64);
             s0.im += __shfl_xor(s0.im, 1, 64);
-            if (part == 0) Xs[a][c] = s0;
+            if (part == 0) Ms[a][c] = s0;  // X
         }
         __syncthreads();
-        for (int pr = tid; pr < B * B; pr += S1T) {
-            const int a = pr % B, c = pr / B;
-            c128 s = {0.0, 0.0};
-            for (int q = 0; q <= a; ++q) s = s + cmulc(Ts[q][a], Xs[q][c]);
-            Ms[a][c] = s;
+        // M = T^H X in place, one column per thread, bottom row first (row a needs
+        // rows 0..a of X)
+        if (tid < B) {
+            const int c = tid;
+            for (int a = B - 1; a >= 0; --a) {
+                c128 s = {0.0, 0.0};
+                for (int q = 0; q <= a; ++q) s = s + cmulc(Ts[q][a], Ms[q][c]);
+                Ms[a][c] = s;
+            }
         }
         __syncthreads();
         // 8. W = Y - 1/2 V M on the matrix cores, one 16-row block per wave at a time
@@ -799,10 +794,10 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
     MSLR_REQUIRE(sbr_steps(m) <= STAGGER * S2W, MSLR_ERR_UNSUPPORTED, "sweep longer than the schedule");
     const int npanel = sbr_panels(m), ks = sbr_steps(m);
     {
-        const size_t lds = (size_t)2 * (m - B) * B * sizeof(c128);
+        const size_t lds = (size_t)(m - B) * B * sizeof(c128);
         raise_lds((const void *)k_sy2sb, lds);
         MSLR_PROF(ctx, "sy2sb");
-        k_sy2sb<<<(unsigned)nb, S1T, lds, ctx->stream>>>(G, m, npanel, w.T1);
+        k_sy2sb<<<(unsigned)nb, S1T, lds, ctx->stream>>>(G, m, npanel, w.T1, w.W1);
         HIP_LAUNCH_CHECK();
     }
     {

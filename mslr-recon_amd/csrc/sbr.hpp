@@ -22,15 +22,20 @@ inline int sbr_panels(int m) {
 // reflector slots per sweep of the band -> tridiagonal stage
 inline int sbr_steps(int m) { return (m - 1 + SBR_B - 1) / SBR_B; }
 
+// per matrix workspace of the dense -> band stage in global memory: the panel's
+// Y = A V and W = Y T - 1/2 V M ((m - B) x B each)
+__host__ __device__ inline size_t sbr_w1_elems(int m) { return (size_t)2 * (m - SBR_B) * SBR_B; }
+
 struct SbrWork {
     c128 *T1 = nullptr;    // (sbr_panels(m), B, B) per matrix: compact-WY T of each panel
     c128 *V2 = nullptr;    // (m, sbr_steps(m), B) per matrix: bulge-chasing reflectors
     c128 *tau2 = nullptr;  // (m, sbr_steps(m)) per matrix
+    c128 *W1 = nullptr;    // sbr_w1_elems(m) per matrix
 };
 
 inline size_t sbr_bytes_per_matrix(int m) {
     return (size_t)sbr_panels(m) * SBR_B * SBR_B * 16 + (size_t)m * sbr_steps(m) * SBR_B * 16 +
-           (size_t)m * sbr_steps(m) * 16;
+           (size_t)m * sbr_steps(m) * 16 + sbr_w1_elems(m) * 16;
 }
 
 // G (nb matrices, m x m, full Hermitian storage) -> d, e (real tridiagonal).
