@@ -207,6 +207,15 @@ mslr_status_t mslr_patch_svst(mslr_ctx ctx, const void *img_dev, void *out_dev,
                               const int64_t N[3], int64_t Nt, const int64_t ps[3],
                               const int64_t ss[3], double beta);
 
+/* ------------------------------------------------------------------ analysis */
+/* tSNR(img) (src/analysis.jl:39-44): out[v] = mean_t |img[v,t]| / (std_t |img[v,t]| +
+ * eps(Float32)), std with n - 1, for img ComplexF32 (V, Nt) column-major (V = Nx Ny Nz);
+ * out: Float32 (V), device.  FP64 sums.  Asynchronous on the context stream. */
+mslr_status_t mslr_tsnr(mslr_ctx ctx, const void *img_dev, int64_t V, int64_t Nt, void *out_dev);
+/* detrend!(img) (src/analysis.jl:122-132) in place: img[v,t] -= t * b_v, t = 1..Nt,
+ * b_v the least-squares slope of img[v,:] against [1 t].  Asynchronous. */
+mslr_status_t mslr_detrend(mslr_ctx ctx, void *img_dev, int64_t V, int64_t Nt);
+
 /* ------------------------------------------------------------------ POGM */
 /* Forms of the POGM iteration. */
 #define MSLR_FORM_PLAIN   0  /* per-scale launches, FP32 Gram products */

@@ -13,7 +13,8 @@ module MSLRHip
 
 export HipContext, HipArray, Asense_hip, img2patches_hip, patches2img_hip, SVST_hip,
        patchSVST_hip, patch_nucnorm_hip, pogm_hip, poweriter_hip, upload, download,
-       set_products!, SVST_vjp_hip, pogm_checkpointed_hip, lambda_vjp_hip, pogm_lambda_grad_hip
+       set_products!, SVST_vjp_hip, pogm_checkpointed_hip, lambda_vjp_hip, pogm_lambda_grad_hip,
+       tSNR_hip, detrend_hip!
 
 const LIB = get(ENV, "MSLR_HIP_LIB", joinpath(@__DIR__, "..", "mslr_hip", "libmslr_hip.so"))
 
@@ -313,6 +314,23 @@ function pogm_hip(A::Asense_hip, ksp, L::Real, λs::Vector{Float64}, PATCH_SIZES
     finally
         ccall((:mslr_pogm_destroy, LIB), Cint, (Ptr{Cvoid},), st[])
     end
+end
+
+"tSNR(img) (src/analysis.jl:39-44) of a device image series; returns the Float32 map."
+function tSNR_hip(img::HipArray{ComplexF32})
+    V = prod(img.dims[1:end-1])
+    out = HipArray{Float32}(img.dims[1:end-1]; ctx = img.ctx)
+    check(ccall((:mslr_tsnr, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}),
+                img.ctx.h, img.ptr, V, img.dims[end], out.ptr))
+    return out
+end
+
+"detrend!(img) (src/analysis.jl:122-132) in place on a device image series."
+function detrend_hip!(img::HipArray{ComplexF32})
+    V = prod(img.dims[1:end-1])
+    check(ccall((:mslr_detrend, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64),
+                img.ctx.h, img.ptr, V, img.dims[end]))
+    return img
 end
 
 "A POGM state kept on the device with its per-iteration checkpoints (the derivative form)."

@@ -119,6 +119,8 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_svst_batched": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, C.c_int]),
     "mslr_patch_nucnorm": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd]),
     "mslr_svst_vjp": (C.c_int, [_p, _p, _p, _p, _i64, _i64, _i64, _pd, C.c_int, _pd]),
+    "mslr_tsnr": (C.c_int, [_p, _p, _i64, _i64, _p]),
+    "mslr_detrend": (C.c_int, [_p, _p, _i64, _i64]),
     "mslr_patch_svst": (C.c_int, [_p, _p, _p, _pi64, _i64, _pi64, _pi64, C.c_double]),
     "mslr_pogm_create": (C.c_int, [_p, C.POINTER(mslr_pogm_desc), C.POINTER(_p)]),
     "mslr_pogm_destroy": (C.c_int, [_p]),

@@ -34,6 +34,10 @@ PINS = {
                 "forms (radial direction, beta direction); divided differences vs (g_i - g_j)/(l_i - l_j)",
     "pogm_lambda_jvp": "X equals run_recon_arrays(mode f64); dloss/dlambda vs FP64 central differences "
                        "of the same loss (restart sequence unchanged)",
+    "tSNR": "constant series (std 0 -> mean/eps), two-level series with known mean and std, "
+            "magnitude taken of complex values",
+    "detrend": "a + b t -> a exactly; constant series unchanged; the residual is orthogonal to "
+               "(t - mean t); pinv form equals the centred least-squares slope",
     "recon_costs": "X = 0 -> (||y||^2/2, 0); noise-free data at the truth -> dc 0; rank-1 single-patch "
                    "scale -> lambda |u||v|; dc quadratic in X; reg positively homogeneous",
 }
@@ -603,6 +607,31 @@ def pogm_lambda_jvp(ksp, omega, smaps, x_ref, *, N, Nt, PATCH_SIZES, STRIDES, NI
     loss = float(np.vdot(r.ravel(), r.ravel()).real) / n
     grads = [2.0 * float(np.vdot(r.ravel(), Xd[j].sum(axis=4).ravel()).real) / n for j in range(S)]
     return loss, grads, restarts, X
+
+
+# --------------------------------------------------------------------------- analysis
+# src/analysis.jl
+
+def tSNR(img):
+    """mean(|img|) / (std(|img|) + eps(Float32)) along the last axis, std with n - 1
+    (src/analysis.jl:39-44).  |img| in Float32 as Julia's abs.(ComplexF32); the sums
+    in FP64 (Julia's Float32 pairwise sums agree to ~1e-7)."""
+    mag = np.abs(np.asarray(img).astype(np.complex64)).astype(np.float64)
+    mean = mag.mean(axis=-1)
+    sd = mag.std(axis=-1, ddof=1) if mag.shape[-1] > 1 else np.full(mean.shape, np.nan)
+    return (mean / (sd + EPS32)).astype(np.float32)
+
+
+def detrend(img):
+    """detrend!(img) (src/analysis.jl:122-132) on a copy: per voxel b = (pinv(M) x)[2]
+    with M = [ones(Nt) 1:Nt], then x -= (1:Nt) * b, in FP64, stored as ComplexF32."""
+    x = np.asarray(img).astype(np.complex128)
+    Nt = x.shape[-1]
+    M = np.stack([np.ones(Nt), np.arange(1, Nt + 1, dtype=np.float64)], axis=1)
+    P = np.linalg.pinv(M)
+    b = x.reshape(-1, Nt) @ P[1]
+    out = x.reshape(-1, Nt) - np.outer(b, M[:, 1])
+    return out.reshape(x.shape).astype(np.complex64)
 
 
 def nrmse(x, ref):

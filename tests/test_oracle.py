@@ -439,3 +439,30 @@ def test_pogm_lambda_jvp_central_differences():
         lm = list(lams); lm[k] -= h
         fd = (lossf(lp) - lossf(lm)) / (2 * h)
         assert grads[k] == pytest.approx(fd, rel=1e-5), k
+
+
+# ----------------------------------------------------------------- analysis
+def test_tsnr_closed_forms():
+    Nt = 6
+    img = np.zeros((2, 1, 1, Nt), np.complex64)
+    img[0, 0, 0, :] = 3.0 * np.array([1, 1j, -1, -1j, 1, 1j])  # |x| = 3 exactly: std 0
+    img[1, 0, 0, :] = np.array([1, 3, 1, 3, 1, 3]) * 1j  # mean 2, std sqrt(6/5)
+    t = O.tSNR(img)
+    assert t.shape == (2, 1, 1)
+    assert t[0, 0, 0] == pytest.approx(3.0 / O.EPS32, rel=1e-6)
+    assert t[1, 0, 0] == pytest.approx(2.0 / (math.sqrt(6 / 5) + O.EPS32), rel=1e-6)
+
+
+def test_detrend_closed_forms(rng):
+    Nt = 9
+    t = np.arange(1, Nt + 1)
+    a = crandn(rng, (3, 2, 2, 1), np.complex128)
+    b = crandn(rng, (3, 2, 2, 1), np.complex128)
+    out = O.detrend((a + b * t).astype(np.complex64))
+    assert np.max(np.abs(out - np.broadcast_to(a, out.shape))) < 1e-5 * (np.abs(a).max() + np.abs(b).max() * Nt)
+    c = np.broadcast_to(a, (3, 2, 2, Nt)).astype(np.complex64)
+    assert np.array_equal(O.detrend(c), c) or np.max(np.abs(O.detrend(c) - c)) < 1e-6
+    x = crandn(rng, (4, 3, 2, Nt), np.complex64)
+    r = O.detrend(x).astype(np.complex128)
+    # the residual has no linear component left: orthogonal to the centred ramp
+    assert np.max(np.abs(r.reshape(-1, Nt) @ (t - t.mean()))) < 1e-4
