@@ -1859,24 +1859,20 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
                                                                          per_patch, w.info, w.r, 0);
             HIP_LAUNCH_CHECK();
             // lanes per eigenvalue: many when the eigenvalues are few (the bisection
-            // latency is the cost), few when they fill the device.  The number kept
-            // per patch is known only on the device, so a large batch takes its
-            // first EIG_FEW eigenvalues with 16 lanes each and the rest with 4
-            // (workgroups past a patch's r leave at once).
-            constexpr int EIG_FEW = 32;
-            const int mfew = (int)std::min<int64_t>(m, nb * m <= 64 * 1024 ? m : EIG_FEW);
-            {
-                dim3 ge((unsigned)nb, (unsigned)cdiv(mfew * 16, 64));
+            // latency is the cost), few when the batch already fills the device.
+            // (Splitting a patch's eigenvalues over a 16-lane and a 4-lane launch
+            // serialises the two latencies for patches keeping more than the first
+            // launch covers; measured slower beside the other scales' kernels.)
+            if (nb * m <= 64 * 1024) {
+                dim3 ge((unsigned)nb, (unsigned)cdiv(m * 16, 64));
                 k_eig_vectors<16><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
                                                               w.info, w.r, w.lam, w.Z, w.U, 0);
-                HIP_LAUNCH_CHECK();
-            }
-            if (mfew < m) {
-                dim3 ge((unsigned)nb, (unsigned)cdiv((m - mfew) * 4, 64));
+            } else {
+                dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
                 k_eig_vectors<4><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
-                                                             w.info, w.r, w.lam, w.Z, w.U, mfew);
-                HIP_LAUNCH_CHECK();
+                                                             w.info, w.r, w.lam, w.Z, w.U, 0);
             }
+            HIP_LAUNCH_CHECK();
             k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
                                                                     w.info, w.r, w.lam, w.Z, w.ws,
                                                                     w.f);
