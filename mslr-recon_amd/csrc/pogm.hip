@@ -548,10 +548,24 @@ static void pogm_iteration(mslr_pogm_s *st) {
         return;
     }
     // Scales are independent: each runs on its own stream with its own scratch.
+    // The 1x1x1 scales (one HBM pass, no scratch) go first on the main stream, so
+    // their traffic does not overlap the other scales' gathers and Gram passes.
     hipStream_t main = ctx->stream;
+    auto voxel_scale = [&](int64_t k) {
+        const PatchGeom &g = st->geoms[k];
+        return g.p == 1 && g.Np == (int64_t)g.N[0] * g.N[1] * g.N[2] && g.exact_tiling();
+    };
+    for (int64_t k = 0; k < st->S; ++k) {
+        if (!voxel_scale(k)) continue;
+        ctx->prof_suffix = k < 8 ? sfx[k] : "@s9+";
+        prox_scale(ctx, st->geoms[k], st->X + k * n, st->X + k * n, st->thr_dev + k, err,
+                   nuc_part(k), nuc_tot(k));
+    }
+    ctx->prof_suffix = "";
     HIP_CHECK(hipEventRecord(st->ev_fork, main));
     try {
         for (int64_t k = 0; k < st->S; ++k) {
+            if (voxel_scale(k)) continue;
             c64 *Xk = st->X + k * n;
             HIP_CHECK(hipStreamWaitEvent(st->aux[k], st->ev_fork, 0));
             ctx->stream = st->aux[k];
@@ -569,7 +583,8 @@ static void pogm_iteration(mslr_pogm_s *st) {
     ctx->stream = main;
     ctx->slot_base = 0;
     ctx->prof_suffix = "";
-    for (int64_t k = 0; k < st->S; ++k) HIP_CHECK(hipStreamWaitEvent(main, st->ev_join[k], 0));
+    for (int64_t k = 0; k < st->S; ++k)
+        if (!voxel_scale(k)) HIP_CHECK(hipStreamWaitEvent(main, st->ev_join[k], 0));
     log_reg_cost(st);
 }
 
