@@ -31,21 +31,21 @@ HBM_PEAK_GBS = 8000.0
 # families, from the committed rocprofv3 counter summary of this workload (mixed
 # form; the SENSE, update and patch kernels are the same in every form)
 TRAFFIC_BYTES = {
-    "fwd_yz": 6994400000,
-    "adj_yz": 4406400000,
+    "fwd_yz": 6949200000,
+    "adj_yz": 4399700000,
     "pogm_update": 15163600000,
     "fwd_x": 8933700000,
-    "adj_x": 6115300000,
+    "adj_x": 6115000000,
     "sum_planes": 4276900000,
-    "sy2sb": 3764600000,
-    "img2patches": 5286600000,
+    "sy2sb": 6122600000,
+    "img2patches": 5299400000,
     "prox_voxel": 3499400000,
-    "patches2img": 3149800000,
-    "apply": 908900000,
+    "patches2img": 3151000000,
+    "apply": 909000000,
     "planes_to_image": 2479700000,
     "split": 2342200000,
-    "gram_tiles": 1864500000,
-    "backtransform": 946800000,
+    "gram_tiles": 1868300000,
+    "backtransform": 946700000,
     "sb2st": 224800000,
 }
 FP32_PEAK_TFLOPS = 157.3
