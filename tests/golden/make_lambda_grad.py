@@ -1,8 +1,11 @@
 """Writes lambda_grad_<config>.json: the FP64 forward-mode oracle's loss and
 d loss / d lambda_k for the differentiation workload (MSE of the image sum after
-NITERS unrolled POGM iterations to the synthetic truth in the k-space's units).
+NITERS unrolled POGM iterations to the synthetic truth in the k-space's units), and
+with `fd` the FP64 oracle's central differences with steps 1e-3 lambda_k
+(`fd_grads`, added to the same file).
 
     python tests/golden/make_lambda_grad.py c4 10
+    python tests/golden/make_lambda_grad.py c4 10 fd
 """
 import json
 import sys
@@ -18,6 +21,37 @@ import numpy as np  # noqa: E402
 
 import oracle as O  # noqa: E402
 from problems import prepared, truth  # noqa: E402
+
+
+def central_differences(name: str, niter: int, rel: float = 1e-3) -> None:
+    cfg, smaps, ksp, omega, sf = prepared(name)
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    x_ref = truth(name).astype(np.complex128) / sf
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    kw = dict(N=N, Nt=Nt, PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"], NITERS=niter,
+              sigma1A=1.0, mode="f64")
+    path = HERE / f"lambda_grad_{name}.json"
+    out = json.loads(path.read_text())
+    t = time.time()
+
+    def loss(lm):
+        X, rs, _ = O.run_recon_arrays(ksp, omega, smaps, lambdas=lm, **kw)
+        r = X.sum(axis=4) - x_ref
+        print(f"  run {lm}: {time.time() - t:.0f} s", flush=True)
+        return float(np.vdot(r.ravel(), r.ravel()).real) / r.size, rs
+
+    fd = []
+    for k in range(len(lams)):
+        h = rel * lams[k]
+        lp = list(lams); lp[k] += h
+        lm = list(lams); lm[k] -= h
+        (fp, rp), (fm, rm) = loss(lp), loss(lm)
+        assert rp == out["restarts"] and rm == out["restarts"], "a step changed the restarts"
+        fd.append((fp - fm) / (2 * h))
+    out["fd_step_rel"] = rel
+    out["fd_grads"] = fd
+    path.write_text(json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out))
 
 
 def main(name: str, niter: int) -> None:
@@ -38,4 +72,7 @@ def main(name: str, niter: int) -> None:
 
 
 if __name__ == "__main__":
-    main(sys.argv[1], int(sys.argv[2]))
+    if len(sys.argv) > 3 and sys.argv[3] == "fd":
+        central_differences(sys.argv[1], int(sys.argv[2]))
+    else:
+        main(sys.argv[1], int(sys.argv[2]))

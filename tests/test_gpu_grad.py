@@ -129,12 +129,25 @@ def test_lambda_grad_graph_form_matches_plain():
 
 
 def test_lambda_grad_c4_central_differences_and_oracle():
-    """c4, 10 iterations: central differences of the device loss with steps 1e-3 lambda_k,
-    and the FP64 forward-mode gradient of the oracle (tests/golden/lambda_grad_c4.json)."""
+    """c4, 10 iterations: the device gradient against the oracle's FP64 forward-mode
+    gradient and the oracle's FP64 central differences with steps 1e-3 lambda_k
+    (tests/golden/lambda_grad_c4.json), both to 1e-3; and against central
+    differences of the device's own FP32 loss with the same steps.  The FP32 loss
+    carries a rounding jitter of ~1e-7 of its value from run to run of lambda (measured
+    2^-23 L at c4), which a central difference divides by 2h: the device-side check
+    allows max(1e-3 |fd|, 2^-21 L / h) (for lambda_2 = 36.8 the jitter term is ~1e-3
+    absolute, 4% of the gradient)."""
     niter = 10
     print("lambda grad c4", flush=True)
     loss, grads, rs, lams = _device_grad("c4", niter)
     print(f"c4 loss {loss:.6e} grads {grads} restarts {rs}", flush=True)
+    g = json.loads((GOLDEN / "lambda_grad_c4.json").read_text())
+    assert g["niter"] == niter and rs == g["restarts"]
+    assert loss == pytest.approx(g["loss"], rel=1e-4)
+    for k in range(len(lams)):
+        assert grads[k] == pytest.approx(g["grads"][k], rel=1e-3), (k, grads, g["grads"])
+        if "fd_grads" in g:
+            assert grads[k] == pytest.approx(g["fd_grads"][k], rel=1e-3), (k, grads, g["fd_grads"])
     for k in range(len(lams)):
         h = 1e-3 * lams[k]
         lp = list(lams); lp[k] += h
@@ -143,10 +156,7 @@ def test_lambda_grad_c4_central_differences_and_oracle():
         fm, rm = _device_loss("c4", niter, lm)
         assert rp == rs and rm == rs
         fd = (fp - fm) / (2 * h)
-        print(f"  lambda_{k}: adjoint {grads[k]:.6e} central difference {fd:.6e}", flush=True)
-        assert grads[k] == pytest.approx(fd, rel=1e-3), k
-    g = json.loads((GOLDEN / "lambda_grad_c4.json").read_text())
-    assert g["niter"] == niter and rs == g["restarts"]
-    assert loss == pytest.approx(g["loss"], rel=1e-4)
-    for k in range(len(lams)):
-        assert grads[k] == pytest.approx(g["grads"][k], rel=1e-3), (k, grads, g["grads"])
+        tol = max(1e-3 * abs(fd), 2.0 ** -21 * loss / h)
+        print(f"  lambda_{k}: adjoint {grads[k]:.6e} central difference {fd:.6e} (allowed {tol:.1e})",
+              flush=True)
+        assert abs(grads[k] - fd) <= tol, k

@@ -81,3 +81,41 @@ def test_invalid_options():
         POGM(A, L=1.0, mom="foo", **kw)
     with pytest.raises(ValueError):
         POGM(A, L=1.0, restart_cutoff=1.5, **kw)
+
+
+@pytest.mark.parametrize("name,niter", [("small", 6), ("c0", 4)])
+def test_per_iterate_parity(name, niter):
+    """Every iterate x_k (k = 1..niter) within 1e-4 of the FP64 oracle's, and the
+    final image's NRMSE to the synthetic truth within 1e-5 of the oracle's."""
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.sense import SenseOp
+    from problems import truth
+    cfg, smaps, ksp, omega, sf = prepared(name)
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    S = len(cfg["patch_sizes"])
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    A = SenseOp(omega, smaps)
+    st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=S * 1.0,
+              niter=niter)
+    st.init(ksp)
+    got = []
+    for _ in range(niter):
+        st.run(1)
+        got.append(st.X.to_host())
+    iterates = []
+
+    def keep(it, xk, yk, r):
+        if it > 0:
+            iterates.append(np.array(xk))
+        return r
+
+    Xo, _, _ = O.run_recon_arrays(ksp, omega, smaps, N=N, Nt=Nt, PATCH_SIZES=cfg["patch_sizes"],
+                                  STRIDES=cfg["strides"], NITERS=niter, sigma1A=1.0, mode="f64",
+                                  fun=keep)
+    assert len(iterates) == niter
+    for k in range(niter):
+        assert rel_err(got[k], iterates[k]) < 1e-4, k
+    xt = truth(name).astype(np.complex128) / sf
+    n_gpu = O.nrmse(got[-1].sum(axis=4), xt)
+    n_ora = O.nrmse(Xo.sum(axis=4), xt)
+    assert abs(n_gpu - n_ora) < 1e-5, (n_gpu, n_ora)
