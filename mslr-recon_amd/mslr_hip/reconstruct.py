@@ -144,6 +144,26 @@ def run_recon(*, fn_ksp: str, fn_smaps: str, fn_recon_base: str, N, Nvc: int, Nt
     out = run_recon_arrays(ksp0, smaps_raw, N=N, Nvc=Nvc, Nt=Nt, PATCH_SIZES=PATCH_SIZES,
                            STRIDES=STRIDES, NITERS=NITERS, σ1A_PRECOMPUTED=σ1A_PRECOMPUTED, form=form)
     fn_out = f"{fn_recon_base}_{len(PATCH_SIZES)}scales.mat"
-    sio.savemat(fn_out, {k: v for k, v in out.items() if v is not None}, do_compression=True)
+    sio.savemat(fn_out, mat_dict(out), do_compression=True, oned_as="column")
     print(f"\n✓ Saved → {fn_out}")
     return None
+
+
+def mat_dict(out: dict) -> dict:
+    """The saved variables of scripts/reconstruct.jl:241-260 in MAT-v5 form: vectors as
+    columns (Julia's Vector), patch_sizes / strides (Vector of Vectors) as cell arrays."""
+    d = {}
+    for k, v in out.items():
+        if v is None:
+            continue
+        if k in ("patch_sizes", "strides"):
+            cell = np.empty(len(v), dtype=object)
+            for i, row in enumerate(v):
+                cell[i] = np.asarray(row, dtype=np.float64)
+            v = cell
+        elif k in ("lambdas", "dc_costs", "reg_costs"):
+            v = np.asarray(v, dtype=np.float64)
+        elif k == "restarts":
+            v = np.asarray(v, dtype=bool)
+        d[k] = v
+    return d

@@ -139,3 +139,23 @@ def test_run_recon_arrays_outputs(name):
     Xo, _, _ = O.run_recon_arrays(ksp, omega, smaps, N=N, Nt=Nt, PATCH_SIZES=cfg["patch_sizes"],
                                   STRIDES=cfg["strides"], NITERS=niter, sigma1A=1.0, mode="f64")
     assert rel_err(out["X_recon"], Xo.sum(axis=4)) < 1e-4
+
+
+def test_run_recon_writes_the_reference_keys(tmp_path):
+    """run_recon from files: .npy k-space and MAT smaps in, the reference's .mat out."""
+    import scipy.io as sio
+    from mslr_hip.reconstruct import run_recon
+    cfg, ksp0, smaps_raw = raw("small")
+    N, Nt, Nc = tuple(cfg["N"]), cfg["Nt"], cfg["Nc"]
+    np.save(tmp_path / "ksp.npy", ksp0)
+    sio.savemat(tmp_path / "smaps.mat", {"smaps": smaps_raw})
+    run_recon(fn_ksp=str(tmp_path / "ksp.npy"), fn_smaps=str(tmp_path / "smaps.mat"),
+              fn_recon_base=str(tmp_path / "recon"), N=N, Nvc=Nc, Nt=Nt, FOV=(1, 1, 1),
+              PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"], NITERS=3, σ1A_PRECOMPUTED=1.0)
+    f = sio.loadmat(tmp_path / "recon_2scales.mat")
+    for k in ["X", "X_recon", "omega", "dc_costs", "reg_costs", "restarts", "R", "sigma1A", "L",
+              "Nscales", "patch_sizes", "strides", "lambdas", "Niters", "scale_factor", "used_gpu"]:
+        assert k in f, k
+    assert f["X"].shape == (*N, Nt, 2) and f["X_recon"].shape == (*N, Nt)
+    assert f["dc_costs"].shape == (4, 1) and f["restarts"].shape == (4, 1)
+    assert f["patch_sizes"].shape[0] == 2 and list(f["patch_sizes"][0].ravel()[0].ravel()) == cfg["patch_sizes"][0]
