@@ -145,6 +145,11 @@ mslr_status_t mslr_sense_forward_f64(mslr_sense plan, const void *x_dev, void *y
                                      int64_t t0, int64_t nframes);
 mslr_status_t mslr_sense_adjoint_f64(mslr_sense plan, const void *y_dev, void *x_dev,
                                      int64_t t0, int64_t nframes);
+/* Forward residual r = A sum_k X_k - y (reference sample order) and rnorm2 = ||r||^2
+ * (FP64 sum of FP32 values), the data term of dc_cost (scripts/reconstruct.jl:176-178).
+ * r_dev: device (K, Nc, Nt), or NULL to keep only the norm. */
+mslr_status_t mslr_sense_residual(mslr_sense plan, const void *X_dev, int64_t nscales,
+                                  const void *y_dev, void *r_dev, double *rnorm2);
 /*
  * Gradient of the data-consistency term on the scale sum
  * (dc_cost_grad, scripts/reconstruct.jl:174,176-183):
@@ -202,6 +207,13 @@ mslr_status_t mslr_svst_vjp(mslr_ctx ctx, const void *Z_dev, const void *Ybar_de
 /* Sum over patches of the nuclear norms (patch_nucnorm, src/recon.jl:138-146). */
 mslr_status_t mslr_patch_nucnorm(mslr_ctx ctx, const void *P_dev, int64_t p, int64_t Nt,
                                  int64_t Np, double *out);
+/* The in-place prox of run_recon over all scales (g_prox, scripts/reconstruct.jl:196-202):
+ * X[..., k] <- patchSVST(X[..., k], beta[k], patch_sizes[k], strides[k]) for the 5D
+ * ComplexF32 array X (N[0], N[1], N[2], Nt, nscales); beta_host[k] = c * lambda_k.
+ * Scales one after another on the context stream. */
+mslr_status_t mslr_prox_scales(mslr_ctx ctx, void *X_dev, const int64_t N[3], int64_t Nt,
+                               int64_t nscales, const int64_t *patch_sizes, const int64_t *strides,
+                               const double *beta_host);
 /* patchSVST (src/recon.jl:251-256): img2patches -> SVST -> patches2img. */
 mslr_status_t mslr_patch_svst(mslr_ctx ctx, const void *img_dev, void *out_dev,
                               const int64_t N[3], int64_t Nt, const int64_t ps[3],

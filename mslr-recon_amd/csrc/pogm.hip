@@ -933,6 +933,37 @@ mslr_status_t mslr_pogm_thresholds(mslr_pogm st, double *beta) {
     });
 }
 
+mslr_status_t mslr_prox_scales(mslr_ctx ctx, void *X_dev, const int64_t N[3], int64_t Nt,
+                               int64_t nscales, const int64_t *patch_sizes, const int64_t *strides,
+                               const double *beta_host) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(X_dev && N && patch_sizes && strides && beta_host, MSLR_ERR_ARG,
+                     "null argument");
+        MSLR_REQUIRE(nscales >= 1, MSLR_ERR_ARG, "nscales must be >= 1");
+        std::vector<PatchGeom> gs;
+        for (int64_t k = 0; k < nscales; ++k) {
+            MSLR_REQUIRE(beta_host[k] >= 0 && std::isfinite(beta_host[k]), MSLR_ERR_ARG,
+                         "beta must be finite and >= 0");
+            gs.push_back(make_geom(N, Nt, patch_sizes + 3 * k, strides + 3 * k));
+        }
+        const int64_t n = N[0] * N[1] * N[2] * Nt;
+        double *bd = (double *)c->scratch_get(SLOT_APPLY, (size_t)(nscales + 1) * sizeof(double) + 64);
+        int *err = (int *)(bd + nscales);
+        HIP_CHECK(hipMemcpyAsync(bd, beta_host, sizeof(double) * nscales, hipMemcpyHostToDevice,
+                                 c->stream));
+        HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+        for (int64_t k = 0; k < nscales; ++k) {
+            c64 *Xk = (c64 *)X_dev + k * n;
+            prox_scale(c, gs[k], Xk, Xk, bd + k, err, nullptr, nullptr);
+        }
+        int herr = 0;
+        HIP_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        MSLR_REQUIRE(herr == 0, MSLR_ERR_NUMERIC, "non-finite values in a patch matrix");
+    });
+}
+
 mslr_status_t mslr_patch_svst(mslr_ctx ctx, const void *img_dev, void *out_dev,
                               const int64_t N[3], int64_t Nt, const int64_t ps[3],
                               const int64_t ss[3], double beta) {

@@ -1005,6 +1005,31 @@ mslr_status_t mslr_sense_grad(mslr_sense plan, const void *X_dev, int64_t nscale
     });
 }
 
+mslr_status_t mslr_sense_residual(mslr_sense plan, const void *X_dev, int64_t nscales,
+                                  const void *y_dev, void *r_dev, double *rnorm2) {
+    return guarded([&] {
+        check_frames(plan, 0, plan ? plan->Nt : 0);
+        MSLR_REQUIRE(nscales >= 1, MSLR_ERR_ARG, "nscales must be >= 1");
+        MSLR_REQUIRE(X_dev && y_dev && rnorm2, MSLR_ERR_ARG, "null argument");
+        mslr_sense_s *P = plan;
+        mslr_ctx_s *ctx = P->ctx;
+        const int64_t ny = P->K * P->Nc * (int64_t)P->Nt;
+        c64 *r = r_dev ? (c64 *)r_dev
+                       : (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, ny) *
+                                                                 sizeof(c64));
+        const int64_t npart = (int64_t)P->Nt * P->Nc * P->nlb;
+        double *part = (double *)ctx->scratch_get(SLOT_REDUCE, (size_t)(npart + 1) * sizeof(double));
+        run_forward<float>(P, (const c64 *)X_dev, (int)nscales, P->V * (int64_t)P->Nt, 0, P->Nt,
+                           (const c64 *)y_dev, r, part);
+        sum_partials(ctx, part, npart, part + npart);
+        double h = 0.0;
+        HIP_CHECK(hipMemcpyAsync(&h, part + npart, sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        *rnorm2 = h;
+    });
+}
+
 mslr_status_t mslr_sense_poweriter(mslr_sense plan, int niter, double tol, double *sigma1,
                                    int *iters) {
     return guarded([&] {

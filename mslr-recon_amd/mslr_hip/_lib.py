@@ -112,6 +112,7 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_sense_forward_f64": (C.c_int, [_p, _p, _p, _i64, _i64]),
     "mslr_sense_adjoint_f64": (C.c_int, [_p, _p, _p, _i64, _i64]),
     "mslr_sense_grad": (C.c_int, [_p, _p, _i64, _p, _p, _pd]),
+    "mslr_sense_residual": (C.c_int, [_p, _p, _i64, _p, _p, _pd]),
     "mslr_sense_poweriter": (C.c_int, [_p, C.c_int, C.c_double, _pd, _pi]),
     "mslr_patch_geometry": (C.c_int, [_pi64, _pi64, _pi64, _pi64, _pi64, _pi64]),
     "mslr_img2patches": (C.c_int, [_p, _p, _pi64, _i64, _pi64, _pi64, _p]),
@@ -122,6 +123,7 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_tsnr": (C.c_int, [_p, _p, _i64, _i64, _p]),
     "mslr_detrend": (C.c_int, [_p, _p, _i64, _i64]),
     "mslr_patch_svst": (C.c_int, [_p, _p, _p, _pi64, _i64, _pi64, _pi64, C.c_double]),
+    "mslr_prox_scales": (C.c_int, [_p, _p, _pi64, _i64, _i64, _pi64, _pi64, _pd]),
     "mslr_pogm_create": (C.c_int, [_p, C.POINTER(mslr_pogm_desc), C.POINTER(_p)]),
     "mslr_pogm_destroy": (C.c_int, [_p]),
     "mslr_pogm_init": (C.c_int, [_p, _p, _p]),

@@ -78,6 +78,28 @@ def svst_batched(P, beta, ctx: Context | None = None, inplace: bool = False) -> 
     return d
 
 
+def g_prox(X, c, lambdas, patch_sizes, strides, ctx: Context | None = None) -> DeviceArray:
+    """run_recon's in-place prox over all scales (scripts/reconstruct.jl:196-202):
+    X[..., k] <- patchSVST(X[..., k], c * lambdas[k], patch_sizes[k], strides[k]) for the
+    5D array X (Nx, Ny, Nz, Nt, S); a DeviceArray is updated in place and returned."""
+    ctx = ctx or default_context()
+    d = X if isinstance(X, DeviceArray) else as_device(X, np.complex64, ctx)
+    if len(d.shape) != 5:
+        raise AssertionError("X must be (Nx, Ny, Nz, Nt, S)")
+    S = d.shape[4]
+    if len(patch_sizes) != S or len(strides) != S or len(lambdas) != S:
+        raise AssertionError("one patch size, stride and lambda per scale")
+    ps = np.ascontiguousarray(np.asarray(patch_sizes, dtype=np.int64).reshape(S, 3))
+    ss = np.ascontiguousarray(np.asarray(strides, dtype=np.int64).reshape(S, 3))
+    b = np.ascontiguousarray([float(c) * float(l) for l in lambdas], dtype=np.float64)
+    N = (C.c_int64 * 3)(*d.shape[:3])
+    _lib.check(ctx._lib.mslr_prox_scales(ctx.handle, d.ptr, N, d.shape[3], S,
+                                         ps.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         ss.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         b.ctypes.data_as(C.POINTER(C.c_double))))
+    return d
+
+
 def svst_vjp(Z, Ybar, beta, ctx: Context | None = None):
     """Pullback of svst_batched: (Zbar DeviceArray, betabar) for Y = SVST(Z, beta) and the
     cotangent Ybar (same shape as Z: (p, Nt) or (p, Nt, Np), p >= Nt).  betabar has one

@@ -110,6 +110,19 @@ class SenseOp:
                                                  C.byref(cost) if want_cost else None))
         return (g, cost.value) if want_cost else g
 
+    def residual(self, X, y, want_residual: bool = False):
+        """||A sum_k X_k - y||^2 (the data term of dc_cost, scripts/reconstruct.jl:176-178),
+        with the residual (K, Nc, Nt) as a DeviceArray when asked."""
+        Xd = as_device(X, np.complex64, self.ctx)
+        yd = as_device(y, np.complex64, self.ctx)
+        V = int(np.prod(self.N))
+        S = Xd.nbytes // (V * self.Nt * 8)
+        r = DeviceArray((self.K, self.Nc, self.Nt), np.complex64, self.ctx) if want_residual else None
+        n2 = C.c_double(0.0)
+        _lib.check(self.ctx._lib.mslr_sense_residual(self.handle, Xd.ptr, int(S), yd.ptr,
+                                                     r.ptr if r is not None else None, C.byref(n2)))
+        return (n2.value, r) if want_residual else n2.value
+
     def poweriter(self, niter: int = 200, tol: float = 1e-6):
         """(sigma1, iterations) by FP64 power iteration from x0 = ones."""
         s = C.c_double()

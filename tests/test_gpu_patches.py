@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from conftest import crandn
+from conftest import crandn, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -74,3 +74,22 @@ def test_production_patch_counts():
     assert recon.patch_geometry(N, (30, 30, 20), (15, 15, 10))[2] == 125
     assert recon.patch_geometry(N, (10, 10, 10), (5, 5, 5))[2] == 3179
     assert recon.patch_geometry(N, (6, 6, 6), (3, 3, 3))[2] == 15979
+
+
+def test_g_prox_all_scales_in_place(ctx, rng):
+    """run_recon's g_prox over a 5D array (global, overlapping 4^3 with stride 2, 1x1x1)."""
+    import oracle as O
+    from mslr_hip.recon import g_prox
+    from mslr_hip.device import DeviceArray
+    N, Nt = (12, 12, 10), 10
+    ps = [[12, 12, 10], [4, 4, 4], [1, 1, 1]]
+    ss = [[12, 12, 10], [2, 2, 2], [1, 1, 1]]
+    lams = [30.0, 6.0, 2.0]
+    X = crandn(rng, (*N, Nt, 3), np.complex64)
+    d = DeviceArray.from_host(X, ctx)
+    g_prox(d, 0.5, lams, ps, ss)
+    out = d.to_host()
+    for k in range(3):
+        want = O.patchSVST(X[..., k].astype(np.complex128), np.float32(0.5 * lams[k]), ps[k], ss[k],
+                           dtype=np.complex128)
+        assert rel_err(out[..., k], want) < 2e-5, k

@@ -240,3 +240,18 @@ def test_frames_with_different_line_counts(ctx, rng):
     assert rel_err(A.forward(x).to_host(), O.A_forward(x.astype(np.complex128), mask, sm)) < 2e-6
     r = crandn(rng, (K, Nc, Nt))
     assert rel_err(A.adjoint(r).to_host(), O.A_adjoint(r.astype(np.complex128), mask, sm)) < 2e-6
+
+
+def test_forward_residual_matches_oracle(ctx, rng):
+    """||A sum_k X_k - y||^2 and the residual itself (the data term of dc_cost)."""
+    from mslr_hip.sense import SenseOp
+    from problems import prepared
+    cfg, smaps, ksp, omega, sf = prepared("small")
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    X = crandn(rng, (*N, Nt, 2), np.complex64)
+    A = SenseOp(omega, smaps)
+    n2, r = A.residual(X, ksp, want_residual=True)
+    ro = O.A_forward(X.astype(np.complex128).sum(axis=4), omega, smaps.astype(np.complex128)) - ksp
+    assert n2 == pytest.approx(float(np.vdot(ro.ravel(), ro.ravel()).real), rel=1e-5)
+    assert rel_err(r.to_host(), ro) < 2e-6
+    assert A.residual(X, ksp) == pytest.approx(n2, rel=1e-12)
