@@ -302,6 +302,8 @@ def main() -> int:
     ap.add_argument("--no-prof", action="store_true", help="skip the pass with per-kernel event timing")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the batched FP32 and plain runs (rates and result differences)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the built-in SVST launch shapes (no autotuning of the measured form)")
     ap.add_argument("--no-grad", action="store_true",
                     help="skip the differentiation workload (c4 lambda-gradient, timed apart)")
     ap.add_argument("--logger", default="off", choices=["on", "off"],
@@ -372,6 +374,19 @@ def main() -> int:
         truth's units (the k-space was divided by sf)."""
         return float(np.linalg.norm(x * np.float32(sf) - pb.x_true) / xt_norm)
 
+    # launch shapes of the measured form from the autotuner (timed POGM iterations
+    # of this workload per candidate; untimed here, like the warmup)
+    tune = None
+    tuned = {}
+    if args.form != "plain" and not args.no_tune:
+        from mslr_hip.autotune import autotune_pogm
+        t = time.time()
+        tune = autotune_pogm(A, yd, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"],
+                             lambdas=lams, L=L, form=args.form, log=log)
+        tuned = tune["tuning"]
+        log(f"autotune: {tuned} ({tune['builtin_ms_per_iter']:.2f} -> {tune['ms_per_iter']:.2f} ms/iter "
+            f"over {len(tune['trials'])} trials, {time.time() - t:.1f}s)")
+
     # the measured form: W untimed iterations, then K timed ones
     st = make(args.form, W + K)
     t = time.time()
@@ -405,6 +420,8 @@ def main() -> int:
     kern = {}
     nr_form = nrmse_truth(x_form)
     if args.form != "plain" and not args.no_compare:
+        # the other forms with the built-in launch shapes
+        ctx.set_tuning()
         for other in (["batched"] if args.form == "mixed" else []) + ["plain"]:
             sp = make(other, W + K)
             sp.run(W)
@@ -419,6 +436,7 @@ def main() -> int:
                 f"||x_{args.form} - x_{other}|| / ||x_{other}|| = {rd:.3e}")
         plain = others["plain"]
     log(f"{args.form}: NRMSE vs truth {nr_form:.5e}")
+    ctx.set_tuning(**tuned)
     if not args.no_prof:
         sp = make(args.form, W + K)
         sp.run(W)
@@ -469,6 +487,7 @@ def main() -> int:
     grad = None
     if not args.no_grad:
         t = time.time()
+        ctx.set_tuning()
         grad = differentiation_run(ctx)
         log(f"differentiation (c4, {grad['niter']} iterations): forward {grad['forward_checkpointed_s']:.3f}s "
             f"backward {grad['backward_s']:.3f}s dloss/dlambda {grad['dloss_dlambda']} ({time.time() - t:.1f}s)")
@@ -497,7 +516,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"{name}: {N[0]}x{N[1]}x{N[2]} Nt={Nt} {Nc} coils R={cfg['R']} scales={cfg['patch_sizes']}",
                    "form": args.form, "parallelism": "single-gpu", "restarts_timed": sum(restarts[W:]),
-                   "logger": args.logger},
+                   "logger": args.logger, "svst_tuning": tuned or "built-in"},
         "nrmse_vs_truth": nr_form,
         "delta_nrmse": abs(nr_form - plain["nrmse_vs_truth"]) if plain else None,
         "plain": plain,
@@ -509,6 +528,11 @@ def main() -> int:
         "with_logging": with_logging,
         "costs": costs,
         "differentiation": grad,
+        "autotune": None if tune is None else {
+            "tuning": tune["tuning"], "ms_per_iter": tune["ms_per_iter"],
+            "builtin_ms_per_iter": tune["builtin_ms_per_iter"],
+            "trials": [{"tuning": t, "ms_per_iter": ms, "iterate_rel_diff": err, "kept": ok}
+                       for t, ms, err, ok in tune["trials"]]},
     }
     print(json.dumps(line), flush=True)
     return 0

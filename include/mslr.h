@@ -94,6 +94,28 @@ mslr_status_t mslr_ctx_info(mslr_ctx ctx, char *name, size_t name_len,
 #define MSLR_PRODUCTS_SPLIT_BF16 1
 mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
 
+/* Launch shapes of the batched SVST on this context.  No counterpart in the
+ * reference (its SVD is a library call per patch); set by the autotuner
+ * (mslr_hip.autotune) or by hand.  0 in a field keeps the built-in choice.
+ * Read when work is enqueued: a POGM state's captured graph keeps the shapes it
+ * was captured with, so create the state after setting them.
+ *   gram_chunk:      contraction rows per Gram workgroup (built-in 4096); a
+ *                    multiple of 256 in [256, 65536].  Partial Grams of the chunks
+ *                    are summed in FP64, so the choice changes rounding only.
+ *   eig_lanes_limit: batches with patches x m at or below this use 16 lanes per
+ *                    eigenvector, larger ones 4 (built-in 65536); -1 = always 4.
+ *   svst_apply:      1 = form W for every patch (dense apply); 0 = low-rank
+ *                    products X U_r for patches with 2 r <= m (built-in).
+ * Invalid values -> MSLR_ERR_ARG. */
+typedef struct mslr_tuning {
+    int gram_chunk;
+    int eig_lanes_limit;
+    int svst_apply;
+    int reserved[5];
+} mslr_tuning;
+mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t);
+mslr_status_t mslr_ctx_get_tuning(mslr_ctx ctx, mslr_tuning *t);
+
 /* ------------------------------------------------------- device buffers */
 /* Replaces cu(...) / Array(...) transfers (scripts/reconstruct.jl:122-130, 233-238). */
 mslr_status_t mslr_malloc(mslr_ctx ctx, size_t bytes, void **dev_ptr);

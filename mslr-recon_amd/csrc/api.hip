@@ -156,6 +156,29 @@ mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products) {
     });
 }
 
+mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(t, MSLR_ERR_ARG, "null tuning");
+        MSLR_REQUIRE(t->gram_chunk == 0 ||
+                         (t->gram_chunk >= 256 && t->gram_chunk <= 65536 && t->gram_chunk % 256 == 0),
+                     MSLR_ERR_ARG, "gram_chunk must be 0 or a multiple of 256 in [256, 65536]");
+        MSLR_REQUIRE(t->eig_lanes_limit >= -1, MSLR_ERR_ARG, "eig_lanes_limit must be >= -1");
+        MSLR_REQUIRE(t->svst_apply == 0 || t->svst_apply == 1, MSLR_ERR_ARG,
+                     "svst_apply must be 0 or 1");
+        for (int r : t->reserved) MSLR_REQUIRE(r == 0, MSLR_ERR_ARG, "reserved fields must be 0");
+        c->tuning = *t;
+    });
+}
+
+mslr_status_t mslr_ctx_get_tuning(mslr_ctx ctx, mslr_tuning *t) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(t, MSLR_ERR_ARG, "null tuning");
+        *t = c->tuning;
+    });
+}
+
 mslr_status_t mslr_ctx_sync(mslr_ctx ctx) {
     return guarded([&] {
         mslr_ctx_s *c = check_ctx(ctx);

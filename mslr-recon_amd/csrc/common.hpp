@@ -111,6 +111,7 @@ struct mslr_ctx_s {
     size_t slot_base = 0;                // added to every slot (per-stream scratch sets)
     uint64_t scratch_gen = 0;            // bumped whenever a slot is reallocated
     int products = 0;                    // MSLR_PRODUCTS_*: SVST Gram / apply products
+    mslr_tuning tuning{};                // SVST launch shapes (0 = built-in)
 
     // Kernel timing with HIP events on `stream` (enabled by mslr_prof_enable).
     struct ProfRec {

@@ -47,7 +47,8 @@ constexpr int WS_PER = 256;  // doubles per Gram row of eigen workspace per patc
 template <class AT>
 __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, int64_t pstride,
                                                     int64_t ks, int64_t cs, int m, int64_t Kc,
-                                                    int nkc, int ntile, C2<AT> *__restrict__ part) {
+                                                    int nkc, int kch, int ntile,
+                                                    C2<AT> *__restrict__ part) {
     using CA = C2<AT>;
     __shared__ CA As[GK][GT + 1];
     __shared__ CA Bs[GK][GT + 1];
@@ -69,8 +70,8 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const c64 *__restrict__ P, i
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = CA{AT(0), AT(0)};
-    const int64_t kbeg = (int64_t)kc * KCH;
-    const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
+    const int64_t kbeg = (int64_t)kc * kch;
+    const int64_t kend = std::min<int64_t>(Kc, kbeg + kch);
     const bool kfast = ks == 1;
     for (int64_t k0 = kbeg; k0 < kend; k0 += GK) {
 #pragma unroll
@@ -160,7 +161,8 @@ typedef double md4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_gram_mfma(const c64 *__restrict__ P, int64_t pstride,
                                                    int64_t ks, int64_t cs, int m, int64_t Kc,
-                                                   int nkc, int ntile, c128 *__restrict__ part,
+                                                   int nkc, int kch, int ntile,
+                                                   c128 *__restrict__ part,
                                                    c128 *__restrict__ G, int conj_out,
                                                    int *__restrict__ err) {
     __shared__ double Ar[GK][GT + 2], Ai[GK][GT + 2];
@@ -190,8 +192,8 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const c64 *__restrict__ P, in
             accr[i][j] = md4{0.0, 0.0, 0.0, 0.0};
             acci[i][j] = md4{0.0, 0.0, 0.0, 0.0};
         }
-    const int64_t kbeg = (int64_t)kc * KCH;
-    const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
+    const int64_t kbeg = (int64_t)kc * kch;
+    const int64_t kend = std::min<int64_t>(Kc, kbeg + kch);
     const bool kfast = ks == 1;
     for (int64_t k0 = kbeg; k0 < kend; k0 += GK) {
 #pragma unroll
@@ -1218,7 +1220,7 @@ __global__ __launch_bounds__(256) void k_split_rows(const c64 *__restrict__ X, i
 __global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ Wh,
                                                    const unsigned *__restrict__ Wl,
                                                    int64_t pstride, int64_t cs, int m, int64_t Kc,
-                                                   int nkc, int ntile, int vec,
+                                                   int nkc, int kch, int ntile, int vec,
                                                    c128 *__restrict__ part, c128 *__restrict__ G,
                                                    int *__restrict__ err) {
     __shared__ __align__(16) unsigned Sh[2][GT][BLD];  // [A cols a, B cols b][col][k]
@@ -1268,8 +1270,8 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ 
             dr[i][j] = md4{0.0, 0.0, 0.0, 0.0};
             di[i][j] = md4{0.0, 0.0, 0.0, 0.0};
         }
-    const int64_t kbeg = (int64_t)kc * KCH;
-    const int64_t kend = std::min<int64_t>(Kc, kbeg + KCH);
+    const int64_t kbeg = (int64_t)kc * kch;
+    const int64_t kend = std::min<int64_t>(Kc, kbeg + kch);
     u4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
     auto fetch = [&](int64_t k0) {
         const int64_t k = k0 + lk;
@@ -1598,6 +1600,7 @@ __global__ void k_m1_nucnorm(const c64 *__restrict__ P, int64_t len, int64_t Np,
 
 struct SvstWork {
     int64_t m, nkc, chunk;
+    int kch;    // contraction rows per Gram workgroup
     c128 *VW;   // panel V, W of the blocked tridiagonalisation
     void *part;  // partial Gram tiles, C2<AT>
     c128 *G;
@@ -1627,7 +1630,8 @@ static size_t svst_bytes_per_patch(int64_t m, int64_t nkc) {
 static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     SvstWork w{};
     w.m = m;
-    w.nkc = cdiv(Kc, KCH);
+    w.kch = ctx->tuning.gram_chunk > 0 ? ctx->tuning.gram_chunk : KCH;
+    w.nkc = cdiv(Kc, w.kch);
     size_t budget = (size_t)4 << 30;
     if (const char *s = getenv("MSLR_SVST_WORK_MB")) {
         long mb = atol(s);
@@ -1704,16 +1708,16 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
             // one contraction chunk: the tiles write the Hermitian matrices directly
             c128 *Gd = w.nkc == 1 ? w.G : nullptr;
             if (Wh && ks == 1)  // split words of row-contiguous patches
-                k_gram_bf16<<<gg, 256, 0, ctx->stream>>>(Wh, Wl, p * Nt, cs, m, Kc, (int)w.nkc,
+                k_gram_bf16<<<gg, 256, 0, ctx->stream>>>(Wh, Wl, p * Nt, cs, m, Kc, (int)w.nkc, w.kch,
                                                          ntile, (p % 4) == 0 ? 1 : 0,
                                                          (c128 *)w.part, Gd, err_dev);
             else
-                k_gram_mfma<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc,
+                k_gram_mfma<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, w.kch,
                                                          ntile, (c128 *)w.part, Gd, left ? 1 : 0,
                                                          err_dev);
         }
         else
-            k_gram_tiles<AT><<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc,
+            k_gram_tiles<AT><<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, w.kch,
                                                           ntile, (C2<AT> *)w.part);
         HIP_LAUNCH_CHECK();
     }
@@ -1863,7 +1867,9 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
             // (Splitting a patch's eigenvalues over a 16-lane and a 4-lane launch
             // serialises the two latencies for patches keeping more than the first
             // launch covers; measured slower beside the other scales' kernels.)
-            if (nb * m <= 64 * 1024) {
+            const int64_t lim = ctx->tuning.eig_lanes_limit == 0 ? 64 * 1024
+                                                                   : ctx->tuning.eig_lanes_limit;
+            if (nb * m <= lim) {
                 dim3 ge((unsigned)nb, (unsigned)cdiv(m * 16, 64));
                 k_eig_vectors<16><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
                                                               w.info, w.r, w.lam, w.Z, w.U, 0);
@@ -1894,7 +1900,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         // 2 p m^2 + m^2 r); the others W = U_r diag(f) U_r^H, then Y = X W.  The
         // choice is per patch, on the device, from r.
         const int half = mi / 2;
-        const bool lowrank = !left && half >= 1 && svst_lowrank_apply();
+        const bool lowrank = !left && half >= 1 && ctx->tuning.svst_apply == 0 && svst_lowrank_apply();
         const RankSel dense = lowrank ? RankSel{w.r, mi, 1} : RankSel{};
         // W = (U diag f) U^H ; r < 0 (beta = 0) gives the identity, r = 0 gives 0
         cgemm(ctx, "form_w", w.U, m * m, 1, m, w.U, m * m, m, 1, 1, w.f, m, Wm, m * m, 1, m, mi, mi,

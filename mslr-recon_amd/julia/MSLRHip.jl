@@ -14,7 +14,7 @@ module MSLRHip
 export HipContext, HipArray, Asense_hip, img2patches_hip, patches2img_hip, SVST_hip,
        patchSVST_hip, patch_nucnorm_hip, pogm_hip, poweriter_hip, upload, download,
        set_products!, SVST_vjp_hip, pogm_checkpointed_hip, lambda_vjp_hip, pogm_lambda_grad_hip,
-       tSNR_hip, detrend_hip!
+       tSNR_hip, detrend_hip!, set_tuning!, get_tuning
 
 const LIB = get(ENV, "MSLR_HIP_LIB", joinpath(@__DIR__, "..", "mslr_hip", "libmslr_hip.so"))
 
@@ -59,6 +59,37 @@ function set_products!(ctx::HipContext, products::Symbol)
            throw(ArgumentError("products must be :fp32 or :split_bf16"))
     check(ccall((:mslr_ctx_set_products, LIB), Cint, (Ptr{Cvoid}, Cint), ctx.h, code))
     return ctx
+end
+
+# mslr_tuning (include/mslr.h): launch shapes of the batched SVST
+struct Tuning
+    gram_chunk::Cint
+    eig_lanes_limit::Cint
+    svst_apply::Cint
+    reserved::NTuple{5,Cint}
+end
+
+"""
+    set_tuning!(ctx; gram_chunk=0, eig_lanes_limit=0, svst_apply=0)
+
+Launch shapes of the batched SVST on `ctx` (0 keeps the built-in choice): rows per
+Gram workgroup (a multiple of 256 in [256, 65536]), the patches x m limit for 16 lanes
+per eigenvector (-1: always 4), `svst_apply = 1` for the dense apply.  The Python
+autotuner (`mslr_hip.autotune.autotune_pogm`) picks them from timed POGM iterations;
+its JSON cache holds the values to pass here.
+"""
+function set_tuning!(ctx::HipContext; gram_chunk::Integer = 0, eig_lanes_limit::Integer = 0,
+                     svst_apply::Integer = 0)
+    t = Ref(Tuning(gram_chunk, eig_lanes_limit, svst_apply, ntuple(_ -> Cint(0), 5)))
+    check(ccall((:mslr_ctx_set_tuning, LIB), Cint, (Ptr{Cvoid}, Ref{Tuning}), ctx.h, t))
+    return ctx
+end
+
+function get_tuning(ctx::HipContext)
+    t = Ref(Tuning(0, 0, 0, ntuple(_ -> Cint(0), 5)))
+    check(ccall((:mslr_ctx_get_tuning, LIB), Cint, (Ptr{Cvoid}, Ref{Tuning}), ctx.h, t))
+    return (gram_chunk = Int(t[].gram_chunk), eig_lanes_limit = Int(t[].eig_lanes_limit),
+            svst_apply = Int(t[].svst_apply))
 end
 
 "Column-major device array owned by the library (ComplexF32 unless stated)."

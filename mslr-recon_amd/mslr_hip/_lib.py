@@ -81,6 +81,15 @@ class mslr_pogm_desc(C.Structure):
     ]
 
 
+class mslr_tuning(C.Structure):
+    _fields_ = [
+        ("gram_chunk", C.c_int),
+        ("eig_lanes_limit", C.c_int),
+        ("svst_apply", C.c_int),
+        ("reserved", C.c_int * 5),
+    ]
+
+
 _i64 = C.c_int64
 _p = C.c_void_p
 _pi64 = C.POINTER(C.c_int64)
@@ -97,6 +106,8 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_ctx_destroy": (C.c_int, [_p]),
     "mslr_ctx_sync": (C.c_int, [_p]),
     "mslr_ctx_set_products": (C.c_int, [_p, C.c_int]),
+    "mslr_ctx_set_tuning": (C.c_int, [_p, C.POINTER(mslr_tuning)]),
+    "mslr_ctx_get_tuning": (C.c_int, [_p, C.POINTER(mslr_tuning)]),
     "mslr_ctx_info": (C.c_int, [_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "mslr_malloc": (C.c_int, [_p, C.c_size_t, C.POINTER(_p)]),
     "mslr_free": (C.c_int, [_p, _p]),

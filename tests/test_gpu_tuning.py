@@ -1,0 +1,113 @@
+"""SVST launch shapes (mslr_ctx_set_tuning) and the autotuner.  Every shape gives
+LAPACK's SVST to the bound of its products (2e-5 FP32, 1e-4 split BF16) and the
+built-in shape's result to rounding; the tuner times every candidate of its
+descent, keeps only candidates whose iterate stays within its tolerance, leaves
+the winner set on the context and caches it per workload."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_err
+from problems import prepared
+from test_gpu_svst import APPLY_RANKS, _beta_between, lowrank, mixed_rank_batch
+
+pytestmark = pytest.mark.gpu
+
+BUILTIN = {"gram_chunk": 0, "eig_lanes_limit": 0, "svst_apply": 0}
+
+
+@pytest.fixture
+def shapes(ctx):
+    ctx.set_tuning()
+    yield ctx
+    ctx.set_tuning()
+    ctx.set_products("fp32")
+
+
+def test_tuning_roundtrip_and_validation(shapes):
+    assert shapes.tuning() == BUILTIN
+    want = {"gram_chunk": 2048, "eig_lanes_limit": -1, "svst_apply": 1}
+    shapes.set_tuning(**want)
+    assert shapes.tuning() == want
+    for bad in [dict(gram_chunk=100), dict(gram_chunk=128), dict(gram_chunk=1 << 17),
+                dict(gram_chunk=-256), dict(eig_lanes_limit=-2), dict(svst_apply=2)]:
+        with pytest.raises(ValueError):
+            shapes.set_tuning(**bad)
+        assert shapes.tuning() == want
+
+
+@pytest.mark.parametrize("products,bound", [("fp32", 2e-5), ("split_bf16", 1e-4)])
+@pytest.mark.parametrize("chunk", [256, 2048, 16384])
+def test_gram_chunk(shapes, rng, products, bound, chunk):
+    """A tall patch (5000 rows: 20 chunks of 256 or one of 16384) and a batch of
+    1000-row patches, against the built-in 4096-row chunks and LAPACK."""
+    from mslr_hip import recon
+    shapes.set_products(products)
+    X = lowrank(rng, 5000, 60, 4)
+    beta = _beta_between(X, 4)
+    P, betas = mixed_rank_batch(rng, [0, 3, 40], p=1000, n=120)
+    Y0 = recon.SVST(X, beta).to_host()
+    B0 = recon.svst_batched(P, betas).to_host()
+    shapes.set_tuning(gram_chunk=chunk)
+    Y1 = recon.SVST(X, beta).to_host()
+    B1 = recon.svst_batched(P, betas).to_host()
+    assert rel_err(Y1, Y0) < 1e-5
+    assert rel_err(B1, B0) < 1e-5
+    assert rel_err(Y1, O.SVST(X.astype(np.complex128), np.float32(beta))) < bound
+    for i in range(P.shape[2]):
+        Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
+        assert rel_err(B1[:, :, i], Yo) < bound, i
+
+
+@pytest.mark.parametrize("tuning", [dict(eig_lanes_limit=-1), dict(eig_lanes_limit=1 << 30),
+                                    dict(svst_apply=1), dict(eig_lanes_limit=-1, svst_apply=1)])
+def test_eig_lanes_and_apply(shapes, rng, tuning):
+    """Both eigenvector kernels and both apply forms on ranks either side of the
+    low-rank boundary (2 r <= m)."""
+    from mslr_hip import recon
+    P, betas = mixed_rank_batch(rng, APPLY_RANKS)
+    Y0 = recon.svst_batched(P, betas).to_host()
+    shapes.set_tuning(**tuning)
+    Y1 = recon.svst_batched(P, betas).to_host()
+    assert rel_err(Y1, Y0) < 1e-5
+    for i, k in enumerate(APPLY_RANKS):
+        Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
+        assert rel_err(Y1[:, :, i], Yo) < 3e-5, (i, k)
+
+
+def test_autotune_pogm(shapes, tmp_path):
+    from mslr_hip.autotune import autotune_pogm, descent_order
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.sense import SenseOp
+    cfg, smaps, ksp, omega, sf = prepared("small")
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    S = len(cfg["patch_sizes"])
+    A = SenseOp(omega, smaps)
+    kw = dict(patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=S * 1.0)
+    cache = tmp_path / "tuning.json"
+    print("autotune small", flush=True)
+    r = autotune_pogm(A, ksp, form="mixed", warmup=1, iters=2, cache=cache, **kw)
+    assert not r["cached"] and len(r["trials"]) == 1 + len(descent_order())
+    # every candidate shape gives the built-in iterate to rounding
+    for t, ms, err, ok in r["trials"]:
+        assert ms > 0 and err < 1e-5, (t, err)
+    assert shapes.tuning() == r["tuning"]
+    kept = [t for t, _, _, ok in r["trials"][1:] if ok]
+    assert r["tuning"] == (dict(BUILTIN, **kept[-1]) if kept else BUILTIN)
+    # a second call on the same workload reads the cache and sets the same shapes
+    shapes.set_tuning()
+    r2 = autotune_pogm(A, ksp, form="mixed", cache=cache, **kw)
+    assert r2["cached"] and r2["tuning"] == r["tuning"] and shapes.tuning() == r["tuning"]
+    # a run under the tuned shapes against one under the built-in shapes
+    out = []
+    for t in (r["tuning"], BUILTIN):
+        shapes.set_tuning(**t)
+        st = POGM(A, niter=4, form="mixed", **kw)
+        st.init(ksp)
+        st.run(4)
+        out.append((st.X.to_host(), st.log()))
+        st.close()
+    A.close()
+    assert out[0][1] == out[1][1]
+    assert rel_err(out[0][0], out[1][0]) < 1e-5
