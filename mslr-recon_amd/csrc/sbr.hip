@@ -31,6 +31,7 @@ constexpr int B = SBR_B;
 constexpr int S1T = 512;       // threads of the dense -> band kernel
 constexpr int S2T = 768;       // threads of the band -> tridiagonal kernel
 constexpr int S2W = S2T / 64;  // concurrent sweeps
+constexpr int S2TG = 1024;     // threads of the band -> tridiagonal kernel, band in global memory
 constexpr int STAGGER = 2;     // time steps between the starts of consecutive sweeps
 constexpr int BTT = 256;       // threads of the back-transformation
 constexpr int BTC = 13;        // eigenvectors per back-transformation workgroup: one thread per
@@ -430,8 +431,8 @@ __device__ __forceinline__ c128 bherm(c128 *Bd, int m, int st, int q, int r) {
 // rest of E.  D and E stay in registers for the whole task: lane l holds
 // column r = l % 16 of rows q = l / 16 + 4t, t = 0..3; row products are DPP
 // row sums, column products sums over the four rows of lanes.
-__device__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw, c128 *sx, c128 tau,
-                           c128 *V2, c128 *tau2, int ks) {
+__device__ __forceinline__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw,
+                                           c128 *sx, c128 tau, c128 *V2, c128 *tau2, int ks) {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
     const int st = s + 1 + k * B;
@@ -569,17 +570,25 @@ __device__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *sv, c128 *sw, c1
     return tau;
 }
 
-__global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, int m, int ks,
-                                               double *__restrict__ dall, double *__restrict__ eall,
-                                               c128 *__restrict__ V2all,
-                                               c128 *__restrict__ tau2all) {
+// NT threads (NT / 64 concurrent sweeps).  GBAND: the band in global memory
+// (Bglob, 2B x m per matrix; L2-resident for the matrices in flight) for sizes
+// whose band does not fit in LDS; the workgroup's waves are on one CU, so the
+// same wave- and workgroup-level ordering holds as for LDS.
+template <int NT, bool GBAND>
+__global__ __launch_bounds__(NT) void k_sb2st(const c128 *__restrict__ Gall, int m, int ks,
+                                              double *__restrict__ dall, double *__restrict__ eall,
+                                              c128 *__restrict__ V2all,
+                                              c128 *__restrict__ tau2all, c128 *__restrict__ Bglob) {
+    constexpr int NW = NT / 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    c128 *Bd = reinterpret_cast<c128 *>(smem_raw);  // 2B x m
-    c128 *scr = Bd + (size_t)2 * B * m;              // per wave: sv[B], sw[B], sx[B]
     const int ip = blockIdx.x;
+    c128 *Bd = GBAND ? Bglob + (int64_t)ip * 2 * B * m
+                     : reinterpret_cast<c128 *>(smem_raw);  // 2B x m
+    // per wave: sv[B], sw[B], sx[B]
+    c128 *scr = GBAND ? reinterpret_cast<c128 *>(smem_raw) : Bd + (size_t)2 * B * m;
     const int tid = threadIdx.x, wave = tid >> 6;
     const c128 *A = Gall + (int64_t)ip * m * m;
-    for (int idx = tid; idx < 2 * B * m; idx += S2T) {
+    for (int idx = tid; idx < 2 * B * m; idx += NT) {
         const int j = idx / (2 * B), dg = idx % (2 * B);
         Bd[idx] = ld_if(dg <= B && j + dg < m, A + (j + dg) + (int64_t)m * j);
     }
@@ -599,14 +608,14 @@ __global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, in
             const int nst = (m - s - 1 + B - 1) / B;
             if (k >= 0 && k < nst) {
                 tau = sb2st_task(Bd, m, s, k, sv, sw, sx, tau, V2, tau2, ks);
-                if (k == nst - 1) s += S2W;
+                if (k == nst - 1) s += NW;
             }
         }
         __syncthreads();
     }
     double *d = dall + (int64_t)ip * m;
     double *e = eall + (int64_t)ip * m;
-    for (int i = tid; i < m; i += S2T) {
+    for (int i = tid; i < m; i += NT) {
         d[i] = Bd[i * 2 * B].re;
         e[i] = i + 1 < m ? Bd[i * 2 * B + 1].re : 0.0;
     }
@@ -622,8 +631,15 @@ __global__ __launch_bounds__(S2T) void k_sb2st(const c128 *__restrict__ Gall, in
 // the panels in reverse order, as U <- U - V (T (V^H U)) with V (unit lower
 // form, tails stored below the band of the reduced matrix) passed through LDS
 // in 16 x 16 tiles, again loading the next tile during the current one.
-constexpr int BT_KS = (SBR_MAXM - 1 + B - 1) / B;  // reflectors per sweep, at most
+// KS: reflectors per sweep, at most (the staging of a sweep in LDS; the smaller
+// instance keeps two workgroups per CU up to m = SBR_MAXM).
+constexpr int BT_KS_LDS = (SBR_MAXM - 1 + B - 1) / B;
+constexpr int BT_KS_MAX = (SBR_MAXG - 1 + B - 1) / B;
+constexpr int BT_KP = BTT / BTC;  // reflectors applied at once
+static_assert(BT_KS_MAX * B <= 2 * BTT, "a sweep's reflectors are staged two per thread");
+static_assert(BT_KS_LDS * B >= B * B, "the staging buffer also holds a 16 x 16 tile");
 
+template <int KS>
 __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, int m, int npanel,
                                                 int ks, const c128 *__restrict__ Tall,
                                                 const c128 *__restrict__ V2all,
@@ -634,8 +650,8 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ c128 Ts[B][B + 1];
     __shared__ c128 XY[B][BTC + 1];
-    __shared__ c128 Vb[2][BT_KS * B];  // a sweep's reflectors / a 16 x 16 tile of V
-    __shared__ c128 Tb[2][BT_KS];
+    __shared__ c128 Vb[2][KS * B];  // a sweep's reflectors / a 16 x 16 tile of V
+    __shared__ c128 Tb[2][KS];
     const int ip = blockIdx.x;
     const int r = rall[ip];
     const int c0 = blockIdx.y * BTC;
@@ -667,10 +683,11 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         };
         auto put = [&](int buf) {
             Vb[buf][tid] = pv0;
-            if (tid + BTT < BT_KS * B) Vb[buf][tid + BTT] = pv1;
-            if (tid < BT_KS) Tb[buf][tid] = pt;
+            if (tid + BTT < KS * B) Vb[buf][tid + BTT] = pv1;
+            if (tid < KS) Tb[buf][tid] = pt;
         };
-        const int k = tid / BTC, c = tid - k * BTC;  // (reflector, eigenvector) of this thread
+        // (reflector, eigenvector) of this thread; reflectors k, k + BT_KP, ...
+        const int k = tid / BTC, c = tid - k * BTC;
         if (m >= 2) {
             fetch(m - 2);
             put(0);
@@ -679,11 +696,12 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         for (int s = m - 2, buf = 0; s >= 0; --s, buf ^= 1) {
             if (s > 0) fetch(s - 1);
             const int nst = nsweep_refl(s);
-            const c128 tk = Tb[buf][k < BT_KS ? k : 0];
-            if (k < nst && c < ncol && nonzero(tk)) {
-                const int st = s + 1 + k * B;
+            for (int kk = k; k < BT_KP && kk < nst; kk += BT_KP) {
+                const c128 tk = Tb[buf][kk];
+                if (!(c < ncol && nonzero(tk))) continue;
+                const int st = s + 1 + kk * B;
                 const int len = min(B, m - st);
-                const c128 *v = &Vb[buf][k * B];
+                const c128 *v = &Vb[buf][kk * B];
                 c128 *u = Uc + c * m + st;
                 c128 vv[B], uu[B];
 #pragma unroll
@@ -790,8 +808,12 @@ static void raise_lds(const void *fn, size_t need) {
 void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, double *d,
                 double *e) {
     MSLR_REQUIRE(sbr_eligible(m), MSLR_ERR_UNSUPPORTED, "two-stage reduction size");
-    // a wave must finish its sweep before its next one (S2W sweeps later) starts
-    MSLR_REQUIRE(sbr_steps(m) <= STAGGER * S2W, MSLR_ERR_UNSUPPORTED, "sweep longer than the schedule");
+    const bool gband = sbr_global_band(m);
+    // a wave must finish its sweep before its next one (one sweep per wave later)
+    // starts
+    MSLR_REQUIRE(sbr_steps(m) <= STAGGER * (gband ? S2TG / 64 : S2W), MSLR_ERR_UNSUPPORTED,
+                 "sweep longer than the schedule");
+    MSLR_REQUIRE(!gband || w.Band, MSLR_ERR_DEVICE, "no band workspace");
     const int npanel = sbr_panels(m), ks = sbr_steps(m);
     {
         const size_t lds = (size_t)(m - B) * B * sizeof(c128);
@@ -800,11 +822,19 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
         k_sy2sb<<<(unsigned)nb, S1T, lds, ctx->stream>>>(G, m, npanel, w.T1, w.W1);
         HIP_LAUNCH_CHECK();
     }
-    {
-        const size_t lds = ((size_t)2 * B * m + (size_t)S2W * 3 * B) * sizeof(c128);
-        raise_lds((const void *)k_sb2st, lds);
+    if (gband) {
+        const size_t lds = (size_t)(S2TG / 64) * 3 * B * sizeof(c128);
+        raise_lds((const void *)k_sb2st<S2TG, true>, lds);
         MSLR_PROF(ctx, "sb2st");
-        k_sb2st<<<(unsigned)nb, S2T, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2);
+        k_sb2st<S2TG, true><<<(unsigned)nb, S2TG, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2,
+                                                                       w.Band);
+        HIP_LAUNCH_CHECK();
+    } else {
+        const size_t lds = ((size_t)2 * B * m + (size_t)S2W * 3 * B) * sizeof(c128);
+        raise_lds((const void *)k_sb2st<S2T, false>, lds);
+        MSLR_PROF(ctx, "sb2st");
+        k_sb2st<S2T, false><<<(unsigned)nb, S2T, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2,
+                                                                      nullptr);
         HIP_LAUNCH_CHECK();
     }
 }
@@ -813,10 +843,15 @@ void sbr_backtransform(mslr_ctx_s *ctx, const c128 *G, int m, int64_t nb, const 
                        const double *Z, const int *r, c64 *U) {
     const int npanel = sbr_panels(m), ks = sbr_steps(m);
     const size_t lds = (size_t)BTC * m * sizeof(c128);
-    raise_lds((const void *)k_sbr_bt, lds);
+    MSLR_REQUIRE(ks <= BT_KS_MAX, MSLR_ERR_UNSUPPORTED, "two-stage back-transformation size");
+    const bool small = ks <= BT_KS_LDS;
+    raise_lds(small ? (const void *)k_sbr_bt<BT_KS_LDS> : (const void *)k_sbr_bt<BT_KS_MAX>, lds);
     dim3 g((unsigned)nb, (unsigned)cdiv(m, BTC));
     MSLR_PROF(ctx, "backtransform");
-    k_sbr_bt<<<g, BTT, lds, ctx->stream>>>(G, m, npanel, ks, w.T1, w.V2, w.tau2, Z, r, U);
+    if (small)
+        k_sbr_bt<BT_KS_LDS><<<g, BTT, lds, ctx->stream>>>(G, m, npanel, ks, w.T1, w.V2, w.tau2, Z, r, U);
+    else
+        k_sbr_bt<BT_KS_MAX><<<g, BTT, lds, ctx->stream>>>(G, m, npanel, ks, w.T1, w.V2, w.tau2, Z, r, U);
     HIP_LAUNCH_CHECK();
 }
 

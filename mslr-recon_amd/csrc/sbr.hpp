@@ -1,6 +1,7 @@
 // Two-stage reduction of batched Hermitian matrices (FP64) to real tridiagonal
 // form: dense -> band of width SBR_B (block Householder, GEMM-shaped updates),
-// then band -> tridiagonal (bulge chasing in LDS), and the back-transformation
+// then band -> tridiagonal (bulge chasing; the band in LDS up to SBR_MAXM, in
+// global memory above), and the back-transformation
 // of tridiagonal eigenvectors through both stages.
 #pragma once
 
@@ -10,9 +11,12 @@ namespace mslr {
 
 constexpr int SBR_B = 16;       // bandwidth of the intermediate band matrix
 constexpr int SBR_MINM = 64;    // smaller matrices use the one-stage reduction
-constexpr int SBR_MAXM = 300;   // the band (2*SBR_B diagonals) must fit in LDS
+constexpr int SBR_MAXM = 300;   // the band (2*SBR_B diagonals) fits in LDS up to here
+constexpr int SBR_MAXG = 480;   // band in global memory up to here (sweep schedule and
+                                // the back-transformation's reflector staging)
 
-inline bool sbr_eligible(int64_t m) { return m >= SBR_MINM && m <= SBR_MAXM; }
+inline bool sbr_eligible(int64_t m) { return m >= SBR_MINM && m <= SBR_MAXG; }
+inline bool sbr_global_band(int64_t m) { return m > SBR_MAXM; }
 // panels of the dense -> band stage
 inline int sbr_panels(int m) {
     int n = 0;
@@ -31,11 +35,13 @@ struct SbrWork {
     c128 *V2 = nullptr;    // (m, sbr_steps(m), B) per matrix: bulge-chasing reflectors
     c128 *tau2 = nullptr;  // (m, sbr_steps(m)) per matrix
     c128 *W1 = nullptr;    // sbr_w1_elems(m) per matrix
+    c128 *Band = nullptr;  // (2B, m) per matrix when sbr_global_band(m)
 };
 
 inline size_t sbr_bytes_per_matrix(int m) {
     return (size_t)sbr_panels(m) * SBR_B * SBR_B * 16 + (size_t)m * sbr_steps(m) * SBR_B * 16 +
-           (size_t)m * sbr_steps(m) * 16 + sbr_w1_elems(m) * 16;
+           (size_t)m * sbr_steps(m) * 16 + sbr_w1_elems(m) * 16 +
+           (sbr_global_band(m) ? (size_t)2 * SBR_B * m * 16 : 0);
 }
 
 // G (nb matrices, m x m, full Hermitian storage) -> d, e (real tridiagonal).

@@ -1659,13 +1659,16 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     size_t o_VW = off; off += al((size_t)chunk * 2 * TNB * m * 16);
     size_t o_info = off; off += al((size_t)chunk * EIG_INFO * 8);
     w.sbr = use_sbr(m);
-    size_t o_T1 = off, o_V2 = off, o_tau2 = off, o_W1 = off;
+    size_t o_T1 = off, o_V2 = off, o_tau2 = off, o_W1 = off, o_band = off;
     if (w.sbr) {
         const int mi = (int)m;
         o_T1 = off; off += al((size_t)chunk * sbr_panels(mi) * SBR_B * SBR_B * 16);
         o_V2 = off; off += al((size_t)chunk * m * sbr_steps(mi) * SBR_B * 16);
         o_tau2 = off; off += al((size_t)chunk * m * sbr_steps(mi) * 16);
         o_W1 = off; off += al((size_t)chunk * sbr_w1_elems(mi) * 16);
+        if (sbr_global_band(m)) {
+            o_band = off; off += al((size_t)chunk * 2 * SBR_B * m * 16);
+        }
     }
     char *base = (char *)ctx->scratch_get(SLOT_GRAM, off);
     w.part = (void *)(base + o_part);
@@ -1686,6 +1689,7 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
         w.sw.V2 = (c128 *)(base + o_V2);
         w.sw.tau2 = (c128 *)(base + o_tau2);
         w.sw.W1 = (c128 *)(base + o_W1);
+        if (sbr_global_band(m)) w.sw.Band = (c128 *)(base + o_band);
     }
     return w;
 }

@@ -174,14 +174,16 @@ def test_patch_svst_vs_oracle(ctx, rng, N, ps, ss):
 
 
 # Gram sizes handled by the two-stage (dense -> band -> tridiagonal) reduction:
-# 64 <= min(p, Nt) <= 300; the production scales use m = Nt = 300.
+# 64 <= min(p, Nt) <= 480, the band in LDS up to 300 and in global memory above;
+# the production scales use m = Nt = 300 and 387.
 @pytest.fixture
 def two_stage(monkeypatch):
     monkeypatch.setenv("MSLR_SVST_REDUCTION", "two")
 
 
 @pytest.mark.parametrize("p,n,r", [(64, 200, 3), (100, 77, 4), (1000, 299, 7), (300, 1000, 5),
-                                   (4000, 300, 12), (300, 300, 150)])
+                                   (4000, 300, 12), (300, 300, 150), (301, 301, 9), (1000, 387, 9),
+                                   (387, 2000, 6), (5000, 450, 5), (480, 480, 200)])
 def test_two_stage_sizes_match_lapack(ctx, rng, two_stage, p, n, r):
     from mslr_hip import recon
     X = lowrank(rng, p, n, r, noise=1e-2)
@@ -191,11 +193,12 @@ def test_two_stage_sizes_match_lapack(ctx, rng, two_stage, p, n, r):
     assert rel_err(Y, Yo) < 2e-5
 
 
-def test_two_stage_batch_with_mixed_ranks(ctx, rng, two_stage):
-    """A batch of production-shaped patches (1000 x 300) with different ranks kept."""
+@pytest.mark.parametrize("n", [300, 387])
+def test_two_stage_batch_with_mixed_ranks(ctx, rng, two_stage, n):
+    """A batch of production-shaped patches (1000 x Nt) with different ranks kept."""
     from mslr_hip import recon
-    p, n, Np = 1000, 300, 5
-    ranks = [0, 1, 3, 40, 299]
+    p, Np = 1000, 5
+    ranks = [0, 1, 3, 40, n - 1]
     P = np.stack([lowrank(rng, p, n, max(1, k), noise=1e-2) for k in ranks], axis=2)
     P = np.asfortranarray(P)
     betas = []
@@ -208,7 +211,7 @@ def test_two_stage_batch_with_mixed_ranks(ctx, rng, two_stage):
         assert rel_err(Y[:, :, i], Yo) < 3e-5, (i, ranks[i])
 
 
-@pytest.mark.parametrize("p,n", [(4000, 300), (500, 120)])
+@pytest.mark.parametrize("p,n", [(4000, 300), (500, 120), (3000, 387)])
 def test_two_stage_and_one_stage_agree(ctx, rng, p, n, monkeypatch):
     """Both tridiagonalisations give the same SVST (high dynamic range input)."""
     from mslr_hip import recon
@@ -228,16 +231,17 @@ def test_two_stage_and_one_stage_agree(ctx, rng, p, n, monkeypatch):
     assert rel_err(Y2, Y1) < 2e-5
 
 
-def test_two_stage_nucnorm(ctx, rng, two_stage):
+@pytest.mark.parametrize("p,n", [(400, 150), (1000, 387)])
+def test_two_stage_nucnorm(ctx, rng, two_stage, p, n):
     from mslr_hip import recon
-    P = np.asfortranarray(np.stack([lowrank(rng, 400, 150, 5, noise=1e-2) for _ in range(3)], axis=2))
+    P = np.asfortranarray(np.stack([lowrank(rng, p, n, 5, noise=1e-2) for _ in range(3)], axis=2))
     got = recon.patch_nucnorm(P)
     ref = sum(np.linalg.svd(P[:, :, i].astype(np.complex128), compute_uv=False).sum() for i in range(3))
     assert abs(got - ref) / ref < 1e-6
 
 
 @pytest.mark.parametrize("reduction", ["one", "two"])
-@pytest.mark.parametrize("p,n,r", [(300, 300, 6), (1000, 300, 20), (216, 387, 5)])
+@pytest.mark.parametrize("p,n,r", [(300, 300, 6), (1000, 300, 20), (216, 387, 5), (1000, 387, 20)])
 def test_both_reductions_match_lapack(ctx, rng, monkeypatch, reduction, p, n, r):
     """The one-stage (column by column) and two-stage (band) tridiagonalisations."""
     from mslr_hip import recon
