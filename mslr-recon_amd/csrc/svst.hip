@@ -1632,7 +1632,10 @@ static SvstWork svst_work(mslr_ctx_s *ctx, int64_t m, int64_t Kc, int64_t Np) {
     w.m = m;
     w.kch = ctx->tuning.gram_chunk > 0 ? ctx->tuning.gram_chunk : KCH;
     w.nkc = cdiv(Kc, w.kch);
-    size_t budget = (size_t)4 << 30;
+    // patches per chunk from a 16 GB workspace (per stream): the 3179 m = 387
+    // patches of a 10^3 scale at Nt = 387 in one chunk instead of nine (each chunk's
+    // launches end in a partly idle round)
+    size_t budget = (size_t)16 << 30;
     if (const char *s = getenv("MSLR_SVST_WORK_MB")) {
         long mb = atol(s);
         if (mb > 0) budget = (size_t)mb << 20;
