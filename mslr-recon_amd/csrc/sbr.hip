@@ -31,7 +31,21 @@ constexpr int B = SBR_B;
 constexpr int S1T = 512;       // threads of the dense -> band kernel
 constexpr int S2T = 768;       // threads of the band -> tridiagonal kernel
 constexpr int S2W = S2T / 64;  // concurrent sweeps
-constexpr int S2TG = 1024;     // threads of the band -> tridiagonal kernel, band in global memory
+// the band -> tridiagonal kernel with the band in global memory: threads and
+// time steps between sweep starts (a wave's sweep, up to 30 tasks at m = 480, must
+// end before its next one starts: tasks <= stagger x waves).  12 waves 3 steps
+// apart (170 VGPRs, no spills): c2 1.58 iter/s against 1.49 with 16 waves 2 steps
+// apart (128 VGPRs, ~40 spilled).
+#ifndef MSLR_SB2ST_G_THREADS
+#define MSLR_SB2ST_G_THREADS 768
+#endif
+#ifndef MSLR_SB2ST_G_STAGGER
+#define MSLR_SB2ST_G_STAGGER 3
+#endif
+constexpr int S2TG = MSLR_SB2ST_G_THREADS;
+constexpr int STAGGER_G = MSLR_SB2ST_G_STAGGER;
+static_assert((SBR_MAXG - 1 + B - 1) / B <= STAGGER_G * (S2TG / 64),
+              "a sweep must end before the wave's next one starts");
 constexpr int STAGGER = 2;     // time steps between the starts of consecutive sweeps
 constexpr int BTT = 256;       // threads of the back-transformation
 constexpr int BTC = 13;        // eigenvectors per back-transformation workgroup: one thread per
@@ -574,7 +588,7 @@ __device__ __forceinline__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *
 // (Bglob, 2B x m per matrix; L2-resident for the matrices in flight) for sizes
 // whose band does not fit in LDS; the workgroup's waves are on one CU, so the
 // same wave- and workgroup-level ordering holds as for LDS.
-template <int NT, bool GBAND>
+template <int NT, bool GBAND, int STG>
 __global__ __launch_bounds__(NT) void k_sb2st(const c128 *__restrict__ Gall, int m, int ks,
                                               double *__restrict__ dall, double *__restrict__ eall,
                                               c128 *__restrict__ V2all,
@@ -601,10 +615,10 @@ __global__ __launch_bounds__(NT) void k_sb2st(const c128 *__restrict__ Gall, int
     c128 tau = {0.0, 0.0};
     int s = wave;  // this wave's current sweep: s, s + S2W, ...
     const int nsweep = m - 1;
-    const int tend = STAGGER * (nsweep - 1) + 1;
+    const int tend = STG * (nsweep - 1) + 1;
     for (int t = 0; t < tend; ++t) {
         if (s < nsweep) {
-            const int k = t - STAGGER * s;
+            const int k = t - STG * s;
             const int nst = (m - s - 1 + B - 1) / B;
             if (k >= 0 && k < nst) {
                 tau = sb2st_task(Bd, m, s, k, sv, sw, sx, tau, V2, tau2, ks);
@@ -811,7 +825,7 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
     const bool gband = sbr_global_band(m);
     // a wave must finish its sweep before its next one (one sweep per wave later)
     // starts
-    MSLR_REQUIRE(sbr_steps(m) <= STAGGER * (gband ? S2TG / 64 : S2W), MSLR_ERR_UNSUPPORTED,
+    MSLR_REQUIRE(sbr_steps(m) <= (gband ? STAGGER_G * (S2TG / 64) : STAGGER * S2W), MSLR_ERR_UNSUPPORTED,
                  "sweep longer than the schedule");
     MSLR_REQUIRE(!gband || w.Band, MSLR_ERR_DEVICE, "no band workspace");
     const int npanel = sbr_panels(m), ks = sbr_steps(m);
@@ -824,16 +838,16 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
     }
     if (gband) {
         const size_t lds = (size_t)(S2TG / 64) * 3 * B * sizeof(c128);
-        raise_lds((const void *)k_sb2st<S2TG, true>, lds);
+        raise_lds((const void *)k_sb2st<S2TG, true, STAGGER_G>, lds);
         MSLR_PROF(ctx, "sb2st");
-        k_sb2st<S2TG, true><<<(unsigned)nb, S2TG, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2,
+        k_sb2st<S2TG, true, STAGGER_G><<<(unsigned)nb, S2TG, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2,
                                                                        w.Band);
         HIP_LAUNCH_CHECK();
     } else {
         const size_t lds = ((size_t)2 * B * m + (size_t)S2W * 3 * B) * sizeof(c128);
-        raise_lds((const void *)k_sb2st<S2T, false>, lds);
+        raise_lds((const void *)k_sb2st<S2T, false, STAGGER>, lds);
         MSLR_PROF(ctx, "sb2st");
-        k_sb2st<S2T, false><<<(unsigned)nb, S2T, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2,
+        k_sb2st<S2T, false, STAGGER><<<(unsigned)nb, S2T, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2,
                                                                       nullptr);
         HIP_LAUNCH_CHECK();
     }
