@@ -31,20 +31,20 @@ HBM_PEAK_GBS = 8000.0
 # families, from the committed rocprofv3 counter summary of this workload (mixed
 # form; the SENSE, update and patch kernels are the same in every form)
 TRAFFIC_BYTES = {
-    "fwd_yz": 6949200000,
-    "adj_yz": 4399700000,
+    "fwd_yz": 7063700000,
+    "adj_yz": 4403400000,
     "pogm_update": 15163600000,
-    "fwd_x": 8933700000,
-    "adj_x": 6115000000,
-    "sum_planes": 4276900000,
-    "sy2sb": 6122600000,
-    "img2patches": 5299400000,
-    "prox_voxel": 3499400000,
-    "patches2img": 3151000000,
-    "apply": 909000000,
+    "fwd_x": 9110200000,
+    "adj_x": 6114600000,
+    "sum_planes": 4432400000,
+    "sy2sb": 6116700000,
+    "img2patches": 4661500000,
+    "prox_voxel": 3499300000,
+    "patches2img": 3153300000,
+    "apply": 908900000,
     "planes_to_image": 2479700000,
     "split": 2342200000,
-    "gram_tiles": 1868300000,
+    "gram_tiles": 1866900000,
     "backtransform": 946700000,
     "sb2st": 224800000,
 }
