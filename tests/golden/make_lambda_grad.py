@@ -1,11 +1,13 @@
 """Writes lambda_grad_<config>.json: the FP64 forward-mode oracle's loss and
 d loss / d lambda_k for the differentiation workload (MSE of the image sum after
 NITERS unrolled POGM iterations to the synthetic truth in the k-space's units), and
-with `fd` the FP64 oracle's central differences with steps 1e-3 lambda_k
-(`fd_grads`, added to the same file).
+with `fd` the FP64 oracle's central differences with steps rel * lambda_k
+(default 1e-3, for every k or the listed ones), added to the same file under
+`fd_by_step[rel]` (and, for 1e-3, `fd_grads`).
 
     python tests/golden/make_lambda_grad.py c4 10
     python tests/golden/make_lambda_grad.py c4 10 fd
+    python tests/golden/make_lambda_grad.py c4 10 fd 1e-4 1
 """
 import json
 import sys
@@ -23,7 +25,7 @@ import oracle as O  # noqa: E402
 from problems import prepared, truth  # noqa: E402
 
 
-def central_differences(name: str, niter: int, rel: float = 1e-3) -> None:
+def central_differences(name: str, niter: int, rel: float = 1e-3, ks=None) -> None:
     cfg, smaps, ksp, omega, sf = prepared(name)
     N, Nt = tuple(cfg["N"]), cfg["Nt"]
     x_ref = truth(name).astype(np.complex128) / sf
@@ -40,16 +42,20 @@ def central_differences(name: str, niter: int, rel: float = 1e-3) -> None:
         print(f"  run {lm}: {time.time() - t:.0f} s", flush=True)
         return float(np.vdot(r.ravel(), r.ravel()).real) / r.size, rs
 
-    fd = []
-    for k in range(len(lams)):
+    fd = [None] * len(lams)
+    for k in (range(len(lams)) if ks is None else ks):
         h = rel * lams[k]
         lp = list(lams); lp[k] += h
         lm = list(lams); lm[k] -= h
         (fp, rp), (fm, rm) = loss(lp), loss(lm)
         assert rp == out["restarts"] and rm == out["restarts"], "a step changed the restarts"
-        fd.append((fp - fm) / (2 * h))
-    out["fd_step_rel"] = rel
-    out["fd_grads"] = fd
+        fd[k] = (fp - fm) / (2 * h)
+    if rel == 1e-3 and ks is None:
+        out["fd_step_rel"] = rel
+        out["fd_grads"] = fd
+    by = out.setdefault("fd_by_step", {})
+    prev = by.get(repr(rel), [None] * len(lams))
+    by[repr(rel)] = [v if v is not None else w for v, w in zip(fd, prev)]
     path.write_text(json.dumps(out, indent=1) + "\n")
     print(json.dumps(out))
 
@@ -73,6 +79,8 @@ def main(name: str, niter: int) -> None:
 
 if __name__ == "__main__":
     if len(sys.argv) > 3 and sys.argv[3] == "fd":
-        central_differences(sys.argv[1], int(sys.argv[2]))
+        rel = float(sys.argv[4]) if len(sys.argv) > 4 else 1e-3
+        ks = [int(v) for v in sys.argv[5].split(",")] if len(sys.argv) > 5 else None
+        central_differences(sys.argv[1], int(sys.argv[2]), rel, ks)
     else:
         main(sys.argv[1], int(sys.argv[2]))
