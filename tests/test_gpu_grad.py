@@ -23,7 +23,8 @@ def _vdot(a, b):
     return float(np.vdot(np.asarray(a, np.complex128).ravel(), np.asarray(b, np.complex128).ravel()).real)
 
 
-@pytest.mark.parametrize("p,n,r,Np", [(200, 40, 5, 1), (1000, 60, 8, 3), (64, 64, 20, 2), (4000, 30, 3, 1)])
+@pytest.mark.parametrize("p,n,r,Np", [(200, 40, 5, 1), (1000, 60, 8, 3), (64, 64, 20, 2), (4000, 30, 3, 1),
+                                      (1000, 300, 12, 1), (1000, 387, 10, 2)])
 def test_svst_vjp_adjoint_identity(ctx, rng, p, n, r, Np):
     from mslr_hip import recon
     Z = np.asfortranarray(np.stack([lowrank(rng, p, n, r, noise=0.1) for _ in range(Np)], axis=2))
