@@ -1172,6 +1172,29 @@ __device__ __forceinline__ void ozaki_words(c64 v, float sg1, float sg2, unsigne
     wl = __builtin_bit_cast(unsigned, lv);
 }
 
+// One wave's block of up to SPL_ROWS values (v0: element lane, v1: lane + 64) as
+// Ozaki words with the block's exponent, n valid values written at W[0 .. n).
+__device__ __forceinline__ void split_block(c64 v0, c64 v1, int lane, int n, unsigned *Wh,
+                                            unsigned *Wl) {
+    float mx = fmaxf(fmaxf(fabsf(v0.re), fabsf(v0.im)), fmaxf(fabsf(v1.re), fabsf(v1.im)));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    int E;
+    (void)frexpf(mx, &E);  // mx < 2^E (E = 0 for a zero block: all slices 0)
+    const float sg1 = ldexpf(3.f, E + 14), sg2 = ldexpf(3.f, E + 6);
+    unsigned h, l;
+    if (lane < n) {
+        ozaki_words(v0, sg1, sg2, h, l);
+        Wh[lane] = h;
+        Wl[lane] = l;
+    }
+    if (lane + 64 < n) {
+        ozaki_words(v1, sg1, sg2, h, l);
+        Wh[lane + 64] = h;
+        Wl[lane + 64] = l;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_split_rows(const c64 *__restrict__ X, int64_t p,
                                                     int64_t ncol, unsigned *__restrict__ Wh,
                                                     unsigned *__restrict__ Wl) {
@@ -1184,23 +1207,31 @@ __global__ __launch_bounds__(256) void k_split_rows(const c64 *__restrict__ X, i
     const int n = (int)std::min<int64_t>(SPL_ROWS, p - blk * SPL_ROWS);
     const c64 v0 = lane < n ? X[off + lane] : c64{0.f, 0.f};
     const c64 v1 = lane + 64 < n ? X[off + lane + 64] : c64{0.f, 0.f};
-    float mx = fmaxf(fmaxf(fabsf(v0.re), fabsf(v0.im)), fmaxf(fabsf(v1.re), fabsf(v1.im)));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    int E;
-    (void)frexpf(mx, &E);  // mx < 2^E (E = 0 for a zero block: all slices 0)
-    const float sg1 = ldexpf(3.f, E + 14), sg2 = ldexpf(3.f, E + 6);
-    unsigned h, l;
-    if (lane < n) {
-        ozaki_words(v0, sg1, sg2, h, l);
-        Wh[off + lane] = h;
-        Wl[off + lane] = l;
-    }
-    if (lane + 64 < n) {
-        ozaki_words(v1, sg1, sg2, h, l);
-        Wh[off + lane + 64] = h;
-        Wl[off + lane + 64] = l;
-    }
+    split_block(v0, v1, lane, n, Wh + off, Wl + off);
+}
+
+// Patches with fewer voxels than frames (the Gram X X^H contracts over time):
+// row a of patch ip as time-contiguous words, W[(ip p + a) Nt + k], blocks of
+// SPL_ROWS times sharing an exponent, so that k_gram_bf16 reads them as it reads
+// row-contiguous words (and conjugates its result).  Consecutive waves take
+// consecutive rows, which share the cache lines of the strided reads.
+__global__ __launch_bounds__(256) void k_split_cols(const c64 *__restrict__ X, int64_t p,
+                                                    int64_t Nt, int64_t nb,
+                                                    unsigned *__restrict__ Wh,
+                                                    unsigned *__restrict__ Wl) {
+    const int64_t nblk = (Nt + SPL_ROWS - 1) / SPL_ROWS;
+    const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= nb * nblk * p) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t a = wid % p, t = wid / p;
+    const int64_t blk = t % nblk, ip = t / nblk;
+    const int64_t k0 = blk * SPL_ROWS;
+    const int n = (int)std::min<int64_t>(SPL_ROWS, Nt - k0);
+    const c64 *src = X + ip * p * Nt + a;
+    const c64 v0 = lane < n ? src[(k0 + lane) * p] : c64{0.f, 0.f};
+    const c64 v1 = lane + 64 < n ? src[(k0 + lane + 64) * p] : c64{0.f, 0.f};
+    const int64_t off = (ip * p + a) * Nt + k0;
+    split_block(v0, v1, lane, n, Wh + off, Wl + off);
 }
 
 // Split-BF16 Gram tiles with error-free sums, from the Ozaki words of
@@ -1222,7 +1253,7 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ 
                                                    int64_t pstride, int64_t cs, int m, int64_t Kc,
                                                    int nkc, int kch, int ntile, int vec,
                                                    c128 *__restrict__ part, c128 *__restrict__ G,
-                                                   int *__restrict__ err) {
+                                                   int conj_out, int *__restrict__ err) {
     __shared__ __align__(16) unsigned Sh[2][GT][BLD];  // [A cols a, B cols b][col][k]
     __shared__ __align__(16) unsigned Sl[2][GT][BLD];
     // the tile pairs of one (patch, chunk) share column tiles: keep them on one XCD
@@ -1368,7 +1399,7 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ 
             for (int v = 0; v < 4; ++v) {
                 const int a = a0 + wa + 16 * i + r16;
                 const int b = b0 + wb + 16 * j + 4 * g4 + v;
-                gram_store(part, G, ip, nkc, kc, m, a, b, dr[i][j][v], di[i][j][v], 0, err);
+                gram_store(part, G, ip, nkc, kc, m, a, b, dr[i][j][v], di[i][j][v], conj_out, err);
             }
 }
 
@@ -1714,10 +1745,10 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
         if constexpr (std::is_same<AT, double>::value) {
             // one contraction chunk: the tiles write the Hermitian matrices directly
             c128 *Gd = w.nkc == 1 ? w.G : nullptr;
-            if (Wh && ks == 1)  // split words of row-contiguous patches
-                k_gram_bf16<<<gg, 256, 0, ctx->stream>>>(Wh, Wl, p * Nt, cs, m, Kc, (int)w.nkc, w.kch,
-                                                         ntile, (p % 4) == 0 ? 1 : 0,
-                                                         (c128 *)w.part, Gd, err_dev);
+            if (Wh)  // split words, contraction-contiguous (k_split_rows / k_split_cols)
+                k_gram_bf16<<<gg, 256, 0, ctx->stream>>>(Wh, Wl, p * Nt, Kc, m, Kc, (int)w.nkc, w.kch,
+                                                         ntile, (Kc % 4) == 0 ? 1 : 0,
+                                                         (c128 *)w.part, Gd, left ? 1 : 0, err_dev);
             else
                 k_gram_mfma<<<gg, 256, 0, ctx->stream>>>(P, p * Nt, ks, cs, m, Kc, (int)w.nkc, w.kch,
                                                          ntile, (c128 *)w.part, Gd, left ? 1 : 0,
@@ -1849,12 +1880,18 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         // split products: the chunk's patches once as split-BF16 words for the Gram
         // (the apply splits the FP32 data per element: no block-relative residual)
         const unsigned *Wh = nullptr, *Wl = nullptr;
-        if (split && !left) {
+        if (split) {
             unsigned *wb = (unsigned *)ctx->scratch_get(SLOT_SPLIT, (size_t)len * nb * 8);
-            const int64_t nw = cdiv(p, SPL_ROWS) * Nt * nb;
             MSLR_PROF(ctx, "split");
-            k_split_rows<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(Pc, p, Nt * nb, wb,
-                                                                         wb + len * nb);
+            if (!left) {
+                const int64_t nw = cdiv(p, SPL_ROWS) * Nt * nb;
+                k_split_rows<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(Pc, p, Nt * nb, wb,
+                                                                             wb + len * nb);
+            } else {
+                const int64_t nw = cdiv(Nt, SPL_ROWS) * p * nb;
+                k_split_cols<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(Pc, p, Nt, nb, wb,
+                                                                             wb + len * nb);
+            }
             HIP_LAUNCH_CHECK();
             Wh = wb;
             Wl = wb + len * nb;

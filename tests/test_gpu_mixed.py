@@ -47,6 +47,20 @@ def test_split_long_contraction(split, rng):
     assert rel_err(Y, Yo) < 1e-4
 
 
+@pytest.mark.parametrize("p,n,Np", [(48, 10000, 1), (216, 387, 5), (100, 130, 3)])
+def test_split_left_patches(split, rng, p, n, Np):
+    """Patches with fewer voxels than frames: the Gram X X^H contracts over time,
+    split from time-contiguous words (k_split_cols) and conjugated; 10000 frames are
+    three contraction chunks summed by the reduce."""
+    from mslr_hip import recon
+    P = np.asfortranarray(np.stack([lowrank(rng, p, n, 2 + i) for i in range(Np)], axis=2))
+    betas = np.array([_beta_between(P[:, :, i], 2 + i) for i in range(Np)])
+    Y = recon.svst_batched(P, betas).to_host()
+    for i in range(Np):
+        Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
+        assert rel_err(Y[:, :, i], Yo) < 1e-4, i
+
+
 def test_split_batch_per_patch_beta(split, rng):
     from mslr_hip import recon
     p, n, Np = 1000, 60, 9
