@@ -26,7 +26,10 @@
 
 namespace mslr {
 
-constexpr int NTX = 256;     // threads of the x-axis passes
+#ifndef MSLR_NTX
+#define MSLR_NTX 256
+#endif
+constexpr int NTX = MSLR_NTX;  // threads of the x-axis passes
 constexpr int MAXE_X = 16;   // elements per thread in the x-axis passes
 constexpr int MAXE_YZ = 16;  // elements per thread in the yz-plane passes
 constexpr int NTYZ_MAX = 512;  // threads of the yz-plane passes (at most)
