@@ -44,7 +44,10 @@ struct PogmDev {
     int err;     // set when a Gram matrix held non-finite values
 };
 
-constexpr int UPD_BLOCKS = 2048;
+#ifndef MSLR_UPD_BLOCKS
+#define MSLR_UPD_BLOCKS 2048
+#endif
+constexpr int UPD_BLOCKS = MSLR_UPD_BLOCKS;  // workgroups of the fused update
 
 // scal_log (nullable): alpha, beta, gamma, zeta of each iteration, for the
 // derivative's backward pass.
