@@ -25,6 +25,7 @@
 // singular values taken from Gram eigenvalues (no vectors).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 
 #include "patches.hpp"
 #include "reduce.hpp"
@@ -667,6 +668,20 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
             if (st->checkpoint) {
                 MSLR_REQUIRE((n * st->S) % 2 == 0, MSLR_ERR_UNSUPPORTED,
                              "checkpointing needs an even element count");
+                // one 5D prox input per iteration: refuse what cannot fit rather than
+                // failing inside hipMalloc
+                const double need = (double)std::max(1, d.niter) * (double)(n * st->S) * sizeof(c64);
+                size_t free_b = 0, total_b = 0;
+                HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+                if (need > 0.9 * (double)free_b) {
+                    char msg[256];
+                    std::snprintf(msg, sizeof msg,
+                                  "checkpoint=1 keeps niter = %d prox inputs of %.2f GB each "
+                                  "(%.1f GB), more than the %.1f GB free on the device",
+                                  d.niter, (double)(n * st->S) * sizeof(c64) / 1e9, need / 1e9,
+                                  (double)free_b / 1e9);
+                    MSLR_FAIL(MSLR_ERR_UNSUPPORTED, msg);
+                }
                 st->ckpt = (c64 *)st->alloc((size_t)std::max(1, d.niter) * (size_t)(n * st->S) *
                                             sizeof(c64));
                 st->scal_log = (double *)st->alloc(sizeof(double) * 4 * (size_t)std::max(1, d.niter));

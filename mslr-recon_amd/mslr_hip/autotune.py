@@ -78,7 +78,7 @@ def _rel(a: np.ndarray, b: np.ndarray) -> float:
 
 
 def autotune_pogm(A, y, *, patch_sizes, strides, lambdas, L: float, form: str = "mixed",
-                  warmup: int = 1, iters: int = 4, rtol: float = 1e-4, min_gain: float = 0.01,
+                  warmup: int = 2, iters: int = 4, rtol: float = 1e-4, min_gain: float = 0.01,
                   cache: str | os.PathLike | None = None, log=None) -> dict:
     """Tunes the context of `A` for POGM on (A, y) and leaves the winner set on it.
 

@@ -423,9 +423,12 @@ def prepare(ksp0, smaps_raw, N, Nvc, Nt):
 
 
 def run_recon_arrays(ksp, omega, smaps, *, N, Nt, PATCH_SIZES, STRIDES, NITERS, sigma1A=1.0,
-                     mode="ref", exact_adjoint=True, fun=None, lambdas=None):
+                     mode="ref", exact_adjoint=True, fun=None, lambdas=None, restart="gr",
+                     restart_cutoff=0.0):
     """POGM part of run_recon (scripts/reconstruct.jl:159-226) on prepared inputs.
-    lambdas (optional) replaces the Ong-Lustig weights (for derivative checks).
+    lambdas (optional) replaces the Ong-Lustig weights (for derivative checks);
+    restart / restart_cutoff are pogm_mod's options (run_recon keeps the defaults
+    :gr and 0, src/mirt_mod.jl:81-82).
 
     Returns (X (Nx,Ny,Nz,Nt,S), restarts list, extras dict).
     """
@@ -463,9 +466,165 @@ def run_recon_arrays(ksp, omega, smaps, *, N, Nt, PATCH_SIZES, STRIDES, NITERS, 
     X0 = np.zeros((*N, Nt, S), dtype=cdt)
     X0[..., 0] = Aa(y)
     X, out = pogm_mod(X0, lambda X: 0.0, dc_cost_grad, L, mom="pogm", niter=NITERS, g_prox=g_prox,
-                      fun=fun or (lambda it, xk, yk, r: r), scalar=scalar)
+                      fun=fun or (lambda it, xk, yk, r: r), scalar=scalar, restart=restart,
+                      restart_cutoff=restart_cutoff)
     restarts = [bool(o) for o in out[1:]] if fun is None else out
     return X, restarts, {"L": L, "lambdas": lams}
+
+
+def _svst_nuc(X, beta):
+    """SVST(X, beta) (the same statements as SVST above) and the nuclear norm of its
+    output, sum(max(S - beta, 0)) in FP64 (src/recon.jl:138-146 applied to the SVST
+    output, whose singular values are the thresholded ones)."""
+    if np.linalg.norm(X) == 0:
+        return np.zeros_like(X), 0.0
+    U, S, Vh = np.linalg.svd(X, full_matrices=False)
+    bT = S.dtype.type(beta)
+    s = np.maximum(S - bT, S.dtype.type(0))
+    keep = s > 0
+    if not keep.any():
+        return np.zeros((U.shape[0], Vh.shape[1]), dtype=X.dtype), 0.0
+    return ((U[:, keep] * s[keep]) @ Vh[keep, :]).astype(X.dtype, copy=False), float(np.sum(s, dtype=np.float64))
+
+
+def _disjoint(N, patch_size, stride_size):
+    return int(coverage_count(N, patch_size, stride_size).max()) <= 1
+
+
+def prox_scale_nuc(z, beta, patch_size, stride_size):
+    """patchSVST of one scale (src/recon.jl:251-269) in z's dtype, returning (image,
+    nuclear norm of the output's patches or None).  The nuclear norm is exact only
+    when the patches are disjoint (then each output patch is the SVST output); for
+    overlapping patches None is returned.  1-voxel patches use the 1 x Nt closed form
+    of the SVD (the group shrink v max(|v| - beta, 0) / |v|, pinned by
+    test_svst_vector_group_shrink); the whole-image patch is thresholded without a
+    patch copy."""
+    N = z.shape[:3]
+    Nt = z.shape[3]
+    ps, nsteps, Np = patch_geometry(N, patch_size, stride_size)
+    p = ps[0] * ps[1] * ps[2]
+    dt = z.dtype
+    if p == 1:
+        v = z.reshape(-1, Nt, order="F")
+        nrm = np.sqrt(np.sum(np.abs(v) ** 2, axis=1))
+        rdt = nrm.dtype.type
+        sh = np.maximum(nrm - rdt(beta), rdt(0))
+        f = np.where(nrm > 0, sh / np.where(nrm > 0, nrm, rdt(1)), rdt(0))
+        out = (v * f[:, None]).astype(dt, copy=False).reshape(z.shape, order="F")
+        return out, float(np.sum(sh, dtype=np.float64))
+    if Np == 1 and tuple(ps) == tuple(N):
+        Y, nuc = _svst_nuc(z.reshape(p, Nt, order="F"), beta)
+        return Y.reshape(z.shape, order="F"), nuc
+    P = img2patches(z, patch_size, stride_size, dtype=dt)
+    nuc = 0.0
+    for i in range(Np):
+        P[:, :, i], c = _svst_nuc(np.array(P[:, :, i]), beta)
+        nuc += c
+    out = patches2img(P, patch_size, stride_size, N, dtype=dt)
+    return out, (nuc if _disjoint(N, patch_size, stride_size) else None)
+
+
+def run_recon_lean(ksp, omega, smaps, *, N, Nt, PATCH_SIZES, STRIDES, NITERS, sigma1A=1.0,
+                   dtype=np.complex128, restart="gr", restart_cutoff=0.0, lambdas=None,
+                   log_costs=True, callback=None):
+    """The POGM run of run_recon (scripts/reconstruct.jl:159-226) with the in-place
+    prox, evaluated with two 5D arrays (X and U_old) instead of pogm_mod's ~10, so
+    the production shapes fit in host memory.
+
+    With g_prox mutating its argument, pogm_mod (src/mirt_mod.jl:111-185) computes
+      g = A'(A sum_s X_s - y) (the same for every scale, reconstruct.jl:180-183),
+      u = x - alpha g,  z = u + beta (u - u_old) + gamma (u - x) - beta alpha/zeta_old * 0
+      (x_old and z_old are the same array), restart iff
+      sum(Float64, real(-g .* (u - u_old))) <= cutoff |g|_5D |u - u_old| (:26-29, :163;
+      Fgrad = fgrad and y_new = u_new because x_new === z_new), x_s = patchSVST(z_s, zeta lambda_s);
+    this function evaluates exactly that, scale by scale.  Pinned against
+    run_recon_arrays / pogm_mod (the statement-by-statement restatement) on small
+    problems: equal iterates and restarts.
+
+    log_costs: the logger (scripts/reconstruct.jl:176-191,205) of every iterate:
+    dc_cost(x_k) from the residual of iteration k + 1's gradient (one more A for the
+    last iterate) and reg_cost(x_k) from the thresholded singular values of the prox
+    for disjoint patches (else a patch_nucnorm pass).
+    callback(it, X, restart) is called after each iteration (it = 0 for X0).
+    Returns (X, restarts, extras) with extras["dc_costs"], ["reg_costs"] when logged and
+    extras["restart_cos"]: the restart test's lhs / (|Fgrad| |y_new - y_old|) per
+    iteration, i.e. its margin to the cutoff."""
+    S = len(PATCH_SIZES)
+    L = S * sigma1A ** 2
+    lams = list(lambdas) if lambdas is not None else lambdas_ong_lustig(N, Nt, PATCH_SIZES)
+    rdt = np.float64 if np.dtype(dtype) == np.complex128 else np.float32
+    sm = smaps.astype(dtype)
+    y = ksp.astype(dtype)
+    alpha = 1.0 / L
+    a = rdt(alpha)
+    X = np.zeros((*N, Nt, S), dtype=dtype)
+    X[..., 0] = A_adjoint(y, omega, sm)
+    U = X.copy()
+    dcs, regs = [], []
+
+    def reg_of(k, img):
+        return lams[k] * patch_nucnorm(img2patches(img, PATCH_SIZES[k], STRIDES[k], dtype=dtype))
+
+    if log_costs:
+        regs.append(sum(reg_of(k, X[..., k]) for k in range(S) if np.any(X[..., k])))
+    if callback:
+        callback(0, X, False)
+    told, sig, restarts, cosines = 1.0, 1.0, [], []
+    for it in range(1, NITERS + 1):
+        s = X[..., 0].copy()
+        for k in range(1, S):
+            s = s + X[..., k]
+        r = A_forward(s, omega, sm) - y
+        del s
+        if log_costs:
+            dcs.append(0.5 * float(np.vdot(r.ravel(), r.ravel()).real))
+        g = A_adjoint(r, omega, sm)
+        del r
+        tnew = 0.5 * (1 + math.sqrt(1 + 8 * told ** 2)) if it == NITERS else 0.5 * (1 + math.sqrt(1 + 4 * told ** 2))
+        beta = (told - 1) / tnew
+        gamma = sig * told / tnew
+        zeta = alpha * (1 + beta + gamma)
+        lhs = 0.0
+        dnorm2 = 0.0
+        reg = 0.0
+        for k in range(S):
+            u = X[..., k] - a * g
+            du = u - U[..., k]
+            lhs += float(np.sum(np.real(-g * du), dtype=np.float64))
+            dnorm2 += float(np.vdot(du.ravel(), du.ravel()).real)
+            z = u + rdt(beta) * du + rdt(gamma) * (u - X[..., k])
+            del du
+            U[..., k] = u
+            del u
+            xk, nuc = prox_scale_nuc(z, zeta * lams[k], PATCH_SIZES[k], STRIDES[k])
+            del z
+            X[..., k] = xk
+            if log_costs:
+                reg += lams[k] * (nuc if nuc is not None else patch_nucnorm(
+                    img2patches(xk, PATCH_SIZES[k], STRIDES[k], dtype=dtype)))
+            del xk
+        is_restart = False
+        gnorm = math.sqrt(S) * float(np.linalg.norm(g))
+        cosines.append(lhs / (gnorm * math.sqrt(dnorm2)) if gnorm * dnorm2 > 0 else 0.0)
+        if restart != "none":
+            is_restart = lhs <= restart_cutoff * gnorm * math.sqrt(dnorm2)
+        del g
+        if is_restart:
+            tnew, sig = 1.0, 1.0
+        restarts.append(bool(is_restart))
+        if log_costs:
+            regs.append(reg)
+        if callback:
+            callback(it, X, is_restart)
+        told = tnew
+    extras = {"L": L, "lambdas": lams, "restart_cos": cosines}
+    if log_costs:
+        s = X.sum(axis=4)
+        r = A_forward(s, omega, sm) - y
+        dcs.append(0.5 * float(np.vdot(r.ravel(), r.ravel()).real))
+        extras["dc_costs"] = dcs
+        extras["reg_costs"] = regs
+    return X, restarts, extras
 
 
 def recon_costs(X, y, omega, smaps, lambdas, PATCH_SIZES, STRIDES, dtype=np.complex128):

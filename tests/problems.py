@@ -49,3 +49,41 @@ def prepared(name: str = "small", seed: int = 7):
     N, Nt, Nc = tuple(cfg["N"]), cfg["Nt"], cfg["Nc"]
     smaps, ksp, omega, R, sf = O.prepare(ksp0, smaps_raw, N, Nc, Nt)
     return cfg, smaps, ksp, omega, sf
+
+
+def production_inputs(name: str, ctx=None):
+    """The synthetic workload at its full size as tests/golden/make_recon_golden.py builds it:
+    (cfg, problem, normalised smaps, y_raw (K, Nc, Nt) complex64) with y = A x_true in FP64
+    (here the device's FP64 SENSE forward, there the oracle's) rounded to ComplexF32, plus
+    the seeded noise.  Needs the GPU."""
+    from mslr_hip import synth
+    from mslr_hip.device import DeviceArray, default_context
+    from mslr_hip.reconstruct import normalise_smaps
+    from mslr_hip.sense import SenseOp
+    ctx = ctx or default_context()
+    cfg = synth.CONFIGS[name]
+    pb = synth.generate(name)
+    smaps = normalise_smaps(pb.smaps)
+    A = SenseOp(pb.mask, smaps, ctx=ctx)
+    xd = DeviceArray.from_host(pb.x_true, ctx, dtype=np.complex128)
+    y = A.forward(xd, f64=True)
+    del xd
+    yh = y.to_host().astype(np.complex64)
+    del y
+    A.close()
+    return cfg, pb, smaps, synth.add_noise(yh, synth.SEEDS[name])
+
+
+def recon_golden(name: str):
+    """(record dict, subset voxel indices, subset values (nvox, Nt) complex128) of
+    tests/golden/recon_<name>.json / _subset.npz, or None when absent."""
+    import json
+    from pathlib import Path
+    here = Path(__file__).resolve().parent / "golden"
+    js, npz = here / f"recon_{name}.json", here / f"recon_{name}_subset.npz"
+    if not (js.exists() and npz.exists()):
+        return None
+    rec = json.loads(js.read_text())
+    with np.load(npz, allow_pickle=False) as z:
+        vox, sub = z["voxels"], z["re"] + 1j * z["im"]
+    return rec, vox, sub

@@ -466,3 +466,129 @@ def test_detrend_closed_forms(rng):
     r = O.detrend(x).astype(np.complex128)
     # the residual has no linear component left: orthogonal to the centred ramp
     assert np.max(np.abs(r.reshape(-1, Nt) @ (t - t.mean()))) < 1e-4
+
+
+# ----------------------------------------------------------------- memory-lean POGM driver
+def _lean_kw(name, niter):
+    from problems import prepared
+    cfg, smaps, ksp, omega, sf = prepared(name)
+    kw = dict(N=tuple(cfg["N"]), Nt=cfg["Nt"], PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"],
+              NITERS=niter, sigma1A=1.0)
+    return cfg, smaps, ksp, omega, kw
+
+
+@pytest.mark.parametrize("name,restart,cutoff", [("small", "gr", 0.0), ("small_overlap", "gr", 0.0),
+                                                 ("small", "none", 0.0), ("small", "gr", -0.5),
+                                                 ("small_overlap", "gr", 0.3)])
+def test_lean_driver_equals_pogm_mod(name, restart, cutoff):
+    """run_recon_lean (two 5D arrays, scale by scale) reproduces the statement-by-statement
+    pogm_mod run with the in-place prox: same restarts, iterates equal to FP64 rounding,
+    and its logged costs equal recon_costs (an independent evaluation) of every iterate."""
+    cfg, smaps, ksp, omega, kw = _lean_kw(name, 7)
+    iters = []
+    Xa, ra, ea = O.run_recon_arrays(ksp, omega, smaps, mode="f64", restart=restart, restart_cutoff=cutoff,
+                                    fun=lambda it, xk, yk, r: (iters.append(np.array(xk)), r)[1], **kw)
+    ra = [bool(v) for v in ra[1:]]
+    seen = []
+    Xb, rb, eb = O.run_recon_lean(ksp, omega, smaps, restart=restart, restart_cutoff=cutoff,
+                                  callback=lambda it, X, r: seen.append(X.copy()), **kw)
+    assert rb == ra
+    assert len(seen) == len(iters) == kw["NITERS"] + 1
+    for a, b in zip(iters, seen):
+        assert rel_err(b, a) < 1e-12
+    lams = ea["lambdas"]
+    for k, X in enumerate(seen):
+        dc, reg = O.recon_costs(X, ksp, omega, smaps, lams, cfg["patch_sizes"], cfg["strides"])
+        assert eb["dc_costs"][k] == pytest.approx(dc, rel=1e-11)
+        assert eb["reg_costs"][k] == pytest.approx(reg, rel=1e-11, abs=1e-12)
+
+
+def test_lean_driver_restart_options_change_the_sequence():
+    """The restart options reach the decision: with :none no iteration restarts; a cutoff
+    of -0.5 restarts less often than 0 (src/mirt_mod.jl:26-29,163)."""
+    cfg, smaps, ksp, omega, kw = _lean_kw("small", 8)
+    _, r0, _ = O.run_recon_lean(ksp, omega, smaps, log_costs=False, **kw)
+    _, rn, _ = O.run_recon_lean(ksp, omega, smaps, log_costs=False, restart="none", **kw)
+    _, rc, _ = O.run_recon_lean(ksp, omega, smaps, log_costs=False, restart_cutoff=-0.5, **kw)
+    assert not any(rn)
+    assert sum(rc) < sum(r0)
+
+
+# ----------------------------------------------------------------- setup (scripts/reconstruct.jl:86-118)
+def test_normalise_smaps_closed_form():
+    """Constant maps S_c = c + 1: RSS = sqrt(sum (c+1)^2), S_c / (RSS + eps(Float32)); the
+    result has unit root-sum-of-squares to FP32 rounding (scripts/reconstruct.jl:89)."""
+    N, Nc = (3, 2, 2), 4
+    raw = np.zeros((*N, Nc), dtype=np.complex64)
+    for c in range(Nc):
+        raw[..., c] = (c + 1) * np.exp(0.3j * c)
+    rss = math.sqrt(sum((c + 1) ** 2 for c in range(Nc)))
+    out = O.normalise_smaps(raw)
+    for c in range(Nc):
+        assert np.allclose(np.abs(out[..., c]), (c + 1) / (rss + O.EPS32), rtol=1e-6)
+    assert np.allclose(np.sum(np.abs(out) ** 2, axis=3), 1.0, rtol=1e-6)
+
+
+def test_sense_comb_closed_forms(rng):
+    """(i) One coil, S = 1, a single sample at the centre of k-space (index N//2): the
+    centred inverse DFT is the constant 1/V.  (ii) Full sampling of A x with unit-RSS maps:
+    sense_comb returns x / sqrt(V) (the unitary forward's scale), src/recon.jl:312-323."""
+    N, Nt = (4, 6, 2), 2
+    V = int(np.prod(N))
+    ksp = np.zeros((*N, 1, Nt), dtype=np.complex128)
+    ksp[N[0] // 2, N[1] // 2, N[2] // 2, 0, :] = [1.0, 2.0 - 1j]
+    img = O.sense_comb(ksp, np.ones((*N, 1), dtype=np.complex128))
+    assert np.allclose(img[..., 0], 1.0 / V, rtol=1e-14)
+    assert np.allclose(img[..., 1], (2.0 - 1j) / V, rtol=1e-14)
+    Nc = 3
+    smaps = O.normalise_smaps(crandn(rng, (*N, Nc), np.complex128)).astype(np.complex128)
+    smaps /= np.sqrt(np.sum(np.abs(smaps) ** 2, axis=3, keepdims=True))
+    x = crandn(rng, (*N, Nt), np.complex128)
+    full = np.ones(N, dtype=bool)
+    ksp = np.zeros((*N, Nc, Nt), dtype=np.complex128)
+    for t in range(Nt):
+        ksp[..., t] = O.Asense_forward(x[..., t], full, smaps).reshape(*N, Nc, order="F")
+    assert rel_err(O.sense_comb(ksp, smaps), x / math.sqrt(V)) < 1e-13
+
+
+def test_prepare_bruteforce(rng):
+    """prepare (scripts/reconstruct.jl:86-118,145-147) against per-voxel loops: the mask is
+    the nonzeros of coil 1, R = V / K, the flattened k-space lists the sampled voxels of each
+    frame in column-major order divided by the type-7 0.99 quantile of |sense_comb|, and the
+    two assertions fire on a coil with another pattern and a frame with another count."""
+    N, Nc, Nt = (4, 6, 2), 2, 3
+    V = int(np.prod(N))
+    samp = rng.random((N[0], N[1], N[2])) < 0.5
+    omega = np.stack([np.roll(samp, t, axis=1) for t in range(Nt)], axis=3)
+    vals = crandn(rng, (*N, Nc, Nt))
+    ksp0 = np.where(omega[:, :, :, None, :], vals, 0).astype(np.complex64)
+    raw = crandn(rng, (*N, Nc))
+    smaps, ksp, om, R, sf = O.prepare(ksp0, raw, N, Nc, Nt)
+    assert np.array_equal(om, omega)
+    K = int(samp.sum())
+    assert R == V / K
+    img = O.sense_comb(ksp0, smaps)
+    mags = sorted(float(np.abs(np.complex64(v))) for v in img.ravel(order="F"))  # abs of ComplexF32
+    h = (len(mags) - 1) * 0.99
+    lo = int(math.floor(h))
+    q = mags[lo] + (h - lo) * (mags[min(lo + 1, len(mags) - 1)] - mags[lo])
+    assert sf == pytest.approx(q, rel=1e-12)
+    for t in range(Nt):
+        k = 0
+        for iz in range(N[2]):
+            for iy in range(N[1]):
+                for ix in range(N[0]):
+                    if omega[ix, iy, iz, t]:
+                        for c in range(Nc):
+                            assert ksp[k, c, t] == np.complex64(ksp0[ix, iy, iz, c, t] / np.float32(sf))
+                        k += 1
+        assert k == K
+    bad = ksp0.copy()
+    bad[0, 0, 0, 1, 0] = 0 if omega[0, 0, 0, 0] else 1
+    with pytest.raises(AssertionError, match="Coil 2"):
+        O.prepare(bad, raw, N, Nc, Nt)
+    bad = ksp0.copy()
+    extra = np.argwhere(~omega[..., 1])[0]
+    bad[extra[0], extra[1], extra[2], :, 1] = 1
+    with pytest.raises(AssertionError, match="Frame 2"):
+        O.prepare(bad, raw, N, Nc, Nt)
