@@ -162,6 +162,18 @@ mslr_status_t mslr_sense_forward(mslr_sense plan, const void *x_dev, void *y_dev
                                  int64_t t0, int64_t nframes);
 mslr_status_t mslr_sense_adjoint(mslr_sense plan, const void *y_dev, void *x_dev,
                                  int64_t t0, int64_t nframes);
+/* mslr_sense_forward without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * result as mslr_sense_forward (the same kernels, so bit-identical output); the
+ * intermediates live in a 64 MiB array of the library's, one per device, set
+ * aside when the library is loaded on that device and held until the process
+ * ends.  Frames run in chunks that fit it; a frame needing more than 64 MiB
+ * (its V ComplexF32 image plus Nc*Nx*lines sampled-line values) gives
+ * MSLR_ERR_UNSUPPORTED before any work is queued.  All calls on one device share
+ * the array: calls that overlap on one device (other contexts or threads) race. */
+mslr_status_t mslr_sense_forward_noalloc(mslr_sense plan, const void *x_dev, void *y_dev,
+                                         int64_t t0, int64_t nframes);
 /* The same in ComplexF64 (used by the FP64 power iteration). */
 mslr_status_t mslr_sense_forward_f64(mslr_sense plan, const void *x_dev, void *y_dev,
                                      int64_t t0, int64_t nframes);

@@ -2,6 +2,7 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "sense.hpp"
 
 namespace mslr {
 
@@ -126,6 +127,13 @@ mslr_status_t mslr_ctx_create(int device, mslr_ctx *out) {
                                              hipGetErrorString(se));
         }
         c->scratch.resize(SLOT_COUNT);
+        try {
+            sense_noalloc_init(device);
+        } catch (...) {
+            hipStreamDestroy(c->stream);
+            delete c;
+            throw;
+        }
         *out = c;
     });
 }

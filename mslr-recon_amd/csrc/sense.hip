@@ -17,6 +17,8 @@
 // transpose of the forward.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "fft.hpp"
@@ -655,16 +657,25 @@ static void adj_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, C2<
 }
 
 // Forward of one chunk [ta, ta+nf): X holds the chunk's frames (nscales images
-// spaced sstride apart), ysub/out/partial start at the chunk.
+// spaced sstride apart), ysub/out/partial start at the chunk; Pm (nf*V elements)
+// and Tb (the chunk's sampled lines) are the intermediates, both fully written
+// before they are read.
+template <class T>
+static void forward_chunk_into(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride,
+                               int64_t ta, int nf, const C2<T> *ysub, C2<T> *out,
+                               double *partial, C2<T> *Pm, C2<T> *Tb) {
+    launch_sum_planes<T>(P, X, nscales, sstride, nf, Pm);
+    fwd_yz_any<T>(P, ta, nf, Pm, Tb);
+    fwd_x_any<T>(P, ta, nf, Tb, ysub, out, partial);
+}
+
 template <class T>
 static void forward_chunk(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride,
                           int64_t ta, int nf, const C2<T> *ysub, C2<T> *out, double *partial) {
     mslr_ctx_s *ctx = P->ctx;
     C2<T> *Pm = (C2<T> *)ctx->scratch_get(SLOT_PLANES, (size_t)nf * P->V * sizeof(C2<T>));
     C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, chunk_line_bytes<T>(P, ta, nf));
-    launch_sum_planes<T>(P, X, nscales, sstride, nf, Pm);
-    fwd_yz_any<T>(P, ta, nf, Pm, Tb);
-    fwd_x_any<T>(P, ta, nf, Tb, ysub, out, partial);
+    forward_chunk_into<T>(P, X, nscales, sstride, ta, nf, ysub, out, partial, Pm, Tb);
 }
 
 template <class T>
@@ -757,6 +768,77 @@ static c128 unit_phase(int64_t num, int64_t den) {
     }
     const double a = 2.0 * M_PI * (double)m / (double)den;
     return {std::cos(a), std::sin(a)};
+}
+
+// ------------------------------------------------ allocation-free forward
+// The intermediates of mslr_sense_forward_noalloc live in this file-scope
+// array, one copy per device, held until the process ends.  Host code never
+// uses the array's name as an address: each device's address comes from
+// hipGetSymbolAddress at context creation (sense_noalloc_init).
+static __device__ __align__(256) c64 scratch[MSLR_NOALLOC_SCRATCH_BYTES / sizeof(c64)];
+
+static std::mutex g_noalloc_mu;
+static std::vector<c64 *> g_noalloc_ptr;  // per device ordinal
+
+void sense_noalloc_init(int device) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if ((size_t)device < g_noalloc_ptr.size() && g_noalloc_ptr[(size_t)device]) return;
+    c64 *p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    MSLR_REQUIRE(p != nullptr, MSLR_ERR_DEVICE, "no device address for the scratch array");
+    if (g_noalloc_ptr.size() <= (size_t)device) g_noalloc_ptr.resize((size_t)device + 1, nullptr);
+    g_noalloc_ptr[(size_t)device] = p;
+}
+
+static c64 *noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return (size_t)device < g_noalloc_ptr.size() ? g_noalloc_ptr[(size_t)device] : nullptr;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Scratch bytes a frame chunk [t, t+n) needs: plane-major image + sampled lines.
+static size_t noalloc_chunk_bytes(mslr_sense_s *P, int64_t t, int64_t n) {
+    return align256((size_t)n * P->V * sizeof(c64)) + align256(chunk_line_bytes<float>(P, t, n));
+}
+
+// The same frames as mslr_sense_forward in chunks that fit the scratch array.
+static void run_forward_noalloc(mslr_sense_s *P, const c64 *X, int64_t t0, int64_t nframes,
+                                c64 *out) {
+    check_lds<float>(P);
+    c64 *base = noalloc_scratch(P->ctx->device);
+    MSLR_REQUIRE(base != nullptr, MSLR_ERR_DEVICE,
+                 "the scratch array of this device has no address (context not created)");
+    for (int64_t t = t0; t < t0 + nframes; ++t) {
+        const size_t need = noalloc_chunk_bytes(P, t, 1);
+        if (need > (size_t)MSLR_NOALLOC_SCRATCH_BYTES)
+            MSLR_FAIL(MSLR_ERR_UNSUPPORTED,
+                      "frame " + std::to_string(t) + " needs " + std::to_string(need) +
+                          " bytes of scratch; the allocation-free forward has " +
+                          std::to_string((size_t)MSLR_NOALLOC_SCRATCH_BYTES));
+    }
+    // no profiling records here: they would create events
+    mslr_ctx_s *ctx = P->ctx;
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    try {
+        int64_t ta = t0;
+        while (ta < t0 + nframes) {
+            int64_t nf = 1;
+            while (ta + nf < t0 + nframes && nf < 65535 &&
+                   noalloc_chunk_bytes(P, ta, nf + 1) <= (size_t)MSLR_NOALLOC_SCRATCH_BYTES)
+                ++nf;
+            c64 *Pm = base;
+            c64 *Tb = base + align256((size_t)nf * P->V * sizeof(c64)) / sizeof(c64);
+            forward_chunk_into<float>(P, X + (ta - t0) * P->V, 1, 0, ta, (int)nf, nullptr,
+                                      out + (ta - t0) * P->Nc * P->K, nullptr, Pm, Tb);
+            ta += nf;
+        }
+    } catch (...) {
+        ctx->prof = prof;
+        throw;
+    }
+    ctx->prof = prof;
 }
 
 }  // namespace mslr
@@ -951,6 +1033,16 @@ mslr_status_t mslr_sense_forward(mslr_sense plan, const void *x_dev, void *y_dev
         if (nframes == 0) return;
         run_forward<float>(plan, (const c64 *)x_dev, 1, 0, t0, nframes, nullptr, (c64 *)y_dev,
                            nullptr);
+    });
+}
+
+mslr_status_t mslr_sense_forward_noalloc(mslr_sense plan, const void *x_dev, void *y_dev,
+                                         int64_t t0, int64_t nframes) {
+    return guarded([&] {
+        check_frames(plan, t0, nframes);
+        if (nframes == 0) return;
+        MSLR_REQUIRE(x_dev && y_dev, MSLR_ERR_ARG, "null argument");
+        run_forward_noalloc(plan, (const c64 *)x_dev, t0, nframes, (c64 *)y_dev);
     });
 }
 

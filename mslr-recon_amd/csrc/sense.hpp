@@ -57,4 +57,12 @@ void sense_resid_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t ss
 // x = A' y for all frames, y in internal order.
 void sense_adjoint_internal(mslr_sense_s *P, const c64 *y_int, c64 *x);
 
+// Bytes of the file-scope scratch array of mslr_sense_forward_noalloc (per device).
+#ifndef MSLR_NOALLOC_SCRATCH_BYTES
+#define MSLR_NOALLOC_SCRATCH_BYTES (64u << 20)
+#endif
+// Looks up this device's address of that array (the device must be current);
+// once per device, at context creation.
+void sense_noalloc_init(int device);
+
 }  // namespace mslr

@@ -119,6 +119,7 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_sense_destroy": (C.c_int, [_p]),
     "mslr_sense_nsamples": (_i64, [_p]),
     "mslr_sense_forward": (C.c_int, [_p, _p, _p, _i64, _i64]),
+    "mslr_sense_forward_noalloc": (C.c_int, [_p, _p, _p, _i64, _i64]),
     "mslr_sense_adjoint": (C.c_int, [_p, _p, _p, _i64, _i64]),
     "mslr_sense_forward_f64": (C.c_int, [_p, _p, _p, _i64, _i64]),
     "mslr_sense_adjoint_f64": (C.c_int, [_p, _p, _p, _i64, _i64]),

@@ -80,6 +80,20 @@ class SenseOp:
         _lib.check(fn(self.handle, xd.ptr, y.ptr, int(t0), int(nf)))
         return y
 
+    def forward_noalloc(self, x: DeviceArray, out: DeviceArray, t0: int = 0):
+        """y = A x for frames t0.. into `out` without any device allocation
+        (mslr_sense_forward_noalloc): x and out must be device arrays."""
+        if not isinstance(x, DeviceArray) or not isinstance(out, DeviceArray):
+            raise TypeError("forward_noalloc takes device arrays")
+        if x.dtype != np.complex64 or out.dtype != np.complex64:
+            raise TypeError("forward_noalloc is ComplexF32")
+        V = int(np.prod(self.N))
+        nf = x.nbytes // (V * 8)
+        if nf * V * 8 != x.nbytes or out.nbytes != self.K * self.Nc * nf * 8:
+            raise AssertionError("x and out sizes do not match a whole number of frames")
+        _lib.check(self.ctx._lib.mslr_sense_forward_noalloc(self.handle, x.ptr, out.ptr, int(t0), int(nf)))
+        return out
+
     def adjoint(self, y, out: DeviceArray | None = None, t0: int = 0, f64: bool = False):
         """x = A' y for frames t0.. (y: (K,Nc,nframes))."""
         dt = np.complex128 if f64 else np.complex64
