@@ -121,6 +121,23 @@ def kernel_flops_per_iter(cfg, split=False):
     return out
 
 
+def sense_fft_flops(cfg):
+    """FP32 arithmetic of the SENSE FFT passes per POGM iteration, 5 n log2(n) per
+    length-n transform plus 6 per coil-map product: the yz passes transform every
+    (y, z) plane of every coil and frame along y and z; the x passes transform the
+    sampled (ky, kz) lines (1/R of the plane) of every coil and frame along x."""
+    N, Nt, Nc, R = cfg["N"], cfg["Nt"], cfg["Nc"], cfg["R"]
+    nx, ny, nz = N
+    yz_plane = nz * 5 * ny * math.log2(ny) + ny * 5 * nz * math.log2(nz) + 6 * ny * nz
+    lines = round(ny * nz / R)
+    x_lines = lines * 5 * nx * math.log2(nx)
+    return {"fwd_yz": yz_plane * nx * Nc * Nt, "adj_yz": yz_plane * nx * Nc * Nt,
+            "fwd_x": x_lines * Nc * Nt, "adj_x": x_lines * Nc * Nt}
+
+
+# FP32 vector peak without packed FP32 instructions (the build disables them):
+# 256 CUs x 64 lanes x 2 flops (FMA) x 2.4 GHz; 157.3 with packed FP32
+FP32_VALU_PEAK_TFLOPS = 78.6
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 # kernel family -> (bound, unit, peak) of its roofline
@@ -134,75 +151,142 @@ BOUNDS_SPLIT = dict(BOUNDS, gram_tiles=("mfma", "TFLOP/s", BF16_PEAK_TFLOPS),
 
 
 # --------------------------------------------------------------------------- CPU baseline
-def cpu_baseline(cfg, pb, y_norm, smaps, part_s=4.0):
-    """The oracle (a restatement of the reference CPU path) timed on samples of one
-    POGM iteration and scaled to a full iteration; each part is sampled for about
-    part_s seconds.  Returns (iter/s, cores, sample text, reference-faithful iter/s).
+# Worker state of the CPU baseline's process pool (inherited through fork; the pool
+# is created before this process touches the GPU).
+_CPU = {}
+
+
+def _cpu_worker_init():
+    from threadpoolctl import threadpool_limits
+    threadpool_limits(1)  # one BLAS thread per worker, as Julia's @threads loop over patches
+
+
+def _cpu_sense_frames(frames):
+    """A and A' (residual form) of the given frames, one after another (FFT workers=1)."""
+    import oracle as O
+    import scipy.fft as sfft
+    d = _CPU
+    t = time.perf_counter()
+    with sfft.set_workers(1):
+        for f in frames:
+            x = d["img"][..., f:f + 1]
+            r = O.A_forward(x, d["mask"][..., f:f + 1], d["smaps"]) - d["y"][:, :, f:f + 1]
+            O.A_adjoint(r, d["mask"][..., f:f + 1], d["smaps"])
+    return time.perf_counter() - t
+
+
+def _cpu_patches(job):
+    """SVST (gesdd) and, separately timed, the singular values of patches [i0, i1) of a scale."""
+    import oracle as O
+    k, i0, i1 = job
+    d = _CPU
+    ps, ss, beta = d["scales"][k]
+    N = d["img"].shape[:3]
+    Nt = d["img"].shape[3]
+    psc, nst, _ = O.patch_geometry(N, ps, ss)
+    p = int(np.prod(psc))
+    starts = list(O._patch_starts(N, psc, ss, nst))[i0:i1] if p > 1 else None
+    img = d["img"]
+    t_svst = t_vals = 0.0
+    for i in range(i0, i1):
+        if p > 1:
+            sx, sy, sz = starts[i - i0]
+            P = img[sx:sx + psc[0], sy:sy + psc[1], sz:sz + psc[2], :].reshape(p, Nt, order="F")
+        else:
+            P = img.reshape(-1, Nt, order="F")[i:i + 1]
+        t = time.perf_counter()
+        O.SVST(np.array(P), beta)
+        t1 = time.perf_counter()
+        np.linalg.svd(P, compute_uv=False)
+        t_svst += t1 - t
+        t_vals += time.perf_counter() - t1
+    return t_svst, t_vals
+
+
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, pb, smaps, workers=16, part_s=5.0):
+    """The oracle (a restatement of the reference CPU path) timed on samples of one POGM
+    iteration on `workers` host processes and scaled to the whole iteration.  Like the
+    reference's CPU run (julia -t auto: @threads over patches with one LAPACK gesdd per
+    patch, src/recon.jl:217-221; FFTW per frame), patches and frames are spread over the
+    processes with one BLAS thread each; the whole-image patch (one gesdd) uses all
+    `workers` BLAS threads.  Must run before this process initialises the GPU (the pool
+    forks).  Returns (iter/s, cores, sample text, reference-faithful iter/s).
 
     One iteration is the gradient (1 A + 1 A'), the update with its restart test and
     the prox.  The reference's own loop also evaluates the unused Fcost (1 A and the
     singular values of every patch, src/mirt_mod.jl:161) and the logger (the same
     again, scripts/reconstruct.jl:205): 3 A + 1 A' + the prox + 2 singular-value
     passes, reported as the reference-faithful rate from the same samples."""
+    import multiprocessing as mp
+
     import oracle as O
+    from threadpoolctl import threadpool_limits
     N, Nt, Nc = tuple(cfg["N"]), cfg["Nt"], cfg["Nc"]
     V = int(np.prod(N))
-    cores = O._workers()
-    parts, notes = {}, []
-    svdvals = {}
-    mask = pb.mask
-    # SENSE A and A' (residual), frame by frame until part_s
-    t = time.perf_counter()
-    fs = 0
-    ta = 0.0
-    while fs < Nt and (fs == 0 or time.perf_counter() - t < part_s):
-        x = pb.x_true[..., fs:fs + 1].astype(np.complex64)
-        t1 = time.perf_counter()
-        r = O.A_forward(x, mask[..., fs:fs + 1], smaps) - y_norm[:, :, fs:fs + 1]
-        ta += time.perf_counter() - t1
-        O.A_adjoint(r, mask[..., fs:fs + 1], smaps)
-        fs += 1
-    parts["sense"] = (time.perf_counter() - t) * Nt / fs
-    t_fwd = ta * Nt / fs
-    notes.append(f"A+A' on {fs}/{Nt} frames")
     lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
     img = pb.x_true
-    for k, (ps, ss) in enumerate(zip(cfg["patch_sizes"], cfg["strides"])):
-        psc, nst, Np = O.patch_geometry(N, ps, ss)
-        p = int(np.prod(psc))
-        beta = 0.5 * lams[k]
-        if Np == 1:
-            # one global patch: gesdd cost is linear in the rows of a tall matrix
-            rows = max(Nt, p // 8)
-            X = img.reshape(V, Nt, order="F")[:rows]
-            t = time.perf_counter()
-            O.SVST(X, beta)
-            parts[f"svst_scale{k + 1}"] = (time.perf_counter() - t) * p / rows
-            t = time.perf_counter()
-            O.patch_nucnorm(X[:, :, None])
-            svdvals[k] = (time.perf_counter() - t) * p / rows
-            notes.append(f"scale {k + 1}: gesdd of {rows}/{p} rows")
-        else:
-            chunk = 4 if p > 1 else 4000
-            it = O._patch_starts(N, psc, ss, nst)
-            t = time.perf_counter()
+    # k-space of the sampled frames (the SENSE part's input), made here by the oracle
+    nfs = 2 * workers
+    y = np.zeros((int(pb.mask[..., 0].sum()), Nc, Nt), dtype=np.complex64, order="F")
+    y[:, :, :nfs] = O.A_forward(img[..., :nfs], pb.mask[..., :nfs], smaps)
+    _CPU.update(img=img, mask=pb.mask, smaps=smaps, y=y,
+                scales=[(ps, ss, 0.5 * lams[k]) for k, (ps, ss) in
+                        enumerate(zip(cfg["patch_sizes"], cfg["strides"]))])
+    parts, notes, vals = {}, [], {}
+    ctx = mp.get_context("fork")
+    with ctx.Pool(workers, initializer=_cpu_worker_init) as pool:
+        pool.map(_cpu_sense_frames, [[0]] * workers)  # warm-up (imports, FFT plans)
+        t = time.perf_counter()
+        pool.map(_cpu_sense_frames, [[f, f + workers] for f in range(workers)])
+        wall = time.perf_counter() - t
+        parts["sense"] = wall * Nt / nfs
+        notes.append(f"A+A' of {nfs}/{Nt} frames on {workers} processes")
+        for k, (ps, ss) in enumerate(zip(cfg["patch_sizes"], cfg["strides"])):
+            psc, _, Np = O.patch_geometry(N, ps, ss)
+            p = int(np.prod(psc))
+            if Np == 1:
+                # one global patch: gesdd with all BLAS threads on a row sample (linear in rows)
+                rows = max(Nt, p // 8)
+                X = img.reshape(V, Nt, order="F")[:rows]
+                with threadpool_limits(workers):
+                    t = time.perf_counter()
+                    O.SVST(X, 0.5 * lams[k])
+                    parts[f"svst_scale{k + 1}"] = (time.perf_counter() - t) * p / rows
+                    t = time.perf_counter()
+                    np.linalg.svd(X, compute_uv=False)
+                    vals[k] = (time.perf_counter() - t) * p / rows
+                notes.append(f"scale {k + 1}: gesdd of {rows}/{p} rows, {workers} BLAS threads")
+                continue
+            # per-patch gesdd in the pool: grow the sample until it takes part_s of wall time
+            per = 1 if p > 1 else 2000
             done = 0
-            tv = 0.0
-            while done < Np and (done == 0 or time.perf_counter() - t < part_s + tv):
-                n = min(chunk, Np - done)
-                Psub = np.empty((p, Nt, n), dtype=np.complex64)
-                for i in range(n):
-                    sx, sy, sz = next(it)
-                    Psub[:, :, i] = img[sx:sx + psc[0], sy:sy + psc[1], sz:sz + psc[2], :].reshape(p, Nt, order="F")
-                O.svst_batched(Psub, beta)
-                t1 = time.perf_counter()
-                O.patch_nucnorm(Psub)
-                tv += time.perf_counter() - t1
+            while True:
+                n = min(per * workers, Np - done)
+                jobs = [(k, done + i * n // workers, done + (i + 1) * n // workers) for i in range(workers)]
+                t = time.perf_counter()
+                res = pool.map(_cpu_patches, jobs)
+                wall = time.perf_counter() - t
                 done += n
-            parts[f"svst_scale{k + 1}"] = (time.perf_counter() - t - tv) * Np / done
-            svdvals[k] = tv * Np / done
-            notes.append(f"scale {k + 1}: {done}/{Np} patches")
-    # POGM update on a tenth of the state
+                if wall > part_s / 5 or done >= Np:
+                    break
+                per *= 4
+            ts = sum(r[0] for r in res)
+            tv = sum(r[1] for r in res)
+            # wall time of the sample split between SVST and the singular values
+            parts[f"svst_scale{k + 1}"] = wall * ts / (ts + tv) * Np / n
+            vals[k] = wall * tv / (ts + tv) * Np / n
+            notes.append(f"scale {k + 1}: {n}/{Np} patches on {workers} processes")
+    # POGM update on a tenth of the state (a Julia broadcast: one thread)
     S = len(cfg["patch_sizes"])
     n = V * Nt * S // 10
     xs = np.ones(n, dtype=np.complex128)
@@ -214,15 +298,18 @@ def cpu_baseline(cfg, pb, y_norm, smaps, part_s=4.0):
     float(np.sum(np.real(-g * (u - us)), dtype=np.float64))
     del z
     parts["pogm_update"] = (time.perf_counter() - t) * 10
-    notes.append("POGM update on 1/10 of the state")
+    notes.append("POGM update (one thread) on 1/10 of the state")
     tot = sum(parts.values())
-    sample = ("oracle (NumPy/SciPy restatement of the reference CPU path, LAPACK gesdd per patch as "
-              "the reference) on samples of one iteration, scaled: " + "; ".join(notes) +
+    t_fwd = parts["sense"] / 2
+    sample = (f"oracle (NumPy/SciPy restatement of the reference CPU path) on {workers} processes of "
+              f"'{cpu_model()}', one BLAS thread each, LAPACK gesdd per patch as the reference, on "
+              "samples of one iteration, scaled: " + "; ".join(notes) +
               "; parts (s/iter): " + ", ".join(f"{k}={v:.2f}" for k, v in parts.items()))
-    faithful = tot + 2 * t_fwd + 2 * sum(svdvals.values())
-    sample += (f"; reference-faithful iteration adds 2 A ({2 * t_fwd:.2f} s) and 2 singular-value "
-               f"passes ({2 * sum(svdvals.values()):.2f} s)")
-    return 1.0 / tot, cores, sample, 1.0 / faithful
+    faithful = tot + 2 * t_fwd + 2 * sum(vals.values())
+    sample += (f"; reference-faithful iteration adds 2 A (~{2 * t_fwd:.2f} s, half of A+A') and 2 "
+               f"singular-value passes ({2 * sum(vals.values()):.2f} s)")
+    _CPU.clear()
+    return 1.0 / tot, workers, sample, 1.0 / faithful
 
 
 # --------------------------------------------------------------------------- counter calibration
@@ -292,6 +379,49 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
+# --------------------------------------------------------------------------- accuracy
+def load_golden(name):
+    """(record, voxel indices, image-sum subset) of the FP64 oracle fixture of this
+    workload (tests/golden/recon_<name>.json + _subset.npz), or None."""
+    js = ROOT / "tests" / "golden" / f"recon_{name}.json"
+    npz = ROOT / "tests" / "golden" / f"recon_{name}_subset.npz"
+    if not (js.exists() and npz.exists()):
+        return None
+    rec = json.loads(js.read_text())
+    with np.load(npz, allow_pickle=False) as z:
+        return rec, z["voxels"], z["re"] + 1j * z["im"]
+
+
+def accuracy_vs_oracle(gold, y, A, cfg, lams, L, pb, nrmse_truth, form, ctx):
+    """The measured form and the plain form run for the fixture's iteration count on the
+    fixture's inputs (this run's k-space divided by the fixture's scale factor), compared
+    with the FP64 oracle's run: final-image NRMSE to the truth (FP64 norms), the image sum
+    at the fixture's voxels, the restart flags."""
+    from mslr_hip.device import DeviceArray
+    from mslr_hip.mirt_mod import POGM
+    rec, vox, sub = gold
+    niter, sf_o = rec["niter"], rec["scale_factor"]
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    V = int(np.prod(N))
+    yd = DeviceArray.from_host(np.asfortranarray(y / np.float32(sf_o)), ctx)
+    out = {"fixture": f"tests/golden/recon_{rec['config']}.json ({rec['mode']}, {niter} iterations)",
+           "oracle_nrmse_vs_truth": rec["nrmse_vs_truth"][-1], "forms": {}}
+    for f in dict.fromkeys([form, "plain"]):
+        st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=L,
+                  niter=niter, form=f)
+        st.init(yd)
+        st.run(niter)
+        _, rs = st.log()
+        xs = st.X.to_host().astype(np.complex128).sum(axis=-1)
+        st.close()
+        nr = nrmse_truth(xs, sf_o)
+        got = xs.reshape(V, Nt, order="F")[vox, :]
+        out["forms"][f] = {"nrmse_vs_truth": nr, "delta_nrmse_vs_oracle": abs(nr - rec["nrmse_vs_truth"][-1]),
+                           "subset_rel_err": float(np.linalg.norm(got - sub) / np.linalg.norm(sub)),
+                           "restarts_match": rs == rec["restarts"]}
+    return out
+
+
 # --------------------------------------------------------------------------- main
 def metric_label(cfg, name):
     """The headline metric's label for the 90x90x60, Nt=300, 18-coil workload; other
@@ -347,13 +477,26 @@ def main() -> int:
     t = time.time()
     pb = synth.generate(name)
     log(f"generated {name}: N={N} Nt={Nt} Nc={Nc} lines/frame={pb.lines} in {time.time() - t:.1f}s")
+    cpu = None
+    if not args.no_cpu_baseline:
+        # first: its process pool forks, which must happen before the GPU is initialised
+        t = time.time()
+        import oracle as O
+        v, cores, sample, vf = cpu_baseline(cfg, pb, O.normalise_smaps(pb.smaps))
+        cpu = {"value": round(v, 6), "unit": "iter/s", "cores": cores, "kind": "port",
+               "cpu_model": cpu_model(), "sample": sample, "reference_faithful_value": round(vf, 6)}
+        log(f"cpu baseline {v:.5f} iter/s ({cores} processes) in {time.time() - t:.1f}s")
     ctx = default_context()
     info = ctx.info()
     log(f"device: {info['name']} total={info['total_bytes'] / 2**30:.0f} GiB")
     t = time.time()
     smaps = normalise_smaps(pb.smaps)
+    # k-space: A x_true in FP64 rounded to ComplexF32, plus the seeded noise (the
+    # inputs of the FP64 oracle fixture tests/golden/recon_<config>.json)
     A0 = SenseOp(pb.mask, smaps, ctx=ctx)
-    yclean = A0.forward(DeviceArray.from_host(pb.x_true, ctx)).to_host()
+    xd64 = DeviceArray.from_host(pb.x_true, ctx, dtype=np.complex128)
+    yclean = A0.forward(xd64, f64=True).to_host().astype(np.complex64)
+    del xd64
     A0.close()
     y = synth.add_noise(yclean, seed)
     del yclean
@@ -381,10 +524,11 @@ def main() -> int:
 
     xt_norm = float(np.linalg.norm(pb.x_true))
 
-    def nrmse_truth(x):
+    def nrmse_truth(x, scale=None):
         """NRMSE of a reconstruction (scale sum) against the synthetic truth, in the
-        truth's units (the k-space was divided by sf)."""
-        return float(np.linalg.norm(x * np.float32(sf) - pb.x_true) / xt_norm)
+        truth's units (the k-space was divided by sf), with FP64 norms."""
+        d = np.asarray(x, dtype=np.complex128) * (sf if scale is None else scale) - pb.x_true
+        return float(np.linalg.norm(d) / xt_norm)
 
     # launch shapes of the measured form from the autotuner (timed POGM iterations
     # of this workload per candidate; untimed here, like the warmup)
@@ -461,6 +605,7 @@ def main() -> int:
             log(f"  {kname:18s} launches={cnt:6d} total={ms:9.2f} ms  ({100 * ms / (el_k * 1e3):5.1f}%)")
 
     roof = None
+    roof_valu = {}
     if kern:
         lines_total = int(pb.mask.reshape(N[0], N[1] * N[2], Nt, order="F").any(axis=0).sum())
         budget = 4 << 30
@@ -495,6 +640,27 @@ def main() -> int:
         if roof is not None:
             roof.update({"avg_launch_ms": round(per_launch_ms, 4), "launches_per_iter": cnt / K,
                          "share_of_kernel_time": round(ms / total_ms, 3)})
+        # the yz FFT passes against the FP32 vector roofline as well (they are bound by
+        # their arithmetic, not by HBM)
+        ff = sense_fft_flops(cfg)
+        for kf in ("fwd_yz", "adj_yz", "fwd_x", "adj_x"):
+            if kf in fam and kf in ff:
+                c_, m_ = fam[kf]
+                t_ = m_ / c_ / 1e3
+                roof_valu[kf] = {"bound": "valu", "flops_per_launch": int(ff[kf]),
+                                 "achieved": round(ff[kf] / t_ / 1e12, 2), "unit": "TFLOP/s",
+                                 "peak": FP32_VALU_PEAK_TFLOPS,
+                                 "frac": round(ff[kf] / t_ / 1e12 / FP32_VALU_PEAK_TFLOPS, 4),
+                                 "avg_launch_ms": round(1e3 * t_, 4)}
+
+    accuracy = None
+    gold = load_golden(name)
+    if gold is not None and not args.no_compare:
+        accuracy = accuracy_vs_oracle(gold, y, A, cfg, lams, L, pb, nrmse_truth, args.form, ctx)
+        for f, r in accuracy["forms"].items():
+            log(f"accuracy ({gold[0]['niter']} iterations) {f}: NRMSE {r['nrmse_vs_truth']:.7f} vs oracle "
+                f"{accuracy['oracle_nrmse_vs_truth']:.7f} (|d| {r['delta_nrmse_vs_oracle']:.2e}), image "
+                f"subset {r['subset_rel_err']:.2e}, restarts match {r['restarts_match']}")
 
     grad = None
     if not args.no_grad:
@@ -503,15 +669,6 @@ def main() -> int:
         grad = differentiation_run(ctx)
         log(f"differentiation (c4, {grad['niter']} iterations): forward {grad['forward_checkpointed_s']:.3f}s "
             f"backward {grad['backward_s']:.3f}s dloss/dlambda {grad['dloss_dlambda']} ({time.time() - t:.1f}s)")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        t = time.time()
-        y_norm = (y / np.float32(sf)).astype(np.complex64)
-        v, cores, sample, vf = cpu_baseline(cfg, pb, y_norm, smaps)
-        cpu = {"value": round(v, 6), "unit": "iter/s", "cores": cores, "kind": "port", "sample": sample,
-               "reference_faithful_value": round(vf, 6)}
-        log(f"cpu baseline {v:.5f} iter/s ({cores} threads) in {time.time() - t:.1f}s")
 
     line = {
         "metric": metric_label(cfg, name),
@@ -530,11 +687,14 @@ def main() -> int:
                    "form": args.form, "parallelism": "single-gpu", "restarts_timed": sum(restarts[W:]),
                    "logger": args.logger, "svst_tuning": tuned or "built-in"},
         "nrmse_vs_truth": nr_form,
-        "delta_nrmse": abs(nr_form - plain["nrmse_vs_truth"]) if plain else None,
+        "delta_nrmse": (accuracy["forms"][args.form]["delta_nrmse_vs_oracle"] if accuracy else None),
+        "delta_nrmse_vs_plain": abs(nr_form - plain["nrmse_vs_truth"]) if plain else None,
+        "accuracy_vs_fp64_oracle": accuracy,
         "plain": plain,
         "batched_fp32": others.get("batched"),
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
         "roofline": roof,
+        "roofline_valu": roof_valu or None,
         "cpu_baseline": cpu,
         "kernels_ms_per_iter": {k: round(v[1] / K, 3) for k, v in kern.items()},
         "with_logging": with_logging,

@@ -106,12 +106,18 @@ mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
  *                    eigenvector, larger ones 4 (built-in 65536); -1 = always 4.
  *   svst_apply:      1 = form W for every patch (dense apply); 0 = low-rank
  *                    products X U_r for patches with 2 r <= m (built-in).
+ *   sense_chunks:    frame groups of the POGM gradient (built-in 1, at most 64):
+ *                    A and A' run group by group and each group's update follows its
+ *                    adjoint; the batched forms alternate groups between two streams
+ *                    so the groups' memory-bound and FFT-bound kernels overlap.
+ *                    Changes the summation order of the restart test's FP64 sums only.
  * Invalid values -> MSLR_ERR_ARG. */
 typedef struct mslr_tuning {
     int gram_chunk;
     int eig_lanes_limit;
     int svst_apply;
-    int reserved[5];
+    int sense_chunks;
+    int reserved[4];
 } mslr_tuning;
 mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t);
 mslr_status_t mslr_ctx_get_tuning(mslr_ctx ctx, mslr_tuning *t);
@@ -252,6 +258,36 @@ mslr_status_t mslr_prox_scales(mslr_ctx ctx, void *X_dev, const int64_t N[3], in
 mslr_status_t mslr_patch_svst(mslr_ctx ctx, const void *img_dev, void *out_dev,
                               const int64_t N[3], int64_t Nt, const int64_t ps[3],
                               const int64_t ss[3], double beta);
+
+/* ------------------------------------------------------------------ setup */
+/* The setup of run_recon (scripts/reconstruct.jl:88-118,145-147) on the device.
+ *
+ * smaps <- smaps ./ (sqrt.(sum(abs2.(smaps); dims=4)) .+ eps(Float32)) in place
+ * (scripts/reconstruct.jl:89): ComplexF32 (Nx,Ny,Nz,Nc), FP32 throughout, the coils
+ * summed in order.  Asynchronous. */
+mslr_status_t mslr_normalise_smaps(mslr_ctx ctx, void *smaps_dev, const int64_t N[3], int64_t Nc);
+/* Mask inference, validation and flatten (scripts/reconstruct.jl:109-118,145-147) of the
+ * zero-filled k-space ksp0_dev, ComplexF32 (Nx,Ny,Nz,Nvc,Nt):
+ *   mask_dev (uint8 (Nx,Ny,Nz,Nt), device, written) = ksp0[:,:,:,1,:] .!= 0;
+ *   MSLR_ERR_SHAPE "Coil c has a different sampling pattern" for the first coil whose
+ *   nonzeros differ from coil 1's, then "Frame t has a different sample count" for the
+ *   first frame whose count differs from the previous frame's (1-based, as the
+ *   reference's @assert messages);
+ *   *K = samples per frame; y_dev (nullable; y_capacity values) receives (K, Nvc, Nt),
+ *   ksp[:, :, t] = reshape(ksp0, :, Nvc, Nt)[vec(mask_t), :, t].
+ * Synchronises (the assertions are decided on the host). */
+mslr_status_t mslr_kspace_flatten(mslr_ctx ctx, const void *ksp0_dev, const int64_t N[3],
+                                  int64_t Nvc, int64_t Nt, uint8_t *mask_dev, void *y_dev,
+                                  int64_t y_capacity, int64_t *K);
+/* Scale factor of run_recon: quantile(vec(abs.(sense_comb(ksp0, smaps))), q), type 7
+ * (scripts/reconstruct.jl:102-103; sense_comb src/recon.jl:312-323), from the flattened
+ * k-space y_dev (K, Nc, Nt) and the plan's maps: sense_comb = A'y / (sqrt(V) (sum_c |S_c|^2
+ * + eps)) with A' the plan's exact adjoint (any 2/3/5 size, odd ones included); the order
+ * statistics by a radix select on the device.  Synchronises. */
+mslr_status_t mslr_sense_scale_factor(mslr_sense plan, const void *y_dev, double q, double *sf);
+/* y <- y ./ max(sf, eps(Float32)) in place for n ComplexF32 values (the division in FP64,
+ * as ComplexF32 ./= Float64; scripts/reconstruct.jl:104).  Asynchronous. */
+mslr_status_t mslr_scale_kspace(mslr_ctx ctx, void *y_dev, int64_t n, double sf);
 
 /* ------------------------------------------------------------------ analysis */
 /* tSNR(img) (src/analysis.jl:39-44): out[v] = mean_t |img[v,t]| / (std_t |img[v,t]| +

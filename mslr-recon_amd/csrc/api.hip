@@ -174,6 +174,8 @@ mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t) {
         MSLR_REQUIRE(t->eig_lanes_limit >= -1, MSLR_ERR_ARG, "eig_lanes_limit must be >= -1");
         MSLR_REQUIRE(t->svst_apply == 0 || t->svst_apply == 1, MSLR_ERR_ARG,
                      "svst_apply must be 0 or 1");
+        MSLR_REQUIRE(t->sense_chunks >= 0 && t->sense_chunks <= 64, MSLR_ERR_ARG,
+                     "sense_chunks must be in [0, 64]");
         for (int r : t->reserved) MSLR_REQUIRE(r == 0, MSLR_ERR_ARG, "reserved fields must be 0");
         c->tuning = *t;
     });

@@ -191,7 +191,8 @@ enum ScratchSlot : size_t {
     SLOT_LOWRANK = 12,  // X U_r of the low-rank SVST apply
     SLOT_VJP = 13,      // m x m intermediates of the SVST derivative
     SLOT_VJP_P = 14,    // patch cotangents of the prox derivative
-    SLOT_COUNT = 15
+    SLOT_SETUP = 15,    // run_recon setup: chunk counts, magnitudes, selection state
+    SLOT_COUNT = 16
 };
 
 }  // namespace mslr

@@ -48,7 +48,11 @@ struct PogmDev {
 #ifndef MSLR_UPD_BLOCKS
 #define MSLR_UPD_BLOCKS 2048
 #endif
-constexpr int UPD_BLOCKS = MSLR_UPD_BLOCKS;  // workgroups of the fused update
+constexpr int UPD_BLOCKS = MSLR_UPD_BLOCKS;  // workgroups of the fused update (per frame group)
+// frame groups of the gradient + update: at c1 (90x90x60x300, 18 coils) 1, 2, 4, 6 and
+// 8 groups measured 47.7, 47.7, 48.2, 48.1, 47.9 ms per mixed iteration: the FFT passes
+// fill every wave slot, so the other stream's memory-bound kernels find no room
+constexpr int SENSE_CHUNKS_DEFAULT = 1;
 
 // scal_log (nullable): alpha, beta, gamma, zeta of each iteration, for the
 // derivative's backward pass.
@@ -162,7 +166,8 @@ __global__ __launch_bounds__(256) void k_pogm_update(c64 *__restrict__ X, c64 *_
     double dot = 0.0, gg = 0.0, dd = 0.0;
     const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if ((n & 1) == 0 && (sstride & 1) == 0) {
+    // (X, U and g start at one frame offset: all 16-byte aligned or none)
+    if ((n & 1) == 0 && (sstride & 1) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0) {
         const int64_t n2 = n / 2, s2 = sstride / 2;
         float4 *X4 = reinterpret_cast<float4 *>(X);
         float4 *U4 = reinterpret_cast<float4 *>(U);
@@ -448,6 +453,11 @@ struct mslr_pogm_s {
     // scratch generation the graph was captured at / of the last eager iteration
     uint64_t graph_gen = 0, warm_gen = 0;
     bool warm = false;
+    // the gradient's frame groups, each followed by its part of the update; the
+    // batched forms alternate them between the context stream and gpipe
+    std::vector<std::pair<int64_t, int64_t>> gchunks;
+    hipStream_t gpipe = nullptr;
+    hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;
     // batched forms: one stream per scale for the prox (fork / join with events)
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> ev_join;
@@ -473,6 +483,9 @@ static void pogm_free(mslr_pogm_s *st) {
     if (!st) return;
     drop_graph(st);
     for (auto s2 : st->aux) hipStreamDestroy(s2);
+    if (st->gpipe) hipStreamDestroy(st->gpipe);
+    if (st->ev_gfork) hipEventDestroy(st->ev_gfork);
+    if (st->ev_gjoin) hipEventDestroy(st->ev_gjoin);
     for (auto e : st->ev_join) hipEventDestroy(e);
     if (st->ev_fork) hipEventDestroy(st->ev_fork);
     for (void *p : st->allocs) hipFree(p);
@@ -493,23 +506,34 @@ static void log_reg_cost(mslr_pogm_s *st) {
 static void pogm_iteration(mslr_pogm_s *st) {
     mslr_ctx_s *ctx = st->ctx;
     const int64_t n = st->V * st->Nt;
-    sense_grad_internal(st->sense, st->X, (int)st->S, n, st->y, st->g,
-                        st->log_costs ? st->dc_part : nullptr);
-    if (st->log_costs) {
-        MSLR_PROF(ctx, "cost_dc");
-        k_cost_dc<<<1, 1024, 0, ctx->stream>>>(st->dc_part, st->n_dc_part, st->dev, st->costs);
-        HIP_LAUNCH_CHECK();
-    }
-    {
+    {   // beta, gamma, zeta of this iteration (they do not depend on the gradient)
         MSLR_PROF(ctx, "pogm_pre");
         k_pogm_pre<<<1, 64, 0, ctx->stream>>>(st->dev, st->lam_dev, (int)st->S, st->thr_dev,
                                               st->scal_log);
         HIP_LAUNCH_CHECK();
     }
-    {
-        MSLR_PROF(ctx, "pogm_update");
-        k_pogm_update<<<UPD_BLOCKS, 256, 0, ctx->stream>>>(st->X, st->U, st->g, n, (int)st->S, n,
-                                                           st->dev, st->partials);
+    // gradient frame group by frame group, each group's update right behind its
+    // adjoint (on the stream the group ran on)
+    SensePipe sp;
+    if (st->gpipe && !ctx->prof) {
+        sp.stream = st->gpipe;
+        sp.slot_base = (size_t)SLOT_COUNT * (size_t)(st->S + 1);
+        sp.fork = st->ev_gfork;
+        sp.join = st->ev_gjoin;
+    }
+    sense_grad_pipelined(st->sense, st->X, (int)st->S, n, st->y, st->g,
+                         st->log_costs ? st->dc_part : nullptr, st->gchunks, sp,
+                         [&](size_t i, int64_t ta, int64_t nf) {
+                             MSLR_PROF(ctx, "pogm_update");
+                             const int64_t o = ta * st->V;
+                             k_pogm_update<<<UPD_BLOCKS, 256, 0, ctx->stream>>>(
+                                 st->X + o, st->U + o, st->g + o, nf * st->V, (int)st->S, n,
+                                 st->dev, st->partials + 3 * (size_t)UPD_BLOCKS * i);
+                             HIP_LAUNCH_CHECK();
+                         });
+    if (st->log_costs) {
+        MSLR_PROF(ctx, "cost_dc");
+        k_cost_dc<<<1, 1024, 0, ctx->stream>>>(st->dc_part, st->n_dc_part, st->dev, st->costs);
         HIP_LAUNCH_CHECK();
     }
     if (st->checkpoint) {
@@ -521,7 +545,8 @@ static void pogm_iteration(mslr_pogm_s *st) {
     }
     {
         MSLR_PROF(ctx, "pogm_post");
-        k_pogm_post<<<1, 256, 0, ctx->stream>>>(st->dev, st->partials, UPD_BLOCKS, st->log_dev);
+        k_pogm_post<<<1, 256, 0, ctx->stream>>>(st->dev, st->partials,
+                                                UPD_BLOCKS * (int)st->gchunks.size(), st->log_dev);
         HIP_LAUNCH_CHECK();
     }
     int *err = (int *)((char *)st->dev + offsetof(PogmDev, err));
@@ -645,7 +670,13 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
             st->dev = (PogmDev *)st->alloc(sizeof(PogmDev));
             st->lam_dev = (double *)st->alloc(sizeof(double) * st->S);
             st->thr_dev = (double *)st->alloc(sizeof(double) * st->S);
-            st->partials = (double *)st->alloc(sizeof(double) * 3 * UPD_BLOCKS);
+            {
+                const int nch = sense->ctx->tuning.sense_chunks > 0 ? sense->ctx->tuning.sense_chunks
+                                                                    : SENSE_CHUNKS_DEFAULT;
+                st->gchunks = sense_grad_chunks(sense, nch);
+            }
+            st->partials = (double *)st->alloc(sizeof(double) * 3 * UPD_BLOCKS *
+                                               std::max<size_t>(1, st->gchunks.size()));
             st->log_dev = (int *)st->alloc(sizeof(int) * std::max(1, d.niter));
             HIP_CHECK(hipMemcpy(st->lam_dev, st->lambdas.data(), sizeof(double) * st->S,
                                 hipMemcpyHostToDevice));
@@ -693,6 +724,11 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
                 st->nuc_tot = (double *)st->alloc(sizeof(double) * (size_t)st->S);
                 for (const PatchGeom &g : st->geoms)
                     st->nuc_part.push_back((double *)st->alloc(sizeof(double) * (size_t)nuc_units(g)));
+            }
+            if (d.form != MSLR_FORM_PLAIN && st->gchunks.size() > 1) {
+                HIP_CHECK(hipStreamCreateWithFlags(&st->gpipe, hipStreamNonBlocking));
+                HIP_CHECK(hipEventCreateWithFlags(&st->ev_gfork, hipEventDisableTiming));
+                HIP_CHECK(hipEventCreateWithFlags(&st->ev_gjoin, hipEventDisableTiming));
             }
             if (d.form != MSLR_FORM_PLAIN && d.nscales > 1) {
                 for (int64_t k = 0; k < d.nscales; ++k) {

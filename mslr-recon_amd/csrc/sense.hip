@@ -17,6 +17,7 @@
 // transpose of the forward.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -734,6 +735,58 @@ void sense_grad_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sst
                              y_int + ta * P->Nc * P->K, r,
                              dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr);
         adjoint_chunk<float>(P, r, ta, (int)nf, g + ta * P->V);
+    }
+}
+
+std::vector<std::pair<int64_t, int64_t>> sense_grad_chunks(mslr_sense_s *P, int nch) {
+    std::vector<std::pair<int64_t, int64_t>> out;
+    nch = std::max(1, std::min(nch, P->Nt));
+    for (int i = 0; i < nch; ++i) {
+        const int64_t a = (int64_t)P->Nt * i / nch, b = (int64_t)P->Nt * (i + 1) / nch;
+        if (b > a)
+            for (auto c : frame_chunks<float>(P, a, b - a)) out.push_back(c);
+    }
+    return out;
+}
+
+void sense_grad_pipelined(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
+                          const c64 *y_int, c64 *g, double *dc_partial,
+                          const std::vector<std::pair<int64_t, int64_t>> &chunks,
+                          const SensePipe &pipe,
+                          const std::function<void(size_t, int64_t, int64_t)> &after) {
+    mslr_ctx_s *ctx = P->ctx;
+    check_lds<float>(P);
+    hipStream_t main = ctx->stream;
+    const size_t base0 = ctx->slot_base;
+    const bool two = pipe.stream != nullptr && chunks.size() > 1;
+    if (two) {
+        HIP_CHECK(hipEventRecord(pipe.fork, main));
+        HIP_CHECK(hipStreamWaitEvent(pipe.stream, pipe.fork, 0));
+    }
+    try {
+        for (size_t i = 0; i < chunks.size(); ++i) {
+            const auto [ta, nf] = chunks[i];
+            const bool on_pipe = two && (i % 2 == 1);
+            ctx->stream = on_pipe ? pipe.stream : main;
+            ctx->slot_base = on_pipe ? pipe.slot_base : base0;
+            c64 *r = (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, nf * P->Nc * P->K) *
+                                                             sizeof(c64));
+            forward_chunk<float>(P, X + ta * P->V, nscales, sstride, ta, (int)nf,
+                                 y_int + ta * P->Nc * P->K, r,
+                                 dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr);
+            adjoint_chunk<float>(P, r, ta, (int)nf, g + ta * P->V);
+            after(i, ta, nf);
+        }
+    } catch (...) {
+        ctx->stream = main;
+        ctx->slot_base = base0;
+        throw;
+    }
+    ctx->stream = main;
+    ctx->slot_base = base0;
+    if (two) {
+        HIP_CHECK(hipEventRecord(pipe.join, pipe.stream));
+        HIP_CHECK(hipStreamWaitEvent(main, pipe.join, 0));
     }
 }
 

@@ -1,6 +1,10 @@
 // All-frames SENSE plan shared by the SENSE entry points and the POGM driver.
 #pragma once
 
+#include <functional>
+#include <utility>
+#include <vector>
+
 #include "common.hpp"
 #include "fft.hpp"
 
@@ -48,6 +52,28 @@ void sense_ref_to_internal(mslr_sense_s *P, const c64 *y_ref, c64 *y_int);
 // order.  dc_partial (device, nullable) receives Nt*Nc*nlb partial ||r||^2.
 void sense_grad_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
                          const c64 *y_int, c64 *g, double *dc_partial);
+
+// Frame chunks of the pipelined gradient: nch groups of consecutive frames (as
+// equal as the frame count allows), each split further where it exceeds the
+// plan's scratch budget.
+std::vector<std::pair<int64_t, int64_t>> sense_grad_chunks(mslr_sense_s *P, int nch);
+
+// Second stream of the pipelined gradient: chunks alternate between the context
+// stream and `stream` (scratch slots offset by slot_base there); fork/join events.
+struct SensePipe {
+    hipStream_t stream = nullptr;  // nullptr: every chunk on the context stream, in order
+    size_t slot_base = 0;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+// g = A'(A sum_k X_k - y_int) chunk by chunk; after(i, ta, nf) enqueues work on
+// the chunk's frames [ta, ta+nf) behind chunk i's adjoint on the stream the chunk
+// ran on (ctx->stream while it is called).  The context stream waits for all.
+void sense_grad_pipelined(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
+                          const c64 *y_int, c64 *g, double *dc_partial,
+                          const std::vector<std::pair<int64_t, int64_t>> &chunks,
+                          const SensePipe &pipe,
+                          const std::function<void(size_t, int64_t, int64_t)> &after);
 
 // Partial ||A sum_k X_k - y_int||^2 (Nt*Nc*nlb doubles into dc_partial), the
 // forward half of sense_grad_internal.

@@ -11,7 +11,9 @@ this file's package directory (../mslr_hip/libmslr_hip.so).
 =#
 module MSLRHip
 
-export HipContext, HipArray, Asense_hip, img2patches_hip, patches2img_hip, SVST_hip,
+using LinearMapsAA: LinearMapAA   # a dependency of the reference (Project.toml)
+
+export HipContext, HipArray, Asense_hip, Asense_hip_frames, img2patches_hip, patches2img_hip, SVST_hip,
        patchSVST_hip, patch_nucnorm_hip, pogm_hip, poweriter_hip, upload, download,
        set_products!, SVST_vjp_hip, pogm_checkpointed_hip, lambda_vjp_hip, pogm_lambda_grad_hip,
        tSNR_hip, detrend_hip!, set_tuning!, get_tuning
@@ -61,35 +63,36 @@ function set_products!(ctx::HipContext, products::Symbol)
     return ctx
 end
 
-# mslr_tuning (include/mslr.h): launch shapes of the batched SVST
+# mslr_tuning (include/mslr.h): launch shapes
 struct Tuning
     gram_chunk::Cint
     eig_lanes_limit::Cint
     svst_apply::Cint
-    reserved::NTuple{5,Cint}
+    sense_chunks::Cint
+    reserved::NTuple{4,Cint}
 end
 
 """
-    set_tuning!(ctx; gram_chunk=0, eig_lanes_limit=0, svst_apply=0)
+    set_tuning!(ctx; gram_chunk=0, eig_lanes_limit=0, svst_apply=0, sense_chunks=0)
 
-Launch shapes of the batched SVST on `ctx` (0 keeps the built-in choice): rows per
-Gram workgroup (a multiple of 256 in [256, 65536]), the patches x m limit for 16 lanes
-per eigenvector (-1: always 4), `svst_apply = 1` for the dense apply.  The Python
-autotuner (`mslr_hip.autotune.autotune_pogm`) picks them from timed POGM iterations;
-its JSON cache holds the values to pass here.
+Launch shapes on `ctx` (0 keeps the built-in choice): rows per Gram workgroup (a
+multiple of 256 in [256, 65536]), the patches x m limit for 16 lanes per eigenvector
+(-1: always 4), `svst_apply = 1` for the dense apply, the frame groups of the POGM
+gradient (at most 64).  The Python autotuner (`mslr_hip.autotune.autotune_pogm`)
+picks them from timed POGM iterations; its JSON cache holds the values to pass here.
 """
 function set_tuning!(ctx::HipContext; gram_chunk::Integer = 0, eig_lanes_limit::Integer = 0,
-                     svst_apply::Integer = 0)
-    t = Ref(Tuning(gram_chunk, eig_lanes_limit, svst_apply, ntuple(_ -> Cint(0), 5)))
+                     svst_apply::Integer = 0, sense_chunks::Integer = 0)
+    t = Ref(Tuning(gram_chunk, eig_lanes_limit, svst_apply, sense_chunks, ntuple(_ -> Cint(0), 4)))
     check(ccall((:mslr_ctx_set_tuning, LIB), Cint, (Ptr{Cvoid}, Ref{Tuning}), ctx.h, t))
     return ctx
 end
 
 function get_tuning(ctx::HipContext)
-    t = Ref(Tuning(0, 0, 0, ntuple(_ -> Cint(0), 5)))
+    t = Ref(Tuning(0, 0, 0, 0, ntuple(_ -> Cint(0), 4)))
     check(ccall((:mslr_ctx_get_tuning, LIB), Cint, (Ptr{Cvoid}, Ref{Tuning}), ctx.h, t))
     return (gram_chunk = Int(t[].gram_chunk), eig_lanes_limit = Int(t[].eig_lanes_limit),
-            svst_apply = Int(t[].svst_apply))
+            svst_apply = Int(t[].svst_apply), sense_chunks = Int(t[].sense_chunks))
 end
 
 "Column-major device array owned by the library (ComplexF32 unless stated)."
@@ -190,6 +193,45 @@ function Base.:*(Ah::AdjointHip, y::HipArray{ComplexF32})
     return x
 end
 Base.:*(Ah::AdjointHip, y::Array) = download(Ah * upload(ComplexF32.(y); ctx = Ah.A.ctx))
+
+"""
+    Asense_hip_frames(Ω, smaps; fft_forward=true, unitary=true) -> Vector of LinearMapAA
+
+One `LinearMapAA` per frame t with idim = (Nx,Ny,Nz), odim = (K,Nc) and T = ComplexF32,
+the interface of `Asense_gpu(samp, smaps)` (src/sense_gpu.jl:56-108), so that the
+reference's own assembly works unchanged:
+
+    A = block_diag(Asense_hip_frames(Ω, smaps)...)      # scripts/reconstruct.jl:142
+    σ1A = poweriter_mod(undim(A))                         # scripts/reconstruct.jl:154
+
+All frames share one all-frames plan (one copy of the coil maps on the device); frame
+t's forward and adjoint are `mslr_sense_forward/adjoint` on frames [t-1, t).  Host
+arrays in and out, as LinearMapsAA passes them.  (The fast path is `Asense_hip`, which
+applies all frames in one call on device arrays.)
+"""
+function Asense_hip_frames(Ω::AbstractArray{Bool,4}, smaps::AbstractArray{<:Complex,4};
+                           fft_forward::Bool = true, unitary::Bool = true, ctx = default_context())
+    A = Asense_hip(Ω, smaps; fft_forward, unitary, ctx)
+    N, K, Nc = A.N, A.K, A.Nc
+    function frame_map(t::Int)
+        fwd = function (x)
+            xd = upload(reshape(ComplexF32.(x), N..., 1); ctx = A.ctx)
+            yd = HipArray{ComplexF32}((K, Nc, 1); ctx = A.ctx)
+            check(ccall((:mslr_sense_forward, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64),
+                        A.h, xd.ptr, yd.ptr, t - 1, 1))
+            return reshape(download(yd), K, Nc)
+        end
+        adj = function (y)
+            yd = upload(reshape(ComplexF32.(y), K, Nc, 1); ctx = A.ctx)
+            xd = HipArray{ComplexF32}((N..., 1); ctx = A.ctx)
+            check(ccall((:mslr_sense_adjoint, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64),
+                        A.h, yd.ptr, xd.ptr, t - 1, 1))
+            return reshape(download(xd), N...)
+        end
+        return LinearMapAA(fwd, adj, (K * Nc, prod(N)); idim = N, odim = (K, Nc), T = ComplexF32)
+    end
+    return [frame_map(t) for t in 1:A.Nt]
+end
 
 "poweriter_mod(undim(A)) in FP64 on the device (src/mirt_mod.jl:215-238); returns (nothing, σ1)."
 function poweriter_hip(A::Asense_hip; niter::Int = 200, tol::Real = 1e-6, chat::Bool = true)

@@ -86,7 +86,8 @@ class mslr_tuning(C.Structure):
         ("gram_chunk", C.c_int),
         ("eig_lanes_limit", C.c_int),
         ("svst_apply", C.c_int),
-        ("reserved", C.c_int * 5),
+        ("sense_chunks", C.c_int),
+        ("reserved", C.c_int * 4),
     ]
 
 
@@ -133,6 +134,11 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_patch_nucnorm": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd]),
     "mslr_svst_vjp": (C.c_int, [_p, _p, _p, _p, _i64, _i64, _i64, _pd, C.c_int, _pd]),
     "mslr_tsnr": (C.c_int, [_p, _p, _i64, _i64, _p]),
+    "mslr_normalise_smaps": (C.c_int, [_p, _p, C.POINTER(C.c_int64), _i64]),
+    "mslr_kspace_flatten": (C.c_int, [_p, _p, C.POINTER(C.c_int64), _i64, _i64, _p, _p, _i64,
+                                      C.POINTER(C.c_int64)]),
+    "mslr_sense_scale_factor": (C.c_int, [_p, _p, C.c_double, C.POINTER(C.c_double)]),
+    "mslr_scale_kspace": (C.c_int, [_p, _p, _i64, C.c_double]),
     "mslr_detrend": (C.c_int, [_p, _p, _i64, _i64]),
     "mslr_patch_svst": (C.c_int, [_p, _p, _p, _pi64, _i64, _pi64, _pi64, C.c_double]),
     "mslr_prox_scales": (C.c_int, [_p, _p, _pi64, _i64, _i64, _pi64, _pi64, _pd]),

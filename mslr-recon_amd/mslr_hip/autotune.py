@@ -3,8 +3,10 @@
 The shapes (mslr_ctx_set_tuning) are the Gram contraction chunk (rows per
 workgroup: more chunks fill the device for tall patches but add partial Grams
 to sum), the eigenvector lanes (16 per eigenvalue hide the bisection latency of
-small batches, 4 suit batches that fill the device) and the apply form (low-rank
-X U_r products or the dense W).  Which is fastest depends on the patch shapes,
+small batches, 4 suit batches that fill the device), the apply form (low-rank
+X U_r products or the dense W) and the frame groups of the gradient (more groups
+overlap more of the memory-bound kernels with the FFT passes on two streams, at
+smaller launches).  Which is fastest depends on the patch shapes,
 the patch count and what the other scales run beside them, so the tuner times
 whole POGM iterations of the workload itself: coordinate descent from the
 built-in choice, one shape at a time, each candidate timed over `iters`
@@ -27,6 +29,7 @@ SPACE: dict[str, tuple[int, ...]] = {
     "gram_chunk": (0, 2048, 8192, 16384),
     "eig_lanes_limit": (0, -1, 1 << 20),
     "svst_apply": (0, 1),
+    "sense_chunks": (0, 2, 4, 8),
 }
 BUILTIN = {k: 0 for k in SPACE}
 

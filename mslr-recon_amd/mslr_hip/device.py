@@ -44,18 +44,20 @@ class Context:
         code = {"fp32": _lib.PRODUCTS_FP32, "split_bf16": _lib.PRODUCTS_SPLIT_BF16}[products]
         _lib.check(self._lib.mslr_ctx_set_products(self.handle, code))
 
-    def set_tuning(self, gram_chunk: int = 0, eig_lanes_limit: int = 0, svst_apply: int = 0) -> None:
-        """Launch shapes of the batched SVST (0 = built-in): contraction rows per
-        Gram workgroup (a multiple of 256), the patches x m limit for 16 lanes per
-        eigenvector (-1: always 4), and svst_apply = 1 for the dense apply."""
-        t = _lib.mslr_tuning(int(gram_chunk), int(eig_lanes_limit), int(svst_apply))
+    def set_tuning(self, gram_chunk: int = 0, eig_lanes_limit: int = 0, svst_apply: int = 0,
+                   sense_chunks: int = 0) -> None:
+        """Launch shapes (0 = built-in): contraction rows per Gram workgroup (a
+        multiple of 256), the patches x m limit for 16 lanes per eigenvector (-1:
+        always 4), svst_apply = 1 for the dense apply, and the frame groups of the
+        POGM gradient (sense_chunks, at most 64)."""
+        t = _lib.mslr_tuning(int(gram_chunk), int(eig_lanes_limit), int(svst_apply), int(sense_chunks))
         _lib.check(self._lib.mslr_ctx_set_tuning(self.handle, C.byref(t)))
 
     def tuning(self) -> dict:
         t = _lib.mslr_tuning()
         _lib.check(self._lib.mslr_ctx_get_tuning(self.handle, C.byref(t)))
         return {"gram_chunk": t.gram_chunk, "eig_lanes_limit": t.eig_lanes_limit,
-                "svst_apply": t.svst_apply}
+                "svst_apply": t.svst_apply, "sense_chunks": t.sense_chunks}
 
     def prof_enable(self, on: bool = True) -> None:
         """Time every kernel launch with HIP events (clears earlier records)."""

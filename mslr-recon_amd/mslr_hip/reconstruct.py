@@ -6,20 +6,20 @@
 `run_recon_arrays` takes the k-space and coil maps as arrays instead of file
 names (the reference reads HDF5 k-space, which needs h5py -- not available in
 this image) and returns the dict the reference writes to its .mat file
-(scripts/reconstruct.jl:242-260).  Setup (smaps normalisation, the 0.99-quantile
-k-space scale, mask inference and validation, scripts/reconstruct.jl:86-118)
-runs on the host; its one transform (the SENSE combination) runs through the
-GPU adjoint.
+(scripts/reconstruct.jl:242-260).  Setup (smaps normalisation, mask inference and
+validation, the k-space flatten, the 0.99-quantile scale factor of the SENSE
+combination, scripts/reconstruct.jl:86-118,145-147) runs on the device.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import time
 
 import numpy as np
 
 from . import _lib
-from .device import Context, DeviceArray, default_context
+from .device import Context, DeviceArray, as_device, default_context
 from .mirt_mod import POGM, poweriter_mod
 from .sense import SenseOp
 
@@ -34,27 +34,33 @@ def lambdas_ong_lustig(N, Nt, patch_sizes):
             math.sqrt(math.log(Nvox * Nt / max(int(np.prod(ps)), Nt))) for ps in patch_sizes]
 
 
-def normalise_smaps(smaps_raw) -> np.ndarray:
-    """ComplexF32 maps divided by their root-sum-of-squares + eps(Float32) (scripts/reconstruct.jl:88-89)."""
-    s = np.asarray(smaps_raw).astype(np.complex64)
-    rss = np.sqrt(np.sum(np.abs(s) ** 2, axis=3, keepdims=True))
-    return np.asfortranarray(s / (rss + np.float32(EPS32)))
+def normalise_smaps(smaps_raw, ctx: Context | None = None) -> np.ndarray:
+    """smaps ./ (sqrt.(sum(abs2.(smaps); dims=4)) .+ eps(Float32)) (scripts/reconstruct.jl:88-89)
+    on the device (mslr_normalise_smaps); returns the ComplexF32 maps on the host, which
+    the SENSE plan is built from."""
+    ctx = ctx or default_context()
+    s = np.asfortranarray(np.asarray(smaps_raw).astype(np.complex64))
+    if s.ndim != 4:
+        raise ValueError("smaps must be (Nx, Ny, Nz, Nc)")
+    d = DeviceArray.from_host(s, ctx)
+    _lib.check(ctx._lib.mslr_normalise_smaps(ctx.handle, d.ptr, _lib.i64x3(s.shape[:3]), s.shape[3]))
+    return d.to_host()
 
 
-def sense_comb_scale(A: SenseOp, y, smaps: np.ndarray) -> float:
-    """0.99 quantile of |sense_comb(ksp)| (scripts/reconstruct.jl:102-103).
+def sense_comb_scale(A: SenseOp, y, q: float = 0.99) -> float:
+    """quantile(vec(abs.(sense_comb(ksp0, smaps))), q) (scripts/reconstruct.jl:102-103) from the
+    flattened k-space y (K, Nc, Nt) through the plan's exact adjoint, on the device
+    (mslr_sense_scale_factor; any 2/3/5 image size)."""
+    yd = as_device(y, np.complex64, A.ctx)
+    sf = C.c_double(0.0)
+    _lib.check(A.ctx._lib.mslr_sense_scale_factor(A.handle, yd.ptr, float(q), C.byref(sf)))
+    return sf.value
 
-    For even sizes sense_comb(ksp) = A'y / (sqrt(prod(N)) * (sum|S|^2 + eps)), with
-    A' the exact adjoint computed on the GPU.
-    """
-    if any(n % 2 for n in A.N):
-        raise _lib.MSLRUnsupported("the scale factor uses the adjoint; odd image sizes shift differently")
-    img = A.adjoint(y).to_host()
-    V = float(np.prod(A.N))
-    den = (np.sum(np.abs(smaps.astype(np.complex128)) ** 2, axis=3) + EPS64) * math.sqrt(V)
-    mag = np.abs(img)
-    mag /= den[..., None].astype(np.float32)
-    return float(np.quantile(mag.ravel(order="F").astype(np.float64), 0.99, method="linear"))
+
+def scale_kspace(yd: DeviceArray, sf: float) -> None:
+    """y ./= max(sf, eps(Float32)) in place (scripts/reconstruct.jl:104; FP64 division)."""
+    n = yd.nbytes // 8
+    _lib.check(yd.ctx._lib.mslr_scale_kspace(yd.ctx.handle, yd.ptr, n, float(sf)))
 
 
 def prepare_flat(smaps_raw, mask, y_raw, ctx: Context | None = None):
@@ -63,30 +69,51 @@ def prepare_flat(smaps_raw, mask, y_raw, ctx: Context | None = None):
     Returns (A, smaps, y_device_normalised, scale_factor, R).
     """
     ctx = ctx or default_context()
-    smaps = normalise_smaps(smaps_raw)
+    smaps = normalise_smaps(smaps_raw, ctx)
     A = SenseOp(mask, smaps, ctx=ctx)
-    yd = DeviceArray.from_host(np.asarray(y_raw), ctx, dtype=np.complex64)
-    sf = sense_comb_scale(A, yd, smaps)
-    yh = np.asfortranarray(np.asarray(y_raw, dtype=np.complex64) / np.float32(max(sf, EPS32)))
-    yd.upload(yh)
+    # a caller's device buffer is copied, not scaled in place
+    yd = y_raw.copy() if isinstance(y_raw, DeviceArray) else as_device(y_raw, np.complex64, ctx)
+    sf = sense_comb_scale(A, yd)
+    scale_kspace(yd, sf)
     R = float(np.prod(A.N)) / float(A.K)
     return A, smaps, yd, sf, R
 
 
-def flatten_kspace(ksp0, Nvc, Nt):
-    """Mask inference, validation and the (K, Nvc, Nt) flatten (scripts/reconstruct.jl:109-118,145-147)."""
-    ksp0 = np.asarray(ksp0)
-    omega = ksp0[:, :, :, 0, :] != 0
-    for ic in range(1, Nvc):
-        if not np.array_equal(omega, ksp0[:, :, :, ic, :] != 0):
-            raise AssertionError(f"Coil {ic + 1} has a different sampling pattern")
-    counts = omega.reshape(-1, Nt, order="F").sum(axis=0)
-    for it in range(1, Nt):
-        if counts[it] != counts[it - 1]:
-            raise AssertionError(f"Frame {it + 1} has a different sample count")
-    flat = ksp0.reshape(-1, Nvc, Nt, order="F")
-    y = np.stack([flat[np.flatnonzero(omega[..., t].ravel(order="F")), :, t] for t in range(Nt)], axis=2)
-    return omega, np.asfortranarray(y.astype(np.complex64))
+def flatten_kspace(ksp0, Nvc: int, Nt: int, ctx: Context | None = None):
+    """Mask inference, validation and the (K, Nvc, Nt) flatten of the zero-filled k-space
+    (scripts/reconstruct.jl:109-118,145-147) on the device (mslr_kspace_flatten).
+    ksp0: (Nx, Ny, Nz, Nvc, Nt), host or device.  Returns (omega host bool (Nx,Ny,Nz,Nt),
+    y DeviceArray (K, Nvc, Nt)); raises AssertionError with the reference's messages."""
+    ctx = ctx or default_context()
+    kd = as_device(ksp0, np.complex64, ctx)
+    N = tuple(int(v) for v in kd.shape[:3])
+    if tuple(kd.shape) != (*N, Nvc, Nt):
+        raise AssertionError(f"Unexpected k-space shape: {kd.shape}")
+    md = DeviceArray((*N, Nt), np.uint8, ctx)
+    K = C.c_int64(0)
+    _lib.check(ctx._lib.mslr_kspace_flatten(ctx.handle, kd.ptr, _lib.i64x3(N), Nvc, Nt, md.ptr, None, 0,
+                                            C.byref(K)))
+    yd = DeviceArray((K.value, Nvc, Nt), np.complex64, ctx)
+    _lib.check(ctx._lib.mslr_kspace_flatten(ctx.handle, kd.ptr, _lib.i64x3(N), Nvc, Nt, md.ptr, yd.ptr,
+                                            K.value * Nvc * Nt, C.byref(K)))
+    return md.to_host().astype(bool), yd
+
+
+def prepare_device(ksp0, smaps_raw, N, Nvc: int, Nt: int, ctx: Context | None = None):
+    """The setup of run_recon (scripts/reconstruct.jl:86-118,145-147) on the device:
+    normalised maps, the mask and its validation, the flattened k-space divided by the
+    0.99-quantile scale factor.  Returns (A, smaps, y DeviceArray, sf, R, omega)."""
+    ctx = ctx or default_context()
+    N = tuple(int(n) for n in N)
+    if tuple(np.shape(ksp0)) != (*N, Nvc, Nt):
+        raise AssertionError(f"Unexpected k-space shape: {tuple(np.shape(ksp0))}")
+    omega, yd = flatten_kspace(ksp0, Nvc, Nt, ctx)
+    smaps = normalise_smaps(smaps_raw, ctx)
+    A = SenseOp(omega, smaps, ctx=ctx)
+    sf = sense_comb_scale(A, yd)
+    scale_kspace(yd, sf)
+    R = float(np.prod(N)) / float(A.K)
+    return A, smaps, yd, sf, R, omega
 
 
 def run_recon_arrays(ksp0, smaps_raw, *, N, Nvc, Nt, PATCH_SIZES, STRIDES, NITERS,
@@ -98,8 +125,7 @@ def run_recon_arrays(ksp0, smaps_raw, *, N, Nvc, Nt, PATCH_SIZES, STRIDES, NITER
     if ksp0.shape != (*N, Nvc, Nt):
         raise AssertionError(f"Unexpected k-space shape: {ksp0.shape}")
     S = len(PATCH_SIZES)
-    omega, y_raw = flatten_kspace(ksp0.astype(np.complex64), Nvc, Nt)
-    A, smaps, yd, sf, R = prepare_flat(smaps_raw, omega, y_raw, ctx)
+    A, smaps, yd, sf, R, omega = prepare_device(ksp0, smaps_raw, N, Nvc, Nt, ctx)
     if verbose:
         print(f"  Scale factor = {sf:.4f}\nAcceleration factor R ≈ {R:.2f}")
     if σ1A_PRECOMPUTED is None:

@@ -380,10 +380,19 @@ def lambdas_ong_lustig(N, Nt, patch_sizes):
 
 
 def normalise_smaps(smaps_raw):
-    """scripts/reconstruct.jl:88-89."""
+    """smaps ./ (sqrt.(sum(abs2.(smaps); dims=4)) .+ eps(Float32)) in Float32
+    (scripts/reconstruct.jl:88-89): abs2 = re*re + im*im, summed over the coils in order
+    (Julia's reduction over a trailing dimension), each part divided by the result."""
     s = np.asarray(smaps_raw).astype(np.complex64)
-    rss = np.sqrt(np.sum(np.abs(s) ** 2, axis=3, keepdims=True))
-    return (s / (rss + np.float32(EPS32))).astype(np.complex64)
+    acc = np.zeros(s.shape[:3], dtype=np.float32)
+    for c in range(s.shape[3]):
+        re, im = s[..., c].real, s[..., c].imag
+        acc = acc + (re * re + im * im)
+    den = (np.sqrt(acc) + np.float32(EPS32))[..., None]
+    out = np.empty_like(s)
+    out.real = s.real / den
+    out.imag = s.imag / den
+    return out
 
 
 def sense_comb(ksp, smaps):
@@ -410,7 +419,8 @@ def prepare(ksp0, smaps_raw, N, Nvc, Nt):
     assert ksp0.shape == (*N, Nvc, Nt), f"Unexpected k-space shape: {ksp0.shape}"
     img0 = sense_comb(ksp0, smaps)
     sf = quantile_linear(np.abs(img0).ravel(order="F"), 0.99)
-    ksp0 = (ksp0 / np.float32(max(sf, EPS32))).astype(np.complex64)
+    # ksp0 ./= max(sf, eps(Float32)): ComplexF32 ./ Float64 divides in Float64, stores Float32
+    ksp0 = (ksp0.astype(np.complex128) / max(sf, EPS32)).astype(np.complex64)
     omega = ksp0[:, :, :, 0, :] != 0
     R = float(np.prod(N)) / float(np.sum(omega[..., 0]))
     for ic in range(1, Nvc):

@@ -42,7 +42,7 @@ def c1_setup():
     assert float(np.vdot(y.ravel(), y.ravel()).real) == pytest.approx(rec["y_norm2"], rel=1e-6)
     A = SenseOp(pb.mask, smaps, ctx=ctx)
     yd = DeviceArray.from_host(y, ctx)
-    sf_dev = sense_comb_scale(A, yd, smaps)
+    sf_dev = sense_comb_scale(A, yd)
     # the oracle's inputs: y / Float32(sf) with its scale factor
     yd.upload(np.asfortranarray(y / np.float32(rec["scale_factor"])))
     lams = lambdas_ong_lustig(tuple(cfg["N"]), cfg["Nt"], cfg["patch_sizes"])
@@ -95,7 +95,7 @@ def c2_setup():
     cfg, pb, smaps, y = production_inputs("c2", ctx)
     A = SenseOp(pb.mask, smaps, ctx=ctx)
     yd = DeviceArray.from_host(y, ctx)
-    sf = sense_comb_scale(A, yd, smaps)
+    sf = sense_comb_scale(A, yd)
     yd.upload(np.asfortranarray(y / np.float32(sf)))
     lams = lambdas_ong_lustig(tuple(cfg["N"]), cfg["Nt"], cfg["patch_sizes"])
     runs = {}

@@ -13,7 +13,7 @@ from test_gpu_svst import APPLY_RANKS, _beta_between, lowrank, mixed_rank_batch
 
 pytestmark = pytest.mark.gpu
 
-BUILTIN = {"gram_chunk": 0, "eig_lanes_limit": 0, "svst_apply": 0}
+BUILTIN = {"gram_chunk": 0, "eig_lanes_limit": 0, "svst_apply": 0, "sense_chunks": 0}
 
 
 @pytest.fixture
@@ -26,11 +26,12 @@ def shapes(ctx):
 
 def test_tuning_roundtrip_and_validation(shapes):
     assert shapes.tuning() == BUILTIN
-    want = {"gram_chunk": 2048, "eig_lanes_limit": -1, "svst_apply": 1}
+    want = {"gram_chunk": 2048, "eig_lanes_limit": -1, "svst_apply": 1, "sense_chunks": 4}
     shapes.set_tuning(**want)
     assert shapes.tuning() == want
     for bad in [dict(gram_chunk=100), dict(gram_chunk=128), dict(gram_chunk=1 << 17),
-                dict(gram_chunk=-256), dict(eig_lanes_limit=-2), dict(svst_apply=2)]:
+                dict(gram_chunk=-256), dict(eig_lanes_limit=-2), dict(svst_apply=2),
+                dict(sense_chunks=-1), dict(sense_chunks=65)]:
         with pytest.raises(ValueError):
             shapes.set_tuning(**bad)
         assert shapes.tuning() == want
@@ -111,3 +112,29 @@ def test_autotune_pogm(shapes, tmp_path):
     A.close()
     assert out[0][1] == out[1][1]
     assert rel_err(out[0][0], out[1][0]) < 1e-5
+
+
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_gradient_frame_groups(shapes, form):
+    """The gradient and update in frame groups (sense_chunks; the batched form alternates
+    them between two streams) give the one-group iterates: the same restarts and X to
+    rounding of the restart test's FP64 sums only."""
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.sense import SenseOp
+    cfg, smaps, ksp, omega, sf = prepared("c0")
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    A = SenseOp(omega, smaps)
+    out = {}
+    for nch in (1, 3, 7):
+        shapes.set_tuning(sense_chunks=nch)
+        st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=2.0,
+                  niter=5, form=form, log_costs=True)
+        st.init(ksp)
+        st.run(5)
+        out[nch] = (st.X.to_host(), st.log()[1], st.costs())
+        st.close()
+    for nch in (3, 7):
+        assert out[nch][1] == out[1][1]
+        assert np.array_equal(out[nch][0], out[1][0])
+        np.testing.assert_allclose(out[nch][2][0], out[1][2][0], rtol=1e-12)

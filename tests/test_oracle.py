@@ -515,6 +515,40 @@ def test_lean_driver_restart_options_change_the_sequence():
 
 
 # ----------------------------------------------------------------- setup (scripts/reconstruct.jl:86-118)
+def test_normalise_smaps_sequential_float32():
+    """Random maps: every value equals the per-voxel loop of scripts/reconstruct.jl:89 in
+    Float32 (abs2 = re*re + im*im, coils summed in order, + eps(Float32), parts divided)."""
+    rng = np.random.default_rng(8)
+    raw = crandn(rng, (3, 2, 2, 5))
+    out = O.normalise_smaps(raw)
+    for idx in np.ndindex(3, 2, 2):
+        acc = np.float32(0)
+        for c in range(5):
+            z = raw[idx + (c,)]
+            acc = np.float32(acc + np.float32(np.float32(z.real * z.real) + np.float32(z.imag * z.imag)))
+        den = np.float32(np.sqrt(acc) + np.float32(O.EPS32))
+        for c in range(5):
+            z = raw[idx + (c,)]
+            assert out[idx + (c,)] == np.complex64(complex(np.float32(z.real / den), np.float32(z.imag / den)))
+
+
+def test_sense_comb_is_scaled_adjoint_odd_sizes(rng):
+    """sense_comb (fftshift o ifft o ifftshift, src/recon.jl:312-323) equals the exact
+    adjoint A'y / (sqrt(V) (sum|S|^2 + eps)) also for odd sizes, the identity the device
+    scale factor uses."""
+    for N in [(5, 3, 4), (6, 5, 3), (4, 6, 2)]:
+        V = int(np.prod(N))
+        smaps = crandn(rng, (*N, 2), np.complex128)
+        samp = rng.random(N) < 0.5
+        y = crandn(rng, (int(samp.sum()), 2), np.complex128)
+        k0 = np.zeros((V, 2), dtype=np.complex128)
+        k0[O.sample_index(samp), :] = y
+        comb = O.sense_comb(k0.reshape(*N, 2, 1, order="F"), smaps)[..., 0]
+        den = np.sum(np.abs(smaps) ** 2, axis=3) + O.EPS64
+        via = O.Asense_adjoint(y, samp, smaps) / (math.sqrt(V) * den)
+        assert rel_err(via, comb) < 1e-13
+
+
 def test_normalise_smaps_closed_form():
     """Constant maps S_c = c + 1: RSS = sqrt(sum (c+1)^2), S_c / (RSS + eps(Float32)); the
     result has unit root-sum-of-squares to FP32 rounding (scripts/reconstruct.jl:89)."""
@@ -580,7 +614,8 @@ def test_prepare_bruteforce(rng):
                 for ix in range(N[0]):
                     if omega[ix, iy, iz, t]:
                         for c in range(Nc):
-                            assert ksp[k, c, t] == np.complex64(ksp0[ix, iy, iz, c, t] / np.float32(sf))
+                            v = complex(ksp0[ix, iy, iz, c, t])  # divided in Float64, stored Float32
+                            assert ksp[k, c, t] == np.complex64(complex(v.real / sf, v.imag / sf))
                         k += 1
         assert k == K
     bad = ksp0.copy()
