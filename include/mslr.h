@@ -238,9 +238,11 @@ mslr_status_t mslr_svst_batched(mslr_ctx ctx, void *P_dev, int64_t p, int64_t Nt
  * reference; the SVST pullback an rrule of SVST, src/recon.jl:183-205, would call):
  * for Y_i = SVST(Z_i, beta_i) and cotangents Ybar_i, writes Zbar_i (device, distinct
  * from Z and Ybar) and betabar = d<Ybar, Y>/d beta (host: Np values with per_patch,
- * else their sum).  Z, Ybar, Zbar: ComplexF32 (p, Nt, Np); p >= Nt >= 2.  Uses
- * every eigenpair of Z_i^H Z_i (FP64) and safeguarded divided differences of
- * g(l) = max(1 - beta / sqrt(l), 0) (no 1/(s_i^2 - s_j^2) is formed). */
+ * else their sum).  Z, Ybar, Zbar: ComplexF32 (p, Nt, Np), any p, Nt >= 1.  Uses
+ * every eigenpair of the Gram matrix Z_i^H Z_i (FP64; for p < Nt that of Z_i Z_i^H,
+ * through Y^H = SVST(Z^H)) and safeguarded divided differences of
+ * g(l) = max(1 - beta / sqrt(l), 0) (no 1/(s_i^2 - s_j^2) is formed); a single row or
+ * column takes the group shrink's closed-form derivative. */
 mslr_status_t mslr_svst_vjp(mslr_ctx ctx, const void *Z_dev, const void *Ybar_dev, void *Zbar_dev,
                             int64_t p, int64_t Nt, int64_t Np, const double *beta_host,
                             int per_patch, double *betabar_host);
@@ -362,9 +364,9 @@ mslr_status_t mslr_pogm_costs(mslr_pogm st, int *n, double *dc_costs, double *re
  * created with checkpoint = 1: lambda_bar[s] = d<xbar, sum_s X_s>/d lambda_s through
  * the k unrolled iterations, xbar (device, ComplexF32 (Nx,Ny,Nz,Nt)) the cotangent
  * of the final image sum (for the MSE to x_ref: 2 (sum_s X_s - x_ref) / numel).
- * The restart decisions are held fixed.  Scales of disjoint patches with at least
- * Nt voxels, or the whole-image patch; others give MSLR_ERR_UNSUPPORTED.
- * Allocates its backward state for the call. */
+ * The restart decisions are held fixed.  Every patch geometry: overlapping,
+ * edge-clipped, fewer voxels than frames, 1x1x1.  Allocates its backward state for
+ * the call. */
 mslr_status_t mslr_pogm_lambda_vjp(mslr_pogm st, const void *xbar_dev, double *lambda_bar);
 /* Per-scale SVST thresholds zeta*lambda_k used in the last iteration (host, nscales). */
 mslr_status_t mslr_pogm_thresholds(mslr_pogm st, double *beta);

@@ -192,7 +192,8 @@ enum ScratchSlot : size_t {
     SLOT_VJP = 13,      // m x m intermediates of the SVST derivative
     SLOT_VJP_P = 14,    // patch cotangents of the prox derivative
     SLOT_SETUP = 15,    // run_recon setup: chunk counts, magnitudes, selection state
-    SLOT_COUNT = 16
+    SLOT_VJP_T = 16,    // conjugate-transposed patches of the SVST derivative (p < Nt)
+    SLOT_COUNT = 17
 };
 
 }  // namespace mslr

@@ -393,10 +393,8 @@ static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 
 static void prox_vjp_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *z, const c64 *xb,
                            c64 *zb, const double *beta_dev, double *bb, int *err, c64 *tmp) {
     const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
-    MSLR_REQUIRE(g.p >= g.Nt, MSLR_ERR_UNSUPPORTED,
-                 "the lambda derivative needs patches with at least Nt voxels");
     if (g.Np == 1 && g.p == V) {
-        svst_vjp_batched(ctx, z, xb, zb, g.p, g.Nt, 1, beta_dev, 0, bb, err);
+        svst_vjp_any(ctx, z, xb, zb, g.p, g.Nt, 1, beta_dev, 0, bb, err);
         return;
     }
     const bool disjoint = g.disjoint();
@@ -411,7 +409,7 @@ static void prox_vjp_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *z, co
         launch_div_coverage(ctx, g, xb, tmp);
         launch_img2patches(ctx, g, tmp, Pb, 0, g.Np);
     }
-    svst_vjp_batched(ctx, P, Pb, Qb, g.p, g.Nt, g.Np, beta_dev, 0, bb, err);
+    svst_vjp_any(ctx, P, Pb, Qb, g.p, g.Nt, g.Np, beta_dev, 0, bb, err);
     if (disjoint)
         launch_patches2img(ctx, g, Qb, zb);  // every covered voxel once: a copy
     else

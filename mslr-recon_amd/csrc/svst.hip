@@ -2117,6 +2117,104 @@ void svst_vjp_batched(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int
     }
 }
 
+// Pullback of the SVST of Np matrices with one row or one column: the group shrink
+// y = v max(1 - beta/|v|, 0) of each vector v (the p x Nt values of the matrix, stored
+// contiguously).  With <a, b> = Re sum conj(a) b, for |v| > beta
+//   vbar = (1 - beta/|v|) ybar + (beta / |v|^3) <ybar, v> v,   betabar = -<ybar, v> / |v|
+// and vbar = 0, betabar = 0 otherwise.  One wave per vector, FP64 sums; beta rounded
+// to Float32 as in the forward (src/recon.jl:195).
+__global__ __launch_bounds__(256) void k_vjp_vector(const c64 *__restrict__ Z,
+                                                    const c64 *__restrict__ Yb,
+                                                    c64 *__restrict__ Zb, int64_t len, int64_t Np,
+                                                    const double *__restrict__ beta, int per_patch,
+                                                    double *__restrict__ betabar) {
+    const int lane = threadIdx.x & 63;
+    const int64_t ip = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (ip >= Np) return;  // the whole wave
+    const c64 *z = Z + ip * len;
+    const c64 *yb = Yb + ip * len;
+    c64 *zb = Zb + ip * len;
+    double s2 = 0.0, d = 0.0;
+    for (int64_t i = lane; i < len; i += 64) {
+        const c64 a = z[i], b = yb[i];
+        s2 += (double)a.re * a.re + (double)a.im * a.im;
+        d += (double)b.re * a.re + (double)b.im * a.im;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s2 += __shfl_xor(s2, o, 64);
+        d += __shfl_xor(d, o, 64);
+    }
+    const double be = (double)(float)beta[per_patch ? ip : 0];
+    const double s = sqrt(s2);
+    if (s > be) {
+        const double g = 1.0 - be / s, c = be * d / (s * s2);
+        for (int64_t i = lane; i < len; i += 64) {
+            const c64 a = z[i], b = yb[i];
+            zb[i] = c64{(float)(g * b.re + c * a.re), (float)(g * b.im + c * a.im)};
+        }
+        if (lane == 0) betabar[ip] = -d / s;
+    } else {
+        for (int64_t i = lane; i < len; i += 64) zb[i] = c64{0.0f, 0.0f};
+        if (lane == 0) betabar[ip] = 0.0;
+    }
+}
+
+// dst[:, :, i] = src[:, :, i]^H for Np r x c matrices (32 x 32 tiles through LDS).
+__global__ __launch_bounds__(256) void k_conj_transpose(const c64 *__restrict__ src,
+                                                        c64 *__restrict__ dst, int64_t r, int64_t c) {
+    __shared__ c64 tile[32][33];
+    const int64_t ip = blockIdx.z;
+    const int64_t i0 = (int64_t)blockIdx.x * 32, j0 = (int64_t)blockIdx.y * 32;
+    const c64 *S = src + ip * r * c;
+    c64 *D = dst + ip * r * c;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int64_t i = i0 + tx, j = j0 + k;
+        if (i < r && j < c) tile[k][tx] = S[i + r * j];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int64_t j = j0 + tx, i = i0 + k;
+        if (i < r && j < c) {
+            const c64 v = tile[tx][k];
+            D[j + c * i] = c64{v.re, -v.im};
+        }
+    }
+}
+
+static void conj_transpose(mslr_ctx_s *ctx, const c64 *src, c64 *dst, int64_t r, int64_t c,
+                           int64_t Np) {
+    MSLR_PROF(ctx, "vjp_transpose");
+    dim3 g((unsigned)cdiv(r, 32), (unsigned)cdiv(c, 32), (unsigned)Np);
+    k_conj_transpose<<<g, 256, 0, ctx->stream>>>(src, dst, r, c);
+    HIP_LAUNCH_CHECK();
+}
+
+void svst_vjp_any(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int64_t p, int64_t Nt,
+                  int64_t Np, const double *beta_dev, int per_patch, double *betabar_dev,
+                  int *err_dev) {
+    if (Np == 0 || p == 0 || Nt == 0) return;
+    if (std::min(p, Nt) == 1) {
+        MSLR_PROF(ctx, "vjp_vector");
+        k_vjp_vector<<<(unsigned)cdiv(Np, 4), 256, 0, ctx->stream>>>(Z, Yb, Zb, p * Nt, Np, beta_dev,
+                                                                      per_patch, betabar_dev);
+        HIP_LAUNCH_CHECK();
+        return;
+    }
+    if (p >= Nt) {
+        svst_vjp_batched(ctx, Z, Yb, Zb, p, Nt, Np, beta_dev, per_patch, betabar_dev, err_dev);
+        return;
+    }
+    // fewer rows than columns: Y^H = SVST(Z^H), the pullback of the transposed problem
+    const size_t len = (size_t)(p * Nt * Np);
+    c64 *T = (c64 *)ctx->scratch_get(SLOT_VJP_T, 3 * len * sizeof(c64));
+    c64 *ZT = T, *YbT = T + len, *ZbT = T + 2 * len;
+    conj_transpose(ctx, Z, ZT, p, Nt, Np);
+    conj_transpose(ctx, Yb, YbT, p, Nt, Np);
+    svst_vjp_batched(ctx, ZT, YbT, ZbT, Nt, p, Np, beta_dev, per_patch, betabar_dev, err_dev);
+    conj_transpose(ctx, ZbT, Zb, Nt, p, Np);
+}
+
 void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,
                      double *out_dev) {
     if (Np == 0) {
@@ -2197,8 +2295,8 @@ mslr_status_t mslr_svst_vjp(mslr_ctx ctx, const void *Z_dev, const void *Ybar_de
         HIP_CHECK(hipMemcpyAsync(bd, beta_host, (size_t)nbeta * sizeof(double),
                                  hipMemcpyHostToDevice, c->stream));
         HIP_CHECK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
-        svst_vjp_batched(c, (const c64 *)Z_dev, (const c64 *)Ybar_dev, (c64 *)Zbar_dev, p, Nt, Np,
-                         bd, per_patch, bb, err);
+        svst_vjp_any(c, (const c64 *)Z_dev, (const c64 *)Ybar_dev, (c64 *)Zbar_dev, p, Nt, Np,
+                     bd, per_patch, bb, err);
         std::vector<double> h((size_t)Np);
         int herr = 0;
         HIP_CHECK(hipMemcpyAsync(h.data(), bb, (size_t)Np * sizeof(double), hipMemcpyDeviceToHost,

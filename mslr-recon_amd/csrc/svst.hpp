@@ -21,6 +21,12 @@ void svst_vjp_batched(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int
                       int64_t Np, const double *beta_dev, int per_patch, double *betabar_dev,
                       int *err_dev);
 
+// The same for any shape: min(p, Nt) = 1 by the group shrink's closed form, p < Nt
+// through the conjugate transposes (Y^H = SVST(Z^H)), p >= Nt >= 2 as above.
+void svst_vjp_any(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int64_t p, int64_t Nt,
+                  int64_t Np, const double *beta_dev, int per_patch, double *betabar_dev,
+                  int *err_dev);
+
 // Sum of singular values of each matrix, accumulated into *out_dev (device double).
 void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,
                      double *out_dev);

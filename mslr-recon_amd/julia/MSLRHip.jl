@@ -301,7 +301,7 @@ end
     SVST_vjp_hip(Z, Ybar, β) -> (Zbar, βbar)
 
 Pullback of `Y = SVST(Z, β)` (src/recon.jl:183-205) for device matrices Z, Ybar of
-size (p, Nt) or (p, Nt, Np) with p ≥ Nt: Zbar and βbar = ∂⟨Ybar, Y⟩/∂β (one per
+size (p, Nt) or (p, Nt, Np) (any p and Nt): Zbar and βbar = ∂⟨Ybar, Y⟩/∂β (one per
 patch when β is a vector).  The rrule of SVST calls it.
 """
 function SVST_vjp_hip(Z::HipArray{ComplexF32}, Ybar::HipArray{ComplexF32}, β)

@@ -102,7 +102,7 @@ def g_prox(X, c, lambdas, patch_sizes, strides, ctx: Context | None = None) -> D
 
 def svst_vjp(Z, Ybar, beta, ctx: Context | None = None):
     """Pullback of svst_batched: (Zbar DeviceArray, betabar) for Y = SVST(Z, beta) and the
-    cotangent Ybar (same shape as Z: (p, Nt) or (p, Nt, Np), p >= Nt).  betabar has one
+    cotangent Ybar (same shape as Z: (p, Nt) or (p, Nt, Np), any p and Nt).  betabar has one
     value per patch when beta does, else it is the scalar d<Ybar, Y>/d beta."""
     ctx = ctx or default_context()
     z = as_device(Z, np.complex64, ctx)

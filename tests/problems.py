@@ -12,6 +12,9 @@ SMALL = {
     # overlapping 4^3 patches (stride 2) and the 1x1x1 voxel scale
     "small_overlap": dict(patch_sizes=([12, 12, 10], [4, 4, 4], [1, 1, 1]),
                           strides=([12, 12, 10], [2, 2, 2], [1, 1, 1])),
+    # patches with fewer voxels than frames (2^3 = 8 < Nt = 10) and the 1x1x1 scale
+    "small_vjp": dict(patch_sizes=([12, 12, 10], [2, 2, 2], [1, 1, 1]),
+                      strides=([12, 12, 10], [2, 2, 2], [1, 1, 1])),
     # odd image sizes (15 = 3*5, 9 = 3^2): the shifts differ from the even-size sign flips
     "small_odd": dict(N=(15, 9, 10), Nt=6, patch_sizes=([15, 9, 10], [5, 3, 5]),
                       strides=([15, 9, 10], [5, 3, 5])),

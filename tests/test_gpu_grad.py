@@ -67,11 +67,38 @@ def test_svst_vjp_zero_patch_and_all_kept(ctx, rng):
     assert abs(_vdot(Zb.to_host(), Zd) - _vdot(Yb, Yd)) <= 1e-4 * np.linalg.norm(Yb) * np.linalg.norm(Yd)
 
 
-def test_svst_vjp_rejects_wide_patches(ctx, rng):
-    from mslr_hip import _lib, recon
-    Z = lowrank(rng, 20, 60, 3)
-    with pytest.raises(_lib.MSLRUnsupported):
-        recon.svst_vjp(Z, Z, 0.5)
+@pytest.mark.parametrize("p,n,r,Np", [(20, 60, 3, 2), (216, 387, 8, 2), (2, 10, 1, 3),
+                                      (1, 300, 1, 6), (40, 1, 1, 4)])
+def test_svst_vjp_wide_and_vector_shapes(ctx, rng, p, n, r, Np):
+    """Patches with fewer voxels than frames (through Y^H = SVST(Z^H)) and single rows or
+    columns (the group shrink's closed form): the adjoint identity against the oracle's FP64
+    forward mode, with thresholds that keep some singular values (and, for vectors, drop
+    some vectors)."""
+    from mslr_hip import recon
+    Z = np.asfortranarray(np.stack([lowrank(rng, p, n, min(r, p, n), noise=0.1) for _ in range(Np)], axis=2))
+    s = [np.linalg.svd(Z[:, :, i].astype(np.complex128), compute_uv=False) for i in range(Np)]
+    if min(p, n) == 1:
+        betas = np.array([si[0] * (0.5 if i % 2 == 0 else 1.5) for i, si in enumerate(s)])
+    else:
+        betas = np.array([0.5 * (si[r - 1] + si[r]) for si in s])
+    Yb = crandn(rng, Z.shape, np.complex64)
+    Zb, bb = recon.svst_vjp(Z, Yb, betas)
+    bb = np.atleast_1d(bb)
+    Zb = Zb.to_host()
+    assert np.all(np.isfinite(Zb))
+    for trial in range(3):
+        Zd = crandn(rng, Z.shape, np.complex128)
+        bd = rng.standard_normal(Np)
+        lhs = scale = 0.0
+        for i in range(Np):
+            _, Yd = O.svst_jvp(Z[:, :, i], float(np.float32(betas[i])), Zd[:, :, i], bd[i])
+            lhs += _vdot(Yb[:, :, i], Yd)
+            scale += np.linalg.norm(Yb[:, :, i]) * np.linalg.norm(Yd)
+        rhs = _vdot(Zb, Zd) + float(np.dot(bb, bd))
+        assert abs(lhs - rhs) <= 1e-4 * scale, (trial, lhs, rhs)
+    if min(p, n) == 1:  # the dropped vectors have no derivative
+        for i in range(1, Np, 2):
+            assert bb[i] == 0.0 and np.all(Zb[:, :, i] == 0)
 
 
 def _device_loss(name, niter, lambdas):
@@ -120,6 +147,24 @@ def test_lambda_grad_matches_fp64_forward_mode(niter):
     assert loss == pytest.approx(lo, rel=1e-4)
     for k in range(len(lams)):
         assert grads[k] == pytest.approx(go[k], rel=1e-3), (k, grads, go)
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_lambda_grad_voxel_and_small_patch_scales(form):
+    """Scales [[12,12,10],[2,2,2],[1,1,1]] over Nt = 10 frames: 2^3 patches have fewer
+    voxels than frames and the last scale is 1x1x1 (the production scale sets carry both).
+    The device gradient against the oracle's FP64 forward mode (3 iterations), 1e-3."""
+    niter = 3
+    cfg, smaps, ksp, omega, sf = prepared("small_vjp")
+    loss, grads, rs, lams = _device_grad("small_vjp", niter, form=form)
+    x_ref = truth("small_vjp").astype(np.complex128) / sf
+    lo, go, ro, _ = O.pogm_lambda_jvp(ksp, omega, smaps, x_ref, N=tuple(cfg["N"]), Nt=cfg["Nt"],
+                                      PATCH_SIZES=cfg["patch_sizes"], STRIDES=cfg["strides"],
+                                      NITERS=niter)
+    assert rs == ro
+    assert loss == pytest.approx(lo, rel=1e-4)
+    for k in range(len(lams)):
+        assert grads[k] == pytest.approx(go[k], rel=1e-3, abs=1e-4 * max(abs(v) for v in go)), (k, grads, go)
 
 
 def test_lambda_grad_graph_form_matches_plain():

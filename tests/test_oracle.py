@@ -385,6 +385,20 @@ def test_svst_jvp_central_differences(rng, p, n):
         assert rel_err(Yd, fd) < 1e-7, bd
 
 
+@pytest.mark.parametrize("shape", [(1, 30), (30, 1)])
+def test_svst_jvp_vector_central_differences(rng, shape):
+    """A single row or column (the 1x1x1 patches): the tangent of the group shrink
+    v max(1 - beta/|v|, 0) against FP64 central differences, kept and dropped."""
+    v = crandn(rng, shape, np.complex128)
+    nv = np.linalg.norm(v)
+    vd = crandn(rng, shape, np.complex128)
+    h = 1e-6
+    for beta, bd in [(0.4 * nv, 0.0), (0.4 * nv, 1.0), (1.5 * nv, 1.0)]:
+        _, Yd = O.svst_jvp(v, beta, vd, bd)
+        fd = (O.SVST(v + h * vd, beta + h * bd) - O.SVST(v - h * vd, beta - h * bd)) / (2 * h)
+        assert np.linalg.norm(Yd - fd) <= 1e-7 * max(np.linalg.norm(fd), 1.0), (beta, bd)
+
+
 def test_svst_jvp_rank1_closed_forms(rng):
     u = crandn(rng, (30, 1), np.complex128)
     v = crandn(rng, (8, 1), np.complex128)
