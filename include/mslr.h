@@ -94,8 +94,8 @@ mslr_status_t mslr_ctx_info(mslr_ctx ctx, char *name, size_t name_len,
 #define MSLR_PRODUCTS_SPLIT_BF16 1
 mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
 
-/* Launch shapes of the batched SVST on this context.  No counterpart in the
- * reference (its SVD is a library call per patch); set by the autotuner
+/* Launch shapes on this context.  No counterpart in the reference (its SVD and
+ * FFTs are library calls); set by the autotuner
  * (mslr_hip.autotune) or by hand.  0 in a field keeps the built-in choice.
  * Read when work is enqueued: a POGM state's captured graph keeps the shapes it
  * was captured with, so create the state after setting them.
@@ -111,12 +111,25 @@ mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
  *                    adjoint; the batched forms alternate groups between two streams
  *                    so the groups' memory-bound and FFT-bound kernels overlap.
  *                    Changes the summation order of the restart test's FP64 sums only.
+ *   yz_threads_fwd / yz_threads_adj: threads of the compiled y-z FFT passes for
+ *                    planes above 2048 points, 512 or 1024 (built-in 1024 forward,
+ *                    512 adjoint).  Results are bit-identical.
+ *   upd_blocks:      workgroups of the fused POGM update per frame group, in
+ *                    [256, 65536] (built-in 2048; read at mslr_pogm_create).
+ *   sb2st_schedule:  band -> tridiagonal kernel for Gram matrices above 300 (band in
+ *                    global memory): 0 = 12 waves starting sweeps 3 time steps apart
+ *                    (built-in), 1 = 16 waves 2 apart, 2 = 6 waves 5 apart, 3 = 4 waves
+ *                    8 apart.  Results are bit-identical.
  * Invalid values -> MSLR_ERR_ARG. */
 typedef struct mslr_tuning {
     int gram_chunk;
     int eig_lanes_limit;
     int svst_apply;
     int sense_chunks;
+    int yz_threads_fwd;
+    int yz_threads_adj;
+    int upd_blocks;
+    int sb2st_schedule;
     int reserved[4];
 } mslr_tuning;
 mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t);

@@ -454,6 +454,7 @@ struct mslr_pogm_s {
     // the gradient's frame groups, each followed by its part of the update; the
     // batched forms alternate them between the context stream and gpipe
     std::vector<std::pair<int64_t, int64_t>> gchunks;
+    int upd_blocks = 0;  // workgroups of the fused update per frame group
     hipStream_t gpipe = nullptr;
     hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;
     // batched forms: one stream per scale for the prox (fork / join with events)
@@ -524,9 +525,9 @@ static void pogm_iteration(mslr_pogm_s *st) {
                          [&](size_t i, int64_t ta, int64_t nf) {
                              MSLR_PROF(ctx, "pogm_update");
                              const int64_t o = ta * st->V;
-                             k_pogm_update<<<UPD_BLOCKS, 256, 0, ctx->stream>>>(
+                             k_pogm_update<<<st->upd_blocks, 256, 0, ctx->stream>>>(
                                  st->X + o, st->U + o, st->g + o, nf * st->V, (int)st->S, n,
-                                 st->dev, st->partials + 3 * (size_t)UPD_BLOCKS * i);
+                                 st->dev, st->partials + 3 * (size_t)st->upd_blocks * i);
                              HIP_LAUNCH_CHECK();
                          });
     if (st->log_costs) {
@@ -544,7 +545,7 @@ static void pogm_iteration(mslr_pogm_s *st) {
     {
         MSLR_PROF(ctx, "pogm_post");
         k_pogm_post<<<1, 256, 0, ctx->stream>>>(st->dev, st->partials,
-                                                UPD_BLOCKS * (int)st->gchunks.size(), st->log_dev);
+                                                st->upd_blocks * (int)st->gchunks.size(), st->log_dev);
         HIP_LAUNCH_CHECK();
     }
     int *err = (int *)((char *)st->dev + offsetof(PogmDev, err));
@@ -673,7 +674,9 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
                                                                     : SENSE_CHUNKS_DEFAULT;
                 st->gchunks = sense_grad_chunks(sense, nch);
             }
-            st->partials = (double *)st->alloc(sizeof(double) * 3 * UPD_BLOCKS *
+            st->upd_blocks = sense->ctx->tuning.upd_blocks > 0 ? sense->ctx->tuning.upd_blocks
+                                                               : UPD_BLOCKS;
+            st->partials = (double *)st->alloc(sizeof(double) * 3 * (size_t)st->upd_blocks *
                                                std::max<size_t>(1, st->gchunks.size()));
             st->log_dev = (int *)st->alloc(sizeof(int) * std::max(1, d.niter));
             HIP_CHECK(hipMemcpy(st->lam_dev, st->lambdas.data(), sizeof(double) * st->S,

@@ -613,7 +613,7 @@ __global__ __launch_bounds__(NT) void k_sb2st(const c128 *__restrict__ Gall, int
     c128 *V2 = V2all + (int64_t)ip * m * ks * B;
     c128 *tau2 = tau2all + (int64_t)ip * m * ks;
     c128 tau = {0.0, 0.0};
-    int s = wave;  // this wave's current sweep: s, s + S2W, ...
+    int s = wave;  // this wave's current sweep: s, s + NW, ...
     const int nsweep = m - 1;
     const int tend = STG * (nsweep - 1) + 1;
     for (int t = 0; t < tend; ++t) {
@@ -825,7 +825,7 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
     const bool gband = sbr_global_band(m);
     // a wave must finish its sweep before its next one (one sweep per wave later)
     // starts
-    MSLR_REQUIRE(sbr_steps(m) <= (gband ? STAGGER_G * (S2TG / 64) : STAGGER * S2W), MSLR_ERR_UNSUPPORTED,
+    MSLR_REQUIRE(gband || sbr_steps(m) <= STAGGER * S2W, MSLR_ERR_UNSUPPORTED,
                  "sweep longer than the schedule");
     MSLR_REQUIRE(!gband || w.Band, MSLR_ERR_DEVICE, "no band workspace");
     const int npanel = sbr_panels(m), ks = sbr_steps(m);
@@ -837,12 +837,22 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
         HIP_LAUNCH_CHECK();
     }
     if (gband) {
-        const size_t lds = (size_t)(S2TG / 64) * 3 * B * sizeof(c128);
-        raise_lds((const void *)k_sb2st<S2TG, true, STAGGER_G>, lds);
-        MSLR_PROF(ctx, "sb2st");
-        k_sb2st<S2TG, true, STAGGER_G><<<(unsigned)nb, S2TG, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2,
-                                                                       w.Band);
-        HIP_LAUNCH_CHECK();
+        // the schedule: built-in 12 waves 3 steps apart, or one of the tuning's
+        // compiled alternatives; each needs a sweep to end before the wave's next one
+        auto run = [&](auto kern, int nt, int stagger) {
+            MSLR_REQUIRE(ks <= stagger * (nt / 64), MSLR_ERR_UNSUPPORTED, "sweep longer than the schedule");
+            const size_t lds = (size_t)(nt / 64) * 3 * B * sizeof(c128);
+            raise_lds((const void *)kern, lds);
+            MSLR_PROF(ctx, "sb2st");
+            kern<<<(unsigned)nb, nt, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2, w.Band);
+            HIP_LAUNCH_CHECK();
+        };
+        switch (ctx->tuning.sb2st_schedule) {
+            case 1: run(k_sb2st<1024, true, 2>, 1024, 2); break;
+            case 2: run(k_sb2st<384, true, 5>, 384, 5); break;
+            case 3: run(k_sb2st<256, true, 8>, 256, 8); break;
+            default: run(k_sb2st<S2TG, true, STAGGER_G>, S2TG, STAGGER_G); break;
+        }
     } else {
         const size_t lds = ((size_t)2 * B * m + (size_t)S2W * 3 * B) * sizeof(c128);
         raise_lds((const void *)k_sb2st<S2T, false, STAGGER>, lds);

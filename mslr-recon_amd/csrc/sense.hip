@@ -48,11 +48,6 @@ constexpr int MAX_NYZ = MAXE_YZ * NTYZ_MAX;
 #ifndef MSLR_NTYZ_ADJ
 #define MSLR_NTYZ_ADJ 512
 #endif
-#ifdef MSLR_YZ_WPE
-#define MSLR_YZ_ATTR __attribute__((amdgpu_waves_per_eu(MSLR_YZ_WPE)))
-#else
-#define MSLR_YZ_ATTR
-#endif
 
 // ------------------------------------------------------------------ FFT policies
 // A policy transforms `nlines` lines in LDS: runtime lengths (any 2/3/5 length)
@@ -178,7 +173,7 @@ struct LineTabs {
 // would halve the workgroups resident per CU.  Compile-time FFT lengths loop
 // over all coils, runtime lengths take one coil per workgroup.
 template <class T, class FY, class FZ>
-__global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) MSLR_YZ_ATTR void k_fwd_yz(
+__global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) void k_fwd_yz(
     const C2<T> *__restrict__ Pm, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny,
     int Nz, int cpw, FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz,
     int64_t ta, int nf, C2<T> *__restrict__ Tb) {
@@ -335,7 +330,7 @@ __global__ __launch_bounds__(NTX) void k_adj_x(
 // registers (addresses kept out of loop-invariant code motion so that two
 // workgroups stay resident per CU), runtime lengths in the output plane itself.
 template <class T, class FY, class FZ>
-__global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) MSLR_YZ_ATTR void k_adj_yz(
+__global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) void k_adj_yz(
     const C2<T> *__restrict__ Tb, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny, int Nz,
     FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz, int64_t ta,
     int nf, T scale, C2<T> *__restrict__ Pm) {
@@ -559,17 +554,28 @@ static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, 
 }
 
 // ---- dispatch: compile-time FFT lengths for the production / test sizes in
-// FP32 with the default exponent sign, runtime lengths otherwise.
+// FP32 with the default exponent sign, runtime lengths otherwise.  Planes above
+// 2048 points have two compiled thread counts per direction (512 and 1024); the
+// built-in choice is the measured one, mslr_ctx_set_tuning selects the other.
 constexpr int fix_nt_yz(int ny, int nz, int sgn) {
     return ny * nz > 2048 ? (sgn < 0 ? MSLR_NTYZ_FWD : MSLR_NTYZ_ADJ) : 256;
 }
 
 template <int NX, int SGN>
 using FixX = FftFix<float, NX, SGN, (MAXE_X * NTX) / NX, NTX>;
-template <int NY, int NZ, int SGN>
-using FixY = FftFix<float, NY, SGN, NZ, fix_nt_yz(NY, NZ, SGN)>;
-template <int NY, int NZ, int SGN>
-using FixZ = FftFix<float, NZ, SGN, NY, fix_nt_yz(NY, NZ, SGN)>;
+template <int NY, int NZ, int SGN, int NT = fix_nt_yz(NY, NZ, SGN)>
+using FixY = FftFix<float, NY, SGN, NZ, NT>;
+template <int NY, int NZ, int SGN, int NT = fix_nt_yz(NY, NZ, SGN)>
+using FixZ = FftFix<float, NZ, SGN, NY, NT>;
+
+// Thread count of the compiled yz pass for this plane and direction: the tuning's
+// (512 or 1024) for planes above 2048 points, else the built-in one.
+static int yz_threads(mslr_sense_s *P, int sgn) {
+    const int built = fix_nt_yz(P->N[1], P->N[2], sgn);
+    if (P->N[1] * P->N[2] <= 2048) return built;
+    const int t = sgn < 0 ? P->ctx->tuning.yz_threads_fwd : P->ctx->tuning.yz_threads_adj;
+    return t ? t : built;
+}
 
 #define MSLR_X_SIZES(M) M(90) M(64) M(32) M(12)
 #define MSLR_YZ_SIZES(M) M(90, 60) M(64, 40) M(32, 24) M(12, 10)
@@ -584,10 +590,16 @@ static void fwd_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Pm, C2<
     if constexpr (std::is_same<T, float>::value) {
         if (fixed_ok<T>(P)) {
             switch (P->N[1] * 10000 + P->N[2]) {
-#define CASE_YZ(ny, nz)                                                                  \
-    case ny * 10000 + nz:                                                                \
-        launch_fwd_yz<float>(P, ta, nf, Pm, Tb, FixY<ny, nz, -1>{P->tw32[1]},             \
-                             FixZ<ny, nz, -1>{P->tw32[2]}, fix_nt_yz(ny, nz, -1));            \
+#define CASE_YZ(ny, nz)                                                                     \
+    case ny * 10000 + nz:                                                                   \
+        if (ny * nz > 2048 && yz_threads(P, -1) != fix_nt_yz(ny, nz, -1)) {                 \
+            constexpr int alt = fix_nt_yz(ny, nz, -1) == 1024 ? 512 : 1024;                  \
+            launch_fwd_yz<float>(P, ta, nf, Pm, Tb, FixY<ny, nz, -1, alt>{P->tw32[1]},         \
+                                 FixZ<ny, nz, -1, alt>{P->tw32[2]}, alt);                      \
+        } else {                                                                            \
+            launch_fwd_yz<float>(P, ta, nf, Pm, Tb, FixY<ny, nz, -1>{P->tw32[1]},              \
+                                 FixZ<ny, nz, -1>{P->tw32[2]}, fix_nt_yz(ny, nz, -1));         \
+        }                                                                                   \
         return;
                 MSLR_YZ_SIZES(CASE_YZ)
 #undef CASE_YZ
@@ -642,10 +654,16 @@ static void adj_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, C2<
     if constexpr (std::is_same<T, float>::value) {
         if (fixed_ok<T>(P)) {
             switch (P->N[1] * 10000 + P->N[2]) {
-#define CASE_YZ(ny, nz)                                                                  \
-    case ny * 10000 + nz:                                                                \
-        launch_adj_yz<float>(P, ta, nf, Tb, Pm, FixY<ny, nz, 1>{P->tw32[1]},              \
-                             FixZ<ny, nz, 1>{P->tw32[2]}, fix_nt_yz(ny, nz, 1));             \
+#define CASE_YZ(ny, nz)                                                                     \
+    case ny * 10000 + nz:                                                                   \
+        if (ny * nz > 2048 && yz_threads(P, 1) != fix_nt_yz(ny, nz, 1)) {                   \
+            constexpr int alt = fix_nt_yz(ny, nz, 1) == 1024 ? 512 : 1024;                   \
+            launch_adj_yz<float>(P, ta, nf, Tb, Pm, FixY<ny, nz, 1, alt>{P->tw32[1]},          \
+                                 FixZ<ny, nz, 1, alt>{P->tw32[2]}, alt);                       \
+        } else {                                                                            \
+            launch_adj_yz<float>(P, ta, nf, Tb, Pm, FixY<ny, nz, 1>{P->tw32[1]},               \
+                                 FixZ<ny, nz, 1>{P->tw32[2]}, fix_nt_yz(ny, nz, 1));          \
+        }                                                                                   \
         return;
                 MSLR_YZ_SIZES(CASE_YZ)
 #undef CASE_YZ

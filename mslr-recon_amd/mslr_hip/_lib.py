@@ -87,6 +87,10 @@ class mslr_tuning(C.Structure):
         ("eig_lanes_limit", C.c_int),
         ("svst_apply", C.c_int),
         ("sense_chunks", C.c_int),
+        ("yz_threads_fwd", C.c_int),
+        ("yz_threads_adj", C.c_int),
+        ("upd_blocks", C.c_int),
+        ("sb2st_schedule", C.c_int),
         ("reserved", C.c_int * 4),
     ]
 

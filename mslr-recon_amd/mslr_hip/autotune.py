@@ -6,7 +6,9 @@ to sum), the eigenvector lanes (16 per eigenvalue hide the bisection latency of
 small batches, 4 suit batches that fill the device), the apply form (low-rank
 X U_r products or the dense W) and the frame groups of the gradient (more groups
 overlap more of the memory-bound kernels with the FFT passes on two streams, at
-smaller launches).  Which is fastest depends on the patch shapes,
+smaller launches), the thread counts of the compiled y-z FFT passes, the update's
+grid and the wave schedule of the global-band tridiagonalisation (Gram matrices
+above 300).  Which is fastest depends on the patch shapes,
 the patch count and what the other scales run beside them, so the tuner times
 whole POGM iterations of the workload itself: coordinate descent from the
 built-in choice, one shape at a time, each candidate timed over `iters`
@@ -30,8 +32,27 @@ SPACE: dict[str, tuple[int, ...]] = {
     "eig_lanes_limit": (0, -1, 1 << 20),
     "svst_apply": (0, 1),
     "sense_chunks": (0, 2, 4, 8),
+    "yz_threads_fwd": (0, 512),
+    "yz_threads_adj": (0, 1024),
+    "upd_blocks": (0, 1024, 8192),
+    "sb2st_schedule": (0, 1, 2, 3),
 }
 BUILTIN = {k: 0 for k in SPACE}
+# plane sizes (Ny, Nz) with compiled y-z FFT passes of two thread counts
+YZ_COMPILED = {(90, 60), (64, 40)}
+SBR_MAXM = 300  # Gram matrices above this keep their band in global memory
+
+
+def relevant(name: str, N, Nt: int, patch_sizes) -> bool:
+    """Whether a shape can change anything for this workload (the others are skipped):
+    the yz thread counts only for the compiled planes above 2048 points, the global-band
+    schedule only when some patch has a Gram matrix above 300."""
+    if name in ("yz_threads_fwd", "yz_threads_adj"):
+        return (int(N[1]), int(N[2])) in YZ_COMPILED
+    if name == "sb2st_schedule":
+        return any(min(int(np.prod([min(int(a), int(n)) for a, n in zip(ps, N)])), int(Nt)) > SBR_MAXM
+                   for ps in patch_sizes)
+    return True
 
 
 def workload_key(A, patch_sizes, strides, form: str) -> str:
@@ -121,6 +142,8 @@ def autotune_pogm(A, y, *, patch_sizes, strides, lambdas, L: float, form: str = 
     if log:
         log(f"tune {best}: {best_ms:.3f} ms/iter")
     for name, value in descent_order():
+        if not relevant(name, A.N, A.Nt, patch_sizes):
+            continue
         cand = dict(best, **{name: value})
         ms, X = trial(cand)
         err = _rel(X, X_ref)

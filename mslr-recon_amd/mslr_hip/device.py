@@ -44,20 +44,27 @@ class Context:
         code = {"fp32": _lib.PRODUCTS_FP32, "split_bf16": _lib.PRODUCTS_SPLIT_BF16}[products]
         _lib.check(self._lib.mslr_ctx_set_products(self.handle, code))
 
-    def set_tuning(self, gram_chunk: int = 0, eig_lanes_limit: int = 0, svst_apply: int = 0,
-                   sense_chunks: int = 0) -> None:
-        """Launch shapes (0 = built-in): contraction rows per Gram workgroup (a
-        multiple of 256), the patches x m limit for 16 lanes per eigenvector (-1:
-        always 4), svst_apply = 1 for the dense apply, and the frame groups of the
-        POGM gradient (sense_chunks, at most 64)."""
-        t = _lib.mslr_tuning(int(gram_chunk), int(eig_lanes_limit), int(svst_apply), int(sense_chunks))
+    TUNING_FIELDS = ("gram_chunk", "eig_lanes_limit", "svst_apply", "sense_chunks", "yz_threads_fwd",
+                     "yz_threads_adj", "upd_blocks", "sb2st_schedule")
+
+    def set_tuning(self, **kw) -> None:
+        """Launch shapes (mslr_ctx_set_tuning; an omitted or 0 field = built-in):
+        gram_chunk (contraction rows per Gram workgroup, a multiple of 256),
+        eig_lanes_limit (patches x m limit for 16 lanes per eigenvector, -1: always 4),
+        svst_apply (1: dense apply), sense_chunks (frame groups of the gradient, at most
+        64), yz_threads_fwd / yz_threads_adj (512 or 1024 threads of the compiled y-z FFT
+        passes), upd_blocks (update workgroups, [256, 65536]), sb2st_schedule (0-3: the
+        global-band tridiagonalisation's wave schedule)."""
+        bad = set(kw) - set(self.TUNING_FIELDS)
+        if bad:
+            raise TypeError(f"unknown tuning fields {sorted(bad)}")
+        t = _lib.mslr_tuning(*(int(kw.get(f, 0)) for f in self.TUNING_FIELDS))
         _lib.check(self._lib.mslr_ctx_set_tuning(self.handle, C.byref(t)))
 
     def tuning(self) -> dict:
         t = _lib.mslr_tuning()
         _lib.check(self._lib.mslr_ctx_get_tuning(self.handle, C.byref(t)))
-        return {"gram_chunk": t.gram_chunk, "eig_lanes_limit": t.eig_lanes_limit,
-                "svst_apply": t.svst_apply, "sense_chunks": t.sense_chunks}
+        return {f: int(getattr(t, f)) for f in self.TUNING_FIELDS}
 
     def prof_enable(self, on: bool = True) -> None:
         """Time every kernel launch with HIP events (clears earlier records)."""

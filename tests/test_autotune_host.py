@@ -42,3 +42,13 @@ def test_cache_roundtrip(tmp_path):
     T.store_cache(None, "c", e2)
     c = T.load_cache(path)
     assert c == {"a": e1, "b": e2}
+
+
+def test_relevance_skips_shapes_that_cannot_matter():
+    ps1 = [[90, 90, 60], [10, 10, 10], [1, 1, 1]]
+    ps2 = [[90, 90, 60], [30, 30, 20], [10, 10, 10], [6, 6, 6]]
+    assert T.relevant("yz_threads_fwd", (90, 90, 60), 300, ps1)
+    assert not T.relevant("yz_threads_adj", (32, 32, 24), 40, ps1)
+    assert not T.relevant("sb2st_schedule", (90, 90, 60), 300, ps1)   # m <= 300 everywhere
+    assert T.relevant("sb2st_schedule", (90, 90, 60), 387, ps2)       # m = 387
+    assert T.relevant("gram_chunk", (12, 12, 10), 10, ps1)

@@ -13,7 +13,8 @@ from test_gpu_svst import APPLY_RANKS, _beta_between, lowrank, mixed_rank_batch
 
 pytestmark = pytest.mark.gpu
 
-BUILTIN = {"gram_chunk": 0, "eig_lanes_limit": 0, "svst_apply": 0, "sense_chunks": 0}
+BUILTIN = {"gram_chunk": 0, "eig_lanes_limit": 0, "svst_apply": 0, "sense_chunks": 0,
+           "yz_threads_fwd": 0, "yz_threads_adj": 0, "upd_blocks": 0, "sb2st_schedule": 0}
 
 
 @pytest.fixture
@@ -26,12 +27,14 @@ def shapes(ctx):
 
 def test_tuning_roundtrip_and_validation(shapes):
     assert shapes.tuning() == BUILTIN
-    want = {"gram_chunk": 2048, "eig_lanes_limit": -1, "svst_apply": 1, "sense_chunks": 4}
+    want = {"gram_chunk": 2048, "eig_lanes_limit": -1, "svst_apply": 1, "sense_chunks": 4,
+            "yz_threads_fwd": 512, "yz_threads_adj": 1024, "upd_blocks": 4096, "sb2st_schedule": 2}
     shapes.set_tuning(**want)
     assert shapes.tuning() == want
     for bad in [dict(gram_chunk=100), dict(gram_chunk=128), dict(gram_chunk=1 << 17),
                 dict(gram_chunk=-256), dict(eig_lanes_limit=-2), dict(svst_apply=2),
-                dict(sense_chunks=-1), dict(sense_chunks=65)]:
+                dict(sense_chunks=-1), dict(sense_chunks=65), dict(yz_threads_fwd=256),
+                dict(yz_threads_adj=768), dict(upd_blocks=100), dict(sb2st_schedule=4)]:
         with pytest.raises(ValueError):
             shapes.set_tuning(**bad)
         assert shapes.tuning() == want
@@ -77,7 +80,7 @@ def test_eig_lanes_and_apply(shapes, rng, tuning):
 
 
 def test_autotune_pogm(shapes, tmp_path):
-    from mslr_hip.autotune import autotune_pogm, descent_order
+    from mslr_hip.autotune import autotune_pogm, descent_order, relevant
     from mslr_hip.mirt_mod import POGM
     from mslr_hip.sense import SenseOp
     cfg, smaps, ksp, omega, sf = prepared("small")
@@ -89,7 +92,8 @@ def test_autotune_pogm(shapes, tmp_path):
     cache = tmp_path / "tuning.json"
     print("autotune small", flush=True)
     r = autotune_pogm(A, ksp, form="mixed", warmup=1, iters=2, cache=cache, **kw)
-    assert not r["cached"] and len(r["trials"]) == 1 + len(descent_order())
+    cands = [c for c in descent_order() if relevant(c[0], A.N, A.Nt, cfg["patch_sizes"])]
+    assert not r["cached"] and len(r["trials"]) == 1 + len(cands)
     # every candidate shape gives the built-in iterate to rounding
     for t, ms, err, ok in r["trials"]:
         assert ms > 0 and err < 1e-5, (t, err)
@@ -138,3 +142,36 @@ def test_gradient_frame_groups(shapes, form):
         assert out[nch][1] == out[1][1]
         assert np.array_equal(out[nch][0], out[1][0])
         np.testing.assert_allclose(out[nch][2][0], out[1][2][0], rtol=1e-12)
+
+
+def test_compiled_variants_are_bit_identical(shapes, rng):
+    """The alternative compiled kernels change the launch shape only: the y-z FFT passes
+    with 512 / 1024 threads at 90 x 90 x 60 (forward and adjoint bit-identical), and the
+    four global-band schedules of the tridiagonalisation on 387 x 387 Gram matrices
+    (bit-identical SVST output)."""
+    from mslr_hip.device import DeviceArray
+    from mslr_hip import recon, synth
+    from mslr_hip.sense import SenseOp
+    N, Nt, Nc = (90, 90, 60), 3, 4
+    A = SenseOp(synth.masks(N, Nt, 8, 3)[0], synth.coil_maps(N, Nc, 3))
+    x = (rng.standard_normal((*N, Nt)) + 1j * rng.standard_normal((*N, Nt))).astype(np.complex64)
+    xd = DeviceArray.from_host(x)
+    ref_f = A.forward(xd).to_host()
+    ref_a = A.adjoint(ref_f).to_host()
+    for fw, ad in [(512, 1024), (1024, 512)]:
+        shapes.set_tuning(yz_threads_fwd=fw, yz_threads_adj=ad)
+        f = A.forward(xd).to_host()
+        assert np.array_equal(f, ref_f)
+        assert np.array_equal(A.adjoint(ref_f).to_host(), ref_a)
+    A.close()
+    Z = np.asfortranarray(np.stack([lowrank(rng, 1000, 387, 9, noise=0.1) for _ in range(3)], axis=2))
+    s = np.linalg.svd(Z[:, :, 0].astype(np.complex128), compute_uv=False)
+    beta = 0.5 * (s[8] + s[9])
+    outs = []
+    for sched in range(4):
+        shapes.set_tuning(sb2st_schedule=sched)
+        outs.append(recon.svst_batched(Z, beta).to_host())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    ref = np.stack([O.SVST(Z[:, :, i].astype(np.complex128), float(np.float32(beta))) for i in range(3)], axis=2)
+    assert rel_err(outs[0], ref) < 2e-5
