@@ -323,6 +323,91 @@ __global__ __launch_bounds__(NTX) void k_adj_x(
     }
 }
 
+// The x passes of the gradient fused: one workgroup per (block of LB lines, coil,
+// frame).  The lines are transformed along x, the sampled kx gathered with the
+// post-phases and the scale, y subtracted (the residual and its squared norm), the
+// residual scattered with the conjugate post-phases into the zeroed lines,
+// transformed back along x and written over the lines it read: k_fwd_x (with ysub)
+// followed by k_adj_x on its residual, without the residual's round trip through
+// memory.  A block holds at most MAXE_X * NTX samples (LB * Nx <= MAXE_X * NTX), so
+// each thread keeps its residuals in registers while the lines are cleared: one
+// line buffer in LDS, four workgroups per CU.
+template <class T, class FXF, class FXA>
+__global__ __launch_bounds__(NTX) void k_x_resid(
+    C2<T> *__restrict__ Tb, int Nc, int Nx, int Ny, int LB, int nlb, FXF fxf, FXA fxa, LineTabs tb,
+    int64_t K, int64_t ta, const C2<T> *__restrict__ phx, const C2<T> *__restrict__ phy,
+    const C2<T> *__restrict__ phz, T scale, const C2<T> *__restrict__ ysub,
+    double *__restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);  // the block's lines
+    C2<T> *pha = buf + LB * Nx;                         // per line: y, z post-phases
+    __shared__ double red[16];
+    const int lb = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
+    const int tid = threadIdx.x;
+    const int g0 = tb.lptr[ta + f];
+    const int Lf = tb.lptr[ta + f + 1] - g0;
+    const int l0 = lb * LB;
+    if (l0 >= Lf) {  // uniform per workgroup
+        if (partial && tid == 0) partial[((int64_t)f * Nc + c) * nlb + lb] = 0.0;
+        return;
+    }
+    const int nl = min(LB, Lf - l0);
+    const int *lines = tb.line_yz + g0;
+    for (int li = tid; li < nl; li += NTX) {
+        const int yz = lines[l0 + li];
+        pha[li] = phy[yz % Ny] * phz[yz / Ny];
+    }
+    C2<T> *src = Tb + (int64_t)Nc * Nx * (g0 - tb.lptr[ta]) + (int64_t)c * Nx * Lf + l0;
+    const int lq = tid % nl, x0 = tid / nl, xs = NTX / nl;
+    if (x0 < xs) {
+#pragma unroll 4
+        for (int x = x0; x < Nx; x += xs) buf[lq * Nx + x] = src[(int64_t)x * Lf + lq];
+    }
+    __syncthreads();
+    fxf(buf, nl, Nx, 1);
+    __syncthreads();
+    const int *kb = tb.kbeg + g0;
+    const int jb = kb[l0];
+    const int je = l0 + nl < Lf ? kb[l0 + nl] : (int)K;
+    const int64_t fa = (ta + f) * K;
+    const int64_t obase = ((int64_t)f * Nc + c) * K;
+    double acc = 0.0;
+    C2<T> rv[MAXE_X];
+    int rp[MAXE_X];
+#pragma unroll
+    for (int q = 0; q < MAXE_X; ++q) {
+        const int j = jb + tid + q * NTX;
+        rp[q] = -1;
+        if (j < je) {
+            const int v = tb.lk[fa + j];  // (line << 16) | kx
+            const int l = v >> 16;
+            const int kx = v & 0xFFFF;
+            const C2<T> p = phx[kx] * pha[l - l0];
+            const C2<T> val = buf[(l - l0) * Nx + kx] * (scale * p) - ysub[obase + j];
+            acc += (double)val.re * (double)val.re + (double)val.im * (double)val.im;
+            rv[q] = cmulc(p, val);
+            rp[q] = (l - l0) * Nx + kx;
+        }
+    }
+    __syncthreads();  // every gather done before the lines are cleared
+    for (int e = tid; e < nl * Nx; e += NTX) buf[e] = C2<T>{T(0), T(0)};
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXE_X; ++q)
+        if (rp[q] >= 0) buf[rp[q]] = rv[q];
+    if (partial) {
+        const double s2 = block_sum(acc, red);
+        if (tid == 0) partial[((int64_t)f * Nc + c) * nlb + lb] = s2;
+    }
+    __syncthreads();
+    fxa(buf, nl, Nx, 1);
+    __syncthreads();
+    if (x0 < xs) {
+#pragma unroll 4
+        for (int x = x0; x < Nx; x += xs) src[(int64_t)x * Lf + lq] = buf[lq * Nx + x];
+    }
+}
+
 // Adjoint pass B: one workgroup per (x, frame).  For each coil the frame's
 // lines at x are scattered into a zeroed (y, z) plane, transformed along z and
 // y, and accumulated times the conjugate coil map; the scaled sum is written
@@ -537,6 +622,22 @@ static void launch_adj_x(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, C2
     HIP_LAUNCH_CHECK();
 }
 
+template <class T, class FXF, class FXA>
+static void launch_x_resid(mslr_sense_s *P, int64_t ta, int nf, C2<T> *Tb, const C2<T> *ysub,
+                           double *partial, FXF fxf, FXA fxa) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int Nx = P->N[0];
+    allow_lds((const void *)k_x_resid<T, FXF, FXA>);
+    dim3 g((unsigned)P->nlb, (unsigned)P->Nc, (unsigned)nf);
+    MSLR_PROF(ctx, "x_resid");
+    MSLR_REQUIRE((int64_t)P->LBx * Nx <= (int64_t)MAXE_X * NTX, MSLR_ERR_UNSUPPORTED,
+                 "x-pass line block larger than the residual registers");
+    k_x_resid<T, FXF, FXA><<<g, NTX, (size_t)P->LBx * (Nx + 1) * sizeof(C2<T>), ctx->stream>>>(
+        Tb, P->Nc, Nx, P->N[1], P->LBx, P->nlb, fxf, fxa, line_tabs(P), P->K, ta, Dev<T>::ph(P, 0),
+        Dev<T>::ph(P, 1), Dev<T>::ph(P, 2), (T)P->scale, ysub, partial);
+    HIP_LAUNCH_CHECK();
+}
+
 template <class T, class FY, class FZ>
 static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, C2<T> *Pm, FY fy,
                           FZ fz, int nt) {
@@ -650,6 +751,28 @@ static void adj_x_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, C2<T>
 }
 
 template <class T>
+static void x_resid_any(mslr_sense_s *P, int64_t ta, int nf, C2<T> *Tb, const C2<T> *ysub,
+                        double *partial) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (fixed_ok<T>(P)) {
+            switch (P->N[0]) {
+#define CASE_X(n)                                                                            \
+    case n:                                                                                  \
+        launch_x_resid<float>(P, ta, nf, Tb, ysub, partial, FixX<n, -1>{P->tw32[0]},          \
+                              FixX<n, 1>{P->tw32[0]});                                        \
+        return;
+                MSLR_X_SIZES(CASE_X)
+#undef CASE_X
+                default: break;
+            }
+        }
+    }
+    launch_x_resid<T>(P, ta, nf, Tb, ysub, partial,
+                      FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), P->sgn},
+                      FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), -P->sgn});
+}
+
+template <class T>
 static void adj_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, C2<T> *Pm) {
     if constexpr (std::is_same<T, float>::value) {
         if (fixed_ok<T>(P)) {
@@ -707,6 +830,21 @@ static void adjoint_chunk(mslr_sense_s *P, const C2<T> *y, int64_t ta, int nf, C
     launch_planes_to_image<T>(P, Pm, nf, xout);
 }
 
+// Gradient of one chunk [ta, ta+nf): g = A'(A sum_k X_k - y) with the x passes fused
+// (the residual stays in LDS); partial (nullable) gets the chunk's ||r||^2 partials.
+template <class T>
+static void grad_chunk(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride, int64_t ta,
+                       int nf, const C2<T> *ysub, double *partial, C2<T> *xout) {
+    mslr_ctx_s *ctx = P->ctx;
+    C2<T> *Pm = (C2<T> *)ctx->scratch_get(SLOT_PLANES, (size_t)nf * P->V * sizeof(C2<T>));
+    C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, chunk_line_bytes<T>(P, ta, nf));
+    launch_sum_planes<T>(P, X, nscales, sstride, nf, Pm);
+    fwd_yz_any<T>(P, ta, nf, Pm, Tb);
+    x_resid_any<T>(P, ta, nf, Tb, ysub, partial);
+    adj_yz_any<T>(P, ta, nf, Tb, Pm);
+    launch_planes_to_image<T>(P, Pm, nf, xout);
+}
+
 // Forward over frames [t0, t0+nframes) of the plan; X holds local frames.
 template <class T>
 static void run_forward(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride,
@@ -742,18 +880,10 @@ void sense_grad_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sst
                          const c64 *y_int, c64 *g, double *dc_partial) {
     mslr_ctx_s *ctx = P->ctx;
     check_lds<float>(P);
-    // Residuals of one frame chunk live in scratch; A and A' alternate per chunk.
-    const auto chunks = frame_chunks<float>(P, 0, P->Nt);
-    int64_t fmax = 1;
-    for (auto [ta, nf] : chunks) fmax = std::max(fmax, nf);
-    c64 *r = (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, fmax * P->Nc * P->K) *
-                                                     sizeof(c64));
-    for (auto [ta, nf] : chunks) {
-        forward_chunk<float>(P, X + ta * P->V, nscales, sstride, ta, (int)nf,
-                             y_int + ta * P->Nc * P->K, r,
-                             dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr);
-        adjoint_chunk<float>(P, r, ta, (int)nf, g + ta * P->V);
-    }
+    (void)ctx;
+    for (auto [ta, nf] : frame_chunks<float>(P, 0, P->Nt))
+        grad_chunk<float>(P, X + ta * P->V, nscales, sstride, ta, (int)nf, y_int + ta * P->Nc * P->K,
+                          dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr, g + ta * P->V);
 }
 
 std::vector<std::pair<int64_t, int64_t>> sense_grad_chunks(mslr_sense_s *P, int nch) {
@@ -787,12 +917,9 @@ void sense_grad_pipelined(mslr_sense_s *P, const c64 *X, int nscales, int64_t ss
             const bool on_pipe = two && (i % 2 == 1);
             ctx->stream = on_pipe ? pipe.stream : main;
             ctx->slot_base = on_pipe ? pipe.slot_base : base0;
-            c64 *r = (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, nf * P->Nc * P->K) *
-                                                             sizeof(c64));
-            forward_chunk<float>(P, X + ta * P->V, nscales, sstride, ta, (int)nf,
-                                 y_int + ta * P->Nc * P->K, r,
-                                 dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr);
-            adjoint_chunk<float>(P, r, ta, (int)nf, g + ta * P->V);
+            grad_chunk<float>(P, X + ta * P->V, nscales, sstride, ta, (int)nf,
+                              y_int + ta * P->Nc * P->K,
+                              dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr, g + ta * P->V);
             after(i, ta, nf);
         }
     } catch (...) {
@@ -1152,14 +1279,10 @@ mslr_status_t mslr_sense_grad(mslr_sense plan, const void *X_dev, int64_t nscale
         MSLR_REQUIRE(nscales >= 1, MSLR_ERR_ARG, "nscales must be >= 1");
         mslr_sense_s *P = plan;
         mslr_ctx_s *ctx = P->ctx;
-        const int64_t ny = P->K * P->Nc * (int64_t)P->Nt;
-        // Residual in reference order for the whole run (public entry point).
-        c64 *r = (c64 *)ctx->scratch_get(SLOT_RESID, (size_t)std::max<int64_t>(1, ny) * sizeof(c64));
         const int64_t npart = (int64_t)P->Nt * P->Nc * P->nlb;
         double *part = (double *)ctx->scratch_get(SLOT_REDUCE, (size_t)(npart + 1) * sizeof(double));
-        run_forward<float>(P, (const c64 *)X_dev, (int)nscales, P->V * (int64_t)P->Nt, 0, P->Nt,
-                           (const c64 *)y_dev, r, part);
-        run_adjoint<float>(P, r, 0, P->Nt, (c64 *)g_dev);
+        sense_grad_internal(P, (const c64 *)X_dev, (int)nscales, P->V * (int64_t)P->Nt,
+                            (const c64 *)y_dev, (c64 *)g_dev, part);
         if (dc_cost) {
             sum_partials(ctx, part, npart, part + npart);
             double h = 0.0;
