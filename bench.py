@@ -34,15 +34,12 @@ TRAFFIC_BYTES = {
     "fwd_yz": 7063700000,
     "adj_yz": 4403400000,
     "pogm_update": 15163600000,
-    "fwd_x": 9110200000,
-    "adj_x": 6114600000,
     "sum_planes": 4432400000,
     "sy2sb": 6116700000,
     "img2patches": 4661500000,
     "prox_voxel": 3499300000,
     "patches2img": 3153300000,
     "apply": 908900000,
-    "planes_to_image": 2479700000,
     "split": 2342200000,
     "gram_tiles": 1866900000,
     "backtransform": 946700000,
@@ -70,14 +67,11 @@ def kernel_bytes_per_iter(cfg, K, lines_total, nchunk=1):
         "sum_planes": S * V * Nt * b + V * Nt * b,
         # image planes in, coil maps in, sampled lines out
         "fwd_yz": V * Nt * b + nchunk * Nc * V * b + lines_b,
-        # lines in, y in, residual out, sample table in
-        "fwd_x": lines_b + 2 * K * Nc * Nt * b + K * Nt * 4,
-        # residual in, sample table in, lines out
-        "adj_x": K * Nc * Nt * b + K * Nt * 4 + lines_b,
+        # lines in, y in, sample table in, lines out (x FFT, residual, inverse x FFT fused)
+        "x_resid": 2 * lines_b + K * Nc * Nt * b + K * Nt * 4,
         # lines in, coil maps in, image planes out
         "adj_yz": lines_b + nchunk * Nc * V * b + V * Nt * b,
-        "planes_to_image": 2 * V * Nt * b,
-        # X, U in and out (all scales), g in
+        # X, U in and out (all scales), g in (plane-major, transposed in LDS)
         "pogm_update": 4 * S * V * Nt * b + V * Nt * b,
     }
     img2p = p2img = svst_rw = 0
@@ -125,14 +119,15 @@ def sense_fft_flops(cfg):
     """FP32 arithmetic of the SENSE FFT passes per POGM iteration, 5 n log2(n) per
     length-n transform plus 6 per coil-map product: the yz passes transform every
     (y, z) plane of every coil and frame along y and z; the x passes transform the
-    sampled (ky, kz) lines (1/R of the plane) of every coil and frame along x."""
+    sampled (ky, kz) lines (1/R of the plane) of every coil and frame along x, forward
+    and inverse (the fused residual pass)."""
     N, Nt, Nc, R = cfg["N"], cfg["Nt"], cfg["Nc"], cfg["R"]
     nx, ny, nz = N
     yz_plane = nz * 5 * ny * math.log2(ny) + ny * 5 * nz * math.log2(nz) + 6 * ny * nz
     lines = round(ny * nz / R)
     x_lines = lines * 5 * nx * math.log2(nx)
     return {"fwd_yz": yz_plane * nx * Nc * Nt, "adj_yz": yz_plane * nx * Nc * Nt,
-            "fwd_x": x_lines * Nc * Nt, "adj_x": x_lines * Nc * Nt}
+            "x_resid": 2 * x_lines * Nc * Nt}
 
 
 # FP32 vector peak without packed FP32 instructions (the build disables them):
@@ -643,7 +638,7 @@ def main() -> int:
         # the yz FFT passes against the FP32 vector roofline as well (they are bound by
         # their arithmetic, not by HBM)
         ff = sense_fft_flops(cfg)
-        for kf in ("fwd_yz", "adj_yz", "fwd_x", "adj_x"):
+        for kf in ("fwd_yz", "adj_yz", "x_resid"):
             if kf in fam and kf in ff:
                 c_, m_ = fam[kf]
                 t_ = m_ / c_ / 1e3

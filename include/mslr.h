@@ -114,8 +114,11 @@ mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
  *   yz_threads_fwd / yz_threads_adj: threads of the compiled y-z FFT passes for
  *                    planes above 2048 points, 512 or 1024 (built-in 1024 forward,
  *                    512 adjoint).  Results are bit-identical.
- *   upd_blocks:      workgroups of the fused POGM update per frame group, in
- *                    [256, 65536] (built-in 2048; read at mslr_pogm_create).
+ *   update_lines:    yz-lines per workgroup of the POGM update, which is fused with
+ *                    the gradient's last transpose (plane-major -> image order): 0 =
+ *                    built-in 64 (fewer where an Nx x 64 tile exceeds the LDS), else a
+ *                    multiple of 8 in [8, 64]; read at mslr_pogm_create.  Changes the
+ *                    grouping of the restart test's FP64 sums only.
  *   sb2st_schedule:  band -> tridiagonal kernel for Gram matrices above 300 (band in
  *                    global memory): 0 = 12 waves starting sweeps 3 time steps apart
  *                    (built-in), 1 = 16 waves 2 apart, 2 = 6 waves 5 apart, 3 = 4 waves
@@ -128,7 +131,7 @@ typedef struct mslr_tuning {
     int sense_chunks;
     int yz_threads_fwd;
     int yz_threads_adj;
-    int upd_blocks;
+    int update_lines;
     int sb2st_schedule;
     int reserved[4];
 } mslr_tuning;

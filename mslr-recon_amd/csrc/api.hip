@@ -179,8 +179,9 @@ mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t) {
         for (int v : {t->yz_threads_fwd, t->yz_threads_adj})
             MSLR_REQUIRE(v == 0 || v == 512 || v == 1024, MSLR_ERR_ARG,
                          "yz_threads_fwd / yz_threads_adj must be 0, 512 or 1024");
-        MSLR_REQUIRE(t->upd_blocks == 0 || (t->upd_blocks >= 256 && t->upd_blocks <= 65536),
-                     MSLR_ERR_ARG, "upd_blocks must be 0 or in [256, 65536]");
+        MSLR_REQUIRE(t->update_lines == 0 ||
+                         (t->update_lines >= 8 && t->update_lines <= 64 && t->update_lines % 8 == 0),
+                     MSLR_ERR_ARG, "update_lines must be 0 or a multiple of 8 in [8, 64]");
         MSLR_REQUIRE(t->sb2st_schedule >= 0 && t->sb2st_schedule <= 3, MSLR_ERR_ARG,
                      "sb2st_schedule must be in [0, 3]");
         for (int r : t->reserved) MSLR_REQUIRE(r == 0, MSLR_ERR_ARG, "reserved fields must be 0");

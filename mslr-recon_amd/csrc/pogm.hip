@@ -45,10 +45,6 @@ struct PogmDev {
     int err;     // set when a Gram matrix held non-finite values
 };
 
-#ifndef MSLR_UPD_BLOCKS
-#define MSLR_UPD_BLOCKS 2048
-#endif
-constexpr int UPD_BLOCKS = MSLR_UPD_BLOCKS;  // workgroups of the fused update (per frame group)
 // frame groups of the gradient + update: at c1 (90x90x60x300, 18 coils) 1, 2, 4, 6 and
 // 8 groups measured 47.7, 47.7, 48.2, 48.1, 47.9 ms per mixed iteration: the FFT passes
 // fill every wave slot, so the other stream's memory-bound kernels find no room
@@ -154,26 +150,46 @@ __device__ __forceinline__ void pogm_elem(float xr, float xi, float uor, float u
     dd += dr * dr + di * di;
 }
 
-// Two consecutive elements per thread (16-byte loads) when n and the scale
-// stride are even, the X and U of up to four scales loaded before any store.
-__global__ __launch_bounds__(256) void k_pogm_update(c64 *__restrict__ X, c64 *__restrict__ U,
-                                                     const c64 *__restrict__ g, int64_t n,
-                                                     int nscales, int64_t sstride,
-                                                     const PogmDev *__restrict__ s,
-                                                     double *__restrict__ partials) {
+// The update fused with the last transpose of the gradient: one workgroup per (block
+// of TB yz-lines, frame f of the chunk).  g of the block is read from the plane-major
+// gradient Pm[f][x][yz] and transposed through LDS; then, in image order, u and z of
+// every scale (pogm_elem) and the block's partials of the restart test, stored at
+// (frame ta + f, block) so that the order of the FP64 sums does not depend on how
+// the frames were grouped.  X and U start at frame ta.
+__global__ __launch_bounds__(256) void k_planes_update(const c64 *__restrict__ Pm, int Nx, int NYZ,
+                                                       int TB, int64_t ta, c64 *__restrict__ X,
+                                                       c64 *__restrict__ U, int64_t sstride,
+                                                       int nscales, const PogmDev *__restrict__ s,
+                                                       double *__restrict__ partials) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    c64 *tile = reinterpret_cast<c64 *>(smem_raw);
     __shared__ double red[16];
+    const int f = blockIdx.y;
+    const int l0 = blockIdx.x * TB;
+    const int nl = min(TB, NYZ - l0);
+    const int64_t V = (int64_t)Nx * NYZ;
+    const c64 *src = Pm + (int64_t)f * V + l0;
+    for (int e = threadIdx.x; e < nl * Nx; e += blockDim.x) {
+        const int x = e / nl;
+        const int l = e - x * nl;
+        tile[x * (TB + 1) + l] = src[(int64_t)x * NYZ + l];
+    }
+    __syncthreads();
     const double alpha = s->alpha, beta = s->beta, gamma = s->gamma;
     double dot = 0.0, gg = 0.0, dd = 0.0;
-    const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // (X, U and g start at one frame offset: all 16-byte aligned or none)
-    if ((n & 1) == 0 && (sstride & 1) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0) {
-        const int64_t n2 = n / 2, s2 = sstride / 2;
-        float4 *X4 = reinterpret_cast<float4 *>(X);
-        float4 *U4 = reinterpret_cast<float4 *>(U);
-        const float4 *g4 = reinterpret_cast<const float4 *>(g);
-        for (int64_t e = t0; e < n2; e += gstride) {
-            const float4 gv = g4[e];
+    const int64_t base = (int64_t)f * V + (int64_t)l0 * Nx;
+    if ((Nx & 1) == 0) {
+        // two x-neighbours per thread: 16-byte accesses (base, the frame offset and
+        // the scale stride are even with Nx), the X and U of up to four scales
+        // loaded before any store
+        const int64_t s2 = sstride / 2;
+        float4 *X4 = reinterpret_cast<float4 *>(X + base);
+        float4 *U4 = reinterpret_cast<float4 *>(U + base);
+        const int half = Nx / 2;
+        for (int e = threadIdx.x; e < nl * half; e += blockDim.x) {
+            const int l = e / half;
+            const int x = 2 * (e - l * half);
+            const c64 g0 = tile[x * (TB + 1) + l], g1 = tile[(x + 1) * (TB + 1) + l];
             for (int k0 = 0; k0 < nscales; k0 += 4) {
                 float4 xv[4], uv[4];
 #pragma unroll
@@ -187,9 +203,9 @@ __global__ __launch_bounds__(256) void k_pogm_update(c64 *__restrict__ X, c64 *_
                 for (int q = 0; q < 4; ++q) {
                     if (k0 + q < nscales) {
                         float4 zo, uo;
-                        pogm_elem(xv[q].x, xv[q].y, uv[q].x, uv[q].y, gv.x, gv.y, alpha, beta,
+                        pogm_elem(xv[q].x, xv[q].y, uv[q].x, uv[q].y, g0.re, g0.im, alpha, beta,
                                   gamma, zo.x, zo.y, uo.x, uo.y, dot, gg, dd);
-                        pogm_elem(xv[q].z, xv[q].w, uv[q].z, uv[q].w, gv.z, gv.w, alpha, beta,
+                        pogm_elem(xv[q].z, xv[q].w, uv[q].z, uv[q].w, g1.re, g1.im, alpha, beta,
                                   gamma, zo.z, zo.w, uo.z, uo.w, dot, gg, dd);
                         X4[(int64_t)(k0 + q) * s2 + e] = zo;
                         U4[(int64_t)(k0 + q) * s2 + e] = uo;
@@ -197,16 +213,19 @@ __global__ __launch_bounds__(256) void k_pogm_update(c64 *__restrict__ X, c64 *_
                 }
             }
         }
-    } else {  // odd sizes: one element at a time
-        for (int64_t e = t0; e < n; e += gstride) {
-            const c64 gv = g[e];
+    } else {
+        for (int e = threadIdx.x; e < nl * Nx; e += blockDim.x) {
+            const int l = e / Nx;
+            const int x = e - l * Nx;
+            const c64 gv = tile[x * (TB + 1) + l];
+            const int64_t i = base + e;
             for (int k = 0; k < nscales; ++k) {
-                const int64_t i = (int64_t)k * sstride + e;
+                const c64 xv = X[(int64_t)k * sstride + i], uv = U[(int64_t)k * sstride + i];
                 float zr, zi, ur, ui;
-                pogm_elem(X[i].re, X[i].im, U[i].re, U[i].im, gv.re, gv.im, alpha, beta, gamma,
-                          zr, zi, ur, ui, dot, gg, dd);
-                X[i] = c64{zr, zi};
-                U[i] = c64{ur, ui};
+                pogm_elem(xv.re, xv.im, uv.re, uv.im, gv.re, gv.im, alpha, beta, gamma, zr, zi, ur,
+                          ui, dot, gg, dd);
+                X[(int64_t)k * sstride + i] = c64{zr, zi};
+                U[(int64_t)k * sstride + i] = c64{ur, ui};
             }
         }
     }
@@ -214,9 +233,10 @@ __global__ __launch_bounds__(256) void k_pogm_update(c64 *__restrict__ X, c64 *_
     const double b = block_sum(gg, red);
     const double c = block_sum(dd, red);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x + 0] = a;
-        partials[3 * blockIdx.x + 1] = b;
-        partials[3 * blockIdx.x + 2] = c;
+        const int64_t o = 3 * ((ta + f) * (int64_t)gridDim.x + blockIdx.x);
+        partials[o + 0] = a;
+        partials[o + 1] = b;
+        partials[o + 2] = c;
     }
 }
 
@@ -429,7 +449,7 @@ struct mslr_pogm_s {
     std::vector<double> lambdas;
     double L = 0;
     // device
-    c64 *X = nullptr, *U = nullptr, *g = nullptr, *y = nullptr;
+    c64 *X = nullptr, *U = nullptr, *y = nullptr;
     PogmDev *dev = nullptr;
     double *lam_dev = nullptr, *thr_dev = nullptr, *partials = nullptr;
     int *log_dev = nullptr;
@@ -454,7 +474,8 @@ struct mslr_pogm_s {
     // the gradient's frame groups, each followed by its part of the update; the
     // batched forms alternate them between the context stream and gpipe
     std::vector<std::pair<int64_t, int64_t>> gchunks;
-    int upd_blocks = 0;  // workgroups of the fused update per frame group
+    int upd_lines = 0;       // yz-lines per workgroup of the fused transpose + update
+    int64_t n_upd_part = 0;  // partials of the restart test (frames x line blocks)
     hipStream_t gpipe = nullptr;
     hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;
     // batched forms: one stream per scale for the prox (fork / join with events)
@@ -520,14 +541,20 @@ static void pogm_iteration(mslr_pogm_s *st) {
         sp.fork = st->ev_gfork;
         sp.join = st->ev_gjoin;
     }
-    sense_grad_pipelined(st->sense, st->X, (int)st->S, n, st->y, st->g,
+    // the update reads each frame group's gradient where the adjoint leaves it
+    // (plane-major), fused with the transpose back to image order
+    const int TB = st->upd_lines;
+    const int Nx = st->sense->N[0], NYZ = st->sense->NYZ;
+    const size_t tile_lds = (size_t)Nx * (TB + 1) * sizeof(c64);
+    sense_grad_pipelined(st->sense, st->X, (int)st->S, n, st->y, nullptr,
                          st->log_costs ? st->dc_part : nullptr, st->gchunks, sp,
-                         [&](size_t i, int64_t ta, int64_t nf) {
+                         [&](size_t, int64_t ta, int64_t nf, const c64 *planes) {
                              MSLR_PROF(ctx, "pogm_update");
                              const int64_t o = ta * st->V;
-                             k_pogm_update<<<st->upd_blocks, 256, 0, ctx->stream>>>(
-                                 st->X + o, st->U + o, st->g + o, nf * st->V, (int)st->S, n,
-                                 st->dev, st->partials + 3 * (size_t)st->upd_blocks * i);
+                             dim3 grid((unsigned)cdiv(NYZ, TB), (unsigned)nf);
+                             k_planes_update<<<grid, 256, tile_lds, ctx->stream>>>(
+                                 planes, Nx, NYZ, TB, ta, st->X + o, st->U + o, n, (int)st->S,
+                                 st->dev, st->partials);
                              HIP_LAUNCH_CHECK();
                          });
     if (st->log_costs) {
@@ -544,8 +571,8 @@ static void pogm_iteration(mslr_pogm_s *st) {
     }
     {
         MSLR_PROF(ctx, "pogm_post");
-        k_pogm_post<<<1, 256, 0, ctx->stream>>>(st->dev, st->partials,
-                                                st->upd_blocks * (int)st->gchunks.size(), st->log_dev);
+        k_pogm_post<<<1, 256, 0, ctx->stream>>>(st->dev, st->partials, (int)st->n_upd_part,
+                                                st->log_dev);
         HIP_LAUNCH_CHECK();
     }
     int *err = (int *)((char *)st->dev + offsetof(PogmDev, err));
@@ -663,7 +690,6 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
             const int64_t n = st->V * st->Nt;
             st->X = (c64 *)st->alloc((size_t)(n * st->S) * sizeof(c64));
             st->U = (c64 *)st->alloc((size_t)(n * st->S) * sizeof(c64));
-            st->g = (c64 *)st->alloc((size_t)n * sizeof(c64));
             st->y = (c64 *)st->alloc((size_t)std::max<int64_t>(1, st->K * st->Nc * st->Nt) *
                                      sizeof(c64));
             st->dev = (PogmDev *)st->alloc(sizeof(PogmDev));
@@ -674,10 +700,17 @@ mslr_status_t mslr_pogm_create(mslr_sense sense, const mslr_pogm_desc *desc, msl
                                                                     : SENSE_CHUNKS_DEFAULT;
                 st->gchunks = sense_grad_chunks(sense, nch);
             }
-            st->upd_blocks = sense->ctx->tuning.upd_blocks > 0 ? sense->ctx->tuning.upd_blocks
-                                                               : UPD_BLOCKS;
-            st->partials = (double *)st->alloc(sizeof(double) * 3 * (size_t)st->upd_blocks *
-                                               std::max<size_t>(1, st->gchunks.size()));
+            // restart-test partials of the fused update: one triple per (frame, line block)
+            st->upd_lines = sense->ctx->tuning.update_lines > 0
+                                ? std::min(sense->ctx->tuning.update_lines, sense->NYZ)
+                                : sense_transpose_lines(sense);
+            MSLR_REQUIRE((size_t)sense->N[0] * (st->upd_lines + 1) * sizeof(c64) <= 160 * 1024 - 256,
+                         MSLR_ERR_UNSUPPORTED, "update_lines: the Nx x lines tile exceeds the LDS");
+            st->n_upd_part = cdiv(sense->NYZ, st->upd_lines) * (int64_t)st->Nt;
+            st->partials = (double *)st->alloc(sizeof(double) * 3 * (size_t)st->n_upd_part);
+            HIP_CHECK(hipFuncSetAttribute((const void *)k_planes_update,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          160 * 1024 - 256));
             st->log_dev = (int *)st->alloc(sizeof(int) * std::max(1, d.niter));
             HIP_CHECK(hipMemcpy(st->lam_dev, st->lambdas.data(), sizeof(double) * st->S,
                                 hipMemcpyHostToDevice));

@@ -832,9 +832,11 @@ static void adjoint_chunk(mslr_sense_s *P, const C2<T> *y, int64_t ta, int nf, C
 
 // Gradient of one chunk [ta, ta+nf): g = A'(A sum_k X_k - y) with the x passes fused
 // (the residual stays in LDS); partial (nullable) gets the chunk's ||r||^2 partials.
+// xout == nullptr leaves g in the plane-major scratch (returned) for a consumer that
+// reads it there.
 template <class T>
-static void grad_chunk(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride, int64_t ta,
-                       int nf, const C2<T> *ysub, double *partial, C2<T> *xout) {
+static C2<T> *grad_chunk(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride, int64_t ta,
+                         int nf, const C2<T> *ysub, double *partial, C2<T> *xout) {
     mslr_ctx_s *ctx = P->ctx;
     C2<T> *Pm = (C2<T> *)ctx->scratch_get(SLOT_PLANES, (size_t)nf * P->V * sizeof(C2<T>));
     C2<T> *Tb = (C2<T> *)ctx->scratch_get(SLOT_FFT, chunk_line_bytes<T>(P, ta, nf));
@@ -842,7 +844,8 @@ static void grad_chunk(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sst
     fwd_yz_any<T>(P, ta, nf, Pm, Tb);
     x_resid_any<T>(P, ta, nf, Tb, ysub, partial);
     adj_yz_any<T>(P, ta, nf, Tb, Pm);
-    launch_planes_to_image<T>(P, Pm, nf, xout);
+    if (xout) launch_planes_to_image<T>(P, Pm, nf, xout);
+    return Pm;
 }
 
 // Forward over frames [t0, t0+nframes) of the plan; X holds local frames.
@@ -901,7 +904,7 @@ void sense_grad_pipelined(mslr_sense_s *P, const c64 *X, int nscales, int64_t ss
                           const c64 *y_int, c64 *g, double *dc_partial,
                           const std::vector<std::pair<int64_t, int64_t>> &chunks,
                           const SensePipe &pipe,
-                          const std::function<void(size_t, int64_t, int64_t)> &after) {
+                          const std::function<void(size_t, int64_t, int64_t, const c64 *)> &after) {
     mslr_ctx_s *ctx = P->ctx;
     check_lds<float>(P);
     hipStream_t main = ctx->stream;
@@ -917,10 +920,10 @@ void sense_grad_pipelined(mslr_sense_s *P, const c64 *X, int nscales, int64_t ss
             const bool on_pipe = two && (i % 2 == 1);
             ctx->stream = on_pipe ? pipe.stream : main;
             ctx->slot_base = on_pipe ? pipe.slot_base : base0;
-            grad_chunk<float>(P, X + ta * P->V, nscales, sstride, ta, (int)nf,
-                              y_int + ta * P->Nc * P->K,
-                              dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr, g + ta * P->V);
-            after(i, ta, nf);
+            const c64 *planes = grad_chunk<float>(
+                P, X + ta * P->V, nscales, sstride, ta, (int)nf, y_int + ta * P->Nc * P->K,
+                dc_partial ? dc_partial + ta * P->Nc * P->nlb : nullptr, g ? g + ta * P->V : nullptr);
+            after(i, ta, nf, planes);
         }
     } catch (...) {
         ctx->stream = main;
@@ -934,6 +937,8 @@ void sense_grad_pipelined(mslr_sense_s *P, const c64 *X, int nscales, int64_t ss
         HIP_CHECK(hipStreamWaitEvent(main, pipe.join, 0));
     }
 }
+
+int sense_transpose_lines(mslr_sense_s *P) { return transpose_tb<float>(P); }
 
 void sense_resid_internal(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
                           const c64 *y_int, double *dc_partial) {

@@ -66,14 +66,21 @@ struct SensePipe {
     hipEvent_t fork = nullptr, join = nullptr;
 };
 
-// g = A'(A sum_k X_k - y_int) chunk by chunk; after(i, ta, nf) enqueues work on
-// the chunk's frames [ta, ta+nf) behind chunk i's adjoint on the stream the chunk
-// ran on (ctx->stream while it is called).  The context stream waits for all.
+// g = A'(A sum_k X_k - y_int) chunk by chunk; after(i, ta, nf, planes) enqueues work
+// on the chunk's frames [ta, ta+nf) behind chunk i's adjoint on the stream the chunk
+// ran on (ctx->stream while it is called).  With g == nullptr the chunk's gradient is
+// left plane-major in scratch, planes[f][x][y + Ny z] (f = 0..nf-1), valid until the
+// stream's next chunk; else it is written to g and planes holds the same values.  The
+// context stream waits for all chunks.
 void sense_grad_pipelined(mslr_sense_s *P, const c64 *X, int nscales, int64_t sstride,
                           const c64 *y_int, c64 *g, double *dc_partial,
                           const std::vector<std::pair<int64_t, int64_t>> &chunks,
                           const SensePipe &pipe,
-                          const std::function<void(size_t, int64_t, int64_t)> &after);
+                          const std::function<void(size_t, int64_t, int64_t, const c64 *)> &after);
+
+// yz-lines per workgroup of the plane-major <-> image transposes (an LDS tile of
+// Nx x (lines + 1) values).
+int sense_transpose_lines(mslr_sense_s *P);
 
 // Partial ||A sum_k X_k - y_int||^2 (Nt*Nc*nlb doubles into dc_partial), the
 // forward half of sense_grad_internal.

@@ -71,14 +71,14 @@ struct Tuning
     sense_chunks::Cint
     yz_threads_fwd::Cint
     yz_threads_adj::Cint
-    upd_blocks::Cint
+    update_lines::Cint
     sb2st_schedule::Cint
     reserved::NTuple{4,Cint}
 end
 
 """
     set_tuning!(ctx; gram_chunk=0, eig_lanes_limit=0, svst_apply=0, sense_chunks=0,
-                yz_threads_fwd=0, yz_threads_adj=0, upd_blocks=0, sb2st_schedule=0)
+                yz_threads_fwd=0, yz_threads_adj=0, update_lines=0, sb2st_schedule=0)
 
 Launch shapes on `ctx` (0 keeps the built-in choice), see `mslr_tuning` in
 include/mslr.h.  The Python autotuner (`mslr_hip.autotune.autotune_pogm`) picks them
@@ -86,9 +86,9 @@ from timed POGM iterations; its JSON cache holds the values to pass here.
 """
 function set_tuning!(ctx::HipContext; gram_chunk::Integer = 0, eig_lanes_limit::Integer = 0,
                      svst_apply::Integer = 0, sense_chunks::Integer = 0, yz_threads_fwd::Integer = 0,
-                     yz_threads_adj::Integer = 0, upd_blocks::Integer = 0, sb2st_schedule::Integer = 0)
+                     yz_threads_adj::Integer = 0, update_lines::Integer = 0, sb2st_schedule::Integer = 0)
     t = Ref(Tuning(gram_chunk, eig_lanes_limit, svst_apply, sense_chunks, yz_threads_fwd, yz_threads_adj,
-                   upd_blocks, sb2st_schedule, ntuple(_ -> Cint(0), 4)))
+                   update_lines, sb2st_schedule, ntuple(_ -> Cint(0), 4)))
     check(ccall((:mslr_ctx_set_tuning, LIB), Cint, (Ptr{Cvoid}, Ref{Tuning}), ctx.h, t))
     return ctx
 end
@@ -100,7 +100,7 @@ function get_tuning(ctx::HipContext)
     return (gram_chunk = Int(v.gram_chunk), eig_lanes_limit = Int(v.eig_lanes_limit),
             svst_apply = Int(v.svst_apply), sense_chunks = Int(v.sense_chunks),
             yz_threads_fwd = Int(v.yz_threads_fwd), yz_threads_adj = Int(v.yz_threads_adj),
-            upd_blocks = Int(v.upd_blocks), sb2st_schedule = Int(v.sb2st_schedule))
+            update_lines = Int(v.update_lines), sb2st_schedule = Int(v.sb2st_schedule))
 end
 
 "Column-major device array owned by the library (ComplexF32 unless stated)."

@@ -89,7 +89,7 @@ class mslr_tuning(C.Structure):
         ("sense_chunks", C.c_int),
         ("yz_threads_fwd", C.c_int),
         ("yz_threads_adj", C.c_int),
-        ("upd_blocks", C.c_int),
+        ("update_lines", C.c_int),
         ("sb2st_schedule", C.c_int),
         ("reserved", C.c_int * 4),
     ]

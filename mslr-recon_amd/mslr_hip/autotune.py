@@ -34,7 +34,7 @@ SPACE: dict[str, tuple[int, ...]] = {
     "sense_chunks": (0, 2, 4, 8),
     "yz_threads_fwd": (0, 512),
     "yz_threads_adj": (0, 1024),
-    "upd_blocks": (0, 1024, 8192),
+    "update_lines": (0, 32, 16),
     "sb2st_schedule": (0, 1, 2, 3),
 }
 BUILTIN = {k: 0 for k in SPACE}

@@ -45,7 +45,7 @@ class Context:
         _lib.check(self._lib.mslr_ctx_set_products(self.handle, code))
 
     TUNING_FIELDS = ("gram_chunk", "eig_lanes_limit", "svst_apply", "sense_chunks", "yz_threads_fwd",
-                     "yz_threads_adj", "upd_blocks", "sb2st_schedule")
+                     "yz_threads_adj", "update_lines", "sb2st_schedule")
 
     def set_tuning(self, **kw) -> None:
         """Launch shapes (mslr_ctx_set_tuning; an omitted or 0 field = built-in):
@@ -53,7 +53,8 @@ class Context:
         eig_lanes_limit (patches x m limit for 16 lanes per eigenvector, -1: always 4),
         svst_apply (1: dense apply), sense_chunks (frame groups of the gradient, at most
         64), yz_threads_fwd / yz_threads_adj (512 or 1024 threads of the compiled y-z FFT
-        passes), upd_blocks (update workgroups, [256, 65536]), sb2st_schedule (0-3: the
+        passes), update_lines (yz-lines per workgroup of the fused transpose + update, a
+        multiple of 8 in [8, 64]), sb2st_schedule (0-3: the
         global-band tridiagonalisation's wave schedule)."""
         bad = set(kw) - set(self.TUNING_FIELDS)
         if bad:

@@ -22,7 +22,7 @@ def _gpu_run(name, niter, form="plain"):
     return X.to_host(), restarts, st
 
 
-@pytest.mark.parametrize("name,niter", [("small", 6), ("c0", 4)])
+@pytest.mark.parametrize("name,niter", [("small", 6), ("c0", 4), ("small_odd", 5)])
 def test_pogm_matches_oracle(name, niter):
     cfg, smaps, ksp, omega, sf = prepared(name)
     X, restarts, _ = _gpu_run(name, niter)

@@ -14,7 +14,7 @@ from test_gpu_svst import APPLY_RANKS, _beta_between, lowrank, mixed_rank_batch
 pytestmark = pytest.mark.gpu
 
 BUILTIN = {"gram_chunk": 0, "eig_lanes_limit": 0, "svst_apply": 0, "sense_chunks": 0,
-           "yz_threads_fwd": 0, "yz_threads_adj": 0, "upd_blocks": 0, "sb2st_schedule": 0}
+           "yz_threads_fwd": 0, "yz_threads_adj": 0, "update_lines": 0, "sb2st_schedule": 0}
 
 
 @pytest.fixture
@@ -28,13 +28,13 @@ def shapes(ctx):
 def test_tuning_roundtrip_and_validation(shapes):
     assert shapes.tuning() == BUILTIN
     want = {"gram_chunk": 2048, "eig_lanes_limit": -1, "svst_apply": 1, "sense_chunks": 4,
-            "yz_threads_fwd": 512, "yz_threads_adj": 1024, "upd_blocks": 4096, "sb2st_schedule": 2}
+            "yz_threads_fwd": 512, "yz_threads_adj": 1024, "update_lines": 32, "sb2st_schedule": 2}
     shapes.set_tuning(**want)
     assert shapes.tuning() == want
     for bad in [dict(gram_chunk=100), dict(gram_chunk=128), dict(gram_chunk=1 << 17),
                 dict(gram_chunk=-256), dict(eig_lanes_limit=-2), dict(svst_apply=2),
                 dict(sense_chunks=-1), dict(sense_chunks=65), dict(yz_threads_fwd=256),
-                dict(yz_threads_adj=768), dict(upd_blocks=100), dict(sb2st_schedule=4)]:
+                dict(yz_threads_adj=768), dict(update_lines=100), dict(update_lines=12), dict(sb2st_schedule=4)]:
         with pytest.raises(ValueError):
             shapes.set_tuning(**bad)
         assert shapes.tuning() == want
@@ -142,6 +142,33 @@ def test_gradient_frame_groups(shapes, form):
         assert out[nch][1] == out[1][1]
         assert np.array_equal(out[nch][0], out[1][0])
         np.testing.assert_allclose(out[nch][2][0], out[1][2][0], rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["c0", "small_odd"])
+def test_update_lines(shapes, name):
+    """The update fused with the gradient's transpose, 8 / 16 / 32 yz-lines per workgroup
+    against the built-in 64 (c0: 16-byte accesses, even Nx; small_odd: Nx = 15): the
+    elementwise results are bit-identical and the restart tests agree."""
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.sense import SenseOp
+    cfg, smaps, ksp, omega, sf = prepared(name)
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    A = SenseOp(omega, smaps)
+    out = {}
+    for lines in (0, 8, 16, 32):
+        shapes.set_tuning(update_lines=lines)
+        st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=2.0,
+                  niter=5, form="plain", log_costs=True)
+        st.init(ksp)
+        st.run(5)
+        out[lines] = (st.X.to_host(), st.log()[1], st.costs())
+        st.close()
+    A.close()
+    for lines in (8, 16, 32):
+        assert out[lines][1] == out[0][1]
+        assert np.array_equal(out[lines][0], out[0][0])
+        np.testing.assert_allclose(out[lines][2][0], out[0][2][0], rtol=1e-12)
 
 
 def test_compiled_variants_are_bit_identical(shapes, rng):
