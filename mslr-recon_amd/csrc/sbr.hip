@@ -28,7 +28,7 @@
 namespace mslr {
 
 constexpr int B = SBR_B;
-constexpr int S1T = 512;       // threads of the dense -> band kernel
+constexpr int S1T = SBR_S1_WAVES * 64;  // threads of the dense -> band kernel
 constexpr int S2T = 768;       // threads of the band -> tridiagonal kernel
 constexpr int S2W = S2T / 64;  // concurrent sweeps
 // the band -> tridiagonal kernel with the band in global memory: threads and
@@ -48,8 +48,9 @@ static_assert((SBR_MAXG - 1 + B - 1) / B <= STAGGER_G * (S2TG / 64),
               "a sweep must end before the wave's next one starts");
 constexpr int STAGGER = 2;     // time steps between the starts of consecutive sweeps
 constexpr int BTT = 256;       // threads of the back-transformation
-constexpr int BTC = 13;        // eigenvectors per back-transformation workgroup: one thread per
+constexpr int BTC = 8;         // eigenvectors per back-transformation workgroup: one thread per
                                // (reflector, eigenvector) of a sweep, LDS for two workgroups per CU
+constexpr int BT_PD = 4;       // sweeps of bulge-chasing reflectors prefetched into registers
 
 __device__ __forceinline__ c128 c_div(c128 a, c128 b) {
     const double den = b.re * b.re + b.im * b.im;
@@ -110,8 +111,8 @@ __device__ __forceinline__ c128 ld_if(bool c, const c128 *p) {
 // ------------------------------------------------------------------ stage 1
 // One workgroup per matrix, two per CU (a batch of a few hundred matrices runs in
 // one round): V ((m - B) x B, column-major), T and one B x B block (S = V^H V, then
-// X = V^H W turned into M = T^H X in place) in LDS, at most 80 KB; Y and W in global
-// memory (L2); at most 128 VGPRs (four waves per SIMD).
+// Z = V^H Y turned into M = T^H Z T in place) in LDS, at most 80 KB; Y, W and the
+// waves' partials of Z in global memory (L2); at most 128 VGPRs (four waves per SIMD).
 __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m, int npanel,
                                                   c128 *__restrict__ Tall,
                                                   c128 *__restrict__ W1all) {
@@ -124,6 +125,7 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
     c128 *V = reinterpret_cast<c128 *>(smem_raw);
     c128 *Y = W1all + (int64_t)blockIdx.x * sbr_w1_elems(m);
     c128 *W = Y + (size_t)Lmax * B;
+    c128 *Zp = W + (size_t)Lmax * B;  // the waves' partials of V^H Y
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     c128 *A = Gall + (int64_t)blockIdx.x * m * m;
     c128 *Tg = Tall + (int64_t)blockIdx.x * npanel * B * B;
@@ -264,15 +266,18 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
         }
         __syncthreads();
         for (int pr = tid; pr < B * B; pr += S1T) Tg[(int64_t)p * B * B + pr] = Ts[pr % B][pr / B];
-        // 5. Y = A22 V -> W on the FP64 matrix cores: one 16-row block per wave at a
-        //    time; the A22 operand is read from global memory once.  Only the lower
-        //    triangle of A22 is kept up to date: an entry above the diagonal is read
-        //    as the conjugate of its mirror.
-        //    v_mfma_f64_16x16x4f64 (D = P Q): lane l holds P[l%16][l/16],
-        //    Q[l/16][l%16] and D[l/16 + 4v][l%16].
+        // 5. Y = A22 V on the FP64 matrix cores, one 16-row block per wave at a time;
+        //    the A22 operand is read from global memory once.  Only the lower triangle
+        //    of A22 is kept up to date: an entry above the diagonal is read as the
+        //    conjugate of its mirror.  v_mfma_f64_16x16x4f64 (D = P Q): lane l holds
+        //    P[l%16][l/16], Q[l/16][l%16] and D[l/16 + 4v][l%16] -- the accumulator
+        //    layout is the Q operand's over four k-steps, so the same pass also
+        //    accumulates the wave's share of Z = V^H Y from the Y block in registers.
+        //    Y is stored row-major (Y[i B + c]) for step 7.
         {
             const int nrb = (L + 15) / 16;
             const int rl = lane & 15, kl = lane >> 4;
+            md4 zr = {0.0, 0.0, 0.0, 0.0}, zi = {0.0, 0.0, 0.0, 0.0};
             for (int rb = wave; rb < nrb; rb += S1T / 64) {
                 // four independent accumulator chains (summed below) keep the
                 // matrix pipe busy between dependent MFMAs
@@ -287,85 +292,92 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
                                                       : A + (r0 + k) + (int64_t)m * (r0 + i));
                     if (!lo) a.im = -a.im;
                     const c128 v = ld_if(k < L, V + rl * L + k);
-                    // operands swapped: the block computed is Y^T, so the output
-                    // lanes run down the columns of W
-                    yr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, a.re, yr, 0, 0, 0);
-                    yi = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, a.re, yi, 0, 0, 0);
-                    yr2 = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, -a.im, yr2, 0, 0, 0);
-                    yi2 = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, a.im, yi2, 0, 0, 0);
+                    // P[i][k] = A, Q[k][c] = V: the block of Y = A V
+                    yr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, v.re, yr, 0, 0, 0);
+                    yi = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, v.im, yi, 0, 0, 0);
+                    yr2 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, v.im, yr2, 0, 0, 0);
+                    yi2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, v.re, yi2, 0, 0, 0);
                 }
                 yr += yr2;
                 yi += yi2;
-                if (i < L) {
+                // lane (rl, kl) holds Y[rb 16 + kl + 4q][rl]
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) Y[(kl + 4 * q) * L + i] = c128{yr[q], yi[q]};
+                for (int q = 0; q < 4; ++q) {
+                    const int ii = rb * 16 + kl + 4 * q;
+                    if (ii < L) Y[ii * B + rl] = c128{yr[q], yi[q]};
+                }
+                // Z += V_blk^H Y_blk over the block's rows: k-step s covers rows
+                // rb 16 + 4s .. +3, P[a][i] = conj(V[i][a]), Q[i][c] = Y[i][c] (yr[s], yi[s])
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int ii = rb * 16 + 4 * q + kl;
+                    const c128 vv = ld_if(ii < L, V + rl * L + ii);
+                    zr = __builtin_amdgcn_mfma_f64_16x16x4f64(vv.re, yr[q], zr, 0, 0, 0);
+                    zr = __builtin_amdgcn_mfma_f64_16x16x4f64(vv.im, yi[q], zr, 0, 0, 0);
+                    zi = __builtin_amdgcn_mfma_f64_16x16x4f64(vv.re, yi[q], zi, 0, 0, 0);
+                    zi = __builtin_amdgcn_mfma_f64_16x16x4f64(-vv.im, yr[q], zi, 0, 0, 0);
                 }
             }
+            // the wave's partial Z (lane holds Z[kl + 4q][rl]; zero for a wave without
+            // blocks) -> its slot of the workspace
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Zp[wave * B * B + (kl + 4 * q) * B + rl] = c128{zr[q], zi[q]};
         }
         __syncthreads();
-        // 6. W <- Y T (T upper)
-        for (int task = tid; task < L * B; task += S1T) {
-            const int i = task % L, c = task / L;
-            c128 acc = {0.0, 0.0};
-            for (int a = 0; a <= c; ++a) acc = acc + Y[a * L + i] * Ts[a][c];
-            W[c * L + i] = acc;
-        }
-        __syncthreads();
-        // 7. X = V^H Y (two threads per entry over interleaved rows, two partial sums
-        //    each; V_a is zero above row a), M = T^H X (Hermitian)
+        // 6. Z = the sum of the wave partials (fixed order), then M = T^H X with
+        //    X = V^H W = V^H Y T, i.e. M = T^H Z T (Hermitian), one entry per thread:
+        //    Z -> Ms, Z T -> Ms, T^H (Z T) -> Ms (T upper triangular)
         {
-            const int pr = tid >> 1, part = tid & 1;
-            const int a = pr % B, c = pr / B;
-            const c128 *va = V + a * L, *wc = W + c * L;
-            c128 s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
-            int i = a + part;
-            for (; i + 2 < L; i += 4) {
-                s0 = s0 + cmulc(va[i], wc[i]);
-                s1 = s1 + cmulc(va[i + 2], wc[i + 2]);
+            const int a = tid % B, c = tid / B;
+            if (tid < B * B) {
+                c128 z = {0.0, 0.0};
+                for (int w = 0; w < S1T / 64; ++w) z = z + Zp[w * B * B + a * B + c];
+                Ms[a][c] = z;
             }
-            if (i < L) s0 = s0 + cmulc(va[i], wc[i]);
-            s0 = s0 + s1;
-            s0.re += __shfl_xor(s0.re, 1, 64);
-            s0.im += __shfl_xor(s0.im, 1, 64);
-            if (part == 0) Ms[a][c] = s0;  // X
+            __syncthreads();
+            c128 acc = {0.0, 0.0};
+            if (tid < B * B)
+                for (int q = 0; q <= c; ++q) acc = acc + Ms[a][q] * Ts[q][c];
+            __syncthreads();
+            if (tid < B * B) Ms[a][c] = acc;
+            __syncthreads();
+            acc = c128{0.0, 0.0};
+            if (tid < B * B)
+                for (int q = 0; q <= a; ++q) acc = acc + cmulc(Ts[q][a], Ms[q][c]);
+            __syncthreads();
+            if (tid < B * B) Ms[a][c] = acc;
         }
         __syncthreads();
-        // M = T^H X in place, one column per thread, bottom row first (row a needs
-        // rows 0..a of X)
-        if (tid < B) {
-            const int c = tid;
-            for (int a = B - 1; a >= 0; --a) {
-                c128 s = {0.0, 0.0};
-                for (int q = 0; q <= a; ++q) s = s + cmulc(Ts[q][a], Ms[q][c]);
-                Ms[a][c] = s;
-            }
-        }
-        __syncthreads();
-        // 8. W = Y - 1/2 V M on the matrix cores, one 16-row block per wave at a time
-        //    (the block computed is W^T: lane (g, rl) holds W[rl][g + 4q])
+        // 7. W = Y T - 1/2 V M on the matrix cores, one 16-row block per wave at a time
+        //    (the block computed is W^T = T^T Y^T - 1/2 M^T V^T: lane (rl, kl) holds
+        //    W[rb 16 + rl][kl + 4q])
         {
             const int nrb = (L + 15) / 16;
             const int rl = lane & 15, kl = lane >> 4;
             for (int rb = wave; rb < nrb; rb += S1T / 64) {
                 const int i = rb * 16 + rl;
-                md4 wr, wi;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const c128 w0 = ld_if(i < L, W + (kl + 4 * q) * L + i);
-                    wr[q] = w0.re;
-                    wi[q] = w0.im;
-                }
-#pragma unroll
+                md4 wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
+                md4 ur = {0.0, 0.0, 0.0, 0.0}, ui = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
                 for (int as = 0; as < B; as += 4) {
                     const int a = as + kl;
-                    const c128 mm = Ms[a][rl];  // P[c][a] = M[a][c], c = rl
+                    const c128 t = Ts[a][rl];   // P[c][a] = T[a][c], c = rl
+                    const c128 mm = Ms[a][rl];  // P[c][a] = M[a][c]
+                    const c128 y = ld_if(i < L, Y + i * B + a);
                     const c128 v = ld_if(i < L, V + a * L + i);
-                    // V[i][a] M[a][c]: re = vr mr - vi mi, im = vr mi + vi mr
-                    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.re, v.re, wr, 0, 0, 0);
-                    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(0.5 * mm.im, v.im, wr, 0, 0, 0);
-                    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.im, v.re, wi, 0, 0, 0);
-                    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.re, v.im, wi, 0, 0, 0);
+                    // T^T Y^T: re = tr yr - ti yi, im = tr yi + ti yr
+                    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(t.re, y.re, wr, 0, 0, 0);
+                    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(t.re, y.im, wi, 0, 0, 0);
+                    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(-t.im, y.im, wr, 0, 0, 0);
+                    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(t.im, y.re, wi, 0, 0, 0);
+                    // -1/2 M^T V^T: re = -(vr mr - vi mi)/2, im = -(vr mi + vi mr)/2
+                    ur = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.re, v.re, ur, 0, 0, 0);
+                    ui = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.im, v.re, ui, 0, 0, 0);
+                    ur = __builtin_amdgcn_mfma_f64_16x16x4f64(0.5 * mm.im, v.im, ur, 0, 0, 0);
+                    ui = __builtin_amdgcn_mfma_f64_16x16x4f64(-0.5 * mm.re, v.im, ui, 0, 0, 0);
                 }
+                wr += ur;
+                wi += ui;
                 if (i < L) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) W[(kl + 4 * q) * L + i] = c128{wr[q], wi[q]};
@@ -373,7 +385,7 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
             }
         }
         __syncthreads();
-        // 9. A22 <- A22 - V W^H - W V^H on the FP64 matrix cores, the 16x16 blocks on
+        // 8. A22 <- A22 - V W^H - W V^H on the FP64 matrix cores, the 16x16 blocks on
         //    and below the diagonal (the lower triangle is what later steps read),
         //    one block per wave at a time, K = 2B complex.
         {
@@ -636,22 +648,27 @@ __global__ __launch_bounds__(NT) void k_sb2st(const c128 *__restrict__ Gall, int
 }
 
 // Back-transformation U[:, c0 + c] = Q1 Q2 Z[:, c0 + c] for BTC eigenvectors per
-// workgroup, accumulated in FP64 in LDS (BTC x m: two workgroups per CU).
+// workgroup, accumulated in FP64 in LDS (BTC x m).
 // Q2, the sweep-major product of the bulge-chasing reflectors, is applied in
 // reverse sweep order, z <- z - tau v (v^H z); the reflectors of one sweep act
 // on disjoint row windows and are applied together, one thread per (reflector,
 // eigenvector) with its 16-term product in registers.  A sweep's reflectors are
-// staged in LDS, the next sweep's loaded while the current one is applied.  Q1,
-// the panels in reverse order, as U <- U - V (T (V^H U)) with V (unit lower
-// form, tails stored below the band of the reduced matrix) passed through LDS
-// in 16 x 16 tiles, again loading the next tile during the current one.
-// KS: reflectors per sweep, at most (the staging of a sweep in LDS; the smaller
-// instance keeps two workgroups per CU up to m = SBR_MAXM).
+// staged in LDS; the next BT_PD sweeps are in flight in registers (a ring, the
+// loop unrolled by its depth), so a sweep waits for no global load.
+// Q1, the panels in reverse order, U <- U - V (T (V^H U)) on the FP64 matrix
+// cores with the V operand (unit lower form, the tails stored below the band of
+// the reduced matrix) loaded from global memory: X = V^H U row block by row block
+// over the four waves (partials summed through LDS in a fixed order), Y = T X,
+// then U -= V Y, each row block by one wave.
+// KS: reflectors per sweep, at most (the smaller instance keeps two workgroups per
+// CU up to m = SBR_MAXM).
 constexpr int BT_KS_LDS = (SBR_MAXM - 1 + B - 1) / B;
 constexpr int BT_KS_MAX = (SBR_MAXG - 1 + B - 1) / B;
 constexpr int BT_KP = BTT / BTC;  // reflectors applied at once
 static_assert(BT_KS_MAX * B <= 2 * BTT, "a sweep's reflectors are staged two per thread");
-static_assert(BT_KS_LDS * B >= B * B, "the staging buffer also holds a 16 x 16 tile");
+static_assert(2 * BT_KS_LDS * B >= 2 * B * B, "the staging buffers also hold two 16 x 16 partials");
+static_assert(BTC <= 16 && BTT == 256, "one 16-column MFMA tile of eigenvectors, four waves");
+static_assert(BT_KP >= BT_KS_MAX, "one reflector of a sweep per thread");
 
 template <int KS>
 __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, int m, int npanel,
@@ -663,15 +680,16 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
                                                 c64 *__restrict__ Uall) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ c128 Ts[B][B + 1];
-    __shared__ c128 XY[B][BTC + 1];
-    __shared__ c128 Vb[2][KS * B];  // a sweep's reflectors / a 16 x 16 tile of V
+    __shared__ c128 XY[B][B + 1];
+    __shared__ c128 Vbuf[2 * KS * B];  // two sweeps' reflectors / two 16 x 16 partials of X
     __shared__ c128 Tb[2][KS];
+    __shared__ c128 xch[2][BT_KP][BTC];  // window rows handed from thread k to k + 1
     const int ip = blockIdx.x;
     const int r = rall[ip];
     const int c0 = blockIdx.y * BTC;
     if (c0 >= r) return;
     const int ncol = min(BTC, r - c0);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     c128 *Uc = reinterpret_cast<c128 *>(smem_raw);  // BTC x m
     const c128 *A = Gall + (int64_t)ip * m * m;
     const double *Z = Zall + (int64_t)ip * m * m;
@@ -681,117 +699,181 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         if (c < ncol) z.re = Z[(int64_t)(c0 + c) * m + i];
         Uc[idx] = z;
     }
-    // Q2
+    // Q2.  Thread (k, c) applies reflector k of every sweep to eigenvector c and
+    // keeps its 16-row window of the eigenvector in registers: the windows of sweep s,
+    // rows s+1+16k .. s+16+16k, tile rows s+1 .. m-1, and from sweep s to s-1 each
+    // moves up one row -- the thread takes row s+16k from thread k-1 (row s, never
+    // touched by a later sweep, from LDS for k = 0) and hands its last row to thread
+    // k+1, through a double-buffered LDS slot.  The sweep loop is unrolled by B so
+    // that the window's rotation is a renaming of registers.
     {
         const c128 *V2 = V2all + (int64_t)ip * m * ks * B;
         const c128 *tau2 = tau2all + (int64_t)ip * m * ks;
         auto nsweep_refl = [&](int s) { return (m - s - 1 + B - 1) / B; };
-        // this thread's share of a sweep's reflectors (<= 2 values) and tau
-        c128 pv0, pv1, pt;
-        auto fetch = [&](int s) {
+        // ring slot j holds this thread's share of a sweep's reflectors (<= 2
+        // values) and tau: sweep s sits in slot (m - 2 - s) % BT_PD
+        c128 rv0[BT_PD], rv1[BT_PD], rt[BT_PD];
+        auto fetch = [&](int s, c128 &v0, c128 &v1, c128 &t) {
             const int n = nsweep_refl(s) * B;
             const c128 *src = V2 + (int64_t)s * ks * B;
-            pv0 = ld_if(tid < n, src + tid);
-            pv1 = ld_if(tid + BTT < n, src + tid + BTT);
-            pt = ld_if(tid < nsweep_refl(s), tau2 + (int64_t)s * ks + tid);
+            v0 = ld_if(tid < n, src + tid);
+            v1 = ld_if(tid + BTT < n, src + tid + BTT);
+            t = ld_if(tid < nsweep_refl(s), tau2 + (int64_t)s * ks + tid);
         };
-        auto put = [&](int buf) {
-            Vb[buf][tid] = pv0;
-            if (tid + BTT < KS * B) Vb[buf][tid + BTT] = pv1;
-            if (tid < KS) Tb[buf][tid] = pt;
+        auto put = [&](int buf, const c128 &v0, const c128 &v1, const c128 &t) {
+            Vbuf[buf * KS * B + tid] = v0;
+            if (tid + BTT < KS * B) Vbuf[buf * KS * B + tid + BTT] = v1;
+            if (tid < KS) Tb[buf][tid] = t;
         };
-        // (reflector, eigenvector) of this thread; reflectors k, k + BT_KP, ...
-        const int k = tid / BTC, c = tid - k * BTC;
-        if (m >= 2) {
-            fetch(m - 2);
-            put(0);
+        const int k = tid / BTC, c = tid - k * BTC;  // BT_KP >= KS: one reflector a thread
+        const bool col = c < ncol;
+        c128 w[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) w[q] = c128{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < BT_PD; ++j)
+            if (m - 2 - j >= 0) fetch(m - 2 - j, rv0[j], rv1[j], rt[j]);
+        if (m >= 2) put(0, rv0[0], rv1[0], rt[0]);
+        __syncthreads();  // (also orders the initialisation of Uc before the read below)
+        if (k == 0 && col && m >= 2) w[0] = Uc[c * m + m - 1];
+        int buf = 0;
+        static_assert(B % BT_PD == 0, "the ring slot of an unrolled sweep is static");
+        for (int s0 = m - 2; s0 >= 0; s0 -= B) {
+#pragma clang loop unroll(full)
+            for (int j = 0; j < B; ++j) {
+                const int s = s0 - j;
+                if (s >= 0) {  // (no early exit: a constant trip count unrolls fully)
+                    const int slot = j % BT_PD;
+                    // slot (sweep s) is staged: refill it with sweep s - BT_PD
+                    if (s - BT_PD >= 0) fetch(s - BT_PD, rv0[slot], rv1[slot], rt[slot]);
+                    const int nst = nsweep_refl(s);
+                    if (k < nst && col) {
+                        const c128 tk = Tb[buf][k];
+                        if (nonzero(tk)) {
+                            const int len = min(B, m - (s + 1 + k * B));
+                            const c128 *v = &Vbuf[buf * KS * B + k * B];
+                            c128 d0 = {0.0, 0.0}, d1 = {0.0, 0.0};
+#pragma unroll
+                            for (int q = 0; q < B; q += 2) {
+                                d0 = d0 + cmulc(ld_if(q < len, v + q), w[(q - j + 16 * B) % B]);
+                                d1 = d1 + cmulc(ld_if(q + 1 < len, v + q + 1), w[(q + 1 - j + 16 * B) % B]);
+                            }
+                            const c128 f = tk * (d0 + d1);
+#pragma unroll
+                            for (int q = 0; q < B; ++q)
+                                if (q < len) {
+                                    c128 &wq = w[(q - j + 16 * B) % B];
+                                    wq = wq - f * v[q];
+                                }
+                        }
+                    }
+                    // the window's last row (logical q = B - 1) to thread k + 1
+                    xch[s & 1][k][c] = w[(B - 1 - j + 16 * B) % B];
+                    // sweep s - 1 (the next slot) -> the other staging buffer
+                    if (s > 0) {
+                        const int nx = (j + 1) % BT_PD;
+                        put(buf ^ 1, rv0[nx], rv1[nx], rt[nx]);
+                    }
+                    __syncthreads();
+                    buf ^= 1;
+                    // rotate: logical q of sweep s - 1 is logical q - 1 of sweep s; the
+                    // freed register (logical B - 1) becomes logical 0 = row s + 16k
+                    if (s > 0) {
+                        c128 in = {0.0, 0.0};
+                        if (col) in = k == 0 ? Uc[c * m + s] : xch[s & 1][k - 1][c];
+                        w[(B - 1 - j + 16 * B) % B] = in;
+                    } else if (col && k < nsweep_refl(0)) {
+                        // after sweep 0: the windows (rows 1+16k ..) back to LDS
+#pragma unroll
+                        for (int q = 0; q < B; ++q) {
+                            const int row = 1 + k * B + q;
+                            if (row < m) Uc[c * m + row] = w[(q - j + 16 * B) % B];
+                        }
+                    }
+                }
+            }
         }
         __syncthreads();
-        for (int s = m - 2, buf = 0; s >= 0; --s, buf ^= 1) {
-            if (s > 0) fetch(s - 1);
-            const int nst = nsweep_refl(s);
-            for (int kk = k; k < BT_KP && kk < nst; kk += BT_KP) {
-                const c128 tk = Tb[buf][kk];
-                if (!(c < ncol && nonzero(tk))) continue;
-                const int st = s + 1 + kk * B;
-                const int len = min(B, m - st);
-                const c128 *v = &Vb[buf][kk * B];
-                c128 *u = Uc + c * m + st;
-                c128 vv[B], uu[B];
-#pragma unroll
-                for (int q = 0; q < B; ++q) {
-                    vv[q] = ld_if(q < len, v + q);
-                    uu[q] = ld_if(q < len, u + q);
-                }
-                c128 d0 = {0.0, 0.0}, d1 = {0.0, 0.0};
-#pragma unroll
-                for (int q = 0; q < B; q += 2) {
-                    d0 = d0 + cmulc(vv[q], uu[q]);
-                    d1 = d1 + cmulc(vv[q + 1], uu[q + 1]);
-                }
-                const c128 f = tk * (d0 + d1);
-#pragma unroll
-                for (int q = 0; q < B; ++q)
-                    if (q < len) u[q] = uu[q] - f * vv[q];
-            }
-            if (s > 0) put(buf ^ 1);
-            __syncthreads();
-        }
     }
-    // Q1
-    const int ta = tid % B, tr = tid / B;  // tile slot: V[i0 + tr][ta]; also (a, c) / (row, c)
+    // Q1 on the matrix cores: v_mfma_f64_16x16x4f64 (D = P Q), lane (rl, kl) holds
+    // P[rl][kl], Q[kl][rl] and D[kl + 4q][rl]
+    const int rl = lane & 15, kl = lane >> 4;
     for (int p = npanel - 1; p >= 0; --p) {
         const int pc0 = p * B, r0 = pc0 + B, L = m - r0;
         const int kq = min(B, L - 1);
+        const int nrb = (L + 15) / 16;
         for (int pr = tid; pr < B * B; pr += BTT)
             Ts[pr % B][pr / B] = Tall[((int64_t)ip * npanel + p) * B * B + pr];
-        auto vtile = [&](int i0) {
-            const int i = i0 + tr;
-            c128 v = ld_if(ta < kq && i > ta && i < L, A + (r0 + i) + (int64_t)m * (pc0 + ta));
-            if (ta < kq && i == ta) v.re = 1.0;
+        // V[i][a] of the panel (i < L): 1 on the diagonal, 0 above, the stored tail below
+        auto vel = [&](int i, int a) {
+            c128 v = ld_if(a < kq && i > a && i < L, A + (r0 + i) + (int64_t)m * (pc0 + a));
+            if (a < kq && i == a) v.re = 1.0;
             return v;
         };
-        // X = V^H U: thread (a, c) = (ta, tr)
-        c128 x0 = {0.0, 0.0}, x1 = {0.0, 0.0};
-        c128 pv = vtile(0);
-        for (int i0 = 0, buf = 0; i0 < L; i0 += B, buf ^= 1) {
-            Vb[buf][tr * B + ta] = pv;
-            __syncthreads();
-            if (i0 + B < L) pv = vtile(i0 + B);
-            if (tr < BTC) {
-                const c128 *uc = Uc + tr * m + r0 + i0;
-                const int n = min(B, L - i0);
+        // X = V^H U: k-steps over the block's rows, P[a][i] = conj(V[i][a]) (a = rl),
+        // Q[i][c] = U[c][r0 + i] (c = rl)
+        md4 xr = {0.0, 0.0, 0.0, 0.0}, xi = {0.0, 0.0, 0.0, 0.0};
+        for (int rb = wave; rb < nrb; rb += BTT / 64) {
 #pragma unroll
-                for (int ii = 0; ii < B; ii += 2) {
-                    if (ii < n) x0 = x0 + cmulc(Vb[buf][ii * B + ta], uc[ii]);
-                    if (ii + 1 < n) x1 = x1 + cmulc(Vb[buf][(ii + 1) * B + ta], uc[ii + 1]);
-                }
+            for (int q = 0; q < 4; ++q) {
+                const int i = rb * 16 + 4 * q + kl;
+                const c128 v = vel(i, rl);
+                const c128 u = ld_if(rl < ncol && i < L, Uc + rl * m + r0 + i);
+                // conj(v) u: re = vr ur + vi ui, im = vr ui - vi ur
+                xr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, u.re, xr, 0, 0, 0);
+                xr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, u.im, xr, 0, 0, 0);
+                xi = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, u.im, xi, 0, 0, 0);
+                xi = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.im, u.re, xi, 0, 0, 0);
             }
         }
-        if (tr < BTC) XY[ta][tr] = x0 + x1;
-        __syncthreads();
-        {   // Y = T X (T upper)
-            c128 y = {0.0, 0.0};
-            if (tr < BTC)
-                for (int q = ta; q < B; ++q) y = y + Ts[ta][q] * XY[q][tr];
-            __syncthreads();
-            if (tr < BTC) XY[ta][tr] = y;
-        }
-        // U <- U - V Y: thread (row i0 + ta, c = tr)
-        pv = vtile(0);
-        for (int i0 = 0, buf = 0; i0 < L; i0 += B, buf ^= 1) {
-            Vb[buf][tr * B + ta] = pv;
-            __syncthreads();
-            if (i0 + B < L) pv = vtile(i0 + B);
-            const int i = i0 + ta;
-            if (tr < BTC && i < L) {
-                c128 y0 = {0.0, 0.0}, y1 = {0.0, 0.0};
+        // the four partials (lane: X[kl + 4q][rl]) summed in a fixed order: waves 0, 1
+        // store, waves 2, 3 add theirs, then the two slots are added
+        c128 *P0 = Vbuf;
+        if (wave < 2) {
 #pragma unroll
-                for (int a2 = 0; a2 < B; a2 += 2) {
-                    y0 = y0 + Vb[buf][ta * B + a2] * XY[a2][tr];
-                    y1 = y1 + Vb[buf][ta * B + a2 + 1] * XY[a2 + 1][tr];
+            for (int q = 0; q < 4; ++q)
+                P0[wave * B * B + (kl + 4 * q) * B + rl] = c128{xr[q], xi[q]};
+        }
+        __syncthreads();
+        if (wave >= 2) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                c128 &d = P0[(wave - 2) * B * B + (kl + 4 * q) * B + rl];
+                d = d + c128{xr[q], xi[q]};
+            }
+        }
+        __syncthreads();
+        {   // Y = T X (T upper), thread (a, c) = (tid / B, tid % B)
+            const int a = tid / B, c = tid % B;
+            c128 y = {0.0, 0.0};
+            for (int q = a; q < B; ++q) y = y + Ts[a][q] * (P0[q * B + c] + P0[B * B + q * B + c]);
+            XY[a][c] = y;
+        }
+        __syncthreads();
+        // U -= V Y: P[i][a] = V[rb 16 + rl][a], Q[a][c] = Y[a][c] (a = 4s + kl, c = rl);
+        // lane (rl, kl) owns U[c = rl][r0 + rb 16 + kl + 4q]
+        for (int rb = wave; rb < nrb; rb += BTT / 64) {
+            md4 dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int a = 4 * s4 + kl;
+                const c128 v = vel(rb * 16 + rl, a);
+                const c128 y = XY[a][rl];
+                // v y: re = vr yr - vi yi, im = vr yi + vi yr
+                dr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, y.re, dr, 0, 0, 0);
+                dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.im, y.im, dr, 0, 0, 0);
+                di = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, y.im, di, 0, 0, 0);
+                di = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, y.re, di, 0, 0, 0);
+            }
+            if (rl < ncol) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = rb * 16 + kl + 4 * q;
+                    if (i < L) {
+                        c128 &u = Uc[rl * m + r0 + i];
+                        u = u - c128{dr[q], di[q]};
+                    }
                 }
-                Uc[tr * m + r0 + i] = Uc[tr * m + r0 + i] - (y0 + y1);
             }
         }
         __syncthreads();

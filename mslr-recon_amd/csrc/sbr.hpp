@@ -26,9 +26,14 @@ inline int sbr_panels(int m) {
 // reflector slots per sweep of the band -> tridiagonal stage
 inline int sbr_steps(int m) { return (m - 1 + SBR_B - 1) / SBR_B; }
 
+constexpr int SBR_S1_WAVES = 8;  // waves of the dense -> band kernel
+
 // per matrix workspace of the dense -> band stage in global memory: the panel's
-// Y = A V and W = Y T - 1/2 V M ((m - B) x B each)
-__host__ __device__ inline size_t sbr_w1_elems(int m) { return (size_t)2 * (m - SBR_B) * SBR_B; }
+// Y = A V and W = Y T - 1/2 V M ((m - B) x B each) and the waves' B x B partials of
+// V^H Y
+__host__ __device__ inline size_t sbr_w1_elems(int m) {
+    return (size_t)2 * (m - SBR_B) * SBR_B + (size_t)SBR_S1_WAVES * SBR_B * SBR_B;
+}
 
 struct SbrWork {
     c128 *T1 = nullptr;    // (sbr_panels(m), B, B) per matrix: compact-WY T of each panel
