@@ -96,6 +96,14 @@ __device__ __forceinline__ bool nonzero(c128 t) { return t.re != 0.0 || t.im != 
 
 typedef double md4 __attribute__((ext_vector_type(4)));
 
+// d + conj(a) b and w - f v, four FMAs each
+__device__ __forceinline__ c128 fma_conj(c128 a, c128 b, c128 d) {
+    return c128{fma(a.re, b.re, fma(a.im, b.im, d.re)), fma(a.re, b.im, fma(-a.im, b.re, d.im))};
+}
+__device__ __forceinline__ c128 fms_c(c128 f, c128 v, c128 w) {
+    return c128{fma(-f.re, v.re, fma(f.im, v.im, w.re)), fma(-f.re, v.im, fma(-f.im, v.re, w.im))};
+}
+
 // Value of *p when c, else 0, through scalar loads (a conditionally assigned
 // struct otherwise ends up in private memory).
 __device__ __forceinline__ c128 ld_if(bool c, const c128 *p) {
@@ -666,7 +674,6 @@ constexpr int BT_KS_LDS = (SBR_MAXM - 1 + B - 1) / B;
 constexpr int BT_KS_MAX = (SBR_MAXG - 1 + B - 1) / B;
 constexpr int BT_KP = BTT / BTC;  // reflectors applied at once
 static_assert(BT_KS_MAX * B <= 2 * BTT, "a sweep's reflectors are staged two per thread");
-static_assert(2 * BT_KS_LDS * B >= 2 * B * B, "the staging buffers also hold two 16 x 16 partials");
 static_assert(BTC <= 16 && BTT == 256, "one 16-column MFMA tile of eigenvectors, four waves");
 static_assert(BT_KP >= BT_KS_MAX, "one reflector of a sweep per thread");
 
@@ -680,9 +687,14 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
                                                 c64 *__restrict__ Uall) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ c128 Ts[B][B + 1];
-    __shared__ c128 XY[B][B + 1];
-    __shared__ c128 Vbuf[2 * KS * B];  // two sweeps' reflectors / two 16 x 16 partials of X
+    // two sweeps' reflectors (stride BS: room for every thread's two values) / two
+    // 16 x 16 partials of X
+    constexpr int BS = KS * B > 2 * BTT ? KS * B : 2 * BTT;
+    __shared__ c128 Vbuf[2 * BS];
     __shared__ c128 Tb[2][KS];
+    // Q1's Y = T X tile, in the second staging buffer (idle during Q1)
+    c128 (*XY)[B + 1] = reinterpret_cast<c128 (*)[B + 1]>(Vbuf + BS);
+    static_assert(B * (B + 1) <= BS, "the Y tile fits the second staging buffer");
     __shared__ c128 xch[2][BT_KP][BTC];  // window rows handed from thread k to k + 1
     const int ip = blockIdx.x;
     const int r = rall[ip];
@@ -713,16 +725,20 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         // ring slot j holds this thread's share of a sweep's reflectors (<= 2
         // values) and tau: sweep s sits in slot (m - 2 - s) % BT_PD
         c128 rv0[BT_PD], rv1[BT_PD], rt[BT_PD];
+        // loads from clamped addresses and selects (no branches): entries past the
+        // sweep's reflectors read as zero
         auto fetch = [&](int s, c128 &v0, c128 &v1, c128 &t) {
-            const int n = nsweep_refl(s) * B;
+            const int nr = nsweep_refl(s), n = nr * B;
             const c128 *src = V2 + (int64_t)s * ks * B;
-            v0 = ld_if(tid < n, src + tid);
-            v1 = ld_if(tid + BTT < n, src + tid + BTT);
-            t = ld_if(tid < nsweep_refl(s), tau2 + (int64_t)s * ks + tid);
+            const c128 a0 = src[min(tid, n - 1)], a1 = src[min(tid + BTT, n - 1)];
+            const c128 a2 = tau2[(int64_t)s * ks + min(tid, nr - 1)];
+            v0 = tid < n ? a0 : c128{0.0, 0.0};
+            v1 = tid + BTT < n ? a1 : c128{0.0, 0.0};
+            t = tid < nr ? a2 : c128{0.0, 0.0};
         };
         auto put = [&](int buf, const c128 &v0, const c128 &v1, const c128 &t) {
-            Vbuf[buf * KS * B + tid] = v0;
-            if (tid + BTT < KS * B) Vbuf[buf * KS * B + tid + BTT] = v1;
+            Vbuf[buf * BS + tid] = v0;
+            Vbuf[buf * BS + tid + BTT] = v1;
             if (tid < KS) Tb[buf][tid] = t;
         };
         const int k = tid / BTC, c = tid - k * BTC;  // BT_KP >= KS: one reflector a thread
@@ -748,23 +764,29 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
                     if (s - BT_PD >= 0) fetch(s - BT_PD, rv0[slot], rv1[slot], rt[slot]);
                     const int nst = nsweep_refl(s);
                     if (k < nst && col) {
+                        // the reflector in registers, zero past its length (rows past
+                        // m - 1), so that the product and the update need no per-row
+                        // condition; a zero tau makes the update a no-op
                         const c128 tk = Tb[buf][k];
-                        if (nonzero(tk)) {
-                            const int len = min(B, m - (s + 1 + k * B));
-                            const c128 *v = &Vbuf[buf * KS * B + k * B];
-                            c128 d0 = {0.0, 0.0}, d1 = {0.0, 0.0};
+                        const int len = min(B, m - (s + 1 + k * B));
+                        const c128 *v = &Vbuf[buf * BS + k * B];
+                        c128 vr[B];
 #pragma unroll
-                            for (int q = 0; q < B; q += 2) {
-                                d0 = d0 + cmulc(ld_if(q < len, v + q), w[(q - j + 16 * B) % B]);
-                                d1 = d1 + cmulc(ld_if(q + 1 < len, v + q + 1), w[(q + 1 - j + 16 * B) % B]);
-                            }
-                            const c128 f = tk * (d0 + d1);
+                        for (int q = 0; q < B; ++q) {
+                            const c128 t = v[q];
+                            vr[q] = q < len ? t : c128{0.0, 0.0};
+                        }
+                        c128 d0 = {0.0, 0.0}, d1 = {0.0, 0.0};
 #pragma unroll
-                            for (int q = 0; q < B; ++q)
-                                if (q < len) {
-                                    c128 &wq = w[(q - j + 16 * B) % B];
-                                    wq = wq - f * v[q];
-                                }
+                        for (int q = 0; q < B; q += 2) {
+                            d0 = fma_conj(vr[q], w[(q - j + 16 * B) % B], d0);
+                            d1 = fma_conj(vr[q + 1], w[(q + 1 - j + 16 * B) % B], d1);
+                        }
+                        const c128 f = tk * (d0 + d1);
+#pragma unroll
+                        for (int q = 0; q < B; ++q) {
+                            c128 &wq = w[(q - j + 16 * B) % B];
+                            wq = fms_c(f, vr[q], wq);
                         }
                     }
                     // the window's last row (logical q = B - 1) to thread k + 1
@@ -779,9 +801,10 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
                     // rotate: logical q of sweep s - 1 is logical q - 1 of sweep s; the
                     // freed register (logical B - 1) becomes logical 0 = row s + 16k
                     if (s > 0) {
-                        c128 in = {0.0, 0.0};
-                        if (col) in = k == 0 ? Uc[c * m + s] : xch[s & 1][k - 1][c];
-                        w[(B - 1 - j + 16 * B) % B] = in;
+                        const c128 from_u = Uc[c * m + s];  // (c < BTC: a valid address)
+                        const c128 from_x = xch[s & 1][k > 0 ? k - 1 : 0][c];
+                        w[(B - 1 - j + 16 * B) % B] =
+                            !col ? c128{0.0, 0.0} : (k == 0 ? from_u : from_x);
                     } else if (col && k < nsweep_refl(0)) {
                         // after sweep 0: the windows (rows 1+16k ..) back to LDS
 #pragma unroll
