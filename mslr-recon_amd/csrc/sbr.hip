@@ -86,8 +86,12 @@ __device__ __forceinline__ Hh householder(c128 alpha, double xn) {
     } else {
         const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn);
         const double beta = alpha.re >= 0.0 ? -nrm : nrm;
-        h.tau = c128{(beta - alpha.re) / beta, -alpha.im / beta};
-        h.scal = c_div(c128{1.0, 0.0}, c128{alpha.re - beta, alpha.im});
+        // tau = (beta - alpha) / beta, scal = 1 / (alpha - beta): two divisions
+        const double ib = 1.0 / beta;
+        h.tau = c128{(beta - alpha.re) * ib, -alpha.im * ib};
+        const double dr = alpha.re - beta, di = alpha.im;
+        const double id = 1.0 / (dr * dr + di * di);
+        h.scal = c128{dr * id, -di * id};
         h.beta = beta;
     }
     return h;
@@ -102,6 +106,10 @@ __device__ __forceinline__ c128 fma_conj(c128 a, c128 b, c128 d) {
 }
 __device__ __forceinline__ c128 fms_c(c128 f, c128 v, c128 w) {
     return c128{fma(-f.re, v.re, fma(f.im, v.im, w.re)), fma(-f.re, v.im, fma(-f.im, v.re, w.im))};
+}
+// w - a conj(b), four FMAs
+__device__ __forceinline__ c128 fms_cc(c128 a, c128 b, c128 w) {
+    return c128{fma(-a.re, b.re, fma(-a.im, b.im, w.re)), fma(-a.im, b.re, fma(a.re, b.im, w.im))};
 }
 
 // Value of *p when c, else 0, through scalar loads (a conditionally assigned
@@ -491,49 +499,51 @@ __device__ __forceinline__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *
     }
     const int st2 = st + B;
     const int len2 = min(B, m - st2);
-    // D (Hermitian, upper part from the lower) and E into registers
+    // D (Hermitian, upper part from the lower) and E into registers; loads from
+    // clamped addresses and selects: entries past the matrix read as zero (sv is
+    // zero past len)
     c128 Dr[4], Er[4], vq[4];
-    const c128 vr = ld_if(r < len, sv + r);
+    const c128 vr = sv[r];
+    const int rc = min(r, len - 1);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int q = g + 4 * t;
-        vq[t] = ld_if(q < len, sv + q);
-        Dr[t] = c128{0.0, 0.0};
-        if (q < len && r < len) {
-            if (q >= r) Dr[t] = bref(Bd, m, st + q, st + r);
-            else Dr[t] = conj(bref(Bd, m, st + r, st + q));
-        }
-        Er[t] = ld_if(q < len2 && r < len, &bref(Bd, m, st2 + q, st + r));
+        vq[t] = sv[q];
+        const int lo = min(min(q, r), len - 1), hi = max(q, r);
+        c128 dv = Bd[(st + lo) * (2 * B) + (hi - lo)];
+        if (q < r) dv.im = -dv.im;
+        Dr[t] = (q < len && r < len) ? dv : c128{0.0, 0.0};
+        const c128 ev = Bd[(st + rc) * (2 * B) + (B + q - rc)];
+        Er[t] = (q < len2 && r < len) ? ev : c128{0.0, 0.0};
     }
     if (nonzero(tau)) {
-        // p = tau D v and tau E v (row sums)
-        c128 pd[4], pe[4];
-        c128 part = {0.0, 0.0};
+        // p = tau D v by columns (D Hermitian: (D v)[c] = sum_q conj(D[q][c]) v[q]):
+        // the lane's four rows, then the four row groups
+        c128 pc = {0.0, 0.0};
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            pd[t] = tau * sum16(Dr[t] * vr);
-            pe[t] = tau * sum16(Er[t] * vr);
-            part = part + cmulc(pd[t], vq[t]);
-        }
-        // p^H v: the rows of lanes hold disjoint q; add the four row results
-        const c128 dot = {(readlane_d(part.re, 0) + readlane_d(part.re, 16)) +
-                              (readlane_d(part.re, 32) + readlane_d(part.re, 48)),
-                          (readlane_d(part.im, 0) + readlane_d(part.im, 16)) +
-                              (readlane_d(part.im, 32) + readlane_d(part.im, 48))};
+        for (int t = 0; t < 4; ++t) pc = fma_conj(Dr[t], vq[t], pc);
+        pc.re += __shfl_xor(pc.re, 16, 64);
+        pc.im += __shfl_xor(pc.im, 16, 64);
+        pc.re += __shfl_xor(pc.re, 32, 64);
+        pc.im += __shfl_xor(pc.im, 32, 64);
+        pc = tau * pc;  // sum_q conj(D[q][c]) v[q] = sum_q D[c][q] v[q] = (D v)[c]
+        // q = tau E v (row sums)
+        c128 pe[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) pe[t] = tau * sum16(Er[t] * vr);
+        // p^H v over the columns of one row of lanes (every row holds all of p)
+        const c128 dot = sum16(cmulc(pc, vr));
         const c128 alpha2 = (-0.5) * (tau * dot);
-        // w_q for the own rows; w_r for the own column through LDS
+        // w = p + alpha2 v: w_r for the own column, w_q for the own rows through LDS
+        const c128 wr = pc + alpha2 * vr;
         wave_lds_sync();
-        if (r == 0) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) sw[g + 4 * t] = pd[t];
-        }
+        if (g == 0) sw[r] = pc;
         wave_lds_sync();
-        const c128 wr = ld_if(r < len, sw + r) + alpha2 * vr;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const c128 wq = pd[t] + alpha2 * vq[t];
-            Dr[t] = Dr[t] - vq[t] * conj(wr) - wq * conj(vr);   // D <- H^H D H
-            Er[t] = Er[t] - pe[t] * conj(vr);                   // E <- E H
+            const c128 wq = sw[g + 4 * t] + alpha2 * vq[t];
+            Dr[t] = fms_cc(wq, vr, fms_cc(vq[t], wr, Dr[t]));  // D <- H^H D H
+            Er[t] = fms_cc(pe[t], vr, Er[t]);                   // E <- E H
         }
         // D back to the band (lower part)
 #pragma unroll
@@ -578,8 +588,8 @@ __device__ __forceinline__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *
             c128 cs = {0.0, 0.0};
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                vn[t] = ld_if(g + 4 * t < len2, sv + g + 4 * t);
-                cs = cs + cmulc(vn[t], Er[t]);
+                vn[t] = sv[g + 4 * t];  // (zero past len2)
+                cs = fma_conj(vn[t], Er[t], cs);
             }
             cs.re += __shfl_xor(cs.re, 16, 64);
             cs.im += __shfl_xor(cs.im, 16, 64);
@@ -588,7 +598,7 @@ __device__ __forceinline__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *
             cs = conj(h.tau) * cs;
             if (r >= 1) {
 #pragma unroll
-                for (int t = 0; t < 4; ++t) Er[t] = Er[t] - vn[t] * cs;
+                for (int t = 0; t < 4; ++t) Er[t] = fms_c(cs, vn[t], Er[t]);
             }
         }
         // E back to the band

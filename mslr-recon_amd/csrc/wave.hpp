@@ -13,11 +13,13 @@ constexpr int DPP_QP_2301 = 0x4E;  // quad_perm [2,3,0,1]: lane i reads i^2
 constexpr int DPP_ROW_HALF_MIRROR = 0x141;
 constexpr int DPP_ROW_MIRROR = 0x140;
 
+// (the row permutations below read a valid lane for every lane, so no "old"
+// value is needed: mov_dpp with bound_ctrl, no zeroing move first)
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
