@@ -1110,6 +1110,7 @@ typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 constexpr int BKS = 32;         // complex k per LDS stage
 constexpr int BLD = BKS + 4;    // LDS row pitch in words: 16-byte reads of 16 rows hit distinct banks
+constexpr int GB_T = 512;       // threads of the split-BF16 Gram (eight waves of 32 x 16)
 
 __device__ __forceinline__ void split_c64(c64 v, unsigned &hi, unsigned &lo) {
     const bf2v h = {(__bf16)v.re, (__bf16)v.im};
@@ -1247,15 +1248,19 @@ __global__ __launch_bounds__(256) void k_split_cols(const c64 *__restrict__ X, i
 // The s2*s2 terms are kept because dropping them is a one-signed bias of the
 // diagonal, ~2^-20 of the dominant eigenvalue, which would bury eigenvalues 1e8
 // below it.  Operands are (X(k, b), X(k, a)): G_re = [br, bi].[ar, ai],
-// G_im = [bi, -br].[ar, ai] over interleaved k.
-__global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ Wh,
-                                                   const unsigned *__restrict__ Wl,
-                                                   int64_t pstride, int64_t cs, int m, int64_t Kc,
-                                                   int nkc, int kch, int ntile, int vec,
-                                                   c128 *__restrict__ part, c128 *__restrict__ G,
-                                                   int conj_out, int *__restrict__ err) {
-    __shared__ __align__(16) unsigned Sh[2][GT][BLD];  // [A cols a, B cols b][col][k]
-    __shared__ __align__(16) unsigned Sl[2][GT][BLD];
+// G_im = [bi, -br].[ar, ai] over interleaved k.  Eight waves of 32 x 16 sub-tiles (two
+// 16 x 16 blocks, so 152 VGPRs and two waves per SIMD) and two LDS stages (one
+// barrier per step).  The block sums being exact, any tiling or product order gives
+// the same Gram bit for bit.
+__global__ __launch_bounds__(GB_T) void k_gram_bf16(const unsigned *__restrict__ Wh,
+                                                    const unsigned *__restrict__ Wl,
+                                                    int64_t pstride, int64_t cs, int m, int64_t Kc,
+                                                    int nkc, int kch, int ntile, int vec,
+                                                    c128 *__restrict__ part, c128 *__restrict__ G,
+                                                    int conj_out, int *__restrict__ err) {
+    // two LDS stages: the next step's words are stored while the current one is read
+    __shared__ __align__(16) unsigned Sh[2][2][GT][BLD];  // [stage][A cols a, B cols b][col][k]
+    __shared__ __align__(16) unsigned Sl[2][2][GT][BLD];
     // the tile pairs of one (patch, chunk) share column tiles: keep them on one XCD
     const int64_t nwg = (int64_t)gridDim.x * gridDim.y * gridDim.z;
     const int64_t wl = xcd_order(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z), nwg);
@@ -1270,109 +1275,104 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ 
     const int a0 = ti * GT, b0 = tj * GT;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wa = (wave & 1) * 32, wb = (wave >> 1) * 32;
+    // eight waves, each a 32 (a) x 16 (b) sub-tile: two 16 x 16 blocks
+    const int wa = (wave & 1) * 32, wb = (wave >> 1) * 16;
     const int r16 = lane & 15, g4 = lane >> 4;
-    // load slot: one column, eight consecutive rows
-    const int lc = tid >> 2;
-    const int lk = (tid & 3) * 8;
+    // load slot: one column, four consecutive k (one 16-byte word group)
+    const int lc = tid >> 3;
+    const int lk = (tid & 7) * 4;
     const bool va = a0 + lc < m, vb = b0 + lc < m;
     const int64_t oa = ip * pstride + (int64_t)(a0 + lc) * cs;
     const int64_t ob = ip * pstride + (int64_t)(b0 + lc) * cs;
-    mf4 hr[2][2], hi[2][2], xr[2][2], xi[2][2], lr[2][2], li[2][2];
-    md4 dr[2][2], di[2][2];
+    mf4 hr[2], hi[2], xr[2], xi[2], lr[2], li[2];
+    md4 dr[2], di[2];
     auto zero32 = [&]() {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                hr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
-                hi[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
-                xr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
-                xi[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
-                lr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
-                li[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
-            }
+        for (int i = 0; i < 2; ++i) {
+            hr[i] = mf4{0.f, 0.f, 0.f, 0.f};
+            hi[i] = mf4{0.f, 0.f, 0.f, 0.f};
+            xr[i] = mf4{0.f, 0.f, 0.f, 0.f};
+            xi[i] = mf4{0.f, 0.f, 0.f, 0.f};
+            lr[i] = mf4{0.f, 0.f, 0.f, 0.f};
+            li[i] = mf4{0.f, 0.f, 0.f, 0.f};
+        }
     };
     zero32();
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            dr[i][j] = md4{0.0, 0.0, 0.0, 0.0};
-            di[i][j] = md4{0.0, 0.0, 0.0, 0.0};
-        }
+    for (int i = 0; i < 2; ++i) {
+        dr[i] = md4{0.0, 0.0, 0.0, 0.0};
+        di[i] = md4{0.0, 0.0, 0.0, 0.0};
+    }
     const int64_t kbeg = (int64_t)kc * kch;
     const int64_t kend = std::min<int64_t>(Kc, kbeg + kch);
-    u4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
+    u4 ah, al, bh, bl;
     auto fetch = [&](int64_t k0) {
         const int64_t k = k0 + lk;
-        if (vec && k + 8 <= kend) {  // 16-byte aligned rows (p % 4 == 0)
+        if (vec && k + 4 <= kend) {  // 16-byte aligned rows (p % 4 == 0)
             const u4 z = {0u, 0u, 0u, 0u};
-            ah0 = va ? *(const u4 *)(Wh + oa + k) : z;
-            ah1 = va ? *(const u4 *)(Wh + oa + k + 4) : z;
-            al0 = va ? *(const u4 *)(Wl + oa + k) : z;
-            al1 = va ? *(const u4 *)(Wl + oa + k + 4) : z;
-            bh0 = vb ? *(const u4 *)(Wh + ob + k) : z;
-            bh1 = vb ? *(const u4 *)(Wh + ob + k + 4) : z;
-            bl0 = vb ? *(const u4 *)(Wl + ob + k) : z;
-            bl1 = vb ? *(const u4 *)(Wl + ob + k + 4) : z;
+            ah = va ? *(const u4 *)(Wh + oa + k) : z;
+            al = va ? *(const u4 *)(Wl + oa + k) : z;
+            bh = vb ? *(const u4 *)(Wh + ob + k) : z;
+            bl = vb ? *(const u4 *)(Wl + ob + k) : z;
         } else {
-            unsigned t[4][8];
+            unsigned t[4][4];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
+            for (int e = 0; e < 4; ++e) {
                 const bool kin = k + e < kend;
                 t[0][e] = (kin && va) ? Wh[oa + k + e] : 0u;
                 t[1][e] = (kin && va) ? Wl[oa + k + e] : 0u;
                 t[2][e] = (kin && vb) ? Wh[ob + k + e] : 0u;
                 t[3][e] = (kin && vb) ? Wl[ob + k + e] : 0u;
             }
-            ah0 = u4{t[0][0], t[0][1], t[0][2], t[0][3]};
-            ah1 = u4{t[0][4], t[0][5], t[0][6], t[0][7]};
-            al0 = u4{t[1][0], t[1][1], t[1][2], t[1][3]};
-            al1 = u4{t[1][4], t[1][5], t[1][6], t[1][7]};
-            bh0 = u4{t[2][0], t[2][1], t[2][2], t[2][3]};
-            bh1 = u4{t[2][4], t[2][5], t[2][6], t[2][7]};
-            bl0 = u4{t[3][0], t[3][1], t[3][2], t[3][3]};
-            bl1 = u4{t[3][4], t[3][5], t[3][6], t[3][7]};
+            ah = u4{t[0][0], t[0][1], t[0][2], t[0][3]};
+            al = u4{t[1][0], t[1][1], t[1][2], t[1][3]};
+            bh = u4{t[2][0], t[2][1], t[2][2], t[2][3]};
+            bl = u4{t[3][0], t[3][1], t[3][2], t[3][3]};
         }
     };
+    auto put = [&](int st) {
+        *(u4 *)&Sh[st][0][lc][lk] = ah;
+        *(u4 *)&Sl[st][0][lc][lk] = al;
+        *(u4 *)&Sh[st][1][lc][lk] = bh;
+        *(u4 *)&Sl[st][1][lc][lk] = bl;
+    };
     fetch(kbeg);
+    put(0);
+    if (kbeg + BKS < kend) fetch(kbeg + BKS);
+    __syncthreads();
+    int st = 0;
     for (int64_t k0 = kbeg; k0 < kend; k0 += BKS) {
-        *(u4 *)&Sh[0][lc][lk] = ah0;
-        *(u4 *)&Sh[0][lc][lk + 4] = ah1;
-        *(u4 *)&Sl[0][lc][lk] = al0;
-        *(u4 *)&Sl[0][lc][lk + 4] = al1;
-        *(u4 *)&Sh[1][lc][lk] = bh0;
-        *(u4 *)&Sh[1][lc][lk + 4] = bh1;
-        *(u4 *)&Sl[1][lc][lk] = bl0;
-        *(u4 *)&Sl[1][lc][lk + 4] = bl1;
-        __syncthreads();
-        if (k0 + BKS < kend) fetch(k0 + BKS);
+        // the next step's words into the other stage (its last readers passed the
+        // barrier that ended the previous step), then the step after it in flight
+        if (k0 + BKS < kend) {
+            put(st ^ 1);
+            if (k0 + 2 * BKS < kend) fetch(k0 + 2 * BKS);
+        }
 #pragma unroll
         for (int s = 0; s < BKS / 16; ++s) {
             const int kw = 16 * s + 4 * g4;
             u4 qh[2], ql[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                qh[i] = *(const u4 *)&Sh[0][wa + 16 * i + r16][kw];
-                ql[i] = *(const u4 *)&Sl[0][wa + 16 * i + r16][kw];
+                qh[i] = *(const u4 *)&Sh[st][0][wa + 16 * i + r16][kw];
+                ql[i] = *(const u4 *)&Sl[st][0][wa + 16 * i + r16][kw];
+            }
+            const u4 ph = *(const u4 *)&Sh[st][1][wb + r16][kw];
+            const u4 pl = *(const u4 *)&Sl[st][1][wb + r16][kw];
+            const u4 ph2 = w_swapneg(ph), pl2 = w_swapneg(pl);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                hr[i] = mfma_bf16(ph, qh[i], hr[i]);
+                xr[i] = mfma_bf16(pl, qh[i], xr[i]);
+                lr[i] = mfma_bf16(pl, ql[i], lr[i]);
+                hi[i] = mfma_bf16(ph2, qh[i], hi[i]);
+                xi[i] = mfma_bf16(pl2, qh[i], xi[i]);
+                li[i] = mfma_bf16(pl2, ql[i], li[i]);
             }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const u4 ph = *(const u4 *)&Sh[1][wb + 16 * j + r16][kw];
-                const u4 pl = *(const u4 *)&Sl[1][wb + 16 * j + r16][kw];
-                const u4 ph2 = w_swapneg(ph), pl2 = w_swapneg(pl);
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    hr[i][j] = mfma_bf16(ph, qh[i], hr[i][j]);
-                    xr[i][j] = mfma_bf16(pl, qh[i], xr[i][j]);
-                    xr[i][j] = mfma_bf16(ph, ql[i], xr[i][j]);
-                    lr[i][j] = mfma_bf16(pl, ql[i], lr[i][j]);
-                    hi[i][j] = mfma_bf16(ph2, qh[i], hi[i][j]);
-                    xi[i][j] = mfma_bf16(pl2, qh[i], xi[i][j]);
-                    xi[i][j] = mfma_bf16(ph2, ql[i], xi[i][j]);
-                    li[i][j] = mfma_bf16(pl2, ql[i], li[i][j]);
-                }
+            for (int i = 0; i < 2; ++i) {
+                xr[i] = mfma_bf16(ph, ql[i], xr[i]);
+                xi[i] = mfma_bf16(ph2, ql[i], xi[i]);
             }
         }
         // end of a SPL_ROWS-row block (the chunk starts on a block boundary) or of
@@ -1381,26 +1381,23 @@ __global__ __launch_bounds__(256) void k_gram_bf16(const unsigned *__restrict__ 
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        dr[i][j][v] += ((double)hr[i][j][v] + (double)xr[i][j][v]) + (double)lr[i][j][v];
-                        di[i][j][v] += ((double)hi[i][j][v] + (double)xi[i][j][v]) + (double)li[i][j][v];
-                    }
+                for (int v = 0; v < 4; ++v) {
+                    dr[i][v] += ((double)hr[i][v] + (double)xr[i][v]) + (double)lr[i][v];
+                    di[i][v] += ((double)hi[i][v] + (double)xi[i][v]) + (double)li[i][v];
+                }
             zero32();
         }
         __syncthreads();
+        st ^= 1;
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int a = a0 + wa + 16 * i + r16;
-                const int b = b0 + wb + 16 * j + 4 * g4 + v;
-                gram_store(part, G, ip, nkc, kc, m, a, b, dr[i][j][v], di[i][j][v], conj_out, err);
-            }
+        for (int v = 0; v < 4; ++v) {
+            const int a = a0 + wa + 16 * i + r16;
+            const int b = b0 + wb + 4 * g4 + v;
+            gram_store(part, G, ip, nkc, kc, m, a, b, dr[i][v], di[i][v], conj_out, err);
+        }
 }
 
 // Split-BF16 complex GEMM C = A B (the SVST apply), FP32 accumulation.  Same
@@ -1746,7 +1743,7 @@ static void gram_and_tridiag(mslr_ctx_s *ctx, const SvstWork &w, const c64 *P, i
             // one contraction chunk: the tiles write the Hermitian matrices directly
             c128 *Gd = w.nkc == 1 ? w.G : nullptr;
             if (Wh)  // split words, contraction-contiguous (k_split_rows / k_split_cols)
-                k_gram_bf16<<<gg, 256, 0, ctx->stream>>>(Wh, Wl, p * Nt, Kc, m, Kc, (int)w.nkc, w.kch,
+                k_gram_bf16<<<gg, GB_T, 0, ctx->stream>>>(Wh, Wl, p * Nt, Kc, m, Kc, (int)w.nkc, w.kch,
                                                          ntile, (Kc % 4) == 0 ? 1 : 0,
                                                          (c128 *)w.part, Gd, left ? 1 : 0, err_dev);
             else
