@@ -529,14 +529,16 @@ __device__ __forceinline__ double fast_rcp(double q) {
 
 // sturm_count with the division replaced by fast_rcp (the bisection steps;
 // k_eig_count keeps the IEEE division for the count that fixes r).
-__device__ int sturm_count_fast(const double *d, const double *e, int m, double x, double pivmin) {
+// e2 = e .* e.
+__device__ int sturm_count_fast(const double *__restrict__ d, const double *__restrict__ e2, int m,
+                                double x, double pivmin) {
     int c = 0;
     double q = d[0] - x;
     if (fabs(q) <= pivmin) q = -pivmin;
     c += q < 0.0;
+#pragma unroll 4
     for (int i = 1; i < m; ++i) {
-        const double el = e[i - 1];
-        q = (d[i] - x) - (el * el) * fast_rcp(q);
+        q = (d[i] - x) - e2[i - 1] * fast_rcp(q);
         if (fabs(q) <= pivmin) q = -pivmin;
         c += q < 0.0;
     }
@@ -610,13 +612,18 @@ __device__ void tri_solve(const double *d, const double *e, int m, double lam, d
 // z), dm = pivots of U D U^T (bottom-up, kept in dm with stride st), twist
 // index r = argmin |dp_r + dm_r - (d_r - lam)|, then z_r = 1 and
 // z_i = -(e_i/dp_i) z_{i+1} above r, z_{i+1} = -(e_i/dm_{i+1}) z_i below r.
-__device__ void twisted_vector(const double *d, const double *e, int m, double lam, double tiny,
-                               double *z, double *dm, int st) {
+// (z and dm do not alias d, e or each other: each value the recurrences carry stays
+// in a register, and the loads of the unrolled loops are issued ahead of their use;
+// e2 = e .* e)
+__device__ void twisted_vector(const double *__restrict__ d, const double *__restrict__ e,
+                               const double *__restrict__ e2, int m, double lam, double tiny,
+                               double *__restrict__ z, double *__restrict__ dm, int st) {
     double p = d[0] - lam;
     if (p == 0.0) p = tiny;
     z[0] = p;
+#pragma unroll 4
     for (int i = 0; i + 1 < m; ++i) {
-        p = (d[i + 1] - lam) - (e[i] * e[i]) * fast_rcp(p);
+        p = (d[i + 1] - lam) - e2[i] * fast_rcp(p);
         if (p == 0.0) p = tiny;
         z[i + 1] = p;
     }
@@ -625,11 +632,13 @@ __device__ void twisted_vector(const double *d, const double *e, int m, double l
     dm[(int64_t)(m - 1) * st] = q;
     int best = m - 1;
     double gbest = fabs(z[m - 1]);
+#pragma unroll 4
     for (int i = m - 2; i >= 0; --i) {
-        q = (d[i] - lam) - (e[i] * e[i]) * fast_rcp(q);
+        const double di = d[i] - lam;
+        q = di - e2[i] * fast_rcp(q);
         if (q == 0.0) q = tiny;
         dm[(int64_t)i * st] = q;
-        const double g = fabs(z[i] + q - (d[i] - lam));
+        const double g = fabs(z[i] + q - di);
         if (g < gbest) {
             gbest = g;
             best = i;
@@ -637,22 +646,30 @@ __device__ void twisted_vector(const double *d, const double *e, int m, double l
     }
     z[best] = 1.0;
     double amax = 1.0;
+    double zn = 1.0;  // z[i + 1]
+#pragma unroll 4
     for (int i = best - 1; i >= 0; --i) {
-        const double v = -(e[i] / z[i]) * z[i + 1];
+        const double v = -(e[i] / z[i]) * zn;
         z[i] = v;
+        zn = v;
         amax = fmax(amax, fabs(v));
     }
+    double zp = 1.0;  // z[i]
+#pragma unroll 4
     for (int i = best; i + 1 < m; ++i) {
-        const double v = -(e[i] / dm[(int64_t)(i + 1) * st]) * z[i];
+        const double v = -(e[i] / dm[(int64_t)(i + 1) * st]) * zp;
         z[i + 1] = v;
+        zp = v;
         amax = fmax(amax, fabs(v));
     }
     double nrm = 0.0;
+#pragma unroll 4
     for (int i = 0; i < m; ++i) {
         const double t = z[i] / amax;
         nrm += t * t;
     }
     const double inv = 1.0 / (amax * sqrt(nrm));
+#pragma unroll 4
     for (int i = 0; i < m; ++i) z[i] *= inv;
 }
 
@@ -733,12 +750,24 @@ __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ d
     static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "group size");
     const int ip = blockIdx.x;
     const int r = rall[ip];
+    if (k0 + (int)(blockIdx.y * 64) / G >= r) return;  // no wanted eigenvalue here
+    // the patch's d, e and e .* e in LDS: the Sturm counts and the twisted
+    // factorisation walk them sequentially, once per bisection step
+    extern __shared__ __align__(16) unsigned char eig_smem[];
+    double *d = reinterpret_cast<double *>(eig_smem);
+    double *e = d + m;
+    double *e2 = e + m;
+    for (int i = threadIdx.x; i < m; i += 64) {
+        d[i] = dall[(int64_t)ip * m + i];
+        const double ei = eall[(int64_t)ip * m + i];
+        e[i] = ei;
+        e2[i] = ei * ei;
+    }
+    __syncthreads();
     const int t = blockIdx.y * 64 + threadIdx.x;
     const int k = k0 + t / G, j = t % G;
     if (k >= r) return;  // whole groups leave together
     const int gbase = (threadIdx.x & 63) - j;
-    const double *d = dall + (int64_t)ip * m;
-    const double *e = eall + (int64_t)ip * m;
     const double *inf = info + (int64_t)ip * EIG_INFO;
     const double tnorm = inf[0], pivmin = inf[1], gu = inf[2];
     const int idx = m - 1 - k;  // ascending index: count(lo) <= idx < count(hi)
@@ -748,7 +777,7 @@ __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ d
         const double w = (hi - lo) / (double)(G + 1);
         if (!(lo + w > lo) || !(hi - w < hi)) break;
         const double xj = lo + (double)(j + 1) * w;
-        const bool above = sturm_count_fast(d, e, m, xj, pivmin) > idx;
+        const bool above = sturm_count_fast(d, e2, m, xj, pivmin) > idx;
         const unsigned long long bal = __ballot(above);
         const unsigned long long gm = (bal >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1));
         const int f = gm ? __builtin_ctzll(gm) : G;  // first point above the index
@@ -764,7 +793,7 @@ __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ d
     const double tiny = DBL_EPSILON * fmax(tnorm, DBL_MIN);
     double *z = Zall + (int64_t)ip * m * m + (int64_t)k * m;
     double *dm = reinterpret_cast<double *>(Uall + (int64_t)ip * m * m + (int64_t)k * m);
-    twisted_vector(d, e, m, lk, tiny, z, dm, 1);
+    twisted_vector(d, e, e2, m, lk, tiny, z, dm, 1);
 }
 
 // One workgroup of NSLOT threads per patch.  Runs of eigenvalues closer than
@@ -1912,11 +1941,11 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
                                                                    : ctx->tuning.eig_lanes_limit;
             if (nb * m <= lim) {
                 dim3 ge((unsigned)nb, (unsigned)cdiv(m * 16, 64));
-                k_eig_vectors<16><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
+                k_eig_vectors<16><<<ge, 64, 3 * m * sizeof(double), ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
                                                               w.info, w.r, w.lam, w.Z, w.U, 0);
             } else {
                 dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
-                k_eig_vectors<4><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
+                k_eig_vectors<4><<<ge, 64, 3 * m * sizeof(double), ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
                                                              w.info, w.r, w.lam, w.Z, w.U, 0);
             }
             HIP_LAUNCH_CHECK();
@@ -2066,7 +2095,7 @@ void svst_vjp_batched(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int
                                                                          per_patch, w.info, w.r, 1);
             HIP_LAUNCH_CHECK();
             dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
-            k_eig_vectors<4><<<ge, 64, 0, ctx->stream>>>(w.d, w.e, mi, bp, per_patch, w.info, w.r,
+            k_eig_vectors<4><<<ge, 64, 3 * m * sizeof(double), ctx->stream>>>(w.d, w.e, mi, bp, per_patch, w.info, w.r,
                                                          w.lam, w.Z, w.U, 0);
             HIP_LAUNCH_CHECK();
             k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, mi, bp, per_patch,

@@ -41,10 +41,26 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Sum over the wave (uniform result): row sums by DPP, then the four rows.
+// Row r's lane 15 (row_bcast:15) or lane 31 (row_bcast:31) into the rows of ROWMASK;
+// the other rows read 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double bcast_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+// Sum over the wave (uniform result): row sums by DPP, then rows 0 + 1 and 2 + 3 by
+// row_bcast:15 and the two halves by row_bcast:31, so that lane 63 holds the sum
+// (one readlane instead of four).
 __device__ __forceinline__ double wave_sum64(double v) {
     v = sum16(v);
-    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+    v += bcast_d<DPP_ROW_BCAST15, 0xA>(v);
+    v += bcast_d<DPP_ROW_BCAST31, 0xC>(v);
+    return readlane_d(v, 63);
 }
 __device__ __forceinline__ C2<double> wave_sum64(C2<double> v) {
     return {wave_sum64(v.re), wave_sum64(v.im)};
