@@ -123,6 +123,11 @@ mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
  *                    global memory): 0 = 12 waves starting sweeps 3 time steps apart
  *                    (built-in), 1 = 16 waves 2 apart, 2 = 6 waves 5 apart, 3 = 4 waves
  *                    8 apart.  Results are bit-identical.
+ *   apply_fp32:      with split-BF16 products, which SVST reconstruction products run
+ *                    in FP32 on the matrix cores instead: 0 = built-in, else 1 + a
+ *                    mask of bit 0 (the dense Y = X W), bit 1 (T = X U_r) and bit 2
+ *                    (Y = T f U_r^H); 1 = all split BF16, 8 = all FP32.  Changes
+ *                    rounding only; the FP32 products form ignores it.
  * Invalid values -> MSLR_ERR_ARG. */
 typedef struct mslr_tuning {
     int gram_chunk;
@@ -133,7 +138,8 @@ typedef struct mslr_tuning {
     int yz_threads_adj;
     int update_lines;
     int sb2st_schedule;
-    int reserved[4];
+    int apply_fp32;
+    int reserved[3];
 } mslr_tuning;
 mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t);
 mslr_status_t mslr_ctx_get_tuning(mslr_ctx ctx, mslr_tuning *t);

@@ -184,6 +184,8 @@ mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t) {
                      MSLR_ERR_ARG, "update_lines must be 0 or a multiple of 8 in [8, 64]");
         MSLR_REQUIRE(t->sb2st_schedule >= 0 && t->sb2st_schedule <= 3, MSLR_ERR_ARG,
                      "sb2st_schedule must be in [0, 3]");
+        MSLR_REQUIRE(t->apply_fp32 >= 0 && t->apply_fp32 <= 8, MSLR_ERR_ARG,
+                     "apply_fp32 must be in [0, 8]");
         for (int r : t->reserved) MSLR_REQUIRE(r == 0, MSLR_ERR_ARG, "reserved fields must be 0");
         c->tuning = *t;
     });

@@ -1867,6 +1867,10 @@ static void cgemm_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
     }
 }
 
+// Reconstruction products that the split-BF16 form runs in FP32 on the matrix
+// cores (bit 0 the dense X W, bit 1 X U_r, bit 2 T f U_r^H).
+constexpr int SVST_APPLY_FP32 = 0;
+
 // MSLR_SVST_APPLY=dense turns the low-rank apply off (every patch forms W).
 static bool svst_lowrank_apply() {
     static const bool on = [] {
@@ -1975,39 +1979,38 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         // W = (U diag f) U^H ; r < 0 (beta = 0) gives the identity, r = 0 gives 0
         cgemm(ctx, "form_w", w.U, m * m, 1, m, w.U, m * m, m, 1, 1, w.f, m, Wm, m * m, 1, m, mi, mi,
               0, w.r, nb, dense);
+        // split-BF16 or FP32 matrix-core products per reconstruction product
+        // (apply_fp32: 1 + mask of bit 0 dense, bit 1 X U_r, bit 2 T f U_r^H)
+        const int amask = ctx->tuning.apply_fp32 > 0 ? ctx->tuning.apply_fp32 - 1 : SVST_APPLY_FP32;
+        auto prod = [&](int bit, const char *tag, const c64 *A, int64_t bA, int64_t ai, int64_t ak,
+                        const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB,
+                        const float *fs, int64_t bfs, c64 *C, int64_t bC, int64_t ci, int64_t cj,
+                        int M, int N, int K, RankSel rs) {
+            if (split && !(amask & bit))
+                cgemm_bf16(ctx, tag, A, bA, ai, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC, ci, cj, M,
+                           N, K, nb, rs);
+            else
+                cgemm(ctx, tag, A, bA, ai, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC, ci, cj, M, N, K,
+                      nullptr, nb, rs);
+        };
         if (!left) {
             // Y (p x m) = X (p x m) W
-            if (split)
-                cgemm_bf16(ctx, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1,
-                           p, (int)p, mi, mi, nb, dense);
-            else
-                cgemm(ctx, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p,
-                      (int)p, mi, mi, nullptr, nb, dense);
+            prod(1, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p, (int)p,
+                 mi, mi, dense);
         } else {
             // Y (p x Nt) = W (p x p) X
-            if (split)
-                cgemm_bf16(ctx, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1,
-                           p, (int)p, (int)Nt, mi, nb);
-            else
-                cgemm(ctx, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1, p,
-                      (int)p, (int)Nt, mi, nullptr, nb);
+            prod(1, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1, p, (int)p,
+                 (int)Nt, mi, RankSel{});
         }
         if (lowrank) {
             // T (p x r) = X U_r, then Y = (T diag f) U_r^H; T holds at most m/2 columns
             const int64_t bT = p * half;
             c64 *T = (c64 *)ctx->scratch_get(SLOT_LOWRANK, (size_t)(bT * nb) * sizeof(c64));
             const RankSel s1{w.r, mi, 2}, s2{w.r, mi, 3};
-            if (split) {
-                cgemm_bf16(ctx, "apply_xu", Pc, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1,
-                           p, (int)p, half, mi, nb, s1);
-                cgemm_bf16(ctx, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p,
-                           (int)p, mi, half, nb, s2);
-            } else {
-                cgemm(ctx, "apply_xu", Pc, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1, p,
-                      (int)p, half, mi, nullptr, nb, s1);
-                cgemm(ctx, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p,
-                      (int)p, mi, half, nullptr, nb, s2);
-            }
+            prod(2, "apply_xu", Pc, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1, p, (int)p,
+                 half, mi, s1);
+            prod(4, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p, (int)p, mi,
+                 half, s2);
         }
     }
 }

@@ -73,12 +73,14 @@ struct Tuning
     yz_threads_adj::Cint
     update_lines::Cint
     sb2st_schedule::Cint
-    reserved::NTuple{4,Cint}
+    apply_fp32::Cint
+    reserved::NTuple{3,Cint}
 end
 
 """
     set_tuning!(ctx; gram_chunk=0, eig_lanes_limit=0, svst_apply=0, sense_chunks=0,
-                yz_threads_fwd=0, yz_threads_adj=0, update_lines=0, sb2st_schedule=0)
+                yz_threads_fwd=0, yz_threads_adj=0, update_lines=0, sb2st_schedule=0,
+                apply_fp32=0)
 
 Launch shapes on `ctx` (0 keeps the built-in choice), see `mslr_tuning` in
 include/mslr.h.  The Python autotuner (`mslr_hip.autotune.autotune_pogm`) picks them
@@ -86,21 +88,23 @@ from timed POGM iterations; its JSON cache holds the values to pass here.
 """
 function set_tuning!(ctx::HipContext; gram_chunk::Integer = 0, eig_lanes_limit::Integer = 0,
                      svst_apply::Integer = 0, sense_chunks::Integer = 0, yz_threads_fwd::Integer = 0,
-                     yz_threads_adj::Integer = 0, update_lines::Integer = 0, sb2st_schedule::Integer = 0)
+                     yz_threads_adj::Integer = 0, update_lines::Integer = 0, sb2st_schedule::Integer = 0,
+                     apply_fp32::Integer = 0)
     t = Ref(Tuning(gram_chunk, eig_lanes_limit, svst_apply, sense_chunks, yz_threads_fwd, yz_threads_adj,
-                   update_lines, sb2st_schedule, ntuple(_ -> Cint(0), 4)))
+                   update_lines, sb2st_schedule, apply_fp32, ntuple(_ -> Cint(0), 3)))
     check(ccall((:mslr_ctx_set_tuning, LIB), Cint, (Ptr{Cvoid}, Ref{Tuning}), ctx.h, t))
     return ctx
 end
 
 function get_tuning(ctx::HipContext)
-    t = Ref(Tuning(0, 0, 0, 0, 0, 0, 0, 0, ntuple(_ -> Cint(0), 4)))
+    t = Ref(Tuning(0, 0, 0, 0, 0, 0, 0, 0, 0, ntuple(_ -> Cint(0), 3)))
     check(ccall((:mslr_ctx_get_tuning, LIB), Cint, (Ptr{Cvoid}, Ref{Tuning}), ctx.h, t))
     v = t[]
     return (gram_chunk = Int(v.gram_chunk), eig_lanes_limit = Int(v.eig_lanes_limit),
             svst_apply = Int(v.svst_apply), sense_chunks = Int(v.sense_chunks),
             yz_threads_fwd = Int(v.yz_threads_fwd), yz_threads_adj = Int(v.yz_threads_adj),
-            update_lines = Int(v.update_lines), sb2st_schedule = Int(v.sb2st_schedule))
+            update_lines = Int(v.update_lines), sb2st_schedule = Int(v.sb2st_schedule),
+            apply_fp32 = Int(v.apply_fp32))
 end
 
 "Column-major device array owned by the library (ComplexF32 unless stated)."

@@ -91,7 +91,8 @@ class mslr_tuning(C.Structure):
         ("yz_threads_adj", C.c_int),
         ("update_lines", C.c_int),
         ("sb2st_schedule", C.c_int),
-        ("reserved", C.c_int * 4),
+        ("apply_fp32", C.c_int),
+        ("reserved", C.c_int * 3),
     ]
 
 

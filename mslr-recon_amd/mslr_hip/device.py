@@ -45,7 +45,7 @@ class Context:
         _lib.check(self._lib.mslr_ctx_set_products(self.handle, code))
 
     TUNING_FIELDS = ("gram_chunk", "eig_lanes_limit", "svst_apply", "sense_chunks", "yz_threads_fwd",
-                     "yz_threads_adj", "update_lines", "sb2st_schedule")
+                     "yz_threads_adj", "update_lines", "sb2st_schedule", "apply_fp32")
 
     def set_tuning(self, **kw) -> None:
         """Launch shapes (mslr_ctx_set_tuning; an omitted or 0 field = built-in):
@@ -55,7 +55,9 @@ class Context:
         64), yz_threads_fwd / yz_threads_adj (512 or 1024 threads of the compiled y-z FFT
         passes), update_lines (yz-lines per workgroup of the fused transpose + update, a
         multiple of 8 in [8, 64]), sb2st_schedule (0-3: the
-        global-band tridiagonalisation's wave schedule)."""
+        global-band tridiagonalisation's wave schedule), apply_fp32 (split-BF16 form:
+        1 + a mask of the SVST reconstruction products run in FP32 instead, bit 0 the
+        dense X W, bit 1 X U_r, bit 2 T f U_r^H)."""
         bad = set(kw) - set(self.TUNING_FIELDS)
         if bad:
             raise TypeError(f"unknown tuning fields {sorted(bad)}")

@@ -14,7 +14,8 @@ from test_gpu_svst import APPLY_RANKS, _beta_between, lowrank, mixed_rank_batch
 pytestmark = pytest.mark.gpu
 
 BUILTIN = {"gram_chunk": 0, "eig_lanes_limit": 0, "svst_apply": 0, "sense_chunks": 0,
-           "yz_threads_fwd": 0, "yz_threads_adj": 0, "update_lines": 0, "sb2st_schedule": 0}
+           "yz_threads_fwd": 0, "yz_threads_adj": 0, "update_lines": 0, "sb2st_schedule": 0,
+           "apply_fp32": 0}
 
 
 @pytest.fixture
@@ -28,13 +29,15 @@ def shapes(ctx):
 def test_tuning_roundtrip_and_validation(shapes):
     assert shapes.tuning() == BUILTIN
     want = {"gram_chunk": 2048, "eig_lanes_limit": -1, "svst_apply": 1, "sense_chunks": 4,
-            "yz_threads_fwd": 512, "yz_threads_adj": 1024, "update_lines": 32, "sb2st_schedule": 2}
+            "yz_threads_fwd": 512, "yz_threads_adj": 1024, "update_lines": 32, "sb2st_schedule": 2,
+            "apply_fp32": 5}
     shapes.set_tuning(**want)
     assert shapes.tuning() == want
     for bad in [dict(gram_chunk=100), dict(gram_chunk=128), dict(gram_chunk=1 << 17),
                 dict(gram_chunk=-256), dict(eig_lanes_limit=-2), dict(svst_apply=2),
                 dict(sense_chunks=-1), dict(sense_chunks=65), dict(yz_threads_fwd=256),
-                dict(yz_threads_adj=768), dict(update_lines=100), dict(update_lines=12), dict(sb2st_schedule=4)]:
+                dict(yz_threads_adj=768), dict(update_lines=100), dict(update_lines=12), dict(sb2st_schedule=4),
+                dict(apply_fp32=9), dict(apply_fp32=-1)]:
         with pytest.raises(ValueError):
             shapes.set_tuning(**bad)
         assert shapes.tuning() == want
@@ -77,6 +80,29 @@ def test_eig_lanes_and_apply(shapes, rng, tuning):
     for i, k in enumerate(APPLY_RANKS):
         Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
         assert rel_err(Y1[:, :, i], Yo) < 3e-5, (i, k)
+
+
+@pytest.mark.parametrize("amask", [1, 2, 5, 8])
+def test_apply_fp32(shapes, rng, amask):
+    """Split-BF16 products with the reconstruction products in either arithmetic
+    (1: all split BF16, 8: all FP32, 2 and 5 mixed), on ranks either side of the
+    low-rank boundary and on a wide patch (p < Nt, the W X form)."""
+    from mslr_hip import recon
+    shapes.set_products("split_bf16")
+    P, betas = mixed_rank_batch(rng, APPLY_RANKS)
+    X = lowrank(rng, 90, 300, 5)
+    beta = _beta_between(X, 5)
+    Y0 = recon.svst_batched(P, betas).to_host()
+    W0 = recon.SVST(X, beta).to_host()
+    shapes.set_tuning(apply_fp32=amask)
+    Y1 = recon.svst_batched(P, betas).to_host()
+    W1 = recon.SVST(X, beta).to_host()
+    assert rel_err(Y1, Y0) < 3e-5
+    assert rel_err(W1, W0) < 3e-5
+    for i, k in enumerate(APPLY_RANKS):
+        Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
+        assert rel_err(Y1[:, :, i], Yo) < 1e-4, (i, k)
+    assert rel_err(W1, O.SVST(X.astype(np.complex128), np.float32(beta))) < 1e-4
 
 
 def test_autotune_pogm(shapes, tmp_path):
