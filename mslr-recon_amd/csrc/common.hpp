@@ -193,7 +193,8 @@ enum ScratchSlot : size_t {
     SLOT_VJP_P = 14,    // patch cotangents of the prox derivative
     SLOT_SETUP = 15,    // run_recon setup: chunk counts, magnitudes, selection state
     SLOT_VJP_T = 16,    // conjugate-transposed patches of the SVST derivative (p < Nt)
-    SLOT_COUNT = 17
+    SLOT_BSPLIT = 17,   // right operand of the split-BF16 apply as hi / lo words
+    SLOT_COUNT = 18
 };
 
 }  // namespace mslr

@@ -1537,6 +1537,160 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
             }
 }
 
+// ---- split-BF16 reconstruction products C = A B' with row-contiguous A (the
+// p >= Nt applies X W, X U_r and T f U_r^H).  The small right operand is split
+// once per launch (k_apply_bsplit) into hi / lo words, B'(k, j) = fs[k] B(k, j)
+// (conjugated when conjB), stored per batch entry as [column j][k], k padded to a
+// multiple of BKS with zeros; k_apply_bf16 then stages it into LDS with plain
+// 16-byte copies and splits only A.  A workgroup takes a block of 64 rows of A and
+// all the column tiles of 32 NB; four waves of 32 x 16 NB.  The grid strides over
+// the (batch entry, row block) units, so entries that the rank selection skips
+// cost no workgroup dispatch.
+constexpr int AP_ROWS = 64;  // rows of A per unit
+
+__global__ __launch_bounds__(256) void k_apply_bsplit(const c64 *__restrict__ B, int64_t bB,
+                                                      int64_t bk, int64_t bj, int conjB,
+                                                      const float *__restrict__ fs, int64_t bfs,
+                                                      int N, int K, int Kp, RankSel rs,
+                                                      unsigned *__restrict__ Wh,
+                                                      unsigned *__restrict__ Wl, int64_t bW) {
+    const int z = blockIdx.y;
+    int Nz = N, Kz = K;
+    if (!rank_select(rs, z, Nz, Kz)) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)Nz * Kp) return;
+    const int j = (int)(e / Kp), k = (int)(e - (int64_t)j * Kp);
+    c64 v{0.f, 0.f};
+    if (k < Kz) {
+        v = B[(int64_t)z * bB + (int64_t)k * bk + (int64_t)j * bj];
+        if (conjB) v.im = -v.im;
+        if (fs) {
+            const float f = fs[(int64_t)z * bfs + k];
+            v = c64{v.re * f, v.im * f};
+        }
+    }
+    unsigned h, l;
+    split_c64(v, h, l);
+    Wh[(int64_t)z * bW + e] = h;
+    Wl[(int64_t)z * bW + e] = l;
+}
+
+template <int NB>
+__global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A, int64_t bA,
+                                                       int64_t ak, const unsigned *__restrict__ Wh,
+                                                       const unsigned *__restrict__ Wl, int64_t bW,
+                                                       int Kp, c64 *__restrict__ C, int64_t bC,
+                                                       int64_t cj, int M, int N, int K, int nb,
+                                                       RankSel rs) {
+    constexpr int TN = 32 * NB;               // columns per tile
+    constexpr int BQ = TN * BKS / 4 / 256;    // 16-byte B words per thread and part
+    static_assert(TN * BKS / 4 % 256 == 0, "B tile must split evenly over the threads");
+    __shared__ __align__(16) unsigned Ah[AP_ROWS][BLD], Al[AP_ROWS][BLD];
+    __shared__ __align__(16) unsigned Bh[TN][BLD], Bl[TN][BLD];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * (16 * NB);
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const int ar = tid & 63, akq = (tid >> 6) * 8;  // A slot: one row, eight consecutive k
+    const int nrt = (M + AP_ROWS - 1) / AP_ROWS;
+    const int64_t nunits = (int64_t)nb * nrt;
+    for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int z = (int)(u / nrt);
+        const int i0 = (int)(u - (int64_t)z * nrt) * AP_ROWS;
+        int Nz = N, Kz = K;
+        if (!rank_select(rs, z, Nz, Kz)) continue;  // uniform per workgroup
+        const bool av = i0 + ar < M;
+        const c64 *pa = A + (int64_t)z * bA + i0 + ar;
+        const unsigned *wh = Wh + (int64_t)z * bW, *wl = Wl + (int64_t)z * bW;
+        c64 *Cz = C + (int64_t)z * bC;
+        // one k stage: the A tile stays in LDS for all the column tiles
+        const bool one = Kz <= BKS;
+        c64 ra[8];
+        auto fetch = [&](int k0) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + akq + e;
+                ra[e] = (av && k < Kz) ? pa[(int64_t)k * ak] : c64{0.f, 0.f};
+            }
+        };
+        for (int j0 = 0; j0 < Nz; j0 += TN) {
+            mf4 accr[2][NB], acci[2][NB];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    accr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                    acci[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                }
+            u4 rbh[BQ], rbl[BQ];
+            auto fetch_b = [&](int k0) {
+#pragma unroll
+                for (int q = 0; q < BQ; ++q) {
+                    const int f = tid + 256 * q;
+                    const int jj = f / (BKS / 4), kq = (f % (BKS / 4)) * 4;
+                    const bool ok = j0 + jj < Nz;
+                    const int64_t o = (int64_t)(ok ? j0 + jj : 0) * Kp + k0 + kq;
+                    const u4 h = *(const u4 *)(wh + o), l = *(const u4 *)(wl + o);
+                    rbh[q] = ok ? h : u4{0u, 0u, 0u, 0u};
+                    rbl[q] = ok ? l : u4{0u, 0u, 0u, 0u};
+                }
+            };
+            const bool sa = !one || j0 == 0;
+            if (sa) fetch(0);
+            for (int k0 = 0; k0 < Kz; k0 += BKS) {
+                fetch_b(k0);
+                if (sa) stage8(ra, &Ah[ar][akq], &Al[ar][akq]);
+#pragma unroll
+                for (int q = 0; q < BQ; ++q) {
+                    const int f = tid + 256 * q;
+                    const int jj = f / (BKS / 4), kq = (f % (BKS / 4)) * 4;
+                    *(u4 *)&Bh[jj][kq] = rbh[q];
+                    *(u4 *)&Bl[jj][kq] = rbl[q];
+                }
+                __syncthreads();
+                if (k0 + BKS < Kz) fetch(k0 + BKS);
+#pragma unroll
+                for (int s = 0; s < BKS / 16; ++s) {
+                    const int kw = 16 * s + 4 * g4;
+                    u4 ch[2], cl[2], sh[2], sl[2];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        const u4 qh = *(const u4 *)&Ah[wi + 16 * i + r16][kw];
+                        const u4 ql = *(const u4 *)&Al[wi + 16 * i + r16][kw];
+                        ch[i] = w_conj(qh);
+                        cl[i] = w_conj(ql);
+                        sh[i] = w_swap(qh);
+                        sl[i] = w_swap(ql);
+                    }
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        if (j0 + wj + 16 * j >= Nz) break;  // uniform per wave
+                        const u4 ph = *(const u4 *)&Bh[wj + 16 * j + r16][kw];
+                        const u4 pl = *(const u4 *)&Bl[wj + 16 * j + r16][kw];
+#pragma unroll
+                        for (int i = 0; i < 2; ++i) {
+                            accr[i][j] = mfma_split(ph, pl, ch[i], cl[i], accr[i][j]);
+                            acci[i][j] = mfma_split(ph, pl, sh[i], sl[i], acci[i][j]);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int ii = i0 + wi + 16 * i + r16;
+                        const int jj = j0 + wj + 16 * j + 4 * g4 + v;
+                        if (ii < M && jj < Nz)
+                            Cz[ii + (int64_t)jj * cj] = c64{accr[i][j][v], acci[i][j][v]};
+                    }
+        }
+    }
+}
+
 // m == 1: each matrix is a vector v (len = p*Nt); SVST = v * max(|v| - beta, 0)/|v|.
 // nuc (nullable): the output's singular value max(|v| - beta, 0) per matrix.
 __global__ __launch_bounds__(256) void k_svst_m1(c64 *__restrict__ P, int64_t len, int64_t Np,
@@ -1867,6 +2021,44 @@ static void cgemm_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
     }
 }
 
+// C = A B' in split BF16 for row-contiguous A and C (ai = ci = 1): B' = (fs[k] B(k, j),
+// conjugated when conjB) split once into hi / lo words, then k_apply_bf16.
+static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, int64_t ak,
+                       const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB, const float *fs,
+                       int64_t bfs, c64 *C, int64_t bC, int64_t cj, int M, int N, int K,
+                       int64_t nbatch, RankSel rs) {
+    if (M <= 0 || N <= 0 || K <= 0 || nbatch <= 0) return;
+    const int Kp = (int)cdiv(K, BKS) * BKS;
+    const int64_t bW = (int64_t)N * Kp;
+    const int64_t zc = std::min<int64_t>(nbatch, 65535);
+    unsigned *Wh = (unsigned *)ctx->scratch_get(SLOT_BSPLIT, (size_t)(2 * zc * bW) * sizeof(unsigned));
+    unsigned *Wl = Wh + zc * bW;
+    MSLR_PROF(ctx, tag);
+    for (int64_t z0 = 0; z0 < nbatch; z0 += zc) {
+        const int nz = (int)std::min<int64_t>(zc, nbatch - z0);
+        const RankSel rz = rank_sel_at(rs, z0);
+        dim3 gs((unsigned)cdiv(bW, 256), (unsigned)nz);
+        k_apply_bsplit<<<gs, 256, 0, ctx->stream>>>(B + z0 * bB, bB, bk, bj, conjB,
+                                                    fs ? fs + z0 * bfs : nullptr, bfs, N, K, Kp, rz,
+                                                    Wh, Wl, bW);
+        HIP_LAUNCH_CHECK();
+        const int64_t units = (int64_t)nz * cdiv(M, AP_ROWS);
+        const unsigned g = (unsigned)std::min<int64_t>(units, 4 * (int64_t)std::max(ctx->num_cus, 1));
+        const c64 *Az = A + z0 * bA;
+        c64 *Cz = C + z0 * bC;
+        if (N > 64)
+            k_apply_bf16<5><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
+                                                        N, K, nz, rz);
+        else if (N > 32)
+            k_apply_bf16<2><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
+                                                        N, K, nz, rz);
+        else
+            k_apply_bf16<1><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
+                                                        N, K, nz, rz);
+        HIP_LAUNCH_CHECK();
+    }
+}
+
 // Reconstruction products that the split-BF16 form runs in FP32 on the matrix
 // cores (bit 0 the dense X W, bit 1 X U_r, bit 2 T f U_r^H).
 constexpr int SVST_APPLY_FP32 = 0;
@@ -1980,13 +2172,19 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         cgemm(ctx, "form_w", w.U, m * m, 1, m, w.U, m * m, m, 1, 1, w.f, m, Wm, m * m, 1, m, mi, mi,
               0, w.r, nb, dense);
         // split-BF16 or FP32 matrix-core products per reconstruction product
-        // (apply_fp32: 1 + mask of bit 0 dense, bit 1 X U_r, bit 2 T f U_r^H)
+        // (apply_fp32: 1 + mask of bit 0 dense, bit 1 X U_r, bit 2 T f U_r^H).  With
+        // p >= Nt the right operands (W, U_r, f U_r^H) are m x m at most and are split
+        // once (apply_bf16); W X (p < Nt) would split the patches themselves, so it
+        // splits per tile (cgemm_bf16).
         const int amask = ctx->tuning.apply_fp32 > 0 ? ctx->tuning.apply_fp32 - 1 : SVST_APPLY_FP32;
         auto prod = [&](int bit, const char *tag, const c64 *A, int64_t bA, int64_t ai, int64_t ak,
                         const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB,
                         const float *fs, int64_t bfs, c64 *C, int64_t bC, int64_t ci, int64_t cj,
                         int M, int N, int K, RankSel rs) {
-            if (split && !(amask & bit))
+            if (split && !(amask & bit) && !left)
+                apply_bf16(ctx, tag, A, bA, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC, cj, M, N, K, nb,
+                           rs);
+            else if (split && !(amask & bit))
                 cgemm_bf16(ctx, tag, A, bA, ai, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC, ci, cj, M,
                            N, K, nb, rs);
             else

@@ -23,7 +23,8 @@ def split(ctx):
 
 
 SHAPES = [(40, 12, 3), (12, 40, 3), (1000, 30, 4), (30, 200, 2), (216, 387, 5), (300, 300, 6),
-          (1000, 300, 20), (5000, 64, 3), (64, 64, 1), (2, 3, 1)]
+          (1000, 300, 20), (5000, 64, 3), (64, 64, 1), (2, 3, 1), (130, 50, 20), (2000, 387, 40),
+          (700, 387, 300)]
 
 
 @pytest.mark.parametrize("p,n,r", SHAPES)
