@@ -194,7 +194,8 @@ enum ScratchSlot : size_t {
     SLOT_SETUP = 15,    // run_recon setup: chunk counts, magnitudes, selection state
     SLOT_VJP_T = 16,    // conjugate-transposed patches of the SVST derivative (p < Nt)
     SLOT_BSPLIT = 17,   // right operand of the split-BF16 apply as hi / lo words
-    SLOT_COUNT = 18
+    SLOT_SVST_COPY = 18,// FP32 copy of an SVST input that is overwritten in place
+    SLOT_COUNT = 19
 };
 
 }  // namespace mslr

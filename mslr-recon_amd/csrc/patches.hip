@@ -72,20 +72,6 @@ __global__ void k_img2patches(const c64 *__restrict__ img, c64 *__restrict__ P, 
     }
 }
 
-// Image offset (frame 0) of element i of patch ip.
-__device__ __forceinline__ int64_t patch_voxel(const PatchGeom &g, int64_t ip, int i) {
-    const int ix = (int)(ip % g.ns[0]);
-    const int iy = (int)((ip / g.ns[0]) % g.ns[1]);
-    const int iz = (int)(ip / ((int64_t)g.ns[0] * g.ns[1]));
-    const int dx = i % g.ps[0];
-    const int dy = (i / g.ps[0]) % g.ps[1];
-    const int dz = i / (g.ps[0] * g.ps[1]);
-    const int x = g.start(0, ix) + dx;
-    const int y = g.start(1, iy) + dy;
-    const int z = g.start(2, iz) + dz;
-    return x + (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z);
-}
-
 // One thread per (patch element i, patch), looping over the frames: the index
 // arithmetic once per thread, then strided copies (P rows coalesced, image
 // reads in runs of ps[0] voxels).

@@ -28,6 +28,20 @@ struct PatchGeom {
 
 PatchGeom make_geom(const int64_t N[3], int64_t Nt, const int64_t ps[3], const int64_t ss[3]);
 
+// Image offset (frame 0) of element i of patch ip.
+__device__ __forceinline__ int64_t patch_voxel(const PatchGeom &g, int64_t ip, int i) {
+    const int ix = (int)(ip % g.ns[0]);
+    const int iy = (int)((ip / g.ns[0]) % g.ns[1]);
+    const int iz = (int)(ip / ((int64_t)g.ns[0] * g.ns[1]));
+    const int dx = i % g.ps[0];
+    const int dy = (i / g.ps[0]) % g.ps[1];
+    const int dz = i / (g.ps[0] * g.ps[1]);
+    const int x = g.start(0, ix) + dx;
+    const int y = g.start(1, iy) + dy;
+    const int z = g.start(2, iz) + dz;
+    return x + (int64_t)g.N[0] * (y + (int64_t)g.N[1] * z);
+}
+
 // P (p, Nt, Np) <- img (Nx,Ny,Nz,Nt), patches [ip0, ip0+np) only (P holds those np patches).
 void launch_img2patches(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *P,
                         int64_t ip0, int64_t np);

@@ -1225,9 +1225,11 @@ __device__ __forceinline__ void split_block(c64 v0, c64 v1, int lane, int n, uns
     }
 }
 
+// Xc (nullable): also an FP32 copy of X (for products that overwrite X).
 __global__ __launch_bounds__(256) void k_split_rows(const c64 *__restrict__ X, int64_t p,
                                                     int64_t ncol, unsigned *__restrict__ Wh,
-                                                    unsigned *__restrict__ Wl) {
+                                                    unsigned *__restrict__ Wl,
+                                                    c64 *__restrict__ Xc) {
     const int64_t nblk = (p + SPL_ROWS - 1) / SPL_ROWS;
     const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wid >= nblk * ncol) return;
@@ -1237,6 +1239,10 @@ __global__ __launch_bounds__(256) void k_split_rows(const c64 *__restrict__ X, i
     const int n = (int)std::min<int64_t>(SPL_ROWS, p - blk * SPL_ROWS);
     const c64 v0 = lane < n ? X[off + lane] : c64{0.f, 0.f};
     const c64 v1 = lane + 64 < n ? X[off + lane + 64] : c64{0.f, 0.f};
+    if (Xc) {
+        if (lane < n) Xc[off + lane] = v0;
+        if (lane + 64 < n) Xc[off + lane + 64] = v1;
+    }
     split_block(v0, v1, lane, n, Wh + off, Wl + off);
 }
 
@@ -1541,11 +1547,12 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16(const c64 *__restrict__ A, i
 // p >= Nt applies X W, X U_r and T f U_r^H).  The small right operand is split
 // once per launch (k_apply_bsplit) into hi / lo words, B'(k, j) = fs[k] B(k, j)
 // (conjugated when conjB), stored per batch entry as [column j][k], k padded to a
-// multiple of BKS with zeros; k_apply_bf16 then stages it into LDS with plain
-// 16-byte copies and splits only A.  A workgroup takes a block of 64 rows of A and
-// all the column tiles of 32 NB; four waves of 32 x 16 NB.  The grid strides over
-// the (batch entry, row block) units, so entries that the rank selection skips
-// cost no workgroup dispatch.
+// multiple of BKS with zeros (the stages up to the selected K only);
+// k_apply_bf16 then stages it into LDS with plain 16-byte copies and splits only
+// A.  A workgroup takes a block of 64 rows of A and all the column tiles of
+// 32 NB; four waves of 32 x 16 NB.  The grid strides over the (batch entry, row
+// block) units, so entries that the rank selection skips cost no workgroup
+// dispatch.
 constexpr int AP_ROWS = 64;  // rows of A per unit
 
 __global__ __launch_bounds__(256) void k_apply_bsplit(const c64 *__restrict__ B, int64_t bB,
@@ -1560,6 +1567,7 @@ __global__ __launch_bounds__(256) void k_apply_bsplit(const c64 *__restrict__ B,
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)Nz * Kp) return;
     const int j = (int)(e / Kp), k = (int)(e - (int64_t)j * Kp);
+    if (k >= (Kz + BKS - 1) / BKS * BKS) return;  // stages k_apply_bf16 never reads
     c64 v{0.f, 0.f};
     if (k < Kz) {
         v = B[(int64_t)z * bB + (int64_t)k * bk + (int64_t)j * bj];
@@ -1581,7 +1589,8 @@ __global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A
                                                        const unsigned *__restrict__ Wl, int64_t bW,
                                                        int Kp, c64 *__restrict__ C, int64_t bC,
                                                        int64_t cj, int M, int N, int K, int nb,
-                                                       RankSel rs) {
+                                                       RankSel rs, PatchGeom sg, int64_t sip0,
+                                                       int scat) {
     constexpr int TN = 32 * NB;               // columns per tile
     constexpr int BQ = TN * BKS / 4 / 256;    // 16-byte B words per thread and part
     static_assert(TN * BKS / 4 % 256 == 0, "B tile must split evenly over the threads");
@@ -1602,7 +1611,7 @@ __global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A
         const bool av = i0 + ar < M;
         const c64 *pa = A + (int64_t)z * bA + i0 + ar;
         const unsigned *wh = Wh + (int64_t)z * bW, *wl = Wl + (int64_t)z * bW;
-        c64 *Cz = C + (int64_t)z * bC;
+        c64 *Cz = scat ? C : C + (int64_t)z * bC;
         // one k stage: the A tile stays in LDS for all the column tiles
         const bool one = Kz <= BKS;
         c64 ra[8];
@@ -1677,16 +1686,19 @@ __global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A
                 __syncthreads();
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 2; ++i) {
+                const int ii = i0 + wi + 16 * i + r16;
+                if (ii >= M) continue;
+                // row ii of the patch: its voxel when scattering into the image
+                const int64_t ro = scat ? patch_voxel(sg, sip0 + z, ii) : ii;
 #pragma unroll
                 for (int j = 0; j < NB; ++j)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int ii = i0 + wi + 16 * i + r16;
                         const int jj = j0 + wj + 16 * j + 4 * g4 + v;
-                        if (ii < M && jj < Nz)
-                            Cz[ii + (int64_t)jj * cj] = c64{accr[i][j][v], acci[i][j][v]};
+                        if (jj < Nz) Cz[ro + (int64_t)jj * cj] = c64{accr[i][j][v], acci[i][j][v]};
                     }
+            }
         }
     }
 }
@@ -2023,10 +2035,13 @@ static void cgemm_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
 
 // C = A B' in split BF16 for row-contiguous A and C (ai = ci = 1): B' = (fs[k] B(k, j),
 // conjugated when conjB) split once into hi / lo words, then k_apply_bf16.
+// scatter (nullable): C is the image of an exact tiling's patches, batch entry z
+// being patch sip0 + z (cj = V, the frame stride).
 static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, int64_t ak,
                        const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB, const float *fs,
                        int64_t bfs, c64 *C, int64_t bC, int64_t cj, int M, int N, int K,
-                       int64_t nbatch, RankSel rs) {
+                       int64_t nbatch, RankSel rs, const PatchGeom *scatter = nullptr,
+                       int64_t sip0 = 0) {
     if (M <= 0 || N <= 0 || K <= 0 || nbatch <= 0) return;
     const int Kp = (int)cdiv(K, BKS) * BKS;
     const int64_t bW = (int64_t)N * Kp;
@@ -2045,16 +2060,19 @@ static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
         const int64_t units = (int64_t)nz * cdiv(M, AP_ROWS);
         const unsigned g = (unsigned)std::min<int64_t>(units, 4 * (int64_t)std::max(ctx->num_cus, 1));
         const c64 *Az = A + z0 * bA;
-        c64 *Cz = C + z0 * bC;
+        c64 *Cz = scatter ? C : C + z0 * bC;
+        const PatchGeom sg = scatter ? *scatter : PatchGeom{};
+        const int64_t s0 = sip0 + z0;
+        const int sc = scatter ? 1 : 0;
         if (N > 64)
             k_apply_bf16<5><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
-                                                        N, K, nz, rz);
+                                                        N, K, nz, rz, sg, s0, sc);
         else if (N > 32)
             k_apply_bf16<2><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
-                                                        N, K, nz, rz);
+                                                        N, K, nz, rz, sg, s0, sc);
         else
             k_apply_bf16<1><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
-                                                        N, K, nz, rz);
+                                                        N, K, nz, rz, sg, s0, sc);
         HIP_LAUNCH_CHECK();
     }
 }
@@ -2072,13 +2090,29 @@ static bool svst_lowrank_apply() {
     return on;
 }
 
+// The products writing the output (bits 0 and 2 of apply_fp32) stay split BF16.
+static int apply_fp32_mask(const mslr_ctx_s *ctx) {
+    return ctx->tuning.apply_fp32 > 0 ? ctx->tuning.apply_fp32 - 1 : SVST_APPLY_FP32;
+}
+
+bool svst_writes_direct(const mslr_ctx_s *ctx, int64_t p, int64_t Nt) {
+    return ctx->products == MSLR_PRODUCTS_SPLIT_BF16 && p >= Nt && Nt > 1 &&
+           (apply_fp32_mask(ctx) & 5) == 0;
+}
+
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
-                  const double *beta_dev, int per_patch, int *err_dev, double *nuc_out) {
+                  const double *beta_dev, int per_patch, int *err_dev, double *nuc_out,
+                  const PatchGeom *scatter) {
     if (Np == 0 || p == 0 || Nt == 0) return;
     const int64_t m = std::min(p, Nt);
     MSLR_REQUIRE(m <= MAXM, MSLR_ERR_UNSUPPORTED,
                  "min(patch voxels, Nt) must be <= " + std::to_string(MAXM));
-    MSLR_REQUIRE(Pin != Pout, MSLR_ERR_ARG, "svst_batched needs distinct input and output");
+    const bool direct = svst_writes_direct(ctx, p, Nt);
+    const bool in_place = Pin == Pout;
+    MSLR_REQUIRE(!in_place || direct, MSLR_ERR_ARG, "svst_batched needs distinct input and output");
+    MSLR_REQUIRE(!scatter || (direct && scatter->p == p && scatter->Nt == Nt &&
+                              scatter->exact_tiling() && scatter->Np == Np),
+                 MSLR_ERR_ARG, "svst_batched scatters only an exact tiling's patches");
     const int64_t len = p * Nt;
     if (m == 1) {
         HIP_CHECK(hipMemcpyAsync(Pout, Pin, (size_t)(len * Np) * sizeof(c64),
@@ -2098,7 +2132,11 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
     for (int64_t ip0 = 0; ip0 < Np; ip0 += w.chunk) {
         const int64_t nb = std::min<int64_t>(w.chunk, Np - ip0);
         const c64 *Pc = Pin + ip0 * len;
-        c64 *Yc = Pout + ip0 * len;
+        c64 *Yc = scatter ? Pout : Pout + ip0 * len;
+        // in place: the reconstruction products read the split pass's FP32 copy
+        c64 *Pcopy = in_place
+                         ? (c64 *)ctx->scratch_get(SLOT_SVST_COPY, (size_t)(len * nb) * sizeof(c64))
+                         : nullptr;
         // split products: the chunk's patches once as split-BF16 words for the Gram
         // (the apply splits the FP32 data per element: no block-relative residual)
         const unsigned *Wh = nullptr, *Wl = nullptr;
@@ -2108,7 +2146,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
             if (!left) {
                 const int64_t nw = cdiv(p, SPL_ROWS) * Nt * nb;
                 k_split_rows<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(Pc, p, Nt * nb, wb,
-                                                                             wb + len * nb);
+                                                                             wb + len * nb, Pcopy);
             } else {
                 const int64_t nw = cdiv(Nt, SPL_ROWS) * p * nb;
                 k_split_cols<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(Pc, p, Nt, nb, wb,
@@ -2176,39 +2214,44 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         // p >= Nt the right operands (W, U_r, f U_r^H) are m x m at most and are split
         // once (apply_bf16); W X (p < Nt) would split the patches themselves, so it
         // splits per tile (cgemm_bf16).
-        const int amask = ctx->tuning.apply_fp32 > 0 ? ctx->tuning.apply_fp32 - 1 : SVST_APPLY_FP32;
-        auto prod = [&](int bit, const char *tag, const c64 *A, int64_t bA, int64_t ai, int64_t ak,
-                        const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB,
+        const int amask = apply_fp32_mask(ctx);
+        // out: a product writing Y (scattered into the image with scatter)
+        auto prod = [&](int bit, bool out, const char *tag, const c64 *A, int64_t bA, int64_t ai,
+                        int64_t ak, const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB,
                         const float *fs, int64_t bfs, c64 *C, int64_t bC, int64_t ci, int64_t cj,
                         int M, int N, int K, RankSel rs) {
-            if (split && !(amask & bit) && !left)
-                apply_bf16(ctx, tag, A, bA, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC, cj, M, N, K, nb,
-                           rs);
-            else if (split && !(amask & bit))
+            if (split && !(amask & bit) && !left) {
+                const bool sc = out && scatter;
+                apply_bf16(ctx, tag, A, bA, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC,
+                           sc ? (int64_t)scatter->N[0] * scatter->N[1] * scatter->N[2] : cj, M, N,
+                           K, nb, rs, sc ? scatter : nullptr, ip0);
+            } else if (split && !(amask & bit)) {
                 cgemm_bf16(ctx, tag, A, bA, ai, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC, ci, cj, M,
                            N, K, nb, rs);
-            else
+            } else {
                 cgemm(ctx, tag, A, bA, ai, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC, ci, cj, M, N, K,
                       nullptr, nb, rs);
+            }
         };
+        const c64 *Pa = in_place ? Pcopy : Pc;  // the products' left operand
         if (!left) {
             // Y (p x m) = X (p x m) W
-            prod(1, "apply", Pc, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p, (int)p,
-                 mi, mi, dense);
+            prod(1, true, "apply", Pa, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p,
+                 (int)p, mi, mi, dense);
         } else {
             // Y (p x Nt) = W (p x p) X
-            prod(1, "apply", Wm, m * m, 1, m, Pc, len, 1, p, 0, nullptr, 0, Yc, len, 1, p, (int)p,
-                 (int)Nt, mi, RankSel{});
+            prod(1, true, "apply", Wm, m * m, 1, m, Pa, len, 1, p, 0, nullptr, 0, Yc, len, 1, p,
+                 (int)p, (int)Nt, mi, RankSel{});
         }
         if (lowrank) {
             // T (p x r) = X U_r, then Y = (T diag f) U_r^H; T holds at most m/2 columns
             const int64_t bT = p * half;
             c64 *T = (c64 *)ctx->scratch_get(SLOT_LOWRANK, (size_t)(bT * nb) * sizeof(c64));
             const RankSel s1{w.r, mi, 2}, s2{w.r, mi, 3};
-            prod(2, "apply_xu", Pc, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1, p, (int)p,
-                 half, mi, s1);
-            prod(4, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p, (int)p, mi,
-                 half, s2);
+            prod(2, false, "apply_xu", Pa, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1, p,
+                 (int)p, half, mi, s1);
+            prod(4, true, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p,
+                 (int)p, mi, half, s2);
         }
     }
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "patches.hpp"
 
 namespace mslr {
 
@@ -11,8 +12,19 @@ namespace mslr {
 // Float32 inside, src/recon.jl:195).  err_dev (device int, nullable) is set to
 // 1 when a Gram matrix holds non-finite values.  nuc_out (device, Np doubles,
 // nullable) receives each output's nuclear norm sum_k max(s_k - beta_i, 0).
+//
+// Two options of the split-BF16 form with p >= Nt (svst_writes_direct):
+//   scatter (nullable): the patches are those of an exact tiling (every voxel in
+//     one patch) and Pout is their image (V x Nt): each output patch is written
+//     straight to its voxels, so no patches2img pass follows;
+//   Pin == Pout (in place): the split pass keeps an FP32 copy of the input for the
+//     reconstruction products.
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
-                  const double *beta_dev, int per_patch, int *err_dev, double *nuc_out = nullptr);
+                  const double *beta_dev, int per_patch, int *err_dev, double *nuc_out = nullptr,
+                  const PatchGeom *scatter = nullptr);
+
+// Whether svst_batched takes the two options above for these shapes on ctx.
+bool svst_writes_direct(const mslr_ctx_s *ctx, int64_t p, int64_t Nt);
 
 // Vector-Jacobian product of the batched SVST (p >= Nt >= 2): Zbar (distinct from
 // the inputs) <- the cotangent of Z, betabar_dev[i] <- d<Ybar, Y>/d beta_i per

@@ -36,6 +36,7 @@ SPACE: dict[str, tuple[int, ...]] = {
     "yz_threads_adj": (0, 1024),
     "update_lines": (0, 32, 16),
     "sb2st_schedule": (0, 1, 2, 3),
+    "apply_fp32": (0, 8),
 }
 BUILTIN = {k: 0 for k in SPACE}
 # plane sizes (Ny, Nz) with compiled y-z FFT passes of two thread counts
@@ -43,10 +44,13 @@ YZ_COMPILED = {(90, 60), (64, 40)}
 SBR_MAXM = 300  # Gram matrices above this keep their band in global memory
 
 
-def relevant(name: str, N, Nt: int, patch_sizes) -> bool:
+def relevant(name: str, N, Nt: int, patch_sizes, form: str | None = None) -> bool:
     """Whether a shape can change anything for this workload (the others are skipped):
     the yz thread counts only for the compiled planes above 2048 points, the global-band
-    schedule only when some patch has a Gram matrix above 300."""
+    schedule only when some patch has a Gram matrix above 300, the reconstruction
+    products' arithmetic only in the split-BF16 (mixed) form."""
+    if name == "apply_fp32":
+        return form in (None, "mixed")
     if name in ("yz_threads_fwd", "yz_threads_adj"):
         return (int(N[1]), int(N[2])) in YZ_COMPILED
     if name == "sb2st_schedule":
@@ -142,7 +146,7 @@ def autotune_pogm(A, y, *, patch_sizes, strides, lambdas, L: float, form: str = 
     if log:
         log(f"tune {best}: {best_ms:.3f} ms/iter")
     for name, value in descent_order():
-        if not relevant(name, A.N, A.Nt, patch_sizes):
+        if not relevant(name, A.N, A.Nt, patch_sizes, form):
             continue
         cand = dict(best, **{name: value})
         ms, X = trial(cand)

@@ -52,3 +52,5 @@ def test_relevance_skips_shapes_that_cannot_matter():
     assert not T.relevant("sb2st_schedule", (90, 90, 60), 300, ps1)   # m <= 300 everywhere
     assert T.relevant("sb2st_schedule", (90, 90, 60), 387, ps2)       # m = 387
     assert T.relevant("gram_chunk", (12, 12, 10), 10, ps1)
+    assert T.relevant("apply_fp32", (90, 90, 60), 300, ps1, "mixed")
+    assert not T.relevant("apply_fp32", (90, 90, 60), 300, ps1, "plain")

@@ -157,6 +157,29 @@ def test_mixed_form_vs_plain():
     assert rel_err(X2.sum(axis=4), X1.sum(axis=4)) < 1e-4
 
 
+@pytest.mark.parametrize("N,ps,ss,Nt", [((12, 12, 8), (4, 4, 4), (4, 4, 4), 20),      # exact tiling
+                                        ((12, 12, 10), (12, 12, 10), (12, 12, 10), 30),  # one patch
+                                        ((12, 12, 10), (6, 6, 6), (3, 3, 3), 20),      # overlapping
+                                        ((12, 12, 8), (4, 4, 4), (4, 4, 4), 80)])      # p < Nt
+def test_split_patch_svst(split, rng, N, ps, ss, Nt):
+    """patchSVST in split BF16 for the layouts whose products write the image
+    directly (an exact tiling's patches scattered to their voxels, the single patch)
+    and for those that go through patches2img, against the FP64 oracle."""
+    from mslr_hip import recon
+    V = int(np.prod(N))
+    L = crandn(rng, (V, 2)) @ crandn(rng, (2, Nt))
+    img = np.asfortranarray((4.0 * L.reshape(*N, Nt, order="F") + crandn(rng, (*N, Nt)))
+                            .astype(np.complex64))
+    beta = 9.0
+    out = recon.patchSVST(img, beta, ps, ss).to_host()
+    want = O.patchSVST(img.astype(np.complex128), np.float32(beta), ps, ss, dtype=np.complex128)
+    assert rel_err(out, want) < 1e-4
+    # the input is left alone
+    img2 = recon.patchSVST(img, 0.0, ps, ss).to_host()
+    want0 = O.patchSVST(img.astype(np.complex128), np.float32(0.0), ps, ss, dtype=np.complex128)
+    assert rel_err(img2, want0) < 1e-4
+
+
 def test_split_apply_forms_at_the_boundary(split, rng):
     from mslr_hip import recon
     from test_gpu_svst import APPLY_RANKS, mixed_rank_batch
