@@ -366,8 +366,9 @@ static void scale_nucnorm(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, d
 // patchSVST for one scale: out <- patches2img(SVST(img2patches(img))).
 // Two layouts need no patch copies: 1x1x1 patches with stride 1 (one voxel per
 // patch), and a single patch covering the image (the patch matrix is the image).
-// With split-BF16 products (p >= Nt) the SVST writes out itself: the single patch
-// in place, an exact tiling's patches scattered to their voxels (no patches2img).
+// With split-BF16 products (p >= Nt) the SVST's split pass gathers the patches from
+// the image (no img2patches) and the SVST writes out itself: the single patch in
+// place, an exact tiling's patches scattered to their voxels (no patches2img).
 // nuc_part/nuc_tot (device, nullable together): *nuc_tot <- sum over patches of
 // the nuclear norm of out's patches; nuc_part holds nuc_units(g) partials.
 static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *out,
@@ -393,13 +394,24 @@ static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 
         return;
     }
     if (direct && g.Np > 1 && g.exact_tiling()) {
-        c64 *P = (c64 *)ctx->scratch_get(SLOT_PATCH, bytes);
-        launch_img2patches(ctx, g, img, P, 0, g.Np);
-        svst_batched(ctx, P, out, g.p, g.Nt, g.Np, beta_dev, 0, err, nuc_part, &g);
+        svst_batched(ctx, img, out, g.p, g.Nt, g.Np, beta_dev, 0, err, nuc_part, &g, &g);
         if (nuc_part) sum_partials(ctx, nuc_part, g.Np, nuc_tot);
         return;
     }
     c64 *Y = (c64 *)ctx->scratch_get(SLOT_PATCH_OUT, bytes);
+    if (direct && g.Np > 1) {
+        // overlapping patches: gathered by the split pass, averaged by patches2img
+        const bool disjoint = g.disjoint();
+        svst_batched(ctx, img, Y, g.p, g.Nt, g.Np, beta_dev, 0, err, disjoint ? nuc_part : nullptr,
+                     nullptr, &g);
+        launch_patches2img(ctx, g, Y, out);
+        if (!nuc_part) return;
+        if (disjoint)
+            sum_partials(ctx, nuc_part, g.Np, nuc_tot);
+        else
+            scale_nucnorm(ctx, g, out, nuc_tot);
+        return;
+    }
     if (g.Np == 1 && g.p == V) {
         svst_batched(ctx, img, Y, g.p, g.Nt, 1, beta_dev, 0, err, nuc_part);
         {

@@ -1246,6 +1246,43 @@ __global__ __launch_bounds__(256) void k_split_rows(const c64 *__restrict__ X, i
     split_block(v0, v1, lane, n, Wh + off, Wl + off);
 }
 
+// k_split_rows reading the patches from their image: patches ip0 .. ip0 + np of
+// the geometry g, words and the FP32 copy Xc in patch-matrix order (columns of the
+// np patches one after another).  One wave per (patch, block of SPL_ROWS rows,
+// group of SG_T frames): the rows' voxel offsets once, then a frame per step.
+constexpr int SG_T = 16;
+__global__ __launch_bounds__(256) void k_split_gather(const c64 *__restrict__ img, PatchGeom g,
+                                                      int64_t ip0, int64_t np,
+                                                      unsigned *__restrict__ Wh,
+                                                      unsigned *__restrict__ Wl,
+                                                      c64 *__restrict__ Xc) {
+    const int64_t p = g.p;
+    const int Nt = g.Nt;
+    const int64_t nblk = (p + SPL_ROWS - 1) / SPL_ROWS;
+    const int ntg = (Nt + SG_T - 1) / SG_T;
+    const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= np * nblk * ntg) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t blk = wid % nblk, r = wid / nblk;
+    const int tg = (int)(r % ntg);
+    const int64_t z = r / ntg;
+    const int n = (int)std::min<int64_t>(SPL_ROWS, p - blk * SPL_ROWS);
+    const int r0 = (int)(blk * SPL_ROWS) + lane;
+    const int64_t o0 = lane < n ? patch_voxel(g, ip0 + z, r0) : 0;
+    const int64_t o1 = lane + 64 < n ? patch_voxel(g, ip0 + z, r0 + 64) : 0;
+    const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
+    const int t1 = min(Nt, (tg + 1) * SG_T);
+#pragma unroll 2
+    for (int t = tg * SG_T; t < t1; ++t) {
+        const c64 v0 = lane < n ? img[o0 + t * V] : c64{0.f, 0.f};
+        const c64 v1 = lane + 64 < n ? img[o1 + t * V] : c64{0.f, 0.f};
+        const int64_t off = (z * Nt + t) * p + blk * SPL_ROWS;
+        if (lane < n) Xc[off + lane] = v0;
+        if (lane + 64 < n) Xc[off + lane + 64] = v1;
+        split_block(v0, v1, lane, n, Wh + off, Wl + off);
+    }
+}
+
 // Patches with fewer voxels than frames (the Gram X X^H contracts over time):
 // row a of patch ip as time-contiguous words, W[(ip p + a) Nt + k], blocks of
 // SPL_ROWS times sharing an exponent, so that k_gram_bf16 reads them as it reads
@@ -2102,7 +2139,7 @@ bool svst_writes_direct(const mslr_ctx_s *ctx, int64_t p, int64_t Nt) {
 
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
                   const double *beta_dev, int per_patch, int *err_dev, double *nuc_out,
-                  const PatchGeom *scatter) {
+                  const PatchGeom *scatter, const PatchGeom *gather) {
     if (Np == 0 || p == 0 || Nt == 0) return;
     const int64_t m = std::min(p, Nt);
     MSLR_REQUIRE(m <= MAXM, MSLR_ERR_UNSUPPORTED,
@@ -2113,6 +2150,10 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
     MSLR_REQUIRE(!scatter || (direct && scatter->p == p && scatter->Nt == Nt &&
                               scatter->exact_tiling() && scatter->Np == Np),
                  MSLR_ERR_ARG, "svst_batched scatters only an exact tiling's patches");
+    MSLR_REQUIRE(!gather || (direct && gather->p == p && gather->Nt == Nt && gather->Np == Np),
+                 MSLR_ERR_ARG, "svst_batched gathers patches only for split products with p >= Nt");
+    // the products read an FP32 copy of the patches, written by the split pass
+    const bool copy = in_place || gather;
     const int64_t len = p * Nt;
     if (m == 1) {
         HIP_CHECK(hipMemcpyAsync(Pout, Pin, (size_t)(len * Np) * sizeof(c64),
@@ -2131,12 +2172,11 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
     c64 *Wm = (c64 *)w.part;
     for (int64_t ip0 = 0; ip0 < Np; ip0 += w.chunk) {
         const int64_t nb = std::min<int64_t>(w.chunk, Np - ip0);
-        const c64 *Pc = Pin + ip0 * len;
+        const c64 *Pc = gather ? nullptr : Pin + ip0 * len;  // (gathered: the copy below)
         c64 *Yc = scatter ? Pout : Pout + ip0 * len;
         // in place: the reconstruction products read the split pass's FP32 copy
-        c64 *Pcopy = in_place
-                         ? (c64 *)ctx->scratch_get(SLOT_SVST_COPY, (size_t)(len * nb) * sizeof(c64))
-                         : nullptr;
+        c64 *Pcopy = copy ? (c64 *)ctx->scratch_get(SLOT_SVST_COPY, (size_t)(len * nb) * sizeof(c64))
+                          : nullptr;
         // split products: the chunk's patches once as split-BF16 words for the Gram
         // (the apply splits the FP32 data per element: no block-relative residual)
         const unsigned *Wh = nullptr, *Wl = nullptr;
@@ -2145,8 +2185,14 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
             MSLR_PROF(ctx, "split");
             if (!left) {
                 const int64_t nw = cdiv(p, SPL_ROWS) * Nt * nb;
-                k_split_rows<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(Pc, p, Nt * nb, wb,
-                                                                             wb + len * nb, Pcopy);
+                if (gather) {
+                    const int64_t ng = cdiv(p, SPL_ROWS) * cdiv(Nt, SG_T) * nb;
+                    k_split_gather<<<(unsigned)cdiv(ng, 4), 256, 0, ctx->stream>>>(
+                        Pin, *gather, ip0, nb, wb, wb + len * nb, Pcopy);
+                } else {
+                    k_split_rows<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(
+                        Pc, p, Nt * nb, wb, wb + len * nb, Pcopy);
+                }
             } else {
                 const int64_t nw = cdiv(Nt, SPL_ROWS) * p * nb;
                 k_split_cols<<<(unsigned)cdiv(nw, 4), 256, 0, ctx->stream>>>(Pc, p, Nt, nb, wb,
@@ -2233,7 +2279,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
                       nullptr, nb, rs);
             }
         };
-        const c64 *Pa = in_place ? Pcopy : Pc;  // the products' left operand
+        const c64 *Pa = copy ? Pcopy : Pc;  // the products' left operand
         if (!left) {
             // Y (p x m) = X (p x m) W
             prod(1, true, "apply", Pa, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p,

@@ -13,15 +13,18 @@ namespace mslr {
 // 1 when a Gram matrix holds non-finite values.  nuc_out (device, Np doubles,
 // nullable) receives each output's nuclear norm sum_k max(s_k - beta_i, 0).
 //
-// Two options of the split-BF16 form with p >= Nt (svst_writes_direct):
+// Options of the split-BF16 form with p >= Nt (svst_writes_direct):
+//   gather (nullable): Pin is the image (V x Nt) of the geometry's patches, which
+//     the split pass reads through the geometry (no img2patches pass);
 //   scatter (nullable): the patches are those of an exact tiling (every voxel in
-//     one patch) and Pout is their image (V x Nt): each output patch is written
-//     straight to its voxels, so no patches2img pass follows;
-//   Pin == Pout (in place): the split pass keeps an FP32 copy of the input for the
-//     reconstruction products.
+//     one patch) and Pout is their image: each output patch is written straight to
+//     its voxels, so no patches2img pass follows;
+//   Pin == Pout (in place, also with gather and scatter on one image).
+// With gather or in place the split pass keeps an FP32 copy of the patch matrices
+// for the reconstruction products.
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
                   const double *beta_dev, int per_patch, int *err_dev, double *nuc_out = nullptr,
-                  const PatchGeom *scatter = nullptr);
+                  const PatchGeom *scatter = nullptr, const PatchGeom *gather = nullptr);
 
 // Whether svst_batched takes the two options above for these shapes on ctx.
 bool svst_writes_direct(const mslr_ctx_s *ctx, int64_t p, int64_t Nt);
