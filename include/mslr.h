@@ -111,9 +111,9 @@ mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
  *                    adjoint; the batched forms alternate groups between two streams
  *                    so the groups' memory-bound and FFT-bound kernels overlap.
  *                    Changes the summation order of the restart test's FP64 sums only.
- *   yz_threads_fwd / yz_threads_adj: threads of the compiled y-z FFT passes for
- *                    planes above 2048 points, 512 or 1024 (built-in 1024 forward,
- *                    512 adjoint).  Results are bit-identical.
+ *   yz_threads_fwd / yz_threads_adj: threads of the compiled y-z FFT passes of
+ *                    the 90 x 60 plane, 512, 768 or 1024 (built-in 512 forward, 768
+ *                    adjoint).  Results are bit-identical.
  *   update_lines:    yz-lines per workgroup of the POGM update, which is fused with
  *                    the gradient's last transpose (plane-major -> image order): 0 =
  *                    built-in 64 (fewer where an Nx x 64 tile exceeds the LDS), else a

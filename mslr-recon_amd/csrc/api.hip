@@ -177,8 +177,8 @@ mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t) {
         MSLR_REQUIRE(t->sense_chunks >= 0 && t->sense_chunks <= 64, MSLR_ERR_ARG,
                      "sense_chunks must be in [0, 64]");
         for (int v : {t->yz_threads_fwd, t->yz_threads_adj})
-            MSLR_REQUIRE(v == 0 || v == 512 || v == 1024, MSLR_ERR_ARG,
-                         "yz_threads_fwd / yz_threads_adj must be 0, 512 or 1024");
+            MSLR_REQUIRE(v == 0 || v == 512 || v == 768 || v == 1024, MSLR_ERR_ARG,
+                         "yz_threads_fwd / yz_threads_adj must be 0, 512, 768 or 1024");
         MSLR_REQUIRE(t->update_lines == 0 ||
                          (t->update_lines >= 8 && t->update_lines <= 64 && t->update_lines % 8 == 0),
                      MSLR_ERR_ARG, "update_lines must be 0 or a multiple of 8 in [8, 64]");

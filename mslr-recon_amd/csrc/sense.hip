@@ -23,6 +23,7 @@
 #include <type_traits>
 
 #include "fft.hpp"
+#include "fft2d.hpp"
 #include "fft_fixed.hpp"
 #include "reduce.hpp"
 #include "sense.hpp"
@@ -35,28 +36,14 @@ namespace mslr {
 constexpr int NTX = MSLR_NTX;  // threads of the x-axis passes
 constexpr int MAXE_X = 16;   // elements per thread in the x-axis passes
 constexpr int MAXE_YZ = 16;  // elements per thread in the yz-plane passes
-constexpr int NTYZ_MAX = 512;  // threads of the yz-plane passes (at most)
+constexpr int NTYZ_MAX = 512;  // threads of the runtime-length yz-plane passes (at most)
 constexpr int MAX_NYZ = MAXE_YZ * NTYZ_MAX;
-// threads of the compile-time yz passes for planes above 2048 points.  Forward:
-// 1024 (a 90 x 60 plane then takes one round of butterflies per pass, at 60
-// VGPRs, two workgroups per CU; 8.4 ms per A at the 90x90x60x300x18 shape
-// against 10.7 ms with 512).  Adjoint: 512 (its plane accumulator in registers
-// needs fewer, larger threads; 8.1 ms against 9.8-13.8 ms with 256-1024).
-#ifndef MSLR_NTYZ_FWD
-#define MSLR_NTYZ_FWD 1024
-#endif
-#ifndef MSLR_NTYZ_ADJ
-#define MSLR_NTYZ_ADJ 512
-#endif
 
 // ------------------------------------------------------------------ FFT policies
 // A policy transforms `nlines` lines in LDS: runtime lengths (any 2/3/5 length)
 // or one compile-time length.
 template <class T, int MAXE>
 struct FftRt {
-    static constexpr bool kPipe = false;  // no per-thread register copy of a plane
-    static constexpr int kPlane = 1;
-    static constexpr int kNT = 0;
     FftDesc d;
     const C2<T> *tw;
     int sgn;
@@ -71,9 +58,6 @@ struct FftRt {
 };
 template <class T, int N, int SGN, int MAXLINES, int NT>
 struct FftFix {
-    static constexpr bool kPipe = true;   // plane kept in registers across coils
-    static constexpr int kPlane = (N * MAXLINES + NT - 1) / NT;  // plane elements per thread
-    static constexpr int kNT = NT;                                // threads of the workgroup
     const C2<T> *tw;
     __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride, int estride,
                                                const C2<T> *twp) const {
@@ -85,19 +69,20 @@ struct FftFix {
     }
 };
 
-// The twiddle table's address, hidden from loop-invariant code motion: in a
-// loop over coils the per-thread twiddle loads then stay inside the loop
-// instead of occupying registers across it.
-template <class T>
-__device__ __forceinline__ const C2<T> *opaque_ptr(const C2<T> *p) {
-    asm volatile("" : "+s"(p));
-    return p;
-}
-// Per-lane address hidden the same way.
-template <class T>
-__device__ __forceinline__ const C2<T> *opaque_lane_ptr(const C2<T> *p) {
-    asm volatile("" : "+v"(p));
-    return p;
+// A pointer into global memory hidden from loop-invariant code motion: in a loop
+// over coils the per-thread loads through it then stay inside the loop instead of
+// occupying registers across it.  It keeps the global address space, so the loads
+// stay global (not flat) loads.  (The host pass has no address spaces; there the
+// type is a plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+using gc64p = const __attribute__((address_space(1))) c64 *;
+#else
+using gc64p = const c64 *;
+#endif
+__device__ __forceinline__ gc64p opaque_gptr(const c64 *p) {
+    uint64_t v = (uint64_t)p;
+    asm volatile("" : "+s"(v));
+    return (gc64p)v;
 }
 
 // ------------------------------------------------------------------ kernels
@@ -165,26 +150,24 @@ struct LineTabs {
     const int *lk;       // (K, Nt) (l << 16) | kx of sample j, l = line within frame
 };
 
-// Forward pass 1: one workgroup per (x, coil group, frame).  The summed image
-// plane x is multiplied by each coil's map, transformed along y and z in LDS,
-// and the values on the frame's sampled (ky, kz) lines are kept:
+// Forward pass 1 for runtime FFT lengths (any 2/3/5 size, FP32 or FP64): one
+// workgroup per (x, coil, frame).  The summed image plane x is multiplied by the
+// coil's map, transformed along y and z in LDS, and the values on the frame's
+// sampled (ky, kz) lines are kept:
 //   T[f][c][x][l] (l = line of the frame), the only part of k-space pass 2 needs.
-// The image plane is re-read per coil (an L2 hit): holding it in registers
-// would halve the workgroups resident per CU.  Compile-time FFT lengths loop
-// over all coils, runtime lengths take one coil per workgroup.
 template <class T, class FY, class FZ>
-__global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) void k_fwd_yz(
+__global__ __launch_bounds__(NTYZ_MAX) void k_fwd_yz(
     const C2<T> *__restrict__ Pm, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny,
-    int Nz, int cpw, FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz,
+    int Nz, FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz,
     int64_t ta, int nf, C2<T> *__restrict__ Tb) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = reinterpret_cast<C2<T> *>(smem_raw);
-    // (frame, x, coil group) with the frame fastest, numbered onto XCDs: the
-    // work-groups resident on one XCD share one or two x-planes of coil maps
+    // (frame, x, coil) with the frame fastest, numbered onto XCDs: the work-groups
+    // resident on one XCD share one or two x-planes of coil maps
     const int64_t wid = xcd_order(blockIdx.x, gridDim.x);
     const int f = (int)(wid % nf);
     const int x = (int)((wid / nf) % Nx);
-    const int cg = (int)(wid / ((int64_t)nf * Nx));
+    const int c = (int)(wid / ((int64_t)nf * Nx));
     const int NYZ = Ny * Nz;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int g0 = lptr[ta + f];
@@ -192,23 +175,13 @@ __global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) void k_fwd_yz(
     const int *lines = line_yz + g0;
     C2<T> *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
     const C2<T> *pl = Pm + ((int64_t)f * Nx + x) * NYZ;
-    // runtime lengths: one coil, no loop (a loop around the runtime FFT spills)
-    const int c0 = FY::kPipe ? cg * cpw : cg;
-    const int c1 = FY::kPipe ? min(Nc, c0 + cpw) : c0 + 1;
-#pragma unroll 1
-    for (int c = c0; c < c1; ++c) {
-        const C2<T> *twy = FY::kPipe ? opaque_ptr(fy.tw) : fy.tw;
-        const C2<T> *twz = FY::kPipe ? opaque_ptr(fz.tw) : fz.tw;
-        const C2<T> *sm = smaps + ((int64_t)c * Nx + x) * NYZ;
-        for (int i = tid; i < NYZ; i += nthr) buf[i] = pl[i] * sm[i];
-        __syncthreads();
-        fy(buf, Nz, Ny, 1, twy);
-        fz(buf, Ny, 1, Ny, twz);
-        C2<T> *dst = Tf + ((int64_t)c * Nx + x) * Lf;
-        for (int l = tid; l < Lf; l += nthr) dst[l] = buf[lines[l]];
-        if constexpr (!FY::kPipe) break;
-        __syncthreads();
-    }
+    const C2<T> *sm = smaps + ((int64_t)c * Nx + x) * NYZ;
+    for (int i = tid; i < NYZ; i += nthr) buf[i] = pl[i] * sm[i];
+    __syncthreads();
+    fy(buf, Nz, Ny, 1);
+    fz(buf, Ny, 1, Ny);
+    C2<T> *dst = Tf + ((int64_t)c * Nx + x) * Lf;
+    for (int l = tid; l < Lf; l += nthr) dst[l] = buf[lines[l]];
 }
 
 // Forward pass 2: one workgroup per (block of LB lines, coil, frame).  The
@@ -408,14 +381,13 @@ __global__ __launch_bounds__(NTX) void k_x_resid(
     }
 }
 
-// Adjoint pass B: one workgroup per (x, frame).  For each coil the frame's
-// lines at x are scattered into a zeroed (y, z) plane, transformed along z and
-// y, and accumulated times the conjugate coil map; the scaled sum is written
-// as the plane-major image plane.  Compile-time FFT lengths accumulate in
-// registers (addresses kept out of loop-invariant code motion so that two
-// workgroups stay resident per CU), runtime lengths in the output plane itself.
+// Adjoint pass B for runtime FFT lengths: one workgroup per (x, frame).  For each
+// coil the frame's lines at x are scattered into a zeroed (y, z) plane, transformed
+// along z and y, and accumulated times the conjugate coil map into the output plane
+// (each thread owns the same plane elements for every coil); the scaled sum is the
+// plane-major image plane.
 template <class T, class FY, class FZ>
-__global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) void k_adj_yz(
+__global__ __launch_bounds__(NTYZ_MAX) void k_adj_yz(
     const C2<T> *__restrict__ Tb, const C2<T> *__restrict__ smaps, int Nc, int Nx, int Ny, int Nz,
     FY fy, FZ fz, const int *__restrict__ lptr, const int *__restrict__ line_yz, int64_t ta,
     int nf, T scale, C2<T> *__restrict__ Pm) {
@@ -432,46 +404,208 @@ __global__ __launch_bounds__(FY::kNT > 0 ? FY::kNT : NTYZ_MAX) void k_adj_yz(
     const int *lines = line_yz + g0;
     const C2<T> *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
     C2<T> *pl = Pm + ((int64_t)f * Nx + x) * NYZ;
-    constexpr bool REG = FY::kPipe;
-    constexpr int PE = REG ? FY::kPlane : 1;
-    C2<T> acc[PE];
-#pragma unroll
-    for (int q = 0; q < PE; ++q) acc[q] = C2<T>{T(0), T(0)};
 #pragma unroll 1
     for (int c = 0; c < Nc; ++c) {
-        const C2<T> *twy = opaque_ptr(fy.tw), *twz = opaque_ptr(fz.tw);
         for (int i = tid; i < NYZ; i += nthr) buf[i] = C2<T>{T(0), T(0)};
         __syncthreads();
         const C2<T> *src = Tf + ((int64_t)c * Nx + x) * Lf;
         for (int l = tid; l < Lf; l += nthr) buf[lines[l]] = src[l];
         __syncthreads();
-        fz(buf, Ny, 1, Ny, twz);
-        fy(buf, Nz, Ny, 1, twy);
+        fz(buf, Ny, 1, Ny);
+        fy(buf, Nz, Ny, 1);
         const C2<T> *sm = smaps + ((int64_t)c * Nx + x) * NYZ;
-        if constexpr (REG) {
-            const C2<T> *smt = opaque_lane_ptr(sm + tid);
-#pragma unroll
-            for (int q = 0; q < PE; ++q) {
-                const int i = tid + q * FY::kNT;
-                if (q < PE - 1 || i < NYZ) acc[q] = acc[q] + cmulc(smt[q * FY::kNT], buf[i]);
-            }
-        } else {
-            // each thread owns the same plane elements for every coil
-            for (int i = tid; i < NYZ; i += nthr) {
-                const C2<T> v = cmulc(sm[i], buf[i]);
-                pl[i] = c == 0 ? v : pl[i] + v;
-            }
+        for (int i = tid; i < NYZ; i += nthr) {
+            const C2<T> v = cmulc(sm[i], buf[i]);
+            pl[i] = c == 0 ? v : pl[i] + v;
         }
         __syncthreads();
     }
-    if constexpr (REG) {
+    for (int i = tid; i < NYZ; i += nthr) pl[i] = scale * pl[i];
+}
+
+// ------------------------------------------------------------------ two-stage y-z passes
+// The compiled plane sizes as two-stage in-place transforms (fft2d.hpp): four stages
+// per coil (y stage A, y stage B, z stage A, z stage B), each one barrier, every LDS
+// offset a compile-time constant from the butterfly's base.  Coil maps come in the
+// y-stage-A lane order (smp: for coil c and x, element a * UYA + u holds the map at
+// the lane's point a), so map loads are contiguous across lanes; line_pos gives the
+// LDS offset of each sampled line's (ky, kz).
+
+// Stage butterflies: lane u of a stage handles butterfly u, u + NT, ... (a runtime
+// loop, so a stage with several rounds holds one butterfly's registers at a time).
+// y stage B (forward: twiddle, then DFT; inverse: DFT, then twiddle).  twl holds
+// W^(a b) in the order b * N1y + a.
+template <class PP, int SGN, bool FWD>
+__device__ __forceinline__ void yz2_stage_yb(c64 *buf, gc64p twl, int tid) {
+    constexpr int N1Y = PP::Y::N1, N2Y = PP::Y::N2;
+#pragma unroll 1
+    for (int u = tid; u < PP::UYB; u += PP::NT) {
+        const int a = u % N1Y;
+        c64 *p = buf + PP::lds(N2Y * a, u / N1Y);
+        const gc64p t = twl + a;
+        c64 w[N2Y];
 #pragma unroll
-        for (int q = 0; q < PE; ++q) {
-            const int i = tid + q * nthr;
-            if (i < NYZ) pl[i] = scale * acc[q];
+        for (int b = 0; b < N2Y; ++b) w[b] = p[b];
+        if constexpr (FWD) {
+#pragma unroll
+            for (int b = 1; b < N2Y; ++b) w[b] = w[b] * t[b * N1Y];
         }
-    } else {
-        for (int i = tid; i < NYZ; i += nthr) pl[i] = scale * pl[i];
+        DftP<float, N2Y, SGN>::run(w);
+        if constexpr (!FWD) {
+#pragma unroll
+            for (int b = 1; b < N2Y; ++b) w[b] = w[b] * t[b * N1Y];
+        }
+#pragma unroll
+        for (int b = 0; b < N2Y; ++b) p[b] = w[b];
+    }
+}
+template <class PP, int SGN>
+__device__ __forceinline__ void yz2_stage_za(c64 *buf, int tid) {
+    constexpr int NY = PP::NY, N1Z = PP::Z::N1, N2Z = PP::Z::N2;
+#pragma unroll 1
+    for (int u = tid; u < PP::UZA; u += PP::NT) {
+        c64 *p = buf + PP::lds(u % NY, u / NY);
+        c64 w[N1Z];
+#pragma unroll
+        for (int a = 0; a < N1Z; ++a) w[a] = p[PP::LY * N2Z * a];
+        DftP<float, N1Z, SGN>::run(w);
+#pragma unroll
+        for (int a = 0; a < N1Z; ++a) p[PP::LY * N2Z * a] = w[a];
+    }
+}
+template <class PP, int SGN>
+__device__ __forceinline__ void yz2_stage_zb(c64 *buf, int tid) {
+    constexpr int NY = PP::NY, N2Z = PP::Z::N2;
+#pragma unroll 1
+    for (int u = tid; u < PP::UZB; u += PP::NT) {
+        c64 *p = buf + PP::lds(u % NY, N2Z * (u / NY));
+        c64 w[N2Z];
+#pragma unroll
+        for (int b = 0; b < N2Z; ++b) w[b] = p[PP::LY * b];
+        DftP<float, N2Z, SGN>::run(w);
+#pragma unroll
+        for (int b = 0; b < N2Z; ++b) p[PP::LY * b] = w[b];
+    }
+}
+
+// Forward: one workgroup per (x, frame).  Per coil the y-stage-A lanes
+// read their points of the image plane (L2 hits after the first coil) and of the
+// coil map, multiply and transform; the transformed plane's sampled lines are
+// gathered into T[f][c][x][l].
+template <class PP, int SGN>
+__global__ __launch_bounds__(PP::NT) void k_fwd_yz2(
+    const c64 *__restrict__ Pm, const c64 *__restrict__ smp, int Nc, int Nx,
+    const c64 *__restrict__ twy, const int *__restrict__ lptr, const int *__restrict__ line_pos,
+    int64_t ta, int nf, c64 *__restrict__ Tb) {
+    constexpr int NT = PP::NT, NY = PP::NY, NYZ = PP::NYZ;
+    constexpr int N1Y = PP::Y::N1, N2Y = PP::Y::N2;
+    __shared__ c64 buf[PP::LY * PP::NZ];
+    // (frame, x) with the frame fastest, numbered onto XCDs (coil-map reuse in L2)
+    const int64_t wid = xcd_order(blockIdx.x, gridDim.x);
+    const int f = (int)(wid % nf);
+    const int x = (int)(wid / nf);
+    const int tid = threadIdx.x;
+    const int g0 = lptr[ta + f];
+    const int Lf = lptr[ta + f + 1] - g0;
+    const int *lpos = line_pos + g0;
+    c64 *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
+    const c64 *pl = Pm + ((int64_t)f * Nx + x) * NYZ;
+#pragma unroll 1
+    for (int c = 0; c < Nc; ++c) {
+        const gc64p sm = opaque_gptr(smp + ((int64_t)c * Nx + x) * NYZ);
+        const gc64p plc = opaque_gptr(pl);
+        __syncthreads();  // the previous coil's gather has read the plane
+#pragma unroll 1
+        for (int u = tid; u < PP::UYA; u += NT) {
+            const int b = u % N2Y, pz = u / N2Y;
+            const gc64p row = plc + PP::zspat(pz) * NY + b;
+            c64 v[N1Y];
+#pragma unroll
+            for (int a = 0; a < N1Y; ++a) v[a] = row[N2Y * a] * sm[a * PP::UYA + u];
+            DftP<float, N1Y, SGN>::run(v);
+            c64 *p = buf + PP::lds(b, pz);
+#pragma unroll
+            for (int a = 0; a < N1Y; ++a) p[N2Y * a] = v[a];
+        }
+        __syncthreads();
+        yz2_stage_yb<PP, SGN, true>(buf, opaque_gptr(twy), tid);
+        __syncthreads();
+        yz2_stage_za<PP, SGN>(buf, tid);
+        __syncthreads();
+        yz2_stage_zb<PP, SGN>(buf, tid);
+        __syncthreads();
+        c64 *dst = Tf + ((int64_t)c * Nx + x) * Lf;
+        for (int l = tid; l < Lf; l += NT) dst[l] = buf[lpos[l]];
+    }
+}
+
+// Adjoint: one workgroup per (x, frame).  Per coil the frame's lines at x are
+// scattered into the (zeroed) plane, the four stages run in reverse with the
+// opposite exponent sign, and the y-stage-A lanes accumulate conj(map) times their
+// points in registers, writing zeros back so the plane is clear for the next coil.
+template <class PP, int SGN>
+__global__ __launch_bounds__(PP::NT) void k_adj_yz2(
+    const c64 *__restrict__ Tb, const c64 *__restrict__ smp, int Nc, int Nx,
+    const c64 *__restrict__ twy, const int *__restrict__ lptr, const int *__restrict__ line_pos,
+    int64_t ta, int nf, float scale, c64 *__restrict__ Pm) {
+    constexpr int NT = PP::NT, NY = PP::NY, NYZ = PP::NYZ;
+    constexpr int N1Y = PP::Y::N1, N2Y = PP::Y::N2;
+    __shared__ c64 buf[PP::LY * PP::NZ];
+    // (frame, x) with the frame fastest, numbered onto XCDs (coil-map reuse in L2)
+    const int64_t wid = xcd_order(blockIdx.x, gridDim.x);
+    const int f = (int)(wid % nf);
+    const int x = (int)(wid / nf);
+    const int tid = threadIdx.x;
+    const int g0 = lptr[ta + f];
+    const int Lf = lptr[ta + f + 1] - g0;
+    const int *lpos = line_pos + g0;
+    const c64 *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
+
+    for (int i = tid; i < PP::LY * PP::NZ; i += NT) buf[i] = c64{0.f, 0.f};
+    c64 acc[PP::RYA][N1Y];
+#pragma unroll
+    for (int r = 0; r < PP::RYA; ++r)
+#pragma unroll
+        for (int a = 0; a < N1Y; ++a) acc[r][a] = c64{0.f, 0.f};
+#pragma unroll 1
+    for (int c = 0; c < Nc; ++c) {
+        __syncthreads();  // the plane is clear
+        const c64 *src = Tf + ((int64_t)c * Nx + x) * Lf;
+        for (int l = tid; l < Lf; l += NT) buf[lpos[l]] = src[l];
+        __syncthreads();
+        yz2_stage_zb<PP, SGN>(buf, tid);
+        __syncthreads();
+        yz2_stage_za<PP, SGN>(buf, tid);
+        __syncthreads();
+        yz2_stage_yb<PP, SGN, false>(buf, opaque_gptr(twy), tid);
+        __syncthreads();
+        const gc64p sm = opaque_gptr(smp + ((int64_t)c * Nx + x) * NYZ);
+#pragma unroll
+        for (int r = 0; r < PP::RYA; ++r) {
+            const int u = tid + r * NT;
+            if (u < PP::UYA) {
+                c64 *p = buf + PP::lds(u % N2Y, u / N2Y);
+                c64 w[N1Y];
+#pragma unroll
+                for (int a = 0; a < N1Y; ++a) w[a] = p[N2Y * a];
+#pragma unroll
+                for (int a = 0; a < N1Y; ++a) p[N2Y * a] = c64{0.f, 0.f};
+                DftP<float, N1Y, SGN>::run(w);
+#pragma unroll
+                for (int a = 0; a < N1Y; ++a)
+                    acc[r][a] = acc[r][a] + cmulc(sm[a * PP::UYA + u], w[a]);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < PP::RYA; ++r) {
+        const int u = tid + r * NT;
+        if (u < PP::UYA) {
+            const int b = u % N2Y, pz = u / N2Y;
+            c64 *row = Pm + ((int64_t)f * Nx + x) * NYZ + PP::zspat(pz) * NY + b;
+#pragma unroll
+            for (int a = 0; a < N1Y; ++a) row[N2Y * a] = scale * acc[r][a];
+        }
     }
 }
 
@@ -583,15 +717,14 @@ static void launch_fwd_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Pm, 
                           FY fy, FZ fz, int nt) {
     mslr_ctx_s *ctx = P->ctx;
     const int Nx = P->N[0];
-    const int cpw = FY::kPipe ? P->Nc : 1;
     allow_lds((const void *)k_fwd_yz<T, FY, FZ>);
-    const int64_t nwg = (int64_t)Nx * cdiv(P->Nc, cpw) * nf;
+    const int64_t nwg = (int64_t)Nx * P->Nc * nf;
     MSLR_REQUIRE(nwg * nt < ((int64_t)1 << 32), MSLR_ERR_UNSUPPORTED, "too many frames per chunk");
     const unsigned g = (unsigned)nwg;
     MSLR_PROF(ctx, "fwd_yz");
     k_fwd_yz<T, FY, FZ><<<g, nt, (size_t)P->NYZ * sizeof(C2<T>), ctx->stream>>>(
-        Pm, Dev<T>::smaps(P), P->Nc, Nx, P->N[1], P->N[2], cpw, fy, fz, P->lptr, P->line_yz, ta,
-        nf, Tb);
+        Pm, Dev<T>::smaps(P), P->Nc, Nx, P->N[1], P->N[2], fy, fz, P->lptr, P->line_yz, ta, nf,
+        Tb);
     HIP_LAUNCH_CHECK();
 }
 
@@ -655,58 +788,105 @@ static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, 
 }
 
 // ---- dispatch: compile-time FFT lengths for the production / test sizes in
-// FP32 with the default exponent sign, runtime lengths otherwise.  Planes above
-// 2048 points have two compiled thread counts per direction (512 and 1024); the
-// built-in choice is the measured one, mslr_ctx_set_tuning selects the other.
-constexpr int fix_nt_yz(int ny, int nz, int sgn) {
-    return ny * nz > 2048 ? (sgn < 0 ? MSLR_NTYZ_FWD : MSLR_NTYZ_ADJ) : 256;
-}
-
+// FP32 with the default exponent sign, runtime lengths otherwise.
 template <int NX, int SGN>
 using FixX = FftFix<float, NX, SGN, (MAXE_X * NTX) / NX, NTX>;
-template <int NY, int NZ, int SGN, int NT = fix_nt_yz(NY, NZ, SGN)>
-using FixY = FftFix<float, NY, SGN, NZ, NT>;
-template <int NY, int NZ, int SGN, int NT = fix_nt_yz(NY, NZ, SGN)>
-using FixZ = FftFix<float, NZ, SGN, NY, NT>;
-
-// Thread count of the compiled yz pass for this plane and direction: the tuning's
-// (512 or 1024) for planes above 2048 points, else the built-in one.
-static int yz_threads(mslr_sense_s *P, int sgn) {
-    const int built = fix_nt_yz(P->N[1], P->N[2], sgn);
-    if (P->N[1] * P->N[2] <= 2048) return built;
-    const int t = sgn < 0 ? P->ctx->tuning.yz_threads_fwd : P->ctx->tuning.yz_threads_adj;
-    return t ? t : built;
-}
 
 #define MSLR_X_SIZES(M) M(90) M(64) M(32) M(12)
-#define MSLR_YZ_SIZES(M) M(90, 60) M(64, 40) M(32, 24) M(12, 10)
 
 template <class T>
 static bool fixed_ok(mslr_sense_s *P) {
     return std::is_same<T, float>::value && P->sgn == -1 && P->use_fixed;
 }
 
+// ---- two-stage y-z passes: the compiled plane plans (y split Cooley-Tukey, z split
+// prime-factor).  The 90 x 60 plane has three compiled thread counts per direction;
+// the built-in ones are the measured fastest at 90x90x60, 300 frames, 18 coils
+// (forward 512: 5.55 ms against 5.9 / 5.9 ms with 768 / 1024; adjoint 768: 5.08 ms
+// against 5.42 / 7.57 ms with 512 / 1024, whose 80 VGPRs leave one workgroup per CU).
+template <int NT>
+using Plane90x60 = PlanePlan<PlaneAxis<10, 9, false>, PlaneAxis<12, 5, true>, NT>;
+using Plane64x40 = PlanePlan<PlaneAxis<8, 8, false>, PlaneAxis<8, 5, true>, 512>;
+using Plane32x24 = PlanePlan<PlaneAxis<8, 4, false>, PlaneAxis<8, 3, true>, 256>;
+using Plane12x10 = PlanePlan<PlaneAxis<4, 3, false>, PlaneAxis<2, 5, true>, 64>;
+#define MSLR_YZ2_PLANS(M) M(64, 40, Plane64x40) M(32, 24, Plane32x24) M(12, 10, Plane12x10)
+constexpr int YZ2_NT_FWD = 512, YZ2_NT_ADJ = 768;
+
+template <class PP>
+static void launch_fwd_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Pm, c64 *Tb) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int Nx = P->N[0];
+    const int64_t nwg = (int64_t)Nx * nf;
+    MSLR_REQUIRE(nwg * PP::NT < ((int64_t)1 << 32), MSLR_ERR_UNSUPPORTED, "too many frames per chunk");
+    MSLR_PROF(ctx, "fwd_yz");
+    k_fwd_yz2<PP, -1><<<(unsigned)nwg, PP::NT, 0, ctx->stream>>>(
+        Pm, P->smaps_yz2, P->Nc, Nx, P->tw_yz2[0], P->lptr, P->line_pos, ta, nf, Tb);
+    HIP_LAUNCH_CHECK();
+}
+
+template <class PP>
+static void launch_adj_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Tb, c64 *Pm) {
+    mslr_ctx_s *ctx = P->ctx;
+    const int Nx = P->N[0];
+    const int64_t nwg = (int64_t)Nx * nf;
+    MSLR_REQUIRE(nwg * PP::NT < ((int64_t)1 << 32), MSLR_ERR_UNSUPPORTED, "too many frames per chunk");
+    MSLR_PROF(ctx, "adj_yz");
+    k_adj_yz2<PP, 1><<<(unsigned)nwg, PP::NT, 0, ctx->stream>>>(
+        Tb, P->smaps_yz2, P->Nc, Nx, P->tw_yz2[1], P->lptr, P->line_pos, ta, nf,
+        (float)P->scale, Pm);
+    HIP_LAUNCH_CHECK();
+}
+
+// The two-stage passes for the plan's plane (tables built at plan creation); the
+// 90 x 60 plane's thread count comes from the tuning (0 = built-in).
+static bool fwd_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Pm, c64 *Tb) {
+    if (!P->yz2) return false;
+    if (P->N[1] == 90 && P->N[2] == 60) {
+        const int t = P->ctx->tuning.yz_threads_fwd ? P->ctx->tuning.yz_threads_fwd : YZ2_NT_FWD;
+        switch (t) {
+            case 768: launch_fwd_yz2<Plane90x60<768>>(P, ta, nf, Pm, Tb); break;
+            case 1024: launch_fwd_yz2<Plane90x60<1024>>(P, ta, nf, Pm, Tb); break;
+            default: launch_fwd_yz2<Plane90x60<512>>(P, ta, nf, Pm, Tb); break;
+        }
+        return true;
+    }
+    switch (P->N[1] * 10000 + P->N[2]) {
+#define CASE_YZ2(ny, nz, PP)                          \
+    case ny * 10000 + nz:                             \
+        launch_fwd_yz2<PP>(P, ta, nf, Pm, Tb);        \
+        return true;
+        MSLR_YZ2_PLANS(CASE_YZ2)
+#undef CASE_YZ2
+        default: return false;
+    }
+}
+
+static bool adj_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Tb, c64 *Pm) {
+    if (!P->yz2) return false;
+    if (P->N[1] == 90 && P->N[2] == 60) {
+        const int t = P->ctx->tuning.yz_threads_adj ? P->ctx->tuning.yz_threads_adj : YZ2_NT_ADJ;
+        switch (t) {
+            case 512: launch_adj_yz2<Plane90x60<512>>(P, ta, nf, Tb, Pm); break;
+            case 1024: launch_adj_yz2<Plane90x60<1024>>(P, ta, nf, Tb, Pm); break;
+            default: launch_adj_yz2<Plane90x60<768>>(P, ta, nf, Tb, Pm); break;
+        }
+        return true;
+    }
+    switch (P->N[1] * 10000 + P->N[2]) {
+#define CASE_YZ2(ny, nz, PP)                          \
+    case ny * 10000 + nz:                             \
+        launch_adj_yz2<PP>(P, ta, nf, Tb, Pm);        \
+        return true;
+        MSLR_YZ2_PLANS(CASE_YZ2)
+#undef CASE_YZ2
+        default: return false;
+    }
+}
+
 template <class T>
 static void fwd_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Pm, C2<T> *Tb) {
     if constexpr (std::is_same<T, float>::value) {
-        if (fixed_ok<T>(P)) {
-            switch (P->N[1] * 10000 + P->N[2]) {
-#define CASE_YZ(ny, nz)                                                                     \
-    case ny * 10000 + nz:                                                                   \
-        if (ny * nz > 2048 && yz_threads(P, -1) != fix_nt_yz(ny, nz, -1)) {                 \
-            constexpr int alt = fix_nt_yz(ny, nz, -1) == 1024 ? 512 : 1024;                  \
-            launch_fwd_yz<float>(P, ta, nf, Pm, Tb, FixY<ny, nz, -1, alt>{P->tw32[1]},         \
-                                 FixZ<ny, nz, -1, alt>{P->tw32[2]}, alt);                      \
-        } else {                                                                            \
-            launch_fwd_yz<float>(P, ta, nf, Pm, Tb, FixY<ny, nz, -1>{P->tw32[1]},              \
-                                 FixZ<ny, nz, -1>{P->tw32[2]}, fix_nt_yz(ny, nz, -1));         \
-        }                                                                                   \
-        return;
-                MSLR_YZ_SIZES(CASE_YZ)
-#undef CASE_YZ
-                default: break;
-            }
-        }
+        if (fixed_ok<T>(P) && fwd_yz2(P, ta, nf, Pm, Tb)) return;
     }
     launch_fwd_yz<T>(P, ta, nf, Pm, Tb, FftRt<T, MAXE_YZ>{P->d[1], Dev<T>::tw(P, 1), P->sgn},
                      FftRt<T, MAXE_YZ>{P->d[2], Dev<T>::tw(P, 2), P->sgn}, P->nt_yz);
@@ -775,24 +955,7 @@ static void x_resid_any(mslr_sense_s *P, int64_t ta, int nf, C2<T> *Tb, const C2
 template <class T>
 static void adj_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, C2<T> *Pm) {
     if constexpr (std::is_same<T, float>::value) {
-        if (fixed_ok<T>(P)) {
-            switch (P->N[1] * 10000 + P->N[2]) {
-#define CASE_YZ(ny, nz)                                                                     \
-    case ny * 10000 + nz:                                                                   \
-        if (ny * nz > 2048 && yz_threads(P, 1) != fix_nt_yz(ny, nz, 1)) {                   \
-            constexpr int alt = fix_nt_yz(ny, nz, 1) == 1024 ? 512 : 1024;                   \
-            launch_adj_yz<float>(P, ta, nf, Tb, Pm, FixY<ny, nz, 1, alt>{P->tw32[1]},          \
-                                 FixZ<ny, nz, 1, alt>{P->tw32[2]}, alt);                       \
-        } else {                                                                            \
-            launch_adj_yz<float>(P, ta, nf, Tb, Pm, FixY<ny, nz, 1>{P->tw32[1]},               \
-                                 FixZ<ny, nz, 1>{P->tw32[2]}, fix_nt_yz(ny, nz, 1));          \
-        }                                                                                   \
-        return;
-                MSLR_YZ_SIZES(CASE_YZ)
-#undef CASE_YZ
-                default: break;
-            }
-        }
+        if (fixed_ok<T>(P) && adj_yz2(P, ta, nf, Tb, Pm)) return;
     }
     launch_adj_yz<T>(P, ta, nf, Tb, Pm, FftRt<T, MAXE_YZ>{P->d[1], Dev<T>::tw(P, 1), -P->sgn},
                      FftRt<T, MAXE_YZ>{P->d[2], Dev<T>::tw(P, 2), -P->sgn}, P->nt_yz);
@@ -1065,6 +1228,57 @@ static U *upload_vec(mslr_sense_s *P, const std::vector<U> &h) {
     return d;
 }
 
+// Tables of the two-stage y-z passes for a compiled plane plan: the coil maps in
+// the y-stage-A lane order and the LDS offset of every sampled line.
+template <class PP>
+static void yz2_tables(mslr_sense_s *P, const std::vector<c64> &sm32,
+                       const std::vector<int> &line_yz) {
+    using AY = typename PP::Y;
+    using AZ = typename PP::Z;
+    const int Nx = P->N[0], Nc = P->Nc, NY = PP::NY, NYZ = PP::NYZ;
+    std::vector<c64> smp(sm32.size());
+    for (int64_t cx = 0; cx < (int64_t)Nc * Nx; ++cx) {
+        const c64 *src = sm32.data() + cx * NYZ;
+        c64 *dst = smp.data() + cx * NYZ;
+        for (int u = 0; u < PP::UYA; ++u) {
+            const int b = u % AY::N2, z = PP::zspat(u / AY::N2);
+            for (int a = 0; a < AY::N1; ++a) dst[a * PP::UYA + u] = src[AY::spatial(a, b) + NY * z];
+        }
+    }
+    std::vector<int> lp(line_yz.size());
+    for (size_t l = 0; l < line_yz.size(); ++l) {
+        const int ky = line_yz[l] % NY, kz = line_yz[l] / NY;
+        lp[l] = PP::lds(AY::freq_position(ky), AZ::freq_position(kz));
+    }
+    // twiddles W^(a b) = exp(-+ 2 pi i a b / Ny) in the order b * N1y + a
+    for (int sgn = 0; sgn < 2; ++sgn) {
+        std::vector<c64> tl((size_t)AY::N1 * AY::N2);
+        for (int b = 0; b < AY::N2; ++b)
+            for (int a = 0; a < AY::N1; ++a) {
+                const c128 w = unit_phase((sgn == 0 ? -1 : 1) * (int64_t)a * b, NY);
+                tl[(size_t)b * AY::N1 + a] = {(float)w.re, (float)w.im};
+            }
+        P->tw_yz2[sgn] = upload_vec(P, tl);
+    }
+    P->smaps_yz2 = upload_vec(P, smp);
+    P->line_pos = upload_vec(P, lp);
+    P->yz2 = 1;
+}
+
+static void build_yz2_tables(mslr_sense_s *P, const std::vector<c64> &sm32,
+                             const std::vector<int> &line_yz) {
+    switch (P->N[1] * 10000 + P->N[2]) {
+#define CASE_YZ2(ny, nz, PP)                  \
+    case ny * 10000 + nz:                     \
+        yz2_tables<PP>(P, sm32, line_yz);     \
+        break;
+        MSLR_YZ2_PLANS(CASE_YZ2)
+        CASE_YZ2(90, 60, Plane90x60<512>)
+#undef CASE_YZ2
+        default: break;
+    }
+}
+
 extern "C" {
 
 mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, int64_t Nt,
@@ -1203,6 +1417,7 @@ mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, in
                     }
             P->smaps64 = upload_vec(P, sm64);
             P->smaps32 = upload_vec(P, sm32);
+            if (P->use_fixed && P->sgn == -1) build_yz2_tables(P, sm32, line_yz);
             *out = P;
         } catch (...) {
             sense_free(P);

@@ -37,6 +37,11 @@ struct mslr_sense_s {
     int *line_yz = nullptr;  // (lines): ky + Ny*kz, ascending within a frame
     int *kbeg = nullptr;     // (lines): first internal sample of the line in its frame
     int *lk = nullptr;       // (K, Nt): (l << 16) | kx of internal sample j (l: line of the frame)
+    // two-stage y-z passes (compiled plane sizes, FP32, default sign)
+    int yz2 = 0;                       // 1: tables below built
+    mslr::c64 *smaps_yz2 = nullptr;    // coil maps in the y-stage-A lane order
+    int *line_pos = nullptr;           // (lines): LDS offset of the line's (ky, kz)
+    mslr::c64 *tw_yz2[2] = {nullptr, nullptr};  // y twiddles, forward / inverse sign
     std::vector<int> lptr_h; // host copy of lptr
     std::vector<void *> allocs;
 };

@@ -32,15 +32,15 @@ SPACE: dict[str, tuple[int, ...]] = {
     "eig_lanes_limit": (0, -1, 1 << 20),
     "svst_apply": (0, 1),
     "sense_chunks": (0, 2, 4, 8),
-    "yz_threads_fwd": (0, 512),
-    "yz_threads_adj": (0, 1024),
+    "yz_threads_fwd": (0, 768, 1024),
+    "yz_threads_adj": (0, 512, 1024),
     "update_lines": (0, 32, 16),
     "sb2st_schedule": (0, 1, 2, 3),
     "apply_fp32": (0, 8),
 }
 BUILTIN = {k: 0 for k in SPACE}
-# plane sizes (Ny, Nz) with compiled y-z FFT passes of two thread counts
-YZ_COMPILED = {(90, 60), (64, 40)}
+# plane sizes (Ny, Nz) with compiled y-z FFT passes of several thread counts
+YZ_COMPILED = {(90, 60)}
 SBR_MAXM = 300  # Gram matrices above this keep their band in global memory
 
 

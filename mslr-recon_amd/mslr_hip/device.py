@@ -52,8 +52,8 @@ class Context:
         gram_chunk (contraction rows per Gram workgroup, a multiple of 256),
         eig_lanes_limit (patches x m limit for 16 lanes per eigenvector, -1: always 4),
         svst_apply (1: dense apply), sense_chunks (frame groups of the gradient, at most
-        64), yz_threads_fwd / yz_threads_adj (512 or 1024 threads of the compiled y-z FFT
-        passes), update_lines (yz-lines per workgroup of the fused transpose + update, a
+        64), yz_threads_fwd / yz_threads_adj (512, 768 or 1024 threads of the compiled
+        y-z FFT passes of the 90 x 60 plane), update_lines (yz-lines per workgroup of the fused transpose + update, a
         multiple of 8 in [8, 64]), sb2st_schedule (0-3: the
         global-band tridiagonalisation's wave schedule), apply_fp32 (split-BF16 form:
         1 + a mask of the SVST reconstruction products run in FP32 instead, bit 0 the
