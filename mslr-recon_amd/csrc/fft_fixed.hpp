@@ -1,10 +1,7 @@
-// Fixed-length Stockham FFT on lines in LDS, specialised at compile time.
-//
-// Each length is split into two (or one) composite radices whose butterflies
-// run entirely in registers (radix 6, 8, 9, 10 are two-level Cooley-Tukey
-// DFTs with constant twiddles), so a length-90 line takes 2 LDS round trips
-// instead of the 4 of a radix-2/3/5 decomposition, and every index
-// computation divides by compile-time constants.
+// In-register DFT codelets of compile-time size: radix 2, 3, 4, 5 and the two-level
+// Cooley-Tukey forms 6, 8, 9, 10 (constant twiddles, computed at compile time).  The
+// prime-factor codelets and the two-stage LDS transforms built on them are in
+// fft2d.hpp.
 #pragma once
 
 #include "fft.hpp"
@@ -111,97 +108,5 @@ template <class T, int SGN>
 struct DftR<T, 9, SGN> : DftCT<T, 3, 3, SGN> {};
 template <class T, int SGN>
 struct DftR<T, 10, SGN> : DftCT<T, 2, 5, SGN> {};
-
-// ---------------------------------------------------------------- plans
-// Radix sequence per supported length (first pass radix R0 with span 1,
-// second R1 with span R0).
-template <int N>
-struct FixPlan {
-    static constexpr bool ok = false;
-    static constexpr int R0 = 1, R1 = 1;
-};
-#define MSLR_FIXPLAN(N_, A_, B_)                 \
-    template <>                                  \
-    struct FixPlan<N_> {                         \
-        static constexpr bool ok = true;         \
-        static constexpr int R0 = A_, R1 = B_;   \
-    };
-MSLR_FIXPLAN(90, 10, 9)
-MSLR_FIXPLAN(60, 10, 6)
-MSLR_FIXPLAN(64, 8, 8)
-MSLR_FIXPLAN(40, 10, 4)
-MSLR_FIXPLAN(32, 8, 4)
-MSLR_FIXPLAN(24, 8, 3)
-MSLR_FIXPLAN(12, 4, 3)
-MSLR_FIXPLAN(10, 10, 1)
-#undef MSLR_FIXPLAN
-
-inline bool fixed_length(int n) {
-    switch (n) {
-        case 90: case 60: case 64: case 40: case 32: case 24: case 12: case 10:
-            return true;
-        default:
-            return false;
-    }
-}
-
-// One Stockham pass of radix R and span NS over `nlines` lines of length N.
-template <class T, int N, int R, int NS, int SGN, int MAXB>
-__device__ __forceinline__ void fix_pass(C2<T> *buf, int nlines, int lstride, int estride,
-                                         const C2<T> *__restrict__ tw, int tid, int nthr) {
-    constexpr int NB = N / R;
-    constexpr int TSTEP = N / (NS * R);
-    const int total = NB * nlines;
-    C2<T> v[MAXB][R];
-    int ob[MAXB];
-#pragma unroll
-    for (int q = 0; q < MAXB; ++q) {
-        const int b = tid + q * nthr;
-        if (b < total) {
-            const int line = b / NB;
-            const int j = b - line * NB;
-            const int k = j % NS;
-            const C2<T> *src = buf + line * lstride + j * estride;
-#pragma unroll
-            for (int i = 0; i < R; ++i) v[q][i] = src[i * NB * estride];
-            if (NS > 1) {
-#pragma unroll
-                for (int i = 1; i < R; ++i) {
-                    C2<T> w = tw[i * k * TSTEP];
-                    if (SGN < 0) w.im = -w.im;
-                    v[q][i] = v[q][i] * w;
-                }
-            }
-            DftR<T, R, SGN>::run(v[q]);
-            ob[q] = line * lstride + ((j - k) * R + k) * estride;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < MAXB; ++q) {
-        const int b = tid + q * nthr;
-        if (b < total) {
-#pragma unroll
-            for (int i = 0; i < R; ++i) buf[ob[q] + i * NS * estride] = v[q][i];
-        }
-    }
-    __syncthreads();
-}
-
-// Transform of nlines <= MAXLINES lines of length N with NT threads.
-template <class T, int N, int SGN, int MAXLINES, int NT>
-__device__ __forceinline__ void fix_fft(C2<T> *buf, int nlines, int lstride, int estride,
-                                        const C2<T> *__restrict__ tw) {
-    using P = FixPlan<N>;
-    static_assert(P::ok, "no fixed plan for this length");
-    static_assert(P::R0 * P::R1 == N, "bad plan");
-    const int tid = threadIdx.x;
-    constexpr int MB0 = (MAXLINES * (N / P::R0) + NT - 1) / NT;
-    fix_pass<T, N, P::R0, 1, SGN, MB0>(buf, nlines, lstride, estride, tw, tid, NT);
-    if (P::R1 > 1) {
-        constexpr int MB1 = (MAXLINES * (N / P::R1) + NT - 1) / NT;
-        fix_pass<T, N, P::R1, P::R0, SGN, MB1>(buf, nlines, lstride, estride, tw, tid, NT);
-    }
-}
 
 }  // namespace mslr

@@ -24,7 +24,6 @@
 
 #include "fft.hpp"
 #include "fft2d.hpp"
-#include "fft_fixed.hpp"
 #include "reduce.hpp"
 #include "sense.hpp"
 
@@ -40,8 +39,8 @@ constexpr int NTYZ_MAX = 512;  // threads of the runtime-length yz-plane passes 
 constexpr int MAX_NYZ = MAXE_YZ * NTYZ_MAX;
 
 // ------------------------------------------------------------------ FFT policies
-// A policy transforms `nlines` lines in LDS: runtime lengths (any 2/3/5 length)
-// or one compile-time length.
+// A policy transforms `nlines` lines in LDS for runtime lengths (any 2/3/5 length);
+// the compiled lengths use the two-stage transforms of fft2d.hpp instead.
 template <class T, int MAXE>
 struct FftRt {
     FftDesc d;
@@ -50,18 +49,6 @@ struct FftRt {
     __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride, int estride,
                                                const C2<T> *twp) const {
         fft_lines<T, MAXE>(buf, d, nlines, lstride, estride, twp, sgn);
-    }
-    __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride,
-                                               int estride) const {
-        (*this)(buf, nlines, lstride, estride, tw);
-    }
-};
-template <class T, int N, int SGN, int MAXLINES, int NT>
-struct FftFix {
-    const C2<T> *tw;
-    __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride, int estride,
-                                               const C2<T> *twp) const {
-        fix_fft<T, N, SGN, MAXLINES, NT>(buf, nlines, lstride, estride, twp);
     }
     __device__ __forceinline__ void operator()(C2<T> *buf, int nlines, int lstride,
                                                int estride) const {
@@ -379,6 +366,254 @@ __global__ __launch_bounds__(NTX) void k_x_resid(
 #pragma unroll 4
         for (int x = x0; x < Nx; x += xs) src[(int64_t)x * Lf + lq] = buf[lq * Nx + x];
     }
+}
+
+// ---- two-stage x passes (fft2d.hpp; CT or PFA split of Nx).  A workgroup of NT
+// threads holds a block of nl lines in LDS with line stride Nx + 1 (odd: the lanes of
+// a stage step across lines without bank conflicts).  The fused gradient pass and the
+// forward-only and adjoint-only passes share these steps, so their results agree to
+// the bit.
+
+// Forward: stage A reads its points straight from the lines in memory (lanes over
+// lines: contiguous), stage B transforms in LDS.  Ends synchronised.
+template <class AX, int NT>
+__device__ __forceinline__ void x2_forward(const c64 *__restrict__ src, int64_t Lf, int nl,
+                                           c64 *buf, const c64 *__restrict__ twf, int tid) {
+    constexpr int N1 = AX::N1, N2 = AX::N2, LS = AX::N + 1;
+#pragma unroll 1
+    for (int u = tid; u < nl * N2; u += NT) {
+        const int l = u % nl, b = u / nl;
+        c64 w[N1];
+#pragma unroll
+        for (int a = 0; a < N1; ++a) w[a] = src[(int64_t)AX::spatial(a, b) * Lf + l];
+        DftP<float, N1, -1>::run(w);
+        c64 *q = buf + l * LS + b;
+#pragma unroll
+        for (int a = 0; a < N1; ++a) q[N2 * a] = w[a];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int u = tid; u < nl * N1; u += NT) {
+        const int l = u % nl, a = u / nl;
+        c64 *q = buf + l * LS + N2 * a;
+        c64 w[N2];
+#pragma unroll
+        for (int b = 0; b < N2; ++b) w[b] = q[b];
+        if constexpr (!AX::PFA) {
+#pragma unroll
+            for (int b = 1; b < N2; ++b) w[b] = w[b] * twf[b * N1 + a];
+        }
+        DftP<float, N2, -1>::run(w);
+#pragma unroll
+        for (int b = 0; b < N2; ++b) q[b] = w[b];
+    }
+    __syncthreads();
+}
+
+// Inverse: stage B in LDS (DFT, then the conjugate twiddles), stage A written back
+// over the lines in memory.  Starts after the caller's barrier.
+template <class AX, int NT>
+__device__ __forceinline__ void x2_inverse(c64 *buf, int nl, const c64 *__restrict__ twi,
+                                           c64 *__restrict__ dst, int64_t Lf, int tid) {
+    constexpr int N1 = AX::N1, N2 = AX::N2, LS = AX::N + 1;
+#pragma unroll 1
+    for (int u = tid; u < nl * N1; u += NT) {
+        const int l = u % nl, a = u / nl;
+        c64 *q = buf + l * LS + N2 * a;
+        c64 w[N2];
+#pragma unroll
+        for (int b = 0; b < N2; ++b) w[b] = q[b];
+        DftP<float, N2, 1>::run(w);
+        if constexpr (!AX::PFA) {
+#pragma unroll
+            for (int b = 1; b < N2; ++b) w[b] = w[b] * twi[b * N1 + a];
+        }
+#pragma unroll
+        for (int b = 0; b < N2; ++b) q[b] = w[b];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int u = tid; u < nl * N2; u += NT) {
+        const int l = u % nl, b = u / nl;
+        const c64 *q = buf + l * LS + b;
+        c64 w[N1];
+#pragma unroll
+        for (int a = 0; a < N1; ++a) w[a] = q[N2 * a];
+        DftP<float, N1, 1>::run(w);
+#pragma unroll
+        for (int a = 0; a < N1; ++a) dst[(int64_t)AX::spatial(a, b) * Lf + l] = w[a];
+    }
+}
+
+// The block's frame-local geometry and its lines' y, z post-phases (into pha).
+struct X2Block {
+    int g0, Lf, l0, nl;
+};
+__device__ __forceinline__ bool x2_block(const LineTabs &tb, int64_t ta, int f, int lb, int LB,
+                                         X2Block &B) {
+    B.g0 = tb.lptr[ta + f];
+    B.Lf = tb.lptr[ta + f + 1] - B.g0;
+    B.l0 = lb * LB;
+    if (B.l0 >= B.Lf) return false;
+    B.nl = min(LB, B.Lf - B.l0);
+    return true;
+}
+template <int NT>
+__device__ __forceinline__ void x2_phases(const LineTabs &tb, const X2Block &B, int Ny,
+                                          const c64 *__restrict__ phy,
+                                          const c64 *__restrict__ phz, c64 *pha, int tid) {
+    const int *lines = tb.line_yz + B.g0;
+    for (int li = tid; li < B.nl; li += NT) {
+        const int yz = lines[B.l0 + li];
+        pha[li] = phy[yz % Ny] * phz[yz / Ny];
+    }
+}
+
+// The fused x passes of the gradient: one workgroup per (block of LB lines, coil,
+// frame).  Forward transform, the residual r = scale * post-phase * X - y at every
+// sample (and its squared norm), conj(post-phase) * r back at the sample's frequency
+// position, inverse transform written over the lines.  FULL: every line holds all
+// Nx samples, so the residual covers every position and the lines need no clearing.
+template <class AX, int NT, bool FULL>
+__global__ __launch_bounds__(NT) void k_x_resid2(
+    c64 *__restrict__ Tb, int Nc, int Ny, int LB, int nlb, const c64 *__restrict__ twf,
+    const c64 *__restrict__ twi, LineTabs tb, int64_t K, int64_t ta,
+    const c64 *__restrict__ phx, const c64 *__restrict__ phy, const c64 *__restrict__ phz,
+    float scale, const c64 *__restrict__ ysub, double *__restrict__ partial) {
+    constexpr int Nx = AX::N, LS = Nx + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    c64 *buf = reinterpret_cast<c64 *>(smem_raw);  // the block's lines, stride LS
+    c64 *pha = buf + LB * LS;                      // per line: y, z post-phases
+    __shared__ double red[16];
+    const int lb = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
+    const int tid = threadIdx.x;
+    X2Block B;
+    if (!x2_block(tb, ta, f, lb, LB, B)) {  // uniform per workgroup
+        if (partial && tid == 0) partial[((int64_t)f * Nc + c) * nlb + lb] = 0.0;
+        return;
+    }
+    x2_phases<NT>(tb, B, Ny, phy, phz, pha, tid);
+    c64 *src = Tb + (int64_t)Nc * Nx * (B.g0 - tb.lptr[ta]) + (int64_t)c * Nx * B.Lf + B.l0;
+    x2_forward<AX, NT>(src, B.Lf, B.nl, buf, twf, tid);
+    const int *kb = tb.kbeg + B.g0;
+    const int jb = kb[B.l0];
+    const int je = B.l0 + B.nl < B.Lf ? kb[B.l0 + B.nl] : (int)K;
+    const int64_t fa = (ta + f) * K;
+    const int64_t obase = ((int64_t)f * Nc + c) * K;
+    double acc = 0.0;
+    c64 rv[MAXE_X];
+    int rp[MAXE_X];
+#pragma unroll
+    for (int qi = 0; qi < MAXE_X; ++qi) {
+        const int j = jb + tid + qi * NT;
+        rp[qi] = -1;
+        if (j < je) {
+            const int v = tb.lk[fa + j];  // (line << 16) | kx
+            const int l = (v >> 16) - B.l0;
+            const int kx = v & 0xFFFF;
+            const c64 ph = phx[kx] * pha[l];
+            const int pos = l * LS + AX::freq_position(kx);
+            const c64 val = buf[pos] * (scale * ph) - ysub[obase + j];
+            acc += (double)val.re * (double)val.re + (double)val.im * (double)val.im;
+            rv[qi] = cmulc(ph, val);
+            rp[qi] = pos;
+        }
+    }
+    if constexpr (!FULL) {
+        __syncthreads();  // every gather done before the lines are cleared
+        for (int e = tid; e < B.nl * LS; e += NT) buf[e] = c64{0.f, 0.f};
+        __syncthreads();
+    }
+    // (FULL: each position is read and written by the one thread owning its sample)
+#pragma unroll
+    for (int qi = 0; qi < MAXE_X; ++qi)
+        if (rp[qi] >= 0) buf[rp[qi]] = rv[qi];
+    if (partial) {
+        const double s2 = block_sum(acc, red);
+        if (tid == 0) partial[((int64_t)f * Nc + c) * nlb + lb] = s2;
+    }
+    __syncthreads();
+    x2_inverse<AX, NT>(buf, B.nl, twi, src, B.Lf, tid);
+}
+
+// Forward x pass alone: the samples scale * post-phase * X (- y with ysub) into out
+// (+ the block's squared norm).  Same steps and sample order as k_x_resid2.
+template <class AX, int NT>
+__global__ __launch_bounds__(NT) void k_fwd_x2(
+    const c64 *__restrict__ Tb, int Nc, int Ny, int LB, int nlb, const c64 *__restrict__ twf,
+    LineTabs tb, int64_t K, int64_t ta, const c64 *__restrict__ phx,
+    const c64 *__restrict__ phy, const c64 *__restrict__ phz, float scale,
+    const c64 *__restrict__ ysub, c64 *__restrict__ out, double *__restrict__ partial) {
+    constexpr int Nx = AX::N, LS = Nx + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    c64 *buf = reinterpret_cast<c64 *>(smem_raw);
+    c64 *pha = buf + LB * LS;
+    __shared__ double red[16];
+    const int lb = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
+    const int tid = threadIdx.x;
+    X2Block B;
+    if (!x2_block(tb, ta, f, lb, LB, B)) {
+        if (partial && tid == 0) partial[((int64_t)f * Nc + c) * nlb + lb] = 0.0;
+        return;
+    }
+    x2_phases<NT>(tb, B, Ny, phy, phz, pha, tid);
+    const c64 *src = Tb + (int64_t)Nc * Nx * (B.g0 - tb.lptr[ta]) + (int64_t)c * Nx * B.Lf + B.l0;
+    x2_forward<AX, NT>(src, B.Lf, B.nl, buf, twf, tid);
+    const int *kb = tb.kbeg + B.g0;
+    const int jb = kb[B.l0];
+    const int je = B.l0 + B.nl < B.Lf ? kb[B.l0 + B.nl] : (int)K;
+    const int64_t fa = (ta + f) * K;
+    const int64_t obase = ((int64_t)f * Nc + c) * K;
+    double acc = 0.0;
+    for (int j = jb + tid; j < je; j += NT) {
+        const int v = tb.lk[fa + j];
+        const int l = (v >> 16) - B.l0;
+        const int kx = v & 0xFFFF;
+        const c64 ph = phx[kx] * pha[l];
+        c64 val = buf[l * LS + AX::freq_position(kx)] * (scale * ph);
+        if (ysub) val = val - ysub[obase + j];
+        out[obase + j] = val;
+        acc += (double)val.re * (double)val.re + (double)val.im * (double)val.im;
+    }
+    if (partial) {
+        const double s2 = block_sum(acc, red);
+        if (tid == 0) partial[((int64_t)f * Nc + c) * nlb + lb] = s2;
+    }
+}
+
+// Adjoint x pass alone: conj(post-phase) * y scattered into cleared lines, inverse
+// transform written as T[f][c][x][l].  Same steps as the second half of k_x_resid2.
+template <class AX, int NT>
+__global__ __launch_bounds__(NT) void k_adj_x2(
+    const c64 *__restrict__ yin, int Nc, int Ny, int LB, const c64 *__restrict__ twi,
+    LineTabs tb, int64_t K, int64_t ta, const c64 *__restrict__ phx,
+    const c64 *__restrict__ phy, const c64 *__restrict__ phz, c64 *__restrict__ Tb) {
+    constexpr int Nx = AX::N, LS = Nx + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    c64 *buf = reinterpret_cast<c64 *>(smem_raw);
+    c64 *pha = buf + LB * LS;
+    const int lb = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
+    const int tid = threadIdx.x;
+    X2Block B;
+    if (!x2_block(tb, ta, f, lb, LB, B)) return;
+    x2_phases<NT>(tb, B, Ny, phy, phz, pha, tid);
+    for (int e = tid; e < B.nl * LS; e += NT) buf[e] = c64{0.f, 0.f};
+    __syncthreads();
+    const int *kb = tb.kbeg + B.g0;
+    const int jb = kb[B.l0];
+    const int je = B.l0 + B.nl < B.Lf ? kb[B.l0 + B.nl] : (int)K;
+    const int64_t fa = (ta + f) * K;
+    const int64_t ibase = ((int64_t)f * Nc + c) * K;
+    for (int j = jb + tid; j < je; j += NT) {
+        const int v = tb.lk[fa + j];
+        const int l = (v >> 16) - B.l0;
+        const int kx = v & 0xFFFF;
+        const c64 ph = phx[kx] * pha[l];
+        buf[l * LS + AX::freq_position(kx)] = cmulc(ph, yin[ibase + j]);
+    }
+    __syncthreads();
+    c64 *dst = Tb + (int64_t)Nc * Nx * (B.g0 - tb.lptr[ta]) + (int64_t)c * Nx * B.Lf + B.l0;
+    x2_inverse<AX, NT>(buf, B.nl, twi, dst, B.Lf, tid);
 }
 
 // Adjoint pass B for runtime FFT lengths: one workgroup per (x, frame).  For each
@@ -789,11 +1024,6 @@ static void launch_adj_yz(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, 
 
 // ---- dispatch: compile-time FFT lengths for the production / test sizes in
 // FP32 with the default exponent sign, runtime lengths otherwise.
-template <int NX, int SGN>
-using FixX = FftFix<float, NX, SGN, (MAXE_X * NTX) / NX, NTX>;
-
-#define MSLR_X_SIZES(M) M(90) M(64) M(32) M(12)
-
 template <class T>
 static bool fixed_ok(mslr_sense_s *P) {
     return std::is_same<T, float>::value && P->sgn == -1 && P->use_fixed;
@@ -883,6 +1113,106 @@ static bool adj_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Tb, c64 *Pm)
     }
 }
 
+// ---- two-stage fused x passes: the compiled line lengths (prime-factor split where
+// the factors are coprime, Cooley-Tukey otherwise)
+constexpr int X2_NT = 512;
+using XAxis90 = PlaneAxis<10, 9, true>;
+using XAxis64 = PlaneAxis<8, 8, false>;
+using XAxis32 = PlaneAxis<8, 4, false>;
+using XAxis12 = PlaneAxis<4, 3, true>;
+#define MSLR_X2_AXES(M) M(90, XAxis90) M(64, XAxis64) M(32, XAxis32) M(12, XAxis12)
+
+template <class AX>
+static void launch_x_resid2(mslr_sense_s *P, int64_t ta, int nf, c64 *Tb, const c64 *ysub,
+                            double *partial) {
+    mslr_ctx_s *ctx = P->ctx;
+    dim3 g((unsigned)P->nlb, (unsigned)P->Nc, (unsigned)nf);
+    MSLR_REQUIRE((int64_t)P->LBx * AX::N <= (int64_t)MAXE_X * X2_NT, MSLR_ERR_UNSUPPORTED,
+                 "x-pass line block larger than the residual registers");
+    const size_t lds = (size_t)P->LBx * (AX::N + 1) * sizeof(c64) + (size_t)P->LBx * sizeof(c64);
+    auto go = [&](auto kern) {
+        allow_lds((const void *)kern);
+        kern<<<g, X2_NT, lds, ctx->stream>>>(Tb, P->Nc, P->N[1], P->LBx, P->nlb, P->tw_x2[0],
+                                             P->tw_x2[1], line_tabs(P), P->K, ta, P->ph32[0],
+                                             P->ph32[1], P->ph32[2], (float)P->scale, ysub,
+                                             partial);
+    };
+    MSLR_PROF(ctx, "x_resid");
+    if (P->full_lines)
+        go(k_x_resid2<AX, X2_NT, true>);
+    else
+        go(k_x_resid2<AX, X2_NT, false>);
+    HIP_LAUNCH_CHECK();
+}
+
+template <class AX>
+static void launch_fwd_x2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Tb, const c64 *ysub,
+                          c64 *out, double *partial) {
+    mslr_ctx_s *ctx = P->ctx;
+    dim3 g((unsigned)P->nlb, (unsigned)P->Nc, (unsigned)nf);
+    const size_t lds = (size_t)P->LBx * (AX::N + 1) * sizeof(c64) + (size_t)P->LBx * sizeof(c64);
+    allow_lds((const void *)k_fwd_x2<AX, X2_NT>);
+    MSLR_PROF(ctx, "fwd_x");
+    k_fwd_x2<AX, X2_NT><<<g, X2_NT, lds, ctx->stream>>>(
+        Tb, P->Nc, P->N[1], P->LBx, P->nlb, P->tw_x2[0], line_tabs(P), P->K, ta, P->ph32[0],
+        P->ph32[1], P->ph32[2], (float)P->scale, ysub, out, partial);
+    HIP_LAUNCH_CHECK();
+}
+
+template <class AX>
+static void launch_adj_x2(mslr_sense_s *P, int64_t ta, int nf, const c64 *y, c64 *Tb) {
+    mslr_ctx_s *ctx = P->ctx;
+    dim3 g((unsigned)P->nlb, (unsigned)P->Nc, (unsigned)nf);
+    const size_t lds = (size_t)P->LBx * (AX::N + 1) * sizeof(c64) + (size_t)P->LBx * sizeof(c64);
+    allow_lds((const void *)k_adj_x2<AX, X2_NT>);
+    MSLR_PROF(ctx, "adj_x");
+    k_adj_x2<AX, X2_NT><<<g, X2_NT, lds, ctx->stream>>>(y, P->Nc, P->N[1], P->LBx, P->tw_x2[1],
+                                                        line_tabs(P), P->K, ta, P->ph32[0],
+                                                        P->ph32[1], P->ph32[2], Tb);
+    HIP_LAUNCH_CHECK();
+}
+
+static bool fwd_x2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Tb, const c64 *ysub, c64 *out,
+                   double *partial) {
+    if (!P->x2) return false;
+    switch (P->N[0]) {
+#define CASE_X2(n, AX)                                            \
+    case n:                                                       \
+        launch_fwd_x2<AX>(P, ta, nf, Tb, ysub, out, partial);     \
+        return true;
+        MSLR_X2_AXES(CASE_X2)
+#undef CASE_X2
+        default: return false;
+    }
+}
+
+static bool adj_x2(mslr_sense_s *P, int64_t ta, int nf, const c64 *y, c64 *Tb) {
+    if (!P->x2) return false;
+    switch (P->N[0]) {
+#define CASE_X2(n, AX)                           \
+    case n:                                      \
+        launch_adj_x2<AX>(P, ta, nf, y, Tb);     \
+        return true;
+        MSLR_X2_AXES(CASE_X2)
+#undef CASE_X2
+        default: return false;
+    }
+}
+
+static bool x_resid2(mslr_sense_s *P, int64_t ta, int nf, c64 *Tb, const c64 *ysub,
+                     double *partial) {
+    if (!P->x2) return false;
+    switch (P->N[0]) {
+#define CASE_X2(n, AX)                                       \
+    case n:                                                  \
+        launch_x_resid2<AX>(P, ta, nf, Tb, ysub, partial);   \
+        return true;
+        MSLR_X2_AXES(CASE_X2)
+#undef CASE_X2
+        default: return false;
+    }
+}
+
 template <class T>
 static void fwd_yz_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Pm, C2<T> *Tb) {
     if constexpr (std::is_same<T, float>::value) {
@@ -896,17 +1226,7 @@ template <class T>
 static void fwd_x_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, const C2<T> *ysub,
                       C2<T> *out, double *partial) {
     if constexpr (std::is_same<T, float>::value) {
-        if (fixed_ok<T>(P)) {
-            switch (P->N[0]) {
-#define CASE_X(n)                                                                          \
-    case n:                                                                                \
-        launch_fwd_x<float>(P, ta, nf, Tb, ysub, out, partial, FixX<n, -1>{P->tw32[0]});   \
-        return;
-                MSLR_X_SIZES(CASE_X)
-#undef CASE_X
-                default: break;
-            }
-        }
+        if (fixed_ok<T>(P) && fwd_x2(P, ta, nf, Tb, ysub, out, partial)) return;
     }
     launch_fwd_x<T>(P, ta, nf, Tb, ysub, out, partial,
                     FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), P->sgn});
@@ -915,17 +1235,7 @@ static void fwd_x_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *Tb, cons
 template <class T>
 static void adj_x_any(mslr_sense_s *P, int64_t ta, int nf, const C2<T> *y, C2<T> *Tb) {
     if constexpr (std::is_same<T, float>::value) {
-        if (fixed_ok<T>(P)) {
-            switch (P->N[0]) {
-#define CASE_X(n)                                                                       \
-    case n:                                                                             \
-        launch_adj_x<float>(P, ta, nf, y, Tb, FixX<n, 1>{P->tw32[0]});                   \
-        return;
-                MSLR_X_SIZES(CASE_X)
-#undef CASE_X
-                default: break;
-            }
-        }
+        if (fixed_ok<T>(P) && adj_x2(P, ta, nf, y, Tb)) return;
     }
     launch_adj_x<T>(P, ta, nf, y, Tb, FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), -P->sgn});
 }
@@ -934,18 +1244,7 @@ template <class T>
 static void x_resid_any(mslr_sense_s *P, int64_t ta, int nf, C2<T> *Tb, const C2<T> *ysub,
                         double *partial) {
     if constexpr (std::is_same<T, float>::value) {
-        if (fixed_ok<T>(P)) {
-            switch (P->N[0]) {
-#define CASE_X(n)                                                                            \
-    case n:                                                                                  \
-        launch_x_resid<float>(P, ta, nf, Tb, ysub, partial, FixX<n, -1>{P->tw32[0]},          \
-                              FixX<n, 1>{P->tw32[0]});                                        \
-        return;
-                MSLR_X_SIZES(CASE_X)
-#undef CASE_X
-                default: break;
-            }
-        }
+        if (fixed_ok<T>(P) && x_resid2(P, ta, nf, Tb, ysub, partial)) return;
     }
     launch_x_resid<T>(P, ta, nf, Tb, ysub, partial,
                       FftRt<T, MAXE_X>{P->d[0], Dev<T>::tw(P, 0), P->sgn},
@@ -1265,6 +1564,34 @@ static void yz2_tables(mslr_sense_s *P, const std::vector<c64> &sm32,
     P->yz2 = 1;
 }
 
+// Tables of the two-stage x passes: twiddles W^(a b) of the Cooley-Tukey split in the
+// order b * N1 + a (forward and inverse sign).
+template <class AX>
+static void x2_tables(mslr_sense_s *P) {
+    for (int sgn = 0; sgn < 2; ++sgn) {
+        std::vector<c64> tl((size_t)AX::N1 * AX::N2);
+        for (int b = 0; b < AX::N2; ++b)
+            for (int a = 0; a < AX::N1; ++a) {
+                const c128 w = unit_phase((sgn == 0 ? -1 : 1) * (int64_t)a * b, AX::N);
+                tl[(size_t)b * AX::N1 + a] = {(float)w.re, (float)w.im};
+            }
+        P->tw_x2[sgn] = upload_vec(P, tl);
+    }
+    P->x2 = 1;
+}
+
+static void build_x2_tables(mslr_sense_s *P) {
+    switch (P->N[0]) {
+#define CASE_X2(n, AX)        \
+    case n:                   \
+        x2_tables<AX>(P);     \
+        break;
+        MSLR_X2_AXES(CASE_X2)
+#undef CASE_X2
+        default: break;
+    }
+}
+
 static void build_yz2_tables(mslr_sense_s *P, const std::vector<c64> &sm32,
                              const std::vector<int> &line_yz) {
     switch (P->N[1] * 10000 + P->N[2]) {
@@ -1367,6 +1694,9 @@ mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, in
             }
             P->LBx = std::max(1, std::min(lmax, (MAXE_X * NTX) / Nx));
             P->nlb = (int)cdiv(lmax, P->LBx);
+            P->full_lines = true;
+            for (int64_t t = 0; t < Nt; ++t)
+                P->full_lines = P->full_lines && K == (int64_t)Nx * (lptr[t + 1] - lptr[t]);
             P->lptr_h = lptr;
             P->lptr = upload_vec(P, lptr);
             P->line_yz = upload_vec(P, line_yz);
@@ -1417,7 +1747,10 @@ mslr_status_t mslr_sense_create(mslr_ctx ctx, const int64_t N[3], int64_t Nc, in
                     }
             P->smaps64 = upload_vec(P, sm64);
             P->smaps32 = upload_vec(P, sm32);
-            if (P->use_fixed && P->sgn == -1) build_yz2_tables(P, sm32, line_yz);
+            if (P->use_fixed && P->sgn == -1) {
+                build_yz2_tables(P, sm32, line_yz);
+                build_x2_tables(P);
+            }
             *out = P;
         } catch (...) {
             sense_free(P);

@@ -42,6 +42,10 @@ struct mslr_sense_s {
     mslr::c64 *smaps_yz2 = nullptr;    // coil maps in the y-stage-A lane order
     int *line_pos = nullptr;           // (lines): LDS offset of the line's (ky, kz)
     mslr::c64 *tw_yz2[2] = {nullptr, nullptr};  // y twiddles, forward / inverse sign
+    // two-stage fused x passes (compiled line lengths, FP32, default sign)
+    int x2 = 0;                                 // 1: tables below built
+    mslr::c64 *tw_x2[2] = {nullptr, nullptr};   // x twiddles (Cooley-Tukey split)
+    bool full_lines = false;                    // every sampled line holds all Nx samples
     std::vector<int> lptr_h; // host copy of lptr
     std::vector<void *> allocs;
 };
