@@ -112,8 +112,8 @@ mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
  *                    so the groups' memory-bound and FFT-bound kernels overlap.
  *                    Changes the summation order of the restart test's FP64 sums only.
  *   yz_threads_fwd / yz_threads_adj: threads of the compiled y-z FFT passes of
- *                    the 90 x 60 plane, 512, 768 or 1024 (built-in 512 forward, 768
- *                    adjoint).  Results are bit-identical.
+ *                    the 90 x 60 plane, 512, 768 or 1024 (built-in 768 both ways).
+ *                    Results are bit-identical.
  *   update_lines:    yz-lines per workgroup of the POGM update, which is fused with
  *                    the gradient's last transpose (plane-major -> image order): 0 =
  *                    built-in 64 (fewer where an Nx x 64 tile exceeds the LDS), else a

@@ -88,16 +88,14 @@ struct PlaneAxis {
     }
 };
 
-// The plane plan: y axis CT (spatial position = natural y, so image rows stay
-// contiguous in memory), z axis PFA; LDS row stride LY (complex values) >= NY;
-// NT threads per workgroup.  Stage butterfly counts:
+// The plane plan: y axis CT or PFA, z axis PFA; LDS row stride LY (complex values)
+// >= NY; NT threads per workgroup.  Stage butterfly counts:
 //   yA: N2y * NZ (lane u: b = u % N2y, z position pz = u / N2y)
 //   yB: N1y * NZ (lane u: a = u % N1y, pz = u / N1y)
 //   zA: N2z * NY (lane u: y position py = u % NY, b = u / NY)
 //   zB: N1z * NY (lane u: py = u % NY, a = u / NY)
 template <class AY, class AZ, int NT_>
 struct PlanePlan {
-    static_assert(!AY::PFA, "the y axis keeps its natural order (Cooley-Tukey)");
     static_assert(AZ::PFA, "the z axis uses the prime-factor map");
     using Y = AY;
     using Z = AZ;

@@ -681,12 +681,12 @@ __device__ __forceinline__ void yz2_stage_yb(c64 *buf, gc64p twl, int tid) {
         c64 w[N2Y];
 #pragma unroll
         for (int b = 0; b < N2Y; ++b) w[b] = p[b];
-        if constexpr (FWD) {
+        if constexpr (FWD && !PP::Y::PFA) {
 #pragma unroll
             for (int b = 1; b < N2Y; ++b) w[b] = w[b] * t[b * N1Y];
         }
         DftP<float, N2Y, SGN>::run(w);
-        if constexpr (!FWD) {
+        if constexpr (!FWD && !PP::Y::PFA) {
 #pragma unroll
             for (int b = 1; b < N2Y; ++b) w[b] = w[b] * t[b * N1Y];
         }
@@ -723,11 +723,11 @@ __device__ __forceinline__ void yz2_stage_zb(c64 *buf, int tid) {
     }
 }
 
-// Forward: one workgroup per (x, frame).  Per coil the y-stage-A lanes
-// read their points of the image plane (L2 hits after the first coil) and of the
-// coil map, multiply and transform; the transformed plane's sampled lines are
-// gathered into T[f][c][x][l].
-template <class PP, int SGN>
+// Forward: one workgroup per (x, frame).  Per coil the y-stage-A lanes multiply their
+// points of the image plane (held in registers across coils when each lane has one
+// butterfly, KEEP; else re-read per coil, L2 hits) by the coil map and transform; the
+// transformed plane's sampled lines are gathered into T[f][c][x][l].
+template <class PP, int SGN, bool KEEP>
 __global__ __launch_bounds__(PP::NT) void k_fwd_yz2(
     const c64 *__restrict__ Pm, const c64 *__restrict__ smp, int Nc, int Nx,
     const c64 *__restrict__ twy, const int *__restrict__ lptr, const int *__restrict__ line_pos,
@@ -745,6 +745,16 @@ __global__ __launch_bounds__(PP::NT) void k_fwd_yz2(
     const int *lpos = line_pos + g0;
     c64 *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
     const c64 *pl = Pm + ((int64_t)f * Nx + x) * NYZ;
+    // KEEP: the y-stage-A lanes' image points in registers across coils (one round)
+    c64 xk[KEEP ? N1Y : 1];
+    if constexpr (KEEP) {
+        static_assert(!KEEP || PP::RYA == 1, "image points kept for one round only");
+        if (tid < PP::UYA) {
+            const c64 *row = pl + PP::zspat(tid / N2Y) * NY;
+#pragma unroll
+            for (int a = 0; a < N1Y; ++a) xk[a] = row[PP::Y::spatial(a, tid % N2Y)];
+        }
+    }
 #pragma unroll 1
     for (int c = 0; c < Nc; ++c) {
         const gc64p sm = opaque_gptr(smp + ((int64_t)c * Nx + x) * NYZ);
@@ -753,10 +763,13 @@ __global__ __launch_bounds__(PP::NT) void k_fwd_yz2(
 #pragma unroll 1
         for (int u = tid; u < PP::UYA; u += NT) {
             const int b = u % N2Y, pz = u / N2Y;
-            const gc64p row = plc + PP::zspat(pz) * NY + b;
+            const gc64p row = plc + PP::zspat(pz) * NY;
             c64 v[N1Y];
 #pragma unroll
-            for (int a = 0; a < N1Y; ++a) v[a] = row[N2Y * a] * sm[a * PP::UYA + u];
+            for (int a = 0; a < N1Y; ++a) {
+                if constexpr (KEEP) v[a] = xk[a] * sm[a * PP::UYA + u];
+                else v[a] = row[PP::Y::spatial(a, b)] * sm[a * PP::UYA + u];
+            }
             DftP<float, N1Y, SGN>::run(v);
             c64 *p = buf + PP::lds(b, pz);
 #pragma unroll
@@ -837,9 +850,9 @@ __global__ __launch_bounds__(PP::NT) void k_adj_yz2(
         const int u = tid + r * NT;
         if (u < PP::UYA) {
             const int b = u % N2Y, pz = u / N2Y;
-            c64 *row = Pm + ((int64_t)f * Nx + x) * NYZ + PP::zspat(pz) * NY + b;
+            c64 *row = Pm + ((int64_t)f * Nx + x) * NYZ + PP::zspat(pz) * NY;
 #pragma unroll
-            for (int a = 0; a < N1Y; ++a) row[N2Y * a] = scale * acc[r][a];
+            for (int a = 0; a < N1Y; ++a) row[PP::Y::spatial(a, b)] = scale * acc[r][a];
         }
     }
 }
@@ -1029,18 +1042,21 @@ static bool fixed_ok(mslr_sense_s *P) {
     return std::is_same<T, float>::value && P->sgn == -1 && P->use_fixed;
 }
 
-// ---- two-stage y-z passes: the compiled plane plans (y split Cooley-Tukey, z split
-// prime-factor).  The 90 x 60 plane has three compiled thread counts per direction;
-// the built-in ones are the measured fastest at 90x90x60, 300 frames, 18 coils
-// (forward 512: 5.55 ms against 5.9 / 5.9 ms with 768 / 1024; adjoint 768: 5.08 ms
-// against 5.42 / 7.57 ms with 512 / 1024, whose 80 VGPRs leave one workgroup per CU).
+// ---- two-stage y-z passes: the compiled plane plans (y split prime-factor where the
+// factors are coprime, Cooley-Tukey for powers of two; z split prime-factor).  The
+// 90 x 60 plane has three compiled thread counts per direction; the built-in ones are
+// the measured fastest at 90x90x60, 300 frames, 18 coils (forward 768: 4.27 ms, its
+// y-stage-A lanes holding their image points across coils, against 5.30 ms with 512,
+// whose lanes take two butterflies and re-read the points per coil, and 4.25 ms with
+// 1024; adjoint 768: 4.82 ms against 5.30 ms with 512 and 7.0 ms with 1024, whose 83
+// VGPRs leave one workgroup per CU).
 template <int NT>
-using Plane90x60 = PlanePlan<PlaneAxis<10, 9, false>, PlaneAxis<12, 5, true>, NT>;
+using Plane90x60 = PlanePlan<PlaneAxis<10, 9, true>, PlaneAxis<12, 5, true>, NT>;
 using Plane64x40 = PlanePlan<PlaneAxis<8, 8, false>, PlaneAxis<8, 5, true>, 512>;
 using Plane32x24 = PlanePlan<PlaneAxis<8, 4, false>, PlaneAxis<8, 3, true>, 256>;
-using Plane12x10 = PlanePlan<PlaneAxis<4, 3, false>, PlaneAxis<2, 5, true>, 64>;
+using Plane12x10 = PlanePlan<PlaneAxis<4, 3, true>, PlaneAxis<2, 5, true>, 64>;
 #define MSLR_YZ2_PLANS(M) M(64, 40, Plane64x40) M(32, 24, Plane32x24) M(12, 10, Plane12x10)
-constexpr int YZ2_NT_FWD = 512, YZ2_NT_ADJ = 768;
+constexpr int YZ2_NT_FWD = 768, YZ2_NT_ADJ = 768;
 
 template <class PP>
 static void launch_fwd_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Pm, c64 *Tb) {
@@ -1049,7 +1065,7 @@ static void launch_fwd_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Pm, c
     const int64_t nwg = (int64_t)Nx * nf;
     MSLR_REQUIRE(nwg * PP::NT < ((int64_t)1 << 32), MSLR_ERR_UNSUPPORTED, "too many frames per chunk");
     MSLR_PROF(ctx, "fwd_yz");
-    k_fwd_yz2<PP, -1><<<(unsigned)nwg, PP::NT, 0, ctx->stream>>>(
+    k_fwd_yz2<PP, -1, PP::RYA == 1><<<(unsigned)nwg, PP::NT, 0, ctx->stream>>>(
         Pm, P->smaps_yz2, P->Nc, Nx, P->tw_yz2[0], P->lptr, P->line_pos, ta, nf, Tb);
     HIP_LAUNCH_CHECK();
 }
@@ -1074,9 +1090,9 @@ static bool fwd_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Pm, c64 *Tb)
     if (P->N[1] == 90 && P->N[2] == 60) {
         const int t = P->ctx->tuning.yz_threads_fwd ? P->ctx->tuning.yz_threads_fwd : YZ2_NT_FWD;
         switch (t) {
-            case 768: launch_fwd_yz2<Plane90x60<768>>(P, ta, nf, Pm, Tb); break;
+            case 512: launch_fwd_yz2<Plane90x60<512>>(P, ta, nf, Pm, Tb); break;
             case 1024: launch_fwd_yz2<Plane90x60<1024>>(P, ta, nf, Pm, Tb); break;
-            default: launch_fwd_yz2<Plane90x60<512>>(P, ta, nf, Pm, Tb); break;
+            default: launch_fwd_yz2<Plane90x60<768>>(P, ta, nf, Pm, Tb); break;
         }
         return true;
     }

@@ -32,7 +32,7 @@ SPACE: dict[str, tuple[int, ...]] = {
     "eig_lanes_limit": (0, -1, 1 << 20),
     "svst_apply": (0, 1),
     "sense_chunks": (0, 2, 4, 8),
-    "yz_threads_fwd": (0, 768, 1024),
+    "yz_threads_fwd": (0, 512, 1024),
     "yz_threads_adj": (0, 512, 1024),
     "update_lines": (0, 32, 16),
     "sb2st_schedule": (0, 1, 2, 3),
