@@ -337,14 +337,68 @@ __global__ __launch_bounds__(256) void k_prox_voxel(const c64 *img, c64 *out, in
     }
 }
 
-static unsigned prox_voxel_blocks(int64_t V) {
-    return (unsigned)std::min<int64_t>(cdiv(V, 256), 1 << 16);
+// The same with one read of the data: a workgroup of 256 threads takes 16 voxels,
+// 16 threads per voxel each holding every 16th frame of its time course in registers
+// (at most PV_T frames per thread); the squared norm is summed per thread over its
+// frames and then over the 16 threads in a fixed order.
+constexpr int PV_V = 16, PV_G = 16, PV_T = 32;
+__global__ __launch_bounds__(256) void k_prox_voxel_reg(const c64 *img, c64 *out, int64_t V,
+                                                        int Nt, const double *__restrict__ beta,
+                                                        double *__restrict__ nuc_part) {
+    __shared__ double part[PV_G][PV_V];
+    __shared__ float fks[PV_V];
+    __shared__ double red[4];
+    const double bd = (double)(float)beta[0];
+    const int vl = threadIdx.x % PV_V, g = threadIdx.x / PV_V;
+    double nacc = 0.0;
+    for (int64_t v0 = (int64_t)blockIdx.x * PV_V; v0 < V; v0 += (int64_t)gridDim.x * PV_V) {
+        const int64_t v = v0 + vl;
+        const bool ok = v < V;
+        c64 a[PV_T];
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < PV_T; ++q) {
+            const int t = g + PV_G * q;
+            a[q] = c64{0.f, 0.f};
+            if (ok && t < Nt) a[q] = img[v + V * t];
+            s += (double)a[q].re * a[q].re + (double)a[q].im * a[q].im;
+        }
+        part[g][vl] = s;
+        __syncthreads();
+        if (g == 0) {
+            double tot = 0.0;
+            for (int i = 0; i < PV_G; ++i) tot += part[i][vl];
+            const double sg = sqrt(tot);
+            float fk = 1.f;
+            if (bd != 0.0) fk = sg > bd ? (float)((sg - bd) / sg) : 0.f;
+            fks[vl] = fk;
+            if (ok) nacc += fmax(sg - bd, 0.0);
+        }
+        __syncthreads();
+        const float fk = fks[vl];
+#pragma unroll
+        for (int q = 0; q < PV_T; ++q) {
+            const int t = g + PV_G * q;
+            if (ok && t < Nt) out[v + V * t] = c64{a[q].re * fk, a[q].im * fk};
+        }
+    }
+    if (nuc_part) {
+        nacc = block_sum(nacc, red);
+        if (threadIdx.x == 0) nuc_part[blockIdx.x] = nacc;
+    }
+}
+
+static bool prox_voxel_reg(int Nt) { return Nt <= PV_G * PV_T; }
+
+static unsigned prox_voxel_blocks(int64_t V, int Nt) {
+    return prox_voxel_reg(Nt) ? (unsigned)std::min<int64_t>(cdiv(V, PV_V), 1 << 16)
+                              : (unsigned)std::min<int64_t>(cdiv(V, 256), 1 << 16);
 }
 
 // Per-unit partial count prox_scale writes for the nuclear norm of its output.
 static int64_t nuc_units(const PatchGeom &g) {
     const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
-    if (g.p == 1 && g.Np == V && g.exact_tiling()) return prox_voxel_blocks(V);
+    if (g.p == 1 && g.Np == V && g.exact_tiling()) return prox_voxel_blocks(V, g.Nt);
     return std::max<int64_t>(1, g.Np);
 }
 
@@ -375,10 +429,13 @@ static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 
                        const double *beta_dev, int *err, double *nuc_part, double *nuc_tot) {
     const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
     if (g.p == 1 && g.Np == V && g.exact_tiling()) {
-        const unsigned nb = prox_voxel_blocks(V);
+        const unsigned nb = prox_voxel_blocks(V, g.Nt);
         {
             MSLR_PROF(ctx, "prox_voxel");
-            k_prox_voxel<<<nb, 256, 0, ctx->stream>>>(img, out, V, g.Nt, beta_dev, nuc_part);
+            if (prox_voxel_reg(g.Nt))
+                k_prox_voxel_reg<<<nb, 256, 0, ctx->stream>>>(img, out, V, g.Nt, beta_dev, nuc_part);
+            else
+                k_prox_voxel<<<nb, 256, 0, ctx->stream>>>(img, out, V, g.Nt, beta_dev, nuc_part);
             HIP_LAUNCH_CHECK();
         }
         if (nuc_part) sum_partials(ctx, nuc_part, nb, nuc_tot);

@@ -508,9 +508,17 @@ __global__ __launch_bounds__(NT) void k_x_resid2(
         const int j = jb + tid + qi * NT;
         rp[qi] = -1;
         if (j < je) {
-            const int v = tb.lk[fa + j];  // (line << 16) | kx
-            const int l = (v >> 16) - B.l0;
-            const int kx = v & 0xFFFF;
+            // FULL: sample j of the block is (line (j - jb) / Nx, kx (j - jb) % Nx),
+            // no table read; else the table's (line << 16) | kx
+            int l, kx;
+            if constexpr (FULL) {
+                l = (j - jb) / Nx;
+                kx = (j - jb) - l * Nx;
+            } else {
+                const int v = tb.lk[fa + j];
+                l = (v >> 16) - B.l0;
+                kx = v & 0xFFFF;
+            }
             const c64 ph = phx[kx] * pha[l];
             const int pos = l * LS + AX::freq_position(kx);
             const c64 val = buf[pos] * (scale * ph) - ysub[obase + j];
@@ -930,11 +938,18 @@ static int transpose_tb(mslr_sense_s *P) {
     return std::max(1, std::min(64, std::min(cap, P->NYZ)));
 }
 
+// yz-lines per workgroup of the scale-sum transpose: 32 (a 90 x 33 tile, 23 KB: six
+// workgroups per CU against three with 64 lines; c1 1.21 -> 0.94 ms, 16 lines 1.03)
+template <class T>
+static int sum_planes_tb(mslr_sense_s *P) {
+    return std::min(32, transpose_tb<T>(P));
+}
+
 template <class T>
 static void launch_sum_planes(mslr_sense_s *P, const C2<T> *X, int nscales, int64_t sstride,
                               int nf, C2<T> *Pm) {
     mslr_ctx_s *ctx = P->ctx;
-    const int TB = transpose_tb<T>(P), Nx = P->N[0];
+    const int TB = sum_planes_tb<T>(P), Nx = P->N[0];
     allow_lds((const void *)k_sum_planes<T>);
     dim3 g((unsigned)cdiv(P->NYZ, TB), (unsigned)nf);
     MSLR_PROF(ctx, "sum_planes");
