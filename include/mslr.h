@@ -271,6 +271,12 @@ mslr_status_t mslr_svst_vjp(mslr_ctx ctx, const void *Z_dev, const void *Ybar_de
 /* Sum over patches of the nuclear norms (patch_nucnorm, src/recon.jl:138-146). */
 mslr_status_t mslr_patch_nucnorm(mslr_ctx ctx, const void *P_dev, int64_t p, int64_t Nt,
                                  int64_t Np, double *out);
+/* Weighted form (patch_nucnorm(P, lambdas), src/recon.jl:157-166): the mean over the
+ * Np patches of lambdas_host[i] * ||P_i||_*, a patch with no nonzero singular value
+ * adding 0; all patches' nuclear norms in one batched pass.  Np >= 1. */
+mslr_status_t mslr_patch_nucnorm_weighted(mslr_ctx ctx, const void *P_dev, int64_t p,
+                                          int64_t Nt, int64_t Np, const double *lambdas_host,
+                                          double *out);
 /* The in-place prox of run_recon over all scales (g_prox, scripts/reconstruct.jl:196-202):
  * X[..., k] <- patchSVST(X[..., k], beta[k], patch_sizes[k], strides[k]) for the 5D
  * ComplexF32 array X (N[0], N[1], N[2], Nt, nscales); beta_host[k] = c * lambda_k.

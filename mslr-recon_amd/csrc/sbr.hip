@@ -131,7 +131,8 @@ __device__ __forceinline__ c128 ld_if(bool c, const c128 *p) {
 // waves' partials of Z in global memory (L2); at most 128 VGPRs (four waves per SIMD).
 __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m, int npanel,
                                                   c128 *__restrict__ Tall,
-                                                  c128 *__restrict__ W1all) {
+                                                  c128 *__restrict__ W1all, int hi) {
+    if (hi) __builtin_amdgcn_s_setprio(3);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ c128 Ts[B][B + 1];
     __shared__ c128 Ms[B][B + 1];
@@ -622,7 +623,9 @@ template <int NT, bool GBAND, int STG>
 __global__ __launch_bounds__(NT) void k_sb2st(const c128 *__restrict__ Gall, int m, int ks,
                                               double *__restrict__ dall, double *__restrict__ eall,
                                               c128 *__restrict__ V2all,
-                                              c128 *__restrict__ tau2all, c128 *__restrict__ Bglob) {
+                                              c128 *__restrict__ tau2all, c128 *__restrict__ Bglob,
+                                              int hi) {
+    if (hi) __builtin_amdgcn_s_setprio(3);
     constexpr int NW = NT / 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int ip = blockIdx.x;
@@ -694,7 +697,8 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
                                                 const c128 *__restrict__ tau2all,
                                                 const double *__restrict__ Zall,
                                                 const int *__restrict__ rall,
-                                                c64 *__restrict__ Uall) {
+                                                c64 *__restrict__ Uall, int hi) {
+    if (hi) __builtin_amdgcn_s_setprio(3);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ c128 Ts[B][B + 1];
     // two sweeps' reflectors (stride BS: room for every thread's two values) / two
@@ -944,11 +948,12 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
                  "sweep longer than the schedule");
     MSLR_REQUIRE(!gband || w.Band, MSLR_ERR_DEVICE, "no band workspace");
     const int npanel = sbr_panels(m), ks = sbr_steps(m);
+    const int hi = svst_small_batch(nb);
     {
         const size_t lds = (size_t)(m - B) * B * sizeof(c128);
         raise_lds((const void *)k_sy2sb, lds);
         MSLR_PROF(ctx, "sy2sb");
-        k_sy2sb<<<(unsigned)nb, S1T, lds, ctx->stream>>>(G, m, npanel, w.T1, w.W1);
+        k_sy2sb<<<(unsigned)nb, S1T, lds, ctx->stream>>>(G, m, npanel, w.T1, w.W1, hi);
         HIP_LAUNCH_CHECK();
     }
     if (gband) {
@@ -959,7 +964,7 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
             const size_t lds = (size_t)(nt / 64) * 3 * B * sizeof(c128);
             raise_lds((const void *)kern, lds);
             MSLR_PROF(ctx, "sb2st");
-            kern<<<(unsigned)nb, nt, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2, w.Band);
+            kern<<<(unsigned)nb, nt, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2, w.Band, hi);
             HIP_LAUNCH_CHECK();
         };
         switch (ctx->tuning.sb2st_schedule) {
@@ -973,7 +978,7 @@ void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, d
         raise_lds((const void *)k_sb2st<S2T, false, STAGGER>, lds);
         MSLR_PROF(ctx, "sb2st");
         k_sb2st<S2T, false, STAGGER><<<(unsigned)nb, S2T, lds, ctx->stream>>>(G, m, ks, d, e, w.V2, w.tau2,
-                                                                      nullptr);
+                                                                      nullptr, hi);
         HIP_LAUNCH_CHECK();
     }
 }
@@ -986,11 +991,12 @@ void sbr_backtransform(mslr_ctx_s *ctx, const c128 *G, int m, int64_t nb, const 
     const bool small = ks <= BT_KS_LDS;
     raise_lds(small ? (const void *)k_sbr_bt<BT_KS_LDS> : (const void *)k_sbr_bt<BT_KS_MAX>, lds);
     dim3 g((unsigned)nb, (unsigned)cdiv(m, BTC));
+    const int hi = svst_small_batch(nb);
     MSLR_PROF(ctx, "backtransform");
     if (small)
-        k_sbr_bt<BT_KS_LDS><<<g, BTT, lds, ctx->stream>>>(G, m, npanel, ks, w.T1, w.V2, w.tau2, Z, r, U);
+        k_sbr_bt<BT_KS_LDS><<<g, BTT, lds, ctx->stream>>>(G, m, npanel, ks, w.T1, w.V2, w.tau2, Z, r, U, hi);
     else
-        k_sbr_bt<BT_KS_MAX><<<g, BTT, lds, ctx->stream>>>(G, m, npanel, ks, w.T1, w.V2, w.tau2, Z, r, U);
+        k_sbr_bt<BT_KS_MAX><<<g, BTT, lds, ctx->stream>>>(G, m, npanel, ks, w.T1, w.V2, w.tau2, Z, r, U, hi);
     HIP_LAUNCH_CHECK();
 }
 

@@ -51,6 +51,12 @@ inline size_t sbr_bytes_per_matrix(int m) {
 
 // G (nb matrices, m x m, full Hermitian storage) -> d, e (real tridiagonal).
 // G keeps the panel reflectors below the band.
+// Batches of at most this many matrices (the whole-image patch) run their eigensolver
+// kernels at raised wave priority: their latency is the cost, and the batched kernels
+// of the other scales share the CUs with them.
+constexpr int64_t SVST_SMALL_BATCH = 8;
+inline int svst_small_batch(int64_t nb) { return nb <= SVST_SMALL_BATCH ? 1 : 0; }
+
 void sbr_reduce(mslr_ctx_s *ctx, c128 *G, int m, int64_t nb, const SbrWork &w, double *d,
                 double *e);
 

@@ -730,6 +730,68 @@ __global__ __launch_bounds__(64) void k_eig_count(const double *__restrict__ dal
     rall[ip] = r;
 }
 
+// k_eig_count for small batches: one 64-thread workgroup per patch.  d and e are
+// staged in LDS by the wave, the Gershgorin bounds are wave reductions (max / min:
+// the same values in any order) and lane 0 takes the Sturm count from LDS, so a
+// lone patch waits for no chain of dependent global loads; bit-identical results.
+// hi: raise the waves' issue priority (the single matrix of a whole-image patch
+// runs beside the batched kernels of the other scales, and its latency is the cost).
+__device__ __forceinline__ double wave_max64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__global__ __launch_bounds__(64) void k_eig_count_wave(const double *__restrict__ dall,
+                                                       const double *__restrict__ eall, int m,
+                                                       const double *__restrict__ beta,
+                                                       int beta_per_patch,
+                                                       double *__restrict__ info,
+                                                       int *__restrict__ rall, int all, int hi) {
+    if (hi) __builtin_amdgcn_s_setprio(3);
+    extern __shared__ __align__(16) unsigned char eig_smem[];
+    double *d = reinterpret_cast<double *>(eig_smem);
+    double *e = d + m;
+    const int64_t ip = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float b32 = (float)beta[beta_per_patch ? ip : 0];
+    const double bd = (double)b32;
+    if (bd == 0.0 && !all) {
+        if (lane == 0) rall[ip] = -1;
+        return;
+    }
+    for (int i = lane; i < m; i += 64) {
+        d[i] = dall[ip * m + i];
+        e[i] = eall[ip * m + i];
+    }
+    __syncthreads();
+    double tn = 0.0, emax = 0.0, gl = INFINITY, gu = -INFINITY;
+    for (int i = lane; i < m; i += 64) {
+        const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
+        const double er = i + 1 < m ? fabs(e[i]) : 0.0;
+        tn = fmax(tn, fabs(d[i]) + el + er);
+        gl = fmin(gl, d[i] - el - er);
+        gu = fmax(gu, d[i] + el + er);
+        if (i + 1 < m) emax = fmax(emax, e[i] * e[i]);
+    }
+    tn = wave_max64(tn);
+    emax = wave_max64(emax);
+    gl = -wave_max64(-gl);
+    gu = wave_max64(gu);
+    if (lane != 0) return;
+    const double tau = bd * bd;
+    const double pivmin = DBL_MIN * fmax(1.0, emax);
+    const double pad = 2.0 * DBL_EPSILON * fmax(fabs(gl), fabs(gu)) + pivmin;
+    gu = gu + pad;
+    int r = 0;
+    if (all) r = m;
+    else if (tau < gu) r = m - sturm_count(d, e, m, tau, pivmin);
+    double *inf = info + ip * EIG_INFO;
+    inf[0] = tn;
+    inf[1] = pivmin;
+    inf[2] = gu;
+    inf[3] = all ? gl - pad : tau;
+    rall[ip] = r;
+}
+
 // grid (patches, cdiv(m * G, 64)); a group of G lanes per eigenvalue k < r.
 // lam[k] is the k-th largest eigenvalue, found by multisection: each step the
 // G lanes take Sturm counts at G interior points of [lo, hi] and the group keeps
@@ -746,8 +808,9 @@ __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ d
                                                     const int *__restrict__ rall,
                                                     double *__restrict__ lamall,
                                                     double *__restrict__ Zall,
-                                                    c64 *__restrict__ Uall, int k0) {
+                                                    c64 *__restrict__ Uall, int k0, int hiprio) {
     static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "group size");
+    if (hiprio) __builtin_amdgcn_s_setprio(3);
     const int ip = blockIdx.x;
     const int r = rall[ip];
     if (k0 + (int)(blockIdx.y * 64) / G >= r) return;  // no wanted eigenvalue here
@@ -2209,8 +2272,13 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         {
             const double *bp = beta_dev + (per_patch ? ip0 : 0);
             MSLR_PROF(ctx, "eigvec");
-            k_eig_count<<<(unsigned)cdiv(nb, 64), 64, 0, ctx->stream>>>(w.d, w.e, (int)m, nb, bp,
-                                                                         per_patch, w.info, w.r, 0);
+            const int hi = svst_small_batch(nb);
+            if (hi)
+                k_eig_count_wave<<<(unsigned)nb, 64, 2 * m * sizeof(double), ctx->stream>>>(
+                    w.d, w.e, (int)m, bp, per_patch, w.info, w.r, 0, hi);
+            else
+                k_eig_count<<<(unsigned)cdiv(nb, 64), 64, 0, ctx->stream>>>(w.d, w.e, (int)m, nb, bp,
+                                                                             per_patch, w.info, w.r, 0);
             HIP_LAUNCH_CHECK();
             // lanes per eigenvalue: many when the eigenvalues are few (the bisection
             // latency is the cost), few when the batch already fills the device.
@@ -2222,11 +2290,11 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
             if (nb * m <= lim) {
                 dim3 ge((unsigned)nb, (unsigned)cdiv(m * 16, 64));
                 k_eig_vectors<16><<<ge, 64, 3 * m * sizeof(double), ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
-                                                              w.info, w.r, w.lam, w.Z, w.U, 0);
+                                                              w.info, w.r, w.lam, w.Z, w.U, 0, hi);
             } else {
                 dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
                 k_eig_vectors<4><<<ge, 64, 3 * m * sizeof(double), ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
-                                                             w.info, w.r, w.lam, w.Z, w.U, 0);
+                                                             w.info, w.r, w.lam, w.Z, w.U, 0, hi);
             }
             HIP_LAUNCH_CHECK();
             k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, (int)m, bp, per_patch,
@@ -2386,7 +2454,7 @@ void svst_vjp_batched(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int
             HIP_LAUNCH_CHECK();
             dim3 ge((unsigned)nb, (unsigned)cdiv(m * 4, 64));
             k_eig_vectors<4><<<ge, 64, 3 * m * sizeof(double), ctx->stream>>>(w.d, w.e, mi, bp, per_patch, w.info, w.r,
-                                                         w.lam, w.Z, w.U, 0);
+                                                         w.lam, w.Z, w.U, 0, 0);
             HIP_LAUNCH_CHECK();
             k_eig_clusters<<<(unsigned)nb, NSLOT, 0, ctx->stream>>>(w.d, w.e, mi, bp, per_patch,
                                                                     w.info, w.r, w.lam, w.Z, w.ws,
@@ -2531,15 +2599,11 @@ void svst_vjp_any(mslr_ctx_s *ctx, const c64 *Z, const c64 *Yb, c64 *Zb, int64_t
     conj_transpose(ctx, ZbT, Zb, Nt, p, Np);
 }
 
-void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,
-                     double *out_dev) {
-    if (Np == 0) {
-        HIP_CHECK(hipMemsetAsync(out_dev, 0, sizeof(double), ctx->stream));
-        return;
-    }
+// Nuclear norm of every patch into per (device, Np doubles).
+static void nucnorm_per_patch(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,
+                              double *per) {
     const int64_t m = std::min(p, Nt);
     MSLR_REQUIRE(m <= MAXM, MSLR_ERR_UNSUPPORTED, "min(patch voxels, Nt) too large");
-    double *per = (double *)ctx->scratch_get(SLOT_EIG, (size_t)(Np + 1) * sizeof(double));
     if (m == 1) {
         k_m1_nucnorm<<<(unsigned)cdiv(Np, 4), 256, 0, ctx->stream>>>(P, p * Nt, Np, per);
         HIP_LAUNCH_CHECK();
@@ -2553,6 +2617,16 @@ void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64
             HIP_LAUNCH_CHECK();
         }
     }
+}
+
+void nucnorm_batched(mslr_ctx_s *ctx, const c64 *P, int64_t p, int64_t Nt, int64_t Np,
+                     double *out_dev) {
+    if (Np == 0) {
+        HIP_CHECK(hipMemsetAsync(out_dev, 0, sizeof(double), ctx->stream));
+        return;
+    }
+    double *per = (double *)ctx->scratch_get(SLOT_EIG, (size_t)(Np + 1) * sizeof(double));
+    nucnorm_per_patch(ctx, P, p, Nt, Np, per);
     sum_partials(ctx, per, Np, out_dev);
 }
 
@@ -2634,6 +2708,27 @@ mslr_status_t mslr_patch_nucnorm(mslr_ctx ctx, const void *P_dev, int64_t p, int
         nucnorm_batched(c, (const c64 *)P_dev, p, Nt, Np, od);
         HIP_CHECK(hipMemcpyAsync(out, od, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+mslr_status_t mslr_patch_nucnorm_weighted(mslr_ctx ctx, const void *P_dev, int64_t p, int64_t Nt,
+                                          int64_t Np, const double *lambdas_host, double *out) {
+    return guarded([&] {
+        mslr_ctx_s *c = check_ctx(ctx);
+        MSLR_REQUIRE(out != nullptr && (lambdas_host != nullptr || Np == 0), MSLR_ERR_ARG,
+                     "out or lambdas is null");
+        MSLR_REQUIRE(p >= 1 && Nt >= 1 && Np >= 1, MSLR_ERR_ARG, "bad sizes");
+        double *per = (double *)c->scratch_get(SLOT_EIG, (size_t)(Np + 1) * sizeof(double));
+        nucnorm_per_patch(c, (const c64 *)P_dev, p, Nt, Np, per);
+        std::vector<double> h((size_t)Np);
+        HIP_CHECK(hipMemcpyAsync(h.data(), per, (size_t)Np * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        // a patch whose largest singular value is 0 (nuclear norm 0) adds nothing;
+        // the mean over all patches, in patch order
+        double tot = 0.0;
+        for (int64_t i = 0; i < Np; ++i) tot += h[(size_t)i] > 0.0 ? lambdas_host[i] * h[(size_t)i] : 0.0;
+        *out = tot / (double)Np;
     });
 }
 

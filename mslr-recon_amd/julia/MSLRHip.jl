@@ -337,6 +337,17 @@ function patch_nucnorm_hip(P::HipArray{ComplexF32,3})
     return r[]
 end
 
+# patch_nucnorm(P, λs) (src/recon.jl:157-166): one batched pass over all patches
+function patch_nucnorm_hip(P::HipArray{ComplexF32,3}, λs::Vector)
+    @assert length(λs) == P.dims[3] "need one weight per patch"
+    r = Ref{Cdouble}(0)
+    lam = Vector{Cdouble}(λs)
+    check(ccall((:mslr_patch_nucnorm_weighted, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cdouble}, Ref{Cdouble}),
+                P.ctx.h, P.ptr, P.dims[1], P.dims[2], P.dims[3], lam, r))
+    return r[]
+end
+
 # ---------------------------------------------------------------- POGM (src/mirt_mod.jl:74-188)
 struct PogmDesc
     N::NTuple{3,Int64}

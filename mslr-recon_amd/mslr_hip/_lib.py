@@ -137,6 +137,7 @@ SIGNATURES: dict[str, tuple] = {
     "mslr_patches2img": (C.c_int, [_p, _p, _pi64, _i64, _pi64, _pi64, _p]),
     "mslr_svst_batched": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, C.c_int]),
     "mslr_patch_nucnorm": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd]),
+    "mslr_patch_nucnorm_weighted": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd]),
     "mslr_svst_vjp": (C.c_int, [_p, _p, _p, _p, _i64, _i64, _i64, _pd, C.c_int, _pd]),
     "mslr_tsnr": (C.c_int, [_p, _p, _i64, _i64, _p]),
     "mslr_normalise_smaps": (C.c_int, [_p, _p, C.POINTER(C.c_int64), _i64]),

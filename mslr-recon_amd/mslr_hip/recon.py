@@ -162,13 +162,11 @@ def patch_nucnorm(P, lambdas=None, ctx: Context | None = None) -> float:
         out = C.c_double()
         _lib.check(ctx._lib.mslr_patch_nucnorm(ctx.handle, d.ptr, p, Nt, Np, C.byref(out)))
         return out.value
-    lam = np.asarray(lambdas, dtype=np.float64)
+    lam = np.ascontiguousarray(lambdas, dtype=np.float64)
     if lam.size != Np:
         raise AssertionError("need one weight per patch")
-    tot = 0.0
-    for i in range(Np):
-        out = C.c_double()
-        view = d.offset_view(i * p * Nt, (p, Nt, 1))
-        _lib.check(ctx._lib.mslr_patch_nucnorm(ctx.handle, view.ptr, p, Nt, 1, C.byref(out)))
-        tot += lam[i] * out.value if out.value > 0 else 0.0
-    return tot / Np
+    out = C.c_double()
+    _lib.check(ctx._lib.mslr_patch_nucnorm_weighted(ctx.handle, d.ptr, p, Nt, Np,
+                                                    lam.ctypes.data_as(C.POINTER(C.c_double)),
+                                                    C.byref(out)))
+    return out.value

@@ -157,6 +157,22 @@ def test_patch_nucnorm(ctx, rng):
     assert abs(v - vo) / vo < 1e-5
 
 
+def test_patch_nucnorm_weighted(ctx, rng):
+    """patch_nucnorm(P, lambdas) (src/recon.jl:157-166) in one batched pass: the mean
+    over patches of lambda_i ||P_i||_*, a zero patch adding nothing."""
+    from mslr_hip import recon
+    P = np.stack([lowrank(rng, 40, 15, 2) for _ in range(5)], axis=2)
+    P[:, :, 2] = 0
+    P = np.asfortranarray(P)
+    lams = [0.5, 2.0, 7.0, 1.25, 3.0]
+    v = recon.patch_nucnorm(P, lams)
+    vo = O.patch_nucnorm(P.astype(np.complex128), lams)
+    assert abs(v - vo) / vo < 1e-5
+    # the same as the per-patch sums weighted on the host
+    per = [O.patch_nucnorm(P[:, :, i:i + 1].astype(np.complex128)) for i in range(5)]
+    assert abs(v - sum(l * n for l, n in zip(lams, per)) / 5) / vo < 1e-5
+
+
 @pytest.mark.parametrize("N,ps,ss", [((12, 12, 10), (4, 4, 4), (4, 4, 4)),
                                      ((12, 12, 10), (6, 6, 6), (3, 3, 3)),
                                      ((12, 12, 10), (12, 12, 10), (12, 12, 10)),
