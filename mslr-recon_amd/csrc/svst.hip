@@ -1340,8 +1340,10 @@ __global__ __launch_bounds__(256) void k_split_gather(const c64 *__restrict__ im
         const c64 v0 = lane < n ? img[o0 + t * V] : c64{0.f, 0.f};
         const c64 v1 = lane + 64 < n ? img[o1 + t * V] : c64{0.f, 0.f};
         const int64_t off = (z * Nt + t) * p + blk * SPL_ROWS;
-        if (lane < n) Xc[off + lane] = v0;
-        if (lane + 64 < n) Xc[off + lane + 64] = v1;
+        if (Xc) {
+            if (lane < n) Xc[off + lane] = v0;
+            if (lane + 64 < n) Xc[off + lane + 64] = v1;
+        }
         split_block(v0, v1, lane, n, Wh + off, Wl + off);
     }
 }
@@ -1683,8 +1685,13 @@ __global__ __launch_bounds__(256) void k_apply_bsplit(const c64 *__restrict__ B,
     Wl[(int64_t)z * bW + e] = l;
 }
 
-template <int NB>
-__global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A, int64_t bA,
+// AW: A arrives as the split pass's Ozaki words (wa: s1, then s2 at wa + wlo, same
+// element offsets as A), staged into LDS as they are; else A is FP32 and split per
+// element when staged.
+template <int NB, bool AW>
+__global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A,
+                                                       const unsigned *__restrict__ wa,
+                                                       int64_t wlo, int64_t bA,
                                                        int64_t ak, const unsigned *__restrict__ Wh,
                                                        const unsigned *__restrict__ Wl, int64_t bW,
                                                        int Kp, c64 *__restrict__ C, int64_t bC,
@@ -1710,16 +1717,24 @@ __global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A
         if (!rank_select(rs, z, Nz, Kz)) continue;  // uniform per workgroup
         const bool av = i0 + ar < M;
         const c64 *pa = A + (int64_t)z * bA + i0 + ar;
+        const unsigned *pw = wa + (int64_t)z * bA + i0 + ar;
         const unsigned *wh = Wh + (int64_t)z * bW, *wl = Wl + (int64_t)z * bW;
         c64 *Cz = scat ? C : C + (int64_t)z * bC;
         // one k stage: the A tile stays in LDS for all the column tiles
         const bool one = Kz <= BKS;
-        c64 ra[8];
+        c64 ra[AW ? 1 : 8];
+        unsigned rwh[AW ? 8 : 1], rwl[AW ? 8 : 1];
         auto fetch = [&](int k0) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int k = k0 + akq + e;
-                ra[e] = (av && k < Kz) ? pa[(int64_t)k * ak] : c64{0.f, 0.f};
+                const bool ok = av && k < Kz;
+                if constexpr (AW) {
+                    rwh[e] = ok ? pw[(int64_t)k * ak] : 0u;
+                    rwl[e] = ok ? pw[wlo + (int64_t)k * ak] : 0u;
+                } else {
+                    ra[e] = ok ? pa[(int64_t)k * ak] : c64{0.f, 0.f};
+                }
             }
         };
         for (int j0 = 0; j0 < Nz; j0 += TN) {
@@ -1748,7 +1763,16 @@ __global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A
             if (sa) fetch(0);
             for (int k0 = 0; k0 < Kz; k0 += BKS) {
                 fetch_b(k0);
-                if (sa) stage8(ra, &Ah[ar][akq], &Al[ar][akq]);
+                if (sa) {
+                    if constexpr (AW) {
+                        *(u4 *)&Ah[ar][akq] = u4{rwh[0], rwh[1], rwh[2], rwh[3]};
+                        *(u4 *)&Ah[ar][akq + 4] = u4{rwh[4], rwh[5], rwh[6], rwh[7]};
+                        *(u4 *)&Al[ar][akq] = u4{rwl[0], rwl[1], rwl[2], rwl[3]};
+                        *(u4 *)&Al[ar][akq + 4] = u4{rwl[4], rwl[5], rwl[6], rwl[7]};
+                    } else {
+                        stage8(ra, &Ah[ar][akq], &Al[ar][akq]);
+                    }
+                }
 #pragma unroll
                 for (int q = 0; q < BQ; ++q) {
                     const int f = tid + 256 * q;
@@ -2137,11 +2161,13 @@ static void cgemm_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
 // conjugated when conjB) split once into hi / lo words, then k_apply_bf16.
 // scatter (nullable): C is the image of an exact tiling's patches, batch entry z
 // being patch sip0 + z (cj = V, the frame stride).
+// Aw (nullable): A as Ozaki words (s1 at Aw, s2 at Aw + Awlo, A's element offsets)
+// instead of the FP32 A.
 static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t bA, int64_t ak,
                        const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB, const float *fs,
                        int64_t bfs, c64 *C, int64_t bC, int64_t cj, int M, int N, int K,
                        int64_t nbatch, RankSel rs, const PatchGeom *scatter = nullptr,
-                       int64_t sip0 = 0) {
+                       int64_t sip0 = 0, const unsigned *Aw = nullptr, int64_t Awlo = 0) {
     if (M <= 0 || N <= 0 || K <= 0 || nbatch <= 0) return;
     const int Kp = (int)cdiv(K, BKS) * BKS;
     const int64_t bW = (int64_t)N * Kp;
@@ -2159,20 +2185,26 @@ static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
         HIP_LAUNCH_CHECK();
         const int64_t units = (int64_t)nz * cdiv(M, AP_ROWS);
         const unsigned g = (unsigned)std::min<int64_t>(units, 4 * (int64_t)std::max(ctx->num_cus, 1));
-        const c64 *Az = A + z0 * bA;
+        const unsigned *Awz = Aw ? Aw + z0 * bA : nullptr;
+        // (with Aw the kernel reads no FP32 A; any valid address stands in)
+        const c64 *Az = A ? A + z0 * bA : reinterpret_cast<const c64 *>(Awz);
         c64 *Cz = scatter ? C : C + z0 * bC;
         const PatchGeom sg = scatter ? *scatter : PatchGeom{};
         const int64_t s0 = sip0 + z0;
         const int sc = scatter ? 1 : 0;
-        if (N > 64)
-            k_apply_bf16<5><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
-                                                        N, K, nz, rz, sg, s0, sc);
-        else if (N > 32)
-            k_apply_bf16<2><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
-                                                        N, K, nz, rz, sg, s0, sc);
-        else
-            k_apply_bf16<1><<<g, 256, 0, ctx->stream>>>(Az, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M,
-                                                        N, K, nz, rz, sg, s0, sc);
+        auto go = [&](auto kern) {
+            kern<<<g, 256, 0, ctx->stream>>>(Az, Awz, Awlo, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M, N,
+                                             K, nz, rz, sg, s0, sc);
+        };
+        if (Aw) {
+            if (N > 64) go(k_apply_bf16<5, true>);
+            else if (N > 32) go(k_apply_bf16<2, true>);
+            else go(k_apply_bf16<1, true>);
+        } else {
+            if (N > 64) go(k_apply_bf16<5, false>);
+            else if (N > 32) go(k_apply_bf16<2, false>);
+            else go(k_apply_bf16<1, false>);
+        }
         HIP_LAUNCH_CHECK();
     }
 }
@@ -2215,8 +2247,13 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
                  MSLR_ERR_ARG, "svst_batched scatters only an exact tiling's patches");
     MSLR_REQUIRE(!gather || (direct && gather->p == p && gather->Nt == Nt && gather->Np == Np),
                  MSLR_ERR_ARG, "svst_batched gathers patches only for split products with p >= Nt");
-    // the products read an FP32 copy of the patches, written by the split pass
-    const bool copy = in_place || gather;
+    // With split products and p >= Nt, the products that read X (the dense X W and
+    // X U_r) take the split pass's Ozaki words as their left operand (x = s1 + s2 to
+    // 2^-17 of the block maximum) when both run split; else they read an FP32 copy
+    // of the patches, written by the split pass, where Y overwrites X.
+    const bool words_a = ctx->products == MSLR_PRODUCTS_SPLIT_BF16 && p >= Nt &&
+                         (apply_fp32_mask(ctx) & 3) == 0;
+    const bool copy = (in_place || gather) && !words_a;
     const int64_t len = p * Nt;
     if (m == 1) {
         HIP_CHECK(hipMemcpyAsync(Pout, Pin, (size_t)(len * Np) * sizeof(c64),
@@ -2330,15 +2367,18 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         // splits per tile (cgemm_bf16).
         const int amask = apply_fp32_mask(ctx);
         // out: a product writing Y (scattered into the image with scatter)
+        // xw: the left operand is X, taken as the split words when words_a
         auto prod = [&](int bit, bool out, const char *tag, const c64 *A, int64_t bA, int64_t ai,
                         int64_t ak, const c64 *B, int64_t bB, int64_t bk, int64_t bj, int conjB,
                         const float *fs, int64_t bfs, c64 *C, int64_t bC, int64_t ci, int64_t cj,
-                        int M, int N, int K, RankSel rs) {
+                        int M, int N, int K, RankSel rs, bool xw = false) {
             if (split && !(amask & bit) && !left) {
                 const bool sc = out && scatter;
+                const bool aw = xw && words_a;
                 apply_bf16(ctx, tag, A, bA, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC,
                            sc ? (int64_t)scatter->N[0] * scatter->N[1] * scatter->N[2] : cj, M, N,
-                           K, nb, rs, sc ? scatter : nullptr, ip0);
+                           K, nb, rs, sc ? scatter : nullptr, ip0, aw ? Wh : nullptr,
+                           aw ? len * nb : 0);
             } else if (split && !(amask & bit)) {
                 cgemm_bf16(ctx, tag, A, bA, ai, ak, B, bB, bk, bj, conjB, fs, bfs, C, bC, ci, cj, M,
                            N, K, nb, rs);
@@ -2351,7 +2391,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         if (!left) {
             // Y (p x m) = X (p x m) W
             prod(1, true, "apply", Pa, len, 1, p, Wm, m * m, 1, m, 0, nullptr, 0, Yc, len, 1, p,
-                 (int)p, mi, mi, dense);
+                 (int)p, mi, mi, dense, true);
         } else {
             // Y (p x Nt) = W (p x p) X
             prod(1, true, "apply", Wm, m * m, 1, m, Pa, len, 1, p, 0, nullptr, 0, Yc, len, 1, p,
@@ -2363,7 +2403,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
             c64 *T = (c64 *)ctx->scratch_get(SLOT_LOWRANK, (size_t)(bT * nb) * sizeof(c64));
             const RankSel s1{w.r, mi, 2}, s2{w.r, mi, 3};
             prod(2, false, "apply_xu", Pa, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1, p,
-                 (int)p, half, mi, s1);
+                 (int)p, half, mi, s1, true);
             prod(4, true, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p,
                  (int)p, mi, half, s2);
         }

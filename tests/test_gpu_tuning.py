@@ -120,9 +120,13 @@ def test_autotune_pogm(shapes, tmp_path):
     r = autotune_pogm(A, ksp, form="mixed", warmup=1, iters=2, cache=cache, **kw)
     cands = [c for c in descent_order() if relevant(c[0], A.N, A.Nt, cfg["patch_sizes"])]
     assert not r["cached"] and len(r["trials"]) == 1 + len(cands)
-    # every candidate shape gives the built-in iterate to rounding
+    # every candidate shape gives the built-in iterate to rounding; the dense-W and
+    # FP32 applies round their products differently from the built-in apply (which
+    # multiplies the split pass's block-exponent words): both stay within the split
+    # form's bound of the FP64 result, so they differ by at most about twice it
     for t, ms, err, ok in r["trials"]:
-        assert ms > 0 and err < 1e-5, (t, err)
+        tol = 5e-5 if t["svst_apply"] or t["apply_fp32"] else 1e-5
+        assert ms > 0 and err < tol, (t, err)
     assert shapes.tuning() == r["tuning"]
     kept = [t for t, _, _, ok in r["trials"][1:] if ok]
     assert r["tuning"] == (dict(BUILTIN, **kept[-1]) if kept else BUILTIN)
@@ -141,7 +145,8 @@ def test_autotune_pogm(shapes, tmp_path):
         st.close()
     A.close()
     assert out[0][1] == out[1][1]
-    assert rel_err(out[0][0], out[1][0]) < 1e-5
+    tol = 5e-5 if r["tuning"]["svst_apply"] or r["tuning"]["apply_fp32"] else 1e-5
+    assert rel_err(out[0][0], out[1][0]) < tol
 
 
 @pytest.mark.parametrize("form", ["plain", "batched"])
