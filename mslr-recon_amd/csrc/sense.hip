@@ -716,11 +716,17 @@ __device__ __forceinline__ void yz2_stage_za(c64 *buf, int tid) {
         for (int a = 0; a < N1Z; ++a) p[PP::LY * N2Z * a] = w[a];
     }
 }
+
+// z stage B over the butterflies of a list only (the frame's butterflies that hold a
+// sampled line): the forward gathers nothing else from the last stage, and in the
+// adjoint every other butterfly's inputs are zero, so its outputs (zero) are already
+// in place.
 template <class PP, int SGN>
-__device__ __forceinline__ void yz2_stage_zb(c64 *buf, int tid) {
+__device__ __forceinline__ void yz2_stage_zb_list(c64 *buf, int tid, const int *list, int cnt) {
     constexpr int NY = PP::NY, N2Z = PP::Z::N2;
 #pragma unroll 1
-    for (int u = tid; u < PP::UZB; u += PP::NT) {
+    for (int i = tid; i < cnt; i += PP::NT) {
+        const int u = list[i];
         c64 *p = buf + PP::lds(u % NY, N2Z * (u / NY));
         c64 w[N2Z];
 #pragma unroll
@@ -739,7 +745,8 @@ template <class PP, int SGN, bool KEEP>
 __global__ __launch_bounds__(PP::NT) void k_fwd_yz2(
     const c64 *__restrict__ Pm, const c64 *__restrict__ smp, int Nc, int Nx,
     const c64 *__restrict__ twy, const int *__restrict__ lptr, const int *__restrict__ line_pos,
-    int64_t ta, int nf, c64 *__restrict__ Tb) {
+    const int *__restrict__ zb_list, const int *__restrict__ zb_cnt, int64_t ta, int nf,
+    c64 *__restrict__ Tb) {
     constexpr int NT = PP::NT, NY = PP::NY, NYZ = PP::NYZ;
     constexpr int N1Y = PP::Y::N1, N2Y = PP::Y::N2;
     __shared__ c64 buf[PP::LY * PP::NZ];
@@ -751,6 +758,8 @@ __global__ __launch_bounds__(PP::NT) void k_fwd_yz2(
     const int g0 = lptr[ta + f];
     const int Lf = lptr[ta + f + 1] - g0;
     const int *lpos = line_pos + g0;
+    const int *zbl = zb_list + g0;
+    const int nzb = zb_cnt[ta + f];
     c64 *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
     const c64 *pl = Pm + ((int64_t)f * Nx + x) * NYZ;
     // KEEP: the y-stage-A lanes' image points in registers across coils (one round)
@@ -788,7 +797,7 @@ __global__ __launch_bounds__(PP::NT) void k_fwd_yz2(
         __syncthreads();
         yz2_stage_za<PP, SGN>(buf, tid);
         __syncthreads();
-        yz2_stage_zb<PP, SGN>(buf, tid);
+        yz2_stage_zb_list<PP, SGN>(buf, tid, zbl, nzb);
         __syncthreads();
         c64 *dst = Tf + ((int64_t)c * Nx + x) * Lf;
         for (int l = tid; l < Lf; l += NT) dst[l] = buf[lpos[l]];
@@ -803,7 +812,8 @@ template <class PP, int SGN>
 __global__ __launch_bounds__(PP::NT) void k_adj_yz2(
     const c64 *__restrict__ Tb, const c64 *__restrict__ smp, int Nc, int Nx,
     const c64 *__restrict__ twy, const int *__restrict__ lptr, const int *__restrict__ line_pos,
-    int64_t ta, int nf, float scale, c64 *__restrict__ Pm) {
+    const int *__restrict__ zb_list, const int *__restrict__ zb_cnt, int64_t ta, int nf,
+    float scale, c64 *__restrict__ Pm) {
     constexpr int NT = PP::NT, NY = PP::NY, NYZ = PP::NYZ;
     constexpr int N1Y = PP::Y::N1, N2Y = PP::Y::N2;
     __shared__ c64 buf[PP::LY * PP::NZ];
@@ -815,6 +825,8 @@ __global__ __launch_bounds__(PP::NT) void k_adj_yz2(
     const int g0 = lptr[ta + f];
     const int Lf = lptr[ta + f + 1] - g0;
     const int *lpos = line_pos + g0;
+    const int *zbl = zb_list + g0;
+    const int nzb = zb_cnt[ta + f];
     const c64 *Tf = Tb + (int64_t)Nc * Nx * (g0 - lptr[ta]);
 
     for (int i = tid; i < PP::LY * PP::NZ; i += NT) buf[i] = c64{0.f, 0.f};
@@ -829,7 +841,7 @@ __global__ __launch_bounds__(PP::NT) void k_adj_yz2(
         const c64 *src = Tf + ((int64_t)c * Nx + x) * Lf;
         for (int l = tid; l < Lf; l += NT) buf[lpos[l]] = src[l];
         __syncthreads();
-        yz2_stage_zb<PP, SGN>(buf, tid);
+        yz2_stage_zb_list<PP, SGN>(buf, tid, zbl, nzb);
         __syncthreads();
         yz2_stage_za<PP, SGN>(buf, tid);
         __syncthreads();
@@ -1081,7 +1093,8 @@ static void launch_fwd_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Pm, c
     MSLR_REQUIRE(nwg * PP::NT < ((int64_t)1 << 32), MSLR_ERR_UNSUPPORTED, "too many frames per chunk");
     MSLR_PROF(ctx, "fwd_yz");
     k_fwd_yz2<PP, -1, PP::RYA == 1><<<(unsigned)nwg, PP::NT, 0, ctx->stream>>>(
-        Pm, P->smaps_yz2, P->Nc, Nx, P->tw_yz2[0], P->lptr, P->line_pos, ta, nf, Tb);
+        Pm, P->smaps_yz2, P->Nc, Nx, P->tw_yz2[0], P->lptr, P->line_pos, P->zb_list, P->zb_cnt,
+        ta, nf, Tb);
     HIP_LAUNCH_CHECK();
 }
 
@@ -1093,8 +1106,8 @@ static void launch_adj_yz2(mslr_sense_s *P, int64_t ta, int nf, const c64 *Tb, c
     MSLR_REQUIRE(nwg * PP::NT < ((int64_t)1 << 32), MSLR_ERR_UNSUPPORTED, "too many frames per chunk");
     MSLR_PROF(ctx, "adj_yz");
     k_adj_yz2<PP, 1><<<(unsigned)nwg, PP::NT, 0, ctx->stream>>>(
-        Tb, P->smaps_yz2, P->Nc, Nx, P->tw_yz2[1], P->lptr, P->line_pos, ta, nf,
-        (float)P->scale, Pm);
+        Tb, P->smaps_yz2, P->Nc, Nx, P->tw_yz2[1], P->lptr, P->line_pos, P->zb_list, P->zb_cnt,
+        ta, nf, (float)P->scale, Pm);
     HIP_LAUNCH_CHECK();
 }
 
@@ -1580,6 +1593,22 @@ static void yz2_tables(mslr_sense_s *P, const std::vector<c64> &sm32,
         const int ky = line_yz[l] % NY, kz = line_yz[l] / NY;
         lp[l] = PP::lds(AY::freq_position(ky), AZ::freq_position(kz));
     }
+    // per frame, the z-stage-B butterflies u = (pz / N2z) NY + py of its lines, ascending
+    const int Nt = P->Nt;
+    std::vector<int> zbl(std::max<size_t>(line_yz.size(), 1)), zbc(Nt);
+    std::vector<char> seen(PP::UZB);
+    for (int t = 0; t < Nt; ++t) {
+        std::fill(seen.begin(), seen.end(), 0);
+        for (int l = P->lptr_h[t]; l < P->lptr_h[t + 1]; ++l) {
+            const int py = AY::freq_position(line_yz[l] % NY);
+            const int pz = AZ::freq_position(line_yz[l] / NY);
+            seen[(pz / AZ::N2) * NY + py] = 1;
+        }
+        int n = 0;
+        for (int u = 0; u < PP::UZB; ++u)
+            if (seen[u]) zbl[P->lptr_h[t] + n++] = u;
+        zbc[t] = n;
+    }
     // twiddles W^(a b) = exp(-+ 2 pi i a b / Ny) in the order b * N1y + a
     for (int sgn = 0; sgn < 2; ++sgn) {
         std::vector<c64> tl((size_t)AY::N1 * AY::N2);
@@ -1592,6 +1621,8 @@ static void yz2_tables(mslr_sense_s *P, const std::vector<c64> &sm32,
     }
     P->smaps_yz2 = upload_vec(P, smp);
     P->line_pos = upload_vec(P, lp);
+    P->zb_list = upload_vec(P, zbl);
+    P->zb_cnt = upload_vec(P, zbc);
     P->yz2 = 1;
 }
 

@@ -41,6 +41,10 @@ struct mslr_sense_s {
     int yz2 = 0;                       // 1: tables below built
     mslr::c64 *smaps_yz2 = nullptr;    // coil maps in the y-stage-A lane order
     int *line_pos = nullptr;           // (lines): LDS offset of the line's (ky, kz)
+    // z-stage-B butterflies holding at least one sampled line, per frame: zb_list[lptr[t] ..
+    // lptr[t] + zb_cnt[t]) (ascending; at most one per line)
+    int *zb_list = nullptr;
+    int *zb_cnt = nullptr;
     mslr::c64 *tw_yz2[2] = {nullptr, nullptr};  // y twiddles, forward / inverse sign
     // two-stage fused x passes (compiled line lengths, FP32, default sign)
     int x2 = 0;                                 // 1: tables below built
