@@ -28,6 +28,7 @@
 #include <cstdio>
 
 #include "patches.hpp"
+#include "ozaki.hpp"
 #include "reduce.hpp"
 #include "sense.hpp"
 #include "svst.hpp"
@@ -156,11 +157,17 @@ __device__ __forceinline__ void pogm_elem(float xr, float xi, float uor, float u
 // every scale (pogm_elem) and the block's partials of the restart test, stored at
 // (frame ta + f, block) so that the order of the FP64 sums does not depend on how
 // the frames were grouped.  X and U start at frame ta.
+// wk >= 0 (even Nx, TB Nx a multiple of SPL_ROWS): z of scale wk, the whole-image
+// patch, goes out as its split words (Wh, Wl, frame-major like X from frame ta)
+// instead of into X: each wave's 64 pairs of x-neighbours are one SPL_ROWS-row block
+// of a frame's column, split as the SVST's split pass splits it.
 __global__ __launch_bounds__(256) void k_planes_update(const c64 *__restrict__ Pm, int Nx, int NYZ,
                                                        int TB, int64_t ta, c64 *__restrict__ X,
                                                        c64 *__restrict__ U, int64_t sstride,
                                                        int nscales, const PogmDev *__restrict__ s,
-                                                       double *__restrict__ partials) {
+                                                       double *__restrict__ partials,
+                                                       unsigned *__restrict__ Wh,
+                                                       unsigned *__restrict__ Wl, int wk) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c64 *tile = reinterpret_cast<c64 *>(smem_raw);
     __shared__ double red[16];
@@ -185,16 +192,22 @@ __global__ __launch_bounds__(256) void k_planes_update(const c64 *__restrict__ P
         const int64_t s2 = sstride / 2;
         float4 *X4 = reinterpret_cast<float4 *>(X + base);
         float4 *U4 = reinterpret_cast<float4 *>(U + base);
-        const int half = Nx / 2;
-        for (int e = threadIdx.x; e < nl * half; e += blockDim.x) {
-            const int l = e / half;
-            const int x = 2 * (e - l * half);
+        uint2 *Wh2 = reinterpret_cast<uint2 *>(Wh + (wk >= 0 ? base : 0));
+        uint2 *Wl2 = reinterpret_cast<uint2 *>(Wl + (wk >= 0 ? base : 0));
+        const int half = Nx / 2, npair = nl * half;
+        const int lane = threadIdx.x & 63;
+        // whole waves step together (the split's block maximum is a wave reduction)
+        for (int e0 = threadIdx.x - lane; e0 < npair; e0 += blockDim.x) {
+            const int e = e0 + lane;
+            const bool act = e < npair;
+            const int l = act ? e / half : 0;
+            const int x = act ? 2 * (e - l * half) : 0;
             const c64 g0 = tile[x * (TB + 1) + l], g1 = tile[(x + 1) * (TB + 1) + l];
             for (int k0 = 0; k0 < nscales; k0 += 4) {
                 float4 xv[4], uv[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    if (k0 + q < nscales) {
+                    if (act && k0 + q < nscales) {
                         xv[q] = X4[(int64_t)(k0 + q) * s2 + e];
                         uv[q] = U4[(int64_t)(k0 + q) * s2 + e];
                     }
@@ -202,13 +215,29 @@ __global__ __launch_bounds__(256) void k_planes_update(const c64 *__restrict__ P
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     if (k0 + q < nscales) {
-                        float4 zo, uo;
-                        pogm_elem(xv[q].x, xv[q].y, uv[q].x, uv[q].y, g0.re, g0.im, alpha, beta,
-                                  gamma, zo.x, zo.y, uo.x, uo.y, dot, gg, dd);
-                        pogm_elem(xv[q].z, xv[q].w, uv[q].z, uv[q].w, g1.re, g1.im, alpha, beta,
-                                  gamma, zo.z, zo.w, uo.z, uo.w, dot, gg, dd);
-                        X4[(int64_t)(k0 + q) * s2 + e] = zo;
-                        U4[(int64_t)(k0 + q) * s2 + e] = uo;
+                        float4 zo = make_float4(0.f, 0.f, 0.f, 0.f), uo;
+                        if (act) {
+                            pogm_elem(xv[q].x, xv[q].y, uv[q].x, uv[q].y, g0.re, g0.im, alpha,
+                                      beta, gamma, zo.x, zo.y, uo.x, uo.y, dot, gg, dd);
+                            pogm_elem(xv[q].z, xv[q].w, uv[q].z, uv[q].w, g1.re, g1.im, alpha,
+                                      beta, gamma, zo.z, zo.w, uo.z, uo.w, dot, gg, dd);
+                            U4[(int64_t)(k0 + q) * s2 + e] = uo;
+                        }
+                        if (k0 + q == wk) {  // uniform over the wave
+                            float sg1, sg2;
+                            ozaki_block_quanta(fmaxf(fmaxf(fabsf(zo.x), fabsf(zo.y)),
+                                                     fmaxf(fabsf(zo.z), fabsf(zo.w))),
+                                               sg1, sg2);
+                            unsigned h0, l0w, h1, l1w;
+                            ozaki_words(c64{zo.x, zo.y}, sg1, sg2, h0, l0w);
+                            ozaki_words(c64{zo.z, zo.w}, sg1, sg2, h1, l1w);
+                            if (act) {
+                                Wh2[e] = make_uint2(h0, h1);
+                                Wl2[e] = make_uint2(l0w, l1w);
+                            }
+                        } else if (act) {
+                            X4[(int64_t)(k0 + q) * s2 + e] = zo;
+                        }
                     }
                 }
             }
@@ -425,8 +454,11 @@ static void scale_nucnorm(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, d
 // place, an exact tiling's patches scattered to their voxels (no patches2img).
 // nuc_part/nuc_tot (device, nullable together): *nuc_tot <- sum over patches of
 // the nuclear norm of out's patches; nuc_part holds nuc_units(g) partials.
+// words (nullable): for the single patch with split-BF16 products, its split words,
+// written by the caller (the update) in place of img (fused_words_scale).
 static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 *out,
-                       const double *beta_dev, int *err, double *nuc_part, double *nuc_tot) {
+                       const double *beta_dev, int *err, double *nuc_part, double *nuc_tot,
+                       const unsigned *words = nullptr) {
     const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
     if (g.p == 1 && g.Np == V && g.exact_tiling()) {
         const unsigned nb = prox_voxel_blocks(V, g.Nt);
@@ -445,8 +477,11 @@ static void prox_scale(mslr_ctx_s *ctx, const PatchGeom &g, const c64 *img, c64 
     // split-BF16 products write out directly: in place for the single patch,
     // scattered to the voxels for an exact tiling
     const bool direct = svst_writes_direct(ctx, g.p, g.Nt);
+    MSLR_REQUIRE(!words || (g.Np == 1 && g.p == V && direct), MSLR_ERR_ARG,
+                 "split words are handed in only for the single patch");
     if (g.Np == 1 && g.p == V && direct) {
-        svst_batched(ctx, img, out, g.p, g.Nt, 1, beta_dev, 0, err, nuc_part);
+        svst_batched(ctx, img, out, g.p, g.Nt, 1, beta_dev, 0, err, nuc_part, nullptr, nullptr,
+                     words);
         if (nuc_part) sum_partials(ctx, nuc_part, 1, nuc_tot);
         return;
     }
@@ -564,6 +599,9 @@ struct mslr_pogm_s {
     int64_t n_upd_part = 0;  // partials of the restart test (frames x line blocks)
     hipStream_t gpipe = nullptr;
     hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;
+    // split words of the whole-image patch's prox input, written by the update
+    // (fused_words_scale); allocated at the first iteration that uses them
+    unsigned *words = nullptr;
     // batched forms: one stream per scale for the prox (fork / join with events)
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> ev_join;
@@ -608,10 +646,38 @@ static void log_reg_cost(mslr_pogm_s *st) {
     HIP_LAUNCH_CHECK();
 }
 
+// The scale whose prox input the update writes as split words instead of into X,
+// or -1: the single patch covering the image when its SVST reads X only through the
+// split words (split-BF16 products, p >= Nt) and writes its output in place, and
+// the update's waves cover whole SPL_ROWS-row blocks (even Nx, TB Nx a multiple of
+// SPL_ROWS).  Not with checkpoints, which keep each prox input.  ctx->products must
+// hold the iteration's setting.
+static int fused_words_scale(const mslr_pogm_s *st, const mslr_ctx_s *ctx, int TB, int Nx) {
+    if (st->checkpoint || Nx % 2 != 0 || ((int64_t)TB * Nx) % SPL_ROWS != 0) return -1;
+    for (int64_t k = 0; k < st->S; ++k) {
+        const PatchGeom &g = st->geoms[k];
+        if (g.Np == 1 && g.p == st->V && svst_writes_direct(ctx, g.p, g.Nt) &&
+            svst_reads_words(ctx, g.p, g.Nt))
+            return (int)k;
+    }
+    return -1;
+}
+
 // Enqueue one iteration on the context stream.
 static void pogm_iteration(mslr_pogm_s *st) {
     mslr_ctx_s *ctx = st->ctx;
     const int64_t n = st->V * st->Nt;
+    // the mixed form runs the SVST products in split BF16; the context setting
+    // is restored after the prox
+    struct ProductsScope {
+        mslr_ctx_s *c;
+        int saved;
+        ProductsScope(mslr_ctx_s *c_, int form) : c(c_), saved(c_->products) {
+            if (form == MSLR_FORM_MIXED) c->products = MSLR_PRODUCTS_SPLIT_BF16;
+            else c->products = MSLR_PRODUCTS_FP32;
+        }
+        ~ProductsScope() { c->products = saved; }
+    } products_scope(ctx, st->form);
     {   // beta, gamma, zeta of this iteration (they do not depend on the gradient)
         MSLR_PROF(ctx, "pogm_pre");
         k_pogm_pre<<<1, 64, 0, ctx->stream>>>(st->dev, st->lam_dev, (int)st->S, st->thr_dev,
@@ -632,6 +698,8 @@ static void pogm_iteration(mslr_pogm_s *st) {
     const int TB = st->upd_lines;
     const int Nx = st->sense->N[0], NYZ = st->sense->NYZ;
     const size_t tile_lds = (size_t)Nx * (TB + 1) * sizeof(c64);
+    const int wk = fused_words_scale(st, ctx, TB, Nx);
+    if (wk >= 0 && !st->words) st->words = (unsigned *)st->alloc((size_t)n * 2 * sizeof(unsigned));
     sense_grad_pipelined(st->sense, st->X, (int)st->S, n, st->y, nullptr,
                          st->log_costs ? st->dc_part : nullptr, st->gchunks, sp,
                          [&](size_t, int64_t ta, int64_t nf, const c64 *planes) {
@@ -640,7 +708,8 @@ static void pogm_iteration(mslr_pogm_s *st) {
                              dim3 grid((unsigned)cdiv(NYZ, TB), (unsigned)nf);
                              k_planes_update<<<grid, 256, tile_lds, ctx->stream>>>(
                                  planes, Nx, NYZ, TB, ta, st->X + o, st->U + o, n, (int)st->S,
-                                 st->dev, st->partials);
+                                 st->dev, st->partials, wk >= 0 ? st->words + o : nullptr,
+                                 wk >= 0 ? st->words + n + o : nullptr, wk);
                              HIP_LAUNCH_CHECK();
                          });
     if (st->log_costs) {
@@ -663,17 +732,7 @@ static void pogm_iteration(mslr_pogm_s *st) {
     }
     int *err = (int *)((char *)st->dev + offsetof(PogmDev, err));
     static const char *sfx[] = {"@s1", "@s2", "@s3", "@s4", "@s5", "@s6", "@s7", "@s8"};
-    // the mixed form runs the SVST products in split BF16; the context setting
-    // is restored after the prox
-    struct ProductsScope {
-        mslr_ctx_s *c;
-        int saved;
-        ProductsScope(mslr_ctx_s *c_, int form) : c(c_), saved(c_->products) {
-            if (form == MSLR_FORM_MIXED) c->products = MSLR_PRODUCTS_SPLIT_BF16;
-            else c->products = MSLR_PRODUCTS_FP32;
-        }
-        ~ProductsScope() { c->products = saved; }
-    } products_scope(ctx, st->form);
+    auto words = [&](int64_t k) { return k == wk ? st->words : nullptr; };
     // serial scales without aux streams, and whenever kernels are being timed (so
     // each kernel's events bracket it alone)
     auto nuc_part = [&](int64_t k) { return st->log_costs ? st->nuc_part[k] : nullptr; };
@@ -682,7 +741,8 @@ static void pogm_iteration(mslr_pogm_s *st) {
         for (int64_t k = 0; k < st->S; ++k) {
             c64 *Xk = st->X + k * n;
             ctx->prof_suffix = k < 8 ? sfx[k] : "@s9+";
-            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err, nuc_part(k), nuc_tot(k));
+            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err, nuc_part(k), nuc_tot(k),
+                       words(k));
         }
         ctx->prof_suffix = "";
         log_reg_cost(st);
@@ -712,7 +772,8 @@ static void pogm_iteration(mslr_pogm_s *st) {
             ctx->stream = st->aux[k];
             ctx->slot_base = (size_t)SLOT_COUNT * (size_t)(k + 1);
             ctx->prof_suffix = k < 8 ? sfx[k] : "@s9+";
-            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err, nuc_part(k), nuc_tot(k));
+            prox_scale(ctx, st->geoms[k], Xk, Xk, st->thr_dev + k, err, nuc_part(k), nuc_tot(k),
+                       words(k));
             HIP_CHECK(hipEventRecord(st->ev_join[k], st->aux[k]));
         }
     } catch (...) {

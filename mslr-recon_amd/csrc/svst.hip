@@ -23,6 +23,7 @@
 #include <cstring>
 #include <type_traits>
 
+#include "ozaki.hpp"
 #include "reduce.hpp"
 #include "sbr.hpp"
 #include "svst.hpp"
@@ -1197,7 +1198,6 @@ __global__ __launch_bounds__(256) void k_cgemm(const c64 *__restrict__ A, int64_
 // complex k.  Its lane layout (D = P Q, P 16x32, Q 32x16): lane l holds
 // P[l%16][8(l/16) .. 8(l/16)+7], Q[8(l/16) .. +7][l%16] and D[4(l/16)+v][l%16].
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
 typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 constexpr int BKS = 32;         // complex k per LDS stage
@@ -1239,42 +1239,17 @@ __device__ __forceinline__ void stage8(const c64 (&r)[8], unsigned *sh, unsigned
     *(u4 *)(sl + 4) = u4{l[4], l[5], l[6], l[7]};
 }
 
-// Ozaki slices of row-contiguous patch matrices (rows of each column contiguous,
-// the columns of all patches one after another): per column and block of
-// SPL_ROWS rows one exponent E, 2^E > max |x| over the block (both parts), and
-// x = s1 + s2 + r with s1 a multiple of 2^(E-8) (|s1| <= 2^8 of those), s2 a
-// multiple of 2^(E-16) (<= 2^7 of those), |r| <= 2^(E-17); both slices exact in
-// BF16.  Written as two words per element (BF16 re in the low half, im in the
-// high half): wh = s1, wl = s2.  One wave per (column, block).
-constexpr int SPL_ROWS = 128;
-
-// (s1, s2) of x for the quanta set by sg1 = 3*2^(E+14), sg2 = 3*2^(E+6)
-__device__ __forceinline__ float2 ozaki_split(float x, float sg1, float sg2) {
-    const float s1 = (x + sg1) - sg1;
-    const float r1 = x - s1;
-    const float s2 = (r1 + sg2) - sg2;
-    return make_float2(s1, s2);
-}
-
-__device__ __forceinline__ void ozaki_words(c64 v, float sg1, float sg2, unsigned &wh, unsigned &wl) {
-    const float2 sr = ozaki_split(v.re, sg1, sg2);
-    const float2 si = ozaki_split(v.im, sg1, sg2);
-    const bf2v hv = {(__bf16)sr.x, (__bf16)si.x};
-    const bf2v lv = {(__bf16)sr.y, (__bf16)si.y};
-    wh = __builtin_bit_cast(unsigned, hv);
-    wl = __builtin_bit_cast(unsigned, lv);
-}
+// Ozaki slices (ozaki.hpp) of row-contiguous patch matrices (rows of each column
+// contiguous, the columns of all patches one after another), a block of SPL_ROWS
+// rows per exponent.  One wave per (column, block).
 
 // One wave's block of up to SPL_ROWS values (v0: element lane, v1: lane + 64) as
 // Ozaki words with the block's exponent, n valid values written at W[0 .. n).
 __device__ __forceinline__ void split_block(c64 v0, c64 v1, int lane, int n, unsigned *Wh,
                                             unsigned *Wl) {
-    float mx = fmaxf(fmaxf(fabsf(v0.re), fabsf(v0.im)), fmaxf(fabsf(v1.re), fabsf(v1.im)));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    int E;
-    (void)frexpf(mx, &E);  // mx < 2^E (E = 0 for a zero block: all slices 0)
-    const float sg1 = ldexpf(3.f, E + 14), sg2 = ldexpf(3.f, E + 6);
+    float sg1, sg2;
+    ozaki_block_quanta(
+        fmaxf(fmaxf(fabsf(v0.re), fabsf(v0.im)), fmaxf(fabsf(v1.re), fabsf(v1.im))), sg1, sg2);
     unsigned h, l;
     if (lane < n) {
         ozaki_words(v0, sg1, sg2, h, l);
@@ -2232,9 +2207,14 @@ bool svst_writes_direct(const mslr_ctx_s *ctx, int64_t p, int64_t Nt) {
            (apply_fp32_mask(ctx) & 5) == 0;
 }
 
+bool svst_reads_words(const mslr_ctx_s *ctx, int64_t p, int64_t Nt) {
+    return ctx->products == MSLR_PRODUCTS_SPLIT_BF16 && p >= Nt && Nt > 1 &&
+           (apply_fp32_mask(ctx) & 3) == 0;
+}
+
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
                   const double *beta_dev, int per_patch, int *err_dev, double *nuc_out,
-                  const PatchGeom *scatter, const PatchGeom *gather) {
+                  const PatchGeom *scatter, const PatchGeom *gather, const unsigned *words) {
     if (Np == 0 || p == 0 || Nt == 0) return;
     const int64_t m = std::min(p, Nt);
     MSLR_REQUIRE(m <= MAXM, MSLR_ERR_UNSUPPORTED,
@@ -2251,8 +2231,9 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
     // X U_r) take the split pass's Ozaki words as their left operand (x = s1 + s2 to
     // 2^-17 of the block maximum) when both run split; else they read an FP32 copy
     // of the patches, written by the split pass, where Y overwrites X.
-    const bool words_a = ctx->products == MSLR_PRODUCTS_SPLIT_BF16 && p >= Nt &&
-                         (apply_fp32_mask(ctx) & 3) == 0;
+    const bool words_a = svst_reads_words(ctx, p, Nt);
+    MSLR_REQUIRE(!words || (words_a && !gather), MSLR_ERR_ARG,
+                 "svst_batched takes split words only where every product reads them");
     const bool copy = (in_place || gather) && !words_a;
     const int64_t len = p * Nt;
     if (m == 1) {
@@ -2266,6 +2247,8 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
     }
     const bool left = p < Nt;
     const SvstWork w = svst_work(ctx, m, left ? Nt : p, Np);
+    MSLR_REQUIRE(!words || w.chunk >= Np, MSLR_ERR_UNSUPPORTED,
+                 "split words handed in need the batch in one workspace chunk");
     // split-BF16 Gram and apply products (the mixed-precision form)
     const bool split = ctx->products == MSLR_PRODUCTS_SPLIT_BF16;
     // W = V_r diag(f) V_r^H per patch (m x m, FP32), reuses the Gram partial space
@@ -2278,9 +2261,12 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         c64 *Pcopy = copy ? (c64 *)ctx->scratch_get(SLOT_SVST_COPY, (size_t)(len * nb) * sizeof(c64))
                           : nullptr;
         // split products: the chunk's patches once as split-BF16 words for the Gram
-        // (the apply splits the FP32 data per element: no block-relative residual)
+        // (and, with words_a, for the products that read X)
         const unsigned *Wh = nullptr, *Wl = nullptr;
-        if (split) {
+        if (words) {
+            Wh = words;
+            Wl = words + len * nb;
+        } else if (split) {
             unsigned *wb = (unsigned *)ctx->scratch_get(SLOT_SPLIT, (size_t)len * nb * 8);
             MSLR_PROF(ctx, "split");
             if (!left) {

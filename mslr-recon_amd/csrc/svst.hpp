@@ -20,14 +20,24 @@ namespace mslr {
 //     one patch) and Pout is their image: each output patch is written straight to
 //     its voxels, so no patches2img pass follows;
 //   Pin == Pout (in place, also with gather and scatter on one image).
-// With gather or in place the split pass keeps an FP32 copy of the patch matrices
-// for the reconstruction products.
+// The products that read X (X W, X U_r) take the split pass's words as their left
+// operand when svst_reads_words; otherwise, with gather or in place, the split pass
+// keeps an FP32 copy of the patch matrices for them.
+//   words (nullable; svst_reads_words, no gather, one chunk): the split words of
+//     Pin (ozaki.hpp: wh of all Np p x Nt matrices, then wl) already written by
+//     the caller; no split pass runs and Pin is not read.
 void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t Nt, int64_t Np,
                   const double *beta_dev, int per_patch, int *err_dev, double *nuc_out = nullptr,
-                  const PatchGeom *scatter = nullptr, const PatchGeom *gather = nullptr);
+                  const PatchGeom *scatter = nullptr, const PatchGeom *gather = nullptr,
+                  const unsigned *words = nullptr);
 
-// Whether svst_batched takes the two options above for these shapes on ctx.
+// Whether svst_batched takes the gather, scatter and in-place options above for
+// these shapes on ctx.
 bool svst_writes_direct(const mslr_ctx_s *ctx, int64_t p, int64_t Nt);
+
+// Whether every product of svst_batched reads the input only through its split
+// words (split-BF16 products, p >= Nt), so that a caller may hand them in.
+bool svst_reads_words(const mslr_ctx_s *ctx, int64_t p, int64_t Nt);
 
 // Vector-Jacobian product of the batched SVST (p >= Nt >= 2): Zbar (distinct from
 // the inputs) <- the cotangent of Z, betabar_dev[i] <- d<Ybar, Y>/d beta_i per

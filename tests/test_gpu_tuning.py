@@ -202,6 +202,35 @@ def test_update_lines(shapes, name):
         np.testing.assert_allclose(out[lines][2][0], out[0][2][0], rtol=1e-12)
 
 
+def test_update_writes_split_words(shapes):
+    """Mixed form: with 64 or 32 yz-lines per update workgroup at Nx = 12 (768 and 384
+    voxels, whole 128-row blocks) the update writes the whole-image patch's split words
+    itself and its SVST runs no split pass; with 16 or 8 (192, 96 voxels) the SVST splits
+    z.  The words are the same bits either way, so the iterates, restarts and costs are
+    identical."""
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.sense import SenseOp
+    cfg, smaps, ksp, omega, sf = prepared("small")
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    assert N[0] == 12 and cfg["patch_sizes"][0] == list(N)
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    A = SenseOp(omega, smaps)
+    out = {}
+    for lines in (0, 32, 16, 8):
+        shapes.set_tuning(update_lines=lines)
+        st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=2.0,
+                  niter=6, form="mixed", log_costs=True)
+        st.init(ksp)
+        st.run(6)
+        out[lines] = (st.X.to_host(), st.log()[1], st.costs())
+        st.close()
+    A.close()
+    for lines in (32, 16, 8):
+        assert out[lines][1] == out[0][1]
+        assert np.array_equal(out[lines][0], out[0][0]), lines
+        np.testing.assert_allclose(out[lines][2], out[0][2], rtol=1e-12)
+
+
 def test_compiled_variants_are_bit_identical(shapes, rng):
     """The alternative compiled kernels change the launch shape only: the y-z FFT passes
     with 512 / 768 / 1024 threads at 90 x 90 x 60 (forward and adjoint bit-identical), and the
