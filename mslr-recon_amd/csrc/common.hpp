@@ -167,6 +167,17 @@ __device__ __forceinline__ int64_t xcd_order(int64_t w, int64_t n) {
     return (w % NXCD) * per + w / NXCD;
 }
 
+// Work-group barrier that orders LDS accesses only.  __syncthreads is a fence over
+// all memory as well, so it also waits for the wave's global loads in flight; this
+// one leaves them in flight, for loops that prefetch global data across their
+// barriers while the waves exchange data through LDS alone (global memory written
+// before it is not visible to the other waves after it).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Lookup of the context behind a handle with validation.
 inline mslr_ctx_s *check_ctx(mslr_ctx c) {
     if (!c) throw Error(MSLR_ERR_ARG, "null context");

@@ -739,21 +739,22 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         // ring slot j holds this thread's share of a sweep's reflectors (<= 2
         // values) and tau: sweep s sits in slot (m - 2 - s) % BT_PD
         c128 rv0[BT_PD], rv1[BT_PD], rt[BT_PD];
-        // loads from clamped addresses and selects (no branches): entries past the
-        // sweep's reflectors read as zero
+        // unconditional loads from clamped addresses; the entries past the sweep's
+        // reflectors are zeroed when the sweep is staged (put), so no loaded value is
+        // consumed before then and the loads of the BT_PD sweeps ahead stay in flight
+        // (with the LDS-only barriers of the sweep loop)
         auto fetch = [&](int s, c128 &v0, c128 &v1, c128 &t) {
             const int nr = nsweep_refl(s), n = nr * B;
             const c128 *src = V2 + (int64_t)s * ks * B;
-            const c128 a0 = src[min(tid, n - 1)], a1 = src[min(tid + BTT, n - 1)];
-            const c128 a2 = tau2[(int64_t)s * ks + min(tid, nr - 1)];
-            v0 = tid < n ? a0 : c128{0.0, 0.0};
-            v1 = tid + BTT < n ? a1 : c128{0.0, 0.0};
-            t = tid < nr ? a2 : c128{0.0, 0.0};
+            v0 = src[min(tid, n - 1)];
+            v1 = src[min(tid + BTT, n - 1)];
+            t = tau2[(int64_t)s * ks + min(tid, nr - 1)];
         };
-        auto put = [&](int buf, const c128 &v0, const c128 &v1, const c128 &t) {
-            Vbuf[buf * BS + tid] = v0;
-            Vbuf[buf * BS + tid + BTT] = v1;
-            if (tid < KS) Tb[buf][tid] = t;
+        auto put = [&](int buf, int s, const c128 &v0, const c128 &v1, const c128 &t) {
+            const int nr = nsweep_refl(s), n = nr * B;
+            Vbuf[buf * BS + tid] = tid < n ? v0 : c128{0.0, 0.0};
+            Vbuf[buf * BS + tid + BTT] = tid + BTT < n ? v1 : c128{0.0, 0.0};
+            if (tid < KS) Tb[buf][tid] = tid < nr ? t : c128{0.0, 0.0};
         };
         const int k = tid / BTC, c = tid - k * BTC;  // BT_KP >= KS: one reflector a thread
         const bool col = c < ncol;
@@ -763,7 +764,7 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
 #pragma unroll
         for (int j = 0; j < BT_PD; ++j)
             if (m - 2 - j >= 0) fetch(m - 2 - j, rv0[j], rv1[j], rt[j]);
-        if (m >= 2) put(0, rv0[0], rv1[0], rt[0]);
+        if (m >= 2) put(0, m - 2, rv0[0], rv1[0], rt[0]);
         __syncthreads();  // (also orders the initialisation of Uc before the read below)
         if (k == 0 && col && m >= 2) w[0] = Uc[c * m + m - 1];
         int buf = 0;
@@ -772,10 +773,12 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
 #pragma clang loop unroll(full)
             for (int j = 0; j < B; ++j) {
                 const int s = s0 - j;
+                // slot (sweep s) is staged: refill it with sweep s - BT_PD, on every
+                // step (clamped to sweep 0; never staged past it), so that the loads
+                // in flight are the same on every path and each staging waits only
+                // for its own sweep's loads
+                fetch(max(s - BT_PD, 0), rv0[j % BT_PD], rv1[j % BT_PD], rt[j % BT_PD]);
                 if (s >= 0) {  // (no early exit: a constant trip count unrolls fully)
-                    const int slot = j % BT_PD;
-                    // slot (sweep s) is staged: refill it with sweep s - BT_PD
-                    if (s - BT_PD >= 0) fetch(s - BT_PD, rv0[slot], rv1[slot], rt[slot]);
                     const int nst = nsweep_refl(s);
                     if (k < nst && col) {
                         // the reflector in registers, zero past its length (rows past
@@ -808,9 +811,9 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
                     // sweep s - 1 (the next slot) -> the other staging buffer
                     if (s > 0) {
                         const int nx = (j + 1) % BT_PD;
-                        put(buf ^ 1, rv0[nx], rv1[nx], rt[nx]);
+                        put(buf ^ 1, s - 1, rv0[nx], rv1[nx], rt[nx]);
                     }
-                    __syncthreads();
+                    lds_barrier();  // (the ring's loads stay in flight)
                     buf ^= 1;
                     // rotate: logical q of sweep s - 1 is logical q - 1 of sweep s; the
                     // freed register (logical B - 1) becomes logical 0 = row s + 16k
