@@ -2171,13 +2171,19 @@ static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
             kern<<<g, 256, 0, ctx->stream>>>(Az, Awz, Awlo, bA, ak, Wh, Wl, bW, Kp, Cz, bC, cj, M, N,
                                              K, nz, rz, sg, s0, sc);
         };
+        // column tiles per pass: five for the dense products (K = m: the A tile is
+        // staged once per k step for 160 columns), at most two for the low-rank ones
+        // (N or K narrowed to r per patch: the smaller tile keeps more waves resident;
+        // c1 scale 2: X U_r 0.42 -> 0.35 ms, T f U_r^H 0.77 -> 0.67 ms, while the
+        // whole-image X W takes 1.70 ms with two against 1.27 with five)
+        const int nbt = rs.mode >= 2 ? (N > 32 ? 2 : 1) : (N > 64 ? 5 : (N > 32 ? 2 : 1));
         if (Aw) {
-            if (N > 64) go(k_apply_bf16<5, true>);
-            else if (N > 32) go(k_apply_bf16<2, true>);
+            if (nbt == 5) go(k_apply_bf16<5, true>);
+            else if (nbt == 2) go(k_apply_bf16<2, true>);
             else go(k_apply_bf16<1, true>);
         } else {
-            if (N > 64) go(k_apply_bf16<5, false>);
-            else if (N > 32) go(k_apply_bf16<2, false>);
+            if (nbt == 5) go(k_apply_bf16<5, false>);
+            else if (nbt == 2) go(k_apply_bf16<2, false>);
             else go(k_apply_bf16<1, false>);
         }
         HIP_LAUNCH_CHECK();
