@@ -1390,12 +1390,15 @@ __global__ __launch_bounds__(GB_T) void k_gram_bf16(const unsigned *__restrict__
     // eight waves, each a 32 (a) x 16 (b) sub-tile: two 16 x 16 blocks
     const int wa = (wave & 1) * 32, wb = (wave >> 1) * 16;
     const int r16 = lane & 15, g4 = lane >> 4;
-    // load slot: one column, four consecutive k (one 16-byte word group)
+    // load slot: one column, four consecutive k (one 16-byte word group); columns
+    // past m load column m - 1 and are zeroed when staged, so that no loaded value
+    // is consumed before its stage's put and the loads stay in flight across the
+    // step (LDS-only barriers)
     const int lc = tid >> 3;
     const int lk = (tid & 7) * 4;
     const bool va = a0 + lc < m, vb = b0 + lc < m;
-    const int64_t oa = ip * pstride + (int64_t)(a0 + lc) * cs;
-    const int64_t ob = ip * pstride + (int64_t)(b0 + lc) * cs;
+    const int64_t oa = ip * pstride + (int64_t)min(a0 + lc, m - 1) * cs;
+    const int64_t ob = ip * pstride + (int64_t)min(b0 + lc, m - 1) * cs;
     mf4 hr[2], hi[2], xr[2], xi[2], lr[2], li[2];
     md4 dr[2], di[2];
     auto zero32 = [&]() {
@@ -1420,12 +1423,11 @@ __global__ __launch_bounds__(GB_T) void k_gram_bf16(const unsigned *__restrict__
     u4 ah, al, bh, bl;
     auto fetch = [&](int64_t k0) {
         const int64_t k = k0 + lk;
-        if (vec && k + 4 <= kend) {  // 16-byte aligned rows (p % 4 == 0)
-            const u4 z = {0u, 0u, 0u, 0u};
-            ah = va ? *(const u4 *)(Wh + oa + k) : z;
-            al = va ? *(const u4 *)(Wl + oa + k) : z;
-            bh = vb ? *(const u4 *)(Wh + ob + k) : z;
-            bl = vb ? *(const u4 *)(Wl + ob + k) : z;
+        if (vec && k0 + BKS <= kend) {  // a whole step of 16-byte aligned rows (p % 4 == 0)
+            ah = *(const u4 *)(Wh + oa + k);
+            al = *(const u4 *)(Wl + oa + k);
+            bh = *(const u4 *)(Wh + ob + k);
+            bl = *(const u4 *)(Wl + ob + k);
         } else {
             unsigned t[4][4];
 #pragma unroll
@@ -1443,10 +1445,11 @@ __global__ __launch_bounds__(GB_T) void k_gram_bf16(const unsigned *__restrict__
         }
     };
     auto put = [&](int st) {
-        *(u4 *)&Sh[st][0][lc][lk] = ah;
-        *(u4 *)&Sl[st][0][lc][lk] = al;
-        *(u4 *)&Sh[st][1][lc][lk] = bh;
-        *(u4 *)&Sl[st][1][lc][lk] = bl;
+        const u4 z = {0u, 0u, 0u, 0u};
+        *(u4 *)&Sh[st][0][lc][lk] = va ? ah : z;
+        *(u4 *)&Sl[st][0][lc][lk] = va ? al : z;
+        *(u4 *)&Sh[st][1][lc][lk] = vb ? bh : z;
+        *(u4 *)&Sl[st][1][lc][lk] = vb ? bl : z;
     };
     fetch(kbeg);
     put(0);
@@ -1499,7 +1502,7 @@ __global__ __launch_bounds__(GB_T) void k_gram_bf16(const unsigned *__restrict__
                 }
             zero32();
         }
-        __syncthreads();
+        lds_barrier();  // (the next step's loads stay in flight)
         st ^= 1;
     }
 #pragma unroll
