@@ -31,19 +31,19 @@ HBM_PEAK_GBS = 8000.0
 # families, from the committed rocprofv3 counter summary of this workload (mixed
 # form; the SENSE, update and patch kernels are the same in every form)
 TRAFFIC_BYTES = {
-    "fwd_yz": 4458100000,
-    "adj_yz": 4085800000,
-    "pogm_update": 15311000000,
-    "x_resid": 9557600000,
-    "sum_planes": 4665600000,
-    "sy2sb": 5783400000,
-    "img2patches": 8539300000,
-    "prox_voxel": 3499200000,
-    "split": 4317000000,
-    "gram_tiles": 1809000000,
-    "backtransform": 1434400000,
-    "apply": 1259100000,
-    "sb2st": 224500000,
+    "fwd_yz": 4445600000,
+    "adj_yz": 4363500000,
+    "pogm_update": 15326200000,
+    "x_resid": 8946600000,
+    "sum_planes": 4666000000,
+    "sy2sb": 5780900000,
+    "img2patches": 8676200000,
+    "prox_voxel": 2333500000,
+    "split": 3978700000,
+    "gram_tiles": 1822700000,
+    "backtransform": 1410400000,
+    "apply": 2126500000,
+    "sb2st": 224800000,
 }
 FP32_PEAK_TFLOPS = 157.3
 
