@@ -523,28 +523,37 @@ __device__ __forceinline__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *
         c128 pc = {0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < 4; ++t) pc = fma_conj(Dr[t], vq[t], pc);
-        pc.re += __shfl_xor(pc.re, 16, 64);
-        pc.im += __shfl_xor(pc.im, 16, 64);
-        pc.re += __shfl_xor(pc.re, 32, 64);
-        pc.im += __shfl_xor(pc.im, 32, 64);
-        pc = tau * pc;  // sum_q conj(D[q][c]) v[q] = sum_q D[c][q] v[q] = (D v)[c]
-        // q = tau E v (row sums)
-        c128 pe[4];
+        pc = tau * sum_rows4(pc);  // sum_q conj(D[q][c]) v[q] = sum_q D[c][q] v[q] = (D v)[c]
+        // e = tau E v by rows, the same way from E transposed in the lanes: lane (r, g)
+        // takes row r at columns g + 4t (loads from clamped addresses and selects, as
+        // above), then the four row groups; e[r] lands in the lanes of column r
+        c128 pt = {0.0, 0.0};
 #pragma unroll
-        for (int t = 0; t < 4; ++t) pe[t] = tau * sum16(Er[t] * vr);
+        for (int t = 0; t < 4; ++t) {
+            const int c = g + 4 * t;
+            const int cc = min(c, len - 1);
+            const c128 ev = Bd[(st + cc) * (2 * B) + (B + r - cc)];
+            const c128 e = (r < len2 && c < len) ? ev : c128{0.0, 0.0};
+            pt = pt + e * vq[t];
+        }
+        pt = tau * sum_rows4(pt);
         // p^H v over the columns of one row of lanes (every row holds all of p)
         const c128 dot = sum16(cmulc(pc, vr));
         const c128 alpha2 = (-0.5) * (tau * dot);
-        // w = p + alpha2 v: w_r for the own column, w_q for the own rows through LDS
+        // w = p + alpha2 v: w_r for the own column, w_q for the own rows through LDS;
+        // e_q for the own rows through LDS as well (sx is free until step (c))
         const c128 wr = pc + alpha2 * vr;
         wave_lds_sync();
-        if (g == 0) sw[r] = pc;
+        if (g == 0) {
+            sw[r] = pc;
+            sx[r] = pt;
+        }
         wave_lds_sync();
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const c128 wq = sw[g + 4 * t] + alpha2 * vq[t];
             Dr[t] = fms_cc(wq, vr, fms_cc(vq[t], wr, Dr[t]));  // D <- H^H D H
-            Er[t] = fms_cc(pe[t], vr, Er[t]);                   // E <- E H
+            Er[t] = fms_cc(sx[g + 4 * t], vr, Er[t]);          // E <- E H
         }
         // D back to the band (lower part)
 #pragma unroll
@@ -592,11 +601,7 @@ __device__ __forceinline__ c128 sb2st_task(c128 *Bd, int m, int s, int k, c128 *
                 vn[t] = sv[g + 4 * t];  // (zero past len2)
                 cs = fma_conj(vn[t], Er[t], cs);
             }
-            cs.re += __shfl_xor(cs.re, 16, 64);
-            cs.im += __shfl_xor(cs.im, 16, 64);
-            cs.re += __shfl_xor(cs.re, 32, 64);
-            cs.im += __shfl_xor(cs.im, 32, 64);
-            cs = conj(h.tau) * cs;
+            cs = conj(h.tau) * sum_rows4(cs);
             if (r >= 1) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) Er[t] = fms_c(cs, vn[t], Er[t]);

@@ -34,6 +34,29 @@ __device__ __forceinline__ double sum16(double v) {
 }
 __device__ __forceinline__ C2<double> sum16(C2<double> v) { return {sum16(v.re), sum16(v.im)}; }
 
+// v + v[lane ^ 16] and v + v[lane ^ 32] through v_permlane16_swap / v_permlane32_swap
+// (no LDS crossbar): swapping a register with itself gives every lane its own value
+// and its partner's, in an order that makes the sum the same in both lanes.
+__device__ __forceinline__ double mk_double(int lo, int hi) {
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double xor16_sum(double v) {
+    const long long b = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((int)b, (int)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((int)(b >> 32), (int)(b >> 32), false, false);
+    return mk_double(lo[0], hi[0]) + mk_double(lo[1], hi[1]);
+}
+__device__ __forceinline__ double xor32_sum(double v) {
+    const long long b = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((int)b, (int)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((int)(b >> 32), (int)(b >> 32), false, false);
+    return mk_double(lo[0], hi[0]) + mk_double(lo[1], hi[1]);
+}
+// Sum over the four rows of 16 lanes (lanes l, l ^ 16, l ^ 32, l ^ 48), in every lane.
+__device__ __forceinline__ C2<double> sum_rows4(C2<double> v) {
+    return {xor32_sum(xor16_sum(v.re)), xor32_sum(xor16_sum(v.im))};
+}
+
 __device__ __forceinline__ double readlane_d(double v, int lane) {
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readlane((int)b, lane);
