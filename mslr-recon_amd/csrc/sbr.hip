@@ -850,20 +850,34 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         for (int pr = tid; pr < B * B; pr += BTT)
             Ts[pr % B][pr / B] = Tall[((int64_t)ip * npanel + p) * B * B + pr];
         // V[i][a] of the panel (i < L): 1 on the diagonal, 0 above, the stored tail below
-        auto vel = [&](int i, int a) {
-            c128 v = ld_if(a < kq && i > a && i < L, A + (r0 + i) + (int64_t)m * (pc0 + a));
-            if (a < kq && i == a) v.re = 1.0;
-            return v;
+        // (as raw loads from rows clamped into the matrix, then the unit lower form:
+        // loads without branches issue together and wait once per row block)
+        auto vraw = [&](int i, int a) {
+            return A[(r0 + min(i, L - 1)) + (int64_t)m * (pc0 + a)];
+        };
+        auto vfix = [&](c128 v, int i, int a) {
+            const bool below = a < kq && i > a && i < L;
+            const bool diag = a < kq && i == a;
+            return c128{below ? v.re : (diag ? 1.0 : 0.0), below ? v.im : 0.0};
         };
         // X = V^H U: k-steps over the block's rows, P[a][i] = conj(V[i][a]) (a = rl),
         // Q[i][c] = U[c][r0 + i] (c = rl)
         md4 xr = {0.0, 0.0, 0.0, 0.0}, xi = {0.0, 0.0, 0.0, 0.0};
         for (int rb = wave; rb < nrb; rb += BTT / 64) {
+            c128 vv[4], uu[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) vv[q] = vraw(rb * 16 + 4 * q + kl, rl);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {  // (clamped into the tile, masked)
+                const int i = rb * 16 + 4 * q + kl;
+                const c128 u0 = Uc[min(rl, BTC - 1) * m + r0 + min(i, L - 1)];
+                uu[q] = (rl < ncol && i < L) ? u0 : c128{0.0, 0.0};
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = rb * 16 + 4 * q + kl;
-                const c128 v = vel(i, rl);
-                const c128 u = ld_if(rl < ncol && i < L, Uc + rl * m + r0 + i);
+                const c128 v = vfix(vv[q], i, rl);
+                const c128 u = uu[q];
                 // conj(v) u: re = vr ur + vi ui, im = vr ui - vi ur
                 xr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, u.re, xr, 0, 0, 0);
                 xr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, u.im, xr, 0, 0, 0);
@@ -899,10 +913,13 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
         // lane (rl, kl) owns U[c = rl][r0 + rb 16 + kl + 4q]
         for (int rb = wave; rb < nrb; rb += BTT / 64) {
             md4 dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+            c128 vv[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) vv[s4] = vraw(rb * 16 + rl, 4 * s4 + kl);
 #pragma unroll
             for (int s4 = 0; s4 < 4; ++s4) {
                 const int a = 4 * s4 + kl;
-                const c128 v = vel(rb * 16 + rl, a);
+                const c128 v = vfix(vv[s4], rb * 16 + rl, a);
                 const c128 y = XY[a][rl];
                 // v y: re = vr yr - vi yi, im = vr yi + vi yr
                 dr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, y.re, dr, 0, 0, 0);
