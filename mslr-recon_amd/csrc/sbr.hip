@@ -112,6 +112,13 @@ __device__ __forceinline__ c128 fms_cc(c128 a, c128 b, c128 w) {
     return c128{fma(-a.re, b.re, fma(-a.im, b.im, w.re)), fma(-a.im, b.re, fma(a.re, b.im, w.im))};
 }
 
+// v when c, else 0: with a load from an address clamped into the array, the mask is a
+// select after it, so loads in a loop carry no branches, issue together and wait
+// once (a load under a branch waits before the next one can issue).
+__device__ __forceinline__ c128 sel(bool c, c128 v) {
+    return c128{c ? v.re : 0.0, c ? v.im : 0.0};
+}
+
 // Value of *p when c, else 0, through scalar loads (a conditionally assigned
 // struct otherwise ends up in private memory).
 __device__ __forceinline__ c128 ld_if(bool c, const c128 *p) {
@@ -301,19 +308,32 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
                 md4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
                 md4 yr2 = {0.0, 0.0, 0.0, 0.0}, yi2 = {0.0, 0.0, 0.0, 0.0};
                 const int i = rb * 16 + rl;
-#pragma unroll 4
-                for (int ks = 0; ks < L; ks += 4) {
-                    const int k = ks + kl;
-                    const bool lo = i >= k;
-                    c128 a = ld_if(i < L && k < L, lo ? A + (r0 + i) + (int64_t)m * (r0 + k)
-                                                      : A + (r0 + k) + (int64_t)m * (r0 + i));
-                    if (!lo) a.im = -a.im;
-                    const c128 v = ld_if(k < L, V + rl * L + k);
-                    // P[i][k] = A, Q[k][c] = V: the block of Y = A V
-                    yr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, v.re, yr, 0, 0, 0);
-                    yi = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, v.im, yi, 0, 0, 0);
-                    yr2 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, v.im, yr2, 0, 0, 0);
-                    yi2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, v.re, yi2, 0, 0, 0);
+                const int ic = min(i, L - 1);
+                // A(i, k) from the lower triangle (or the conjugate of its mirror), from
+                // rows and columns clamped into the matrix; YP k-steps of loads are in
+                // flight before their products
+                auto aload = [&](int k) {
+                    const int kc = min(k, L - 1);
+                    return A[i >= k ? (r0 + ic) + (int64_t)m * (r0 + kc)
+                                    : (r0 + kc) + (int64_t)m * (r0 + ic)];
+                };
+                constexpr int YP = 4;
+                for (int ks = 0; ks < L; ks += 4 * YP) {
+                    c128 araw[YP];
+#pragma unroll
+                    for (int u = 0; u < YP; ++u) araw[u] = aload(ks + 4 * u + kl);
+#pragma unroll
+                    for (int u = 0; u < YP; ++u) {
+                        const int k = ks + 4 * u + kl;
+                        c128 a = sel(i < L && k < L, araw[u]);
+                        if (i < k) a.im = -a.im;
+                        const c128 v = sel(k < L, V[rl * L + min(k, L - 1)]);
+                        // P[i][k] = A, Q[k][c] = V: the block of Y = A V
+                        yr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, v.re, yr, 0, 0, 0);
+                        yi = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, v.im, yi, 0, 0, 0);
+                        yr2 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, v.im, yr2, 0, 0, 0);
+                        yi2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, v.re, yi2, 0, 0, 0);
+                    }
                 }
                 yr += yr2;
                 yi += yi2;
@@ -328,7 +348,7 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int ii = rb * 16 + 4 * q + kl;
-                    const c128 vv = ld_if(ii < L, V + rl * L + ii);
+                    const c128 vv = sel(ii < L, V[rl * L + min(ii, L - 1)]);
                     zr = __builtin_amdgcn_mfma_f64_16x16x4f64(vv.re, yr[q], zr, 0, 0, 0);
                     zr = __builtin_amdgcn_mfma_f64_16x16x4f64(vv.im, yi[q], zr, 0, 0, 0);
                     zi = __builtin_amdgcn_mfma_f64_16x16x4f64(vv.re, yi[q], zi, 0, 0, 0);
@@ -380,8 +400,8 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
                     const int a = as + kl;
                     const c128 t = Ts[a][rl];   // P[c][a] = T[a][c], c = rl
                     const c128 mm = Ms[a][rl];  // P[c][a] = M[a][c]
-                    const c128 y = ld_if(i < L, Y + i * B + a);
-                    const c128 v = ld_if(i < L, V + a * L + i);
+                    const c128 y = sel(i < L, Y[min(i, L - 1) * B + a]);
+                    const c128 v = sel(i < L, V[a * L + min(i, L - 1)]);
                     // T^T Y^T: re = tr yr - ti yi, im = tr yi + ti yr
                     wr = __builtin_amdgcn_mfma_f64_16x16x4f64(t.re, y.re, wr, 0, 0, 0);
                     wi = __builtin_amdgcn_mfma_f64_16x16x4f64(t.re, y.im, wi, 0, 0, 0);
@@ -416,7 +436,8 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int i = bi * 16 + rl, k = bk * 16 + kl + 4 * q;
-                    const c128 a = ld_if(i < L && k < L, A + (r0 + i) + (int64_t)m * (r0 + k));
+                    const c128 a = sel(i < L && k < L, A[(r0 + min(i, L - 1)) +
+                                                         (int64_t)m * (r0 + min(k, L - 1))]);
                     dr[q] = a.re;
                     di[q] = a.im;
                 }
@@ -428,8 +449,9 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
 #pragma unroll
                 for (int js = 0; js < B; js += 4) {
                     const int j = js + kl;
-                    const c128 vI = ld_if(ii < L, V + j * L + ii), wI = ld_if(ii < L, W + j * L + ii);
-                    const c128 vK = ld_if(kk < L, V + j * L + kk), wK = ld_if(kk < L, W + j * L + kk);
+                    const int iic = min(ii, L - 1), kkc = min(kk, L - 1);
+                    const c128 vI = sel(ii < L, V[j * L + iic]), wI = sel(ii < L, W[j * L + iic]);
+                    const c128 vK = sel(kk < L, V[j * L + kkc]), wK = sel(kk < L, W[j * L + kkc]);
                     // V_I W_K^H + W_I V_K^H: re = vr wr + vi wi, im = vi wr - vr wi.
                     // Operands swapped (K-side first): the block computed is the
                     // transpose, so the output lanes run down the columns of A.
