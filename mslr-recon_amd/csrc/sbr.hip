@@ -158,6 +158,7 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
         const int L = m - c0 - B, r0 = c0 + B;
         const int kq = min(B, L - 1);
         // 1. panel rows r0.. of columns c0 .. c0+B-1 -> V
+#pragma unroll 4
         for (int idx = tid; idx < L * B; idx += S1T) {
             const int i = idx % L, j = idx / L;
             V[j * L + i] = A[(r0 + i) + (int64_t)m * (c0 + j)];
@@ -367,8 +368,13 @@ __global__ __launch_bounds__(S1T, 4) void k_sy2sb(c128 *__restrict__ Gall, int m
         {
             const int a = tid % B, c = tid / B;
             if (tid < B * B) {
+                // the partials loaded together, then summed in wave order
+                c128 zp[S1T / 64];
+#pragma unroll
+                for (int w = 0; w < S1T / 64; ++w) zp[w] = Zp[w * B * B + a * B + c];
                 c128 z = {0.0, 0.0};
-                for (int w = 0; w < S1T / 64; ++w) z = z + Zp[w * B * B + a * B + c];
+#pragma unroll
+                for (int w = 0; w < S1T / 64; ++w) z = z + zp[w];
                 Ms[a][c] = z;
             }
             __syncthreads();
@@ -662,9 +668,10 @@ __global__ __launch_bounds__(NT) void k_sb2st(const c128 *__restrict__ Gall, int
     c128 *scr = GBAND ? reinterpret_cast<c128 *>(smem_raw) : Bd + (size_t)2 * B * m;
     const int tid = threadIdx.x, wave = tid >> 6;
     const c128 *A = Gall + (int64_t)ip * m * m;
+#pragma unroll 4
     for (int idx = tid; idx < 2 * B * m; idx += NT) {
         const int j = idx / (2 * B), dg = idx % (2 * B);
-        Bd[idx] = ld_if(dg <= B && j + dg < m, A + (j + dg) + (int64_t)m * j);
+        Bd[idx] = sel(dg <= B && j + dg < m, A[min(j + dg, m - 1) + (int64_t)m * j]);
     }
     __syncthreads();
     c128 *sv = scr + wave * 3 * B;
@@ -746,11 +753,11 @@ __global__ __launch_bounds__(BTT) void k_sbr_bt(const c128 *__restrict__ Gall, i
     c128 *Uc = reinterpret_cast<c128 *>(smem_raw);  // BTC x m
     const c128 *A = Gall + (int64_t)ip * m * m;
     const double *Z = Zall + (int64_t)ip * m * m;
+#pragma unroll 4
     for (int idx = tid; idx < BTC * m; idx += BTT) {
         const int c = idx / m, i = idx % m;
-        c128 z = {0.0, 0.0};
-        if (c < ncol) z.re = Z[(int64_t)(c0 + c) * m + i];
-        Uc[idx] = z;
+        const double zr = Z[(int64_t)(c0 + min(c, ncol - 1)) * m + i];
+        Uc[idx] = c128{c < ncol ? zr : 0.0, 0.0};
     }
     // Q2.  Thread (k, c) applies reflector k of every sweep to eigenvector c and
     // keeps its 16-row window of the eigenvector in registers: the windows of sweep s,
