@@ -821,7 +821,8 @@ __global__ __launch_bounds__(64) void k_eig_vectors(const double *__restrict__ d
     double *d = reinterpret_cast<double *>(eig_smem);
     double *e = d + m;
     double *e2 = e + m;
-    for (int i = threadIdx.x; i < m; i += 64) {
+#pragma unroll 4
+    for (int i = threadIdx.x; i < m; i += 64) {  // (unrolled: the loads issue together)
         d[i] = dall[(int64_t)ip * m + i];
         const double ei = eall[(int64_t)ip * m + i];
         e[i] = ei;
