@@ -36,8 +36,8 @@ TRAFFIC_BYTES = {
     "pogm_update": 15326200000,
     "x_resid": 8946600000,
     "sum_planes": 4666000000,
-    "sy2sb": 5780900000,
-    "img2patches": 8676200000,
+    "sy2sb": 5727400000,
+    "img2patches": 8473100000,
     "prox_voxel": 2333500000,
     "split": 3978700000,
     "gram_tiles": 1822700000,
