@@ -105,7 +105,10 @@ mslr_status_t mslr_ctx_set_products(mslr_ctx ctx, int products);
  *   eig_lanes_limit: batches with patches x m at or below this use 16 lanes per
  *                    eigenvector, larger ones 4 (built-in 65536); -1 = always 4.
  *   svst_apply:      1 = form W for every patch (dense apply); 0 = low-rank
- *                    products X U_r for patches with 2 r <= m (built-in).
+ *                    products X U_r for patches with 2 r <= m (built-in; split BF16:
+ *                    those keeping r <= 64 in one kernel holding X U_r on chip);
+ *                    2 = the low-rank products always as two launches (X U_r through
+ *                    memory; the same bits as 0).
  *   sense_chunks:    frame groups of the POGM gradient (built-in 1, at most 64):
  *                    A and A' run group by group and each group's update follows its
  *                    adjoint; the batched forms alternate groups between two streams

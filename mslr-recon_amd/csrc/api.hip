@@ -172,8 +172,8 @@ mslr_status_t mslr_ctx_set_tuning(mslr_ctx ctx, const mslr_tuning *t) {
                          (t->gram_chunk >= 256 && t->gram_chunk <= 65536 && t->gram_chunk % 256 == 0),
                      MSLR_ERR_ARG, "gram_chunk must be 0 or a multiple of 256 in [256, 65536]");
         MSLR_REQUIRE(t->eig_lanes_limit >= -1, MSLR_ERR_ARG, "eig_lanes_limit must be >= -1");
-        MSLR_REQUIRE(t->svst_apply == 0 || t->svst_apply == 1, MSLR_ERR_ARG,
-                     "svst_apply must be 0 or 1");
+        MSLR_REQUIRE(t->svst_apply >= 0 && t->svst_apply <= 2, MSLR_ERR_ARG,
+                     "svst_apply must be 0, 1 or 2");
         MSLR_REQUIRE(t->sense_chunks >= 0 && t->sense_chunks <= 64, MSLR_ERR_ARG,
                      "sense_chunks must be in [0, 64]");
         for (int v : {t->yz_threads_fwd, t->yz_threads_adj})

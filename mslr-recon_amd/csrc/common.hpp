@@ -206,7 +206,8 @@ enum ScratchSlot : size_t {
     SLOT_VJP_T = 16,    // conjugate-transposed patches of the SVST derivative (p < Nt)
     SLOT_BSPLIT = 17,   // right operand of the split-BF16 apply as hi / lo words
     SLOT_SVST_COPY = 18,// FP32 copy of an SVST input that is overwritten in place
-    SLOT_COUNT = 19
+    SLOT_BSPLIT_LR = 19,// U_r and f U_r^H as hi / lo words for the fused low-rank apply
+    SLOT_COUNT = 20
 };
 
 }  // namespace mslr

@@ -1041,6 +1041,7 @@ struct RankSel {
     const int *r = nullptr;
     int m = 0;
     int mode = 0;
+    int lim = -1;  // modes 2 and 3: only patches keeping more than lim (when lim >= 0)
 };
 
 // Applies the selection for batch entry z: false when the workgroup has nothing
@@ -1050,7 +1051,7 @@ __device__ __forceinline__ bool rank_select(const RankSel &rs, int z, int &N, in
     const int r = rs.r[z];
     const bool low = r >= 0 && 2 * r <= rs.m;
     if (rs.mode == 1) return !low;
-    if (!low) return false;
+    if (!low || (rs.lim >= 0 && r <= rs.lim)) return false;
     if (rs.mode == 2) N = min(N, r);
     else K = r;
     return true;
@@ -1806,6 +1807,191 @@ __global__ __launch_bounds__(256, 2) void k_apply_bf16(const c64 *__restrict__ A
     }
 }
 
+// The low-rank reconstruction Y = (X U_r) diag(f) U_r^H of the patches keeping
+// r <= LR_MAX singular values (and 2r <= m), one pass per 64-row block: T = X U_r
+// from the split pass's words (as k_apply_bf16<2, true> computes it), kept on chip as
+// the split words of T, then Y = T B2 with B2 = diag(f) U_r^H split (as
+// k_apply_bf16<2, false> computes it from T in memory: the same products in the same
+// order, so the same bits).  Saves T's round trip through memory and a launch; the
+// read of X and the (scattered) write of Y overlap in one kernel.
+// W1 (U_r, [column j][k], k < Kp1) and W2 (f U_r^H, [column j][k], k < Kp2) as
+// k_apply_bsplit writes them.
+constexpr int LR_MAX = 64;
+constexpr int LR_LD = LR_MAX + 4;  // LDS row pitch of T's words
+__global__ __launch_bounds__(256, 2) void k_apply_lr_bf16(
+    const unsigned *__restrict__ wa, int64_t wlo, int64_t bA, const unsigned *__restrict__ W1h,
+    const unsigned *__restrict__ W1l, int64_t bW1, int Kp1, const unsigned *__restrict__ W2h,
+    const unsigned *__restrict__ W2l, int64_t bW2, int Kp2, c64 *__restrict__ C, int64_t bC,
+    int64_t cj, int M, int N2, int K1, int nb, const int *__restrict__ rall, int m,
+    PatchGeom sg, int64_t sip0, int scat) {
+    constexpr int NB = 2, TN = 32 * NB;
+    constexpr int BQ = TN * BKS / 4 / 256;
+    __shared__ __align__(16) unsigned Ah[AP_ROWS][BLD], Al[AP_ROWS][BLD];
+    __shared__ __align__(16) unsigned Bh[TN][BLD], Bl[TN][BLD];
+    __shared__ __align__(16) unsigned Th[AP_ROWS][LR_LD], Tl[AP_ROWS][LR_LD];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * (16 * NB);
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const int ar = tid & 63, akq = (tid >> 6) * 8;
+    const int nrt = (M + AP_ROWS - 1) / AP_ROWS;
+    const int64_t nunits = (int64_t)nb * nrt;
+    for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int z = (int)(u / nrt);
+        const int i0 = (int)(u - (int64_t)z * nrt) * AP_ROWS;
+        const int r = rall[z];
+        if (!(r >= 0 && 2 * r <= m && r <= LR_MAX)) continue;  // uniform per workgroup
+        const bool av = i0 + ar < M;
+        const unsigned *pw = wa + (int64_t)z * bA + i0 + ar;
+        c64 *Cz = scat ? C : C + (int64_t)z * bC;
+        mf4 accr[2][NB], acci[2][NB];
+        auto zero_acc = [&]() {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    accr[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                    acci[i][j] = mf4{0.f, 0.f, 0.f, 0.f};
+                }
+        };
+        // ---- phase 1: T = X U_r (N = r, K = K1)
+        zero_acc();
+        {
+            unsigned rwh[8], rwl[8];
+            auto fetch = [&](int k0) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int k = k0 + akq + e;
+                    const bool ok = av && k < K1;
+                    rwh[e] = ok ? pw[(int64_t)k * M] : 0u;
+                    rwl[e] = ok ? pw[wlo + (int64_t)k * M] : 0u;
+                }
+            };
+            const unsigned *wh = W1h + (int64_t)z * bW1, *wl = W1l + (int64_t)z * bW1;
+            fetch(0);
+            for (int k0 = 0; k0 < K1; k0 += BKS) {
+#pragma unroll
+                for (int q = 0; q < BQ; ++q) {
+                    const int f = tid + 256 * q;
+                    const int jj = f / (BKS / 4), kq = (f % (BKS / 4)) * 4;
+                    const bool ok = jj < r;
+                    const int64_t o = (int64_t)(ok ? jj : 0) * Kp1 + k0 + kq;
+                    const u4 h = *(const u4 *)(wh + o), l = *(const u4 *)(wl + o);
+                    *(u4 *)&Bh[jj][kq] = ok ? h : u4{0u, 0u, 0u, 0u};
+                    *(u4 *)&Bl[jj][kq] = ok ? l : u4{0u, 0u, 0u, 0u};
+                }
+                *(u4 *)&Ah[ar][akq] = u4{rwh[0], rwh[1], rwh[2], rwh[3]};
+                *(u4 *)&Ah[ar][akq + 4] = u4{rwh[4], rwh[5], rwh[6], rwh[7]};
+                *(u4 *)&Al[ar][akq] = u4{rwl[0], rwl[1], rwl[2], rwl[3]};
+                *(u4 *)&Al[ar][akq + 4] = u4{rwl[4], rwl[5], rwl[6], rwl[7]};
+                __syncthreads();
+                if (k0 + BKS < K1) fetch(k0 + BKS);
+#pragma unroll
+                for (int s = 0; s < BKS / 16; ++s) {
+                    const int kw = 16 * s + 4 * g4;
+                    u4 ch[2], cl[2], sh[2], sl[2];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        const u4 qh = *(const u4 *)&Ah[wi + 16 * i + r16][kw];
+                        const u4 ql = *(const u4 *)&Al[wi + 16 * i + r16][kw];
+                        ch[i] = w_conj(qh);
+                        cl[i] = w_conj(ql);
+                        sh[i] = w_swap(qh);
+                        sl[i] = w_swap(ql);
+                    }
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        if (wj + 16 * j >= r) break;  // uniform per wave
+                        const u4 ph = *(const u4 *)&Bh[wj + 16 * j + r16][kw];
+                        const u4 pl = *(const u4 *)&Bl[wj + 16 * j + r16][kw];
+#pragma unroll
+                        for (int i = 0; i < 2; ++i) {
+                            accr[i][j] = mfma_split(ph, pl, ch[i], cl[i], accr[i][j]);
+                            acci[i][j] = mfma_split(ph, pl, sh[i], sl[i], acci[i][j]);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // T's split words into LDS (T[row][k], k < r; zero past r up to the padded
+        // stage end), as the second product's A operand
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = wi + 16 * i + r16;
+                    const int col = wj + 16 * j + 4 * g4 + v;
+                    const bool keep = col < r && i0 + row < M;
+                    unsigned h, l;
+                    split_c64(keep ? c64{accr[i][j][v], acci[i][j][v]} : c64{0.f, 0.f}, h, l);
+                    Th[row][col] = h;
+                    Tl[row][col] = l;
+                }
+        __syncthreads();
+        // ---- phase 2: Y = T B2 (N = N2, K = r), column tiles of TN
+        const unsigned *w2h = W2h + (int64_t)z * bW2, *w2l = W2l + (int64_t)z * bW2;
+        for (int j0 = 0; j0 < N2; j0 += TN) {
+            zero_acc();
+            for (int k0 = 0; k0 < r; k0 += BKS) {
+#pragma unroll
+                for (int q = 0; q < BQ; ++q) {
+                    const int f = tid + 256 * q;
+                    const int jj = f / (BKS / 4), kq = (f % (BKS / 4)) * 4;
+                    const bool ok = j0 + jj < N2;
+                    const int64_t o = (int64_t)(ok ? j0 + jj : 0) * Kp2 + k0 + kq;
+                    const u4 h = *(const u4 *)(w2h + o), l = *(const u4 *)(w2l + o);
+                    *(u4 *)&Bh[jj][kq] = ok ? h : u4{0u, 0u, 0u, 0u};
+                    *(u4 *)&Bl[jj][kq] = ok ? l : u4{0u, 0u, 0u, 0u};
+                }
+                __syncthreads();
+#pragma unroll
+                for (int s = 0; s < BKS / 16; ++s) {
+                    const int kw = 16 * s + 4 * g4;
+                    u4 ch[2], cl[2], sh[2], sl[2];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        const u4 qh = *(const u4 *)&Th[wi + 16 * i + r16][k0 + kw];
+                        const u4 ql = *(const u4 *)&Tl[wi + 16 * i + r16][k0 + kw];
+                        ch[i] = w_conj(qh);
+                        cl[i] = w_conj(ql);
+                        sh[i] = w_swap(qh);
+                        sl[i] = w_swap(ql);
+                    }
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        if (j0 + wj + 16 * j >= N2) break;  // uniform per wave
+                        const u4 ph = *(const u4 *)&Bh[wj + 16 * j + r16][kw];
+                        const u4 pl = *(const u4 *)&Bl[wj + 16 * j + r16][kw];
+#pragma unroll
+                        for (int i = 0; i < 2; ++i) {
+                            accr[i][j] = mfma_split(ph, pl, ch[i], cl[i], accr[i][j]);
+                            acci[i][j] = mfma_split(ph, pl, sh[i], sl[i], acci[i][j]);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int ii = i0 + wi + 16 * i + r16;
+                if (ii >= M) continue;
+                const int64_t ro = scat ? patch_voxel(sg, sip0 + z, ii) : ii;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int jj = j0 + wj + 16 * j + 4 * g4 + v;
+                        if (jj < N2) Cz[ro + (int64_t)jj * cj] = c64{accr[i][j][v], acci[i][j][v]};
+                    }
+            }
+        }
+        __syncthreads();  // T and the B tile are rewritten by the next unit
+    }
+}
+
 // m == 1: each matrix is a vector v (len = p*Nt); SVST = v * max(|v| - beta, 0)/|v|.
 // nuc (nullable): the output's singular value max(|v| - beta, 0) per matrix.
 __global__ __launch_bounds__(256) void k_svst_m1(c64 *__restrict__ P, int64_t len, int64_t Np,
@@ -2194,6 +2380,44 @@ static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
     }
 }
 
+// Y = (X U_r) diag(f) U_r^H for the patches keeping r <= LR_MAX (k_apply_lr_bf16),
+// X as the split pass's words (Aw, s2 at Aw + Awlo), U (m x m, batch stride bU), f
+// (batch stride bf); Y as apply_bf16 writes it (scatter: into the image).  The
+// patches keeping more go through the two-product path (RankSel lim = LR_MAX).
+static void apply_lowrank_fused(mslr_ctx_s *ctx, const unsigned *Aw, int64_t Awlo, int64_t bA,
+                                const c64 *U, int64_t bU, const float *f, int64_t bf, c64 *C,
+                                int64_t bC, int64_t cj, int M, int m, int nbatch, const int *r,
+                                const PatchGeom *scatter, int64_t sip0) {
+    const int half = m / 2;
+    if (M <= 0 || half < 1 || nbatch <= 0) return;
+    const int Kp1 = (int)cdiv(m, BKS) * BKS, Kp2 = (int)cdiv(half, BKS) * BKS;
+    const int64_t bW1 = (int64_t)half * Kp1, bW2 = (int64_t)m * Kp2;
+    const int64_t zc = std::min<int64_t>(nbatch, 65535);
+    unsigned *W = (unsigned *)ctx->scratch_get(SLOT_BSPLIT_LR,
+                                               (size_t)(2 * zc * (bW1 + bW2)) * sizeof(unsigned));
+    unsigned *W1h = W, *W1l = W + zc * bW1, *W2h = W + 2 * zc * bW1, *W2l = W2h + zc * bW2;
+    MSLR_PROF(ctx, "apply_lr");
+    for (int64_t z0 = 0; z0 < nbatch; z0 += zc) {
+        const int nz = (int)std::min<int64_t>(zc, nbatch - z0);
+        // the same words as the two-product path's splits (modes 2 and 3 of its RankSel)
+        const RankSel s1{r + z0, m, 2}, s2{r + z0, m, 3};
+        k_apply_bsplit<<<dim3((unsigned)cdiv(bW1, 256), (unsigned)nz), 256, 0, ctx->stream>>>(
+            U + z0 * bU, bU, 1, m, 0, nullptr, 0, half, m, Kp1, s1, W1h, W1l, bW1);
+        HIP_LAUNCH_CHECK();
+        k_apply_bsplit<<<dim3((unsigned)cdiv(bW2, 256), (unsigned)nz), 256, 0, ctx->stream>>>(
+            U + z0 * bU, bU, m, 1, 1, f + z0 * bf, bf, m, half, Kp2, s2, W2h, W2l, bW2);
+        HIP_LAUNCH_CHECK();
+        const int64_t units = (int64_t)nz * cdiv(M, AP_ROWS);
+        const unsigned g = (unsigned)std::min<int64_t>(units, 4 * (int64_t)std::max(ctx->num_cus, 1));
+        const PatchGeom sg = scatter ? *scatter : PatchGeom{};
+        k_apply_lr_bf16<<<g, 256, 0, ctx->stream>>>(
+            Aw + z0 * bA, Awlo, bA, W1h, W1l, bW1, Kp1, W2h, W2l, bW2, Kp2,
+            scatter ? C : C + z0 * bC, bC, cj, M, m, m, nz, r + z0, m, sg, sip0 + z0,
+            scatter ? 1 : 0);
+        HIP_LAUNCH_CHECK();
+    }
+}
+
 // Reconstruction products that the split-BF16 form runs in FP32 on the matrix
 // cores (bit 0 the dense X W, bit 1 X U_r, bit 2 T f U_r^H).
 constexpr int SVST_APPLY_FP32 = 0;
@@ -2351,7 +2575,7 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
         // 2 p m^2 + m^2 r); the others W = U_r diag(f) U_r^H, then Y = X W.  The
         // choice is per patch, on the device, from r.
         const int half = mi / 2;
-        const bool lowrank = !left && half >= 1 && ctx->tuning.svst_apply == 0 && svst_lowrank_apply();
+        const bool lowrank = !left && half >= 1 && ctx->tuning.svst_apply != 1 && svst_lowrank_apply();
         const RankSel dense = lowrank ? RankSel{w.r, mi, 1} : RankSel{};
         // W = (U diag f) U^H ; r < 0 (beta = 0) gives the identity, r = 0 gives 0
         cgemm(ctx, "form_w", w.U, m * m, 1, m, w.U, m * m, m, 1, 1, w.f, m, Wm, m * m, 1, m, mi, mi,
@@ -2397,7 +2621,15 @@ void svst_batched(mslr_ctx_s *ctx, const c64 *Pin, c64 *Pout, int64_t p, int64_t
             // T (p x r) = X U_r, then Y = (T diag f) U_r^H; T holds at most m/2 columns
             const int64_t bT = p * half;
             c64 *T = (c64 *)ctx->scratch_get(SLOT_LOWRANK, (size_t)(bT * nb) * sizeof(c64));
-            const RankSel s1{w.r, mi, 2}, s2{w.r, mi, 3};
+            RankSel s1{w.r, mi, 2}, s2{w.r, mi, 3};
+            if (split && !(amask & 6) && words_a && ctx->tuning.svst_apply == 0) {
+                // r <= LR_MAX in one kernel, T kept on chip; the rest below
+                apply_lowrank_fused(ctx, Wh, len * nb, len, w.U, m * m, w.f, m, Yc, len,
+                                    scatter ? (int64_t)scatter->N[0] * scatter->N[1] * scatter->N[2]
+                                            : (int64_t)p,
+                                    (int)p, mi, (int)nb, w.r, scatter, ip0);
+                s1.lim = s2.lim = LR_MAX;
+            }
             prod(2, false, "apply_xu", Pa, len, 1, p, w.U, m * m, 1, m, 0, nullptr, 0, T, bT, 1, p,
                  (int)p, half, mi, s1, true);
             prod(4, true, "apply_tu", T, bT, 1, p, w.U, m * m, m, 1, 1, w.f, m, Yc, len, 1, p,

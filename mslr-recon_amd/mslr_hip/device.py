@@ -51,7 +51,7 @@ class Context:
         """Launch shapes (mslr_ctx_set_tuning; an omitted or 0 field = built-in):
         gram_chunk (contraction rows per Gram workgroup, a multiple of 256),
         eig_lanes_limit (patches x m limit for 16 lanes per eigenvector, -1: always 4),
-        svst_apply (1: dense apply), sense_chunks (frame groups of the gradient, at most
+        svst_apply (1: dense apply, 2: low-rank products never fused), sense_chunks (frame groups of the gradient, at most
         64), yz_threads_fwd / yz_threads_adj (512, 768 or 1024 threads of the compiled
         y-z FFT passes of the 90 x 60 plane), update_lines (yz-lines per workgroup of the fused transpose + update, a
         multiple of 8 in [8, 64]), sb2st_schedule (0-3: the

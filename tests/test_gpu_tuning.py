@@ -34,7 +34,7 @@ def test_tuning_roundtrip_and_validation(shapes):
     shapes.set_tuning(**want)
     assert shapes.tuning() == want
     for bad in [dict(gram_chunk=100), dict(gram_chunk=128), dict(gram_chunk=1 << 17),
-                dict(gram_chunk=-256), dict(eig_lanes_limit=-2), dict(svst_apply=2),
+                dict(gram_chunk=-256), dict(eig_lanes_limit=-2), dict(svst_apply=3),
                 dict(sense_chunks=-1), dict(sense_chunks=65), dict(yz_threads_fwd=256),
                 dict(yz_threads_adj=640), dict(update_lines=100), dict(update_lines=12), dict(sb2st_schedule=4),
                 dict(apply_fp32=9), dict(apply_fp32=-1)]:
@@ -262,3 +262,42 @@ def test_compiled_variants_are_bit_identical(shapes, rng):
         assert np.array_equal(o, outs[0])
     ref = np.stack([O.SVST(Z[:, :, i].astype(np.complex128), float(np.float32(beta))) for i in range(3)], axis=2)
     assert rel_err(outs[0], ref) < 2e-5
+
+
+def test_fused_lowrank_apply_is_bit_identical(shapes, rng):
+    """Split BF16: patches keeping r <= 64 singular values (and r <= m/2) reconstruct
+    Y = (X U_r) f U_r^H in one kernel that holds X U_r on chip (svst_apply 0); with
+    svst_apply 2 the two products go through memory.  The same products in the same
+    order give the same bits: ranks either side of 64 and of m/2 = 150 in one batch, and
+    a mixed POGM run whose whole-image patch is written straight into the image."""
+    from mslr_hip import recon
+    from mslr_hip.mirt_mod import POGM
+    from mslr_hip.sense import SenseOp
+    ranks = [0, 1, 2, 33, 64, 65, 149, 150, 151, 299]
+    P, betas = mixed_rank_batch(rng, ranks)
+    shapes.set_products("split_bf16")
+    out = {}
+    for a in (0, 2):
+        shapes.set_tuning(svst_apply=a)
+        out[a] = recon.svst_batched(P, betas).to_host()
+    assert np.array_equal(out[0], out[2])
+    for i, k in enumerate(ranks):
+        Yo = O.SVST(P[:, :, i].astype(np.complex128), np.float32(betas[i]))
+        assert rel_err(out[0][:, :, i], Yo) < 1e-4, (i, k)
+    cfg, smaps, ksp, omega, sf = prepared("small")
+    N, Nt = tuple(cfg["N"]), cfg["Nt"]
+    lams = O.lambdas_ong_lustig(N, Nt, cfg["patch_sizes"])
+    A = SenseOp(omega, smaps)
+    runs = {}
+    for a in (0, 2):
+        shapes.set_tuning(svst_apply=a)
+        st = POGM(A, patch_sizes=cfg["patch_sizes"], strides=cfg["strides"], lambdas=lams, L=2.0,
+                  niter=6, form="mixed", log_costs=True)
+        st.init(ksp)
+        st.run(6)
+        runs[a] = (st.X.to_host(), st.log()[1], st.costs())
+        st.close()
+    A.close()
+    assert runs[2][1] == runs[0][1]
+    assert np.array_equal(runs[2][0], runs[0][0])
+    np.testing.assert_allclose(runs[2][2], runs[0][2], rtol=1e-12)
