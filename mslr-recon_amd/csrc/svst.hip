@@ -1646,23 +1646,27 @@ __global__ __launch_bounds__(256) void k_apply_bsplit(const c64 *__restrict__ B,
     const int z = blockIdx.y;
     int Nz = N, Kz = K;
     if (!rank_select(rs, z, Nz, Kz)) return;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // four consecutive k of one column j per thread (Kp is a multiple of BKS), so a
+    // launch dispatches a quarter of the workgroups, most of which exit here when the
+    // rank selection narrows N or K
+    const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (e >= (int64_t)Nz * Kp) return;
-    const int j = (int)(e / Kp), k = (int)(e - (int64_t)j * Kp);
-    if (k >= (Kz + BKS - 1) / BKS * BKS) return;  // stages k_apply_bf16 never reads
-    c64 v{0.f, 0.f};
-    if (k < Kz) {
-        v = B[(int64_t)z * bB + (int64_t)k * bk + (int64_t)j * bj];
-        if (conjB) v.im = -v.im;
+    const int j = (int)(e / Kp), k0 = (int)(e - (int64_t)j * Kp);
+    if (k0 >= (Kz + BKS - 1) / BKS * BKS) return;  // stages k_apply_bf16 never reads
+    unsigned h[4], l[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int k = k0 + v, kc = min(k, Kz - 1);  // Kz >= 1 past the test above
+        c64 b = B[(int64_t)z * bB + (int64_t)kc * bk + (int64_t)j * bj];
+        if (conjB) b.im = -b.im;
         if (fs) {
-            const float f = fs[(int64_t)z * bfs + k];
-            v = c64{v.re * f, v.im * f};
+            const float f = fs[(int64_t)z * bfs + kc];
+            b = c64{b.re * f, b.im * f};
         }
+        split_c64(k < Kz ? b : c64{0.f, 0.f}, h[v], l[v]);
     }
-    unsigned h, l;
-    split_c64(v, h, l);
-    Wh[(int64_t)z * bW + e] = h;
-    Wl[(int64_t)z * bW + e] = l;
+    *(u4 *)(Wh + (int64_t)z * bW + e) = u4{h[0], h[1], h[2], h[3]};
+    *(u4 *)(Wl + (int64_t)z * bW + e) = u4{l[0], l[1], l[2], l[3]};
 }
 
 // AW: A arrives as the split pass's Ozaki words (wa: s1, then s2 at wa + wlo, same
@@ -2343,7 +2347,7 @@ static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
     for (int64_t z0 = 0; z0 < nbatch; z0 += zc) {
         const int nz = (int)std::min<int64_t>(zc, nbatch - z0);
         const RankSel rz = rank_sel_at(rs, z0);
-        dim3 gs((unsigned)cdiv(bW, 256), (unsigned)nz);
+        dim3 gs((unsigned)cdiv(bW, 1024), (unsigned)nz);
         k_apply_bsplit<<<gs, 256, 0, ctx->stream>>>(B + z0 * bB, bB, bk, bj, conjB,
                                                     fs ? fs + z0 * bfs : nullptr, bfs, N, K, Kp, rz,
                                                     Wh, Wl, bW);
@@ -2401,10 +2405,10 @@ static void apply_lowrank_fused(mslr_ctx_s *ctx, const unsigned *Aw, int64_t Awl
         const int nz = (int)std::min<int64_t>(zc, nbatch - z0);
         // the same words as the two-product path's splits (modes 2 and 3 of its RankSel)
         const RankSel s1{r + z0, m, 2}, s2{r + z0, m, 3};
-        k_apply_bsplit<<<dim3((unsigned)cdiv(bW1, 256), (unsigned)nz), 256, 0, ctx->stream>>>(
+        k_apply_bsplit<<<dim3((unsigned)cdiv(bW1, 1024), (unsigned)nz), 256, 0, ctx->stream>>>(
             U + z0 * bU, bU, 1, m, 0, nullptr, 0, half, m, Kp1, s1, W1h, W1l, bW1);
         HIP_LAUNCH_CHECK();
-        k_apply_bsplit<<<dim3((unsigned)cdiv(bW2, 256), (unsigned)nz), 256, 0, ctx->stream>>>(
+        k_apply_bsplit<<<dim3((unsigned)cdiv(bW2, 1024), (unsigned)nz), 256, 0, ctx->stream>>>(
             U + z0 * bU, bU, m, 1, 1, f + z0 * bf, bf, m, half, Kp2, s2, W2h, W2l, bW2);
         HIP_LAUNCH_CHECK();
         const int64_t units = (int64_t)nz * cdiv(M, AP_ROWS);
