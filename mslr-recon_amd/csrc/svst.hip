@@ -1290,6 +1290,10 @@ __global__ __launch_bounds__(256) void k_split_rows(const c64 *__restrict__ X, i
 // the geometry g, words and the FP32 copy Xc in patch-matrix order (columns of the
 // np patches one after another).  One wave per (patch, block of SPL_ROWS rows,
 // group of SG_T frames): the rows' voxel offsets once, then a frame per step.
+// The patches' x runs share cache lines with their x neighbours' (a run of 10
+// voxels is 80 bytes), so the waves are ordered patch before frame group and the
+// workgroups mapped to XCDs so that consecutive ones share an XCD (workgroup b runs
+// on XCD b mod 8): x-neighbouring patches then read those lines through one L2.
 constexpr int SG_T = 16;
 __global__ __launch_bounds__(256) void k_split_gather(const c64 *__restrict__ img, PatchGeom g,
                                                       int64_t ip0, int64_t np,
@@ -1300,28 +1304,44 @@ __global__ __launch_bounds__(256) void k_split_gather(const c64 *__restrict__ im
     const int Nt = g.Nt;
     const int64_t nblk = (p + SPL_ROWS - 1) / SPL_ROWS;
     const int ntg = (Nt + SG_T - 1) / SG_T;
-    const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t G = gridDim.x, b = blockIdx.x, G8 = G - G % 8;
+    const int64_t L = b < G8 ? (b % 8) * (G8 / 8) + b / 8 : b;  // a bijection of [0, G)
+    const int64_t wid = L * 4 + (threadIdx.x >> 6);
     if (wid >= np * nblk * ntg) return;
     const int lane = threadIdx.x & 63;
     const int64_t blk = wid % nblk, r = wid / nblk;
-    const int tg = (int)(r % ntg);
-    const int64_t z = r / ntg;
+    const int64_t z = r % np;
+    const int tg = (int)(r / np);
     const int n = (int)std::min<int64_t>(SPL_ROWS, p - blk * SPL_ROWS);
     const int r0 = (int)(blk * SPL_ROWS) + lane;
     const int64_t o0 = lane < n ? patch_voxel(g, ip0 + z, r0) : 0;
     const int64_t o1 = lane + 64 < n ? patch_voxel(g, ip0 + z, r0 + 64) : 0;
     const int64_t V = (int64_t)g.N[0] * g.N[1] * g.N[2];
     const int t1 = min(Nt, (tg + 1) * SG_T);
-#pragma unroll 2
-    for (int t = tg * SG_T; t < t1; ++t) {
-        const c64 v0 = lane < n ? img[o0 + t * V] : c64{0.f, 0.f};
-        const c64 v1 = lane + 64 < n ? img[o1 + t * V] : c64{0.f, 0.f};
-        const int64_t off = (z * Nt + t) * p + blk * SPL_ROWS;
-        if (Xc) {
-            if (lane < n) Xc[off + lane] = v0;
-            if (lane + 64 < n) Xc[off + lane + 64] = v1;
+    // four frames' loads issued together (unconditional: o0 = o1 = 0 past the block,
+    // voxel 0 of the frame, and the last frame repeated past t1), the lanes past the
+    // block zeroed afterwards
+    for (int t0 = tg * SG_T; t0 < t1; t0 += 4) {
+        c64 a0[4], a1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = min(t0 + q, t1 - 1);
+            a0[q] = img[o0 + t * V];
+            a1[q] = img[o1 + t * V];
         }
-        split_block(v0, v1, lane, n, Wh + off, Wl + off);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = t0 + q;
+            if (t >= t1) break;  // uniform per wave
+            const c64 v0 = lane < n ? a0[q] : c64{0.f, 0.f};
+            const c64 v1 = lane + 64 < n ? a1[q] : c64{0.f, 0.f};
+            const int64_t off = (z * Nt + t) * p + blk * SPL_ROWS;
+            if (Xc) {
+                if (lane < n) Xc[off + lane] = v0;
+                if (lane + 64 < n) Xc[off + lane + 64] = v1;
+            }
+            split_block(v0, v1, lane, n, Wh + off, Wl + off);
+        }
     }
 }
 
