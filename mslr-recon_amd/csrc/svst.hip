@@ -1666,27 +1666,36 @@ __global__ __launch_bounds__(256) void k_apply_bsplit(const c64 *__restrict__ B,
     const int z = blockIdx.y;
     int Nz = N, Kz = K;
     if (!rank_select(rs, z, Nz, Kz)) return;
-    // four consecutive k of one column j per thread (Kp is a multiple of BKS), so a
-    // launch dispatches a quarter of the workgroups, most of which exit here when the
-    // rank selection narrows N or K
-    const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-    if (e >= (int64_t)Nz * Kp) return;
-    const int j = (int)(e / Kp), k0 = (int)(e - (int64_t)j * Kp);
-    if (k0 >= (Kz + BKS - 1) / BKS * BKS) return;  // stages k_apply_bf16 never reads
-    unsigned h[4], l[4];
+    // four consecutive k of one column j per thread (Kp is a multiple of BKS), the
+    // grid striding over the entry (the host caps it: most of a batch's entries are
+    // skipped by the rank selection, or narrowed to a few columns or k)
+    const int64_t ne = (int64_t)Nz * Kp;
+    const int kend = (Kz + BKS - 1) / BKS * BKS;  // stages k_apply_bf16 reads
+    for (int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; e < ne;
+         e += (int64_t)gridDim.x * blockDim.x * 4) {
+        const int j = (int)(e / Kp), k0 = (int)(e - (int64_t)j * Kp);
+        if (k0 >= kend) continue;
+        unsigned h[4], l[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int k = k0 + v, kc = min(k, Kz - 1);  // Kz >= 1 past the test above
-        c64 b = B[(int64_t)z * bB + (int64_t)kc * bk + (int64_t)j * bj];
-        if (conjB) b.im = -b.im;
-        if (fs) {
-            const float f = fs[(int64_t)z * bfs + kc];
-            b = c64{b.re * f, b.im * f};
+        for (int v = 0; v < 4; ++v) {
+            const int k = k0 + v, kc = min(k, Kz - 1);  // Kz >= 1 past the test above
+            c64 b = B[(int64_t)z * bB + (int64_t)kc * bk + (int64_t)j * bj];
+            if (conjB) b.im = -b.im;
+            if (fs) {
+                const float f = fs[(int64_t)z * bfs + kc];
+                b = c64{b.re * f, b.im * f};
+            }
+            split_c64(k < Kz ? b : c64{0.f, 0.f}, h[v], l[v]);
         }
-        split_c64(k < Kz ? b : c64{0.f, 0.f}, h[v], l[v]);
+        *(u4 *)(Wh + (int64_t)z * bW + e) = u4{h[0], h[1], h[2], h[3]};
+        *(u4 *)(Wl + (int64_t)z * bW + e) = u4{l[0], l[1], l[2], l[3]};
     }
-    *(u4 *)(Wh + (int64_t)z * bW + e) = u4{h[0], h[1], h[2], h[3]};
-    *(u4 *)(Wl + (int64_t)z * bW + e) = u4{l[0], l[1], l[2], l[3]};
+}
+
+// Workgroups per batch entry of k_apply_bsplit: enough for the whole entry alone, at
+// most about 16384 over the batch.
+static unsigned bsplit_groups(int64_t bW, int64_t nz) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(bW, 1024), cdiv(16384, nz)));
 }
 
 // AW: A arrives as the split pass's Ozaki words (wa: s1, then s2 at wa + wlo, same
@@ -2367,7 +2376,7 @@ static void apply_bf16(mslr_ctx_s *ctx, const char *tag, const c64 *A, int64_t b
     for (int64_t z0 = 0; z0 < nbatch; z0 += zc) {
         const int nz = (int)std::min<int64_t>(zc, nbatch - z0);
         const RankSel rz = rank_sel_at(rs, z0);
-        dim3 gs((unsigned)cdiv(bW, 1024), (unsigned)nz);
+        dim3 gs(bsplit_groups(bW, nz), (unsigned)nz);
         k_apply_bsplit<<<gs, 256, 0, ctx->stream>>>(B + z0 * bB, bB, bk, bj, conjB,
                                                     fs ? fs + z0 * bfs : nullptr, bfs, N, K, Kp, rz,
                                                     Wh, Wl, bW);
@@ -2425,10 +2434,10 @@ static void apply_lowrank_fused(mslr_ctx_s *ctx, const unsigned *Aw, int64_t Awl
         const int nz = (int)std::min<int64_t>(zc, nbatch - z0);
         // the same words as the two-product path's splits (modes 2 and 3 of its RankSel)
         const RankSel s1{r + z0, m, 2}, s2{r + z0, m, 3};
-        k_apply_bsplit<<<dim3((unsigned)cdiv(bW1, 1024), (unsigned)nz), 256, 0, ctx->stream>>>(
+        k_apply_bsplit<<<dim3(bsplit_groups(bW1, nz), (unsigned)nz), 256, 0, ctx->stream>>>(
             U + z0 * bU, bU, 1, m, 0, nullptr, 0, half, m, Kp1, s1, W1h, W1l, bW1);
         HIP_LAUNCH_CHECK();
-        k_apply_bsplit<<<dim3((unsigned)cdiv(bW2, 1024), (unsigned)nz), 256, 0, ctx->stream>>>(
+        k_apply_bsplit<<<dim3(bsplit_groups(bW2, nz), (unsigned)nz), 256, 0, ctx->stream>>>(
             U + z0 * bU, bU, m, 1, 1, f + z0 * bf, bf, m, half, Kp2, s2, W2h, W2l, bW2);
         HIP_LAUNCH_CHECK();
         const int64_t units = (int64_t)nz * cdiv(M, AP_ROWS);
