@@ -31,19 +31,19 @@ HBM_PEAK_GBS = 8000.0
 # families, from the committed rocprofv3 counter summary of this workload (mixed
 # form; the SENSE, update and patch kernels are the same in every form)
 TRAFFIC_BYTES = {
-    "fwd_yz": 4454400000,
-    "adj_yz": 4362500000,
+    "fwd_yz": 4455800000,
+    "adj_yz": 4359800000,
     "pogm_update": 15326200000,
     "x_resid": 8946600000,
     "sum_planes": 4666000000,
-    "sy2sb": 5726700000,
-    "img2patches": 8529800000,
+    "sy2sb": 5725100000,
+    "img2patches": 8589500000,
     "prox_voxel": 2333500000,
-    "split": 2473400000,
-    "gram_tiles": 1821500000,
-    "backtransform": 1406200000,
-    "apply": 2126800000,
-    "apply_lr": 1843500000,
+    "split": 2473000000,
+    "gram_tiles": 1822200000,
+    "backtransform": 1406400000,
+    "apply": 2125900000,
+    "apply_lr": 1842100000,
     "sb2st": 240800000,
 }
 FP32_PEAK_TFLOPS = 157.3
